@@ -1,0 +1,769 @@
+// Grid-side kernels: level set + redistancing + cell flags (a6-a8), P2G
+// (a9), extrapolation and wall conditions (a10), two-phase divergence (a11),
+// pressure gradient (a13), heat grid stages (a16), marching cubes (a18) and
+// the solid distance/velocity fields used by the collision advection (a15).
+#include "mpf_internal.h"
+
+#include "mc_tables.inc"
+
+namespace mpf {
+
+namespace {
+
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+#define CELL_LOOP_HEAD                                           \
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; \
+  if (idx >= g.G) return;                                        \
+  int i, j, k;                                                   \
+  ijk_of(g, idx, i, j, k);
+
+// ---------------- level set (a6) ----------------
+// genlevelset + mindis_cell, spray_k.cu:2753-2806.  The minimum is taken over
+// squared distances and square-rooted once: sqrt is correctly rounded and
+// monotone, so sqrt(min d^2) == min sqrt(d^2) bit for bit.
+__global__ void k_genlevelset(Grid g, const f3* __restrict__ pos, const int8_t* __restrict__ flag,
+                              const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
+                              const uint8_t* __restrict__ cmask, float* __restrict__ lsfluid,
+                              float* __restrict__ lsair, int8_t* __restrict__ mark) {
+  CELL_LOOP_HEAD
+  const float h = g.h;
+  mark[idx] = VACUUM;
+  float r = 0.5f * h;
+  f3 gp = mk3(((float)i + 0.5f) * h + g.gx0, ((float)j + 0.5f) * h + g.gy0, ((float)k + 0.5f) * h + g.gz0);
+  float ma2 = INFINITY, mf2 = INFINITY;
+  for (int di = -2; di <= 2; ++di)
+    for (int dj = -2; dj <= 2; ++dj)
+      for (int dk = -2; dk <= 2; ++dk) {
+        if (!valid(g, i + di, j + dj, k + dk)) continue;
+        int64_t c = cid(g, i + di, j + dj, k + dk);
+        if (!(cmask[c] & (M_FLUIDSOLID | M_AIR | M_AIRSOLO))) continue;
+        uint32_t s = cstart[c], e = cend[c];
+        for (uint32_t p = s; p < e; ++p) {
+          f3 d = sub3(pos[p], gp);
+          float d2 = dot3(d, d);
+          int8_t f = flag[p];
+          if (f == AIR || f == AIRSOLO) ma2 = fminf(ma2, d2);
+          else if (f == FLUID || f == SOLID) mf2 = fminf(mf2, d2);
+        }
+      }
+  float mair = 2.5f * h, mfl = 2.5f * h;
+  float da = sqrtf(ma2), df = sqrtf(mf2);
+  mair = (da < mair) ? da : mair;
+  mfl = (df < mfl) ? df : mfl;
+  lsfluid[idx] = mfl - r;
+  lsair[idx] = mair - r;
+}
+
+// markLS_bigpositive spray_k.cu:3683-3697 (mode 0) / preparels 3708-3722 (mode 1)
+__global__ void k_ls_prepare(int64_t G, float* __restrict__ ls, int8_t* __restrict__ mark, float h, int mode) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= G) return;
+  float v = ls[idx];
+  if (mode == 0) {
+    v = v / h;
+    if (v > 1.99f) { v = 5.0f; mark[idx] = AIR; }
+    else mark[idx] = FLUID;
+  } else {
+    v = -v / h;
+    if (v > 0) { v = 5.0f; mark[idx] = AIR; }
+    else mark[idx] = FLUID;
+  }
+  ls[idx] = v;
+}
+// setLSback_bigpositive spray_k.cu:3699-3706 (mode 0) / setLSback 3724-3731 (mode 1)
+__global__ void k_ls_back(int64_t G, float* __restrict__ ls, float h, int mode) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= G) return;
+  ls[idx] = (mode == 0) ? ls[idx] * h : -ls[idx] * h;
+}
+
+// solvedistance spray_k.cu:746-754
+__device__ __forceinline__ void solvedistance(float a, float b, float c, float& x) {
+  float d = fminf(a, fminf(b, c)) + 1.0f;
+  if (d > fmaxf(a, fmaxf(b, c)))
+    d = (a + b + c + sqrtf(3.0f - (a - b) * (a - b) - (a - c) * (a - c) - (b - c) * (b - c))) / 3.0f;
+  if (d < x) x = d;
+}
+// sweepphibytype (spray_k.cu:771-789) / sweepphi (755-770, typeflag < 0), two buffers
+__global__ void k_sweep(Grid g, const float* __restrict__ in, float* __restrict__ out, const int8_t* __restrict__ mark,
+                        int typeflag) {
+  CELL_LOOP_HEAD
+  float res = in[idx];
+  if (!(typeflag >= 0 && mark[idx] == typeflag)) {
+    for (int di = -1; di <= 1; di += 2)
+      for (int dj = -1; dj <= 1; dj += 2)
+        for (int dk = -1; dk <= 1; dk += 2)
+          if (valid(g, i + di, j, k) && valid(g, i, j + dj, k) && valid(g, i, j, k + dk))
+            solvedistance(in[cid(g, i + di, j, k)], in[cid(g, i, j + dj, k)], in[cid(g, i, j, k + dk)], res);
+  }
+  out[idx] = res;
+}
+
+// mergeLSAndMarkGrid spray_k.cu:3733-3777
+__global__ void k_merge(Grid g, const float* __restrict__ lsfluid, const float* __restrict__ lsair,
+                        float* __restrict__ lsmerge, int8_t* __restrict__ mark) {
+  CELL_LOOP_HEAD
+  const float h = g.h;
+  float lf = lsfluid[idx], la = lsair[idx], lm;
+  int8_t m;
+  if (la > 4.99f * h) { lm = lf; m = (lf > 0) ? VACUUM : FLUID; }
+  else if (lf > 4.99f * h) { lm = la; m = (la > 0) ? VACUUM : AIR; }
+  else if (la > 0.8f * h && lf > 0.8f * h) { m = VACUUM; lm = fminf(lf, la); }
+  else { lm = (lf - la) * 0.5f; m = (lm > 0) ? AIR : FLUID; }
+  if (i == 0 || i == g.nx - 1 || j == 0 || j == g.ny - 1 || k == 0 || k == g.nz - 1) {
+    m = BOUNDARY;
+    lm = -0.5f * h;
+  }
+  lsmerge[idx] = lm;
+  mark[idx] = m;
+}
+
+// computePhigra spray_k.cu:3994-4019
+__global__ void k_phigrad(Grid g, const float* __restrict__ ls, float* __restrict__ gx, float* __restrict__ gy,
+                          float* __restrict__ gz) {
+  CELL_LOOP_HEAD
+  float c0 = ls[idx];
+  float x1 = valid(g, i + 1, j, k) ? ls[cid(g, i + 1, j, k)] : c0;
+  float x0 = valid(g, i - 1, j, k) ? ls[cid(g, i - 1, j, k)] : c0;
+  float y1 = valid(g, i, j + 1, k) ? ls[cid(g, i, j + 1, k)] : c0;
+  float y0 = valid(g, i, j - 1, k) ? ls[cid(g, i, j - 1, k)] : c0;
+  float z1 = valid(g, i, j, k + 1) ? ls[cid(g, i, j, k + 1)] : c0;
+  float z0 = valid(g, i, j, k - 1) ? ls[cid(g, i, j, k - 1)] : c0;
+  gx[idx] = ((x1 - x0) * 0.5f) / g.h;
+  gy[idx] = ((y1 - y0) * 0.5f) / g.h;
+  gz[idx] = ((z1 - z0) * 0.5f) / g.h;
+}
+
+// markfluid_dense + markBoundaryCell spray_k.cu:506-545 (all cells)
+__global__ void k_mark_counts(Grid g, const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
+                              const int8_t* __restrict__ flag, int8_t* __restrict__ mark) {
+  CELL_LOOP_HEAD
+  int cfs = 0, ca = 0;
+  uint32_t s = cstart[idx];
+  if (s != MPF_CELL_UNDEF)
+    for (uint32_t p = s; p < cend[idx]; ++p) {
+      int8_t f = flag[p];
+      if (f == FLUID || f == SOLID) cfs++;
+      else if (f == AIR) ca++;
+    }
+  int8_t m = (cfs == 0 && ca == 0) ? VACUUM : (cfs > ca ? FLUID : AIR);
+  if (i == 0 || i == g.nx - 1 || j == 0 || j == g.ny - 1 || k == 0 || k == g.nz - 1) m = BOUNDARY;
+  mark[idx] = m;
+}
+
+// Solid constraint: non-boundary cells whose particles are at least half SOLID.
+__global__ void k_mark_solid(Grid g, const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
+                             const int8_t* __restrict__ flag, const uint8_t* __restrict__ cmask,
+                             int8_t* __restrict__ mark) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= g.G) return;
+  if (!(cmask[idx] & M_SOLID) || mark[idx] == BOUNDARY) return;
+  int cs = 0, co = 0;
+  for (uint32_t p = cstart[idx]; p < cend[idx]; ++p) {
+    if (flag[p] == SOLID) cs++;
+    else co++;
+  }
+  if (cs > 0 && cs >= co) mark[idx] = SOLID;
+}
+
+// ---------------- P2G (a9) ----------------
+// mapvelp2g_k_fluidSolid / mapvelp2g_k_air with sumcell_* and sharp_kernel,
+// spray_k.cu:255-258, 2809-2937.  One thread per face of axis A; windows
+// normal [-1,0] x transverse [-1,1] (liquid+solid) and normal [-2,1] x
+// transverse [-2,2] (air); cells without particles of the phase are skipped
+// through the occupancy mask.
+template <int A, bool AIRPH>
+__global__ void k_p2g(Grid g, const f3* __restrict__ pos, const f3* __restrict__ vel, const float* __restrict__ mass,
+                      const int8_t* __restrict__ flag, const uint32_t* __restrict__ cstart,
+                      const uint32_t* __restrict__ cend, const uint8_t* __restrict__ cmask, float* __restrict__ out,
+                      int64_t nf) {
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int hh = g.ny + (A == 1), d = g.nz + (A == 2);
+  int i, j, k;
+  ijk3(f, hh, d, i, j, k);
+  const float scale = 1.0f / g.h;
+  f3 gp = mk3((float)i + (A == 0 ? 0.0f : 0.5f), (float)j + (A == 1 ? 0.0f : 0.5f), (float)k + (A == 2 ? 0.0f : 0.5f));
+  constexpr int lo = AIRPH ? -2 : -1, hin = AIRPH ? 1 : 0, lot = AIRPH ? -2 : -1, hit = AIRPH ? 2 : 1;
+  constexpr uint8_t need = AIRPH ? M_AIR : M_FLUIDSOLID;
+  float weight = 0, usum = 0;
+  const float RE = 1.4f;
+  const float RE2 = RE * RE;
+  for (int di = (A == 0 ? lo : lot); di <= (A == 0 ? hin : hit); ++di)
+    for (int dj = (A == 1 ? lo : lot); dj <= (A == 1 ? hin : hit); ++dj)
+      for (int dk = (A == 2 ? lo : lot); dk <= (A == 2 ? hin : hit); ++dk) {
+        if (!valid(g, i + di, j + dj, k + dk)) continue;
+        int64_t c = cid(g, i + di, j + dj, k + dk);
+        if (!(cmask[c] & need)) continue;
+        uint32_t s = cstart[c], e = cend[c];
+        for (uint32_t p = s; p < e; ++p) {
+          int8_t fl = flag[p];
+          bool take = AIRPH ? (fl == AIR) : (fl == FLUID || fl == SOLID);
+          if (!take) continue;
+          f3 x = pos[p];
+          f3 dv = mk3((x.x - g.gx0) * scale - gp.x, (x.y - g.gy0) * scale - gp.y, (x.z - g.gz0) * scale - gp.z);
+          float r2 = dot3(dv, dv);
+          float w = mass[p] * fmaxf(RE2 / fmaxf(r2, 0.0001f) - 1.0f, 0.0f);
+          weight += w;
+          const float* vp = &vel[p].x;
+          usum += w * vp[A];
+        }
+      }
+  out[f] = (weight > 0) ? (usum / weight) : 0.0f;
+}
+
+// ---------------- extrapolation + wall conditions (a10) ----------------
+// initphi, spray_k.cu:712-723
+__global__ void k_initphi(int64_t G, const int8_t* __restrict__ mark, float* __restrict__ phi, int8_t type, float big) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < G) phi[idx] = (mark[idx] == type) ? -0.5f : big;
+}
+
+// sweepu_k_bubble spray_k.cu:3779-3880 (one axis, out-of-place)
+template <int A>
+__global__ void k_sweep_u(Grid g, const float* __restrict__ u, float* __restrict__ out, const float* __restrict__ ls,
+                          const int8_t* __restrict__ mark, int8_t sf, int64_t nf) {
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int hh = g.ny + (A == 1), d = g.nz + (A == 2);
+  int i, j, k;
+  ijk3(f, hh, d, i, j, k);
+  float res = u[f];
+  const int n[3] = {i, j, k};
+  const int lim = (A == 0) ? g.nx : (A == 1) ? g.ny : g.nz;
+  if (n[A] > 1 && n[A] < lim - 1) {
+    const int bi = i - (A == 0), bj = j - (A == 1), bk = k - (A == 2);
+    if (mark[cid(g, i, j, k)] != sf && mark[cid(g, bi, bj, bk)] != sf) {
+      const float lsc = ls[cid(g, i, j, k)], lsb = ls[cid(g, bi, bj, bk)];
+      constexpr int t1 = (A == 0) ? 1 : (A == 1) ? 0 : 1;
+      constexpr int t2 = (A == 0) ? 2 : (A == 1) ? 2 : 0;
+      const int lims[3] = {g.nx, g.ny, g.nz};
+      for (int di = -1; di <= 1; di += 2)
+        for (int dj = -1; dj <= 1; dj += 2)
+          for (int dk = -1; dk <= 1; dk += 2) {
+            const int dd[3] = {di, dj, dk};
+            if (n[t1] + dd[t1] < 0 || n[t1] + dd[t1] > lims[t1] - 1) continue;
+            if (n[t2] + dd[t2] < 0 || n[t2] + dd[t2] > lims[t2] - 1) continue;
+            float wv[3];
+            wv[A] = (float)(-dd[A]) * (lsc - lsb);
+            if (wv[A] < 0) continue;
+            int o1[3] = {0, 0, 0};
+            o1[t1] = dd[t1];
+            wv[t1] = (lsc + lsb - ls[cid(g, i + o1[0], j + o1[1], k + o1[2])] - ls[cid(g, bi + o1[0], bj + o1[1], bk + o1[2])]) * 0.5f;
+            if (wv[t1] < 0) continue;
+            int o2[3] = {0, 0, 0};
+            o2[t2] = dd[t2];
+            wv[t2] = (lsc + lsb - ls[cid(g, i + o2[0], j + o2[1], k + o2[2])] - ls[cid(g, bi + o2[0], bj + o2[1], bk + o2[2])]) * 0.5f;
+            if (wv[t2] < 0) continue;
+            float wx = wv[0], wy = wv[1], wz = wv[2];
+            float wsum = wx + wy + wz;
+            if (wsum == 0) wx = wy = wz = 1.0f / 3;
+            else { wx /= wsum; wy /= wsum; wz /= wsum; }
+            res = wx * u[fid(g, A, i + di, j, k)] + wy * u[fid(g, A, i, j + dj, k)] + wz * u[fid(g, A, i, j, k + dk)];
+          }
+    }
+  }
+  out[f] = res;
+}
+
+// setWaterBoundaryU_k spray_k.cu:956-995 (in place).  The ceiling rule reads
+// uz(k-1) after that face's own rule.
+__device__ __forceinline__ bool isb(const Grid& g, const int8_t* mark, int i, int j, int k) {
+  return mark[cid(g, i, j, k)] == BOUNDARY;
+}
+template <int A>
+__global__ void k_bc(Grid g, float* __restrict__ u, const int8_t* __restrict__ mark, int64_t nf) {
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int hh = g.ny + (A == 1), d = g.nz + (A == 2);
+  int i, j, k;
+  ijk3(f, hh, d, i, j, k);
+  if (A == 0) {
+    if (i <= 1 || i >= (g.nx + 1) - 2) u[f] = 0.0f;
+    else if (i > 1 && i < g.nx - 1 && (isb(g, mark, i, j, k) != isb(g, mark, i - 1, j, k))) u[f] = 0.0f;
+  } else if (A == 1) {
+    if (j <= 1 || j >= (g.ny + 1) - 2) u[f] = 0.0f;
+    else if (j > 0 && j < g.ny && (isb(g, mark, i, j, k) != isb(g, mark, i, j - 1, k))) u[f] = 0.0f;
+  } else {
+    const int zn = g.nz + 1;
+    if (k <= 1 || k >= zn - 1) u[f] = 0.0f;
+    else if (k == zn - 2) {
+      // post-rule value of uz(k-1): k-1 >= 2 is neither wall nor ceiling here
+      int kb = k - 1;
+      float b = u[f - 1];
+      if (kb <= 1) b = 0.0f;
+      else if (kb > 0 && kb < g.nz && (isb(g, mark, i, j, kb) != isb(g, mark, i, j, kb - 1))) b = 0.0f;
+      u[f] = (b < 0) ? b : 0.0f;
+    } else if (k > 0 && k < g.nz && (isb(g, mark, i, j, k) != isb(g, mark, i, j, k - 1))) u[f] = 0.0f;
+  }
+}
+
+// ---------------- divergence (a11) and gradient (a13) ----------------
+__device__ __forceinline__ int8_t mark_at(const Grid& g, const int8_t* mark, int i, int j, int k) {
+  return valid(g, i, j, k) ? mark[cid(g, i, j, k)] : BOUNDARY;
+}
+// cptdivergence_bubble2, spray_k.cu:3138-3263, in arithmetic R.
+template <class R>
+__global__ void k_divergence(Grid g, const float* __restrict__ wux, const float* __restrict__ wuy,
+                             const float* __restrict__ wuz, const float* __restrict__ aux,
+                             const float* __restrict__ auy, const float* __restrict__ auz,
+                             const int8_t* __restrict__ mark, const float* __restrict__ ls, int solid_c,
+                             R* __restrict__ div) {
+  CELL_LOOP_HEAD
+  R dv = 0;
+  const int8_t m = mark[idx];
+  if (m == FLUID || m == AIR) {
+    const float* wu[3] = {wux, wuy, wuz};
+    const float* au[3] = {aux, auy, auz};
+    R uu[6];
+    int q = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int s = 1; s >= 0; --s) {
+        int ni = i + (a == 0 ? (s ? 1 : -1) : 0), nj = j + (a == 1 ? (s ? 1 : -1) : 0), nk = k + (a == 2 ? (s ? 1 : -1) : 0);
+        int64_t fx = fid(g, a, i + (a == 0 && s), j + (a == 1 && s), k + (a == 2 && s));
+        int8_t mn = mark_at(g, mark, ni, nj, nk);
+        R v;
+        if (solid_c && mn == SOLID) v = R(0);
+        else if (m == FLUID && mn != AIR) v = (R)wu[a][fx];
+        else if (m == AIR && mn != FLUID) v = (R)au[a][fx];
+        else {
+          R lc = (R)ls[idx], ln = (R)ls[cid(g, ni, nj, nk)];
+          R th = (R(0.0f) - lc) / (ln - lc);
+          if (m == FLUID) v = th * (R)wu[a][fx] + (R(1) - th) * (R)au[a][fx];
+          else v = th * (R)au[a][fx] + (R(1) - th) * (R)wu[a][fx];
+        }
+        uu[q++] = v;
+      }
+    dv = (uu[0] - uu[1] + uu[2] - uu[3] + uu[4] - uu[5]) / (R)g.h;
+  }
+  div[idx] = dv;
+}
+
+// subGradPress spray_k.cu:198-224 (unit form, in R) or the 1/rho form (double).
+template <int A, class R>
+__global__ void k_subgrad(Grid g, float* __restrict__ u, const R* __restrict__ p, const int8_t* __restrict__ mark,
+                          int vardens, double inv_rho, int solid_c, int64_t nf) {
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int hh = g.ny + (A == 1), d = g.nz + (A == 2);
+  int i, j, k;
+  ijk3(f, hh, d, i, j, k);
+  const int n[3] = {i, j, k};
+  const int lim = (A == 0) ? g.nx : (A == 1) ? g.ny : g.nz;
+  if (!(n[A] > 0 && n[A] < lim)) return;
+  const int bi = i - (A == 0), bj = j - (A == 1), bk = k - (A == 2);
+  const int64_t c0 = cid(g, i, j, k), c1 = cid(g, bi, bj, bk);
+  if (solid_c && (mark[c0] == SOLID || mark[c1] == SOLID)) {
+    u[f] = 0.0f;
+    return;
+  }
+  if (!vardens) u[f] = (float)((R)u[f] - (p[c0] - p[c1]) / (R)g.h);
+  else u[f] = (float)((double)u[f] - inv_rho * (((double)p[c0] - (double)p[c1]) / (double)g.h));
+}
+
+// ---------------- heat grid stages (a16) ----------------
+// mapHeatp2g_hash + sumHeat, spray_k.cu:4619-4653
+__global__ void k_heat_p2g(Grid g, const f3* __restrict__ pos, const float* __restrict__ temp,
+                           const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
+                           const uint8_t* __restrict__ cmask, float deft, float* __restrict__ tp) {
+  CELL_LOOP_HEAD
+  const float scale = 1.0f / g.h;
+  f3 gp = mk3((float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f);
+  const float RE = 1.4f, RE2 = RE * RE;
+  float weight = 0, hs = 0;
+  for (int di = -1; di <= 1; ++di)
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int dk = -1; dk <= 1; ++dk) {
+        if (!valid(g, i + di, j + dj, k + dk)) continue;
+        int64_t c = cid(g, i + di, j + dj, k + dk);
+        if (!cmask[c]) continue;
+        for (uint32_t p = cstart[c]; p < cend[c]; ++p) {
+          f3 x = pos[p];
+          f3 dv = mk3((x.x - g.gx0) * scale - gp.x, (x.y - g.gy0) * scale - gp.y, (x.z - g.gz0) * scale - gp.z);
+          float w = fmaxf(RE2 / fmaxf(dot3(dv, dv), 0.0001f) - 1.0f, 0.0f);
+          weight += w;
+          hs += w * temp[p];
+        }
+      }
+  tp[idx] = (weight > 0) ? (hs / weight) : deft;
+}
+
+// addHeatAtBottom spray_k.cu:4492-4504, then save T and form b (compb_heat
+// 4507-4523) for the implicit solve.
+__global__ void k_heat_pre(Grid g, float* __restrict__ tp, float* __restrict__ tsave, float* __restrict__ b,
+                           const int8_t* __restrict__ mark, int add_bottom, float inc, float clampv, float a0, float a1,
+                           float a2, float a3, float a4, float a5, float dt) {
+  CELL_LOOP_HEAD
+  float t = tp[idx];
+  if (add_bottom) {
+    if (i >= 1 && i < g.nx - 1 && j >= 1 && j < g.ny - 1 && k <= 3) t += inc;
+    t = fminf(clampv, t);
+  }
+  tp[idx] = t;
+  tsave[idx] = t;
+  const float al[6] = {a0, a1, a2, a3, a4, a5};
+  int8_t m = mark[idx];
+  b[idx] = (m == BOUNDARY) ? 0.0f : (float)((double)t * ((double)g.h * (double)g.h) / (double)al[m] / (double)dt);
+}
+
+// One Jacobi sweep of the heat operator (computeAx_heat, spray_k.cu:4526-4567).
+__global__ void k_heat_jacobi(Grid g, const float* __restrict__ x, float* __restrict__ xn, const float* __restrict__ b,
+                              const int8_t* __restrict__ mark, float a0, float a1, float a2, float a3, float a4,
+                              float a5, float dt, int vac_neumann) {
+  CELL_LOOP_HEAD
+  int8_t m = mark[idx];
+  if (m == BOUNDARY) { xn[idx] = 0.0f; return; }
+  const float al[6] = {a0, a1, a2, a3, a4, a5};
+  double dg = (double)g.h * (double)g.h / (double)al[m] / (double)dt + 6.0, s = (double)b[idx];
+  const int off[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    int8_t mn = mark_at(g, mark, i + off[q][0], j + off[q][1], k + off[q][2]);
+    if (mn == BOUNDARY || (vac_neumann && mn == VACUUM)) dg -= 1.0;
+    else s += (double)x[cid(g, i + off[q][0], j + off[q][1], k + off[q][2])];
+  }
+  xn[idx] = (float)(s / dg);
+}
+
+__global__ void k_heat_start(int64_t G, const float* __restrict__ tsave, const int8_t* __restrict__ mark,
+                             float* __restrict__ x) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < G) x[idx] = (mark[idx] == BOUNDARY) ? 0.0f : tsave[idx];
+}
+
+// setBoundaryHeat spray_k.cu:4585-4605 (reads the pass's input) ; compTpChange 4607-4617
+__global__ void k_heat_post(Grid g, const float* __restrict__ tin, float* __restrict__ tp,
+                            const float* __restrict__ tsave, float* __restrict__ tch, const int8_t* __restrict__ mark,
+                            int extend) {
+  CELL_LOOP_HEAD
+  float t = tin[idx];
+  if (extend) {
+    if (i == g.nx - 1) t = tin[cid(g, i - 1, j, k)];
+    else if (i == 0) t = tin[cid(g, i + 1, j, k)];
+    else if (j == g.ny - 1) t = tin[cid(g, i, j - 1, k)];
+    else if (j == 0) t = tin[cid(g, i, j + 1, k)];
+    else if (k == g.nz - 1) t = tin[cid(g, i, j, k - 1)];
+    else if (k == 0) t = tin[cid(g, i, j, k + 1)];
+  }
+  tp[idx] = t;
+  tch[idx] = (mark[idx] != BOUNDARY) ? t - tsave[idx] : 0.0f;
+}
+
+// ---------------- solid distance + velocity (a15 support) ----------------
+// initSolidPhi spray_k.cu:724-744
+__global__ void k_init_solid_phi(int64_t G, const uint8_t* __restrict__ cmask, float* __restrict__ phi, float big) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < G) phi[idx] = (cmask[idx] & M_SOLID) ? -0.5f : big;
+}
+
+// ---------------- marching cubes (a18) ----------------
+// classifyVoxel spray_k.cu:1853-1888 on lsfluid at cell centres; marks the
+// edges its triangles use (markActiveEdge_MC 2172-2220).
+__constant__ int c_eo[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
+                                {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
+__constant__ signed char c_tri[256][16];
+__constant__ unsigned char c_nv[256];
+
+__device__ __forceinline__ uint32_t mc_case_of(const Grid& g, const float* f, int i, int j, int k) {
+  const int64_t si = (int64_t)g.ny * g.nz, sj = g.nz;
+  const float* b = f + cid(g, i, j, k);
+  float v[8] = {b[0], b[si], b[si + sj], b[sj], b[1], b[si + 1], b[si + sj + 1], b[sj + 1]};
+  uint32_t ci = 0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) ci += (uint32_t)(v[q] < 0.0f) << q;
+  return ci;
+}
+
+__global__ void k_mc_classify(Grid g, const float* __restrict__ f, uint8_t* __restrict__ cases,
+                              uint32_t* __restrict__ tcount, uint32_t* __restrict__ edge_used, int64_t V,
+                              uint32_t* __restrict__ active) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int nv = 0;
+  if (v < V) {
+    const int VY = g.ny - 1, VZ = g.nz - 1;
+    int i, j, k;
+    ijk3(v, VY, VZ, i, j, k);
+    uint32_t ci = mc_case_of(g, f, i, j, k);
+    cases[v] = (uint8_t)ci;
+    nv = c_nv[ci];
+    tcount[v] = (uint32_t)(nv / 3);
+    for (int t = 0; t < nv; ++t) {
+      int e = c_tri[ci][t];
+      edge_used[3 * cid(g, i + c_eo[e][0], j + c_eo[e][1], k + c_eo[e][2]) + c_eo[e][3]] = 1u;
+    }
+  }
+  unsigned long long m = __ballot(nv > 0);
+  if (m && __lane_id() == (unsigned)(__ffsll((long long)m) - 1)) atomicAdd(active, (uint32_t)__popcll(m));
+}
+
+// vertexInterp spray_k.cu:1905-1910 along each used edge; edge_id holds the
+// exclusive scan of the used flags (= vertex index).
+__global__ void k_mc_verts(Grid g, const float* __restrict__ f, const uint32_t* __restrict__ used,
+                           const uint32_t* __restrict__ vid, float* __restrict__ verts) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= 3 * g.G || !used[e]) return;
+  int64_t n = e / 3;
+  int a = (int)(e % 3);
+  int i, j, k;
+  ijk_of(g, n, i, j, k);
+  int i2 = i + (a == 0), j2 = j + (a == 1), k2 = k + (a == 2);
+  float f0 = f[n], f1 = f[cid(g, i2, j2, k2)];
+  float t = (0.0f - f0) / (f1 - f0);
+  const float h = g.h;
+  float p0x = ((float)i + 0.5f) * h + g.gx0, p0y = ((float)j + 0.5f) * h + g.gy0, p0z = ((float)k + 0.5f) * h + g.gz0;
+  float p1x = ((float)i2 + 0.5f) * h + g.gx0, p1y = ((float)j2 + 0.5f) * h + g.gy0, p1z = ((float)k2 + 0.5f) * h + g.gz0;
+  uint32_t o = vid[e];
+  verts[3 * (int64_t)o + 0] = p0x + (p1x - p0x) * t;
+  verts[3 * (int64_t)o + 1] = p0y + (p1y - p0y) * t;
+  verts[3 * (int64_t)o + 2] = p0z + (p1z - p0z) * t;
+}
+
+// generateTriangles_indices spray_k.cu:2074-2170 (indices = scanned edge ids)
+__global__ void k_mc_tris(Grid g, const uint8_t* __restrict__ cases, const uint32_t* __restrict__ toff,
+                          const uint32_t* __restrict__ vid, uint32_t* __restrict__ tris, int64_t V) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= V) return;
+  uint32_t ci = cases[v];
+  int nv = c_nv[ci];
+  if (!nv) return;
+  const int VY = g.ny - 1, VZ = g.nz - 1;
+  int i, j, k;
+  ijk3(v, VY, VZ, i, j, k);
+  int64_t w = 3 * (int64_t)toff[v];
+  for (int t = 0; t < nv; ++t) {
+    int e = c_tri[ci][t];
+    tris[w + t] = vid[3 * cid(g, i + c_eo[e][0], j + c_eo[e][1], k + c_eo[e][2]) + c_eo[e][3]];
+  }
+}
+
+}  // namespace
+
+// ================= host stage drivers =================
+static void sweep_n(Ctx& c, float* phi, int typeflag, int n) {
+  for (int it = 0; it < n; ++it) {
+    launch(c, "sweep", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_sweep, c.g, (const float*)phi, c.ctmp,
+           (const int8_t*)c.mark, typeflag);
+    hipMemcpyAsync(phi, c.ctmp, (size_t)c.g.G * 4, hipMemcpyDeviceToDevice, c.stream);
+  }
+}
+
+static void redistance(Ctx& c, float* ls) {
+  const int64_t G = c.g.G;
+  launch(c, "ls_prepare", dim3(nblocks(G, 256)), dim3(256), 0, k_ls_prepare, G, ls, c.mark, c.g.h, 0);
+  sweep_n(c, ls, FLUID, 5);
+  launch(c, "ls_back", dim3(nblocks(G, 256)), dim3(256), 0, k_ls_back, G, ls, c.g.h, 0);
+  launch(c, "ls_prepare", dim3(nblocks(G, 256)), dim3(256), 0, k_ls_prepare, G, ls, c.mark, c.g.h, 1);
+  sweep_n(c, ls, FLUID, 5);
+  launch(c, "ls_back", dim3(nblocks(G, 256)), dim3(256), 0, k_ls_back, G, ls, c.g.h, 1);
+}
+
+void run_levelset(Ctx& c) {
+  const int64_t G = c.g.G;
+  const int cur = c.cur;
+  launch(c, "genlevelset", dim3(nblocks(G, 256)), dim3(256), 0, k_genlevelset, c.g, (const f3*)c.pos[cur],
+         (const int8_t*)c.flag[cur], (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask,
+         c.lsfluid, c.lsair, c.mark);
+  redistance(c, c.lsfluid);
+  redistance(c, c.lsair);
+  launch(c, "merge_mark", dim3(nblocks(G, 256)), dim3(256), 0, k_merge, c.g, (const float*)c.lsfluid,
+         (const float*)c.lsair, c.lsmerge, c.mark);
+  launch(c, "phigrad", dim3(nblocks(G, 256)), dim3(256), 0, k_phigrad, c.g, (const float*)c.lsmerge, c.phig[0],
+         c.phig[1], c.phig[2]);
+  if (c.cfg.flags_from_counts)
+    launch(c, "mark_counts", dim3(nblocks(G, 256)), dim3(256), 0, k_mark_counts, c.g, (const uint32_t*)c.cstart,
+           (const uint32_t*)c.cend, (const int8_t*)c.flag[cur], c.mark);
+  if (c.cfg.solid_constraint)
+    launch(c, "mark_solid", dim3(nblocks(G, 256)), dim3(256), 0, k_mark_solid, c.g, (const uint32_t*)c.cstart,
+           (const uint32_t*)c.cend, (const int8_t*)c.flag[cur], (const uint8_t*)c.cmask, c.mark);
+}
+
+template <bool AIRPH>
+static void p2g_phase(Ctx& c, float* const* out) {
+  const int cur = c.cur;
+  const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
+  const f3* pos = c.pos[cur];
+  const f3* vel = c.vel[cur];
+  const float* mass = c.mass[cur];
+  const int8_t* flag = c.flag[cur];
+  const char* nm = AIRPH ? "p2g_air" : "p2g_liquid";
+  launch(c, nm, dim3(nblocks(fc[0], 256)), dim3(256), 0, k_p2g<0, AIRPH>, c.g, pos, vel, mass, flag,
+         (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask, out[0], fc[0]);
+  launch(c, nm, dim3(nblocks(fc[1], 256)), dim3(256), 0, k_p2g<1, AIRPH>, c.g, pos, vel, mass, flag,
+         (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask, out[1], fc[1]);
+  launch(c, nm, dim3(nblocks(fc[2], 256)), dim3(256), 0, k_p2g<2, AIRPH>, c.g, pos, vel, mass, flag,
+         (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask, out[2], fc[2]);
+}
+
+void run_p2g(Ctx& c) {
+  p2g_phase<false>(c, c.wu);
+  p2g_phase<true>(c, c.au);
+  const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
+  for (int a = 0; a < 3; ++a) {
+    hipMemcpyAsync(c.wuo[a], c.wu[a], (size_t)fc[a] * 4, hipMemcpyDeviceToDevice, c.stream);
+    hipMemcpyAsync(c.auo[a], c.au[a], (size_t)fc[a] * 4, hipMemcpyDeviceToDevice, c.stream);
+  }
+}
+
+void run_solid_phi_and_velocity(Ctx& c) {
+  const int64_t G = c.g.G;
+  launch(c, "init_solid_phi", dim3(nblocks(G, 256)), dim3(256), 0, k_init_solid_phi, G, (const uint8_t*)c.cmask,
+         c.phisolid, (float)(3 * c.g.nx));
+  sweep_n(c, c.phisolid, -1, 3);
+  p2g_phase<false>(c, c.su);
+}
+
+static void sweep_phi(Ctx& c, float* phi, int8_t type) {
+  const int64_t G = c.g.G;
+  launch(c, "initphi", dim3(nblocks(G, 256)), dim3(256), 0, k_initphi, G, (const int8_t*)c.mark, phi, type,
+         (float)(c.g.nx * 3));
+  sweep_n(c, phi, type, 3);
+}
+
+static void sweep_u(Ctx& c, float* const* u, const float* phi, int8_t sf) {
+  const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
+  for (int it = 0; it < 2; ++it) {
+    launch(c, "sweep_u", dim3(nblocks(fc[0], 256)), dim3(256), 0, k_sweep_u<0>, c.g, (const float*)u[0], c.ftmp[0],
+           phi, (const int8_t*)c.mark, sf, fc[0]);
+    launch(c, "sweep_u", dim3(nblocks(fc[1], 256)), dim3(256), 0, k_sweep_u<1>, c.g, (const float*)u[1], c.ftmp[1],
+           phi, (const int8_t*)c.mark, sf, fc[1]);
+    launch(c, "sweep_u", dim3(nblocks(fc[2], 256)), dim3(256), 0, k_sweep_u<2>, c.g, (const float*)u[2], c.ftmp[2],
+           phi, (const int8_t*)c.mark, sf, fc[2]);
+    for (int a = 0; a < 3; ++a)
+      hipMemcpyAsync(u[a], c.ftmp[a], (size_t)fc[a] * 4, hipMemcpyDeviceToDevice, c.stream);
+  }
+}
+
+static void bc(Ctx& c, float* const* u) {
+  const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
+  launch(c, "bc", dim3(nblocks(fc[0], 256)), dim3(256), 0, k_bc<0>, c.g, u[0], (const int8_t*)c.mark, fc[0]);
+  launch(c, "bc", dim3(nblocks(fc[1], 256)), dim3(256), 0, k_bc<1>, c.g, u[1], (const int8_t*)c.mark, fc[1]);
+  launch(c, "bc", dim3(nblocks(fc[2], 256)), dim3(256), 0, k_bc<2>, c.g, u[2], (const int8_t*)c.mark, fc[2]);
+}
+
+void run_extrapolate_bc(Ctx& c, bool recompute_phi) {
+  if (recompute_phi) sweep_phi(c, c.phiair, AIR);
+  sweep_u(c, c.au, c.phiair, AIR);
+  if (recompute_phi) sweep_phi(c, c.phifluid, FLUID);
+  sweep_u(c, c.wu, c.phifluid, FLUID);
+  bc(c, c.au);
+  bc(c, c.wu);
+}
+
+void run_divergence(Ctx& c) {
+  const int64_t G = c.g.G;
+  if (c.cfg.precision == MPF_FP64)
+    launch(c, "divergence", dim3(nblocks(G, 256)), dim3(256), 0, k_divergence<double>, c.g, (const float*)c.wu[0],
+           (const float*)c.wu[1], (const float*)c.wu[2], (const float*)c.au[0], (const float*)c.au[1],
+           (const float*)c.au[2], (const int8_t*)c.mark, (const float*)c.lsmerge, c.cfg.solid_constraint,
+           (double*)c.div);
+  else
+    launch(c, "divergence", dim3(nblocks(G, 256)), dim3(256), 0, k_divergence<float>, c.g, (const float*)c.wu[0],
+           (const float*)c.wu[1], (const float*)c.wu[2], (const float*)c.au[0], (const float*)c.au[1],
+           (const float*)c.au[2], (const int8_t*)c.mark, (const float*)c.lsmerge, c.cfg.solid_constraint,
+           (float*)c.div);
+}
+
+template <class R>
+static void subgrad_t(Ctx& c, float* const* u, double inv_rho) {
+  const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
+  const int vd = c.cfg.op == MPF_OPERATOR_VARDENS, sc = c.cfg.solid_constraint;
+  const R* p = (const R*)c.press;
+  launch(c, "subgrad", dim3(nblocks(fc[0], 256)), dim3(256), 0, k_subgrad<0, R>, c.g, u[0], p, (const int8_t*)c.mark, vd,
+         inv_rho, sc, fc[0]);
+  launch(c, "subgrad", dim3(nblocks(fc[1], 256)), dim3(256), 0, k_subgrad<1, R>, c.g, u[1], p, (const int8_t*)c.mark, vd,
+         inv_rho, sc, fc[1]);
+  launch(c, "subgrad", dim3(nblocks(fc[2], 256)), dim3(256), 0, k_subgrad<2, R>, c.g, u[2], p, (const int8_t*)c.mark, vd,
+         inv_rho, sc, fc[2]);
+}
+
+void run_subgrad(Ctx& c) {
+  double il = 1.0 / (double)c.cfg.rho_liquid, ia = 1.0 / (double)c.cfg.rho_air;
+  if (c.cfg.precision == MPF_FP32) {
+    subgrad_t<float>(c, c.wu, il);
+    subgrad_t<float>(c, c.au, ia);
+  } else {
+    subgrad_t<double>(c, c.wu, il);
+    subgrad_t<double>(c, c.au, ia);
+  }
+}
+
+void run_heat_grid_pre(Ctx& c) {
+  const mpf_config& k = c.cfg;
+  const int64_t G = c.g.G;
+  launch(c, "heat_p2g", dim3(nblocks(G, 256)), dim3(256), 0, k_heat_p2g, c.g, (const f3*)c.pos[c.cur],
+         (const float*)c.temp[c.cur], (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask,
+         k.default_temperature, c.tp);
+  launch(c, "heat_pre", dim3(nblocks(G, 256)), dim3(256), 0, k_heat_pre, c.g, c.tp, c.tsave, c.ctmp,
+         (const int8_t*)c.mark, k.heat_add_bottom, k.heat_increase_bottom, k.heat_bottom_clamp, k.heat_alpha[0],
+         k.heat_alpha[1], k.heat_alpha[2], k.heat_alpha[3], k.heat_alpha[4], k.heat_alpha[5], k.dt);
+}
+
+void run_heat_jacobi(Ctx& c, int iters) {
+  const mpf_config& k = c.cfg;
+  const int64_t G = c.g.G;
+  float* x = (float*)c.vx;
+  float* xn = (float*)c.vr;
+  launch(c, "heat_start", dim3(nblocks(G, 256)), dim3(256), 0, k_heat_start, G, (const float*)c.tsave,
+         (const int8_t*)c.mark, x);
+  for (int it = 0; it < iters; ++it) {
+    launch(c, "heat_jacobi", dim3(nblocks(G, 256)), dim3(256), 0, k_heat_jacobi, c.g, (const float*)x, xn,
+           (const float*)c.ctmp, (const int8_t*)c.mark, k.heat_alpha[0], k.heat_alpha[1], k.heat_alpha[2],
+           k.heat_alpha[3], k.heat_alpha[4], k.heat_alpha[5], k.dt, k.heat_vacuum_neumann);
+    float* t = x; x = xn; xn = t;
+  }
+  hipMemcpyAsync(c.tp, x, (size_t)G * 4, hipMemcpyDeviceToDevice, c.stream);
+}
+
+void run_heat_grid_post(Ctx& c) {
+  const int64_t G = c.g.G;
+  hipMemcpyAsync(c.ctmp, c.tp, (size_t)G * 4, hipMemcpyDeviceToDevice, c.stream);
+  launch(c, "heat_post", dim3(nblocks(G, 256)), dim3(256), 0, k_heat_post, c.g, (const float*)c.ctmp, c.tp,
+         (const float*)c.tsave, c.tpchange, (const int8_t*)c.mark, c.cfg.heat_extend_boundary);
+}
+
+int run_marching_cubes(Ctx& c, mpf_mc_info* info) {
+  static bool tables_loaded = false;  // per process; the tables are constants
+  if (!tables_loaded) {
+    hipMemcpyToSymbol(HIP_SYMBOL(c_tri), MC_TRI_TABLE, sizeof(MC_TRI_TABLE));
+    hipMemcpyToSymbol(HIP_SYMBOL(c_nv), MC_NUM_VERTS, sizeof(MC_NUM_VERTS));
+    tables_loaded = true;
+  }
+  const int64_t G = c.g.G;
+  const int64_t V = (int64_t)(c.g.nx - 1) * (c.g.ny - 1) * (c.g.nz - 1);
+  c.mc_nvox = V;
+  hipMemsetAsync(c.mc_edge, 0, (size_t)3 * G * 4, c.stream);
+  hipMemsetAsync(c.counters + 3, 0, 4, c.stream);
+  launch(c, "mc_classify", dim3(nblocks(V, 256)), dim3(256), 0, k_mc_classify, c.g, (const float*)c.lsfluid, c.mc_case,
+         c.mc_tcount, c.mc_edge, V, c.counters + 3);
+  // vertex ids = exclusive scan of the used-edge flags (second half of mc_edge)
+  uint32_t* vid = c.mc_edge + 3 * G;
+  int64_t nv = run_scan_u32(c, c.mc_edge, vid, 3 * G, true);
+  // triangle offsets = exclusive scan of per-voxel triangle counts (second half of mc_tcount)
+  int64_t nt = run_scan_u32(c, c.mc_tcount, c.mc_tcount + V, V, true);
+  if (nv < 0 || nt < 0) return MPF_HIP_ERROR;
+  hipMemcpyAsync(c.counters_host + 3, c.counters + 3, 4, hipMemcpyDeviceToHost, c.stream);
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "marching_cubes")) return MPF_HIP_ERROR;
+  if (nv > c.cfg.mc_max_verts || nt > c.cfg.mc_max_tris) {
+    c.err = "marching cubes mesh exceeds mc_max_verts/mc_max_tris";
+    return MPF_CAPACITY;
+  }
+  launch(c, "mc_verts", dim3(nblocks(3 * G, 256)), dim3(256), 0, k_mc_verts, c.g, (const float*)c.lsfluid,
+         (const uint32_t*)c.mc_edge, (const uint32_t*)vid, c.mc_verts);
+  launch(c, "mc_tris", dim3(nblocks(V, 256)), dim3(256), 0, k_mc_tris, c.g, (const uint8_t*)c.mc_case,
+         (const uint32_t*)(c.mc_tcount + V), (const uint32_t*)vid, c.mc_tris, V);
+  c.mc_nv = nv;
+  c.mc_nt = nt;
+  if (info) {
+    info->verts = nv;
+    info->tris = nt;
+    info->active_voxels = (int64_t)c.counters_host[3];
+  }
+  return MPF_OK;
+}
+
+}  // namespace mpf

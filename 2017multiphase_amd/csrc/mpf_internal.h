@@ -1,0 +1,206 @@
+// Internal declarations of the MI355X FLIP multiphase library (gfx950).
+//
+// Device code follows the per-element expressions of the reference kernels
+// (Multiphase/Multiphase/spray_k.cu) in the same operation order; the library
+// is built with -ffp-contract=off so those expressions round exactly like the
+// CPU oracle wherever an integer (hash key, flag, case index) is derived from
+// floating point.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mpflip.h"
+
+namespace mpf {
+
+constexpr int8_t FLUID = MPF_TYPEFLUID, AIR = MPF_TYPEAIR, BOUNDARY = MPF_TYPEBOUNDARY,
+                 VACUUM = MPF_TYPEVACUUM, AIRSOLO = MPF_TYPEAIRSOLO, SOLID = MPF_TYPESOLID;
+
+// Per-cell occupancy mask built after every sort (bit set when the cell holds
+// at least one particle of that kind); lets gathers skip cells cheaply.
+constexpr uint8_t M_FLUIDSOLID = 1, M_AIR = 2, M_AIRSOLO = 4, M_SOLID = 8, M_ANY = 16;
+
+struct f3 {
+  float x, y, z;
+};
+
+// Everything a kernel needs about the grid and the configuration; passed by
+// value as a kernel argument (replaces __constant__ FlipConstant dparam and
+// NX/NY/NZ, spray_k.cu:8-14).
+struct Grid {
+  int nx, ny, nz;
+  int64_t G, FX, FY, FZ;
+  float h;
+  float gx0, gy0, gz0;  // gmin
+  float gx1, gy1, gz1;  // gmax
+};
+
+__host__ __device__ inline int64_t cid(const Grid& g, int i, int j, int k) {
+  return ((int64_t)i * g.ny + j) * g.nz + k;
+}
+__host__ __device__ inline bool valid(const Grid& g, int i, int j, int k) {
+  return !(i < 0 || i > g.nx - 1 || j < 0 || j > g.ny - 1 || k < 0 || k > g.nz - 1);
+}
+__host__ __device__ inline void ijk_of(const Grid& g, int64_t idx, int& i, int& j, int& k) {
+  i = (int)(idx / ((int64_t)g.nz * g.ny));
+  j = (int)((idx / g.nz) % g.ny);
+  k = (int)(idx % g.nz);
+}
+__host__ __device__ inline void ijk3(int64_t idx, int hh, int d, int& i, int& j, int& k) {
+  i = (int)(idx / d / hh);
+  j = (int)((idx / d) % hh);
+  k = (int)(idx % d);
+}
+// Face array index for axis a (ux: (nx+1,ny,nz), uy: (nx,ny+1,nz), uz: (nx,ny,nz+1)).
+__host__ __device__ inline int64_t fid(const Grid& g, int a, int i, int j, int k) {
+  if (a == 0) return ((int64_t)i * g.ny + j) * g.nz + k;
+  if (a == 1) return ((int64_t)i * (g.ny + 1) + j) * g.nz + k;
+  return ((int64_t)i * g.ny + j) * (g.nz + 1) + k;
+}
+
+// Device-side scalars of an iterative solve; lives in device memory so the
+// recurrence runs without host round trips.
+struct SolveScalars {
+  double rz, rz_new, pq, rr, bb, alpha, beta, rr_true;
+  int iter, done, converged, nonfinite, refinements, refine_now, pad0, pad1;
+};
+
+struct KernelStat {
+  std::string name;
+  int64_t count = 0;
+  double ms = 0;
+};
+
+struct PendingEvent {
+  int stat;
+  hipEvent_t a, b;
+};
+
+struct Ctx {
+  mpf_config cfg;
+  Grid g;
+  hipStream_t stream = nullptr;
+  std::string err;
+  int device = 0;
+
+  int64_t P = 0, cap = 0;
+  int cur = 0;  // which particle buffer is current
+  // particles, double-buffered for the sort
+  f3* pos[2] = {};
+  f3* vel[2] = {};
+  int8_t* flag[2] = {};
+  float* mass[2] = {};
+  float* temp[2] = {};
+  float* lheat[2] = {};
+  float* solub[2] = {};
+  float* gas[2] = {};
+  uint32_t* key = nullptr;    // per particle: unsorted keys
+  uint32_t* perm = nullptr;   // sorted slot -> old index (gridIndex)
+  uint32_t* ghash = nullptr;  // sorted keys (gridHash)
+  bool sorted = false;
+
+  // cells
+  uint32_t* count = nullptr;  // per-cell counts, then scatter cursors
+  uint32_t* cstart = nullptr;
+  uint32_t* cend = nullptr;
+  uint8_t* cmask = nullptr;
+  int8_t* mark = nullptr;
+  float *lsfluid = nullptr, *lsair = nullptr, *lsmerge = nullptr;
+  float *phig[3] = {};
+  float *phifluid = nullptr, *phiair = nullptr, *phisolid = nullptr;
+  float* ctmp = nullptr;  // scratch cell field for two-buffer sweeps
+  void* div = nullptr;    // float, or double under MPF_FP64
+  void* press = nullptr;  // float under MPF_FP32, else double
+  float *tp = nullptr, *tpchange = nullptr, *tsave = nullptr;
+  // iterative solve work vectors (sized for double)
+  void *vx = nullptr, *vr = nullptr, *vp = nullptr, *vq = nullptr, *vdinv = nullptr;
+  double* vx64 = nullptr;
+  double* partials = nullptr;
+  int npart = 0;
+  SolveScalars* scal = nullptr;
+  SolveScalars* scal_host = nullptr;  // pinned
+  uint32_t* counters = nullptr;       // [0] reduction arrival, [1..] misc counters
+  uint32_t* counters_host = nullptr;  // pinned
+
+  // faces
+  float* wu[3] = {};
+  float* au[3] = {};
+  float* wuo[3] = {};
+  float* auo[3] = {};
+  float* su[3] = {};
+  float* ftmp[3] = {};
+
+  // scan scratch
+  uint32_t* scan_blk = nullptr;
+  int64_t scan_blk_n = 0;
+  uint32_t* ctmp_u32 = nullptr;  // G
+  // marching cubes
+  uint8_t* mc_case = nullptr;
+  uint32_t* mc_edge = nullptr;  // 3G flags -> vertex ids
+  uint32_t* mc_tcount = nullptr;  // per voxel triangle offsets
+  float* mc_verts = nullptr;
+  uint32_t* mc_tris = nullptr;
+  int64_t mc_nv = 0, mc_nt = 0, mc_nvox = 0;
+  // nucleation seeds
+  int32_t* seeds = nullptr;
+  int32_t nseeds = 0;
+
+  // timing
+  bool timing = false;
+  std::vector<KernelStat> stats;
+  std::vector<PendingEvent> pending;
+  std::vector<hipEvent_t> event_pool;
+
+  int stat_index(const char* name);
+  void flush_timing();
+  hipEvent_t get_event();
+};
+
+// ---- launch helper: optional HIP-event timing around every launch ----------
+bool launch_begin(Ctx& c, const char* name, PendingEvent& pe);
+void launch_end(Ctx& c, PendingEvent& pe, bool timed);
+
+template <typename... KArgs, typename... Args>
+inline void launch(Ctx& c, const char* name, dim3 grid, dim3 block, size_t smem, void (*k)(KArgs...),
+                   Args... args) {
+  if (grid.x == 0 || grid.y == 0 || grid.z == 0) return;
+  PendingEvent pe;
+  bool timed = launch_begin(c, name, pe);
+  hipLaunchKernelGGL(k, grid, block, smem, c.stream, static_cast<KArgs>(args)...);
+  launch_end(c, pe, timed);
+}
+
+inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// ---- stage entry points implemented in the .hip files ----------------------
+// sort.hip
+void run_hash_sort(Ctx& c);
+void run_cell_mask(Ctx& c);
+int64_t run_scan_u32(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n, bool want_total);
+// particles.hip
+void run_apply_forces(Ctx& c);
+void run_g2p(Ctx& c);
+void run_advect(Ctx& c);
+void run_heat_g2p(Ctx& c);
+int run_phase_change(Ctx& c, int frame, mpf_phase_info* info);
+// grid.hip
+void run_levelset(Ctx& c);
+void run_p2g(Ctx& c);
+void run_extrapolate_bc(Ctx& c, bool recompute_phi);
+void run_divergence(Ctx& c);
+void run_subgrad(Ctx& c);
+void run_heat_grid_pre(Ctx& c);
+void run_heat_grid_post(Ctx& c);
+void run_heat_jacobi(Ctx& c, int iters);
+int run_marching_cubes(Ctx& c, mpf_mc_info* info);
+void run_solid_phi_and_velocity(Ctx& c);
+// pcg.hip
+int run_pressure_solve(Ctx& c, mpf_solve_info* info);
+int run_heat_cg(Ctx& c, mpf_solve_info* info);
+
+bool hip_ok(Ctx& c, hipError_t e, const char* what);
+
+}  // namespace mpf

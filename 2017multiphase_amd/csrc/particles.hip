@@ -1,0 +1,464 @@
+// Particle-side kernels: external force (a5), G2P / FLIP-PIC blend (a14),
+// advection with solid collision or RK2 (a15), heat G2P with latent heat
+// (a16, particle half) and phase change with nucleation (a17).
+#include "mpf_internal.h"
+
+namespace mpf {
+
+namespace {
+
+__device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 add3(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 mul3(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
+
+// trilinear, spray_k.cu:111-122
+__device__ __forceinline__ float trilinear(const float* __restrict__ u, float x, float y, float z, int w, int hh,
+                                           int d) {
+  x = fmaxf(0.0f, fminf(x, (float)w));
+  y = fmaxf(0.0f, fminf(y, (float)hh));
+  z = fmaxf(0.0f, fminf(z, (float)d));
+  int i = (int)fminf(x, (float)(w - 2));
+  int j = (int)fminf(y, (float)(hh - 2));
+  int k = (int)fminf(z, (float)(d - 2));
+  const int64_t sj = d, si = (int64_t)hh * d;
+  const float* b = u + i * si + j * sj + k;
+  float ix1 = (float)(i + 1) - x, ix0 = x - (float)i;
+  float jy1 = (float)(j + 1) - y, jy0 = y - (float)j;
+  float kz1 = (float)(k + 1) - z, kz0 = z - (float)k;
+  return kz1 * (jy1 * (ix1 * b[0] + ix0 * b[si]) + jy0 * (ix1 * b[sj] + ix0 * b[si + sj])) +
+         kz0 * (jy1 * (ix1 * b[1] + ix0 * b[si + 1]) + jy0 * (ix1 * b[sj + 1] + ix0 * b[si + sj + 1]));
+}
+// getParticleVelFromGrid, spray_k.cu:227-240
+__device__ __forceinline__ f3 vel_from_grid(const Grid& g, f3 p, const float* ux, const float* uy, const float* uz) {
+  float x = (p.x - g.gx0) / g.h, y = (p.y - g.gy0) / g.h, z = (p.z - g.gz0) / g.h;
+  f3 v;
+  v.x = trilinear(ux, x, y - 0.5f, z - 0.5f, g.nx + 1, g.ny, g.nz);
+  v.y = trilinear(uy, x - 0.5f, y, z - 0.5f, g.nx, g.ny + 1, g.nz);
+  v.z = trilinear(uz, x - 0.5f, y - 0.5f, z, g.nx, g.ny, g.nz + 1);
+  return v;
+}
+// getScaleFromFrid, spray_k.cu:139-151
+__device__ __forceinline__ float scalar_from_grid(const Grid& g, f3 p, const float* f) {
+  float x = (p.x - g.gx0) / g.h, y = (p.y - g.gy0) / g.h, z = (p.z - g.gz0) / g.h;
+  return trilinear(f, x - 0.5f, y - 0.5f, z - 0.5f, g.nx, g.ny, g.nz);
+}
+
+struct Faces3 {
+  const float* u[3];
+};
+
+// addgravityforce_k spray_k.cu:330-338 ; addbuoyancyforce_vel spray_k.cu:356-371
+__global__ void k_forces(int64_t P, f3* __restrict__ vel, const int8_t* __restrict__ flag, float dt, f3 gr, int kick,
+                         float vmax, float rate_air, float rate_solo) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  int8_t fl = flag[p];
+  f3 v = vel[p];
+  if (fl == FLUID || fl == SOLID) v = add3(v, mul3(gr, dt));
+  if (kick) {
+    float rate = fmaxf(vmax - v.z, 0.0f) / vmax;
+    if (fl == AIR) v.z -= dt * gr.z * rate * rate_air;
+    else if (fl == AIRSOLO) v.z -= dt * gr.z * rate * rate_solo;
+    else if (fl == SOLID) v.z += dt * gr.z * 0.1f;
+  }
+  vel[p] = v;
+}
+
+// computeDeltaU, spray_k.cu:997-1006 (old <- new - old), one axis
+__global__ void k_delta_u(int64_t n, const float* __restrict__ u, float* __restrict__ uo) {
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < n) uo[f] = u[f] - uo[f];
+}
+
+// mapvelg2p_flip_bubble spray_k.cu:3525-3540 ; blend of advectparticle_RK2
+// spray_k.cu:421-424 when alpha < 1.
+__global__ void k_g2p(Grid g, int64_t P, const f3* __restrict__ pos, f3* __restrict__ vel,
+                      const int8_t* __restrict__ flag, Faces3 wdu, Faces3 adu, Faces3 wu, Faces3 au, float alpha) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  int8_t fl = flag[p];
+  const Faces3* du;
+  const Faces3* un;
+  if (fl == FLUID || fl == SOLID) { du = &wdu; un = &wu; }
+  else if (fl == AIR) { du = &adu; un = &au; }
+  else return;
+  f3 x = pos[p], v = vel[p];
+  f3 gv = vel_from_grid(g, x, du->u[0], du->u[1], du->u[2]);
+  if (alpha == 1.0f) v = add3(v, gv);
+  else {
+    f3 u = vel_from_grid(g, x, un->u[0], un->u[1], un->u[2]);
+    v = add3(mul3(u, 1.0f - alpha), mul3(add3(v, gv), alpha));
+  }
+  vel[p] = v;
+}
+
+// setVelZeroSolid_k spray_k.cu:5213-5218
+__global__ void k_zero_solid_vel(int64_t P, f3* __restrict__ vel, const int8_t* __restrict__ flag) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < P && flag[p] == SOLID) vel[p] = mk3(0.f, 0.f, 0.f);
+}
+
+// CollisionWithSolid_k, spray_k.cu:5276-5345
+__global__ void k_collide_move(Grid g, int64_t P, f3* __restrict__ pos, f3* __restrict__ vel,
+                               const int8_t* __restrict__ flag, const float* __restrict__ phis, Faces3 su, float dt,
+                               float bounce_vel, float bounce_pos) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  int8_t fl = flag[p];
+  if (fl == SOLID) return;
+  f3 x = pos[p], v = vel[p];
+  const float h = g.h;
+  float iphi = scalar_from_grid(g, x, phis);
+  if (iphi <= 0.5f) {
+    f3 sv = vel_from_grid(g, x, su.u[0], su.u[1], su.u[2]);
+    f3 rv = sub3(v, sv);
+    float d = h * 0.5f;
+    f3 gd;
+    gd.x = scalar_from_grid(g, add3(x, mk3(d, 0.f, 0.f)), phis) - scalar_from_grid(g, sub3(x, mk3(d, 0.f, 0.f)), phis);
+    gd.y = scalar_from_grid(g, add3(x, mk3(0.f, d, 0.f)), phis) - scalar_from_grid(g, sub3(x, mk3(0.f, d, 0.f)), phis);
+    gd.z = scalar_from_grid(g, add3(x, mk3(0.f, 0.f, d)), phis) - scalar_from_grid(g, sub3(x, mk3(0.f, 0.f, d)), phis);
+    float gl = len3(gd);
+    if (gl > 0) {
+      gd = mul3(gd, 1.0f / gl);
+      float dn = dot3(rv, gd);
+      if (dn < 0) v = sub3(v, mul3(gd, bounce_vel * dn));
+      x = add3(x, mul3(mul3(mul3(gd, bounce_pos), 0.5f - iphi), h));
+    }
+  }
+  x = add3(x, mul3(v, dt));
+  float rate = (fl == AIRSOLO) ? 0.8f : 0.5f;
+  float m = h + rate * h;
+  f3 lo = mk3(g.gx0 + m, g.gy0 + m, g.gz0 + m), hi = mk3(g.gx1 - m, g.gy1 - m, g.gz1 - m);
+  if (x.x <= lo.x) { x.x = lo.x; v.x = 0; }
+  if (x.y <= lo.y) { x.y = lo.y; v.y = 0; }
+  if (x.z <= lo.z) { x.z = lo.z; v.z = 0; }
+  if (x.x >= hi.x) { x.x = hi.x; v.x = 0; }
+  if (x.y >= hi.y) { x.y = hi.y; v.y = 0; }
+  if (x.z >= hi.z) { x.z = hi.z; v.z = 0; }
+  vel[p] = v;
+  pos[p] = x;
+}
+
+// advectparticle_RK2 spray_k.cu:409-459 (positions; the blend is in G2P)
+__global__ void k_advect_rk2(Grid g, int64_t P, f3* __restrict__ pos, f3* __restrict__ vel,
+                             const int8_t* __restrict__ flag, Faces3 wu, Faces3 au, float dt, float ss) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  int8_t fl = flag[p];
+  if (fl == AIRSOLO) return;
+  const Faces3& u = (fl == AIR) ? au : wu;
+  f3 x = pos[p], v = vel[p];
+  float m = g.h + 0.5f * ss;
+  f3 lo = mk3(g.gx0 + m, g.gy0 + m, g.gz0 + m), hi = mk3(g.gx1 - m, g.gy1 - m, g.gz1 - m);
+  f3 gv = vel_from_grid(g, x, u.u[0], u.u[1], u.u[2]);
+  f3 mid = add3(x, mul3(mul3(gv, dt), 0.5f));
+  f3 gm = vel_from_grid(g, mid, u.u[0], u.u[1], u.u[2]);
+  x = add3(x, mul3(gm, dt));
+  if (x.x <= lo.x) { x.x = lo.x; v.x = 0; }
+  if (x.y <= lo.y) { x.y = lo.y; v.y = 0; }
+  if (x.z <= lo.z) { x.z = lo.z; v.z = 0; }
+  if (x.x >= hi.x) { x.x = hi.x; v.x = 0; }
+  if (x.y >= hi.y) { x.y = hi.y; v.y = 0; }
+  if (x.z >= hi.z) { x.z = hi.z; v.z = 0; }
+  if (fl != SOLID) pos[p] = x;
+  vel[p] = v;
+}
+
+// mapHeatg2p / mapHeatg2p_MeltAndBoil spray_k.cu:4655-4679 + updateLatentHeat_k 5546-5573
+__global__ void k_heat_g2p(Grid g, int64_t P, const f3* __restrict__ pos, const int8_t* __restrict__ flag,
+                           float* __restrict__ temp, float* __restrict__ lheat, const float* __restrict__ tchange,
+                           const float* __restrict__ tgrid, float at, int melt_and_boil, float mp, float bp,
+                           float th) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  f3 x = pos[p];
+  int8_t fl = flag[p];
+  float t = temp[p];
+  float nt = at * (t + scalar_from_grid(g, x, tchange)) + (1.0f - at) * scalar_from_grid(g, x, tgrid);
+  if (melt_and_boil && fl == SOLID) t = 0.95f * t + 0.05f * nt;
+  else t = nt;
+  float lh = lheat[p];
+  if (fl == SOLID && t > mp) { lh += t - mp; t = mp; }
+  if (fl == FLUID) {
+    if (t < mp) { lh -= mp - t; t = mp; }
+    else if (t > bp) { lh += t - bp; t = bp; }
+    else lh = th;
+  }
+  temp[p] = t;
+  lheat[p] = lh;
+}
+
+// ---- phase change (a17) ----
+// SplitMix64 finalizer (Steele, Lea, Flood, OOPSLA 2014) as a counter hash.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ float uniform24(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
+  uint64_t h = mix64(seed ^ mix64(a ^ mix64(b ^ mix64(c))));
+  return (float)(h >> 40) * (1.0f / 16777216.0f);
+}
+
+// updatesolubility, spray_k.cu:4719-4727
+__global__ void k_solubility(int64_t P, const int8_t* __restrict__ flag, const float* __restrict__ temp,
+                             float* __restrict__ solub, float rate_m0, float T0) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < P && flag[p] == FLUID) solub[p] = rate_m0 * expf(1018.9f * (1.0f / temp[p] - 1.0f / T0));
+}
+
+struct GasArgs {
+  const f3* pos;
+  f3* vel_w;
+  f3* pos_w;
+  int8_t* flag;
+  float* mass;
+  float* temp;
+  float* lheat;
+  float* solub;
+  float* gas;
+  const uint32_t* cstart;
+  const uint32_t* cend;
+  const int32_t* seeds;
+  int nseeds;
+  const float* tgrid;
+  float m0, th, vr;
+  uint64_t seed;
+  int frame;
+};
+
+__device__ __forceinline__ bool in_boundary_cell(const Grid& g, int x, int y) {  // spray_k.cu:4214-4221
+  const int level = 2;
+  return x <= level || x >= g.nx - 1 - level || y <= level || y >= g.ny - 1 - level;
+}
+
+// GenerateGasParticle_k spray_k.cu:4730-4817: decision of one cell; returns
+// true when it emits and fills the new particle position.
+__device__ bool gas_decide(const Grid& g, const GasArgs& a, int64_t idx, f3& npos) {
+  int i, j, k;
+  ijk_of(g, idx, i, j, k);
+  if (k <= 1 || in_boundary_cell(g, i, j)) return false;
+  uint32_t s = a.cstart[idx];
+  if (s == MPF_CELL_UNDEF) return false;
+  float gc = 0, gs = 0, gair = 0, vap = 0;
+  int gasc = 0;
+  f3 gp = mk3(0.f, 0.f, 0.f);
+  for (uint32_t p = s; p < a.cend[idx]; ++p) {
+    int8_t fl = a.flag[p];
+    if (fl == FLUID) {
+      gc += a.gas[p];
+      gs += a.solub[p];
+      vap += fmaxf(0.0f, a.lheat[p] - a.th) * a.vr * a.m0;
+    } else if (fl == AIRSOLO || fl == AIR) {
+      gair += a.mass[p];
+      gp = add3(gp, a.pos[p]);
+      gasc++;
+    }
+  }
+  bool seed = false;
+  for (int q = 0; q < a.nseeds; ++q)
+    if (a.seeds[q] == (int32_t)idx) seed = true;
+  if (!(gc - gs + vap > a.m0 && (seed || gasc > 0))) return false;
+  f3 rp = mk3(uniform24(a.seed, (uint64_t)a.frame, (uint64_t)idx, 0), uniform24(a.seed, (uint64_t)a.frame, (uint64_t)idx, 1),
+              uniform24(a.seed, (uint64_t)a.frame, (uint64_t)idx, 2));
+  float rn = uniform24(a.seed, (uint64_t)a.frame, (uint64_t)idx, 3);
+  float r = g.h * 0.25f;
+  if (gasc > 0) {
+    f3 cen = mk3(gp.x / (float)gasc, gp.y / (float)gasc, gp.z / (float)gasc);
+    npos = add3(cen, mul3(mul3(sub3(rp, mk3(0.5f, 0.5f, 0.5f)), fmaxf(0.5f, rn) * r), 2.0f));
+  } else {
+    npos = add3(add3(mk3((float)i * g.h, (float)j * g.h, (float)k * g.h), mk3(g.gx0, g.gy0, g.gz0)), mul3(rp, g.h));
+  }
+  return true;
+}
+
+__global__ void k_gas_count(Grid g, GasArgs a, uint32_t* __restrict__ add) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= g.G) return;
+  f3 np;
+  add[idx] = gas_decide(g, a, idx, np) ? 1u : 0u;
+}
+
+__global__ void k_gas_emit(Grid g, GasArgs a, const uint32_t* __restrict__ add, const uint32_t* __restrict__ slot,
+                           int64_t P) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= g.G || !add[idx]) return;
+  f3 np;
+  gas_decide(g, a, idx, np);
+  int64_t q = P + slot[idx];
+  a.pos_w[q] = np;
+  a.vel_w[q] = mk3(0.f, 0.f, 0.f);
+  a.mass[q] = a.m0;
+  a.flag[q] = AIRSOLO;
+  a.temp[q] = a.tgrid[idx];
+  a.lheat[q] = 0;
+  a.gas[q] = 0;
+  a.solub[q] = 0;
+  for (uint32_t p = a.cstart[idx]; p < a.cend[idx]; ++p)
+    if (a.flag[p] == FLUID) {
+      a.gas[p] = fminf(a.gas[p], a.solub[p]);
+      a.lheat[p] = fminf(a.lheat[p], a.th);
+    }
+}
+
+// MeltingSolidByHeat, spray_k.cu:5059-5072
+__global__ void k_melt(int64_t P, int8_t* __restrict__ flag, float* __restrict__ lheat, float th,
+                       uint32_t* __restrict__ counter) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool ch = false;
+  if (p < P && flag[p] == SOLID && lheat[p] > th) {
+    flag[p] = FLUID;
+    lheat[p] = th;
+    ch = true;
+  }
+  unsigned long long m = __ballot(ch);
+  if (m && __lane_id() == (unsigned)(__ffsll((long long)m) - 1)) atomicAdd(counter, (uint32_t)__popcll(m));
+}
+
+// FreezingSolidByHeat, spray_k.cu:5074-5124 (reads the pass's input f0/p0)
+__global__ void k_freeze(Grid g, int64_t Pn, const int8_t* __restrict__ f0, const f3* __restrict__ p0,
+                         int8_t* __restrict__ flag, f3* __restrict__ pos, float* __restrict__ lheat,
+                         const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend, float ss,
+                         uint32_t* __restrict__ counter) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool ch = false;
+  if (p < Pn && f0[p] == FLUID && lheat[p] < 0) {
+    f3 x = p0[p];
+    float qx = (x.x - g.gx0) / g.h, qy = (x.y - g.gy0) / g.h, qz = (x.z - g.gz0) / g.h;
+    int i = (qx >= 0 && qx < (float)g.nx) ? (int)qx : 0;
+    int j = (qy >= 0 && qy < (float)g.ny) ? (int)qy : 0;
+    int k = (qz >= 0 && qz < (float)g.nz) ? (int)qz : 0;
+    float mind = 1000.0f;
+    int64_t mi = -1;
+    int cs = 0;
+    for (int di = -1; di <= 1; ++di)
+      for (int dj = -1; dj <= 1; ++dj)
+        for (int dk = -1; dk <= 1; ++dk) {
+          if (!valid(g, i + di, j + dj, k + dk)) continue;
+          int64_t c = cid(g, i + di, j + dj, k + dk);
+          uint32_t s = cstart[c];
+          if (s == MPF_CELL_UNDEF) continue;
+          for (uint32_t q = s; q < cend[c]; ++q) {
+            if (f0[q] != SOLID) continue;
+            float d = len3(sub3(p0[q], x));
+            if (d < g.h) cs++;
+            if (d < mind) { mind = d; mi = q; }
+          }
+        }
+    if (mi != -1 && mind < g.h && cs > 2) {
+      flag[p] = SOLID;
+      lheat[p] = 0;
+      ch = true;
+      if (mind > ss) {
+        f3 y = p0[mi];
+        f3 dd = sub3(x, y);
+        float l = len3(dd);
+        pos[p] = add3(mul3(mul3(dd, 1.0f / l), ss), y);
+      }
+    }
+  }
+  unsigned long long m = __ballot(ch);
+  if (m && __lane_id() == (unsigned)(__ffsll((long long)m) - 1)) atomicAdd(counter, (uint32_t)__popcll(m));
+}
+
+}  // namespace
+
+void run_apply_forces(Ctx& c) {
+  const mpf_config& k = c.cfg;
+  launch(c, "forces", dim3(nblocks(c.P, 256)), dim3(256), 0, k_forces, c.P, c.vel[c.cur], (const int8_t*)c.flag[c.cur],
+         k.dt, f3{k.gravity[0], k.gravity[1], k.gravity[2]}, k.buoyancy_kick, k.bubble_max_vel,
+         k.buoyancy_rate_air, k.buoyancy_rate_solo);
+}
+
+void run_g2p(Ctx& c) {
+  const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
+  for (int a = 0; a < 3; ++a) {
+    launch(c, "delta_u", dim3(nblocks(fc[a], 256)), dim3(256), 0, k_delta_u, fc[a], (const float*)c.wu[a], c.wuo[a]);
+    launch(c, "delta_u", dim3(nblocks(fc[a], 256)), dim3(256), 0, k_delta_u, fc[a], (const float*)c.au[a], c.auo[a]);
+  }
+  Faces3 wdu{{c.wuo[0], c.wuo[1], c.wuo[2]}}, adu{{c.auo[0], c.auo[1], c.auo[2]}};
+  Faces3 wu{{c.wu[0], c.wu[1], c.wu[2]}}, au{{c.au[0], c.au[1], c.au[2]}};
+  launch(c, "g2p", dim3(nblocks(c.P, 256)), dim3(256), 0, k_g2p, c.g, c.P, (const f3*)c.pos[c.cur], c.vel[c.cur],
+         (const int8_t*)c.flag[c.cur], wdu, adu, wu, au, c.cfg.flip_alpha);
+}
+
+void run_advect(Ctx& c) {
+  const int cur = c.cur;
+  if (c.cfg.fix_solids)
+    launch(c, "zero_solid_vel", dim3(nblocks(c.P, 256)), dim3(256), 0, k_zero_solid_vel, c.P, c.vel[cur],
+           (const int8_t*)c.flag[cur]);
+  if (c.cfg.integrator == MPF_ADVECT_RK2) {
+    Faces3 wu{{c.wu[0], c.wu[1], c.wu[2]}}, au{{c.au[0], c.au[1], c.au[2]}};
+    launch(c, "advect_rk2", dim3(nblocks(c.P, 256)), dim3(256), 0, k_advect_rk2, c.g, c.P, c.pos[cur], c.vel[cur],
+           (const int8_t*)c.flag[cur], wu, au, c.cfg.dt, c.cfg.samplespace);
+  } else {
+    run_solid_phi_and_velocity(c);
+    Faces3 su{{c.su[0], c.su[1], c.su[2]}};
+    launch(c, "collide_move", dim3(nblocks(c.P, 256)), dim3(256), 0, k_collide_move, c.g, c.P, c.pos[cur],
+           c.vel[cur], (const int8_t*)c.flag[cur], (const float*)c.phisolid, su, c.cfg.dt, c.cfg.bounce_vel,
+           c.cfg.bounce_pos);
+  }
+  c.sorted = false;
+}
+
+void run_heat_g2p(Ctx& c) {
+  const mpf_config& k = c.cfg;
+  launch(c, "heat_g2p", dim3(nblocks(c.P, 256)), dim3(256), 0, k_heat_g2p, c.g, c.P, (const f3*)c.pos[c.cur],
+         (const int8_t*)c.flag[c.cur], c.temp[c.cur], c.lheat[c.cur], (const float*)c.tpchange, (const float*)c.tp,
+         k.alpha_temp_trans, k.heat_melt_and_boil, k.melting_point, k.boiling_point, k.liquid_heat_th);
+}
+
+int run_phase_change(Ctx& c, int frame, mpf_phase_info* info) {
+  const mpf_config& k = c.cfg;
+  const int cur = c.cur;
+  int added = 0;
+  hipMemsetAsync(c.counters + 1, 0, 8, c.stream);
+  if (k.gas_enabled && k.gas_every > 0 && frame % k.gas_every == 0) {
+    launch(c, "solubility", dim3(nblocks(c.P, 256)), dim3(256), 0, k_solubility, c.P, (const int8_t*)c.flag[cur],
+           (const float*)c.temp[cur], c.solub[cur], k.dissolve_gas_rate * k.airm0, k.temperature0);
+    GasArgs a;
+    a.pos = c.pos[cur]; a.pos_w = c.pos[cur]; a.vel_w = c.vel[cur]; a.flag = c.flag[cur]; a.mass = c.mass[cur];
+    a.temp = c.temp[cur]; a.lheat = c.lheat[cur]; a.solub = c.solub[cur]; a.gas = c.gas[cur];
+    a.cstart = c.cstart; a.cend = c.cend; a.seeds = c.seeds; a.nseeds = c.nseeds; a.tgrid = c.tp;
+    a.m0 = k.airm0; a.th = k.liquid_heat_th; a.vr = k.vapor_gen_rate; a.seed = k.seed; a.frame = frame;
+    launch(c, "gas_count", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_gas_count, c.g, a, c.ctmp_u32);
+    int64_t na = run_scan_u32(c, c.ctmp_u32, c.count, c.g.G, true);
+    if (na < 0) return MPF_HIP_ERROR;
+    if (c.P + na > c.cap) {
+      c.err = "particle capacity exceeded by nucleation";
+      return MPF_CAPACITY;
+    }
+    launch(c, "gas_emit", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_gas_emit, c.g, a, (const uint32_t*)c.ctmp_u32,
+           (const uint32_t*)c.count, c.P);
+    added = (int)na;
+  }
+  const int64_t Pn = c.P;  // particles covered by the cell lists
+  c.P += added;
+  if (added) c.sorted = false;
+  if (k.melt_enabled)
+    launch(c, "melt", dim3(nblocks(c.P, 256)), dim3(256), 0, k_melt, c.P, c.flag[cur], c.lheat[cur], k.liquid_heat_th,
+           c.counters + 1);
+  if (k.freeze_enabled) {
+    const int alt = 1 - cur;
+    hipMemcpyAsync(c.flag[alt], c.flag[cur], (size_t)Pn, hipMemcpyDeviceToDevice, c.stream);
+    hipMemcpyAsync(c.pos[alt], c.pos[cur], (size_t)Pn * sizeof(f3), hipMemcpyDeviceToDevice, c.stream);
+    launch(c, "freeze", dim3(nblocks(Pn, 256)), dim3(256), 0, k_freeze, c.g, Pn, (const int8_t*)c.flag[alt],
+           (const f3*)c.pos[alt], c.flag[cur], c.pos[cur], c.lheat[cur], (const uint32_t*)c.cstart,
+           (const uint32_t*)c.cend, k.samplespace, c.counters + 2);
+  }
+  hipMemcpyAsync(c.counters_host + 1, c.counters + 1, 8, hipMemcpyDeviceToHost, c.stream);
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "phase_change")) return MPF_HIP_ERROR;
+  if (info) {
+    info->melted = (int32_t)c.counters_host[1];
+    info->frozen = (int32_t)c.counters_host[2];
+    info->gas_added = added;
+    info->particles = c.P;
+  }
+  return MPF_OK;
+}
+
+}  // namespace mpf

@@ -1,0 +1,342 @@
+// Cell hash, stable counting sort and reorder (a1-a3).
+//
+// Replaces cspray::hashAndSortParticles (spray.cu:614-637): calcHashD
+// (spray_k.cu:1010-1028) + thrust::sort_by_key (spray_thrust.cu:12-17) +
+// reorderDataAndFindCellStartD (spray_k.cu:1052-1129).  Result contract (the
+// reference's, thrust's radix sort being stable for uint keys): particles
+// ordered by (cell key, original index), gridHash = sorted keys, gridIndex =
+// the permutation, cell_start/cell_end per cell (MPF_CELL_UNDEF when empty),
+// all eight attribute arrays permuted.
+//
+// Counting sort instead of a comparison/radix sort, since keys are cell ids:
+//  1. hash + count: one atomic per run of equal keys in a wavefront (input is
+//     nearly sorted from the previous step, so runs are long);
+//  2. exclusive scan of the per-cell counts (reduce-then-scan, LDS tiles);
+//  3. scatter: one atomic per run reserves slots, lanes of a run take
+//     consecutive slots in index order;
+//  4. per-cell segment sort of the permutation (runs from different waves can
+//     interleave within a cell): a register sorting network for <=16 entries;
+//  5. gather of the 45-byte particle record into the other buffer.
+#include "mpf_internal.h"
+
+namespace mpf {
+
+namespace {
+
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_PER_THREAD = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_PER_THREAD;
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+// getijkfrompos (spray_k.cu:51-57) + getidx (spray_k.cu:66-69): IEEE division,
+// per-component range test, truncation.
+__device__ __forceinline__ uint32_t hash_key(const Grid& g, f3 p) {
+  float qx = (p.x - g.gx0) / g.h, qy = (p.y - g.gy0) / g.h, qz = (p.z - g.gz0) / g.h;
+  int i = (qx >= 0 && qx < (float)g.nx) ? (int)qx : 0;
+  int j = (qy >= 0 && qy < (float)g.ny) ? (int)qy : 0;
+  int k = (qz >= 0 && qz < (float)g.nz) ? (int)qz : 0;
+  return (uint32_t)cid(g, i, j, k);
+}
+
+// Run-length of equal keys starting at a head lane.  Active lanes form a
+// prefix of the wavefront (particle indices are contiguous).
+__device__ __forceinline__ void wave_runs(bool act, uint32_t k, bool& head, int& runlen, int& headlane) {
+  const int lane = lane_id();
+  uint32_t prev = __shfl_up(k, 1);
+  head = act && (lane == 0 || prev != k);
+  unsigned long long heads = __ballot(head);
+  unsigned long long actm = __ballot(act);
+  int nact = __popcll(actm);
+  unsigned long long le = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  unsigned long long above = heads & ~le;
+  int next = above ? (__ffsll((long long)above) - 1) : 64;
+  runlen = min(next, nact) - lane;
+  unsigned long long below = heads & le;
+  headlane = below ? (63 - __clzll((long long)below)) : 0;
+}
+
+__global__ void k_hash_count(Grid g, const f3* __restrict__ pos, int64_t P, uint32_t* __restrict__ key,
+                             uint32_t* __restrict__ count) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool act = p < P;
+  uint32_t k = 0xffffffffu;
+  if (act) {
+    k = hash_key(g, pos[p]);
+    key[p] = k;
+  }
+  bool head;
+  int runlen, headlane;
+  wave_runs(act, k, head, runlen, headlane);
+  if (head) atomicAdd(&count[k], (uint32_t)runlen);
+}
+
+__global__ void k_scatter(int64_t P, const uint32_t* __restrict__ key, uint32_t* __restrict__ cursor,
+                          uint32_t* __restrict__ perm, uint32_t* __restrict__ skey) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool act = p < P;
+  uint32_t k = act ? key[p] : 0xffffffffu;
+  bool head;
+  int runlen, headlane;
+  wave_runs(act, k, head, runlen, headlane);
+  uint32_t base = 0;
+  if (head) base = atomicAdd(&cursor[k], (uint32_t)runlen);
+  base = __shfl(base, headlane);
+  if (act) {
+    uint32_t slot = base + (uint32_t)(lane_id() - headlane);
+    perm[slot] = (uint32_t)p;
+    skey[slot] = k;
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void bitonic(uint32_t (&a)[N]) {
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        int l = i ^ j;
+        if (l > i) {
+          uint32_t x = a[i], y = a[l];
+          bool up = (i & k) == 0;
+          bool sw = up ? (x > y) : (x < y);
+          a[i] = sw ? y : x;
+          a[l] = sw ? x : y;
+        }
+      }
+    }
+  }
+}
+
+template <int N>
+__device__ __forceinline__ void sort_small(uint32_t* __restrict__ v, uint32_t n) {
+  uint32_t a[N];
+#pragma unroll
+  for (int q = 0; q < N; ++q) a[q] = ((uint32_t)q < n) ? v[q] : 0xffffffffu;
+  bitonic<N>(a);
+#pragma unroll
+  for (int q = 0; q < N; ++q)
+    if ((uint32_t)q < n) v[q] = a[q];
+}
+
+// Sort each cell's slice of the permutation by particle index.
+__global__ void k_segsort(int64_t G, const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
+                          uint32_t* __restrict__ perm) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G) return;
+  uint32_t s = cstart[c];
+  if (s == MPF_CELL_UNDEF) return;
+  uint32_t n = cend[c] - s;
+  if (n < 2) return;
+  uint32_t* v = perm + s;
+  // already ordered? (runs usually arrive in order)
+  bool ordered = true;
+  for (uint32_t q = 1; q < n && ordered; ++q) ordered = v[q - 1] < v[q];
+  if (ordered) return;
+  if (n <= 8) sort_small<8>(v, n);
+  else if (n <= 16) sort_small<16>(v, n);
+  else if (n <= 32) sort_small<32>(v, n);
+  else {
+    for (uint32_t q = 1; q < n; ++q) {
+      uint32_t x = v[q];
+      uint32_t r = q;
+      while (r > 0 && v[r - 1] > x) { v[r] = v[r - 1]; --r; }
+      v[r] = x;
+    }
+  }
+}
+
+__global__ void k_reorder(int64_t P, const uint32_t* __restrict__ perm,
+                          const f3* __restrict__ ipos, const f3* __restrict__ ivel, const int8_t* __restrict__ iflag,
+                          const float* __restrict__ imass, const float* __restrict__ itemp,
+                          const float* __restrict__ ilheat, const float* __restrict__ isolub,
+                          const float* __restrict__ igas,
+                          f3* __restrict__ opos, f3* __restrict__ ovel, int8_t* __restrict__ oflag,
+                          float* __restrict__ omass, float* __restrict__ otemp, float* __restrict__ olheat,
+                          float* __restrict__ osolub, float* __restrict__ ogas) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P) return;
+  uint32_t p = perm[s];
+  opos[s] = ipos[p];
+  ovel[s] = ivel[p];
+  oflag[s] = iflag[p];
+  omass[s] = imass[p];
+  otemp[s] = itemp[p];
+  olheat[s] = ilheat[p];
+  osolub[s] = isolub[p];
+  ogas[s] = igas[p];
+}
+
+// counts -> cell_start / cell_end (UNDEF when empty) and scatter cursors.
+__global__ void k_cells_from_scan(int64_t G, uint32_t* __restrict__ count, uint32_t* __restrict__ cstart,
+                                  uint32_t* __restrict__ cend) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G) return;
+  uint32_t n = count[c], s = cstart[c];
+  count[c] = s;
+  cstart[c] = n ? s : MPF_CELL_UNDEF;
+  cend[c] = n ? s + n : MPF_CELL_UNDEF;
+}
+
+__global__ void k_cell_mask(int64_t G, const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
+                            const int8_t* __restrict__ flag, uint8_t* __restrict__ cmask) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G) return;
+  uint32_t s = cstart[c];
+  uint8_t m = 0;
+  if (s != MPF_CELL_UNDEF) {
+    m = M_ANY;
+    for (uint32_t p = s; p < cend[c]; ++p) {
+      int8_t f = flag[p];
+      if (f == FLUID || f == SOLID) m |= M_FLUIDSOLID;
+      if (f == SOLID) m |= M_SOLID;
+      if (f == AIR) m |= M_AIR;
+      if (f == AIRSOLO) m |= M_AIRSOLO;
+    }
+  }
+  cmask[c] = m;
+}
+
+// ---- exclusive scan of uint32 (reduce, scan of block sums, apply) ----------
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    uint32_t t = __shfl_up(v, d);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Exclusive scan of one value per thread across the block (256 threads = 4 waves).
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* wsum, uint32_t& total) {
+  const int lane = lane_id(), w = threadIdx.x >> 6;
+  uint32_t inc = wave_incl_scan(v);
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  uint32_t off = 0, tot = 0;
+  for (int q = 0; q < SCAN_THREADS / 64; ++q) {
+    if (q < w) off += wsum[q];
+    tot += wsum[q];
+  }
+  total = tot;
+  __syncthreads();
+  return off + inc - v;
+}
+
+__global__ void k_scan_reduce(const uint32_t* __restrict__ in, int64_t n, uint32_t* __restrict__ blk) {
+  __shared__ uint32_t wsum[SCAN_THREADS / 64];
+  int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  uint32_t s = 0;
+  for (int q = 0; q < SCAN_PER_THREAD; ++q) {
+    int64_t e = base + (int64_t)q * SCAN_THREADS + threadIdx.x;
+    if (e < n) s += in[e];
+  }
+  uint32_t total;
+  block_excl_scan(s, wsum, total);
+  if (threadIdx.x == 0) blk[blockIdx.x] = total;
+}
+
+// One block scans nb block sums in place (exclusive) and writes the total at blk[nb].
+__global__ void k_scan_blocks(uint32_t* blk, int64_t nb) {
+  __shared__ uint32_t wsum[SCAN_THREADS / 64];
+  uint32_t carry = 0;
+  for (int64_t b0 = 0; b0 < nb; b0 += SCAN_THREADS) {
+    int64_t e = b0 + threadIdx.x;
+    uint32_t v = (e < nb) ? blk[e] : 0;
+    uint32_t total;
+    uint32_t ex = block_excl_scan(v, wsum, total);
+    if (e < nb) blk[e] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) blk[nb] = carry;
+}
+
+__global__ void k_scan_apply(const uint32_t* __restrict__ in, int64_t n, const uint32_t* __restrict__ blk,
+                             uint32_t* __restrict__ out) {
+  __shared__ uint32_t tile[SCAN_TILE + SCAN_TILE / 16];
+  __shared__ uint32_t wsum[SCAN_THREADS / 64];
+  int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+  for (int q = 0; q < SCAN_PER_THREAD; ++q) {
+    int e = q * SCAN_THREADS + threadIdx.x;
+    int64_t ge = base + e;
+    tile[e + (e >> 4)] = (ge < n) ? in[ge] : 0u;
+  }
+  __syncthreads();
+  uint32_t loc[SCAN_PER_THREAD];
+  uint32_t run = 0;
+  const int t0 = threadIdx.x * SCAN_PER_THREAD;
+#pragma unroll
+  for (int q = 0; q < SCAN_PER_THREAD; ++q) {
+    int e = t0 + q;
+    loc[q] = run;
+    run += tile[e + (e >> 4)];
+  }
+  uint32_t total;
+  uint32_t off = block_excl_scan(run, wsum, total) + blk[blockIdx.x];
+#pragma unroll
+  for (int q = 0; q < SCAN_PER_THREAD; ++q) {
+    int e = t0 + q;
+    tile[e + (e >> 4)] = loc[q] + off;
+  }
+  __syncthreads();
+  for (int q = 0; q < SCAN_PER_THREAD; ++q) {
+    int e = q * SCAN_THREADS + threadIdx.x;
+    int64_t ge = base + e;
+    if (ge < n) out[ge] = tile[e + (e >> 4)];
+  }
+}
+
+}  // namespace
+
+int64_t run_scan_u32(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n, bool want_total) {
+  if (n <= 0) return 0;
+  int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+  if (nb + 1 > c.scan_blk_n) {
+    c.err = "scan scratch too small";
+    return -1;
+  }
+  launch(c, "scan_reduce", dim3((unsigned)nb), dim3(SCAN_THREADS), 0, k_scan_reduce, in, n, c.scan_blk);
+  launch(c, "scan_blocks", dim3(1), dim3(SCAN_THREADS), 0, k_scan_blocks, c.scan_blk, nb);
+  launch(c, "scan_apply", dim3((unsigned)nb), dim3(SCAN_THREADS), 0, k_scan_apply, in, n,
+         (const uint32_t*)c.scan_blk, out);
+  if (!want_total) return 0;
+  uint32_t tot = 0;
+  hipMemcpyAsync(c.counters_host + 15, c.scan_blk + nb, 4, hipMemcpyDeviceToHost, c.stream);
+  hipStreamSynchronize(c.stream);
+  tot = c.counters_host[15];
+  return (int64_t)tot;
+}
+
+void run_cell_mask(Ctx& c) {
+  const int64_t G = c.g.G;
+  launch(c, "cell_mask", dim3(nblocks(G, 256)), dim3(256), 0, k_cell_mask, G, (const uint32_t*)c.cstart,
+         (const uint32_t*)c.cend, (const int8_t*)c.flag[c.cur], c.cmask);
+}
+
+void run_hash_sort(Ctx& c) {
+  const int64_t P = c.P, G = c.g.G;
+  const int cur = c.cur, nxt = 1 - c.cur;
+  hipMemsetAsync(c.count, 0, (size_t)G * 4, c.stream);
+  launch(c, "hash_count", dim3(nblocks(P, 256)), dim3(256), 0, k_hash_count, c.g, (const f3*)c.pos[cur], P,
+         c.key, c.count);
+  run_scan_u32(c, c.count, c.cstart, G, false);
+  launch(c, "cells_from_scan", dim3(nblocks(G, 256)), dim3(256), 0, k_cells_from_scan, G, c.count, c.cstart,
+         c.cend);
+  launch(c, "scatter", dim3(nblocks(P, 256)), dim3(256), 0, k_scatter, P, (const uint32_t*)c.key, c.count,
+         c.perm, c.ghash);
+  launch(c, "segsort", dim3(nblocks(G, 256)), dim3(256), 0, k_segsort, G, (const uint32_t*)c.cstart,
+         (const uint32_t*)c.cend, c.perm);
+  launch(c, "reorder", dim3(nblocks(P, 256)), dim3(256), 0, k_reorder, P, (const uint32_t*)c.perm,
+         (const f3*)c.pos[cur], (const f3*)c.vel[cur], (const int8_t*)c.flag[cur], (const float*)c.mass[cur],
+         (const float*)c.temp[cur], (const float*)c.lheat[cur], (const float*)c.solub[cur],
+         (const float*)c.gas[cur], c.pos[nxt], c.vel[nxt], c.flag[nxt], c.mass[nxt], c.temp[nxt], c.lheat[nxt],
+         c.solub[nxt], c.gas[nxt]);
+  c.cur = nxt;
+  c.sorted = true;
+  run_cell_mask(c);
+}
+
+}  // namespace mpf

@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark: FLIP steps/s of the melt/freeze workload at 256^3 (~90M particles).
+
+One "step" is mpf_step: hash/sort, forces, level set + flags, P2G, extrapolation,
+variable-density Jacobi-PCG projection to 1e-6 relative residual, G2P, advection
+with solid collision, re-sort + level set, heat (Jacobi-10), melt/freeze and
+marching cubes — on inputs already resident in HBM.  The mixed-precision form
+(FP32 stencils/transfers, FP64 dots and refinement) is timed, then the all-FP64
+PCG form on the same initial state; both solve to the same residual.
+
+Prints ONE JSON line (rank 0, N=1).  Extra fields: fp64 steps/s and the ratio,
+particle-steps/s, PCG iterations/s, a roofline record for the dominant kernel
+(per-launch duration from HIP events in this process) and a CPU baseline (the
+oracle, OpenMP, on a bounded sample of the same workload).
+"""
+import argparse
+import importlib
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per launch, as (bytes per cell, bytes per particle), for the
+# kernels that can dominate (FP32 vectors of the MIXED form; FP64 form doubles
+# the vector bytes).  Each array element compulsorily read or written once.
+KERNEL_BYTES = {
+    # q = A p: p 4, mark 1, q 4
+    "pcg_spmv": (9, 0),
+    # x, r read+write 16, p 4, q 4, 1/diag 4, mark 1
+    "pcg_update": (29, 0),
+    # p = r/diag + beta p: r 4, 1/diag 4, p read+write 8
+    "pcg_dir": (16, 0),
+    # start/end 8 + mask 1 per cell, pos 12 + flag 1 per particle
+    "genlevelset": (17, 13),
+    # per axis launch: start/end 8, mask 1, u 4 per face; pos 12 + vel 12 + mass 4 + flag 1 per particle
+    "p2g_liquid": (13, 29),
+    "p2g_air": (13, 29),
+    # 45-byte record read and written + perm 4
+    "reorder": (0, 94),
+    "sweep": (9, 0),
+    "sweep_u": (13, 0),
+}
+FP64_VEC = {"pcg_spmv": (17, 0), "pcg_update": (57, 0), "pcg_dir": (32, 0)}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_form(pkg, scenes, name, n, precision, steps, warmup, timing_step=True):
+    s = scenes.make(name, n=n, precision=precision)
+    cfg = s["cfg"]
+    sim = pkg.Simulator(cfg)
+    sim.set_particles(s["n_particles"], **s["particles"])
+    if len(s["seeds"]):
+        sim.set_seed_cells(s["seeds"])
+    del s["particles"]
+    log("[bench] %s n=%d precision=%d particles=%d: %d warmup + %d timed steps" % (
+        name, n, precision, s["n_particles"], warmup, steps))
+    frame = 0
+    for _ in range(warmup):
+        sim.step(frame, allow_no_convergence=True)
+        frame += 1
+    sim.synchronize()
+    import torch
+    if torch.cuda.is_available():
+        torch.cuda.synchronize()
+    iters = 0
+    t0 = time.perf_counter()
+    infos = []
+    for _ in range(steps):
+        info = sim.step(frame, allow_no_convergence=True)
+        infos.append(info)
+        iters += info["project"]["iterations"]
+        frame += 1
+    sim.synchronize()
+    if torch.cuda.is_available():
+        torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    log("[bench]   %.3f s for %d steps, PCG iterations %s" % (dt, steps, [i["project"]["iterations"] for i in infos]))
+    ktimes = None
+    if timing_step:
+        sim.set_kernel_timing(True)
+        sim.reset_kernel_times()
+        sim.step(frame, allow_no_convergence=True)
+        ktimes = sim.kernel_times()
+        sim.set_kernel_timing(False)
+    res = {"seconds": dt, "steps": steps, "pcg_iters": iters, "infos": infos, "ktimes": ktimes,
+           "particles": sim.particle_count(), "G": cfg.nx * cfg.ny * cfg.nz}
+    sim.close()
+    return res
+
+
+def roofline(res, precision):
+    kt = res["ktimes"] or {}
+    if not kt:
+        return None, None
+    dom = max(kt.items(), key=lambda kv: kv[1][1])
+    name, (cnt, ms) = dom
+    table = dict(KERNEL_BYTES)
+    if precision == 1:
+        table.update(FP64_VEC)
+    per_cell, per_par = table.get(name, (0, 0))
+    G, P = res["G"], res["particles"]
+    launches_per_axis = 1
+    if name.startswith("p2g"):
+        launches_per_axis = 3  # one launch per face axis: G faces (approx) per launch
+    bytes_per_launch = per_cell * G + per_par * P * (1 if launches_per_axis == 1 else 1)
+    avg_ms = ms / max(cnt, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if bytes_per_launch else None
+    rf = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+          "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+          "avg_launch_us": avg_ms * 1e3, "launches_per_step": cnt, "bytes_per_launch": bytes_per_launch}
+    top = sorted(((k, v[1]) for k, v in kt.items()), key=lambda kv: -kv[1])[:8]
+    return rf, top
+
+
+def cpu_baseline(scenes, name, n_full, n_sample, steps):
+    import oracle
+    s = scenes.make(name, n=n_sample)
+    threads = min(os.cpu_count() or 1, 16)
+    oracle.set_threads(threads)
+    o = oracle.Oracle(s["cfg"])
+    o.set_particles(s["n_particles"], **s["particles"])
+    if len(s["seeds"]):
+        o.set_seed_cells(s["seeds"])
+    t0 = time.perf_counter()
+    for f in range(steps):
+        o.step(f, allow_no_convergence=True)
+    dt = time.perf_counter() - t0
+    scale = (n_sample / float(n_full)) ** 3
+    return {"value": (steps / dt) * scale, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": "%s at %d^3 (%d particles), %d steps in %.1f s on %d threads, scaled by the cell ratio "
+                      "(%d/%d)^3 to %d^3 (PCG iteration growth with size not modelled)" % (
+                          name, n_sample, s["n_particles"], steps, dt, threads, n_sample, n_full, n_full)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--scene", default="melt_freeze")
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=64)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md for the "
+              "z-slab sharding design); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    pkg = importlib.import_module("2017multiphase_amd")
+    scenes = importlib.import_module("2017multiphase_amd.scenes")
+    mixed = run_form(pkg, scenes, args.scene, args.n, pkg.MIXED, args.steps, args.warmup)
+    fp64 = None
+    if not args.no_fp64:
+        fp64 = run_form(pkg, scenes, args.scene, args.n, pkg.FP64, args.steps, args.warmup)
+    sps = mixed["steps"] / mixed["seconds"]
+    rf, top = roofline(mixed, pkg.MIXED)
+    out = {
+        "metric": "FLIP steps/s at 256^3 (mixed vs FP64 PCG, same residual); % HBM peak",
+        "value": sps,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * mixed["seconds"] / mixed["steps"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (PCG dots, norms and refinement residual f64)",
+        "data": "synthetic (seeded scene generator)",
+        "config": {"workload": "%s %d^3, %d particles, variable-density Jacobi-PCG to 1e-6 rel. residual, "
+                               "mixed precision (refine every 20)" % (args.scene, args.n, mixed["particles"]),
+                   "grid": args.n, "particles": mixed["particles"], "parallelism": "single GPU"},
+        "particle_steps_per_s": sps * mixed["particles"],
+        "pcg_iters_per_s": mixed["pcg_iters"] / mixed["seconds"],
+        "pcg_iters_per_step": mixed["pcg_iters"] / mixed["steps"],
+        "roofline": rf,
+        "top_kernels_ms": top,
+    }
+    if fp64:
+        f_sps = fp64["steps"] / fp64["seconds"]
+        out["fp64_steps_per_s"] = f_sps
+        out["mixed_over_fp64"] = sps / f_sps
+        out["fp64_pcg_iters_per_step"] = fp64["pcg_iters"] / fp64["steps"]
+        rf64, _ = roofline(fp64, pkg.FP64)
+        out["fp64_roofline"] = rf64
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scenes, args.scene, args.n, args.cpu_n, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,287 @@
+/*
+ * mpflip.h — C ABI of the MI355X-native FLIP multiphase step.
+ *
+ * The reference's "plugin API" for this path is the set of argument-less host
+ * step methods of `class cspray` (Multiphase/Multiphase/spray.h:18-389), which
+ * operate on member state (device buffers owned by the object) and are driven
+ * by `cspray::bubblesim()` (spray.cpp:979-1233).  This header keeps that shape:
+ * an opaque context owns every device buffer and one HIP stream, and each
+ * entry point below replaces one of those methods.  Everything crossing the
+ * boundary is a plain pointer, a size or a POD struct; no C++ or torch types.
+ *
+ * Layouts accepted (the reference's, spray.h:165-169, utility.h:49-88):
+ *   particles  float pos[3*P], vel[3*P] (AoS float3), int8 flag[P],
+ *              float mass[P], T[P], LHeat[P], solubility[P], gascontain[P]
+ *   cells      x-major, index i*NY*NZ + j*NZ + k (z contiguous)
+ *   faces      ux[(NX+1)*NY*NZ], uy[NX*(NY+1)*NZ], uz[NX*NY*(NZ+1)], same order
+ *   cell lists uint32 cell_start[G], cell_end[G]; empty cell = MPF_CELL_UNDEF
+ *
+ * Every function returns mpf_status.  A context is not thread-safe; distinct
+ * contexts are independent.  Pointers passed in are borrowed for the call.
+ */
+#ifndef MPFLIP_H
+#define MPFLIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPF_ABI_VERSION 1
+#define MPF_CELL_UNDEF 0xffffffffu
+
+/* Particle / cell type codes: utility.h:13-19. */
+enum {
+  MPF_TYPEFLUID = 0,
+  MPF_TYPEAIR = 1,
+  MPF_TYPEBOUNDARY = 2,
+  MPF_TYPEVACUUM = 3,
+  MPF_TYPEAIRSOLO = 4,
+  MPF_TYPESOLID = 5,
+  MPF_TYPECNT = 6
+};
+
+typedef enum {
+  MPF_OK = 0,
+  MPF_INVALID_ARG = 1,
+  MPF_NONFINITE = 2,
+  MPF_NO_CONVERGENCE = 3,
+  MPF_CAPACITY = 4,
+  MPF_HIP_ERROR = 5,
+  MPF_IO = 6
+} mpf_status;
+
+/* Where a caller buffer lives. */
+enum { MPF_HOST = 0, MPF_DEVICE = 1 };
+
+/* Arithmetic of the pressure projection (divergence, PCG, gradient). */
+enum {
+  MPF_FP32 = 0,  /* FP32 vectors, FP32 block sums of the dot products        */
+  MPF_FP64 = 1,  /* FP64 divergence, vectors, dots and pressure               */
+  MPF_MIXED = 2  /* FP32 vectors and SpMV, FP64 dots, FP64 pressure updated
+                    and true residual recomputed in FP64 every refine_every
+                    iterations (reliable update)                              */
+};
+
+/* Pressure solver. */
+enum {
+  MPF_SOLVER_REFERENCE_CG = 0, /* spray.cu:1613-1687: no preconditioner, stop
+                                  when r.r < 1e-2*G (absolute), cap ref_max_iters */
+  MPF_SOLVER_JACOBI_PCG = 1    /* Jacobi PCG, stop when |r|/|b| < tol        */
+};
+
+/* Pressure operator. */
+enum {
+  MPF_OPERATOR_UNIT = 0,   /* computeAx_bubble, spray_k.cu:3423-3447          */
+  MPF_OPERATOR_VARDENS = 1 /* 1/rho face weights; equals UNIT at rho_l=rho_a=1 */
+};
+
+/* Particle integrator. */
+enum {
+  MPF_ADVECT_EULER_COLLIDE = 0, /* CollisionWithSolid_k, spray_k.cu:5276-5345 */
+  MPF_ADVECT_RK2 = 1            /* advectparticle_RK2,   spray_k.cu:409-459    */
+};
+
+/* Heat diffusion solver. */
+enum {
+  MPF_HEAT_CG = 0,    /* solver_cg_heat, spray.cu:1780-1852                  */
+  MPF_HEAT_JACOBI = 1 /* fixed number of Jacobi sweeps on the same operator  */
+};
+
+typedef struct mpf_config {
+  int32_t nx, ny, nz;
+  float dx;            /* cell size (the reference fixes 1/64, spray.cpp:78)  */
+  float dt;
+  float gravity[3];
+  float gmin[3];       /* domain origin; gmax = gmin + dx*(nx,ny,nz)          */
+  float samplespace;   /* particle spacing, dx/2 (spray.cpp:79)                */
+  float m0, airm0;     /* particle masses (spray.cpp:581-582)                  */
+  int64_t particle_capacity;
+
+  /* forces (spray.cu:376-393) */
+  int32_t buoyancy_kick;     /* addbuoyancyforce_vel (reference mode)         */
+  float bubble_max_vel, buoyancy_rate_air, buoyancy_rate_solo;
+
+  /* projection */
+  int32_t precision;         /* MPF_FP32 / MPF_FP64 / MPF_MIXED               */
+  int32_t solver;            /* MPF_SOLVER_*                                  */
+  int32_t op;                /* MPF_OPERATOR_*                                */
+  float rho_liquid, rho_air; /* densities for MPF_OPERATOR_VARDENS            */
+  double tol;                /* relative residual for MPF_SOLVER_JACOBI_PCG   */
+  int32_t max_iters;         /* PCG iteration cap                             */
+  int32_t ref_max_iters;     /* reference CG cap (spray.cu:1642, 800)         */
+  int32_t refine_every;      /* MPF_MIXED: true-residual interval (20)        */
+  int32_t check_every;       /* host convergence poll interval (iterations)   */
+  int32_t solid_constraint;  /* cells dominated by SOLID particles become
+                                velocity-constrained (Neumann) in the solve   */
+  int32_t flags_from_counts; /* markgrid_bubble (MELTING/FREEZING scenes)     */
+
+  /* FLIP / advection */
+  float flip_alpha;          /* 1 = pure FLIP (bubblesim), 0.95 = watersim    */
+  int32_t integrator;        /* MPF_ADVECT_*                                  */
+  int32_t fix_solids;        /* solidmotion_fixed (spray.cu:2685-2689)        */
+  float bounce_vel, bounce_pos;
+
+  /* heat (spray.cpp:35-38, 62-65; spray.cu:3076-3090) */
+  int32_t heat_enabled;
+  int32_t heat_solver;       /* MPF_HEAT_*                                    */
+  int32_t heat_jacobi_iters; /* 10                                            */
+  int32_t heat_max_iters;    /* CG cap (3800)                                 */
+  float heat_alpha[MPF_TYPECNT]; /* diffusivity per cell type                 */
+  float default_temperature; /* T of cells without particles                  */
+  int32_t heat_add_bottom;   /* addHeatAtBottom                               */
+  float heat_increase_bottom, heat_bottom_clamp;
+  int32_t heat_extend_boundary; /* setBoundaryHeat                            */
+  int32_t heat_vacuum_neumann;  /* boiling scenes: VACUUM neighbours Neumann  */
+  int32_t heat_melt_and_boil;   /* mapHeatg2p_MeltAndBoil                     */
+  float alpha_temp_trans;
+
+  /* phase change */
+  int32_t melt_enabled, freeze_enabled, gas_enabled;
+  int32_t gas_every;           /* GenerateGasParticle every n frames (2)      */
+  float melting_point, boiling_point, liquid_heat_th;
+  float temperature0, dissolve_gas_rate, vapor_gen_rate;
+
+  /* marching cubes on the liquid level set */
+  int32_t mc_enabled;
+  int64_t mc_max_verts, mc_max_tris;
+
+  uint64_t seed;
+} mpf_config;
+
+/* Opaque context. */
+typedef struct mpf_ctx mpf_ctx;
+
+/* Grid / particle fields addressable by mpf_get_field / mpf_set_field. */
+typedef enum {
+  /* cell fields (G = nx*ny*nz) */
+  MPF_F_MARK = 0,     /* int8   */
+  MPF_F_LSFLUID, MPF_F_LSAIR, MPF_F_LSMERGE,
+  MPF_F_PHIGRAD_X, MPF_F_PHIGRAD_Y, MPF_F_PHIGRAD_Z,
+  MPF_F_PHIFLUID, MPF_F_PHIAIR, MPF_F_PHISOLID,
+  MPF_F_DIV,          /* float, or double under MPF_FP64                      */
+  MPF_F_PRESSURE,     /* float under MPF_FP32, double under MPF_FP64/MIXED    */
+  MPF_F_TEMPERATURE, MPF_F_TEMPERATURE_CHANGE,
+  MPF_F_CELL_START, MPF_F_CELL_END, /* uint32 */
+  /* face fields */
+  MPF_F_WATER_UX, MPF_F_WATER_UY, MPF_F_WATER_UZ,
+  MPF_F_AIR_UX, MPF_F_AIR_UY, MPF_F_AIR_UZ,
+  MPF_F_WATER_UX_OLD, MPF_F_WATER_UY_OLD, MPF_F_WATER_UZ_OLD,
+  MPF_F_AIR_UX_OLD, MPF_F_AIR_UY_OLD, MPF_F_AIR_UZ_OLD,
+  MPF_F_SOLID_UX, MPF_F_SOLID_UY, MPF_F_SOLID_UZ,
+  /* particle fields (P = current count) */
+  MPF_F_GRID_HASH, MPF_F_GRID_INDEX, /* uint32, from the last mpf_hash_sort  */
+  MPF_F_COUNT
+} mpf_field;
+
+typedef struct mpf_solve_info {
+  int32_t iterations;      /* solver iterations                               */
+  int32_t refinements;     /* FP64 true-residual recomputations (MIXED)       */
+  int32_t converged;
+  int32_t pad_;
+  double rel_residual;     /* |b - A x| / |b|, true residual, FP64            */
+  double abs_rr;           /* recurrence r.r at exit                          */
+} mpf_solve_info;
+
+typedef struct mpf_phase_info {
+  int32_t melted, frozen, gas_added, pad_;
+  int64_t particles;
+} mpf_phase_info;
+
+typedef struct mpf_mc_info {
+  int64_t active_voxels, verts, tris;
+} mpf_mc_info;
+
+typedef struct mpf_step_info {
+  mpf_solve_info project;
+  mpf_solve_info heat;
+  mpf_phase_info phase;
+  mpf_mc_info mc;
+  int64_t particles;
+} mpf_step_info;
+
+/* ---- lifecycle --------------------------------------------------------- */
+/* Reference defaults of initparam (spray.cpp:23-79, 573-618) for an nx*ny*nz
+   grid.  Replaces the compile-time NX/NY/NZ (spray.h:9-16) and __constant__
+   FlipConstant dparam (spray_k.cu:8, utility.h:26-46). */
+mpf_status mpf_config_default(mpf_config* cfg, int32_t nx, int32_t ny, int32_t nz);
+/* Replaces cspray::init()/initmem_bubble() (spray.cpp:628-670, spray.cu:25-218). */
+mpf_status mpf_create(const mpf_config* cfg, mpf_ctx** out);
+mpf_status mpf_destroy(mpf_ctx* ctx);
+/* Message of the last failing call on ctx (or of mpf_create when ctx is NULL). */
+const char* mpf_last_error(mpf_ctx* ctx);
+int32_t mpf_abi_version(void);
+/* The context's HIP stream (hipStream_t), for callers that time or order work. */
+void* mpf_stream(mpf_ctx* ctx);
+mpf_status mpf_synchronize(mpf_ctx* ctx);
+
+/* ---- data movement ----------------------------------------------------- */
+/* Replaces initparticle_* H2D copies (spray.cu:395-484). Any pointer may be
+   NULL, which zero-fills that attribute (flag NULL = TYPEFLUID). */
+mpf_status mpf_set_particles(mpf_ctx* ctx, int64_t n, const float* pos, const float* vel,
+                             const int8_t* flag, const float* mass, const float* T,
+                             const float* lheat, const float* solubility,
+                             const float* gascontain, int32_t where);
+/* Any output pointer may be NULL (skipped). */
+mpf_status mpf_get_particles(mpf_ctx* ctx, int64_t* n, float* pos, float* vel, int8_t* flag,
+                             float* mass, float* T, float* lheat, float* solubility,
+                             float* gascontain, int32_t where);
+int64_t mpf_particle_count(mpf_ctx* ctx);
+/* Element count and element size in bytes of a field. */
+mpf_status mpf_field_info(mpf_ctx* ctx, int32_t field, int64_t* count, int32_t* elem_bytes);
+mpf_status mpf_get_field(mpf_ctx* ctx, int32_t field, void* dst, int32_t where);
+mpf_status mpf_set_field(mpf_ctx* ctx, int32_t field, const void* src, int32_t where);
+/* Nucleation seed cells (cell indices), spray.cu:2907-2951. */
+mpf_status mpf_set_seed_cells(mpf_ctx* ctx, const int32_t* cells, int32_t n);
+
+/* ---- step entry points (one per reference host step method) ------------ */
+/* cspray::hashAndSortParticles, spray.cu:614-637: calcHashD + stable sort by
+   (key, index) + reorderDataAndFindCellStartD. */
+mpf_status mpf_hash_sort(mpf_ctx* ctx);
+/* cspray::addexternalforce, spray.cu:376-393. */
+mpf_status mpf_apply_forces(mpf_ctx* ctx);
+/* cspray::computeLevelset + sweepLSAndMardGrid, spray.cu:1465-1471, 2011-2065
+   (deterministic two-buffer sweeps); then markgrid_bubble (spray.cu:522-529)
+   when cfg.flags_from_counts, and the solid marking when cfg.solid_constraint. */
+mpf_status mpf_levelset_update(mpf_ctx* ctx);
+/* cspray::mapvelp2g_bubble, spray.cu:1562-1588 (both phases + FLIP snapshot). */
+mpf_status mpf_p2g(mpf_ctx* ctx);
+/* sweepPhi/sweepU/setWaterBoundaryU for both phases, spray.cpp:1088-1094
+   (recompute_phi = 1) or spray.cpp:1101-1104 (recompute_phi = 0). */
+mpf_status mpf_extrapolate_bc(mpf_ctx* ctx, int32_t recompute_phi);
+/* cspray::project_CG_bubble, spray.cu:1590-1611.  MPF_NO_CONVERGENCE when the
+   solver stops at its cap without meeting its criterion (results are kept). */
+mpf_status mpf_project(mpf_ctx* ctx, mpf_solve_info* info);
+/* cspray::mapvelg2p_bubble, spray.cu:1689-1705, with blend cfg.flip_alpha. */
+mpf_status mpf_g2p(mpf_ctx* ctx);
+/* cspray::solidmotion_fixed + CollisionSolid, spray.cu:2685-2689, 2959-2983
+   (MPF_ADVECT_EULER_COLLIDE) or wateradvect, spray.cu:293-305 (MPF_ADVECT_RK2). */
+mpf_status mpf_advect(mpf_ctx* ctx);
+/* cspray::updateTemperature, spray.cu:1889-1947 (incl. updateLatentHeat). */
+mpf_status mpf_heat(mpf_ctx* ctx, mpf_solve_info* info);
+/* GenerateGasParticle, MeltSolid, Freezing: spray.cu:2653-2683, 2816-2875. */
+mpf_status mpf_phase_change(mpf_ctx* ctx, int32_t frame, mpf_phase_info* info);
+/* runMC_smooth stages 2-7 (spray.cu:804-979) on the liquid level set. */
+mpf_status mpf_marching_cubes(mpf_ctx* ctx, mpf_mc_info* info);
+/* Copy the last mesh out: verts float[3*nv], tri indices uint32[3*nt]. */
+mpf_status mpf_get_mesh(mpf_ctx* ctx, float* verts, uint32_t* tris, int32_t where);
+/* Per-voxel case index (uint8, (nx-1)*(ny-1)*(nz-1) voxels over cell centres). */
+mpf_status mpf_get_mc_cases(mpf_ctx* ctx, uint8_t* cases, int32_t where);
+
+/* One full step in bubblesim order (spray.cpp:979-1233, north-star subset). */
+mpf_status mpf_step(mpf_ctx* ctx, int32_t frame, mpf_step_info* info);
+
+/* ---- introspection ------------------------------------------------------ */
+/* Kernel timing with HIP events around every launch on the context stream. */
+mpf_status mpf_set_kernel_timing(mpf_ctx* ctx, int32_t on);
+/* Up to cap entries: names (48 bytes each, NUL-terminated), launch counts and
+   total milliseconds. Returns the number of distinct kernels (may exceed cap). */
+int32_t mpf_kernel_times(mpf_ctx* ctx, char* names, int64_t* counts, double* total_ms, int32_t cap);
+mpf_status mpf_reset_kernel_times(mpf_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPFLIP_H */

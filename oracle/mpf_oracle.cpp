@@ -1,0 +1,1477 @@
+// CPU oracle of the FLIP multiphase step — TEST INFRASTRUCTURE ONLY.
+//
+// Restates the reference kernel bodies of Multiphase/Multiphase/spray_k.cu and
+// the host sequencing of spray.cu / spray.cpp as scalar loops over host arrays.
+// Each function cites the reference lines it follows.  Every per-element
+// expression keeps the reference's operation order; build with
+// -ffp-contract=off so the float instantiation rounds like the GPU library
+// (also built without contraction) wherever integers are derived from floats.
+// Reductions use a fixed blocking (ORC_RBLK) and a sequential sum of block
+// partials, so results do not depend on the OpenMP thread count.
+//
+// Deviations from the reference, each deliberate and shared with the GPU
+// library (the reference's versions are racy or non-reproducible):
+//  * eikonal sweeps are two-buffer Jacobi sweeps (reference: in-place, racy,
+//    spray_k.cu:755-789);
+//  * FreezingSolidByHeat reads flags/positions of the pass's input, not the
+//    ones being written (reference: racy, spray_k.cu:5074-5124);
+//  * nucleation slots come from an exclusive scan over cells in index order
+//    and random numbers from a counter-based SplitMix64 hash (reference:
+//    atomicAdd slots and a curand table, spray_k.cu:4786, spray.cu:180-190);
+//  * the ceiling rule of setWaterBoundaryU_k reads uz(k-1) after that face's
+//    own rule (sequential semantics; the reference reads it while it may be
+//    rewritten, spray_k.cu:986-989);
+//  * dot products accumulate in double (reference: float block sums added on
+//    the host in float, spray.cu:1101-1116).
+#include "mpf_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../2017multiphase_amd/csrc/mc_tables.inc"
+
+namespace {
+
+constexpr int64_t ORC_RBLK = 4096;
+constexpr int8_t FLUID = MPF_TYPEFLUID, AIR = MPF_TYPEAIR, BOUNDARY = MPF_TYPEBOUNDARY,
+                 VACUUM = MPF_TYPEVACUUM, AIRSOLO = MPF_TYPEAIRSOLO, SOLID = MPF_TYPESOLID;
+
+// SplitMix64 finalizer (Steele, Lea, Flood, "Fast splittable pseudorandom
+// number generators", OOPSLA 2014) used as a counter-based hash.
+inline uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+inline float uniform24(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
+  uint64_t h = mix64(seed ^ mix64(a ^ mix64(b ^ mix64(c))));
+  return (float)(h >> 40) * (1.0f / 16777216.0f);
+}
+
+template <class T> struct V3 { T x, y, z; };
+template <class T> inline V3<T> mk(T x, T y, T z) { return V3<T>{x, y, z}; }
+template <class T> inline V3<T> operator-(V3<T> a, V3<T> b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class T> inline V3<T> operator+(V3<T> a, V3<T> b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class T> inline V3<T> operator*(V3<T> a, T s) { return mk(a.x * s, a.y * s, a.z * s); }
+template <class T> inline T dot3(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> inline T len3(V3<T> a) { return std::sqrt(dot3(a, a)); }
+
+template <class T> inline T tmin(T a, T b) { return std::fmin(a, b); }
+template <class T> inline T tmax(T a, T b) { return std::fmax(a, b); }
+
+// Deterministic blocked reduction.
+template <class F> double rsum(int64_t n, F f) {
+  int64_t nb = (n + ORC_RBLK - 1) / ORC_RBLK;
+  std::vector<double> part(nb > 0 ? nb : 1, 0.0);
+#pragma omp parallel for schedule(static)
+  for (int64_t b = 0; b < nb; ++b) {
+    double s = 0.0;
+    int64_t e = std::min(n, (b + 1) * ORC_RBLK);
+    for (int64_t i = b * ORC_RBLK; i < e; ++i) s += f(i);
+    part[b] = s;
+  }
+  double s = 0.0;
+  for (int64_t b = 0; b < nb; ++b) s += part[b];
+  return s;
+}
+
+struct OrcBase {
+  virtual ~OrcBase() {}
+  mpf_config c;
+};
+
+template <class T> struct Orc : OrcBase {
+  int NX, NY, NZ;
+  int64_t G, FX, FY, FZ;
+  T h;
+  int64_t P = 0;
+  // particles (reference layout, T precision)
+  std::vector<T> pos, vel, mass, temp, lheat, solub, gas;
+  std::vector<int8_t> flag;
+  std::vector<uint32_t> ghash, gindex, cstart, cend;
+  // cell fields
+  std::vector<int8_t> mark;
+  std::vector<T> lsfluid, lsair, lsmerge, phigx, phigy, phigz, phifluid, phiair, phisolid, div, press,
+      tp, tpchange;
+  // face fields
+  std::vector<T> wu[3], au[3], wuo[3], auo[3], su[3];
+  std::vector<int32_t> seeds;
+  // marching cubes output
+  std::vector<uint8_t> mc_case;
+  std::vector<T> mc_verts;
+  std::vector<uint32_t> mc_tris;
+
+  explicit Orc(const mpf_config& cfg) {
+    c = cfg;
+    NX = c.nx; NY = c.ny; NZ = c.nz;
+    G = (int64_t)NX * NY * NZ;
+    FX = (int64_t)(NX + 1) * NY * NZ;
+    FY = (int64_t)NX * (NY + 1) * NZ;
+    FZ = (int64_t)NX * NY * (NZ + 1);
+    h = (T)c.dx;
+    mark.assign(G, VACUUM);
+    for (auto* v : {&lsfluid, &lsair, &lsmerge, &phigx, &phigy, &phigz, &phifluid, &phiair, &phisolid,
+                    &div, &press, &tp, &tpchange})
+      v->assign(G, T(0));
+    cstart.assign(G, MPF_CELL_UNDEF);
+    cend.assign(G, MPF_CELL_UNDEF);
+    int64_t fc[3] = {FX, FY, FZ};
+    for (int a = 0; a < 3; ++a) {
+      wu[a].assign(fc[a], T(0)); au[a].assign(fc[a], T(0));
+      wuo[a].assign(fc[a], T(0)); auo[a].assign(fc[a], T(0)); su[a].assign(fc[a], T(0));
+    }
+  }
+
+  // ---- indexing (utility.h:55-58, spray_k.cu:45-74) ----
+  inline int64_t cid(int i, int j, int k) const { return ((int64_t)i * NY + j) * NZ + k; }
+  inline bool valid(int i, int j, int k) const {  // verifycellidx, spray_k.cu:317-322
+    return !(i < 0 || i > NX - 1 || j < 0 || j > NY - 1 || k < 0 || k > NZ - 1);
+  }
+  inline void ijk(int64_t idx, int& i, int& j, int& k) const {
+    i = (int)(idx / ((int64_t)NZ * NY)); j = (int)((idx / NZ) % NY); k = (int)(idx % NZ);
+  }
+  static inline void ijk3(int64_t idx, int w, int hh, int d, int& i, int& j, int& k) {
+    (void)w;
+    i = (int)(idx / d / hh); j = (int)((idx / d) % hh); k = (int)(idx % d);
+  }
+  inline int64_t fid(int a, int i, int j, int k) const {  // face arrays with their own dims
+    if (a == 0) return ((int64_t)i * NY + j) * NZ + k;
+    if (a == 1) return ((int64_t)i * (NY + 1) + j) * NZ + k;
+    return ((int64_t)i * NY + j) * (NZ + 1) + k;
+  }
+  inline V3<T> P3(const std::vector<T>& a, int64_t p) const { return mk(a[3 * p], a[3 * p + 1], a[3 * p + 2]); }
+  inline void S3(std::vector<T>& a, int64_t p, V3<T> v) { a[3 * p] = v.x; a[3 * p + 1] = v.y; a[3 * p + 2] = v.z; }
+  inline V3<T> gmin() const { return mk((T)c.gmin[0], (T)c.gmin[1], (T)c.gmin[2]); }
+  inline V3<T> gmax() const {
+    return mk((T)c.gmin[0] + h * (T)NX, (T)c.gmin[1] + h * (T)NY, (T)c.gmin[2] + h * (T)NZ);
+  }
+
+  // getijkfrompos, spray_k.cu:51-57
+  inline void ijk_of_pos(V3<T> p, int& i, int& j, int& k) const {
+    V3<T> q = mk((p.x - (T)c.gmin[0]) / h, (p.y - (T)c.gmin[1]) / h, (p.z - (T)c.gmin[2]) / h);
+    i = (q.x >= 0 && q.x < (T)NX) ? (int)q.x : 0;
+    j = (q.y >= 0 && q.y < (T)NY) ? (int)q.y : 0;
+    k = (q.z >= 0 && q.z < (T)NZ) ? (int)q.z : 0;
+  }
+
+  // trilinear, spray_k.cu:111-122 (clamps to [0,w], base index min(x, w-2))
+  T trilinear(const std::vector<T>& u, T x, T y, T z, int w, int hh, int d) const {
+    x = tmax(T(0), tmin(x, (T)w));
+    y = tmax(T(0), tmin(y, (T)hh));
+    z = tmax(T(0), tmin(z, (T)d));
+    int i = (int)tmin(x, (T)(w - 2));
+    int j = (int)tmin(y, (T)(hh - 2));
+    int k = (int)tmin(z, (T)(d - 2));
+    auto U = [&](int a, int b, int cc) { return u[((int64_t)a * hh + b) * d + cc]; };
+    T fi = (T)i, fj = (T)j, fk = (T)k;
+    T ix1 = (T)(i + 1) - x, ix0 = x - fi;
+    T jy1 = (T)(j + 1) - y, jy0 = y - fj;
+    T kz1 = (T)(k + 1) - z, kz0 = z - fk;
+    return kz1 * (jy1 * (ix1 * U(i, j, k) + ix0 * U(i + 1, j, k)) + jy0 * (ix1 * U(i, j + 1, k) + ix0 * U(i + 1, j + 1, k))) +
+           kz0 * (jy1 * (ix1 * U(i, j, k + 1) + ix0 * U(i + 1, j, k + 1)) + jy0 * (ix1 * U(i, j + 1, k + 1) + ix0 * U(i + 1, j + 1, k + 1)));
+  }
+  // getParticleVelFromGrid, spray_k.cu:227-240
+  V3<T> vel_from_grid(V3<T> p, const std::vector<T>* g) const {
+    T x = (p.x - (T)c.gmin[0]) / h, y = (p.y - (T)c.gmin[1]) / h, z = (p.z - (T)c.gmin[2]) / h;
+    V3<T> v;
+    v.x = trilinear(g[0], x, y - T(0.5), z - T(0.5), NX + 1, NY, NZ);
+    v.y = trilinear(g[1], x - T(0.5), y, z - T(0.5), NX, NY + 1, NZ);
+    v.z = trilinear(g[2], x - T(0.5), y - T(0.5), z, NX, NY, NZ + 1);
+    return v;
+  }
+  // getScaleFromFrid, spray_k.cu:139-151
+  T scalar_from_grid(V3<T> p, const std::vector<T>& f) const {
+    T x = (p.x - (T)c.gmin[0]) / h, y = (p.y - (T)c.gmin[1]) / h, z = (p.z - (T)c.gmin[2]) / h;
+    return trilinear(f, x - T(0.5), y - T(0.5), z - T(0.5), NX, NY, NZ);
+  }
+  // sharp_kernel, spray_k.cu:255-258 (RE = 1.4, spray_k.cu:2815)
+  static inline T sharp(T r2) {
+    const T RE = (T)1.4f;
+    return tmax(RE * RE / tmax(r2, (T)0.0001f) - T(1), T(0));
+  }
+
+  // ================= a1-a3: hash, stable sort, reorder =================
+  // calcHashD spray_k.cu:1010-1028 ; sortParticles spray_thrust.cu:12-17 (stable
+  // by (key,index)) ; reorderDataAndFindCellStartD spray_k.cu:1052-1129.
+  void hash_sort() {
+    std::vector<uint32_t> key(P), idx(P);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < P; ++p) {
+      int i, j, k;
+      ijk_of_pos(P3(pos, p), i, j, k);
+      key[p] = (uint32_t)cid(i, j, k);
+      idx[p] = (uint32_t)p;
+    }
+    stable_counting_sort(key, idx);
+    std::fill(cstart.begin(), cstart.end(), MPF_CELL_UNDEF);
+    std::fill(cend.begin(), cend.end(), MPF_CELL_UNDEF);
+    for (int64_t s = 0; s < P; ++s) {
+      if (s == 0 || key[s] != key[s - 1]) {
+        cstart[key[s]] = (uint32_t)s;
+        if (s > 0) cend[key[s - 1]] = (uint32_t)s;
+      }
+      if (s == P - 1) cend[key[s]] = (uint32_t)(s + 1);
+    }
+    permute(idx);
+    ghash = key;
+    gindex = idx;
+  }
+  // Counting sort: count keys, exclusive scan, scatter in increasing index
+  // order.  Stable by construction, so the result is ordered by (key, idx).
+  void stable_counting_sort(std::vector<uint32_t>& key, std::vector<uint32_t>& idx) {
+    std::vector<uint32_t> cnt(G + 1, 0);
+    for (int64_t p = 0; p < P; ++p) cnt[key[p]]++;
+    uint32_t run = 0;
+    for (int64_t g = 0; g < G; ++g) { uint32_t t = cnt[g]; cnt[g] = run; run += t; }
+    std::vector<uint32_t> k2(P), i2(P);
+    for (int64_t p = 0; p < P; ++p) { uint32_t s = cnt[key[p]]++; k2[s] = key[p]; i2[s] = idx[p]; }
+    key.swap(k2); idx.swap(i2);
+  }
+  template <class V> void gather(V& a, const std::vector<uint32_t>& idx, int w) {
+    V o(a.size());
+    for (int64_t s = 0; s < P; ++s)
+      for (int q = 0; q < w; ++q) o[w * s + q] = a[w * (int64_t)idx[s] + q];
+    a.swap(o);
+  }
+  void permute(const std::vector<uint32_t>& idx) {
+    gather(pos, idx, 3); gather(vel, idx, 3); gather(flag, idx, 1); gather(mass, idx, 1);
+    gather(temp, idx, 1); gather(lheat, idx, 1); gather(solub, idx, 1); gather(gas, idx, 1);
+  }
+
+  // ================= a5: external forces =================
+  // addgravityforce_k spray_k.cu:330-338 ; addbuoyancyforce_vel spray_k.cu:356-371
+  void apply_forces() {
+    const T dt = (T)c.dt;
+    V3<T> g = mk((T)c.gravity[0], (T)c.gravity[1], (T)c.gravity[2]);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < P; ++p) {
+      V3<T> v = P3(vel, p);
+      if (flag[p] == FLUID || flag[p] == SOLID) v = v + g * dt;
+      if (c.buoyancy_kick) {
+        T vmax = (T)c.bubble_max_vel;
+        T rate = tmax(vmax - v.z, T(0)) / vmax;
+        if (flag[p] == AIR) v.z -= dt * g.z * rate * (T)c.buoyancy_rate_air;
+        else if (flag[p] == AIRSOLO) v.z -= dt * g.z * rate * (T)c.buoyancy_rate_solo;
+        else if (flag[p] == SOLID) v.z += dt * g.z * (T)0.1f;
+      }
+      S3(vel, p, v);
+    }
+  }
+
+  // ================= a6-a8: level set, redistancing, flags =================
+  // genlevelset + mindis_cell, spray_k.cu:2753-2806
+  void genlevelset() {
+#pragma omp parallel for schedule(static)
+    for (int64_t idx = 0; idx < G; ++idx) {
+      int i, j, k;
+      ijk(idx, i, j, k);
+      mark[idx] = VACUUM;
+      T r = T(0.5f) * h;
+      V3<T> gp = mk(((T)i + T(0.5f)) * h, ((T)j + T(0.5f)) * h, ((T)k + T(0.5f)) * h) + gmin();
+      T mair = T(2.5f) * h, mfl = T(2.5f) * h;
+      for (int di = -2; di <= 2; ++di)
+        for (int dj = -2; dj <= 2; ++dj)
+          for (int dk = -2; dk <= 2; ++dk) {
+            if (!valid(i + di, j + dj, k + dk)) continue;
+            int64_t g = cid(i + di, j + dj, k + dk);
+            if (cstart[g] == MPF_CELL_UNDEF) continue;
+            for (uint32_t p = cstart[g]; p < cend[g]; ++p) {
+              T d = len3(P3(pos, p) - gp);
+              if (flag[p] == AIR || flag[p] == AIRSOLO) mair = (d < mair) ? d : mair;
+              else if (flag[p] == FLUID || flag[p] == SOLID) mfl = (d < mfl) ? d : mfl;
+            }
+          }
+      mair -= r;
+      mfl -= r;
+      lsfluid[idx] = mfl;
+      lsair[idx] = mair;
+    }
+  }
+  // solvedistance, spray_k.cu:746-754
+  static inline void solvedistance(T a, T b, T cc, T& x) {
+    T d = tmin(a, tmin(b, cc)) + T(1);
+    if (d > tmax(a, tmax(b, cc)))
+      d = (a + b + cc + std::sqrt(T(3) - (a - b) * (a - b) - (a - cc) * (a - cc) - (b - cc) * (b - cc))) / T(3);
+    if (d < x) x = d;
+  }
+  // sweepphibytype (spray_k.cu:771-789) as a two-buffer sweep; typeflag < 0
+  // sweeps every cell (sweepphi, spray_k.cu:755-770).
+  void sweep(std::vector<T>& phi, const std::vector<int8_t>& mk_, int typeflag) {
+    std::vector<T> out(phi);
+#pragma omp parallel for schedule(static)
+    for (int64_t idx = 0; idx < G; ++idx) {
+      if (typeflag >= 0 && mk_[idx] == typeflag) continue;
+      int i, j, k;
+      ijk(idx, i, j, k);
+      T res = phi[idx];
+      for (int di = -1; di <= 1; di += 2)
+        for (int dj = -1; dj <= 1; dj += 2)
+          for (int dk = -1; dk <= 1; dk += 2)
+            if (valid(i + di, j, k) && valid(i, j + dj, k) && valid(i, j, k + dk))
+              solvedistance(phi[cid(i + di, j, k)], phi[cid(i, j + dj, k)], phi[cid(i, j, k + dk)], res);
+      out[idx] = res;
+    }
+    phi.swap(out);
+  }
+  // markLS_bigpositive / setLSback_bigpositive / preparels / setLSback,
+  // spray_k.cu:3683-3731, sequenced as in sweepLSAndMardGrid spray.cu:2011-2065
+  void redistance(std::vector<T>& ls) {
+    for (int64_t idx = 0; idx < G; ++idx) {
+      ls[idx] = ls[idx] / h;
+      if (ls[idx] > (T)1.99f) { ls[idx] = T(5.0f); mark[idx] = AIR; }
+      else mark[idx] = FLUID;
+    }
+    for (int it = 0; it < 5; ++it) sweep(ls, mark, FLUID);
+    for (int64_t idx = 0; idx < G; ++idx) ls[idx] = ls[idx] * h;
+    for (int64_t idx = 0; idx < G; ++idx) {
+      ls[idx] = -ls[idx] / h;
+      if (ls[idx] > 0) { ls[idx] = T(5.0f); mark[idx] = AIR; }
+      else mark[idx] = FLUID;
+    }
+    for (int it = 0; it < 5; ++it) sweep(ls, mark, FLUID);
+    for (int64_t idx = 0; idx < G; ++idx) ls[idx] = -ls[idx] * h;
+  }
+  // mergeLSAndMarkGrid, spray_k.cu:3733-3777
+  void merge_and_mark() {
+#pragma omp parallel for schedule(static)
+    for (int64_t idx = 0; idx < G; ++idx) {
+      T lf = lsfluid[idx], la = lsair[idx];
+      if (la > (T)4.99f * h) {
+        lsmerge[idx] = lf;
+        mark[idx] = (lf > 0) ? VACUUM : FLUID;
+      } else if (lf > (T)4.99f * h) {
+        lsmerge[idx] = la;
+        mark[idx] = (la > 0) ? VACUUM : AIR;
+      } else if (la > (T)0.8f * h && lf > (T)0.8f * h) {
+        mark[idx] = VACUUM;
+        lsmerge[idx] = tmin(lf, la);
+      } else {
+        lsmerge[idx] = (lf - la) * (T)0.5f;
+        mark[idx] = (lsmerge[idx] > 0) ? AIR : FLUID;
+      }
+      int i, j, k;
+      ijk(idx, i, j, k);
+      if (i == 0 || i == NX - 1 || j == 0 || j == NY - 1 || k == 0 || k == NZ - 1) {
+        mark[idx] = BOUNDARY;
+        lsmerge[idx] = (T)-0.5f * h;
+      }
+    }
+  }
+  // computePhigra, spray_k.cu:3994-4019
+  void phigrad() {
+#pragma omp parallel for schedule(static)
+    for (int64_t idx = 0; idx < G; ++idx) {
+      int i, j, k;
+      ijk(idx, i, j, k);
+      T c0 = lsmerge[idx];
+      T x1 = valid(i + 1, j, k) ? lsmerge[cid(i + 1, j, k)] : c0;
+      T x0 = valid(i - 1, j, k) ? lsmerge[cid(i - 1, j, k)] : c0;
+      T y1 = valid(i, j + 1, k) ? lsmerge[cid(i, j + 1, k)] : c0;
+      T y0 = valid(i, j - 1, k) ? lsmerge[cid(i, j - 1, k)] : c0;
+      T z1 = valid(i, j, k + 1) ? lsmerge[cid(i, j, k + 1)] : c0;
+      T z0 = valid(i, j, k - 1) ? lsmerge[cid(i, j, k - 1)] : c0;
+      phigx[idx] = ((x1 - x0) * T(0.5f)) / h;
+      phigy[idx] = ((y1 - y0) * T(0.5f)) / h;
+      phigz[idx] = ((z1 - z0) * T(0.5f)) / h;
+    }
+  }
+  // markfluid_dense + markBoundaryCell, spray_k.cu:506-545 (over all cells;
+  // the reference launches only min(P, G) threads, spray.cu:525)
+  void mark_from_counts() {
+#pragma omp parallel for schedule(static)
+    for (int64_t idx = 0; idx < G; ++idx) {
+      int cfs = 0, ca = 0;
+      if (cstart[idx] != MPF_CELL_UNDEF)
+        for (uint32_t p = cstart[idx]; p < cend[idx]; ++p) {
+          if (flag[p] == FLUID || flag[p] == SOLID) cfs++;
+          else if (flag[p] == AIR) ca++;
+        }
+      mark[idx] = (cfs == 0 && ca == 0) ? VACUUM : (cfs > ca ? FLUID : AIR);
+      int i, j, k;
+      ijk(idx, i, j, k);
+      if (i == 0 || i == NX - 1 || j == 0 || j == NY - 1 || k == 0 || k == NZ - 1) mark[idx] = BOUNDARY;
+    }
+  }
+  // Solid constraint (no reference counterpart): a non-boundary cell whose
+  // particles are at least half SOLID is marked SOLID.
+  void mark_solid_cells() {
+#pragma omp parallel for schedule(static)
+    for (int64_t idx = 0; idx < G; ++idx) {
+      if (mark[idx] == BOUNDARY || cstart[idx] == MPF_CELL_UNDEF) continue;
+      int cs = 0, co = 0;
+      for (uint32_t p = cstart[idx]; p < cend[idx]; ++p) {
+        if (flag[p] == SOLID) cs++; else co++;
+      }
+      if (cs > 0 && cs >= co) mark[idx] = SOLID;
+    }
+  }
+  void levelset_update() {
+    genlevelset();
+    redistance(lsfluid);
+    redistance(lsair);
+    merge_and_mark();
+    phigrad();
+    if (c.flags_from_counts) mark_from_counts();
+    if (c.solid_constraint) mark_solid_cells();
+  }
+
+  // ================= a9: P2G =================
+  // mapvelp2g_k_fluidSolid + sumcell_fluidSolid (spray_k.cu:2809-2871) and
+  // mapvelp2g_k_air + sumcell_air (spray_k.cu:2873-2937); the FLIP snapshot
+  // copies of mapvelp2g_bubble (spray.cu:1578-1587).
+  void p2g_phase(std::vector<T>* out, bool air) {
+    const T scale = T(1) / h;
+    const int lo = air ? -2 : -1, hi_n = air ? 1 : 0, lo_t = air ? -2 : -1, hi_t = air ? 2 : 1;
+    const int64_t fc[3] = {FX, FY, FZ};
+    for (int a = 0; a < 3; ++a) {
+      const int w = NX + (a == 0), hh = NY + (a == 1), d = NZ + (a == 2);
+#pragma omp parallel for schedule(static)
+      for (int64_t f = 0; f < fc[a]; ++f) {
+        int i, j, k;
+        ijk3(f, w, hh, d, i, j, k);
+        V3<T> gp = mk((T)i + (a == 0 ? T(0) : T(0.5)), (T)j + (a == 1 ? T(0) : T(0.5)), (T)k + (a == 2 ? T(0) : T(0.5)));
+        T weight = 0, usum = 0;
+        int r0[3], r1[3];
+        for (int q = 0; q < 3; ++q) { r0[q] = (q == a) ? lo : lo_t; r1[q] = (q == a) ? hi_n : hi_t; }
+        for (int di = r0[0]; di <= r1[0]; ++di)
+          for (int dj = r0[1]; dj <= r1[1]; ++dj)
+            for (int dk = r0[2]; dk <= r1[2]; ++dk) {
+              if (!valid(i + di, j + dj, k + dk)) continue;
+              int64_t g = cid(i + di, j + dj, k + dk);
+              if (cstart[g] == MPF_CELL_UNDEF) continue;
+              for (uint32_t p = cstart[g]; p < cend[g]; ++p) {
+                bool take = air ? (flag[p] == AIR) : (flag[p] == FLUID || flag[p] == SOLID);
+                if (!take) continue;
+                V3<T> dv = (P3(pos, p) - gmin()) * scale - gp;
+                T w2 = mass[p] * sharp(dot3(dv, dv));
+                weight += w2;
+                usum += w2 * vel[3 * (int64_t)p + a];
+              }
+            }
+        out[a][f] = (weight > 0) ? (usum / weight) : T(0);
+      }
+    }
+  }
+  void p2g() {
+    p2g_phase(wu, false);
+    p2g_phase(au, true);
+    for (int a = 0; a < 3; ++a) { wuo[a] = wu[a]; auo[a] = au[a]; }
+  }
+
+  // ================= a10: extrapolation and wall conditions =================
+  // initphi spray_k.cu:712-723 + 3 x sweepphibytype: sweepPhi spray.cu:570-582
+  void sweep_phi(std::vector<T>& phi, int8_t type) {
+    for (int64_t idx = 0; idx < G; ++idx) phi[idx] = (mark[idx] == type) ? T(-0.5) : (T)(NX * 3);
+    for (int it = 0; it < 3; ++it) sweep(phi, mark, type);
+  }
+  // sweepu_k_bubble spray_k.cu:3779-3880 (two ping-pong passes, spray.cu:584-596)
+  void sweep_u_pass(std::vector<T>* out, const std::vector<T>* u, const std::vector<T>& ls, int8_t sf) {
+    const int64_t fc[3] = {FX, FY, FZ};
+    for (int a = 0; a < 3; ++a) {
+      const int w = NX + (a == 0), hh = NY + (a == 1), d = NZ + (a == 2);
+#pragma omp parallel for schedule(static)
+      for (int64_t f = 0; f < fc[a]; ++f) {
+        out[a][f] = u[a][f];
+        int i, j, k;
+        ijk3(f, w, hh, d, i, j, k);
+        int n[3] = {i, j, k};
+        int lim = (a == 0) ? NX : (a == 1) ? NY : NZ;
+        if (!(n[a] > 1 && n[a] < lim - 1)) continue;
+        int bi = i - (a == 0), bj = j - (a == 1), bk = k - (a == 2);  // the other cell of the face
+        if (!(mark[cid(i, j, k)] != sf && mark[cid(bi, bj, bk)] != sf)) continue;
+        for (int di = -1; di <= 1; di += 2)
+          for (int dj = -1; dj <= 1; dj += 2)
+            for (int dk = -1; dk <= 1; dk += 2) {
+              int dd[3] = {di, dj, dk};
+              // transverse neighbour range check
+              bool bad = false;
+              for (int q = 0; q < 3; ++q) {
+                if (q == a) continue;
+                int lq = (q == 0) ? NX : (q == 1) ? NY : NZ;
+                if (n[q] + dd[q] < 0 || n[q] + dd[q] > lq - 1) bad = true;
+              }
+              if (bad) continue;
+              T lsc = ls[cid(i, j, k)], lsb = ls[cid(bi, bj, bk)];
+              T wv[3];
+              wv[a] = (T)(-dd[a]) * (lsc - lsb);
+              if (wv[a] < 0) continue;
+              // the two transverse weights, in the reference's order
+              int t1 = (a == 0) ? 1 : (a == 1) ? 0 : 1;  // x-face: wy then wz; y-face: wx then wz; z-face: wy then wx
+              int t2 = (a == 0) ? 2 : (a == 1) ? 2 : 0;
+              bool skip = false;
+              for (int tt = 0; tt < 2 && !skip; ++tt) {
+                int q = tt == 0 ? t1 : t2;
+                int o[3] = {0, 0, 0};
+                o[q] = dd[q];
+                T lsn = ls[cid(i + o[0], j + o[1], k + o[2])];
+                T lsnb = ls[cid(bi + o[0], bj + o[1], bk + o[2])];
+                wv[q] = (lsc + lsb - lsn - lsnb) * T(0.5f);
+                if (wv[q] < 0) skip = true;
+              }
+              if (skip) continue;
+              T wx = wv[0], wy = wv[1], wz = wv[2];
+              T wsum = wx + wy + wz;
+              if (wsum == 0) wx = wy = wz = T(1.0f) / T(3);
+              else { wx /= wsum; wy /= wsum; wz /= wsum; }
+              auto U = [&](int ii, int jj, int kk) { return u[a][fid(a, ii, jj, kk)]; };
+              out[a][f] = wx * U(i + di, j, k) + wy * U(i, j + dj, k) + wz * U(i, j, k + dk);
+            }
+      }
+    }
+  }
+  void sweep_u(std::vector<T>* u, const std::vector<T>& phi, int8_t sf) {
+    std::vector<T> tmp[3] = {u[0], u[1], u[2]};
+    for (int it = 0; it < 2; ++it) {
+      sweep_u_pass(tmp, u, phi, sf);
+      for (int a = 0; a < 3; ++a) u[a].swap(tmp[a]);
+    }
+  }
+  // setWaterBoundaryU_k, spray_k.cu:956-995
+  T bc_face_rule(int a, int i, int j, int k, T v) const {
+    auto isb = [&](int ii, int jj, int kk) { return mark[cid(ii, jj, kk)] == BOUNDARY; };
+    if (a == 0) {
+      if (i <= 1 || i >= (NX + 1) - 2) return T(0);
+      if (i > 1 && i < NX - 1 && (isb(i, j, k) != isb(i - 1, j, k))) return T(0);
+      return v;
+    }
+    if (a == 1) {
+      if (j <= 1 || j >= (NY + 1) - 2) return T(0);
+      if (j > 0 && j < NY && (isb(i, j, k) != isb(i, j - 1, k))) return T(0);
+      return v;
+    }
+    return v;  // uz handled in bc()
+  }
+  void bc(std::vector<T>* u) {
+    for (int a = 0; a < 2; ++a) {
+      const int64_t fc = a == 0 ? FX : FY;
+      const int w = NX + (a == 0), hh = NY + (a == 1), d = NZ;
+#pragma omp parallel for schedule(static)
+      for (int64_t f = 0; f < fc; ++f) {
+        int i, j, k;
+        ijk3(f, w, hh, d, i, j, k);
+        u[a][f] = bc_face_rule(a, i, j, k, u[a][f]);
+      }
+    }
+    // uz in index order (k fastest): the ceiling rule sees uz(k-1) after its own rule
+    auto isb = [&](int ii, int jj, int kk) { return mark[cid(ii, jj, kk)] == BOUNDARY; };
+    const int zn = NZ + 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t col = 0; col < (int64_t)NX * NY; ++col) {
+      int i = (int)(col / NY), j = (int)(col % NY);
+      for (int k = 0; k < zn; ++k) {
+        int64_t f = fid(2, i, j, k);
+        T& v = u[2][f];
+        if (k <= 1 || k >= zn - 1) v = T(0);
+        else if (k == zn - 2) { T b = u[2][fid(2, i, j, k - 1)]; v = (b < 0) ? b : T(0); }
+        else if (k > 0 && k < NZ && (isb(i, j, k) != isb(i, j, k - 1))) v = T(0);
+      }
+    }
+  }
+  void extrapolate_bc(bool recompute_phi) {
+    if (recompute_phi) sweep_phi(phiair, AIR);
+    sweep_u(au, phiair, AIR);
+    if (recompute_phi) sweep_phi(phifluid, FLUID);
+    sweep_u(wu, phifluid, FLUID);
+    bc(au);
+    bc(wu);
+  }
+
+  // ================= a11-a13: projection =================
+  bool unknown(int8_t m) const { return m == FLUID || m == AIR; }
+  bool neumann(int8_t m) const { return m == BOUNDARY || (c.solid_constraint && m == SOLID); }
+  int8_t mark_at(int i, int j, int k) const { return valid(i, j, k) ? mark[cid(i, j, k)] : BOUNDARY; }
+  // cptdivergence_bubble2, spray_k.cu:3138-3263; SOLID neighbours under the
+  // solid constraint contribute the (fixed) solid face velocity 0.
+  void divergence() {
+#pragma omp parallel for schedule(static)
+    for (int64_t idx = 0; idx < G; ++idx) {
+      T dv = 0;
+      int i, j, k;
+      ijk(idx, i, j, k);
+      int8_t m = mark[idx];
+      if (m == FLUID || m == AIR) {
+        T uu[6];
+        int q = 0;
+        for (int a = 0; a < 3; ++a)
+          for (int s = 1; s >= 0; --s) {  // order: +face (u1) then -face (u0)
+            int o[3] = {0, 0, 0};
+            o[a] = s ? 1 : -1;
+            int ni = i + o[0], nj = j + o[1], nk = k + o[2];
+            int fi = i + (a == 0 && s), fj = j + (a == 1 && s), fk = k + (a == 2 && s);
+            int64_t fidx = fid(a, fi, fj, fk);
+            int8_t mn = mark_at(ni, nj, nk);
+            T v;
+            if (c.solid_constraint && mn == SOLID) v = T(0);
+            else if (m == FLUID && mn != AIR) v = wu[a][fidx];
+            else if (m == AIR && mn != FLUID) v = au[a][fidx];
+            else {
+              T lc = lsmerge[idx], ln = lsmerge[cid(ni, nj, nk)];
+              T th = (T(0.0f) - lc) / (ln - lc);
+              if (m == FLUID) v = th * wu[a][fidx] + (T(1) - th) * au[a][fidx];
+              else v = th * au[a][fidx] + (T(1) - th) * wu[a][fidx];
+            }
+            uu[q++] = v;
+          }
+        // uu: ux1, ux0, uy1, uy0, uz1, uz0
+        dv = (uu[0] - uu[1] + uu[2] - uu[3] + uu[4] - uu[5]) / h;
+      }
+      div[idx] = dv;
+    }
+  }
+  // 1/rho face coefficient of the variable-density operator; the face between
+  // a FLUID and an AIR cell uses theta of the lower-index cell of the face, so
+  // the operator is exactly symmetric.
+  double face_beta(int64_t ca, int64_t cb) const {
+    double il = 1.0 / (double)c.rho_liquid, ia = 1.0 / (double)c.rho_air;
+    int8_t ma = mark[ca], mb = mark[cb];
+    if (!unknown(mb)) return ma == FLUID ? il : ia;
+    if (!unknown(ma)) return mb == FLUID ? il : ia;
+    if (ma == mb) return ma == FLUID ? il : ia;
+    int64_t lo = ca < cb ? ca : cb, hi = ca < cb ? cb : ca;
+    double ll = (double)lsmerge[lo], lh = (double)lsmerge[hi];
+    double th = -ll / (lh - ll);
+    th = std::min(1.0, std::max(0.0, th));
+    double blo = mark[lo] == FLUID ? il : ia, bhi = mark[hi] == FLUID ? il : ia;
+    return bhi + th * (blo - bhi);
+  }
+  // A x: computeAx_bubble (spray_k.cu:3423-3447) for UNIT, 1/rho weights for VARDENS.
+  void apply_A(const std::vector<double>& x, std::vector<double>& y) const {
+    const double h2 = (double)h * (double)h;
+#pragma omp parallel for schedule(static)
+    for (int64_t idx = 0; idx < G; ++idx) {
+      if (!unknown(mark[idx])) { y[idx] = 0.0; continue; }
+      int i, j, k;
+      ijk(idx, i, j, k);
+      double xc = x[idx], s = 0.0;
+      const int off[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+      if (c.op == MPF_OPERATOR_UNIT) {
+        s = -6.0 * xc;
+        for (auto& o : off) {
+          int8_t mn = mark_at(i + o[0], j + o[1], k + o[2]);
+          s += neumann(mn) ? xc : (unknown(mn) ? x[cid(i + o[0], j + o[1], k + o[2])] : 0.0);
+        }
+      } else {
+        for (auto& o : off) {
+          int8_t mn = mark_at(i + o[0], j + o[1], k + o[2]);
+          if (neumann(mn)) continue;
+          int64_t nb = cid(i + o[0], j + o[1], k + o[2]);
+          double xn = unknown(mn) ? x[nb] : 0.0;
+          s += face_beta(idx, nb) * (xn - xc);
+        }
+      }
+      y[idx] = s / h2;
+    }
+  }
+  void diag_A(std::vector<double>& d) const {
+    const double h2 = (double)h * (double)h;
+#pragma omp parallel for schedule(static)
+    for (int64_t idx = 0; idx < G; ++idx) {
+      if (!unknown(mark[idx])) { d[idx] = 1.0; continue; }
+      int i, j, k;
+      ijk(idx, i, j, k);
+      double s = 0.0;
+      const int off[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+      for (auto& o : off) {
+        int8_t mn = mark_at(i + o[0], j + o[1], k + o[2]);
+        if (neumann(mn)) continue;
+        s += (c.op == MPF_OPERATOR_UNIT) ? 1.0 : face_beta(idx, cid(i + o[0], j + o[1], k + o[2]));
+      }
+      d[idx] = s > 0 ? -s / h2 : 1.0;
+    }
+  }
+  // solver_cg_bubble (spray.cu:1613-1687) and the Jacobi-PCG form, in double.
+  int solve_pressure(mpf_solve_info* info) {
+    std::vector<double> x(G, 0.0), r(G), z(G), p(G), q(G), d(G), b(G);
+    for (int64_t i = 0; i < G; ++i) b[i] = unknown(mark[i]) ? (double)div[i] : 0.0;
+    const bool ref = c.solver == MPF_SOLVER_REFERENCE_CG;
+    if (!ref) diag_A(d);
+    auto dotv = [&](const std::vector<double>& u, const std::vector<double>& v) {
+      return rsum(G, [&](int64_t i) { return u[i] * v[i]; });
+    };
+    // x = 0: r = b
+    for (int64_t i = 0; i < G; ++i) r[i] = b[i];
+    double bnorm = std::sqrt(dotv(b, b));
+    double eps = 1e-2 * (double)G;
+    double tol = c.tol;
+    int it = 0;
+    bool conv = false;
+    double rr = dotv(r, r);
+    if (ref ? (rr < eps) : (bnorm == 0.0 || std::sqrt(rr) <= tol * bnorm)) conv = true;
+    if (!conv) {
+      for (int64_t i = 0; i < G; ++i) z[i] = ref ? r[i] : (unknown(mark[i]) ? r[i] / d[i] : 0.0);
+      p = z;
+      double rz = dotv(z, r);
+      const int cap = ref ? c.ref_max_iters : c.max_iters;
+      for (it = 0; it < cap;) {
+        apply_A(p, q);
+        double pq = dotv(q, p);
+        if (pq == 0.0) { conv = true; break; }
+        double alpha = rz / pq;
+        for (int64_t i = 0; i < G; ++i) {
+          if (!unknown(mark[i])) continue;
+          x[i] += alpha * p[i];
+          r[i] -= alpha * q[i];
+        }
+        ++it;
+        rr = dotv(r, r);
+        if (ref ? (rr < eps) : (std::sqrt(rr) <= tol * bnorm)) { conv = true; break; }
+        for (int64_t i = 0; i < G; ++i) z[i] = ref ? r[i] : (unknown(mark[i]) ? r[i] / d[i] : 0.0);
+        double rz2 = dotv(z, r);
+        double beta = rz2 / rz;
+        for (int64_t i = 0; i < G; ++i) p[i] = unknown(mark[i]) ? z[i] + beta * p[i] : 0.0;
+        rz = rz2;
+      }
+    }
+    apply_A(x, q);
+    double tr = rsum(G, [&](int64_t i) { double e = b[i] - q[i]; return e * e; });
+    for (int64_t i = 0; i < G; ++i) press[i] = (T)x[i];
+    if (info) {
+      info->iterations = it;
+      info->refinements = 0;
+      info->converged = conv ? 1 : 0;
+      info->rel_residual = bnorm > 0 ? std::sqrt(tr) / bnorm : 0.0;
+      info->abs_rr = rr;
+    }
+    return conv ? MPF_OK : MPF_NO_CONVERGENCE;
+  }
+  // subGradPress, spray_k.cu:198-224 (both grids, spray.cu:1605-1606)
+  void sub_grad(std::vector<T>* u, double inv_rho) {
+    const int64_t fc[3] = {FX, FY, FZ};
+    for (int a = 0; a < 3; ++a) {
+      const int w = NX + (a == 0), hh = NY + (a == 1), d = NZ + (a == 2);
+#pragma omp parallel for schedule(static)
+      for (int64_t f = 0; f < fc[a]; ++f) {
+        int i, j, k;
+        ijk3(f, w, hh, d, i, j, k);
+        int n[3] = {i, j, k};
+        int lim = (a == 0) ? NX : (a == 1) ? NY : NZ;
+        if (!(n[a] > 0 && n[a] < lim)) continue;
+        int bi = i - (a == 0), bj = j - (a == 1), bk = k - (a == 2);
+        if (c.solid_constraint && (mark[cid(i, j, k)] == SOLID || mark[cid(bi, bj, bk)] == SOLID)) {
+          u[a][f] = T(0);
+          continue;
+        }
+        if (c.op == MPF_OPERATOR_UNIT)
+          u[a][f] -= (press[cid(i, j, k)] - press[cid(bi, bj, bk)]) / h;
+        else
+          u[a][f] = (T)((double)u[a][f] - inv_rho * (((double)press[cid(i, j, k)] - (double)press[cid(bi, bj, bk)]) / (double)h));
+      }
+    }
+  }
+  int project(mpf_solve_info* info) {
+    divergence();
+    int st = solve_pressure(info);
+    sub_grad(wu, 1.0 / (double)c.rho_liquid);
+    sub_grad(au, 1.0 / (double)c.rho_air);
+    return st;
+  }
+
+  // ================= a14: G2P =================
+  // computeDeltaU spray_k.cu:997-1006 ; mapvelg2p_flip_bubble spray_k.cu:3525-3540 ;
+  // blend of advectparticle_RK2 spray_k.cu:421-424 when flip_alpha < 1.
+  void g2p() {
+    for (int a = 0; a < 3; ++a) {
+      for (size_t f = 0; f < wu[a].size(); ++f) wuo[a][f] = wu[a][f] - wuo[a][f];
+      for (size_t f = 0; f < au[a].size(); ++f) auo[a][f] = au[a][f] - auo[a][f];
+    }
+    const T alpha = (T)c.flip_alpha;
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < P; ++p) {
+      int8_t fl = flag[p];
+      const std::vector<T>* du = nullptr;
+      const std::vector<T>* un = nullptr;
+      if (fl == FLUID || fl == SOLID) { du = wuo; un = wu; }
+      else if (fl == AIR) { du = auo; un = au; }
+      if (!du) continue;
+      V3<T> x = P3(pos, p), v = P3(vel, p);
+      V3<T> g = vel_from_grid(x, du);
+      if (alpha == T(1)) v = v + g;
+      else {
+        V3<T> u = vel_from_grid(x, un);
+        v = u * (T(1) - alpha) + (v + g) * alpha;
+      }
+      S3(vel, p, v);
+    }
+  }
+
+  // ================= a15: advection =================
+  void solid_phi_and_velocity() {
+    // initSolidPhi spray_k.cu:724-744 ; sweepphi x3 ; solid P2G (the
+    // fluidSolid kernel on the solid grid, spray.cu:2972)
+    for (int64_t idx = 0; idx < G; ++idx) {
+      bool any = false;
+      if (cstart[idx] != MPF_CELL_UNDEF)
+        for (uint32_t p = cstart[idx]; p < cend[idx]; ++p) if (flag[p] == SOLID) any = true;
+      phisolid[idx] = any ? T(-0.5f) : (T)(3 * NX);
+    }
+    for (int it = 0; it < 3; ++it) sweep(phisolid, mark, -1);
+    p2g_phase(su, false);
+  }
+  // CollisionWithSolid_k, spray_k.cu:5276-5345
+  void collide_and_move() {
+    const T dt = (T)c.dt;
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < P; ++p) {
+      if (flag[p] == SOLID) continue;
+      V3<T> x = P3(pos, p), v = P3(vel, p);
+      T iphi = scalar_from_grid(x, phisolid);
+      if (iphi <= T(0.5f)) {
+        V3<T> sv = vel_from_grid(x, su);
+        V3<T> rv = v - sv;
+        T d = h * T(0.5f);
+        V3<T> g;
+        g.x = scalar_from_grid(x + mk(d, T(0), T(0)), phisolid) - scalar_from_grid(x - mk(d, T(0), T(0)), phisolid);
+        g.y = scalar_from_grid(x + mk(T(0), d, T(0)), phisolid) - scalar_from_grid(x - mk(T(0), d, T(0)), phisolid);
+        g.z = scalar_from_grid(x + mk(T(0), T(0), d), phisolid) - scalar_from_grid(x - mk(T(0), T(0), d), phisolid);
+        T gl = len3(g);
+        if (gl > 0) {
+          T inv = T(1) / gl;  // normalize: v * rsqrt(dot(v,v)) restated as 1/length
+          g = g * inv;
+          T dn = dot3(rv, g);
+          if (dn < 0) v = v - g * ((T)c.bounce_vel * dn);
+          x = x + ((g * (T)c.bounce_pos) * (T(0.5f) - iphi)) * h;
+        }
+      }
+      x = x + v * dt;
+      T rate = (flag[p] == AIRSOLO) ? T(0.8f) : T(0.5f);
+      V3<T> lo = gmin() + mk(h + rate * h, h + rate * h, h + rate * h);
+      V3<T> hi = gmax() - mk(h + rate * h, h + rate * h, h + rate * h);
+      if (x.x <= lo.x) { x.x = lo.x; v.x = 0; }
+      if (x.y <= lo.y) { x.y = lo.y; v.y = 0; }
+      if (x.z <= lo.z) { x.z = lo.z; v.z = 0; }
+      if (x.x >= hi.x) { x.x = hi.x; v.x = 0; }
+      if (x.y >= hi.y) { x.y = hi.y; v.y = 0; }
+      if (x.z >= hi.z) { x.z = hi.z; v.z = 0; }
+      S3(vel, p, v);
+      S3(pos, p, x);
+    }
+  }
+  // advectparticle_RK2 spray_k.cu:409-459 (position part; velocity blend is in g2p),
+  // two-phase grids as advectparticle_RK2_bubble spray_k.cu:3463-3523.
+  void advect_rk2() {
+    const T dt = (T)c.dt;
+    const T ss = (T)c.samplespace;
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < P; ++p) {
+      int8_t fl = flag[p];
+      if (fl == AIRSOLO) continue;
+      const std::vector<T>* g = (fl == AIR) ? au : wu;
+      V3<T> x = P3(pos, p), v = P3(vel, p);
+      V3<T> lo = gmin() + mk(h + T(0.5f) * ss, h + T(0.5f) * ss, h + T(0.5f) * ss);
+      V3<T> hi = gmax() - mk(h + T(0.5f) * ss, h + T(0.5f) * ss, h + T(0.5f) * ss);
+      V3<T> gv = vel_from_grid(x, g);
+      V3<T> mid = x + (gv * dt) * T(0.5);
+      V3<T> gm = vel_from_grid(mid, g);
+      x = x + gm * dt;
+      if (x.x <= lo.x) { x.x = lo.x; v.x = 0; }
+      if (x.y <= lo.y) { x.y = lo.y; v.y = 0; }
+      if (x.z <= lo.z) { x.z = lo.z; v.z = 0; }
+      if (x.x >= hi.x) { x.x = hi.x; v.x = 0; }
+      if (x.y >= hi.y) { x.y = hi.y; v.y = 0; }
+      if (x.z >= hi.z) { x.z = hi.z; v.z = 0; }
+      if (fl != SOLID) S3(pos, p, x);
+      S3(vel, p, v);
+    }
+  }
+  void advect() {
+    if (c.fix_solids)  // setVelZeroSolid_k spray_k.cu:5213-5218
+      for (int64_t p = 0; p < P; ++p) if (flag[p] == SOLID) S3(vel, p, mk(T(0), T(0), T(0)));
+    if (c.integrator == MPF_ADVECT_RK2) { advect_rk2(); return; }
+    solid_phi_and_velocity();
+    collide_and_move();
+  }
+
+  // ================= a16: heat =================
+  // mapHeatp2g_hash + sumHeat, spray_k.cu:4619-4653
+  void heat_p2g() {
+    const T scale = T(1) / h;
+#pragma omp parallel for schedule(static)
+    for (int64_t idx = 0; idx < G; ++idx) {
+      int i, j, k;
+      ijk(idx, i, j, k);
+      V3<T> gp = mk((T)i + T(0.5), (T)j + T(0.5), (T)k + T(0.5));
+      T weight = 0, hs = 0;
+      for (int di = -1; di <= 1; ++di)
+        for (int dj = -1; dj <= 1; ++dj)
+          for (int dk = -1; dk <= 1; ++dk) {
+            if (!valid(i + di, j + dj, k + dk)) continue;
+            int64_t g = cid(i + di, j + dj, k + dk);
+            if (cstart[g] == MPF_CELL_UNDEF) continue;
+            for (uint32_t p = cstart[g]; p < cend[g]; ++p) {
+              V3<T> dv = (P3(pos, p) - gmin()) * scale - gp;
+              T w = sharp(dot3(dv, dv));
+              weight += w;
+              hs += w * temp[p];
+            }
+          }
+      tp[idx] = (weight > 0) ? (hs / weight) : (T)c.default_temperature;
+    }
+  }
+  bool heat_nb_neumann(int8_t m) const {
+    return m == BOUNDARY || (c.heat_vacuum_neumann && m == VACUUM);
+  }
+  // computeAx_heat spray_k.cu:4526-4567 (boundary rows left 0)
+  void apply_heat_A(const std::vector<double>& x, std::vector<double>& y) const {
+    const double hh = (double)h * (double)h, dt = (double)c.dt;
+#pragma omp parallel for schedule(static)
+    for (int64_t idx = 0; idx < G; ++idx) {
+      if (mark[idx] == BOUNDARY) { y[idx] = 0.0; continue; }
+      int i, j, k;
+      ijk(idx, i, j, k);
+      double al = (double)c.heat_alpha[(int)mark[idx]];
+      double xc = x[idx];
+      double s = (hh / al / dt + 6.0) * xc;
+      const int off[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+      for (auto& o : off) {
+        int8_t mn = mark_at(i + o[0], j + o[1], k + o[2]);
+        s -= heat_nb_neumann(mn) ? xc : x[cid(i + o[0], j + o[1], k + o[2])];
+      }
+      y[idx] = s;
+    }
+  }
+  int heat(mpf_solve_info* info) {
+    heat_p2g();
+    if (c.heat_add_bottom) {  // addHeatAtBottom spray_k.cu:4492-4504
+      for (int64_t idx = 0; idx < G; ++idx) {
+        int i, j, k;
+        ijk(idx, i, j, k);
+        if (i >= 1 && i < NX - 1 && j >= 1 && j < NY - 1 && k <= 3) tp[idx] += (T)c.heat_increase_bottom;
+        tp[idx] = tmin((T)c.heat_bottom_clamp, tp[idx]);
+      }
+    }
+    std::vector<T> tsave = tp;
+    // compb_heat spray_k.cu:4507-4523
+    std::vector<double> b(G), x(G, 0.0), r(G), p(G), q(G);
+    const double hh = (double)h * (double)h, dt = (double)c.dt;
+    for (int64_t idx = 0; idx < G; ++idx) {
+      double al = (double)c.heat_alpha[(int)mark[idx]];
+      b[idx] = (mark[idx] == BOUNDARY) ? 0.0 : (double)tp[idx] * hh / al / dt;
+    }
+    int it = 0;
+    bool conv = true;
+    double rr = 0;
+    if (c.heat_solver == MPF_HEAT_CG) {
+      // solver_cg_heat spray.cu:1780-1852 (x0 = 0, stop r.r < 1e-2 G)
+      const double eps = 1e-2 * (double)G;
+      for (int64_t i = 0; i < G; ++i) r[i] = b[i];
+      rr = rsum(G, [&](int64_t i) { return r[i] * r[i]; });
+      conv = rr < eps;
+      if (!conv) {
+        p = r;
+        double a = rr;
+        for (it = 0; it < c.heat_max_iters;) {
+          apply_heat_A(p, q);
+          double zp = rsum(G, [&](int64_t i) { return q[i] * p[i]; });
+          if (zp == 0.0) { conv = true; break; }
+          double al = a / zp;
+          for (int64_t i = 0; i < G; ++i) {
+            if (mark[i] == BOUNDARY) continue;
+            x[i] += al * p[i];
+            r[i] -= al * q[i];
+          }
+          ++it;
+          rr = rsum(G, [&](int64_t i) { return r[i] * r[i]; });
+          if (rr < eps) { conv = true; break; }
+          double beta = rr / a;
+          for (int64_t i = 0; i < G; ++i) p[i] = (mark[i] == BOUNDARY) ? 0.0 : r[i] + beta * p[i];
+          a = rr;
+        }
+      }
+    } else {
+      // Jacobi sweeps on the same operator, starting from the mapped field
+      for (int64_t i = 0; i < G; ++i) x[i] = (mark[i] == BOUNDARY) ? 0.0 : (double)tsave[i];
+      std::vector<double> xn(G);
+      for (it = 0; it < c.heat_jacobi_iters; ++it) {
+#pragma omp parallel for schedule(static)
+        for (int64_t idx = 0; idx < G; ++idx) {
+          if (mark[idx] == BOUNDARY) { xn[idx] = 0.0; continue; }
+          int i, j, k;
+          ijk(idx, i, j, k);
+          double al = (double)c.heat_alpha[(int)mark[idx]];
+          double dg = hh / al / dt + 6.0, s = b[idx];
+          const int off[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+          for (auto& o : off) {
+            int8_t mn = mark_at(i + o[0], j + o[1], k + o[2]);
+            if (heat_nb_neumann(mn)) dg -= 1.0;
+            else s += x[cid(i + o[0], j + o[1], k + o[2])];
+          }
+          xn[idx] = s / dg;
+        }
+        x.swap(xn);
+      }
+      apply_heat_A(x, q);
+      rr = rsum(G, [&](int64_t i) { double e = (mark[i] == BOUNDARY) ? 0.0 : b[i] - q[i]; return e * e; });
+    }
+    for (int64_t i = 0; i < G; ++i) tp[i] = (T)x[i];
+    if (c.heat_extend_boundary) {  // setBoundaryHeat spray_k.cu:4585-4605 (in place, index order)
+      for (int64_t idx = 0; idx < G; ++idx) {
+        int i, j, k;
+        ijk(idx, i, j, k);
+        if (i == NX - 1) tp[idx] = tp[cid(i - 1, j, k)];
+        else if (i == 0) tp[idx] = tp[cid(i + 1, j, k)];
+        else if (j == NY - 1) tp[idx] = tp[cid(i, j - 1, k)];
+        else if (j == 0) tp[idx] = tp[cid(i, j + 1, k)];
+        else if (k == NZ - 1) tp[idx] = tp[cid(i, j, k - 1)];
+        else if (k == 0) tp[idx] = tp[cid(i, j, k + 1)];
+      }
+    }
+    // compTpChange spray_k.cu:4607-4617
+    for (int64_t i = 0; i < G; ++i) tpchange[i] = (mark[i] != BOUNDARY) ? tp[i] - tsave[i] : T(0);
+    // mapHeatg2p / mapHeatg2p_MeltAndBoil spray_k.cu:4655-4679 ; updateLatentHeat_k 5546-5573
+    const T at = (T)c.alpha_temp_trans;
+    const T mp = (T)c.melting_point, bp = (T)c.boiling_point, th = (T)c.liquid_heat_th;
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < P; ++p) {
+      V3<T> x3 = P3(pos, p);
+      T nt = at * (temp[p] + scalar_from_grid(x3, tpchange)) + (T(1) - at) * scalar_from_grid(x3, tp);
+      if (c.heat_melt_and_boil && flag[p] == SOLID) temp[p] = T(0.95f) * temp[p] + T(0.05f) * nt;
+      else temp[p] = nt;
+      if (flag[p] == SOLID && temp[p] > mp) { lheat[p] += temp[p] - mp; temp[p] = mp; }
+      if (flag[p] == FLUID) {
+        if (temp[p] < mp) { lheat[p] -= mp - temp[p]; temp[p] = mp; }
+        else if (temp[p] > bp) { lheat[p] += temp[p] - bp; temp[p] = bp; }
+        else lheat[p] = th;
+      }
+    }
+    if (info) {
+      info->iterations = it; info->refinements = 0; info->converged = conv ? 1 : 0;
+      double bn = std::sqrt(rsum(G, [&](int64_t i) { return b[i] * b[i]; }));
+      info->rel_residual = bn > 0 ? std::sqrt(rr) / bn : 0.0;
+      info->abs_rr = rr;
+    }
+    return MPF_OK;
+  }
+
+  // ================= a17: phase change =================
+  // updatesolubility spray_k.cu:4719-4727 ; GenerateGasParticle_k 4730-4817
+  // (slots by exclusive scan over cells) ; MeltingSolidByHeat 5059-5072 ;
+  // FreezingSolidByHeat 5074-5124 (reads the pass's input).
+  static bool in_boundary_cell(int x, int y, int NX_, int NY_) {  // isInBoundaryCell spray_k.cu:4214-4221
+    const int level = 2;
+    return x <= level || x >= NX_ - 1 - level || y <= level || y >= NY_ - 1 - level;
+  }
+  int phase_change(int frame, mpf_phase_info* info) {
+    int added = 0, melted = 0, frozen = 0;
+    if (c.gas_enabled && c.gas_every > 0 && frame % c.gas_every == 0) {
+      const T T0 = (T)c.temperature0;
+      for (int64_t p = 0; p < P; ++p)
+        if (flag[p] == FLUID)
+          solub[p] = (T)c.dissolve_gas_rate * (T)c.airm0 * std::exp((T)1018.9f * (T(1) / temp[p] - T(1) / T0));
+      std::vector<uint8_t> add(G, 0);
+      std::vector<V3<T>> npos(G);
+      const T m0 = (T)c.airm0, th = (T)c.liquid_heat_th, vr = (T)c.vapor_gen_rate;
+#pragma omp parallel for schedule(static)
+      for (int64_t idx = 0; idx < G; ++idx) {
+        int i, j, k;
+        ijk(idx, i, j, k);
+        if (k <= 1 || in_boundary_cell(i, j, NX, NY)) continue;
+        if (cstart[idx] == MPF_CELL_UNDEF) continue;
+        T gc = 0, gs = 0, gair = 0, vap = 0;
+        int gasc = 0;
+        V3<T> gpsum = mk(T(0), T(0), T(0));
+        for (uint32_t p = cstart[idx]; p < cend[idx]; ++p) {
+          if (flag[p] == FLUID) {
+            gc += gas[p]; gs += solub[p];
+            vap += tmax(T(0), lheat[p] - th) * vr * m0;
+          } else if (flag[p] == AIRSOLO || flag[p] == AIR) {
+            gair += mass[p]; gpsum = gpsum + P3(pos, p); gasc++;
+          }
+        }
+        bool seed = false;
+        for (int32_t s : seeds) if (s == idx) seed = true;
+        if (gc - gs + vap > m0 && (seed || gasc > 0)) {
+          add[idx] = 1;
+          V3<T> rp = mk((T)uniform24(c.seed, (uint64_t)frame, (uint64_t)idx, 0),
+                        (T)uniform24(c.seed, (uint64_t)frame, (uint64_t)idx, 1),
+                        (T)uniform24(c.seed, (uint64_t)frame, (uint64_t)idx, 2));
+          T rn = (T)uniform24(c.seed, (uint64_t)frame, (uint64_t)idx, 3);
+          T r = h * T(0.25f);
+          if (gasc > 0) {
+            V3<T> cen = mk(gpsum.x / (T)gasc, gpsum.y / (T)gasc, gpsum.z / (T)gasc);
+            npos[idx] = cen + (rp - mk(T(0.5f), T(0.5f), T(0.5f))) * (tmax(T(0.5f), rn) * r) * T(2);
+          } else {
+            npos[idx] = mk((T)i * h, (T)j * h, (T)k * h) + gmin() + rp * h;
+          }
+        }
+      }
+      int64_t na = 0;
+      for (int64_t idx = 0; idx < G; ++idx) na += add[idx];
+      if (P + na > c.particle_capacity) return MPF_CAPACITY;
+      int64_t slot = P;
+      pos.resize(3 * (P + na)); vel.resize(3 * (P + na)); flag.resize(P + na); mass.resize(P + na);
+      temp.resize(P + na); lheat.resize(P + na); solub.resize(P + na); gas.resize(P + na);
+      for (int64_t idx = 0; idx < G; ++idx) {
+        if (!add[idx]) continue;
+        S3(pos, slot, npos[idx]); S3(vel, slot, mk(T(0), T(0), T(0)));
+        mass[slot] = m0; flag[slot] = AIRSOLO; temp[slot] = tp[idx]; lheat[slot] = 0;
+        gas[slot] = 0; solub[slot] = 0;
+        for (uint32_t p = cstart[idx]; p < cend[idx]; ++p)
+          if (flag[p] == FLUID) { gas[p] = tmin(gas[p], solub[p]); lheat[p] = tmin(lheat[p], th); }
+        ++slot;
+      }
+      P += na;
+      added = (int)na;
+    }
+    const int64_t Pn = P - added;  // particles covered by the cell lists
+    if (c.melt_enabled) {
+      const T th = (T)c.liquid_heat_th;
+      for (int64_t p = 0; p < P; ++p)
+        if (flag[p] == SOLID && lheat[p] > th) { flag[p] = FLUID; lheat[p] = th; ++melted; }
+    }
+    if (c.freeze_enabled) {
+      std::vector<int8_t> f0(flag.begin(), flag.begin() + Pn);
+      std::vector<T> p0(pos.begin(), pos.begin() + 3 * Pn);
+      const T ss = (T)c.samplespace;
+      int cnt = 0;
+#pragma omp parallel for schedule(static) reduction(+ : cnt)
+      for (int64_t p = 0; p < Pn; ++p) {
+        if (f0[p] != FLUID || !(lheat[p] < 0)) continue;
+        int i, j, k;
+        V3<T> x = mk(p0[3 * p], p0[3 * p + 1], p0[3 * p + 2]);
+        ijk_of_pos(x, i, j, k);
+        T mind = T(1000);
+        int64_t mi = -1;
+        int cs = 0;
+        for (int di = -1; di <= 1; ++di)
+          for (int dj = -1; dj <= 1; ++dj)
+            for (int dk = -1; dk <= 1; ++dk) {
+              if (!valid(i + di, j + dj, k + dk)) continue;
+              int64_t g = cid(i + di, j + dj, k + dk);
+              if (cstart[g] == MPF_CELL_UNDEF) continue;
+              for (uint32_t q = cstart[g]; q < cend[g]; ++q) {
+                if (f0[q] != SOLID) continue;
+                V3<T> y = mk(p0[3 * q], p0[3 * q + 1], p0[3 * q + 2]);
+                T d = len3(y - x);
+                if (d < h) cs++;
+                if (d < mind) { mind = d; mi = q; }
+              }
+            }
+        if (mi != -1 && mind < h && cs > 2) {
+          flag[p] = SOLID;
+          lheat[p] = 0;
+          cnt++;
+          if (mind > ss) {
+            V3<T> y = mk(p0[3 * mi], p0[3 * mi + 1], p0[3 * mi + 2]);
+            V3<T> dd = x - y;
+            T l = len3(dd);
+            x = dd * (T(1) / l) * ss + y;
+            S3(pos, p, x);
+          }
+        }
+      }
+      frozen = cnt;
+    }
+    if (info) { info->melted = melted; info->frozen = frozen; info->gas_added = added; info->particles = P; }
+    return MPF_OK;
+  }
+
+  // ================= a18: marching cubes on the liquid level set =================
+  // classifyVoxel spray_k.cu:1853-1888 (case index), scans + compactVoxels
+  // 1891-1900, markActiveEdge_MC 2172-2220, generateTriangles_indices
+  // 2074-2170 (shared vertices indexed by the scan of active edges).  Nodes
+  // are cell centres of lsfluid; voxel (i,j,k) spans cells i..i+1 etc.
+  int marching_cubes(mpf_mc_info* info) {
+    const int VX = NX - 1, VY = NY - 1, VZ = NZ - 1;
+    const int64_t V = (int64_t)VX * VY * VZ;
+    mc_case.assign(V > 0 ? V : 0, 0);
+    const T iso = 0;
+    auto F = [&](int i, int j, int k) { return lsfluid[cid(i, j, k)]; };
+    int64_t active = 0, nverts_tri = 0;
+    for (int64_t v = 0; v < V; ++v) {
+      int i, j, k;
+      ijk3(v, VX, VY, VZ, i, j, k);
+      T f[8] = {F(i, j, k), F(i + 1, j, k), F(i + 1, j + 1, k), F(i, j + 1, k),
+                F(i, j, k + 1), F(i + 1, j, k + 1), F(i + 1, j + 1, k + 1), F(i, j + 1, k + 1)};
+      uint32_t ci = 0;
+      for (int q = 0; q < 8; ++q) ci += (uint32_t)(f[q] < iso) << q;
+      mc_case[v] = (uint8_t)ci;
+      if (MC_NUM_VERTS[ci] > 0) { active++; nverts_tri += MC_NUM_VERTS[ci]; }
+    }
+    // Edge ids: node n = cid(i,j,k); 3*n + axis for the edge from node n to n+e_axis.
+    std::vector<uint8_t> used((size_t)3 * G, 0);
+    const int eo[12][4] = {  // (di,dj,dk,axis) of the cube edges
+        {0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
+        {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
+    for (int64_t v = 0; v < V; ++v) {
+      uint32_t ci = mc_case[v];
+      if (!MC_NUM_VERTS[ci]) continue;
+      int i, j, k;
+      ijk3(v, VX, VY, VZ, i, j, k);
+      for (int t = 0; t < MC_NUM_VERTS[ci]; ++t) {
+        int e = MC_TRI_TABLE[ci][t];
+        used[3 * cid(i + eo[e][0], j + eo[e][1], k + eo[e][2]) + eo[e][3]] = 1;
+      }
+    }
+    std::vector<uint32_t> vid((size_t)3 * G, 0);
+    uint32_t nv = 0;
+    for (size_t e = 0; e < used.size(); ++e) { vid[e] = nv; nv += used[e]; }
+    if ((int64_t)nv > c.mc_max_verts || nverts_tri / 3 > c.mc_max_tris) return MPF_CAPACITY;
+    mc_verts.assign((size_t)3 * nv, T(0));
+    mc_tris.assign((size_t)nverts_tri, 0);
+    // vertex positions: vertexInterp spray_k.cu:1905-1910 along each active edge
+    for (int64_t n = 0; n < G; ++n)
+      for (int a = 0; a < 3; ++a) {
+        if (!used[3 * n + a]) continue;
+        int i, j, k;
+        ijk(n, i, j, k);
+        int i2 = i + (a == 0), j2 = j + (a == 1), k2 = k + (a == 2);
+        T f0 = F(i, j, k), f1 = F(i2, j2, k2);
+        T t = (iso - f0) / (f1 - f0);
+        V3<T> p0 = mk(((T)i + T(0.5)) * h, ((T)j + T(0.5)) * h, ((T)k + T(0.5)) * h) + gmin();
+        V3<T> p1 = mk(((T)i2 + T(0.5)) * h, ((T)j2 + T(0.5)) * h, ((T)k2 + T(0.5)) * h) + gmin();
+        V3<T> pv = p0 + (p1 - p0) * t;
+        S3(mc_verts, vid[3 * n + a], pv);
+      }
+    int64_t w = 0;
+    for (int64_t v = 0; v < V; ++v) {
+      uint32_t ci = mc_case[v];
+      if (!MC_NUM_VERTS[ci]) continue;
+      int i, j, k;
+      ijk3(v, VX, VY, VZ, i, j, k);
+      for (int t = 0; t < MC_NUM_VERTS[ci]; ++t) {
+        int e = MC_TRI_TABLE[ci][t];
+        mc_tris[w++] = vid[3 * cid(i + eo[e][0], j + eo[e][1], k + eo[e][2]) + eo[e][3]];
+      }
+    }
+    if (info) { info->active_voxels = active; info->verts = nv; info->tris = nverts_tri / 3; }
+    return MPF_OK;
+  }
+
+  // ================= composite step (spray.cpp:979-1233, north-star subset) =================
+  int step(int frame, mpf_step_info* info) {
+    mpf_step_info loc;
+    std::memset(&loc, 0, sizeof(loc));
+    hash_sort();
+    apply_forces();
+    levelset_update();
+    p2g();
+    extrapolate_bc(true);
+    int st = project(&loc.project);
+    extrapolate_bc(false);
+    g2p();
+    advect();
+    hash_sort();
+    levelset_update();
+    if (c.heat_enabled) heat(&loc.heat);
+    int st2 = phase_change(frame, &loc.phase);
+    if (st2 != MPF_OK) return st2;
+    if (c.mc_enabled) {
+      int st3 = marching_cubes(&loc.mc);
+      if (st3 != MPF_OK) return st3;
+    }
+    loc.particles = P;
+    if (info) *info = loc;
+    return st == MPF_NO_CONVERGENCE ? MPF_NO_CONVERGENCE : MPF_OK;
+  }
+
+  // ---- field access ----
+  std::vector<T>* real_field(int f) {
+    switch (f) {
+      case MPF_F_LSFLUID: return &lsfluid; case MPF_F_LSAIR: return &lsair; case MPF_F_LSMERGE: return &lsmerge;
+      case MPF_F_PHIGRAD_X: return &phigx; case MPF_F_PHIGRAD_Y: return &phigy; case MPF_F_PHIGRAD_Z: return &phigz;
+      case MPF_F_PHIFLUID: return &phifluid; case MPF_F_PHIAIR: return &phiair; case MPF_F_PHISOLID: return &phisolid;
+      case MPF_F_DIV: return &div; case MPF_F_PRESSURE: return &press;
+      case MPF_F_TEMPERATURE: return &tp; case MPF_F_TEMPERATURE_CHANGE: return &tpchange;
+      case MPF_F_WATER_UX: return &wu[0]; case MPF_F_WATER_UY: return &wu[1]; case MPF_F_WATER_UZ: return &wu[2];
+      case MPF_F_AIR_UX: return &au[0]; case MPF_F_AIR_UY: return &au[1]; case MPF_F_AIR_UZ: return &au[2];
+      case MPF_F_WATER_UX_OLD: return &wuo[0]; case MPF_F_WATER_UY_OLD: return &wuo[1]; case MPF_F_WATER_UZ_OLD: return &wuo[2];
+      case MPF_F_AIR_UX_OLD: return &auo[0]; case MPF_F_AIR_UY_OLD: return &auo[1]; case MPF_F_AIR_UZ_OLD: return &auo[2];
+      case MPF_F_SOLID_UX: return &su[0]; case MPF_F_SOLID_UY: return &su[1]; case MPF_F_SOLID_UZ: return &su[2];
+      default: return nullptr;
+    }
+  }
+  std::vector<uint32_t>* u32_field(int f) {
+    switch (f) {
+      case MPF_F_CELL_START: return &cstart; case MPF_F_CELL_END: return &cend;
+      case MPF_F_GRID_HASH: return &ghash; case MPF_F_GRID_INDEX: return &gindex;
+      default: return nullptr;
+    }
+  }
+  int64_t field_count(int f) {
+    if (f == MPF_F_MARK) return G;
+    if (auto* r = real_field(f)) return (int64_t)r->size();
+    if (f == MPF_F_GRID_HASH || f == MPF_F_GRID_INDEX) return P;
+    if (auto* u = u32_field(f)) return (int64_t)u->size();
+    return -1;
+  }
+  int get_field(int f, void* dst) {
+    if (f == MPF_F_MARK) { std::memcpy(dst, mark.data(), G); return 0; }
+    if (auto* r = real_field(f)) {
+      double* d = (double*)dst;
+      for (size_t i = 0; i < r->size(); ++i) d[i] = (double)(*r)[i];
+      return 0;
+    }
+    if (auto* u = u32_field(f)) {
+      if (f == MPF_F_GRID_HASH || f == MPF_F_GRID_INDEX) {
+        if ((int64_t)u->size() != P) return MPF_INVALID_ARG;
+      }
+      std::memcpy(dst, u->data(), u->size() * 4);
+      return 0;
+    }
+    return MPF_INVALID_ARG;
+  }
+  int set_field(int f, const void* src) {
+    if (f == MPF_F_MARK) { std::memcpy(mark.data(), src, G); return 0; }
+    if (auto* r = real_field(f)) {
+      const double* d = (const double*)src;
+      for (size_t i = 0; i < r->size(); ++i) (*r)[i] = (T)d[i];
+      return 0;
+    }
+    if (f == MPF_F_CELL_START || f == MPF_F_CELL_END) {
+      std::memcpy(u32_field(f)->data(), src, (size_t)G * 4);
+      return 0;
+    }
+    return MPF_INVALID_ARG;
+  }
+};
+
+}  // namespace
+
+struct orc_state {
+  OrcBase* impl;
+  int dbl;
+};
+
+template <class F> static int dispatch(orc_state* s, F f) {
+  if (s->dbl) return f(*static_cast<Orc<double>*>(s->impl));
+  return f(*static_cast<Orc<float>*>(s->impl));
+}
+
+extern "C" {
+
+orc_state* orc_create(const mpf_config* cfg, int use_double) {
+  if (!cfg || cfg->nx < 2 || cfg->ny < 2 || cfg->nz < 2) return nullptr;
+  orc_state* s = new orc_state;
+  s->dbl = use_double ? 1 : 0;
+  if (s->dbl) s->impl = new Orc<double>(*cfg);
+  else s->impl = new Orc<float>(*cfg);
+  return s;
+}
+void orc_destroy(orc_state* s) {
+  if (!s) return;
+  delete s->impl;
+  delete s;
+}
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+int orc_set_particles(orc_state* s, int64_t n, const float* pos, const float* vel, const int8_t* flag,
+                      const float* mass, const float* T, const float* lheat, const float* solub,
+                      const float* gas) {
+  return dispatch(s, [&](auto& o) {
+    using R = typename std::remove_reference<decltype(o.pos)>::type::value_type;
+    if (n < 0 || n > o.c.particle_capacity) return (int)MPF_CAPACITY;
+    o.P = n;
+    auto fill = [&](std::vector<R>& v, const float* src, int w) {
+      v.assign((size_t)w * n, R(0));
+      if (src) for (int64_t i = 0; i < w * n; ++i) v[i] = (R)src[i];
+    };
+    fill(o.pos, pos, 3); fill(o.vel, vel, 3); fill(o.mass, mass, 1); fill(o.temp, T, 1);
+    fill(o.lheat, lheat, 1); fill(o.solub, solub, 1); fill(o.gas, gas, 1);
+    o.flag.assign(n, MPF_TYPEFLUID);
+    if (flag) std::memcpy(o.flag.data(), flag, n);
+    return (int)MPF_OK;
+  });
+}
+int orc_get_particles(orc_state* s, int64_t* n, double* pos, double* vel, int8_t* flag, double* mass,
+                      double* T, double* lheat, double* solub, double* gas) {
+  return dispatch(s, [&](auto& o) {
+    if (n) *n = o.P;
+    auto out = [&](const auto& v, double* dst) {
+      if (dst) for (size_t i = 0; i < v.size(); ++i) dst[i] = (double)v[i];
+    };
+    out(o.pos, pos); out(o.vel, vel); out(o.mass, mass); out(o.temp, T); out(o.lheat, lheat);
+    out(o.solub, solub); out(o.gas, gas);
+    if (flag) std::memcpy(flag, o.flag.data(), o.flag.size());
+    return (int)MPF_OK;
+  });
+}
+int64_t orc_particle_count(orc_state* s) {
+  return s->dbl ? static_cast<Orc<double>*>(s->impl)->P : static_cast<Orc<float>*>(s->impl)->P;
+}
+int64_t orc_field_count(orc_state* s, int f) {
+  return s->dbl ? static_cast<Orc<double>*>(s->impl)->field_count(f) : static_cast<Orc<float>*>(s->impl)->field_count(f);
+}
+int orc_get_field(orc_state* s, int f, void* dst) { return dispatch(s, [&](auto& o) { return o.get_field(f, dst); }); }
+int orc_set_field(orc_state* s, int f, const void* src) { return dispatch(s, [&](auto& o) { return o.set_field(f, src); }); }
+int orc_set_seed_cells(orc_state* s, const int32_t* cells, int32_t n) {
+  return dispatch(s, [&](auto& o) { o.seeds.assign(cells, cells + n); return 0; });
+}
+int orc_hash_sort(orc_state* s) { return dispatch(s, [](auto& o) { o.hash_sort(); return 0; }); }
+int orc_apply_forces(orc_state* s) { return dispatch(s, [](auto& o) { o.apply_forces(); return 0; }); }
+int orc_levelset_update(orc_state* s) { return dispatch(s, [](auto& o) { o.levelset_update(); return 0; }); }
+int orc_p2g(orc_state* s) { return dispatch(s, [](auto& o) { o.p2g(); return 0; }); }
+int orc_extrapolate_bc(orc_state* s, int rp) { return dispatch(s, [&](auto& o) { o.extrapolate_bc(rp != 0); return 0; }); }
+int orc_project(orc_state* s, mpf_solve_info* info) { return dispatch(s, [&](auto& o) { return o.project(info); }); }
+int orc_g2p(orc_state* s) { return dispatch(s, [](auto& o) { o.g2p(); return 0; }); }
+int orc_advect(orc_state* s) { return dispatch(s, [](auto& o) { o.advect(); return 0; }); }
+int orc_heat(orc_state* s, mpf_solve_info* info) { return dispatch(s, [&](auto& o) { return o.heat(info); }); }
+int orc_phase_change(orc_state* s, int frame, mpf_phase_info* info) {
+  return dispatch(s, [&](auto& o) { return o.phase_change(frame, info); });
+}
+int orc_marching_cubes(orc_state* s, mpf_mc_info* info) { return dispatch(s, [&](auto& o) { return o.marching_cubes(info); }); }
+int orc_get_mesh(orc_state* s, double* verts, uint32_t* tris) {
+  return dispatch(s, [&](auto& o) {
+    for (size_t i = 0; i < o.mc_verts.size(); ++i) verts[i] = (double)o.mc_verts[i];
+    std::memcpy(tris, o.mc_tris.data(), o.mc_tris.size() * 4);
+    return 0;
+  });
+}
+int orc_get_mc_cases(orc_state* s, uint8_t* cases) {
+  return dispatch(s, [&](auto& o) { std::memcpy(cases, o.mc_case.data(), o.mc_case.size()); return 0; });
+}
+int orc_step(orc_state* s, int frame, mpf_step_info* info) { return dispatch(s, [&](auto& o) { return o.step(frame, info); }); }
+
+uint32_t orc_hash_key(const mpf_config* cfg, float x, float y, float z) {
+  Orc<float> o(*cfg);
+  int i, j, k;
+  o.ijk_of_pos(mk(x, y, z), i, j, k);
+  return (uint32_t)o.cid(i, j, k);
+}
+void orc_stable_sort_pairs(int64_t n, uint32_t* keys, uint32_t* vals) {
+  std::vector<std::pair<uint32_t, uint32_t>> v(n);
+  for (int64_t i = 0; i < n; ++i) v[i] = {keys[i], vals[i]};
+  std::stable_sort(v.begin(), v.end(), [](auto& a, auto& b) { return a.first < b.first; });
+  for (int64_t i = 0; i < n; ++i) { keys[i] = v[i].first; vals[i] = v[i].second; }
+}
+int orc_apply_operator(orc_state* s, const double* x, double* y) {
+  return dispatch(s, [&](auto& o) {
+    std::vector<double> xv(x, x + o.G), yv(o.G);
+    o.apply_A(xv, yv);
+    std::memcpy(y, yv.data(), o.G * sizeof(double));
+    return 0;
+  });
+}
+int orc_apply_heat_operator(orc_state* s, const double* x, double* y) {
+  return dispatch(s, [&](auto& o) {
+    std::vector<double> xv(x, x + o.G), yv(o.G);
+    o.apply_heat_A(xv, yv);
+    std::memcpy(y, yv.data(), o.G * sizeof(double));
+    return 0;
+  });
+}
+double orc_uniform(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) { return (double)uniform24(seed, a, b, c); }
+int orc_mc_tables(int32_t* edge_table, int32_t* tri_table, int32_t* num_verts) {
+  for (int c = 0; c < 256; ++c) {
+    edge_table[c] = MC_EDGE_TABLE[c];
+    num_verts[c] = MC_NUM_VERTS[c];
+    for (int q = 0; q < 16; ++q) tri_table[16 * c + q] = MC_TRI_TABLE[c][q];
+  }
+  return 0;
+}
+
+}  // extern "C"

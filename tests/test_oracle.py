@@ -1,0 +1,588 @@
+"""Pins of the CPU oracle (no GPU).
+
+The reference has no tests, fixtures or golden vectors for this path, and its
+implementation cannot be built here (Windows + CUDA 7.5), so every oracle
+function is pinned by closed forms, brute force on tiny inputs, invariants and
+special cases chosen so that a dropped term, a wrong sign/index or a
+transposed operand fails one of them.  Each test names the reference lines it
+pins.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import load_scene, rel_l2
+
+H = 1.0 / 64
+
+
+def small_cfg(pkg, n=8, **kw):
+    kw.setdefault("particle_capacity", 200000)
+    kw.setdefault("mc_enabled", 0)
+    return pkg.default_config(n, n, n, **kw)
+
+
+# ---------------------------------------------------------------- a1 hash
+def test_hash_key_known_answers(pkg, orc):
+    """getijkfrompos/getidx (spray_k.cu:51-69): truncation of (p-gmin)/dx per
+    component, any component outside [0,N) maps to 0, key = i*N*N + j*N + k."""
+    cfg = small_cfg(pkg, 8)
+    f = np.float32
+    kat = [
+        ((0, 0, 0), 0),
+        ((H, 0, 0), 64),                     # exactly on the face x = 1*dx -> i = 1
+        ((f(np.nextafter(f(H), f(0))), 0, 0), 0),  # just below the face -> i = 0
+        ((2 * H, 3 * H, 5 * H), 2 * 64 + 3 * 8 + 5),
+        ((8 * H, 1 * H, 1 * H), 0 * 64 + 8 + 1),   # x == N*dx is out of range -> i = 0
+        ((-1e-7, 7.5 * H, 7.999 * H), 0 + 7 * 8 + 7),
+        ((1e9, -1e9, np.nan), 0),
+        ((7.5 * H, 7.5 * H, 7.5 * H), 7 * 64 + 7 * 8 + 7),
+    ]
+    for p, key in kat:
+        assert orc.hash_key(cfg, *[f(v) for v in p]) == key, p
+
+
+# ---------------------------------------------------------------- a2-a3 sort, cell ranges
+def test_stable_counting_sort_matches_std_stable_sort(orc):
+    """thrust::sort_by_key on uint keys (spray_thrust.cu:12-17) is a stable
+    radix sort: result ordered by (key, index).  The oracle's counting sort
+    must equal std::stable_sort and numpy's stable argsort."""
+    rng = np.random.default_rng(1)
+    keys = rng.integers(0, 50, 5000).astype(np.uint32)
+    vals = np.arange(5000, dtype=np.uint32)
+    k1, v1 = orc.stable_sort_pairs(keys, vals)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(k1, keys[order]) and np.array_equal(v1, vals[order])
+
+
+def test_hash_sort_state(pkg, orc):
+    """reorderDataAndFindCellStartD (spray_k.cu:1052-1129): permutation sorted
+    by (key, index), cell ranges partition [0,P), attributes permuted."""
+    cfg = small_cfg(pkg, 8)
+    rng = np.random.default_rng(2)
+    P = 3000
+    pos = (rng.random((P, 3)) * 8 * H).astype(np.float32)
+    pos[::7] = pos[::7].round(decimals=0)  # many duplicates at the origin cell corner
+    o = orc.Oracle(cfg)
+    mass = rng.random(P).astype(np.float32)
+    o.set_particles(P, pos=pos, mass=mass)
+    o.hash_sort()
+    h, idx = o.get_field("GRID_HASH"), o.get_field("GRID_INDEX")
+    keys = np.array([orc.hash_key(cfg, *p) for p in pos], np.uint32)
+    order = np.lexsort((np.arange(P), keys))
+    assert np.array_equal(idx, order) and np.array_equal(h, keys[order])
+    cs, ce = o.get_field("CELL_START"), o.get_field("CELL_END")
+    ne = cs != pkg.CELL_UNDEF
+    counts = np.bincount(keys, minlength=512)
+    assert np.array_equal(ne, counts > 0)
+    assert np.array_equal((ce[ne] - cs[ne]), counts[ne])
+    assert np.array_equal(cs[ne], np.cumsum(counts)[ne] - counts[ne])
+    assert np.array_equal(o.get_particles()["mass"].astype(np.float32), mass[order])
+
+
+# ---------------------------------------------------------------- a5 forces
+def test_forces_closed_form(pkg, orc):
+    """addgravityforce_k (spray_k.cu:330-338): v += dt*g for FLUID and SOLID only;
+    addbuoyancyforce_vel (356-371) in reference mode."""
+    cfg = small_cfg(pkg, 8, dt=0.01, buoyancy_kick=1)
+    P = 6
+    flag = np.arange(6, dtype=np.int8)
+    vel = np.zeros((P, 3), np.float32)
+    o = orc.Oracle(cfg, use_double=True)
+    o.set_particles(P, pos=np.full((P, 3), 0.05, np.float32), vel=vel, flag=flag)
+    o.apply_forces()
+    v = o.get_particles()["vel"]
+    g = -9.8 * 0.01
+    rate_fluid = 1.0
+    assert v[0, 2] == pytest.approx(g + (0.0), rel=1e-6)              # FLUID: gravity only
+    assert v[1, 2] == pytest.approx(-0.01 * -9.8 * rate_fluid * 1.25, rel=1e-5)  # AIR kick, rate from v=0
+    assert v[5, 2] == pytest.approx(g + 0.01 * -9.8 * 0.1 * 1.0, rel=1e-5)  # SOLID: gravity then 0.1 g kick
+    assert np.all(v[[2, 3], :] == 0)
+
+
+# ---------------------------------------------------------------- a6-a7 level set
+def _ls_scene(pkg, orc, n=10, P=400, seed=3, double=True):
+    cfg = small_cfg(pkg, n)
+    rng = np.random.default_rng(seed)
+    pos = (rng.random((P, 3)) * (n - 2) * H + H).astype(np.float32)
+    flag = rng.choice(np.array([0, 1, 4, 5], np.int8), P)
+    o = orc.Oracle(cfg, use_double=double)
+    o.set_particles(P, pos=pos, flag=flag)
+    o.hash_sort()
+    o.levelset_update()
+    return cfg, o, pos, flag
+
+
+def test_levelset_generation_bruteforce(pkg, orc):
+    """genlevelset/mindis_cell (spray_k.cu:2753-2806): min distance over a
+    5x5x5-cell window, initial 2.5h, minus 0.5h.  Any particle within 2.5h of a
+    cell centre lies in that window, so the windowed value equals the global
+    brute-force minimum capped at 2.5h.  Cells with 0 <= ls <= 1.99h are not
+    changed by the redistancing that follows, so they are compared exactly."""
+    n = 10
+    cfg, o, pos, flag = _ls_scene(pkg, orc, n)
+    c = (np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3) + 0.5) * H
+    d = np.sqrt(((c[:, None, :] - pos[None, :, :].astype(np.float64)) ** 2).sum(-1))
+    air = np.isin(flag, [1, 4])
+    fl = np.isin(flag, [0, 5])
+    for name, sel in (("LSAIR", air), ("LSFLUID", fl)):
+        bf = np.minimum(d[:, sel].min(axis=1), 2.5 * H) - 0.5 * H
+        got = o.get_field(name)
+        keep = (bf >= 0) & (bf <= 1.99 * H)
+        assert keep.sum() > 50
+        assert np.allclose(got[keep], bf[keep], rtol=0, atol=1e-12), name
+
+
+def test_eikonal_sweep_closed_form(pkg, orc):
+    """solvedistance (spray_k.cu:746-754) through sweepPhi (spray.cu:570-582):
+    a planar FLUID slab gives phi = -0.5 + m at the m-th layer above it after
+    3 Jacobi sweeps (m <= 3) and stays 3N beyond; the 3-D update of three
+    equal neighbours a gives a + 1/sqrt(3)."""
+    n = 10
+    cfg = small_cfg(pkg, n)
+    o = orc.Oracle(cfg, use_double=True)
+    mark = np.full((n, n, n), 3, np.int8)
+    mark[:, :, :4] = 0
+    o.set_field("MARK", mark)
+    zeros = np.zeros((n + 1) * n * n)
+    for f in ("WATER_UX", "AIR_UX"):
+        o.set_field(f, zeros)
+    o.extrapolate_bc(True)
+    phi = o.get_field("PHIFLUID").reshape(n, n, n)
+    assert np.all(phi[:, :, :4] == -0.5)
+    for m, kk in enumerate(range(4, 7)):
+        assert np.allclose(phi[:, :, kk], -0.5 + (m + 1)), kk
+    assert np.all(phi[:, :, 7:] == 3 * n)
+    # three equal upwind neighbours a: discrete eikonal 3 (phi - a)^2 = 1
+    mark = np.full((n, n, n), 3, np.int8)
+    mark[:5, :, :] = 0
+    mark[:, :5, :] = 0
+    mark[:, :, :5] = 0
+    o.set_field("MARK", mark)
+    o.extrapolate_bc(True)
+    phi = o.get_field("PHIFLUID").reshape(n, n, n)
+    assert phi[5, 5, 5] == pytest.approx(-0.5 + 1 / np.sqrt(3), abs=1e-14)
+    assert phi[5, 7, 7] == pytest.approx(0.5, abs=1e-14)  # one known neighbour: a + 1
+
+
+def test_sweeps_never_increase(pkg, orc):
+    """Invariant of the redistancing sweeps: the redistanced level set never
+    exceeds the generated value where the generated value is below 1.99h, and
+    stays within [-6h, 5h] (cell units clamp of markLS/preparels, spray_k.cu:3683-3722)."""
+    cfg, o, pos, flag = _ls_scene(pkg, orc, 12, 900)
+    for name in ("LSFLUID", "LSAIR"):
+        ls = o.get_field(name)
+        assert np.all(ls <= 5 * H + 1e-15) and np.all(ls >= -6 * H)
+
+
+def test_merge_flags_rules(pkg, orc):
+    """mergeLSAndMarkGrid (spray_k.cu:3733-3777) thresholds 4.99h and 0.8h, outer
+    layer BOUNDARY with lsmerge = -0.5h."""
+    cfg, o, pos, flag = _ls_scene(pkg, orc, 10, 400)
+    lf, la, lm = o.get_field("LSFLUID"), o.get_field("LSAIR"), o.get_field("LSMERGE")
+    mk = o.get_field("MARK")
+    n = 10
+    I, J, K = np.meshgrid(*[np.arange(n)] * 3, indexing="ij")
+    border = ((I == 0) | (I == n - 1) | (J == 0) | (J == n - 1) | (K == 0) | (K == n - 1)).ravel()
+    assert np.all(mk[border] == 2) and np.allclose(lm[border], -0.5 * H)
+    inner = ~border
+    for c in np.flatnonzero(inner):
+        if la[c] > 4.99 * H:
+            exp = (3 if lf[c] > 0 else 0, lf[c])
+        elif lf[c] > 4.99 * H:
+            exp = (3 if la[c] > 0 else 1, la[c])
+        elif la[c] > 0.8 * H and lf[c] > 0.8 * H:
+            exp = (3, min(lf[c], la[c]))
+        else:
+            m = (lf[c] - la[c]) * 0.5
+            exp = (1 if m > 0 else 0, m)
+        assert mk[c] == exp[0] and lm[c] == pytest.approx(exp[1], abs=1e-15)
+
+
+# ---------------------------------------------------------------- a9 P2G
+def test_p2g_particle_centric_bruteforce(pkg, orc):
+    """mapvelp2g_k_fluidSolid / _air (spray_k.cu:2809-2937) restated the other
+    way round: each particle scatters m*sharp(r^2) to the faces whose window
+    contains its cell (liquid+solid: normal offset {0,1}, transverse {-1,0,1};
+    air: normal {-1..2}, transverse {-2..2}); u = sum(w v)/sum(w).  The kernel
+    constants are the reference's float literals (RE = 1.4f, 0.0001f), taken
+    exactly in double."""
+    RE2 = float(np.float32(1.4)) ** 2
+    R2MIN = float(np.float32(0.0001))
+    n = 8
+    cfg = small_cfg(pkg, n)
+    rng = np.random.default_rng(4)
+    P = 700
+    pos = (rng.random((P, 3)) * (n - 2) * H + H).astype(np.float32)
+    vel = rng.standard_normal((P, 3)).astype(np.float32)
+    mass = (rng.random(P) + 0.5).astype(np.float32)
+    flag = rng.choice(np.array([0, 1, 4, 5], np.int8), P)
+    o = orc.Oracle(cfg, use_double=True)
+    o.set_particles(P, pos=pos, vel=vel, flag=flag, mass=mass)
+    o.hash_sort()
+    o.p2g()
+    pp = o.get_particles()
+    x = pp["pos"] / H
+    cell = np.floor(x).astype(int)
+    dims = {0: (n + 1, n, n), 1: (n, n + 1, n), 2: (n, n, n + 1)}
+    for phase, sel, nrm, tr, names in ((0, np.isin(pp["flag"], [0, 5]), (0, 1), (-1, 0, 1), ("WATER_UX", "WATER_UY", "WATER_UZ")),
+                                       (1, pp["flag"] == 1, (-1, 0, 1, 2), (-2, -1, 0, 1, 2), ("AIR_UX", "AIR_UY", "AIR_UZ"))):
+        for a in range(3):
+            W = np.zeros(dims[a])
+            U = np.zeros(dims[a])
+            for p in np.flatnonzero(sel):
+                offs = [nrm if q == a else tr for q in range(3)]
+                for di in offs[0]:
+                    for dj in offs[1]:
+                        for dk in offs[2]:
+                            f = (cell[p, 0] + di, cell[p, 1] + dj, cell[p, 2] + dk)
+                            if any(f[q] < 0 or f[q] >= dims[a][q] for q in range(3)):
+                                continue
+                            fc = np.array(f, float) + np.where(np.arange(3) == a, 0.0, 0.5)
+                            r2 = ((x[p] - fc) ** 2).sum()
+                            w = pp["mass"][p] * max(RE2 / max(r2, R2MIN) - 1.0, 0.0)
+                            W[f] += w
+                            U[f] += w * pp["vel"][p, a]
+            exp = np.where(W > 0, U / np.where(W > 0, W, 1), 0.0).ravel()
+            got = o.get_field(names[a])
+            assert np.allclose(got, exp, rtol=1e-6, atol=1e-9), names[a]
+            assert np.array_equal(got, o.get_field(names[a] + "_OLD"))
+
+
+def test_p2g_partition_of_unity(pkg, orc):
+    """A uniform particle velocity is reproduced on every face with weight."""
+    n = 8
+    cfg = small_cfg(pkg, n)
+    rng = np.random.default_rng(5)
+    P = 2000
+    pos = (rng.random((P, 3)) * (n - 2) * H + H).astype(np.float32)
+    vel = np.tile(np.array([[0.25, -0.5, 0.125]], np.float32), (P, 1))
+    o = orc.Oracle(cfg, use_double=True)
+    o.set_particles(P, pos=pos, vel=vel, flag=np.zeros(P, np.int8), mass=np.ones(P, np.float32))
+    o.hash_sort()
+    o.p2g()
+    for a, f in enumerate(("WATER_UX", "WATER_UY", "WATER_UZ")):
+        u = o.get_field(f)
+        nz = u != 0
+        assert nz.sum() > 100
+        assert np.allclose(u[nz], vel[0, a], rtol=1e-12)
+
+
+# ---------------------------------------------------------------- a11-a13 projection
+def _proj_oracle(pkg, orc, n=8, seed=6, op=1, rho=(1000.0, 1.0), solid=False, solver=1, tol=1e-12):
+    cfg = small_cfg(pkg, n, op=op, rho_liquid=rho[0], rho_air=rho[1], solver=solver, tol=tol, max_iters=20000,
+                    solid_constraint=1 if solid else 0)
+    o = orc.Oracle(cfg, use_double=True)
+    rng = np.random.default_rng(seed)
+    mark = rng.choice(np.array([0, 0, 0, 1, 1, 3], np.int8), (n, n, n))
+    if solid:
+        mark[3:5, 3:5, 3:5] = 5
+    mark[[0, -1], :, :] = 2
+    mark[:, [0, -1], :] = 2
+    mark[:, :, [0, -1]] = 2
+    ls = rng.standard_normal(n ** 3) * H
+    ls[mark.ravel() == 0] = -np.abs(ls[mark.ravel() == 0])
+    ls[mark.ravel() == 1] = np.abs(ls[mark.ravel() == 1]) + 1e-3 * H
+    o.set_field("MARK", mark)
+    o.set_field("LSMERGE", ls)
+    return cfg, o, mark, rng
+
+
+def test_operator_symmetric_negative_definite(pkg, orc):
+    """computeAx_bubble (spray_k.cu:3423-3447) and the 1/rho form: x.Ay = y.Ax,
+    x.Ax < 0 on unknowns, zero rows/columns elsewhere; with rho_l = rho_a = 1
+    the 1/rho form equals the unit form."""
+    for op, solid in ((0, False), (1, False), (1, True)):
+        cfg, o, mark, rng = _proj_oracle(pkg, orc, op=op, solid=solid)
+        G = cfg.nx ** 3
+        unk = np.isin(mark.ravel(), [0, 1])
+        x = rng.standard_normal(G) * unk
+        y = rng.standard_normal(G) * unk
+        Ax, Ay = o.apply_operator(x), o.apply_operator(y)
+        assert x @ Ay == pytest.approx(y @ Ax, rel=1e-12)
+        assert x @ Ax < 0
+        assert np.all(Ax[~unk] == 0)
+    cfg, o1, mark, rng = _proj_oracle(pkg, orc, op=1, rho=(1.0, 1.0))
+    _, o0, _, _ = _proj_oracle(pkg, orc, op=0)
+    x = rng.standard_normal(cfg.nx ** 3) * np.isin(mark.ravel(), [0, 1])
+    assert np.allclose(o1.apply_operator(x), o0.apply_operator(x), rtol=1e-14, atol=1e-9)
+
+
+def test_projection_matches_dense_solve_and_removes_divergence(pkg, orc):
+    """The PCG pressure equals a dense FP64 solve of the assembled operator, and
+    after subtracting the gradient a single-phase velocity field is discrete
+    divergence free (cptdivergence_bubble2 spray_k.cu:3138-3263, subGradPress
+    198-224)."""
+    n = 8
+    cfg = small_cfg(pkg, n, op=1, tol=1e-13, max_iters=20000)
+    o = orc.Oracle(cfg, use_double=True)
+    mark = np.full((n, n, n), 0, np.int8)
+    mark[:, :, 6] = 3  # free surface (Dirichlet) near the top
+    mark[[0, -1], :, :] = 2
+    mark[:, [0, -1], :] = 2
+    mark[:, :, [0, -1]] = 2
+    o.set_field("MARK", mark)
+    o.set_field("LSMERGE", np.full(n ** 3, -H))
+    rng = np.random.default_rng(8)
+    for f, sz in (("WATER_UX", (n + 1) * n * n), ("WATER_UY", (n + 1) * n * n), ("WATER_UZ", (n + 1) * n * n)):
+        o.set_field(f, rng.standard_normal(sz))
+    o.extrapolate_bc(True)  # wall faces to zero (setWaterBoundaryU_k)
+    info = o.project()
+    assert info["converged"] == 1 and info["rel_residual"] < 1e-12
+    p = o.get_field("PRESSURE")
+    div = o.get_field("DIV")
+    unk = mark.ravel() == 0
+    G = n ** 3
+    idx = np.flatnonzero(unk)
+    A = np.zeros((idx.size, idx.size))
+    for col, c in enumerate(idx):
+        e = np.zeros(G)
+        e[c] = 1.0
+        A[:, col] = o.apply_operator(e)[idx]
+    pd = np.linalg.solve(A, div[idx])
+    assert rel_l2(p[idx], pd) < 1e-10
+    o.extrapolate_bc(False)  # walls back to zero, as in the step after the solve
+    o.project()  # recompute the divergence of the projected field
+    d2 = o.get_field("DIV")
+    assert np.linalg.norm(d2[unk]) < 1e-10 * np.linalg.norm(div[unk])
+
+
+def test_divergence_of_linear_field(pkg, orc):
+    """cptdivergence_bubble2: u = (a x, b y, c z) gives div = a + b + c in every
+    FLUID cell (single phase, exact differences on the staggered faces)."""
+    n = 8
+    cfg = small_cfg(pkg, n, op=0, solver=0, ref_max_iters=1)
+    o = orc.Oracle(cfg, use_double=True)
+    mark = np.full((n, n, n), 0, np.int8)
+    mark[[0, -1], :, :] = 2
+    mark[:, [0, -1], :] = 2
+    mark[:, :, [0, -1]] = 2
+    o.set_field("MARK", mark)
+    a, b, c = 0.5, -2.0, 1.25
+    i = np.arange(n + 1) * H
+    j = np.arange(n) * H
+    ux = np.broadcast_to(a * i[:, None, None], (n + 1, n, n))
+    uy = np.broadcast_to(b * i[None, :, None], (n, n + 1, n))
+    uz = np.broadcast_to(c * i[None, None, :], (n, n, n + 1))
+    o.set_field("WATER_UX", ux)
+    o.set_field("WATER_UY", uy)
+    o.set_field("WATER_UZ", uz)
+    o.project(allow_no_convergence=True)
+    d = o.get_field("DIV").reshape(n, n, n)
+    assert np.allclose(d[1:-1, 1:-1, 1:-1], a + b + c, rtol=1e-12)
+    assert np.all(d[0] == 0)
+
+
+# ---------------------------------------------------------------- a14 G2P
+def test_g2p_linear_field_and_blend(pkg, orc):
+    """trilinear (spray_k.cu:111-122) is exact for linear fields inside the clamp
+    range; mapvelg2p_flip_bubble adds delta-u (alpha = 1); alpha = 0 is PIC."""
+    n = 8
+    for alpha in (1.0, 0.0, 0.95):
+        cfg = small_cfg(pkg, n, flip_alpha=alpha)
+        o = orc.Oracle(cfg, use_double=True)
+        rng = np.random.default_rng(9)
+        P = 200
+        pos = (rng.random((P, 3)) * 4 * H + 2 * H).astype(np.float32)
+        vel = rng.standard_normal((P, 3)).astype(np.float32)
+        o.set_particles(P, pos=pos, vel=vel, flag=np.zeros(P, np.int8))
+        gx = np.arange(n + 1) * H
+        gc = (np.arange(n) + 0.5) * H
+        ux = np.broadcast_to(2.0 * gx[:, None, None] + 3.0 * gc[None, :, None], (n + 1, n, n))
+        uy = np.broadcast_to(-1.0 * gc[:, None, None] + 0.5 * gc[None, None, :], (n, n + 1, n))
+        uz = np.broadcast_to(1.5 * gx[None, None, :], (n, n, n + 1))
+        for f, u in (("WATER_UX", ux), ("WATER_UY", uy), ("WATER_UZ", uz)):
+            o.set_field(f, u)
+            o.set_field(f + "_OLD", np.zeros(u.size))
+        o.g2p()
+        v = o.get_particles()["vel"]
+        p = pos.astype(np.float64)
+        ex = np.stack([2.0 * p[:, 0] + 3.0 * p[:, 1], -1.0 * p[:, 0] + 0.5 * p[:, 2], 1.5 * p[:, 2]], 1)
+        al = float(np.float32(alpha))  # the config holds alpha as a float
+        want = (1 - al) * ex + al * (vel + ex)
+        assert np.allclose(v, want, rtol=1e-9, atol=1e-9), alpha
+
+
+# ---------------------------------------------------------------- a15 advection
+def test_advection_translation_and_clamp(pkg, orc):
+    """CollisionWithSolid_k (spray_k.cu:5276-5345) without solids: x += v dt,
+    then clamp to [gmin + 1.5h, gmax - 1.5h] zeroing that velocity component."""
+    n = 8
+    cfg = small_cfg(pkg, n, dt=0.01)
+    o = orc.Oracle(cfg, use_double=True)
+    P = 3
+    pos = np.array([[4 * H, 4 * H, 4 * H], [1.6 * H, 4 * H, 4 * H], [4 * H, 4 * H, 6.4 * H]], np.float32)
+    vel = np.array([[0.1, -0.2, 0.3], [-1.0, 0, 0], [0, 0, 1.0]], np.float32)
+    o.set_particles(P, pos=pos, vel=vel, flag=np.zeros(P, np.int8))
+    o.hash_sort()
+    o.advect()
+    q = o.get_particles()
+    order = o.get_field("GRID_INDEX")
+    pp = {k: np.empty_like(v) for k, v in q.items()}
+    for k in pp:
+        pp[k][order] = q[k]
+    assert np.allclose(pp["pos"][0], pos[0] + np.float32(0.01) * vel[0], rtol=1e-7)
+    assert pp["pos"][1, 0] == pytest.approx(1.5 * H) and pp["vel"][1, 0] == 0
+    assert pp["pos"][2, 2] == pytest.approx((n - 1.5) * H) and pp["vel"][2, 2] == 0
+
+
+# ---------------------------------------------------------------- a16 heat
+@pytest.mark.parametrize("solver", [0, 1])
+def test_heat_uniform_field_is_fixed_point(pkg, orc, solver):
+    """computeAx_heat (spray_k.cu:4526-4567): a uniform temperature with Neumann
+    walls and no heater is a fixed point of the implicit step; particle T and
+    latent heat follow updateLatentHeat_k (5546-5573)."""
+    n = 8
+    cfg = small_cfg(pkg, n, heat_solver=solver, heat_add_bottom=0, default_temperature=300.0, dt=0.003,
+                    heat_extend_boundary=1)
+    o = orc.Oracle(cfg, use_double=True)
+    rng = np.random.default_rng(10)
+    P = 4000
+    pos = (rng.random((P, 3)) * (n - 3) * H + 1.5 * H).astype(np.float32)
+    o.set_particles(P, pos=pos, T=np.full(P, 300.0, np.float32), lheat=np.full(P, 10.0, np.float32),
+                    flag=np.zeros(P, np.int8))
+    o.hash_sort()
+    mark = np.zeros((n, n, n), np.int8)
+    mark[[0, -1], :, :] = 2
+    mark[:, [0, -1], :] = 2
+    mark[:, :, [0, -1]] = 2
+    o.set_field("MARK", mark)
+    o.heat()
+    t = o.get_field("TEMPERATURE").reshape(n, n, n)
+    assert np.allclose(t[1:-1, 1:-1, 1:-1], 300.0, rtol=1e-4)
+    pT = o.get_particles()["T"]
+    assert np.allclose(pT, 300.0, rtol=1e-3)
+
+
+def test_heat_operator_spd(pkg, orc):
+    n = 8
+    cfg = small_cfg(pkg, n)
+    o = orc.Oracle(cfg, use_double=True)
+    rng = np.random.default_rng(11)
+    mark = rng.choice(np.array([0, 1, 3, 5], np.int8), (n, n, n))
+    mark[[0, -1], :, :] = 2
+    o.set_field("MARK", mark)
+    G = n ** 3
+    unk = mark.ravel() != 2
+    x = rng.standard_normal(G) * unk
+    y = rng.standard_normal(G) * unk
+    Ax, Ay = o.apply_heat_operator(x), o.apply_heat_operator(y)
+    assert x @ Ay == pytest.approx(y @ Ax, rel=1e-12)
+    assert x @ Ax > 0
+
+
+# ---------------------------------------------------------------- a17 phase change
+def test_phase_change_rules(pkg, orc):
+    """MeltingSolidByHeat (spray_k.cu:5059-5072): SOLID -> FLUID when LHeat >
+    threshold; FreezingSolidByHeat (5074-5124): FLUID with LHeat < 0 becomes
+    SOLID when more than 2 solid particles lie within h and the nearest within h,
+    snapping to samplespace from it."""
+    n = 8
+    cfg = small_cfg(pkg, n, melt_enabled=1, freeze_enabled=1, gas_enabled=0, liquid_heat_th=1.0)
+    o = orc.Oracle(cfg, use_double=True)
+    ss = cfg.samplespace
+    c = np.array([4 * H, 4 * H, 4 * H])
+    pos = np.array([c, c + [0.3 * H, 0, 0], c + [0, 0.3 * H, 0], c + [0, 0, 0.3 * H],   # solids
+                    c + [0.9 * H, 0, 0],                                               # freezes, far -> snapped
+                    c + [6 * H, 0, 0] - [0, 0, 0],                                     # isolated liquid
+                    c + [0, -0.2 * H, -0.2 * H]], np.float32)                           # solid that melts
+    flag = np.array([5, 5, 5, 5, 0, 0, 5], np.int8)
+    lh = np.array([0, 0, 0, 0, -1, -1, 2.0], np.float32)
+    o.set_particles(len(flag), pos=pos, flag=flag, lheat=lh)
+    o.hash_sort()
+    info = o.phase_change(1)
+    pp = o.get_particles()
+    order = o.get_field("GRID_INDEX")
+    newflag = np.empty_like(flag)
+    newflag[order] = pp["flag"]
+    assert info["melted"] == 1 and info["frozen"] == 1
+    assert list(newflag) == [5, 5, 5, 5, 5, 0, 0]
+    fp = np.empty((len(flag), 3))
+    fp[order] = pp["pos"]
+    # snapped to distance samplespace from its nearest solid (the one at +0.3h x)
+    d = np.linalg.norm(fp[4] - pos[1].astype(np.float64))
+    assert d == pytest.approx(ss, rel=1e-5)
+
+
+def test_counter_rng(orc):
+    """SplitMix64-based counter hash: deterministic, in [0,1), roughly uniform."""
+    u = np.array([orc.uniform(7, f, c, k) for f in range(4) for c in range(200) for k in range(4)])
+    assert np.all((u >= 0) & (u < 1))
+    assert abs(u.mean() - 0.5) < 0.02
+    assert orc.uniform(7, 1, 2, 3) == orc.uniform(7, 1, 2, 3) != orc.uniform(8, 1, 2, 3)
+
+
+# ---------------------------------------------------------------- a18 marching cubes
+def test_mc_edge_table_closed_form(orc):
+    """edgeTable[c] bit e set iff the corners of edge e differ in case c."""
+    et, tt, nv = orc.mc_tables()
+    edges = [(0, 1), (1, 2), (2, 3), (3, 0), (4, 5), (5, 6), (6, 7), (7, 4), (0, 4), (1, 5), (2, 6), (3, 7)]
+    for c in range(256):
+        exp = sum(1 << e for e, (a, b) in enumerate(edges) if ((c >> a) & 1) != ((c >> b) & 1))
+        assert et[c] == exp
+        used = set(int(x) for x in tt[c][:nv[c]])
+        assert used == {e for e in range(12) if (exp >> e) & 1}
+        assert nv[c] % 3 == 0
+
+
+REF_TABLES = "/root/reference/Multiphase/Multiphase/tables.h"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_TABLES), reason="reference tables not mounted")
+def test_mc_vertex_counts_match_reference_tables(orc):
+    """numVertsTable (tables.h:318-575), read here as data, equals the counts of
+    the tables generated from the cube geometry for all 256 cases."""
+    import re
+    src = open(REF_TABLES, encoding="latin-1").read()
+    i = src.find("numVertsTable")
+    body = src[i:src.find("};", i)].split("=", 1)[1]
+    ref = [int(x) for x in re.findall(r"\b\d+\b", body)]
+    _, _, nv = orc.mc_tables()
+    assert len(ref) == 256 and list(nv) == ref
+
+
+def test_mc_sphere_watertight(pkg, orc):
+    """On a sphere SDF the mesh is closed and consistently oriented: every
+    undirected edge is shared by exactly two triangles, every directed edge
+    appears once, V - E + F = 2, and normals point outward."""
+    n = 16
+    cfg = pkg.default_config(n, n, n, mc_enabled=1, particle_capacity=10)
+    o = orc.Oracle(cfg, use_double=True)
+    c = (np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3) + 0.5) * H
+    cen = np.array([8.1, 7.7, 8.3]) * H
+    sdf = np.linalg.norm(c - cen, axis=1) - 4.3 * H
+    o.set_field("LSFLUID", sdf)
+    info = o.marching_cubes()
+    v, t = o.get_mesh(info["verts"], info["tris"])
+    assert info["tris"] > 100
+    e = np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]])
+    assert len(set(map(tuple, e))) == len(e)  # directed edges unique
+    und = np.sort(e, axis=1)
+    _, cnt = np.unique(und, axis=0, return_counts=True)
+    assert np.all(cnt == 2)
+    V, E, F = len(v), len(cnt), len(t)
+    assert V - E + F == 2
+    nrm = np.cross(v[t[:, 1]] - v[t[:, 0]], v[t[:, 2]] - v[t[:, 0]])
+    outward = ((v[t].mean(1) - cen) * nrm).sum(1)
+    assert np.all(outward > 0)
+    r = np.linalg.norm(v - cen, axis=1)
+    assert np.all(np.abs(r - 4.3 * H) < 0.5 * H)
+    cases = o.get_mc_cases()
+    assert cases.size == (n - 1) ** 3 and np.count_nonzero(cases == 255) > 0
+
+
+# ---------------------------------------------------------------- whole step
+def test_oracle_step_runs_small_scenes(pkg, scenes, orc):
+    """Whole bubblesim-order step on the small scenes: finite, particle count
+    bookkeeping, projection converged."""
+    for name, n in (("bubble_rise", 16), ("boiling", 16), ("melt_freeze", 20)):
+        s = scenes.make(name, n=n)
+        o = orc.Oracle(s["cfg"])
+        load_scene(o, s)
+        for f in range(2):
+            info = o.step(f)
+            assert info["project"]["converged"] == 1
+        pp = o.get_particles()
+        assert np.all(np.isfinite(pp["pos"])) and np.all(np.isfinite(pp["vel"]))
+        assert o.particle_count() == s["n_particles"] + info["phase"]["particles"] - s["n_particles"]
