@@ -114,7 +114,7 @@ void free_all(Ctx& c) {
   for (int a = 0; a < 3; ++a) dfree(c.phig[a]);
   dfree(c.phifluid); dfree(c.phiair); dfree(c.phisolid); dfree(c.ctmp);
   dfree(c.div); dfree(c.press); dfree(c.tp); dfree(c.tpchange); dfree(c.tsave);
-  dfree(c.vx); dfree(c.vr); dfree(c.vp); dfree(c.vq); dfree(c.vdinv);
+  dfree(c.vx); dfree(c.vr); dfree(c.vp); dfree(c.vq); dfree(c.vdinv); dfree(c.nbm);
   dfree(c.partials); dfree(c.scal); dfree(c.counters);
   for (int a = 0; a < 3; ++a) {
     dfree(c.wu[a]); dfree(c.au[a]); dfree(c.wuo[a]); dfree(c.auo[a]); dfree(c.su[a]); dfree(c.ftmp[a]);
@@ -160,6 +160,7 @@ bool alloc_all(Ctx& c) {
   ok = ok && dalloc(c, dd, G); c.vp = dd;
   ok = ok && dalloc(c, dd, G); c.vq = dd;
   ok = ok && dalloc(c, dd, G); c.vdinv = dd;
+  ok = ok && dalloc(c, c.nbm, G);
   c.npart = 2048 * 4;
   ok = ok && dalloc(c, c.partials, c.npart) && dalloc(c, c.scal, 1) && dalloc(c, c.counters, 64);
   for (int a = 0; a < 3 && ok; ++a)
@@ -345,6 +346,10 @@ mpf_status mpf_create(const mpf_config* cfg, mpf_ctx** out) {
   g.gx1 = cfg->gmin[0] + cfg->dx * (float)g.nx;
   g.gy1 = cfg->gmin[1] + cfg->dx * (float)g.ny;
   g.gz1 = cfg->gmin[2] + cfg->dx * (float)g.nz;
+  g.dnz = make_fastdiv((uint32_t)g.nz);
+  g.dny = make_fastdiv((uint32_t)g.ny);
+  g.dnz1 = make_fastdiv((uint32_t)g.nz + 1);
+  g.dny1 = make_fastdiv((uint32_t)g.ny + 1);
   c.cap = cfg->particle_capacity;
   hipGetDevice(&c.device);
   if (!hip_ok(c, hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "stream")) {

@@ -4,6 +4,10 @@
 // the solid distance/velocity fields used by the collision advection (a15).
 #include "mpf_internal.h"
 
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "mc_tables.inc"
 
 namespace mpf {
@@ -23,7 +27,16 @@ __device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y
 // ---------------- level set (a6) ----------------
 // genlevelset + mindis_cell, spray_k.cu:2753-2806.  The minimum is taken over
 // squared distances and square-rooted once: sqrt is correctly rounded and
-// monotone, so sqrt(min d^2) == min sqrt(d^2) bit for bit.
+// monotone, so sqrt(min d^2) == min sqrt(d^2) bit for bit.  The 5x5x5 window
+// is visited in rings of increasing lower bound on the distance from the cell
+// centre to any point of a ring cell (sum over axes of max(0,|d|-1/2)^2, in
+// h^2); a phase stops once its minimum is below the next ring's bound (with a
+// 0.1% margin for rounding), and the 8 corner cells (bound 6.75 h^2 > 2.5^2)
+// can never beat the initial 2.5h.  The result equals the full-window minimum.
+__constant__ signed char c_ring_off[117][3];
+__constant__ unsigned char c_ring_start[10];  // ring r = offsets [start[r], start[r+1])
+__constant__ float c_ring_lb[9];              // lower bound of ring r, in h^2 units
+
 __global__ void k_genlevelset(Grid g, const f3* __restrict__ pos, const int8_t* __restrict__ flag,
                               const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
                               const uint8_t* __restrict__ cmask, float* __restrict__ lsfluid,
@@ -34,21 +47,27 @@ __global__ void k_genlevelset(Grid g, const f3* __restrict__ pos, const int8_t* 
   float r = 0.5f * h;
   f3 gp = mk3(((float)i + 0.5f) * h + g.gx0, ((float)j + 0.5f) * h + g.gy0, ((float)k + 0.5f) * h + g.gz0);
   float ma2 = INFINITY, mf2 = INFINITY;
-  for (int di = -2; di <= 2; ++di)
-    for (int dj = -2; dj <= 2; ++dj)
-      for (int dk = -2; dk <= 2; ++dk) {
-        if (!valid(g, i + di, j + dj, k + dk)) continue;
-        int64_t c = cid(g, i + di, j + dj, k + dk);
-        if (!(cmask[c] & (M_FLUIDSOLID | M_AIR | M_AIRSOLO))) continue;
-        uint32_t s = cstart[c], e = cend[c];
-        for (uint32_t p = s; p < e; ++p) {
-          f3 d = sub3(pos[p], gp);
-          float d2 = dot3(d, d);
-          int8_t f = flag[p];
-          if (f == AIR || f == AIRSOLO) ma2 = fminf(ma2, d2);
-          else if (f == FLUID || f == SOLID) mf2 = fminf(mf2, d2);
-        }
+  const float h2 = h * h;
+  for (int ring = 0; ring < 9; ++ring) {
+    const float lb = 0.999f * c_ring_lb[ring] * h2;
+    const bool need_f = mf2 > lb, need_a = ma2 > lb;
+    if (!need_f && !need_a) break;
+    const uint8_t want = (need_f ? M_FLUIDSOLID : 0) | (need_a ? (M_AIR | M_AIRSOLO) : 0);
+    for (int q = c_ring_start[ring]; q < c_ring_start[ring + 1]; ++q) {
+      int ci = i + c_ring_off[q][0], cj = j + c_ring_off[q][1], ck = k + c_ring_off[q][2];
+      if (!valid(g, ci, cj, ck)) continue;
+      int64_t c = cid(g, ci, cj, ck);
+      if (!(cmask[c] & want)) continue;
+      uint32_t s = cstart[c], e = cend[c];
+      for (uint32_t p = s; p < e; ++p) {
+        f3 d = sub3(pos[p], gp);
+        float d2 = dot3(d, d);
+        int8_t f = flag[p];
+        if (f == AIR || f == AIRSOLO) ma2 = fminf(ma2, d2);
+        else if (f == FLUID || f == SOLID) mf2 = fminf(mf2, d2);
       }
+    }
+  }
   float mair = 2.5f * h, mfl = 2.5f * h;
   float da = sqrtf(ma2), df = sqrtf(mf2);
   mair = (da < mair) ? da : mair;
@@ -184,7 +203,7 @@ __global__ void k_p2g(Grid g, const f3* __restrict__ pos, const f3* __restrict__
   if (f >= nf) return;
   const int hh = g.ny + (A == 1), d = g.nz + (A == 2);
   int i, j, k;
-  ijk3(f, hh, d, i, j, k);
+  ijk_face<A>(g, f, i, j, k);
   const float scale = 1.0f / g.h;
   f3 gp = mk3((float)i + (A == 0 ? 0.0f : 0.5f), (float)j + (A == 1 ? 0.0f : 0.5f), (float)k + (A == 2 ? 0.0f : 0.5f));
   constexpr int lo = AIRPH ? -2 : -1, hin = AIRPH ? 1 : 0, lot = AIRPH ? -2 : -1, hit = AIRPH ? 2 : 1;
@@ -215,6 +234,151 @@ __global__ void k_p2g(Grid g, const f3* __restrict__ pos, const f3* __restrict__
   out[f] = (weight > 0) ? (usum / weight) : 0.0f;
 }
 
+// Liquid+solid P2G for all three face axes in one pass, cell-centric.
+// A block owns the faces of a 4x8x8 cell tile (each cell's -x, -y, -z faces,
+// plus the domain-end faces for the last tile) and one thread per cell of the
+// tile grown by one cell on every side, which is exactly the set of cells
+// whose window reaches an owned face.  Each thread keeps its cell's particles
+// in registers.  The 27 window offsets o = (di,dj,dk) are visited in the
+// gather's lexicographic order; in phase o every thread adds its particles,
+// in index order, to the owned face f = c - o of each axis whose window
+// contains o (normal offset in {-1,0}), accumulating in LDS.  Within a phase
+// the map c -> f is one-to-one, so no two threads touch the same face, and a
+// face receives exactly the reference's sequence of additions
+// (mapvelp2g_k_fluidSolid, spray_k.cu:2829-2871): bit-identical results
+// without global atomics.
+constexpr int PT_X = 4, PT_Y = 8, PT_Z = 8;
+constexpr int PB_X = PT_X + 2, PB_Y = PT_Y + 2, PB_Z = PT_Z + 2;
+constexpr int PBOX = PB_X * PB_Y * PB_Z;  // 600
+constexpr int P2G_THREADS = 640;
+constexpr int PFX = PT_X + 1, PFY = PT_Y + 1, PFZ = PT_Z + 1;
+constexpr int PFACE = PFX * PFY * PFZ;
+constexpr int PCHUNK = 8;
+
+__global__ void __launch_bounds__(P2G_THREADS)
+    k_p2g_liquid3(Grid g, const f3* __restrict__ pos, const f3* __restrict__ vel, const float* __restrict__ mass,
+                  const int8_t* __restrict__ flag, const uint32_t* __restrict__ cstart,
+                  const uint32_t* __restrict__ cend, const uint8_t* __restrict__ cmask, float* __restrict__ ox,
+                  float* __restrict__ oy, float* __restrict__ oz, int ntx, int nty, int ntz) {
+  __shared__ float accW[3][PFACE];
+  __shared__ float accU[3][PFACE];
+  int t = blockIdx.x;
+  const int tz = t % ntz;
+  t /= ntz;
+  const int ty = t % nty;
+  const int tx = t / nty;
+  const int x0 = tx * PT_X, y0 = ty * PT_Y, z0 = tz * PT_Z;
+  const int n3[3] = {g.nx, g.ny, g.nz};
+  const int o3[3] = {x0, y0, z0};
+  const int T3[3] = {PT_X, PT_Y, PT_Z};
+  // owned face index ranges [lo, hi) per axis a and direction d
+  int hi[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      int e = o3[d] + T3[d];
+      hi[a][d] = (d == a) ? ((e >= n3[d]) ? n3[d] + 1 : e) : min(e, n3[d]);
+    }
+  for (int q = threadIdx.x; q < 3 * PFACE; q += P2G_THREADS) {
+    (&accW[0][0])[q] = 0.0f;
+    (&accU[0][0])[q] = 0.0f;
+  }
+  // this thread's box cell
+  const int b = threadIdx.x;
+  const int bx = b / (PB_Y * PB_Z), by = (b / PB_Z) % PB_Y, bz = b % PB_Z;
+  const int ci = x0 - 1 + bx, cj = y0 - 1 + by, ck = z0 - 1 + bz;
+  uint32_t s = 0, e = 0;
+  if (b < PBOX && valid(g, ci, cj, ck)) {
+    int64_t c = cid(g, ci, cj, ck);
+    if (cmask[c] & M_FLUIDSOLID) {
+      s = cstart[c];
+      e = cend[c];
+    }
+  }
+  const int any = __syncthreads_or(e > s);
+  if (any) {
+    const float scale = 1.0f / g.h;
+    const float RE = 1.4f, RE2 = RE * RE;
+    float X[PCHUNK], Y[PCHUNK], Z[PCHUNK], VX[PCHUNK], VY[PCHUNK], VZ[PCHUNK], M[PCHUNK];
+    bool ok[PCHUNK];
+    const uint32_t n = e - s;
+#pragma unroll
+    for (int q = 0; q < PCHUNK; ++q) {
+      ok[q] = false;
+      X[q] = Y[q] = Z[q] = VX[q] = VY[q] = VZ[q] = M[q] = 0.0f;
+      if ((uint32_t)q < n) {
+        uint32_t p = s + q;
+        int8_t fl = flag[p];
+        ok[q] = (fl == FLUID || fl == SOLID);
+        f3 x = pos[p], v = vel[p];
+        X[q] = (x.x - g.gx0) * scale;
+        Y[q] = (x.y - g.gy0) * scale;
+        Z[q] = (x.z - g.gz0) * scale;
+        VX[q] = v.x; VY[q] = v.y; VZ[q] = v.z;
+        M[q] = mass[p];
+      }
+    }
+    for (int ph = 0; ph < 27; ++ph) {
+      const int o[3] = {ph / 9 - 1, (ph / 3) % 3 - 1, ph % 3 - 1};
+      if (n > 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          if (o[a] > 0) continue;
+          const int f[3] = {ci - o[0], cj - o[1], ck - o[2]};
+          bool own = true;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) own = own && f[d] >= o3[d] && f[d] < hi[a][d];
+          if (!own) continue;
+          const int li = ((f[0] - x0) * PFY + (f[1] - y0)) * PFZ + (f[2] - z0);
+          float W = accW[a][li], U = accU[a][li];
+          const float gx = (float)f[0] + (a == 0 ? 0.0f : 0.5f);
+          const float gy = (float)f[1] + (a == 1 ? 0.0f : 0.5f);
+          const float gz = (float)f[2] + (a == 2 ? 0.0f : 0.5f);
+#pragma unroll
+          for (int q = 0; q < PCHUNK; ++q) {
+            if (!ok[q]) continue;
+            float dx = X[q] - gx, dy = Y[q] - gy, dz = Z[q] - gz;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float w = M[q] * fmaxf(RE2 / fmaxf(r2, 0.0001f) - 1.0f, 0.0f);
+            W += w;
+            U += w * (a == 0 ? VX[q] : (a == 1 ? VY[q] : VZ[q]));
+          }
+          // particles beyond the register chunk, still in index order
+          for (uint32_t p = s + PCHUNK; p < e; ++p) {
+            int8_t fl = flag[p];
+            if (!(fl == FLUID || fl == SOLID)) continue;
+            f3 x = pos[p];
+            float dx = (x.x - g.gx0) * scale - gx, dy = (x.y - g.gy0) * scale - gy, dz = (x.z - g.gz0) * scale - gz;
+            float r2 = dx * dx + dy * dy + dz * dz;
+            float w = mass[p] * fmaxf(RE2 / fmaxf(r2, 0.0001f) - 1.0f, 0.0f);
+            W += w;
+            const float* vp = &vel[p].x;
+            U += w * vp[a];
+          }
+          accW[a][li] = W;
+          accU[a][li] = U;
+        }
+      }
+      __syncthreads();
+    }
+  } else {
+    __syncthreads();
+  }
+  // write the owned faces
+  float* outs[3] = {ox, oy, oz};
+  for (int a = 0; a < 3; ++a) {
+    const int lx = hi[a][0] - x0, ly = hi[a][1] - y0, lz = hi[a][2] - z0;
+    const int cnt = lx * ly * lz;
+    for (int q = threadIdx.x; q < cnt; q += P2G_THREADS) {
+      int fx = q / (ly * lz), fy = (q / lz) % ly, fz = q % lz;
+      int li = (fx * PFY + fy) * PFZ + fz;
+      float W = accW[a][li], U = accU[a][li];
+      outs[a][fid(g, a, x0 + fx, y0 + fy, z0 + fz)] = (W > 0) ? (U / W) : 0.0f;
+    }
+  }
+}
+
 // ---------------- extrapolation + wall conditions (a10) ----------------
 // initphi, spray_k.cu:712-723
 __global__ void k_initphi(int64_t G, const int8_t* __restrict__ mark, float* __restrict__ phi, int8_t type, float big) {
@@ -230,7 +394,7 @@ __global__ void k_sweep_u(Grid g, const float* __restrict__ u, float* __restrict
   if (f >= nf) return;
   const int hh = g.ny + (A == 1), d = g.nz + (A == 2);
   int i, j, k;
-  ijk3(f, hh, d, i, j, k);
+  ijk_face<A>(g, f, i, j, k);
   float res = u[f];
   const int n[3] = {i, j, k};
   const int lim = (A == 0) ? g.nx : (A == 1) ? g.ny : g.nz;
@@ -280,7 +444,7 @@ __global__ void k_bc(Grid g, float* __restrict__ u, const int8_t* __restrict__ m
   if (f >= nf) return;
   const int hh = g.ny + (A == 1), d = g.nz + (A == 2);
   int i, j, k;
-  ijk3(f, hh, d, i, j, k);
+  ijk_face<A>(g, f, i, j, k);
   if (A == 0) {
     if (i <= 1 || i >= (g.nx + 1) - 2) u[f] = 0.0f;
     else if (i > 1 && i < g.nx - 1 && (isb(g, mark, i, j, k) != isb(g, mark, i - 1, j, k))) u[f] = 0.0f;
@@ -352,7 +516,7 @@ __global__ void k_subgrad(Grid g, float* __restrict__ u, const R* __restrict__ p
   if (f >= nf) return;
   const int hh = g.ny + (A == 1), d = g.nz + (A == 2);
   int i, j, k;
-  ijk3(f, hh, d, i, j, k);
+  ijk_face<A>(g, f, i, j, k);
   const int n[3] = {i, j, k};
   const int lim = (A == 0) ? g.nx : (A == 1) ? g.ny : g.nz;
   if (!(n[A] > 0 && n[A] < lim)) return;
@@ -391,6 +555,95 @@ __global__ void k_heat_p2g(Grid g, const f3* __restrict__ pos, const float* __re
         }
       }
   tp[idx] = (weight > 0) ? (hs / weight) : deft;
+}
+
+// Heat P2G in the same cell-centric phased form as k_p2g_liquid3: a block
+// owns a 4x8x8 cell tile, one thread per cell of the tile grown by one cell;
+// the 27 offsets are visited in the gather's order and each target cell
+// receives its additions in the reference's sequence (mapHeatp2g_hash +
+// sumHeat, spray_k.cu:4619-4653), so results are bit-identical.
+__global__ void __launch_bounds__(P2G_THREADS)
+    k_heat_p2g3(Grid g, const f3* __restrict__ pos, const float* __restrict__ temp,
+                const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
+                const uint8_t* __restrict__ cmask, float deft, float* __restrict__ tp, int ntx, int nty, int ntz) {
+  constexpr int TN = PT_X * PT_Y * PT_Z;
+  __shared__ float accW[TN];
+  __shared__ float accH[TN];
+  int t = blockIdx.x;
+  const int tz = t % ntz;
+  t /= ntz;
+  const int ty = t % nty;
+  const int tx = t / nty;
+  const int x0 = tx * PT_X, y0 = ty * PT_Y, z0 = tz * PT_Z;
+  const int hx = min(x0 + PT_X, g.nx), hy = min(y0 + PT_Y, g.ny), hz = min(z0 + PT_Z, g.nz);
+  for (int q = threadIdx.x; q < TN; q += P2G_THREADS) {
+    accW[q] = 0.0f;
+    accH[q] = 0.0f;
+  }
+  const int b = threadIdx.x;
+  const int bx = b / (PB_Y * PB_Z), by = (b / PB_Z) % PB_Y, bz = b % PB_Z;
+  const int ci = x0 - 1 + bx, cj = y0 - 1 + by, ck = z0 - 1 + bz;
+  uint32_t s = 0, e = 0;
+  if (b < PBOX && valid(g, ci, cj, ck)) {
+    int64_t c = cid(g, ci, cj, ck);
+    if (cmask[c]) {
+      s = cstart[c];
+      e = cend[c];
+    }
+  }
+  const int any = __syncthreads_or(e > s);
+  if (any) {
+    const float scale = 1.0f / g.h;
+    const float RE = 1.4f, RE2 = RE * RE;
+    float X[PCHUNK], Y[PCHUNK], Z[PCHUNK], TT[PCHUNK];
+    const uint32_t n = e - s;
+#pragma unroll
+    for (int q = 0; q < PCHUNK; ++q) {
+      X[q] = Y[q] = Z[q] = TT[q] = 0.0f;
+      if ((uint32_t)q < n) {
+        f3 x = pos[s + q];
+        X[q] = (x.x - g.gx0) * scale;
+        Y[q] = (x.y - g.gy0) * scale;
+        Z[q] = (x.z - g.gz0) * scale;
+        TT[q] = temp[s + q];
+      }
+    }
+    for (int ph = 0; ph < 27; ++ph) {
+      const int f0 = ci - (ph / 9 - 1), f1 = cj - ((ph / 3) % 3 - 1), f2 = ck - (ph % 3 - 1);
+      if (n > 0 && f0 >= x0 && f0 < hx && f1 >= y0 && f1 < hy && f2 >= z0 && f2 < hz) {
+        const int li = ((f0 - x0) * PT_Y + (f1 - y0)) * PT_Z + (f2 - z0);
+        float W = accW[li], Hs = accH[li];
+        const float gx = (float)f0 + 0.5f, gy = (float)f1 + 0.5f, gz = (float)f2 + 0.5f;
+#pragma unroll
+        for (int q = 0; q < PCHUNK; ++q) {
+          if ((uint32_t)q >= n) break;
+          float dx = X[q] - gx, dy = Y[q] - gy, dz = Z[q] - gz;
+          float w = fmaxf(RE2 / fmaxf(dx * dx + dy * dy + dz * dz, 0.0001f) - 1.0f, 0.0f);
+          W += w;
+          Hs += w * TT[q];
+        }
+        for (uint32_t p = s + PCHUNK; p < e; ++p) {
+          f3 x = pos[p];
+          float dx = (x.x - g.gx0) * scale - gx, dy = (x.y - g.gy0) * scale - gy, dz = (x.z - g.gz0) * scale - gz;
+          float w = fmaxf(RE2 / fmaxf(dx * dx + dy * dy + dz * dz, 0.0001f) - 1.0f, 0.0f);
+          W += w;
+          Hs += w * temp[p];
+        }
+        accW[li] = W;
+        accH[li] = Hs;
+      }
+      __syncthreads();
+    }
+  } else {
+    __syncthreads();
+  }
+  const int lx = hx - x0, ly = hy - y0, lz = hz - z0;
+  for (int q = threadIdx.x; q < lx * ly * lz; q += P2G_THREADS) {
+    int fx = q / (ly * lz), fy = (q / lz) % ly, fz = q % lz;
+    int li = (fx * PT_Y + fy) * PT_Z + fz;
+    float W = accW[li];
+    tp[cid(g, x0 + fx, y0 + fy, z0 + fz)] = (W > 0) ? (accH[li] / W) : deft;
+  }
 }
 
 // addHeatAtBottom spray_k.cu:4492-4504, then save T and form b (compb_heat
@@ -562,7 +815,45 @@ static void redistance(Ctx& c, float* ls) {
   launch(c, "ls_back", dim3(nblocks(G, 256)), dim3(256), 0, k_ls_back, G, ls, c.g.h, 1);
 }
 
+static void upload_ring_tables() {
+  static bool done = false;
+  if (done) return;
+  struct O { int d[3]; float lb; };
+  std::vector<O> offs;
+  for (int di = -2; di <= 2; ++di)
+    for (int dj = -2; dj <= 2; ++dj)
+      for (int dk = -2; dk <= 2; ++dk) {
+        int d[3] = {di, dj, dk};
+        float lb = 0;
+        for (int a = 0; a < 3; ++a) {
+          float m = fabsf((float)d[a]) - 0.5f;
+          if (m > 0) lb += m * m;
+        }
+        if (lb > 6.25f) continue;  // corners of the window: never below 2.5h
+        offs.push_back(O{{di, dj, dk}, lb});
+      }
+  std::stable_sort(offs.begin(), offs.end(), [](const O& x, const O& y) { return x.lb < y.lb; });
+  signed char tab[117][3];
+  unsigned char start[10];
+  float lbs[9];
+  int nr = 0;
+  for (size_t q = 0; q < offs.size(); ++q) {
+    if (q == 0 || offs[q].lb != offs[q - 1].lb) {
+      start[nr] = (unsigned char)q;
+      lbs[nr] = offs[q].lb;
+      ++nr;
+    }
+    for (int a = 0; a < 3; ++a) tab[q][a] = (signed char)offs[q].d[a];
+  }
+  start[nr] = (unsigned char)offs.size();
+  hipMemcpyToSymbol(HIP_SYMBOL(c_ring_off), tab, sizeof(tab));
+  hipMemcpyToSymbol(HIP_SYMBOL(c_ring_start), start, sizeof(start));
+  hipMemcpyToSymbol(HIP_SYMBOL(c_ring_lb), lbs, sizeof(lbs));
+  done = true;
+}
+
 void run_levelset(Ctx& c) {
+  upload_ring_tables();
   const int64_t G = c.g.G;
   const int cur = c.cur;
   launch(c, "genlevelset", dim3(nblocks(G, 256)), dim3(256), 0, k_genlevelset, c.g, (const f3*)c.pos[cur],
@@ -599,8 +890,17 @@ static void p2g_phase(Ctx& c, float* const* out) {
          (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask, out[2], fc[2]);
 }
 
+static void p2g_liquid(Ctx& c, float* const* out) {
+  const int cur = c.cur;
+  const int ntx = (c.g.nx + PT_X - 1) / PT_X, nty = (c.g.ny + PT_Y - 1) / PT_Y, ntz = (c.g.nz + PT_Z - 1) / PT_Z;
+  launch(c, "p2g_liquid", dim3((unsigned)(ntx * nty * ntz)), dim3(P2G_THREADS), 0, k_p2g_liquid3, c.g,
+         (const f3*)c.pos[cur], (const f3*)c.vel[cur], (const float*)c.mass[cur], (const int8_t*)c.flag[cur],
+         (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask, out[0], out[1], out[2], ntx,
+         nty, ntz);
+}
+
 void run_p2g(Ctx& c) {
-  p2g_phase<false>(c, c.wu);
+  p2g_liquid(c, c.wu);
   p2g_phase<true>(c, c.au);
   const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
   for (int a = 0; a < 3; ++a) {
@@ -614,7 +914,7 @@ void run_solid_phi_and_velocity(Ctx& c) {
   launch(c, "init_solid_phi", dim3(nblocks(G, 256)), dim3(256), 0, k_init_solid_phi, G, (const uint8_t*)c.cmask,
          c.phisolid, (float)(3 * c.g.nx));
   sweep_n(c, c.phisolid, -1, 3);
-  p2g_phase<false>(c, c.su);
+  p2g_liquid(c, c.su);
 }
 
 static void sweep_phi(Ctx& c, float* phi, int8_t type) {
@@ -695,9 +995,10 @@ void run_subgrad(Ctx& c) {
 void run_heat_grid_pre(Ctx& c) {
   const mpf_config& k = c.cfg;
   const int64_t G = c.g.G;
-  launch(c, "heat_p2g", dim3(nblocks(G, 256)), dim3(256), 0, k_heat_p2g, c.g, (const f3*)c.pos[c.cur],
-         (const float*)c.temp[c.cur], (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask,
-         k.default_temperature, c.tp);
+  const int ntx = (c.g.nx + PT_X - 1) / PT_X, nty = (c.g.ny + PT_Y - 1) / PT_Y, ntz = (c.g.nz + PT_Z - 1) / PT_Z;
+  launch(c, "heat_p2g", dim3((unsigned)(ntx * nty * ntz)), dim3(P2G_THREADS), 0, k_heat_p2g3, c.g,
+         (const f3*)c.pos[c.cur], (const float*)c.temp[c.cur], (const uint32_t*)c.cstart, (const uint32_t*)c.cend,
+         (const uint8_t*)c.cmask, k.default_temperature, c.tp, ntx, nty, ntz);
   launch(c, "heat_pre", dim3(nblocks(G, 256)), dim3(256), 0, k_heat_pre, c.g, c.tp, c.tsave, c.ctmp,
          (const int8_t*)c.mark, k.heat_add_bottom, k.heat_increase_bottom, k.heat_bottom_clamp, k.heat_alpha[0],
          k.heat_alpha[1], k.heat_alpha[2], k.heat_alpha[3], k.heat_alpha[4], k.heat_alpha[5], k.dt);

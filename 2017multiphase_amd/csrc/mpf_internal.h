@@ -27,6 +27,33 @@ struct f3 {
   float x, y, z;
 };
 
+// Division of a 32-bit unsigned numerator by a run-time-invariant divisor
+// with one multiply-high, an add and a shift (Granlund & Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994, the
+// round-up variant): q = (mulhi(n, m) + n) >> s, exact for every n < 2^32.
+// Replaces the 64-bit divisions of index decomposition in every kernel.
+struct FastDiv {
+  uint32_t d, m, s;
+};
+__host__ inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f;
+  f.d = d;
+  uint32_t s = 0;
+  while ((1ull << s) < d) ++s;
+  f.s = s;
+  // m = floor(2^32 * (2^s - d) / d) + 1, fits in 32 bits for d >= 1
+  f.m = (uint32_t)(((((uint64_t)1 << s) - d) << 32) / d + 1);
+  return f;
+}
+__host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv& f) {
+#ifdef __HIP_DEVICE_COMPILE__
+  uint32_t t = __umulhi(n, f.m);
+#else
+  uint32_t t = (uint32_t)(((uint64_t)n * f.m) >> 32);
+#endif
+  return (uint32_t)(((uint64_t)t + n) >> f.s);
+}
+
 // Everything a kernel needs about the grid and the configuration; passed by
 // value as a kernel argument (replaces __constant__ FlipConstant dparam and
 // NX/NY/NZ, spray_k.cu:8-14).
@@ -36,6 +63,7 @@ struct Grid {
   float h;
   float gx0, gy0, gz0;  // gmin
   float gx1, gy1, gz1;  // gmax
+  FastDiv dnz, dny, dnz1, dny1;  // nz, ny, nz+1, ny+1
 };
 
 __host__ __device__ inline int64_t cid(const Grid& g, int i, int j, int k) {
@@ -44,11 +72,28 @@ __host__ __device__ inline int64_t cid(const Grid& g, int i, int j, int k) {
 __host__ __device__ inline bool valid(const Grid& g, int i, int j, int k) {
   return !(i < 0 || i > g.nx - 1 || j < 0 || j > g.ny - 1 || k < 0 || k > g.nz - 1);
 }
+// idx -> (i, j, k) of a cell (all grids here have fewer than 2^32 cells/faces).
 __host__ __device__ inline void ijk_of(const Grid& g, int64_t idx, int& i, int& j, int& k) {
-  i = (int)(idx / ((int64_t)g.nz * g.ny));
-  j = (int)((idx / g.nz) % g.ny);
-  k = (int)(idx % g.nz);
+  uint32_t n = (uint32_t)idx;
+  uint32_t r = fdiv(n, g.dnz);
+  k = (int)(n - r * (uint32_t)g.nz);
+  uint32_t q = fdiv(r, g.dny);
+  j = (int)(r - q * (uint32_t)g.ny);
+  i = (int)q;
 }
+// idx -> (i, j, k) of face axis A (dims (nx+1,ny,nz), (nx,ny+1,nz), (nx,ny,nz+1)).
+template <int A>
+__host__ __device__ inline void ijk_face(const Grid& g, int64_t idx, int& i, int& j, int& k) {
+  const FastDiv& dz = (A == 2) ? g.dnz1 : g.dnz;
+  const FastDiv& dy = (A == 1) ? g.dny1 : g.dny;
+  uint32_t n = (uint32_t)idx;
+  uint32_t r = fdiv(n, dz);
+  k = (int)(n - r * dz.d);
+  uint32_t q = fdiv(r, dy);
+  j = (int)(r - q * dy.d);
+  i = (int)q;
+}
+// Generic decomposition for other dims (marching-cubes voxels); 64-bit safe.
 __host__ __device__ inline void ijk3(int64_t idx, int hh, int d, int& i, int& j, int& k) {
   i = (int)(idx / d / hh);
   j = (int)((idx / d) % hh);
@@ -64,8 +109,8 @@ __host__ __device__ inline int64_t fid(const Grid& g, int a, int i, int j, int k
 // Device-side scalars of an iterative solve; lives in device memory so the
 // recurrence runs without host round trips.
 struct SolveScalars {
-  double rz, rz_new, pq, rr, bb, alpha, beta, rr_true;
-  int iter, done, converged, nonfinite, refinements, refine_now, pad0, pad1;
+  double rz, rz_prev, pq, rr, bb, alpha, beta, rr_true;
+  int iter, done, converged, nonfinite, refinements, rec_conv, fold_pending, pad0;
 };
 
 struct KernelStat {
@@ -117,6 +162,7 @@ struct Ctx {
   float *tp = nullptr, *tpchange = nullptr, *tsave = nullptr;
   // iterative solve work vectors (sized for double)
   void *vx = nullptr, *vr = nullptr, *vp = nullptr, *vq = nullptr, *vdinv = nullptr;
+  uint8_t* nbm = nullptr;  // per-cell neighbour mask of the current solve
   double* vx64 = nullptr;
   double* partials = nullptr;
   int npart = 0;

@@ -6,35 +6,48 @@
 // solver_cg_heat (spray.cu:1780-1852) with computeAx_heat / pcg_op_heat
 // (spray_k.cu:4526-4582).
 //
-// The scalar recurrence (alpha, beta, convergence) stays on the device: each
-// kernel reduces its dot products into per-block partials; the last block to
-// arrive (arrival counter + fences) sums the partials in block order —
-// deterministic — and updates the SolveScalars record the next kernel reads.
-// The host only polls the record every check_every iterations.
+// Two kernels per iteration, both HBM-bound:
+//   spmv   : p_k = z_k + beta p_{k-1} (z = r * diag^-1, recomputed, also at the
+//            six neighbours from r and p_{k-1}), written to the other p buffer;
+//            q = A p_k; p.q                       -> alpha
+//   update : x += alpha p_k ; r -= alpha q ; r.z ; r.r   -> beta, convergence
+// FP32 vectors: spmv moves r 4 + p_{k-1} 4 + mask 1 + p_k 4 + q 4 = 17 B/cell,
+// update x 8 + r 8 + p 4 + q 4 + mask 1 = 25 B/cell (FP64: 33 + 49).
+// The Jacobi diagonal is rebuilt from a per-cell neighbour-mask byte (bits
+// 0-5: face open, i.e. not Neumann; bit 6: unknown; bit 7: slow path — an air
+// cell or a liquid/air face of the variable-density operator, which reads the
+// flags and lsmerge), so no diagonal vector is stored.
 //
-// Per iteration: spmv (q = A p, p.q) -> update (x, r, z = r/diag, r.z, r.r)
-// -> [MIXED: fold + FP64 true residual when due] -> direction (p = z + beta p).
-// Blocks own contiguous chunks of cells, chunk order spread so each XCD (the
-// hardware dispatches blocks round-robin over the 8 XCDs) walks one
-// contiguous slab and keeps stencil neighbours in its own L2.
+// The scalar recurrence stays on the device: each kernel reduces into
+// per-block partials; the last block to arrive (arrival counter + fences)
+// sums the partials in block order — deterministic — and advances the
+// SolveScalars record the next kernel reads.  The host polls the record
+// every check_every iterations.  MIXED adds, every refine_every iterations, a
+// reliable update: the FP64 true residual b - A (x64 + x) replaces r (beta from
+// it), and the next update folds the FP32 correction x into the FP64 x64.
+// Blocks own contiguous cell chunks, mapped so each XCD (blocks are dispatched
+// round-robin over the 8 XCDs) walks one contiguous slab.
 #include "mpf_internal.h"
 
 namespace mpf {
 
 namespace {
 
-constexpr int PB = 256;        // threads per block
-constexpr int MAX_BLK = 2048;  // blocks per solver launch
+constexpr int PB = 512;        // threads per block
+constexpr int MAX_BLK = 1024;  // blocks per solver launch
 constexpr int OP_UNIT = 0, OP_VARDENS = 1, OP_HEAT = 2;
+constexpr uint8_t NB_UNKNOWN = 64, NB_SLOW = 128;
 
 struct OpArgs {
   Grid g;
   const int8_t* mark;
   const float* ls;
+  const uint8_t* nbm;
   double il, ia;    // 1/rho_liquid, 1/rho_air
   double hal[6];    // h^2/(alpha*dt) per cell type (heat)
   int solid_c;      // SOLID cells Neumann (pressure)
   int vac_neumann;  // VACUUM neighbours Neumann (heat)
+  int precond;      // Jacobi preconditioner on
 };
 
 struct Launch {
@@ -48,9 +61,21 @@ __device__ __forceinline__ bool is_unknown(int OPk, int8_t m) {
 __device__ __forceinline__ int8_t mark_at(const Grid& g, const int8_t* mark, int i, int j, int k) {
   return valid(g, i, j, k) ? mark[cid(g, i, j, k)] : BOUNDARY;
 }
+__device__ __forceinline__ bool neumann(const OpArgs& o, int OPk, int8_t m) {
+  if (OPk == OP_HEAT) return m == BOUNDARY || (o.vac_neumann && m == VACUUM);
+  return m == BOUNDARY || (o.solid_c && m == SOLID);
+}
 
-// 1/rho of the face between cells ca (an unknown) and cb; FLUID|AIR faces
-// use theta of the lower-index cell so the operator is exactly symmetric.
+__constant__ int c_off[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+
+// Neighbour-cell index offsets in the flat array, in the order of c_off.
+__device__ __forceinline__ void nb_strides(const Grid& g, int64_t (&st)[6]) {
+  const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz;
+  st[0] = sx; st[1] = sy; st[2] = 1; st[3] = -sx; st[4] = -sy; st[5] = -1;
+}
+
+// 1/rho of the face between cells ca (an unknown) and cb; a FLUID|AIR face
+// uses theta of the lower-index cell so the operator is exactly symmetric.
 template <class A>
 __device__ __forceinline__ A face_beta(const OpArgs& o, int64_t ca, int64_t cb, int8_t ma, int8_t mb) {
   const A il = (A)o.il, ia = (A)o.ia;
@@ -65,88 +90,131 @@ __device__ __forceinline__ A face_beta(const OpArgs& o, int64_t ca, int64_t cb, 
   return bhi + th * (blo - bhi);
 }
 
-// (A x)[idx] in arithmetic A, x stored as X.  Neighbour order +x,+y,+z,-x,-y,-z
-// as computeAx_bubble.
+// Neighbour mask of one cell.
+template <int OPk>
+__global__ void k_nbm(OpArgs o, uint8_t* __restrict__ nbm) {
+  const Grid& g = o.g;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= g.G) return;
+  int i, j, k;
+  ijk_of(g, idx, i, j, k);
+  const int8_t m = o.mark[idx];
+  uint8_t b = 0;
+  if (is_unknown(OPk, m)) b |= NB_UNKNOWN;
+  bool slow = (OPk == OP_VARDENS) && m == AIR;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    int8_t mn = mark_at(g, o.mark, i + c_off[q][0], j + c_off[q][1], k + c_off[q][2]);
+    if (!neumann(o, OPk, mn)) b |= (uint8_t)(1u << q);
+    if (OPk == OP_VARDENS && m == FLUID && mn == AIR) slow = true;
+  }
+  if (OPk == OP_HEAT) slow = true;  // the heat operator reads the flags (diffusivity per type)
+  if (slow) b |= NB_SLOW;
+  nbm[idx] = b;
+}
+
+// Generic operator row (flags and lsmerge), arithmetic A, x stored as X.
 template <class A, int OPk, class X>
-__device__ __forceinline__ A apply_op(const OpArgs& o, const X* __restrict__ x, int64_t idx) {
+__device__ __forceinline__ A apply_generic(const OpArgs& o, const X* __restrict__ x, int64_t idx) {
   const Grid& g = o.g;
   const int8_t m = o.mark[idx];
   if (!is_unknown(OPk, m)) return A(0);
   int i, j, k;
   ijk_of(g, idx, i, j, k);
   const A xc = (A)x[idx];
-  const int off[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
-  if (OPk == OP_UNIT) {
-    A s = A(-6) * xc;
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      int8_t mn = mark_at(g, o.mark, i + off[q][0], j + off[q][1], k + off[q][2]);
-      bool neu = mn == BOUNDARY || (o.solid_c && mn == SOLID);
-      A xn = (mn == FLUID || mn == AIR) ? (A)x[cid(g, i + off[q][0], j + off[q][1], k + off[q][2])] : A(0);
-      s += neu ? xc : xn;
-    }
-    return s / ((A)g.h * (A)g.h);
-  } else if (OPk == OP_VARDENS) {
-    A s = A(0);
-#pragma unroll
-    for (int q = 0; q < 6; ++q) {
-      int8_t mn = mark_at(g, o.mark, i + off[q][0], j + off[q][1], k + off[q][2]);
-      if (mn == BOUNDARY || (o.solid_c && mn == SOLID)) continue;
-      int64_t nb = cid(g, i + off[q][0], j + off[q][1], k + off[q][2]);
-      A xn = (mn == FLUID || mn == AIR) ? (A)x[nb] : A(0);
-      s += face_beta<A>(o, idx, nb, m, mn) * (xn - xc);
-    }
-    return s / ((A)g.h * (A)g.h);
-  } else {
+  if (OPk == OP_HEAT) {
     A s = ((A)o.hal[m] + A(6)) * xc;
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-      int8_t mn = mark_at(g, o.mark, i + off[q][0], j + off[q][1], k + off[q][2]);
-      bool neu = mn == BOUNDARY || (o.vac_neumann && mn == VACUUM);
-      s -= neu ? xc : (A)x[cid(g, i + off[q][0], j + off[q][1], k + off[q][2])];
+      int8_t mn = mark_at(g, o.mark, i + c_off[q][0], j + c_off[q][1], k + c_off[q][2]);
+      s -= neumann(o, OPk, mn) ? xc : (A)x[cid(g, i + c_off[q][0], j + c_off[q][1], k + c_off[q][2])];
     }
     return s;
   }
+  A s = A(0);
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    int8_t mn = mark_at(g, o.mark, i + c_off[q][0], j + c_off[q][1], k + c_off[q][2]);
+    if (neumann(o, OPk, mn)) continue;
+    int64_t nb = cid(g, i + c_off[q][0], j + c_off[q][1], k + c_off[q][2]);
+    A xn = (mn == FLUID || mn == AIR) ? (A)x[nb] : A(0);
+    A be = (OPk == OP_UNIT) ? A(1) : face_beta<A>(o, idx, nb, m, mn);
+    s += be * (xn - xc);
+  }
+  return s / ((A)g.h * (A)g.h);
 }
 
-// Inverse Jacobi diagonal (1 for non-unknowns or when unpreconditioned).
 template <class A, int OPk>
-__device__ __forceinline__ A inv_diag(const OpArgs& o, int64_t idx, int precond) {
+__device__ __forceinline__ A inv_diag_generic(const OpArgs& o, int64_t idx) {
   const Grid& g = o.g;
   const int8_t m = o.mark[idx];
-  if (!precond || !is_unknown(OPk, m)) return A(1);
   int i, j, k;
   ijk_of(g, idx, i, j, k);
-  const int off[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
   A s = A(0);
   if (OPk == OP_HEAT) {
     s = (A)o.hal[m] + A(6);
 #pragma unroll
     for (int q = 0; q < 6; ++q) {
-      int8_t mn = mark_at(g, o.mark, i + off[q][0], j + off[q][1], k + off[q][2]);
-      if (mn == BOUNDARY || (o.vac_neumann && mn == VACUUM)) s -= A(1);
+      int8_t mn = mark_at(g, o.mark, i + c_off[q][0], j + c_off[q][1], k + c_off[q][2]);
+      if (neumann(o, OPk, mn)) s -= A(1);
     }
     return A(1) / s;
   }
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
-    int8_t mn = mark_at(g, o.mark, i + off[q][0], j + off[q][1], k + off[q][2]);
-    if (mn == BOUNDARY || (o.solid_c && mn == SOLID)) continue;
-    s += (OPk == OP_UNIT) ? A(1) : face_beta<A>(o, idx, cid(g, i + off[q][0], j + off[q][1], k + off[q][2]), m, mn);
+    int8_t mn = mark_at(g, o.mark, i + c_off[q][0], j + c_off[q][1], k + c_off[q][2]);
+    if (neumann(o, OPk, mn)) continue;
+    s += (OPk == OP_UNIT) ? A(1)
+                          : face_beta<A>(o, idx, cid(g, i + c_off[q][0], j + c_off[q][1], k + c_off[q][2]), m, mn);
   }
   if (s == A(0)) return A(1);
   return -((A)g.h * (A)g.h) / s;
 }
 
-// ---- deterministic grid reduction with last-block finalisation ----
+// 1/diag from the mask byte (fast) or the flags (slow); 1 off the unknowns or
+// without preconditioning.
+template <class A, int OPk>
+__device__ __forceinline__ A inv_diag(const OpArgs& o, int64_t idx, uint8_t b) {
+  if (!o.precond || !(b & NB_UNKNOWN)) return A(1);
+  if (b & NB_SLOW) return inv_diag_generic<A, OPk>(o, idx);
+  const int n = __popc((unsigned)(b & 63));
+  if (n == 0) return A(1);
+  const A be = (OPk == OP_UNIT) ? A(1) : (A)o.il;
+  return -((A)o.g.h * (A)o.g.h) / ((A)n * be);
+}
+
+// (A x)[idx] with the mask byte: fast path for liquid cells of uniform density.
+template <class A, int OPk, class X>
+__device__ __forceinline__ A apply_op(const OpArgs& o, const X* __restrict__ x, int64_t idx, uint8_t b) {
+  if (!(b & NB_UNKNOWN)) return A(0);
+  if (b & NB_SLOW) return apply_generic<A, OPk, X>(o, x, idx);
+  int64_t st[6];
+  nb_strides(o.g, st);
+  const A xc = (A)x[idx];
+  A s = A(0);
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+    if (b & (1u << q)) s += (A)x[idx + st[q]] - xc;
+  const A be = (OPk == OP_UNIT) ? A(1) : (A)o.il;
+  return (be * s) / ((A)o.g.h * (A)o.g.h);
+}
+
+// ---- deterministic grid reductions, reduced by the consumer ----
+// A kernel writes one partial per block (plain stores, no fences or atomics);
+// the next kernel's blocks each sum all partials in the same fixed order, so
+// every block derives the same scalars (alpha, beta, convergence) bit for bit.
+// Kernel boundaries make the partials visible across XCDs.  Block 0 also
+// records the scalars for the host to poll.
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
   return v;
 }
 
+// Block-wide sum of N values; the result is returned in every thread.
 template <int N>
-__device__ __forceinline__ void block_sum(double (&v)[N], double (*lds)[N]) {
+__device__ __forceinline__ void block_sum_all(double (&v)[N]) {
+  __shared__ double lds[PB / 64][N];
   const int lane = __lane_id(), w = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < N; ++q) v[q] = wave_sum(v[q]);
@@ -154,42 +222,34 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double (*lds)[N]) {
 #pragma unroll
     for (int q = 0; q < N; ++q) lds[w][q] = v[q];
   __syncthreads();
-  if (threadIdx.x == 0)
 #pragma unroll
-    for (int q = 0; q < N; ++q) v[q] = lds[0][q] + lds[1][q] + lds[2][q] + lds[3][q];
+  for (int q = 0; q < N; ++q) {
+    double s = 0.0;
+#pragma unroll
+    for (int ww = 0; ww < PB / 64; ++ww) s += lds[ww][q];
+    v[q] = s;
+  }
   __syncthreads();
 }
 
-// Writes this block's partial sums; returns true in every thread of the last
-// block, with tot[] holding the grid totals (summed in block order).
+// This block's partial sums -> P[blockIdx.x * N + q].
 template <int N>
-__device__ bool grid_sum(double (&v)[N], double* __restrict__ partials, uint32_t* counter, double (&tot)[N]) {
-  __shared__ double lds[PB / 64][N];
-  __shared__ int is_last;
-  block_sum<N>(v, lds);
-  if (threadIdx.x == 0) {
+__device__ __forceinline__ void write_partials(double (&v)[N], double* __restrict__ P) {
+  block_sum_all<N>(v);
+  if (threadIdx.x == 0)
 #pragma unroll
-    for (int q = 0; q < N; ++q) partials[(int64_t)blockIdx.x * N + q] = v[q];
-    __threadfence();
-    uint32_t old = atomicAdd(counter, 1u);
-    is_last = (old == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!is_last) return false;
-  __threadfence();
-  double s[N];
+    for (int q = 0; q < N; ++q) P[(int64_t)blockIdx.x * N + q] = v[q];
+}
+
+// Sum of the previous kernel's partials (nb blocks), identical in every block.
+template <int N>
+__device__ __forceinline__ void read_partials(const double* __restrict__ P, int nb, double (&out)[N]) {
 #pragma unroll
-  for (int q = 0; q < N; ++q) s[q] = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += PB)
+  for (int q = 0; q < N; ++q) out[q] = 0.0;
+  for (int b = threadIdx.x; b < nb; b += PB)
 #pragma unroll
-    for (int q = 0; q < N; ++q) s[q] += __builtin_nontemporal_load(&partials[(int64_t)b * N + q]);
-  block_sum<N>(s, lds);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int q = 0; q < N; ++q) tot[q] = s[q];
-    *counter = 0u;
-  }
-  return true;
+    for (int q = 0; q < N; ++q) out[q] += P[(int64_t)b * N + q];
+  block_sum_all<N>(out);
 }
 
 __device__ __forceinline__ int64_t chunk_begin(const Launch& L) {
@@ -206,195 +266,260 @@ __device__ __forceinline__ int64_t chunk_begin(const Launch& L) {
 
 __device__ __forceinline__ bool bad(double v) { return !(v == v) || v == INFINITY || v == -INFINITY; }
 
-// r = b (unknowns), x = 0, dinv, z = r*dinv, p = z; sums r.z, r.r, b.b.
+// Partial-sum buffers (device memory, doubles):
+//   U[2][nblk][2]  (r.z, r.r) written by update(k) into slot k&1; init writes slot 1
+//   S[nblk]        p.q written by spmv(k)
+struct Parts {
+  double* U0;
+  double* U1;
+  double* S;
+};
+__device__ __forceinline__ double* uslot(const Parts& P, int k) { return (k & 1) ? P.U1 : P.U0; }
+
+// r = b (unknowns), x = 0 (and x64 = 0), both p buffers 0; partials r.z, r.r into U slot 1.
 template <class V, class Dacc, int OPk>
 __global__ void __launch_bounds__(PB) k_init(OpArgs o, Launch L, const V* __restrict__ b, V* __restrict__ x,
-                                             V* __restrict__ r, V* __restrict__ p, V* __restrict__ dinv,
-                                             double* __restrict__ x64, int precond, double* partials,
-                                             uint32_t* counter, SolveScalars* sc, int ref, double eps_abs,
-                                             double tol) {
-  Dacc a_rz = 0, a_rr = 0, a_bb = 0;
+                                             V* __restrict__ r, V* __restrict__ pa, V* __restrict__ pb,
+                                             double* __restrict__ x64, Parts P, SolveScalars* sc) {
+  Dacc a_rz = 0, a_rr = 0;
   CELLS_OF_BLOCK(o.g.G) {
-    bool u = is_unknown(OPk, o.mark[idx]);
-    V bv = u ? b[idx] : V(0);
-    V di = inv_diag<V, OPk>(o, idx, precond);
-    V z = bv * di;
+    const uint8_t nb = o.nbm[idx];
+    V bv = (nb & NB_UNKNOWN) ? b[idx] : V(0);
+    V z = bv * inv_diag<V, OPk>(o, idx, nb);
     x[idx] = V(0);
     r[idx] = bv;
-    p[idx] = z;
-    dinv[idx] = di;
+    pa[idx] = V(0);
+    pb[idx] = V(0);
     if (x64) x64[idx] = 0.0;
     a_rz += (Dacc)bv * (Dacc)z;
     a_rr += (Dacc)bv * (Dacc)bv;
-    a_bb += (Dacc)bv * (Dacc)bv;
   }
-  double v[3] = {(double)a_rz, (double)a_rr, (double)a_bb}, t[3];
-  if (grid_sum<3>(v, partials, counter, t) && threadIdx.x == 0) {
-    sc->rz = t[0];
-    sc->rr = t[1];
-    sc->bb = t[2];
+  double v[2] = {(double)a_rz, (double)a_rr};
+  write_partials<2>(v, uslot(P, 1));
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->done = 0;
+    sc->converged = 0;
+    sc->nonfinite = 0;
     sc->iter = 0;
     sc->refinements = 0;
-    sc->refine_now = 0;
-    sc->nonfinite = bad(t[0]) || bad(t[1]);
-    bool conv = ref ? (t[1] < eps_abs) : (t[2] == 0.0 || sqrt(t[1]) <= tol * sqrt(t[2]));
-    sc->converged = conv;
-    sc->done = conv || sc->nonfinite;
   }
 }
 
-// q = A p ; p.q ; alpha = rz / pq
+// Iteration k.  From update(k-1): rz_k, rr_k (slot (k-1)&1) and rz_{k-1}
+// (slot k&1) -> convergence, beta.  Then p_k = r*dinv + beta p_{k-1} (own cell
+// and neighbours), stored; q = A p_k; partial p.q.
 template <class V, class Dacc, int OPk>
-__global__ void __launch_bounds__(PB) k_spmv(OpArgs o, Launch L, const V* __restrict__ p, V* __restrict__ q,
-                                             double* partials, uint32_t* counter, SolveScalars* sc) {
+__global__ void __launch_bounds__(PB) k_spmv(OpArgs o, Launch L, const V* __restrict__ r, const V* __restrict__ pp,
+                                             V* __restrict__ pc, V* __restrict__ q, Parts P, SolveScalars* sc, int k,
+                                             int ref, double eps_abs, double tol, int cap) {
   if (sc->done) return;
-  Dacc a_pq = 0;
-  CELLS_OF_BLOCK(o.g.G) {
-    V qv = apply_op<V, OPk, V>(o, p, idx);
-    q[idx] = qv;
-    a_pq += (Dacc)p[idx] * (Dacc)qv;
-  }
-  double v[1] = {(double)a_pq}, t[1];
-  if (grid_sum<1>(v, partials, counter, t) && threadIdx.x == 0) {
-    sc->pq = t[0];
-    if (t[0] == 0.0) {
+  double cur[2], prv[2];
+  read_partials<2>(uslot(P, k - 1), L.nblk, cur);
+  if (k > 0) read_partials<2>(uslot(P, k), L.nblk, prv);
+  else prv[0] = prv[1] = 0.0;
+  const double bb = (k == 0) ? cur[1] : sc->bb;
+  const bool nonfin = bad(cur[0]) || bad(cur[1]);
+  const bool conv = ref ? (cur[1] < eps_abs) : (cur[1] == 0.0 || sqrt(cur[1]) <= tol * sqrt(bb));
+  if (conv || nonfin || k >= cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->done = 1;
-      sc->converged = 1;
-    } else if (bad(t[0])) {
-      sc->done = 1;
-      sc->nonfinite = 1;
-    } else {
-      sc->alpha = sc->rz / t[0];
+      sc->converged = conv && !nonfin;
+      sc->nonfinite = nonfin;
+      sc->iter = k;
+      sc->rr = cur[1];
+      sc->rz = cur[0];
+      if (k == 0) sc->bb = bb;
     }
+    return;
   }
+  const V be = (k == 0) ? V(0) : (V)(cur[0] / prv[0]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rz = cur[0];
+    sc->rr = cur[1];
+    sc->iter = k;
+    if (k == 0) sc->bb = bb;
+  }
+  const Grid& g = o.g;
+  int64_t st[6];
+  nb_strides(g, st);
+  Dacc a_pq = 0;
+  CELLS_OF_BLOCK(g.G) {
+    const uint8_t nb = o.nbm[idx];
+    if (!(nb & NB_UNKNOWN)) continue;
+    const V pcv = r[idx] * inv_diag<V, OPk>(o, idx, nb) + be * pp[idx];
+    pc[idx] = pcv;
+    V qv;
+    if (!(nb & NB_SLOW)) {
+      V s = V(0);
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        if (!(nb & (1u << f))) continue;
+        const int64_t n = idx + st[f];
+        const uint8_t nn = o.nbm[n];
+        const V pn = r[n] * inv_diag<V, OPk>(o, n, nn) + be * pp[n];
+        s += pn - pcv;
+      }
+      const V bc = (OPk == OP_UNIT) ? V(1) : (V)o.il;
+      qv = (bc * s) / ((V)g.h * (V)g.h);
+    } else {
+      // slow row: the operator on p_k with neighbours rebuilt the same way
+      int i, j, kk;
+      ijk_of(g, idx, i, j, kk);
+      const int8_t m = o.mark[idx];
+      V s = V(0);
+      if (OPk == OP_HEAT) s = ((V)o.hal[m] + V(6)) * pcv;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const int8_t mn = mark_at(g, o.mark, i + c_off[f][0], j + c_off[f][1], kk + c_off[f][2]);
+        if (neumann(o, OPk, mn)) {
+          if (OPk == OP_HEAT) s -= pcv;
+          continue;
+        }
+        const int64_t n = idx + st[f];
+        const uint8_t nn = o.nbm[n];
+        const V pn = (nn & NB_UNKNOWN) ? r[n] * inv_diag<V, OPk>(o, n, nn) + be * pp[n] : V(0);
+        if (OPk == OP_HEAT) s -= pn;
+        else {
+          const V bf = (OPk == OP_UNIT) ? V(1) : face_beta<V>(o, idx, n, m, mn);
+          s += bf * (pn - pcv);
+        }
+      }
+      qv = (OPk == OP_HEAT) ? s : s / ((V)g.h * (V)g.h);
+    }
+    q[idx] = qv;
+    a_pq += (Dacc)pcv * (Dacc)qv;
+  }
+  double v[1] = {(double)a_pq};
+  write_partials<1>(v, P.S);
 }
 
-// x += alpha p ; r -= alpha q ; z = r*dinv ; sums r.z, r.r ; convergence.
+// x += alpha p_k (folding x into x64 right after a reliable update);
+// r -= alpha q; partials r.z, r.r into U slot k&1.
 template <class V, class Dacc, int OPk>
 __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict__ x, V* __restrict__ r,
-                                               const V* __restrict__ p, const V* __restrict__ q,
-                                               const V* __restrict__ dinv, double* partials, uint32_t* counter,
-                                               SolveScalars* sc, int ref, double eps_abs, double tol, int cap,
-                                               int mixed, int refine_every) {
+                                               const V* __restrict__ pc, const V* __restrict__ q,
+                                               double* __restrict__ x64, Parts P, SolveScalars* sc, int k, int fold) {
   if (sc->done) return;
-  const V al = (V)sc->alpha;
+  double t[1];
+  read_partials<1>(P.S, L.nblk, t);
+  if (t[0] == 0.0 || bad(t[0])) {  // spray.cu:1654-1656: zp == 0 ends the solve
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sc->done = 1;
+      sc->converged = t[0] == 0.0;
+      sc->nonfinite = bad(t[0]);
+      sc->iter = k;
+    }
+    return;
+  }
+  const V al = (V)(sc->rz / t[0]);
   Dacc a_rz = 0, a_rr = 0;
   CELLS_OF_BLOCK(o.g.G) {
-    if (!is_unknown(OPk, o.mark[idx])) continue;
-    V xv = x[idx] + al * p[idx];
-    V rv = r[idx] - al * q[idx];
-    x[idx] = xv;
+    const uint8_t nb = o.nbm[idx];
+    if (!(nb & NB_UNKNOWN)) continue;
+    const V pv = pc[idx];
+    if (fold) {
+      x64[idx] += (double)(x[idx] + al * pv);
+      x[idx] = V(0);
+    } else {
+      x[idx] += al * pv;
+    }
+    const V rv = r[idx] - al * q[idx];
     r[idx] = rv;
-    V z = rv * dinv[idx];
+    const V z = rv * inv_diag<V, OPk>(o, idx, nb);
     a_rz += (Dacc)rv * (Dacc)z;
     a_rr += (Dacc)rv * (Dacc)rv;
   }
-  double v[2] = {(double)a_rz, (double)a_rr}, t[2];
-  if (grid_sum<2>(v, partials, counter, t) && threadIdx.x == 0) {
-    int it = sc->iter + 1;
-    sc->iter = it;
-    sc->rr = t[1];
-    if (bad(t[0]) || bad(t[1])) {
-      sc->done = 1;
-      sc->nonfinite = 1;
-      return;
-    }
-    bool conv = ref ? (t[1] < eps_abs) : (sqrt(t[1]) <= tol * sqrt(sc->bb));
-    double rzp = sc->rz;
-    sc->rz_new = rzp;  // rz_new holds the previous r.z here (reliable-update beta)
-    sc->rz = t[0];
-    sc->beta = t[0] / rzp;
-    if (mixed) {
-      // the FP64 true residual decides convergence and the cap
-      if (conv || (refine_every > 0 && it % refine_every == 0) || it >= cap) sc->refine_now = 1;
-    } else {
-      if (conv) {
-        sc->done = 1;
-        sc->converged = 1;
-      } else if (it >= cap) {
-        sc->done = 1;
-      }
-    }
-  }
+  double v[2] = {(double)a_rz, (double)a_rr};
+  write_partials<2>(v, uslot(P, k));
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->iter = k + 1;
 }
 
-// MIXED: x64 += x ; x = 0 (only when a refinement is due)
-__global__ void __launch_bounds__(PB) k_fold(Launch L, int64_t G, float* __restrict__ x, double* __restrict__ x64,
-                                             const SolveScalars* sc, int force) {
-  if (!force && (sc->done || !sc->refine_now)) return;
+// MIXED reliable update after update(k): r = b - A (x64 + x) in FP64, stored
+// to the FP32 residual; (r.z, |r|^2) overwrite U slot k&1, so spmv(k+1)
+// takes beta and the convergence test from the true residual.
+template <int OPk>
+__global__ void __launch_bounds__(PB) k_resid64(OpArgs o, Launch L, const float* __restrict__ b,
+                                                const double* __restrict__ x64, const float* __restrict__ x,
+                                                float* __restrict__ r, Parts P, SolveScalars* sc, int k) {
+  if (sc->done) return;
+  const Grid& g = o.g;
+  int64_t st[6];
+  nb_strides(g, st);
+  double a_rz = 0, a_rr = 0;
+  CELLS_OF_BLOCK(g.G) {
+    const uint8_t nb = o.nbm[idx];
+    if (!(nb & NB_UNKNOWN)) continue;
+    double ax;
+    if (!(nb & NB_SLOW)) {
+      const double xc = x64[idx] + (double)x[idx];
+      double s = 0.0;
+#pragma unroll
+      for (int f = 0; f < 6; ++f)
+        if (nb & (1u << f)) s += (x64[idx + st[f]] + (double)x[idx + st[f]]) - xc;
+      const double bc = (OPk == OP_UNIT) ? 1.0 : o.il;
+      ax = (bc * s) / ((double)g.h * (double)g.h);
+    } else {
+      int i, j, kk;
+      ijk_of(g, idx, i, j, kk);
+      const int8_t m = o.mark[idx];
+      const double xc = x64[idx] + (double)x[idx];
+      double s = 0.0;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const int8_t mn = mark_at(g, o.mark, i + c_off[f][0], j + c_off[f][1], kk + c_off[f][2]);
+        if (neumann(o, OPk, mn)) continue;
+        const int64_t n = idx + st[f];
+        const double xn = (mn == FLUID || mn == AIR) ? x64[n] + (double)x[n] : 0.0;
+        const double bf = (OPk == OP_UNIT) ? 1.0 : face_beta<double>(o, idx, n, m, mn);
+        s += bf * (xn - xc);
+      }
+      ax = s / ((double)g.h * (double)g.h);
+    }
+    const double rv = (double)b[idx] - ax;
+    const float rf = (float)rv;
+    r[idx] = rf;
+    a_rr += rv * rv;
+    a_rz += (double)rf * (double)(rf * inv_diag<float, OPk>(o, idx, nb));
+  }
+  double v[2] = {a_rz, a_rr};
+  write_partials<2>(v, uslot(P, k));
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->refinements += 1;
+}
+
+// MIXED: x64 += x at exit.
+__global__ void __launch_bounds__(PB) k_final_fold(Launch L, int64_t G, const uint8_t* __restrict__ nbm,
+                                                   float* __restrict__ x, double* __restrict__ x64) {
   CELLS_OF_BLOCK(G) {
+    if (!(nbm[idx] & NB_UNKNOWN)) continue;
     x64[idx] += (double)x[idx];
     x[idx] = 0.0f;
   }
 }
 
-// MIXED: r = b - A x64 in FP64, stored to the FP32 residual; r.z and |r|^2 in
-// FP64; convergence on the true residual; reliable-update beta.
-template <int OPk>
-__global__ void __launch_bounds__(PB) k_resid64(OpArgs o, Launch L, const float* __restrict__ b,
-                                                const double* __restrict__ x64, float* __restrict__ r,
-                                                const float* __restrict__ dinv, double* partials, uint32_t* counter,
-                                                SolveScalars* sc, double tol, int cap) {
-  if (sc->done || !sc->refine_now) return;
-  double a_rz = 0, a_rr = 0;
-  CELLS_OF_BLOCK(o.g.G) {
-    bool u = is_unknown(OPk, o.mark[idx]);
-    double rv = u ? ((double)b[idx] - apply_op<double, OPk, double>(o, x64, idx)) : 0.0;
-    float rf = (float)rv;
-    r[idx] = rf;
-    a_rr += rv * rv;
-    a_rz += (double)rf * (double)(rf * dinv[idx]);
-  }
-  double v[2] = {a_rz, a_rr}, t[2];
-  if (grid_sum<2>(v, partials, counter, t) && threadIdx.x == 0) {
-    sc->refine_now = 0;
-    sc->refinements += 1;
-    sc->rr_true = t[1];
-    sc->rr = t[1];
-    if (bad(t[0]) || bad(t[1])) {
-      sc->done = 1;
-      sc->nonfinite = 1;
-      return;
-    }
-    if (sqrt(t[1]) <= tol * sqrt(sc->bb)) {
-      sc->done = 1;
-      sc->converged = 1;
-    } else if (sc->iter >= cap) {
-      sc->done = 1;
-    } else {
-      sc->rz = t[0];
-      sc->beta = t[0] / sc->rz_new;
-    }
-  }
-}
-
-// p = r*dinv + beta p
-template <class V>
-__global__ void __launch_bounds__(PB) k_dir(Launch L, int64_t G, const V* __restrict__ r, const V* __restrict__ dinv,
-                                            V* __restrict__ p, const SolveScalars* sc) {
-  if (sc->done) return;
-  const V be = (V)sc->beta;
-  CELLS_OF_BLOCK(G) { p[idx] = r[idx] * dinv[idx] + be * p[idx]; }
-}
-
-// |b - A x|^2 in FP64 for the report.
+// |b - A x|^2 in FP64 for the report (partials into S).
 template <int OPk, class B, class X>
 __global__ void __launch_bounds__(PB) k_true_resid(OpArgs o, Launch L, const B* __restrict__ b, const X* __restrict__ x,
-                                                   double* partials, uint32_t* counter, SolveScalars* sc) {
+                                                   Parts P) {
   double a_rr = 0;
   CELLS_OF_BLOCK(o.g.G) {
-    bool u = is_unknown(OPk, o.mark[idx]);
-    double rv = u ? ((double)b[idx] - apply_op<double, OPk, X>(o, x, idx)) : 0.0;
+    const uint8_t nb = o.nbm[idx];
+    if (!(nb & NB_UNKNOWN)) continue;
+    double rv = (double)b[idx] - apply_op<double, OPk, X>(o, x, idx, nb);
     a_rr += rv * rv;
   }
-  double v[1] = {a_rr}, t[1];
-  if (grid_sum<1>(v, partials, counter, t) && threadIdx.x == 0) sc->rr_true = t[0];
+  double v[1] = {a_rr};
+  write_partials<1>(v, P.S);
 }
 
-template <class V>
-__global__ void k_copy_cast(int64_t n, const V* __restrict__ in, double* __restrict__ out) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (double)in[i];
+// Sum of the S partials into sc->rr_true (one block).
+__global__ void __launch_bounds__(PB) k_collect_rr(const double* __restrict__ S, int nb, SolveScalars* sc) {
+  double t[1];
+  read_partials<1>(S, nb, t);
+  if (threadIdx.x == 0) sc->rr_true = t[0];
+}
+
+__global__ void k_clear_done(SolveScalars* sc) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) sc->done = 0;
 }
 
 Launch make_launch(int64_t G) {
@@ -408,11 +533,12 @@ Launch make_launch(int64_t G) {
   return L;
 }
 
-OpArgs make_op(Ctx& c) {
+OpArgs make_op(Ctx& c, int precond) {
   OpArgs o;
   o.g = c.g;
   o.mark = c.mark;
   o.ls = c.lsmerge;
+  o.nbm = c.nbm;
   o.il = 1.0 / (double)c.cfg.rho_liquid;
   o.ia = 1.0 / (double)c.cfg.rho_air;
   const double h2 = (double)c.g.h * (double)c.g.h;
@@ -420,6 +546,7 @@ OpArgs make_op(Ctx& c) {
     o.hal[t] = c.cfg.heat_alpha[t] > 0 ? h2 / (double)c.cfg.heat_alpha[t] / (double)c.cfg.dt : 0.0;
   o.solid_c = c.cfg.solid_constraint;
   o.vac_neumann = c.cfg.heat_vacuum_neumann;
+  o.precond = precond;
   return o;
 }
 
@@ -430,51 +557,68 @@ bool read_scalars(Ctx& c) {
 
 // Generic (P)CG driver.  V: vector type, Dacc: dot accumulation type.
 // b: right-hand side (type V), x: solution (type V; MIXED adds x64).
+// MIXED: after every refine_every iterations the FP64 true residual replaces
+// the recurrence residual; convergence is accepted only when the residual
+// that met the criterion was a true one.
 template <class V, class Dacc, int OPk>
 int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int cap, double tol, bool mixed,
              mpf_solve_info* info) {
   const int64_t G = c.g.G;
   const Launch L = make_launch(G);
-  const OpArgs o = make_op(c);
+  const OpArgs o = make_op(c, precond);
   V* r = (V*)c.vr;
-  V* p = (V*)c.vp;
+  V* pbuf[2] = {(V*)c.vp, (V*)c.vdinv};
   V* q = (V*)c.vq;
-  V* dinv = (V*)c.vdinv;
-  uint32_t* ctr = c.counters;
+  Parts P{c.partials, c.partials + 2 * MAX_BLK, c.partials + 4 * MAX_BLK};
   const double eps_abs = 1e-2 * (double)G;
-  const int refine_every = c.cfg.refine_every;
-  const int check = c.cfg.check_every > 0 ? c.cfg.check_every : 10;
-  launch(c, "pcg_init", dim3(L.nblk), dim3(PB), 0, k_init<V, Dacc, OPk>, o, L, b, x, r, p, dinv,
-         mixed ? x64 : (double*)nullptr, precond, c.partials, ctr, c.scal, (int)ref, eps_abs, tol);
-  if (!read_scalars(c)) return MPF_HIP_ERROR;
-  int launched = 0;
-  while (!c.scal_host->done && launched < cap + 1) {
-    int n = check;
-    for (int s = 0; s < n; ++s) {
-      launch(c, "pcg_spmv", dim3(L.nblk), dim3(PB), 0, k_spmv<V, Dacc, OPk>, o, L, (const V*)p, q, c.partials, ctr,
-             c.scal);
-      launch(c, "pcg_update", dim3(L.nblk), dim3(PB), 0, k_update<V, Dacc, OPk>, o, L, x, r, (const V*)p,
-             (const V*)q, (const V*)dinv, c.partials, ctr, c.scal, (int)ref, eps_abs, tol, cap, (int)mixed,
-             refine_every);
-      if (mixed) {
-        launch(c, "pcg_fold", dim3(L.nblk), dim3(PB), 0, k_fold, L, G, (float*)x, x64, (const SolveScalars*)c.scal, 0);
-        launch(c, "pcg_resid64", dim3(L.nblk), dim3(PB), 0, k_resid64<OPk>, o, L, (const float*)b,
-               (const double*)x64, (float*)r, (const float*)dinv, c.partials, ctr, c.scal, tol, cap);
+  const int R = c.cfg.refine_every > 0 ? c.cfg.refine_every : 20;
+  int check = mixed ? R : c.cfg.check_every;
+  if (check <= 0) check = 16;
+  launch(c, "pcg_nbm", dim3(nblocks(G, 256)), dim3(256), 0, k_nbm<OPk>, o, c.nbm);
+  launch(c, "pcg_init", dim3(L.nblk), dim3(PB), 0, k_init<V, Dacc, OPk>, o, L, b, x, r, pbuf[0], pbuf[1],
+         mixed ? x64 : (double*)nullptr, P, c.scal);
+  int k = 0;
+  int last_true = -1;   // update index after which the residual was last made true
+  bool fold_next = false;
+  while (k <= cap) {
+    const int k_end = k + check;
+    for (; k < k_end && k <= cap; ++k) {
+      launch(c, "pcg_spmv", dim3(L.nblk), dim3(PB), 0, k_spmv<V, Dacc, OPk>, o, L, (const V*)r,
+             (const V*)pbuf[k & 1], pbuf[(k + 1) & 1], q, P, c.scal, k, (int)ref, eps_abs, tol, cap);
+      launch(c, "pcg_update", dim3(L.nblk), dim3(PB), 0, k_update<V, Dacc, OPk>, o, L, x, r,
+             (const V*)pbuf[(k + 1) & 1], (const V*)q, mixed ? x64 : (double*)nullptr, P, c.scal, k,
+             (int)fold_next);
+      fold_next = false;
+      if (mixed && (k + 1) % R == 0) {
+        launch(c, "pcg_resid64", dim3(L.nblk), dim3(PB), 0, k_resid64<OPk>, o, L, (const float*)b, (const double*)x64,
+               (const float*)x, (float*)r, P, c.scal, k);
+        last_true = k;
+        fold_next = true;
       }
-      launch(c, "pcg_dir", dim3(L.nblk), dim3(PB), 0, k_dir<V>, L, G, (const V*)r, (const V*)dinv, p,
-             (const SolveScalars*)c.scal);
     }
-    launched += n;
     if (!read_scalars(c)) return MPF_HIP_ERROR;
+    const SolveScalars& s = *c.scal_host;
+    if (!s.done) continue;
+    if (!mixed || !s.converged || s.nonfinite) break;
+    // spmv(kd) met the criterion on the residual of update(kd-1)
+    const int kd = s.iter;
+    if (kd == 0 || last_true == kd - 1) break;
+    // resid64 returns early while done is set: clear it first
+    launch(c, "pcg_clear", dim3(1), dim3(64), 0, k_clear_done, c.scal);
+    launch(c, "pcg_resid64", dim3(L.nblk), dim3(PB), 0, k_resid64<OPk>, o, L, (const float*)b, (const double*)x64,
+           (const float*)x, (float*)r, P, c.scal, kd - 1);
+    last_true = kd - 1;
+    fold_next = true;
+    k = kd;
   }
   if (mixed) {
-    launch(c, "pcg_fold", dim3(L.nblk), dim3(PB), 0, k_fold, L, G, (float*)x, x64, (const SolveScalars*)c.scal, 1);
+    launch(c, "pcg_fold", dim3(L.nblk), dim3(PB), 0, k_final_fold, L, G, (const uint8_t*)c.nbm, (float*)x, x64);
     launch(c, "pcg_true_resid", dim3(L.nblk), dim3(PB), 0, k_true_resid<OPk, V, double>, o, L, b,
-           (const double*)x64, c.partials, ctr, c.scal);
+           (const double*)x64, P);
   } else {
-    launch(c, "pcg_true_resid", dim3(L.nblk), dim3(PB), 0, k_true_resid<OPk, V, V>, o, L, b, (const V*)x, c.partials,
-           ctr, c.scal);
+    launch(c, "pcg_true_resid", dim3(L.nblk), dim3(PB), 0, k_true_resid<OPk, V, V>, o, L, b, (const V*)x, P);
   }
+  launch(c, "pcg_collect", dim3(1), dim3(PB), 0, k_collect_rr, (const double*)P.S, L.nblk, c.scal);
   if (!read_scalars(c)) return MPF_HIP_ERROR;
   const SolveScalars& s = *c.scal_host;
   if (info) {

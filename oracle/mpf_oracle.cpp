@@ -21,6 +21,8 @@
 //  * the ceiling rule of setWaterBoundaryU_k reads uz(k-1) after that face's
 //    own rule (sequential semantics; the reference reads it while it may be
 //    rewritten, spray_k.cu:986-989);
+//  * setBoundaryHeat reads the pass's input values (the reference copies in
+//    place, so edge and corner cells read racing neighbours, spray_k.cu:4585-4605);
 //  * dot products accumulate in double (reference: float block sums added on
 //    the host in float, spray.cu:1101-1116).
 #include "mpf_oracle.h"
@@ -1010,16 +1012,17 @@ template <class T> struct Orc : OrcBase {
       rr = rsum(G, [&](int64_t i) { double e = (mark[i] == BOUNDARY) ? 0.0 : b[i] - q[i]; return e * e; });
     }
     for (int64_t i = 0; i < G; ++i) tp[i] = (T)x[i];
-    if (c.heat_extend_boundary) {  // setBoundaryHeat spray_k.cu:4585-4605 (in place, index order)
+    if (c.heat_extend_boundary) {  // setBoundaryHeat spray_k.cu:4585-4605, reading the pass's input
+      std::vector<T> tin = tp;
       for (int64_t idx = 0; idx < G; ++idx) {
         int i, j, k;
         ijk(idx, i, j, k);
-        if (i == NX - 1) tp[idx] = tp[cid(i - 1, j, k)];
-        else if (i == 0) tp[idx] = tp[cid(i + 1, j, k)];
-        else if (j == NY - 1) tp[idx] = tp[cid(i, j - 1, k)];
-        else if (j == 0) tp[idx] = tp[cid(i, j + 1, k)];
-        else if (k == NZ - 1) tp[idx] = tp[cid(i, j, k - 1)];
-        else if (k == 0) tp[idx] = tp[cid(i, j, k + 1)];
+        if (i == NX - 1) tp[idx] = tin[cid(i - 1, j, k)];
+        else if (i == 0) tp[idx] = tin[cid(i + 1, j, k)];
+        else if (j == NY - 1) tp[idx] = tin[cid(i, j - 1, k)];
+        else if (j == 0) tp[idx] = tin[cid(i, j + 1, k)];
+        else if (k == NZ - 1) tp[idx] = tin[cid(i, j, k - 1)];
+        else if (k == 0) tp[idx] = tin[cid(i, j, k + 1)];
       }
     }
     // compTpChange spray_k.cu:4607-4617
