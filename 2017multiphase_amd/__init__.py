@@ -53,6 +53,8 @@ ABI_FUNCTIONS = [
     "mpf_extrapolate_bc", "mpf_project", "mpf_g2p", "mpf_advect", "mpf_heat", "mpf_phase_change",
     "mpf_marching_cubes", "mpf_get_mesh", "mpf_get_mc_cases", "mpf_step",
     "mpf_set_kernel_timing", "mpf_kernel_times", "mpf_reset_kernel_times",
+    "mpf_project_adjoint", "mpf_extrapolate_adjoint", "mpf_p2g_adjoint", "mpf_g2p_adjoint",
+    "mpf_advect_adjoint",
 ]
 
 
@@ -164,6 +166,11 @@ def lib():
     L.mpf_kernel_times.argtypes = [vp, vp, vp, vp, i32]
     L.mpf_kernel_times.restype = i32
     L.mpf_reset_kernel_times.argtypes = [vp]
+    L.mpf_project_adjoint.argtypes = [vp, vp, vp, i32, c.POINTER(SolveInfo)]
+    L.mpf_extrapolate_adjoint.argtypes = [vp, vp, vp, i32]
+    L.mpf_p2g_adjoint.argtypes = [vp, vp, vp, vp, i32]
+    L.mpf_g2p_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, i32]
+    L.mpf_advect_adjoint.argtypes = [vp, vp, vp, vp, vp, i32, c.POINTER(i64)]
     _lib_handle = L
     return L
 
@@ -364,6 +371,84 @@ class Simulator:
         self._check(self.L.mpf_step(self.h, frame, ctypes.byref(info)), allow)
         return {"project": info.project.as_dict(), "heat": info.heat.as_dict(),
                 "phase": info.phase.as_dict(), "mc": info.mc.as_dict(), "particles": info.particles}
+
+    # -- adjoints (FP64 cotangents in host memory) --
+    def face_sizes(self):
+        nx, ny, nz = self.cfg.nx, self.cfg.ny, self.cfg.nz
+        f = [(nx + 1) * ny * nz, nx * (ny + 1) * nz, nx * ny * (nz + 1)]
+        return f + f
+
+    def _faces_in(self, faces):
+        """6 FP64 face arrays (or None = zero) -> (keep-alive list, void*[6] or None)."""
+        if faces is None:
+            return [], None
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).ravel() for a in faces]
+        for a, n in zip(arrs, self.face_sizes()):
+            assert a is None or a.size == n, (a.size, n)
+        tab = (ctypes.c_void_p * 6)(*[None if a is None else a.ctypes.data for a in arrs])
+        return arrs, tab
+
+    def _faces_out(self):
+        arrs = [np.empty(n, dtype=np.float64) for n in self.face_sizes()]
+        return arrs, (ctypes.c_void_p * 6)(*[a.ctypes.data for a in arrs])
+
+    @staticmethod
+    def _vec(a, n):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        assert a.size == n, (a.size, n)
+        return a
+
+    def project_adjoint(self, g_out, allow_no_convergence=False):
+        """g_out: 6 face cotangents of mpf_project's output -> 6 of its input."""
+        keep, tin = self._faces_in(g_out)
+        res, tout = self._faces_out()
+        info = SolveInfo()
+        allow = (OK, NO_CONVERGENCE) if allow_no_convergence else (OK,)
+        self._check(self.L.mpf_project_adjoint(self.h, ctypes.cast(tin, ctypes.c_void_p) if tin else None,
+                                               ctypes.cast(tout, ctypes.c_void_p), HOST, ctypes.byref(info)), allow)
+        return res, info.as_dict()
+
+    def extrapolate_adjoint(self, g_out):
+        keep, tin = self._faces_in(g_out)
+        res, tout = self._faces_out()
+        self._check(self.L.mpf_extrapolate_adjoint(self.h, ctypes.cast(tin, ctypes.c_void_p) if tin else None,
+                                                   ctypes.cast(tout, ctypes.c_void_p), HOST))
+        return res
+
+    def p2g_adjoint(self, g_faces):
+        """-> (g_vel[P,3], g_pos[P,3]) from the cotangents of the P2G output faces."""
+        P = self.particle_count()
+        keep, tin = self._faces_in(g_faces)
+        gv = np.empty((P, 3), dtype=np.float64)
+        gx = np.empty((P, 3), dtype=np.float64)
+        self._check(self.L.mpf_p2g_adjoint(self.h, ctypes.cast(tin, ctypes.c_void_p) if tin else None,
+                                           _ptr(gv), _ptr(gx), HOST))
+        return gv, gx
+
+    def g2p_adjoint(self, g_vel_out):
+        """-> dict(vel, u[6], u_old[6], pos) cotangents of G2P's inputs."""
+        P = self.particle_count()
+        gvo = self._vec(g_vel_out, 3 * P)
+        gv = np.empty((P, 3), dtype=np.float64)
+        gx = np.empty((P, 3), dtype=np.float64)
+        gu, tu = self._faces_out()
+        guo, tuo = self._faces_out()
+        self._check(self.L.mpf_g2p_adjoint(self.h, _ptr(gvo), _ptr(gv), ctypes.cast(tu, ctypes.c_void_p),
+                                           ctypes.cast(tuo, ctypes.c_void_p), _ptr(gx), HOST))
+        return {"vel": gv, "u": gu, "u_old": guo, "pos": gx}
+
+    def advect_adjoint(self, g_pos_out, g_vel_out):
+        """-> (g_pos_in[P,3], g_vel_in[P,3], particles inside the solid band)."""
+        P = self.particle_count()
+        gxo, gvo = self._vec(g_pos_out, 3 * P), self._vec(g_vel_out, 3 * P)
+        gx = np.empty((P, 3), dtype=np.float64)
+        gv = np.empty((P, 3), dtype=np.float64)
+        band = ctypes.c_int64()
+        self._check(self.L.mpf_advect_adjoint(self.h, _ptr(gxo), _ptr(gvo), _ptr(gx), _ptr(gv), HOST,
+                                              ctypes.byref(band)))
+        return gx, gv, int(band.value)
 
     # -- timing --
     def set_kernel_timing(self, on):

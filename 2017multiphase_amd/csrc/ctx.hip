@@ -122,6 +122,7 @@ void free_all(Ctx& c) {
   dfree(c.scan_blk); dfree(c.ctmp_u32);
   dfree(c.mc_case); dfree(c.mc_edge); dfree(c.mc_tcount); dfree(c.mc_verts); dfree(c.mc_tris);
   dfree(c.seeds);
+  adj_free(c);
   if (c.scal_host) hipHostFree(c.scal_host);
   if (c.counters_host) hipHostFree(c.counters_host);
   c.scal_host = nullptr;
@@ -576,6 +577,46 @@ mpf_status mpf_advect(mpf_ctx* ctx) {
   if (c.cfg.integrator != MPF_ADVECT_RK2) REQUIRE_SORTED(c);
   run_advect(c);
   return check_launch(c);
+}
+
+#define ADJ_PRELUDE(ctx)                                                                       \
+  if (!ctx) return MPF_INVALID_ARG;                                                           \
+  Ctx& c = ctx->c;                                                                            \
+  c.err.clear();                                                                              \
+  if (where != MPF_HOST && where != MPF_DEVICE) return fail(c, MPF_INVALID_ARG, "bad memory location");
+
+mpf_status mpf_project_adjoint(mpf_ctx* ctx, const double* const* g_out, double* const* g_in, int32_t where,
+                               mpf_solve_info* info) {
+  ADJ_PRELUDE(ctx)
+  if (!g_in) return fail(c, MPF_INVALID_ARG, "g_in is NULL");
+  int st = run_project_adjoint(c, g_out, g_in, where, info);
+  if (st == MPF_NO_CONVERGENCE) c.err = "adjoint solve stopped at its iteration cap";
+  return (mpf_status)st;
+}
+
+mpf_status mpf_extrapolate_adjoint(mpf_ctx* ctx, const double* const* g_out, double* const* g_in, int32_t where) {
+  ADJ_PRELUDE(ctx)
+  if (!g_in) return fail(c, MPF_INVALID_ARG, "g_in is NULL");
+  return (mpf_status)run_extrapolate_adjoint(c, g_out, g_in, where);
+}
+
+mpf_status mpf_p2g_adjoint(mpf_ctx* ctx, const double* const* g_faces, double* g_vel, double* g_pos,
+                           int32_t where) {
+  ADJ_PRELUDE(ctx)
+  REQUIRE_SORTED(c);
+  return (mpf_status)run_p2g_adjoint(c, g_faces, g_vel, g_pos, where);
+}
+
+mpf_status mpf_g2p_adjoint(mpf_ctx* ctx, const double* g_vel_out, double* g_vel_in, double* const* g_u,
+                           double* const* g_uold, double* g_pos, int32_t where) {
+  ADJ_PRELUDE(ctx)
+  return (mpf_status)run_g2p_adjoint(c, g_vel_out, g_vel_in, g_u, g_uold, g_pos, where);
+}
+
+mpf_status mpf_advect_adjoint(mpf_ctx* ctx, const double* g_pos_out, const double* g_vel_out, double* g_pos_in,
+                              double* g_vel_in, int32_t where, int64_t* in_band) {
+  ADJ_PRELUDE(ctx)
+  return (mpf_status)run_advect_adjoint(c, g_pos_out, g_vel_out, g_pos_in, g_vel_in, where, in_band);
 }
 
 mpf_status mpf_heat(mpf_ctx* ctx, mpf_solve_info* info) {

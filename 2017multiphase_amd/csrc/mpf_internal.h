@@ -110,7 +110,8 @@ __host__ __device__ inline int64_t fid(const Grid& g, int a, int i, int j, int k
 // recurrence runs without host round trips.
 struct SolveScalars {
   double rz, rz_prev, pq, rr, bb, alpha, beta, rr_true;
-  int iter, done, converged, nonfinite, refinements, rec_conv, fold_pending, pad0;
+  int iter, done, converged, nonfinite, refinements, rec_conv, fold_pending;
+  int last_true;  // MIXED: update index whose residual the last executed true-residual pass replaced (-1 none)
 };
 
 struct KernelStat {
@@ -190,6 +191,13 @@ struct Ctx {
   float* mc_verts = nullptr;
   uint32_t* mc_tris = nullptr;
   int64_t mc_nv = 0, mc_nt = 0, mc_nvox = 0;
+  // adjoint scratch (allocated on first use): 6 face cotangent arrays in and
+  // out, 2 cell fields, per-particle cotangents
+  double* adj_fin[6] = {};
+  double* adj_fout[6] = {};
+  double* adj_cell[2] = {};
+  double* adj_part[3] = {};  // vel out / vel in / pos, 3P each
+  bool adj_ready = false;
   // nucleation seeds
   int32_t* seeds = nullptr;
   int32_t nseeds = 0;
@@ -246,6 +254,16 @@ void run_solid_phi_and_velocity(Ctx& c);
 // pcg.hip
 int run_pressure_solve(Ctx& c, mpf_solve_info* info);
 int run_heat_cg(Ctx& c, mpf_solve_info* info);
+int run_adjoint_solve(Ctx& c, const double* rhs, double* sol, mpf_solve_info* info);
+// adjoint.hip
+void adj_free(Ctx& c);
+int run_project_adjoint(Ctx& c, const double* const* gout, double* const* gin, int where, mpf_solve_info* info);
+int run_extrapolate_adjoint(Ctx& c, const double* const* gout, double* const* gin, int where);
+int run_p2g_adjoint(Ctx& c, const double* const* gfaces, double* gvel, double* gpos, int where);
+int run_g2p_adjoint(Ctx& c, const double* gvel_out, double* gvel_in, double* const* gu, double* const* guo,
+                    double* gpos, int where);
+int run_advect_adjoint(Ctx& c, const double* gpos_out, const double* gvel_out, double* gpos_in, double* gvel_in,
+                       int where, int64_t* in_band);
 
 bool hip_ok(Ctx& c, hipError_t e, const char* what);
 

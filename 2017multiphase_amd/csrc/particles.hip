@@ -2,6 +2,7 @@
 // advection with solid collision or RK2 (a15), heat G2P with latent heat
 // (a16, particle half) and phase change with nucleation (a17).
 #include "mpf_internal.h"
+#include "interp.h"
 
 namespace mpf {
 
@@ -13,38 +14,6 @@ __device__ __forceinline__ f3 sub3(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y
 __device__ __forceinline__ f3 mul3(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
 __device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 __device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
-
-// trilinear, spray_k.cu:111-122
-__device__ __forceinline__ float trilinear(const float* __restrict__ u, float x, float y, float z, int w, int hh,
-                                           int d) {
-  x = fmaxf(0.0f, fminf(x, (float)w));
-  y = fmaxf(0.0f, fminf(y, (float)hh));
-  z = fmaxf(0.0f, fminf(z, (float)d));
-  int i = (int)fminf(x, (float)(w - 2));
-  int j = (int)fminf(y, (float)(hh - 2));
-  int k = (int)fminf(z, (float)(d - 2));
-  const int64_t sj = d, si = (int64_t)hh * d;
-  const float* b = u + i * si + j * sj + k;
-  float ix1 = (float)(i + 1) - x, ix0 = x - (float)i;
-  float jy1 = (float)(j + 1) - y, jy0 = y - (float)j;
-  float kz1 = (float)(k + 1) - z, kz0 = z - (float)k;
-  return kz1 * (jy1 * (ix1 * b[0] + ix0 * b[si]) + jy0 * (ix1 * b[sj] + ix0 * b[si + sj])) +
-         kz0 * (jy1 * (ix1 * b[1] + ix0 * b[si + 1]) + jy0 * (ix1 * b[sj + 1] + ix0 * b[si + sj + 1]));
-}
-// getParticleVelFromGrid, spray_k.cu:227-240
-__device__ __forceinline__ f3 vel_from_grid(const Grid& g, f3 p, const float* ux, const float* uy, const float* uz) {
-  float x = (p.x - g.gx0) / g.h, y = (p.y - g.gy0) / g.h, z = (p.z - g.gz0) / g.h;
-  f3 v;
-  v.x = trilinear(ux, x, y - 0.5f, z - 0.5f, g.nx + 1, g.ny, g.nz);
-  v.y = trilinear(uy, x - 0.5f, y, z - 0.5f, g.nx, g.ny + 1, g.nz);
-  v.z = trilinear(uz, x - 0.5f, y - 0.5f, z, g.nx, g.ny, g.nz + 1);
-  return v;
-}
-// getScaleFromFrid, spray_k.cu:139-151
-__device__ __forceinline__ float scalar_from_grid(const Grid& g, f3 p, const float* f) {
-  float x = (p.x - g.gx0) / g.h, y = (p.y - g.gy0) / g.h, z = (p.z - g.gz0) / g.h;
-  return trilinear(f, x - 0.5f, y - 0.5f, z - 0.5f, g.nx, g.ny, g.nz);
-}
 
 struct Faces3 {
   const float* u[3];

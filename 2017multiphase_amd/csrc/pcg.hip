@@ -302,6 +302,7 @@ __global__ void __launch_bounds__(PB) k_init(OpArgs o, Launch L, const V* __rest
     sc->nonfinite = 0;
     sc->iter = 0;
     sc->refinements = 0;
+    sc->last_true = -1;
   }
 }
 
@@ -483,7 +484,10 @@ __global__ void __launch_bounds__(PB) k_resid64(OpArgs o, Launch L, const float*
   }
   double v[2] = {a_rz, a_rr};
   write_partials<2>(v, uslot(P, k));
-  if (blockIdx.x == 0 && threadIdx.x == 0) sc->refinements += 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->refinements += 1;
+    sc->last_true = k;
+  }
 }
 
 // MIXED: x64 += x at exit.
@@ -520,6 +524,15 @@ __global__ void __launch_bounds__(PB) k_collect_rr(const double* __restrict__ S,
 
 __global__ void k_clear_done(SolveScalars* sc) {
   if (threadIdx.x == 0 && blockIdx.x == 0) sc->done = 0;
+}
+
+__global__ void k_to_float(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+__global__ void k_to_double(int64_t n, const float* __restrict__ a, double* __restrict__ b) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (double)a[i];
 }
 
 Launch make_launch(int64_t G) {
@@ -578,7 +591,6 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
   launch(c, "pcg_init", dim3(L.nblk), dim3(PB), 0, k_init<V, Dacc, OPk>, o, L, b, x, r, pbuf[0], pbuf[1],
          mixed ? x64 : (double*)nullptr, P, c.scal);
   int k = 0;
-  int last_true = -1;   // update index after which the residual was last made true
   bool fold_next = false;
   while (k <= cap) {
     const int k_end = k + check;
@@ -592,7 +604,6 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
       if (mixed && (k + 1) % R == 0) {
         launch(c, "pcg_resid64", dim3(L.nblk), dim3(PB), 0, k_resid64<OPk>, o, L, (const float*)b, (const double*)x64,
                (const float*)x, (float*)r, P, c.scal, k);
-        last_true = k;
         fold_next = true;
       }
     }
@@ -600,14 +611,15 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
     const SolveScalars& s = *c.scal_host;
     if (!s.done) continue;
     if (!mixed || !s.converged || s.nonfinite) break;
-    // spmv(kd) met the criterion on the residual of update(kd-1)
+    // spmv(kd) met the criterion on the residual of update(kd-1).  Which
+    // true-residual pass last ran is read from the device: passes launched
+    // after the solve finished returned without running.
     const int kd = s.iter;
-    if (kd == 0 || last_true == kd - 1) break;
+    if (kd == 0 || s.last_true == kd - 1) break;
     // resid64 returns early while done is set: clear it first
     launch(c, "pcg_clear", dim3(1), dim3(64), 0, k_clear_done, c.scal);
     launch(c, "pcg_resid64", dim3(L.nblk), dim3(PB), 0, k_resid64<OPk>, o, L, (const float*)b, (const double*)x64,
            (const float*)x, (float*)r, P, c.scal, kd - 1);
-    last_true = kd - 1;
     fold_next = true;
     k = kd;
   }
@@ -653,6 +665,31 @@ int run_pressure_solve(Ctx& c, mpf_solve_info* info) {
   }
   if (vd) return cg_solve<float, float, OP_VARDENS>(c, (const float*)c.div, (float*)c.press, nullptr, ref, precond, cap, k.tol, false, info);
   return cg_solve<float, float, OP_UNIT>(c, (const float*)c.div, (float*)c.press, nullptr, ref, precond, cap, k.tol, false, info);
+}
+
+// A x = rhs for the adjoint of the projection (the operator is symmetric), in
+// the configured precision; rhs and the solution are FP64 cell fields.
+int run_adjoint_solve(Ctx& c, const double* rhs, double* sol, mpf_solve_info* info) {
+  const mpf_config& k = c.cfg;
+  const bool ref = k.solver == MPF_SOLVER_REFERENCE_CG;
+  const int precond = ref ? 0 : 1;
+  const int cap = ref ? k.ref_max_iters : k.max_iters;
+  const bool vd = k.op == MPF_OPERATOR_VARDENS;
+  if (k.precision == MPF_FP64) {
+    if (vd) return cg_solve<double, double, OP_VARDENS>(c, rhs, sol, nullptr, ref, precond, cap, k.tol, false, info);
+    return cg_solve<double, double, OP_UNIT>(c, rhs, sol, nullptr, ref, precond, cap, k.tol, false, info);
+  }
+  // FP32 vectors: the right-hand side rounded to float into the scratch field
+  const int64_t G = c.g.G;
+  launch(c, "adj_to_float", dim3(nblocks(G, 256)), dim3(256), 0, k_to_float, G, rhs, c.ctmp);
+  if (k.precision == MPF_MIXED) {
+    if (vd) return cg_solve<float, double, OP_VARDENS>(c, (const float*)c.ctmp, (float*)c.vx, sol, ref, precond, cap, k.tol, true, info);
+    return cg_solve<float, double, OP_UNIT>(c, (const float*)c.ctmp, (float*)c.vx, sol, ref, precond, cap, k.tol, true, info);
+  }
+  int st = vd ? cg_solve<float, float, OP_VARDENS>(c, (const float*)c.ctmp, (float*)c.vx, nullptr, ref, precond, cap, k.tol, false, info)
+              : cg_solve<float, float, OP_UNIT>(c, (const float*)c.ctmp, (float*)c.vx, nullptr, ref, precond, cap, k.tol, false, info);
+  launch(c, "adj_to_double", dim3(nblocks(G, 256)), dim3(256), 0, k_to_double, G, (const float*)c.vx, sol);
+  return st;
 }
 
 // Heat: reference CG (no preconditioner, r.r < 1e-2 G), FP32 vectors, FP64 dots.

@@ -25,27 +25,33 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per launch, as (bytes per cell, bytes per particle), for the
-# kernels that can dominate (FP32 vectors of the MIXED form; FP64 form doubles
-# the vector bytes).  Each array element compulsorily read or written once.
+# Algorithmic bytes per launch, as (bytes per cell, bytes per particle): each
+# array element the kernel must touch, read or written once (neighbour reads
+# of a stencil come from cache).  FP32 vectors of the MIXED form; FP64_VEC
+# holds the FP64 form's figures.
 KERNEL_BYTES = {
-    # q = A p: p 4, mark 1, q 4
-    "pcg_spmv": (9, 0),
-    # x, r read+write 16, p 4, q 4, 1/diag 4, mark 1
-    "pcg_update": (29, 0),
-    # p = r/diag + beta p: r 4, 1/diag 4, p read+write 8
-    "pcg_dir": (16, 0),
-    # start/end 8 + mask 1 per cell, pos 12 + flag 1 per particle
+    # p_k = r/diag + beta p_{k-1} fused with q = A p_k: r 4, p_{k-1} 4, mask 1; p_k 4, q 4 written
+    "pcg_spmv": (17, 0),
+    # x += alpha p, r -= alpha q, dots: x, r read+write 16, p 4, q 4, mask 1
+    "pcg_update": (25, 0),
+    # FP64 true residual: b 4, x64 8, x 4, mask 1; r 4 written
+    "pcg_resid64": (21, 0),
+    # start/end 8, mask 1, ls fluid+air 8 written; pos 12 + flag 1 per particle
     "genlevelset": (17, 13),
-    # per axis launch: start/end 8, mask 1, u 4 per face; pos 12 + vel 12 + mass 4 + flag 1 per particle
-    "p2g_liquid": (13, 29),
+    # one launch, 3 axes: start/end 8, mask 1, 3 faces 12 written; pos 12, vel 12, mass 4, flag 1
+    "p2g_liquid": (21, 29),
+    # one launch per axis: start/end 8, mask 1, face 4 written; particles as above
     "p2g_air": (13, 29),
+    # start/end 8, mask 1, T 4 written; pos 12, T 4, flag 1
+    "heat_p2g": (13, 17),
     # 45-byte record read and written + perm 4
     "reorder": (0, 94),
+    # level-set sweep: ls 4 read, mark 1, ls 4 written
     "sweep": (9, 0),
-    "sweep_u": (13, 0),
+    # extrapolation pass per face: u 4, out 4, ls 2x4, mark 2
+    "sweep_u": (18, 0),
 }
-FP64_VEC = {"pcg_spmv": (17, 0), "pcg_update": (57, 0), "pcg_dir": (32, 0)}
+FP64_VEC = {"pcg_spmv": (33, 0), "pcg_update": (49, 0)}
 
 
 def log(*a):
@@ -107,10 +113,7 @@ def roofline(res, precision):
         table.update(FP64_VEC)
     per_cell, per_par = table.get(name, (0, 0))
     G, P = res["G"], res["particles"]
-    launches_per_axis = 1
-    if name.startswith("p2g"):
-        launches_per_axis = 3  # one launch per face axis: G faces (approx) per launch
-    bytes_per_launch = per_cell * G + per_par * P * (1 if launches_per_axis == 1 else 1)
+    bytes_per_launch = per_cell * G + per_par * P
     avg_ms = ms / max(cnt, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if bytes_per_launch else None
     rf = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -153,8 +156,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path (see DESIGN.md for the "
-              "z-slab sharding design); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the z-slab multi-GPU path is not built; run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     pkg = importlib.import_module("2017multiphase_amd")
     scenes = importlib.import_module("2017multiphase_amd.scenes")

@@ -273,6 +273,38 @@ mpf_status mpf_get_mc_cases(mpf_ctx* ctx, uint8_t* cases, int32_t where);
 /* One full step in bubblesim order (spray.cpp:979-1233, north-star subset). */
 mpf_status mpf_step(mpf_ctx* ctx, int32_t frame, mpf_step_info* info);
 
+/* ---- adjoints ----------------------------------------------------------
+   No counterpart in the reference (it has no differentiation).  Each call is
+   the transposed Jacobian of one stage, evaluated at the context's current
+   state, which must be the state right after that stage's forward call;
+   every discrete decision of that call (flags, level sets, cell lists,
+   branch choices) is held fixed.  Cotangents are FP64 arrays in `where`
+   memory.  Face sets are 6 pointers: water ux, uy, uz, air ux, uy, uz
+   (sizes (nx+1)ny nz, nx(ny+1)nz, nx ny(nz+1)); particle arrays are 3P.
+   Input cotangent pointers may be NULL (zero); output pointers may be NULL
+   (not written).
+   Projection (mpf_project, spray.cu:1590-1611): g_in = P^T g_out, one PCG
+   solve on the same symmetric operator in cfg.precision. */
+mpf_status mpf_project_adjoint(mpf_ctx* ctx, const double* const* g_out, double* const* g_in, int32_t where,
+                               mpf_solve_info* info);
+/* Extrapolation and wall rule (mpf_extrapolate_bc, spray.cpp:1088-1104). */
+mpf_status mpf_extrapolate_adjoint(mpf_ctx* ctx, const double* const* g_out, double* const* g_in, int32_t where);
+/* P2G of both phases (mpf_p2g, spray.cu:1562-1588): cotangents of the P2G
+   output faces to particle velocities and positions (g_vel, g_pos are
+   overwritten). */
+mpf_status mpf_p2g_adjoint(mpf_ctx* ctx, const double* const* g_faces, double* g_vel, double* g_pos,
+                           int32_t where);
+/* G2P / FLIP-PIC blend (mpf_g2p, spray.cu:1689-1705): from the cotangent of
+   the new particle velocity to the old velocity, the face velocities u (6),
+   the P2G snapshot u_old (6) and positions (all overwritten). */
+mpf_status mpf_g2p_adjoint(mpf_ctx* ctx, const double* g_vel_out, double* g_vel_in, double* const* g_u,
+                           double* const* g_uold, double* g_pos, int32_t where);
+/* Advection (mpf_advect, MPF_ADVECT_EULER_COLLIDE, spray.cu:2959-2983): the
+   domain clamp is differentiated; the solid-collision response is not, and
+   *in_band (may be NULL) returns how many particles were inside its band. */
+mpf_status mpf_advect_adjoint(mpf_ctx* ctx, const double* g_pos_out, const double* g_vel_out, double* g_pos_in,
+                              double* g_vel_in, int32_t where, int64_t* in_band);
+
 /* ---- introspection ------------------------------------------------------ */
 /* Kernel timing with HIP events around every launch on the context stream. */
 mpf_status mpf_set_kernel_timing(mpf_ctx* ctx, int32_t on);

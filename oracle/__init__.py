@@ -37,6 +37,7 @@ def lib():
     L.orc_set_particles.argtypes = [vp, i64] + [vp] * 8
     L.orc_get_particles.argtypes = [vp, c.POINTER(i64)] + [vp] * 8
     L.orc_particle_count.argtypes = [vp]
+    L.orc_set_kinematics64.argtypes = [vp, vp, vp]
     L.orc_particle_count.restype = i64
     L.orc_field_count.argtypes = [vp, c.c_int]
     L.orc_field_count.restype = i64
@@ -105,6 +106,15 @@ class Oracle:
                 x = np.ascontiguousarray(x, dtype=np.int8 if nm == "flag" else np.float32)
             arrs.append(x)
         self._ok(self.L.orc_set_particles(self.h, n, *[_p(x) for x in arrs]))
+
+    def set_kinematics64(self, pos=None, vel=None):
+        """Overwrite positions/velocities (FP64, current order) keeping all discrete state."""
+        n = self.particle_count()
+        pos = None if pos is None else np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(-1)
+        assert pos is None or pos.size == 3 * n
+        assert vel is None or vel.size == 3 * n
+        self._ok(self.L.orc_set_kinematics64(self.h, _p(pos), _p(vel)))
 
     def get_particles(self):
         n = int(self.L.orc_particle_count(self.h))

@@ -1390,6 +1390,16 @@ int orc_set_particles(orc_state* s, int64_t n, const float* pos, const float* ve
     return (int)MPF_OK;
   });
 }
+int orc_set_kinematics64(orc_state* s, const double* pos, const double* vel) {
+  return dispatch(s, [&](auto& o) {
+    using R = typename std::remove_reference<decltype(o.pos)>::type::value_type;
+    for (int64_t i = 0; i < 3 * o.P; ++i) {
+      if (pos) o.pos[i] = (R)pos[i];
+      if (vel) o.vel[i] = (R)vel[i];
+    }
+    return (int)MPF_OK;
+  });
+}
 int orc_get_particles(orc_state* s, int64_t* n, double* pos, double* vel, int8_t* flag, double* mass,
                       double* T, double* lheat, double* solub, double* gas) {
   return dispatch(s, [&](auto& o) {

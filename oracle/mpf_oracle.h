@@ -39,6 +39,10 @@ int orc_set_particles(orc_state* s, int64_t n, const float* pos, const float* ve
 int orc_get_particles(orc_state* s, int64_t* n, double* pos, double* vel, int8_t* flag,
                       double* mass, double* T, double* lheat, double* solubility,
                       double* gascontain);
+/* Overwrite positions / velocities (3n doubles, current particle order; either
+   may be NULL) and nothing else: cell lists, flags and fields are kept.  Used
+   for finite differences with the discrete state held fixed. */
+int orc_set_kinematics64(orc_state* s, const double* pos, const double* vel);
 int64_t orc_particle_count(orc_state* s);
 int64_t orc_field_count(orc_state* s, int field);
 /* Real fields are exchanged as double; MARK as int8; CELL_*, GRID_* as uint32. */

@@ -1,0 +1,453 @@
+"""Gradient-check harness for the differentiable two-phase step sequence.
+
+TEST INFRASTRUCTURE.  A run of `steps` FLIP steps of the water-slab scene
+(water slab + air bubbles, closed box) is differentiated with the library's
+hand-written adjoints (mpf_project_adjoint, mpf_extrapolate_adjoint,
+mpf_p2g_adjoint, mpf_g2p_adjoint, mpf_advect_adjoint), wrapped in
+torch.autograd.Function objects that checkpoint every `checkpoint` steps:
+the forward keeps only each segment's input particles; the backward replays
+the segment from that checkpoint to recover every step's input, then runs the
+adjoint stages of each step in reverse, replaying the step up to the stage
+whose state an adjoint needs.
+
+Parameters and loss:
+  u_init   initial grid velocity (3 face arrays), transferred to the
+           particles of both phases by a PIC G2P (alpha = 0);
+  s        per-cell buoyancy source: every step adds dt * (s[c-] + s[c+]) / 2
+           to each interior water z-face after P2G (a vertical body
+           acceleration field);
+  loss     sum over water faces of (u_final - u_target)^2, u_final being the
+           water face velocity after the last step's projection and
+           extrapolation, u_target a sum of 8 seed-random low-frequency
+           sinusoids per component.
+
+Each step: hash/sort, forces, level set, P2G, source, extrapolation,
+projection, extrapolation, G2P, advection (heat, phase change and meshing off).
+Discrete decisions (sort permutation and cell lists, flags, level sets,
+branch choices inside stages) are constants of the gradient.  The reference
+check therefore evaluates the loss with those decisions frozen at the values
+the GPU run took: the FP64 oracle replays the same sequence from the same
+recorded permutation, cell lists, flags and level sets, and central finite
+differences along random directions are compared with the adjoint's
+directional derivative.
+"""
+import argparse
+import importlib
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+FACE_W = ["WATER_UX", "WATER_UY", "WATER_UZ"]
+FACE_A = ["AIR_UX", "AIR_UY", "AIR_UZ"]
+FACE = FACE_W + FACE_A
+FACE_OLD = [f + "_OLD" for f in FACE]
+LEVELSET = ["MARK", "LSFLUID", "LSAIR", "LSMERGE", "PHIGRAD_X", "PHIGRAD_Y", "PHIGRAD_Z"]
+PART_KEYS = ["pos", "vel", "flag", "mass", "T", "lheat", "solubility", "gascontain"]
+
+
+def face_shapes(cfg):
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    return [(nx + 1, ny, nz), (nx, ny + 1, nz), (nx, ny, nz + 1)]
+
+
+def target_field(cfg, seed, amp=0.05):
+    """Sum of 8 seed-random low-frequency sinusoids per component, on the faces."""
+    rng = np.random.default_rng(seed)
+    h = float(cfg.dx)
+    out = []
+    for a, shp in enumerate(face_shapes(cfg)):
+        i, j, k = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shp], indexing="ij")
+        x = (i + (0.0 if a == 0 else 0.5)) * h
+        y = (j + (0.0 if a == 1 else 0.5)) * h
+        z = (k + (0.0 if a == 2 else 0.5)) * h
+        L = np.array([cfg.nx, cfg.ny, cfg.nz], np.float64) * h
+        f = np.zeros(shp)
+        for _ in range(8):
+            kv = rng.integers(1, 4, size=3)
+            ph = rng.uniform(0, 2 * np.pi, size=3)
+            c = rng.normal()
+            f += c * (np.sin(2 * np.pi * kv[0] * x / L[0] + ph[0]) * np.sin(2 * np.pi * kv[1] * y / L[1] + ph[1]) *
+                      np.sin(2 * np.pi * kv[2] * z / L[2] + ph[2]))
+        out.append((amp * f / 8.0).ravel())
+    return out
+
+
+def source_to_faces(cfg, s, dt):
+    """dt * (s[k-1] + s[k]) / 2 on interior z-faces (k = 1..nz-1), 0 elsewhere."""
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    s3 = np.asarray(s, np.float64).reshape(nx, ny, nz)
+    uz = np.zeros((nx, ny, nz + 1))
+    uz[:, :, 1:nz] = 0.5 * dt * (s3[:, :, :-1] + s3[:, :, 1:])
+    return uz.ravel()
+
+
+def source_vjp(cfg, g_uz, dt):
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    g3 = np.asarray(g_uz, np.float64).reshape(nx, ny, nz + 1)[:, :, 1:nz] * (0.5 * dt)
+    gs = np.zeros((nx, ny, nz))
+    gs[:, :, :-1] += g3
+    gs[:, :, 1:] += g3
+    return gs.ravel()
+
+
+def forces_vjp(cfg, g_v, v_before, flag):
+    """Transposed Jacobian of addgravityforce/addbuoyancyforce (per particle)."""
+    g = np.array(g_v, np.float64, copy=True)
+    if cfg.buoyancy_kick:
+        vmax = float(cfg.bubble_max_vel)
+        gz = float(cfg.gravity[2])
+        dt = float(cfg.dt)
+        rate = np.where(flag == 1, float(cfg.buoyancy_rate_air),
+                        np.where(flag == 4, float(cfg.buoyancy_rate_solo), 0.0))
+        slope = np.where((vmax - v_before[:, 2] > 0) & (rate != 0), dt * gz * rate / vmax, 0.0)
+        g[:, 2] *= 1.0 + slope
+    return g
+
+
+class Runner:
+    """Drives one backend (GPU Simulator or FP64 oracle) through the step sequence."""
+
+    def __init__(self, backend, cfg, oracle=False):
+        self.b = backend
+        self.cfg = cfg
+        self.oracle = oracle
+        self.dt = float(cfg.dt)
+
+    def load(self, part):
+        n = len(part["flag"])
+        f32 = {k: np.asarray(part[k], np.float32) if k != "flag" else np.asarray(part[k], np.int8)
+               for k in PART_KEYS}
+        self.b.set_particles(n, **f32)
+        if self.oracle:
+            self.b.set_kinematics64(pos=part["pos"], vel=part["vel"])
+
+    def particles(self):
+        p = self.b.get_particles()
+        out = {}
+        for k in PART_KEYS:
+            a = p[k]
+            out[k] = np.asarray(a, np.int8 if k == "flag" else np.float64)
+        out["pos"] = out["pos"].reshape(-1, 3)
+        out["vel"] = out["vel"].reshape(-1, 3)
+        return out
+
+    def step(self, part, s, frozen=None, stop=None, want_u=False):
+        """One step from `part` (current order).  Returns (out_particles, u_final, record)."""
+        b = self.b
+        rec = {}
+        if frozen is None:
+            self.load(part)
+            b.hash_sort()
+            perm = b.get_field("GRID_INDEX").astype(np.int64)
+            rec["perm"] = perm
+            rec["CELL_START"] = b.get_field("CELL_START")
+            rec["CELL_END"] = b.get_field("CELL_END")
+        else:
+            perm = frozen["perm"]
+            self.load({k: part[k][perm] for k in PART_KEYS})
+            b.set_field("CELL_START", frozen["CELL_START"])
+            b.set_field("CELL_END", frozen["CELL_END"])
+        rec["v_sorted"] = part["vel"][perm]
+        rec["flag_sorted"] = part["flag"][perm]
+        b.apply_forces()
+        if frozen is None:
+            b.levelset_update()
+            for nm in LEVELSET:
+                rec[nm] = b.get_field(nm)
+        else:
+            for nm in LEVELSET:
+                b.set_field(nm, frozen[nm])
+        b.p2g()
+        if s is not None:
+            uz = b.get_field("WATER_UZ").astype(np.float64) + source_to_faces(self.cfg, s, self.dt)
+            b.set_field("WATER_UZ", uz)
+        b.extrapolate_bc(True)
+        if stop == "extrap1":
+            return None, None, rec
+        b.project(allow_no_convergence=True)
+        b.extrapolate_bc(False)
+        u_final = [b.get_field(nm).astype(np.float64) for nm in FACE_W] if want_u else None
+        b.g2p()
+        if stop == "g2p":
+            return None, u_final, rec
+        b.advect()
+        return self.particles(), u_final, rec
+
+    def backward_step(self, part, s, gx_out, gv_out, gu_final=None):
+        """Cotangents of one step's inputs (particles in input order, s)."""
+        b = self.b
+        _, _, rec = self.step(part, s, stop="g2p")
+        gxa, gva, _ = b.advect_adjoint(gx_out, gv_out)
+        r = b.g2p_adjoint(gva)
+        gu = [a.copy() for a in r["u"]]
+        if gu_final is not None:
+            for q in range(3):
+                gu[q] += gu_final[q]
+        gu = b.extrapolate_adjoint(gu)
+        gu, _ = b.project_adjoint(gu, allow_no_convergence=True)
+        self.step(part, s, stop="extrap1")
+        gu = b.extrapolate_adjoint(gu)
+        gs = source_vjp(self.cfg, gu[2], self.dt) if s is not None else None
+        gp = [a + c for a, c in zip(gu, r["u_old"])]
+        gvc, gxc = b.p2g_adjoint(gp)
+        gx_sorted = gxa + r["pos"] + gxc
+        gv_sorted = forces_vjp(self.cfg, r["vel"] + gvc, rec["v_sorted"], rec["flag_sorted"])
+        perm = rec["perm"]
+        gx_in = np.empty_like(gx_sorted)
+        gv_in = np.empty_like(gv_sorted)
+        gx_in[perm] = gx_sorted
+        gv_in[perm] = gv_sorted
+        return gx_in, gv_in, gs
+
+
+class InitTransfer:
+    """v0 = PIC G2P of u_init (both phases) at the initial positions."""
+
+    def __init__(self, backend, cfg, oracle=False):
+        self.b = backend
+        self.cfg = cfg
+        self.oracle = oracle
+
+    def _run(self, part, u_init):
+        n = len(part["flag"])
+        f32 = {k: np.asarray(part[k], np.float32) if k != "flag" else np.asarray(part[k], np.int8)
+               for k in PART_KEYS}
+        self.b.set_particles(n, **f32)
+        if self.oracle:
+            self.b.set_kinematics64(pos=part["pos"], vel=part["vel"])
+        for q, nm in enumerate(FACE):
+            self.b.set_field(nm, u_init[q % 3])
+            self.b.set_field(FACE_OLD[q], np.zeros_like(u_init[q % 3]))
+        self.b.g2p()
+
+    def forward(self, part, u_init):
+        self._run(part, u_init)
+        v = self.b.get_particles()["vel"]
+        return np.asarray(v, np.float64).reshape(-1, 3)
+
+    def backward(self, part, u_init, g_v):
+        self._run(part, u_init)
+        r = self.b.g2p_adjoint(g_v)
+        return [r["u"][q] + r["u"][q + 3] for q in range(3)]
+
+
+def loss_of(u_final, target):
+    return float(sum(np.sum((u - t) ** 2) for u, t in zip(u_final, target)))
+
+
+# ---------------------------------------------------------------- autograd
+
+def make_functions(torch):
+    class InitFn(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, u_init, init, part0, shapes):
+            sizes = [int(np.prod(sh)) for sh in shapes]
+            u = np.split(u_init.detach().cpu().numpy(), np.cumsum(sizes)[:-1])
+            ctx.init, ctx.part0, ctx.u = init, part0, u
+            return torch.from_numpy(init.forward(part0, u))
+
+        @staticmethod
+        def backward(ctx, g_v):
+            gu = ctx.init.backward(ctx.part0, ctx.u, g_v.detach().cpu().numpy())
+            return torch.from_numpy(np.concatenate(gu)), None, None, None
+
+    class SegmentFn(torch.autograd.Function):
+        """K steps from a checkpoint; outputs (pos, vel, u_final of the last step)."""
+
+        @staticmethod
+        def forward(ctx, x, v, s, runner, static, nsteps, last, side):
+            part = dict(static)
+            part["pos"] = x.detach().cpu().numpy()
+            part["vel"] = v.detach().cpu().numpy()
+            s_np = s.detach().cpu().numpy()
+            u_final = None
+            for k in range(nsteps):
+                part, u, _ = runner.step(part, s_np, want_u=last and k == nsteps - 1)
+                if u is not None:
+                    u_final = u
+            side["static"] = {k: part[k] for k in PART_KEYS if k not in ("pos", "vel")}
+            ctx.runner, ctx.static, ctx.nsteps, ctx.last = runner, static, nsteps, last
+            ctx.save_for_backward(x, v, s)
+            uf = torch.from_numpy(np.concatenate(u_final)) if u_final is not None else torch.zeros(0,
+                                                                                                   dtype=torch.float64)
+            return torch.from_numpy(part["pos"]), torch.from_numpy(part["vel"]), uf
+
+        @staticmethod
+        def backward(ctx, gx, gv, gu):
+            x, v, s = ctx.saved_tensors
+            runner = ctx.runner
+            s_np = s.detach().cpu().numpy()
+            # recompute the inputs of every step of the segment from the checkpoint
+            part = dict(ctx.static)
+            part["pos"] = x.detach().cpu().numpy()
+            part["vel"] = v.detach().cpu().numpy()
+            inputs = []
+            for k in range(ctx.nsteps):
+                inputs.append(part)
+                if k + 1 < ctx.nsteps:
+                    part, _, _ = runner.step(part, s_np)
+            gx = gx.detach().cpu().numpy().reshape(-1, 3)
+            gv = gv.detach().cpu().numpy().reshape(-1, 3)
+            gu_final = None
+            if ctx.last and gu.numel():
+                sizes = [int(np.prod(sh)) for sh in face_shapes(runner.cfg)]
+                gu_final = np.split(gu.detach().cpu().numpy(), np.cumsum(sizes)[:-1])
+            gs = np.zeros_like(s_np)
+            for k in reversed(range(ctx.nsteps)):
+                gx, gv, gsk = runner.backward_step(inputs[k], s_np, gx, gv,
+                                                   gu_final if k == ctx.nsteps - 1 else None)
+                gs += gsk
+            return torch.from_numpy(gx), torch.from_numpy(gv), torch.from_numpy(gs), None, None, None, None, None
+
+    return InitFn, SegmentFn
+
+
+def adjoint_gradient(pkg, scene, precision, steps, checkpoint, u_init, s, target, tol=1e-10):
+    """Loss and its gradient w.r.t. (u_init, s) through the GPU adjoints; also the
+    discrete records of every step (for the frozen-decision check)."""
+    import torch
+    cfg = pkg.Config.from_buffer_copy(scene["cfg"])
+    cfg.precision = precision
+    cfg.tol = tol
+    cfg.max_iters = 20000
+    sim = pkg.Simulator(cfg)
+    icfg = pkg.Config.from_buffer_copy(cfg)
+    icfg.flip_alpha = 0.0
+    isim = pkg.Simulator(icfg)
+    runner = Runner(sim, cfg)
+    init = InitTransfer(isim, icfg)
+    InitFn, SegmentFn = make_functions(torch)
+    part0 = {k: np.asarray(scene["particles"][k]) for k in PART_KEYS}
+    part0["pos"] = part0["pos"].astype(np.float64).reshape(-1, 3)
+    part0["vel"] = np.zeros_like(part0["pos"])
+    shapes = face_shapes(cfg)
+    ut = torch.tensor(np.concatenate(u_init), dtype=torch.float64, requires_grad=True)
+    st = torch.tensor(np.asarray(s, np.float64), requires_grad=True)
+    v = InitFn.apply(ut, init, part0, shapes)
+    x = torch.from_numpy(part0["pos"])
+    static = {k: part0[k] for k in PART_KEYS if k not in ("pos", "vel")}
+    done = 0
+    uf = None
+    while done < steps:
+        k = min(checkpoint, steps - done)
+        side = {}
+        x, v, uf = SegmentFn.apply(x, v, st, runner, static, k, done + k == steps, side)
+        static = side["static"]
+        done += k
+    tt = torch.from_numpy(np.concatenate(target))
+    loss = ((uf - tt) ** 2).sum()
+    loss.backward()
+    # discrete records of the run, for the frozen-decision reference
+    records = []
+    v0 = init.forward(part0, u_init)
+    part = dict(part0)
+    part["vel"] = v0
+    for k in range(steps):
+        part, _, rec = runner.step(part, np.asarray(s, np.float64))
+        records.append(rec)
+    out = {"loss": float(loss.item()), "g_u": ut.grad.numpy().copy(), "g_s": st.grad.numpy().copy(),
+           "records": records, "part0": part0}
+    sim.close()
+    isim.close()
+    return out
+
+
+def frozen_loss(orc, pkg, scene, records, u_init, s, target, tol=1e-12):
+    """Loss of the same run on the FP64 oracle with every discrete decision frozen."""
+    cfg = pkg.Config.from_buffer_copy(scene["cfg"])
+    cfg.precision = 1
+    cfg.tol = tol
+    cfg.max_iters = 20000
+    icfg = pkg.Config.from_buffer_copy(cfg)
+    icfg.flip_alpha = 0.0
+    o = orc.Oracle(cfg, use_double=True)
+    oi = orc.Oracle(icfg, use_double=True)
+    part0 = {k: np.asarray(scene["particles"][k]) for k in PART_KEYS}
+    part0["pos"] = part0["pos"].astype(np.float64).reshape(-1, 3)
+    part0["vel"] = np.zeros_like(part0["pos"])
+    v0 = InitTransfer(oi, icfg, oracle=True).forward(part0, u_init)
+    runner = Runner(o, cfg, oracle=True)
+    part = dict(part0)
+    part["vel"] = v0
+    u_final = None
+    for k, rec in enumerate(records):
+        part, u, _ = runner.step(part, s, frozen=rec, want_u=k == len(records) - 1)
+        if u is not None:
+            u_final = u
+    return loss_of(u_final, target)
+
+
+def fd_check(orc, pkg, scene, grad, u_init, s, target, ndirs, eps, seed=0, say=print):
+    """Central differences of the frozen-decision FP64 loss along random
+    directions vs the adjoint's directional derivative.  Directions are scaled
+    to the RMS of each parameter; eps is relative to that scale."""
+    rng = np.random.default_rng(seed)
+    su = max(float(np.sqrt(np.mean(np.concatenate(u_init) ** 2))), 1e-3)
+    ss = max(float(np.sqrt(np.mean(np.asarray(s) ** 2))), 1e-3)
+    sizes = [len(a) for a in u_init]
+    errs = []
+    for d in range(ndirs):
+        du = rng.standard_normal(sum(sizes)) * su
+        ds = rng.standard_normal(len(s)) * ss
+        dus = np.split(du, np.cumsum(sizes)[:-1])
+        lp = frozen_loss(orc, pkg, scene, grad["records"], [a + eps * b for a, b in zip(u_init, dus)],
+                         np.asarray(s) + eps * ds, target)
+        lm = frozen_loss(orc, pkg, scene, grad["records"], [a - eps * b for a, b in zip(u_init, dus)],
+                         np.asarray(s) - eps * ds, target)
+        fd = (lp - lm) / (2 * eps)
+        ad = float(np.dot(grad["g_u"], du) + np.dot(grad["g_s"], ds))
+        e = abs(fd - ad) / max(abs(fd), abs(ad), 1e-300)
+        errs.append(e)
+        say("  direction %d: FD %.10e adjoint %.10e rel %.2e" % (d, fd, ad, e))
+    return errs
+
+
+def parameters(scene, seed):
+    cfg = scene["cfg"]
+    rng = np.random.default_rng(seed + 100)
+    target = target_field(cfg, seed)
+    u_init = [0.5 * a + 0.01 * rng.standard_normal(a.size) for a in target_field(cfg, seed + 1)]
+    G = cfg.nx * cfg.ny * cfg.nz
+    s = 0.5 * rng.standard_normal(G)
+    return u_init, s, target
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--n", type=int, default=96)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--checkpoint", type=int, default=4)
+    ap.add_argument("--dirs", type=int, default=8)
+    ap.add_argument("--eps", type=float, default=1e-6)
+    ap.add_argument("--seed", type=int, default=3)
+    args = ap.parse_args()
+    pkg = importlib.import_module("2017multiphase_amd")
+    scenes = importlib.import_module("2017multiphase_amd.scenes")
+    import oracle as orc
+    orc.set_threads(min(os.cpu_count() or 1, 16))
+    scene = scenes.make("diff_slab", n=args.n)
+    u_init, s, target = parameters(scene, args.seed)
+    say = lambda *a: print(*a, flush=True)  # noqa: E731
+    say("diff_slab n=%d particles=%d steps=%d checkpoint=%d" % (args.n, scene["n_particles"], args.steps,
+                                                                  args.checkpoint))
+    g64 = adjoint_gradient(pkg, scene, pkg.FP64, args.steps, args.checkpoint, u_init, s, target)
+    g32 = adjoint_gradient(pkg, scene, pkg.FP32, args.steps, args.checkpoint, u_init, s, target, tol=1e-7)
+    gm = adjoint_gradient(pkg, scene, pkg.MIXED, args.steps, args.checkpoint, u_init, s, target)
+    for nm, g in (("FP32", g32), ("MIXED", gm)):
+        for key in ("g_u", "g_s"):
+            e = np.linalg.norm(g[key] - g64[key]) / max(np.linalg.norm(g64[key]), 1e-300)
+            say("%s %s vs FP64 gradient: rel L2 %.2e" % (nm, key, e))
+    say("loss FP64 %.12e FP32 %.12e" % (g64["loss"], g32["loss"]))
+    errs = fd_check(orc, pkg, scene, g64, u_init, s, target, args.dirs, args.eps, say=say)
+    say("FD vs FP64 adjoint: max rel %.2e over %d directions" % (max(errs), len(errs)))
+    errs = fd_check(orc, pkg, scene, g32, u_init, s, target, min(args.dirs, 4), args.eps, say=say)
+    say("FD vs FP32 adjoint: max rel %.2e" % max(errs))
+
+
+if __name__ == "__main__":
+    main()
