@@ -415,6 +415,7 @@ mpf_status mpf_set_particles(mpf_ctx* ctx, int64_t n, const float* pos, const fl
   put(c.gas[b], gas, (size_t)n * 4, 0);
   c.P = n;
   c.sorted = false;
+  c.user_cstart = c.user_cend = false;
   return hip_ok(c, hipStreamSynchronize(c.stream), "set_particles") ? MPF_OK : MPF_HIP_ERROR;
 }
 
@@ -482,8 +483,21 @@ mpf_status mpf_set_field(mpf_ctx* ctx, int32_t field, const void* src, int32_t w
     hipMemcpyAsync(p, src, (size_t)n * es, where == MPF_HOST ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice,
                    c.stream);
   if (field == MPF_F_CELL_START || field == MPF_F_CELL_END) {
-    c.sorted = true;  // caller supplies cell lists (particles assumed in that order)
-    run_cell_mask(c);
+    // Caller-supplied cell lists (particles assumed in that order) take effect
+    // once both arrays have been given and every list indexes the particles.
+    c.sorted = false;
+    if (field == MPF_F_CELL_START) c.user_cstart = true;
+    else c.user_cend = true;
+    if (!hip_ok(c, hipStreamSynchronize(c.stream), "set_field")) return MPF_HIP_ERROR;
+    if (c.user_cstart && c.user_cend) {
+      const int64_t bad = run_check_cells(c);
+      if (bad < 0) return MPF_HIP_ERROR;
+      if (bad > 0)
+        return fail(c, MPF_INVALID_ARG,
+                    std::to_string(bad) + " cell lists do not index the particle array (need start < end <= P)");
+      run_cell_mask(c);
+      c.sorted = true;
+    }
   }
   return hip_ok(c, hipStreamSynchronize(c.stream), "set_field") ? MPF_OK : MPF_HIP_ERROR;
 }

@@ -147,6 +147,7 @@ struct Ctx {
   uint32_t* perm = nullptr;   // sorted slot -> old index (gridIndex)
   uint32_t* ghash = nullptr;  // sorted keys (gridHash)
   bool sorted = false;
+  bool user_cstart = false, user_cend = false;  // cell lists set by the caller since the last sort
 
   // cells
   uint32_t* count = nullptr;  // per-cell counts, then scatter cursors
@@ -233,6 +234,7 @@ inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); 
 // sort.hip
 void run_hash_sort(Ctx& c);
 void run_cell_mask(Ctx& c);
+int64_t run_check_cells(Ctx& c);  // cells whose list does not index the particles; -1 on a HIP error
 int64_t run_scan_u32(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n, bool want_total);
 // particles.hip
 void run_apply_forces(Ctx& c);

@@ -7,26 +7,29 @@
 // (spray_k.cu:4526-4582).
 //
 // Two kernels per iteration, both HBM-bound:
-//   spmv   : p_k = z_k + beta p_{k-1} (z = r * diag^-1, recomputed, also at the
-//            six neighbours from r and p_{k-1}), written to the other p buffer;
-//            q = A p_k; p.q                       -> alpha
+//   spmv   : p_k = z_k + beta p_{k-1} (z = r * diag^-1), written to the other
+//            p buffer; q = A p_k; p.q                       -> alpha
+//            2.5-D tiling: a block owns a 16 x 64 (j, k) column tile over a
+//            run of planes along i; each p_k is formed once, the current
+//            plane (with a one-cell ring) sits in LDS for the j/k
+//            neighbours and the i neighbours stay in registers.
 //   update : x += alpha p_k ; r -= alpha q ; r.z ; r.r   -> beta, convergence
 // FP32 vectors: spmv moves r 4 + p_{k-1} 4 + mask 1 + p_k 4 + q 4 = 17 B/cell,
 // update x 8 + r 8 + p 4 + q 4 + mask 1 = 25 B/cell (FP64: 33 + 49).
 // The Jacobi diagonal is rebuilt from a per-cell neighbour-mask byte (bits
 // 0-5: face open, i.e. not Neumann; bit 6: unknown; bit 7: slow path — an air
 // cell or a liquid/air face of the variable-density operator, which reads the
-// flags and lsmerge), so no diagonal vector is stored.
+// flags and lsmerge) through a 7-entry table, so no diagonal vector is stored.
 //
-// The scalar recurrence stays on the device: each kernel reduces into
-// per-block partials; the last block to arrive (arrival counter + fences)
-// sums the partials in block order — deterministic — and advances the
-// SolveScalars record the next kernel reads.  The host polls the record
+// Reductions are deterministic and reduced by the consumer: each kernel
+// writes one partial per block with plain stores; every block of the next
+// kernel sums all partials in the same fixed order and so derives the same
+// alpha, beta and convergence decision.  The host polls the scalar record
 // every check_every iterations.  MIXED adds, every refine_every iterations, a
-// reliable update: the FP64 true residual b - A (x64 + x) replaces r (beta from
-// it), and the next update folds the FP32 correction x into the FP64 x64.
-// Blocks own contiguous cell chunks, mapped so each XCD (blocks are dispatched
-// round-robin over the 8 XCDs) walks one contiguous slab.
+// reliable update: the FP64 true residual b - A (x64 + x) replaces r (beta
+// from it), and the next update folds the FP32 correction x into the FP64 x64.
+// Streaming kernels walk contiguous cell chunks, mapped so each XCD (blocks
+// are dispatched round-robin over the 8 XCDs) walks one contiguous slab.
 #include "mpf_internal.h"
 
 namespace mpf {
@@ -48,7 +51,20 @@ struct OpArgs {
   int solid_c;      // SOLID cells Neumann (pressure)
   int vac_neumann;  // VACUUM neighbours Neumann (heat)
   int precond;      // Jacobi preconditioner on
+  // fast-path rows (uniform density): 1/diag for n open faces, and the factor
+  // 1/(rho h^2) applied to the neighbour differences
+  float dinv_f[7];
+  double dinv_d[7];
+  float qc_f;
+  double qc_d;
 };
+
+template <class A> __device__ __forceinline__ A dinv_tab(const OpArgs& o, int n);
+template <> __device__ __forceinline__ float dinv_tab<float>(const OpArgs& o, int n) { return o.dinv_f[n]; }
+template <> __device__ __forceinline__ double dinv_tab<double>(const OpArgs& o, int n) { return o.dinv_d[n]; }
+template <class A> __device__ __forceinline__ A qcoef(const OpArgs& o);
+template <> __device__ __forceinline__ float qcoef<float>(const OpArgs& o) { return o.qc_f; }
+template <> __device__ __forceinline__ double qcoef<double>(const OpArgs& o) { return o.qc_d; }
 
 struct Launch {
   int nblk;
@@ -177,10 +193,7 @@ template <class A, int OPk>
 __device__ __forceinline__ A inv_diag(const OpArgs& o, int64_t idx, uint8_t b) {
   if (!o.precond || !(b & NB_UNKNOWN)) return A(1);
   if (b & NB_SLOW) return inv_diag_generic<A, OPk>(o, idx);
-  const int n = __popc((unsigned)(b & 63));
-  if (n == 0) return A(1);
-  const A be = (OPk == OP_UNIT) ? A(1) : (A)o.il;
-  return -((A)o.g.h * (A)o.g.h) / ((A)n * be);
+  return dinv_tab<A>(o, __popc((unsigned)(b & 63)));
 }
 
 // (A x)[idx] with the mask byte: fast path for liquid cells of uniform density.
@@ -195,8 +208,7 @@ __device__ __forceinline__ A apply_op(const OpArgs& o, const X* __restrict__ x, 
 #pragma unroll
   for (int q = 0; q < 6; ++q)
     if (b & (1u << q)) s += (A)x[idx + st[q]] - xc;
-  const A be = (OPk == OP_UNIT) ? A(1) : (A)o.il;
-  return (be * s) / ((A)o.g.h * (A)o.g.h);
+  return s * qcoef<A>(o);
 }
 
 // ---- deterministic grid reductions, reduced by the consumer ----
@@ -211,10 +223,10 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block-wide sum of N values; the result is returned in every thread.
-template <int N>
+// Block-wide sum of N values over NT threads; the result is returned in every thread.
+template <int N, int NT = PB>
 __device__ __forceinline__ void block_sum_all(double (&v)[N]) {
-  __shared__ double lds[PB / 64][N];
+  __shared__ double lds[NT / 64][N];
   const int lane = __lane_id(), w = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < N; ++q) v[q] = wave_sum(v[q]);
@@ -226,30 +238,30 @@ __device__ __forceinline__ void block_sum_all(double (&v)[N]) {
   for (int q = 0; q < N; ++q) {
     double s = 0.0;
 #pragma unroll
-    for (int ww = 0; ww < PB / 64; ++ww) s += lds[ww][q];
+    for (int ww = 0; ww < NT / 64; ++ww) s += lds[ww][q];
     v[q] = s;
   }
   __syncthreads();
 }
 
 // This block's partial sums -> P[blockIdx.x * N + q].
-template <int N>
+template <int N, int NT = PB>
 __device__ __forceinline__ void write_partials(double (&v)[N], double* __restrict__ P) {
-  block_sum_all<N>(v);
+  block_sum_all<N, NT>(v);
   if (threadIdx.x == 0)
 #pragma unroll
     for (int q = 0; q < N; ++q) P[(int64_t)blockIdx.x * N + q] = v[q];
 }
 
 // Sum of the previous kernel's partials (nb blocks), identical in every block.
-template <int N>
+template <int N, int NT = PB>
 __device__ __forceinline__ void read_partials(const double* __restrict__ P, int nb, double (&out)[N]) {
 #pragma unroll
   for (int q = 0; q < N; ++q) out[q] = 0.0;
-  for (int b = threadIdx.x; b < nb; b += PB)
+  for (int b = threadIdx.x; b < nb; b += NT)
 #pragma unroll
     for (int q = 0; q < N; ++q) out[q] += P[(int64_t)b * N + q];
-  block_sum_all<N>(out);
+  block_sum_all<N, NT>(out);
 }
 
 __device__ __forceinline__ int64_t chunk_begin(const Launch& L) {
@@ -306,17 +318,71 @@ __global__ void __launch_bounds__(PB) k_init(OpArgs o, Launch L, const V* __rest
   }
 }
 
+// ---- spmv tiling ----
+constexpr int TJ = 16, TK = 64;               // (j, k) column tile
+constexpr int ST = 512;                       // threads per spmv block
+constexpr int RPT = TJ * TK / ST;             // tile rows per thread (2)
+constexpr int LJ = TJ + 2, LK = TK + 2;       // LDS plane with a one-cell ring
+constexpr int NRING = 2 * TK + 2 * TJ;        // ring cells per plane (160)
+static_assert(NRING <= ST, "one ring cell per thread");
+
+struct Tiling {
+  int ntj, ntk, seg, nblk;
+};
+
+// p_k of one cell (0 off the unknowns and outside the grid).
+template <class V, int OPk>
+__device__ __forceinline__ V p_of(const OpArgs& o, const V* __restrict__ r, const V* __restrict__ pp, V be, int i,
+                                  int j, int k, uint8_t& nbo) {
+  const Grid& g = o.g;
+  nbo = 0;
+  if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return V(0);
+  const int64_t idx = cid(g, i, j, k);
+  const uint8_t nb = o.nbm[idx];
+  nbo = nb;
+  if (!(nb & NB_UNKNOWN)) return V(0);
+  return r[idx] * inv_diag<V, OPk>(o, idx, nb) + be * pp[idx];
+}
+
+// q of a slow row (flags, per-face 1/rho, heat diagonal) from the p_k of its
+// six neighbours (order +x, +y, +z, -x, -y, -z; 0 off the unknowns).
+template <class V, int OPk>
+__device__ __attribute__((noinline)) V slow_row(const OpArgs& o, int i, int j, int k, int64_t idx, V pcv, V n0, V n1,
+                                                V n2, V n3, V n4, V n5) {
+  const Grid& g = o.g;
+  const V pn[6] = {n0, n1, n2, n3, n4, n5};
+  const int8_t m = o.mark[idx];
+  V s = V(0);
+  if (OPk == OP_HEAT) s = ((V)o.hal[m] + V(6)) * pcv;
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    const int8_t mn = mark_at(g, o.mark, i + c_off[f][0], j + c_off[f][1], k + c_off[f][2]);
+    if (neumann(o, OPk, mn)) {
+      if (OPk == OP_HEAT) s -= pcv;
+      continue;
+    }
+    if (OPk == OP_HEAT) s -= pn[f];
+    else {
+      const int64_t n = cid(g, i + c_off[f][0], j + c_off[f][1], k + c_off[f][2]);
+      const V bf = (OPk == OP_UNIT) ? V(1) : face_beta<V>(o, idx, n, m, mn);
+      s += bf * (pn[f] - pcv);
+    }
+  }
+  return (OPk == OP_HEAT) ? s : s / ((V)g.h * (V)g.h);
+}
+
 // Iteration k.  From update(k-1): rz_k, rr_k (slot (k-1)&1) and rz_{k-1}
-// (slot k&1) -> convergence, beta.  Then p_k = r*dinv + beta p_{k-1} (own cell
-// and neighbours), stored; q = A p_k; partial p.q.
+// (slot k&1) -> convergence, beta.  Then p_k = r*dinv + beta p_{k-1} for the
+// block's tile (plus its ring), stored; q = A p_k; partial p.q.
 template <class V, class Dacc, int OPk>
-__global__ void __launch_bounds__(PB) k_spmv(OpArgs o, Launch L, const V* __restrict__ r, const V* __restrict__ pp,
-                                             V* __restrict__ pc, V* __restrict__ q, Parts P, SolveScalars* sc, int k,
-                                             int ref, double eps_abs, double tol, int cap) {
+__global__ void __launch_bounds__(ST) k_spmv(OpArgs o, Tiling T, int nbu, const V* __restrict__ r,
+                                             const V* __restrict__ pp, V* __restrict__ pc, V* __restrict__ q,
+                                             Parts P, SolveScalars* sc, int k, int ref, double eps_abs, double tol,
+                                             int cap) {
   if (sc->done) return;
   double cur[2], prv[2];
-  read_partials<2>(uslot(P, k - 1), L.nblk, cur);
-  if (k > 0) read_partials<2>(uslot(P, k), L.nblk, prv);
+  read_partials<2, ST>(uslot(P, k - 1), nbu, cur);
+  if (k > 0) read_partials<2, ST>(uslot(P, k), nbu, prv);
   else prv[0] = prv[1] = 0.0;
   const double bb = (k == 0) ? cur[1] : sc->bb;
   const bool nonfin = bad(cur[0]) || bad(cur[1]);
@@ -340,58 +406,100 @@ __global__ void __launch_bounds__(PB) k_spmv(OpArgs o, Launch L, const V* __rest
     sc->iter = k;
     if (k == 0) sc->bb = bb;
   }
+  __shared__ V pl[3][LJ][LK];
   const Grid& g = o.g;
-  int64_t st[6];
-  nb_strides(g, st);
+  int lb = blockIdx.x;
+  if ((T.nblk & 7) == 0) lb = (lb & 7) * (T.nblk >> 3) + (lb >> 3);  // XCD-contiguous tiles
+  const int tk = lb % T.ntk;
+  const int rest = lb / T.ntk;
+  const int tj = rest % T.ntj;
+  const int i_lo = (rest / T.ntj) * T.seg;
+  const int i_hi = min(i_lo + T.seg, g.nx);
+  const int j0 = tj * TJ, k0 = tk * TK;
+  const int tid = threadIdx.x;
+  const int kl = tid & (TK - 1), jl0 = (tid >> 6) * RPT;
+  const int kk = k0 + kl;
+  // this thread's ring cell (local coords in [-1, TJ] x [-1, TK])
+  int hj = 0, hk = 0;
+  const bool ring = tid < NRING;
+  if (tid < 2 * TK) {
+    hj = (tid < TK) ? -1 : TJ;
+    hk = tid & (TK - 1);
+  } else if (ring) {
+    const int t2 = tid - 2 * TK;
+    hj = t2 % TJ;
+    hk = (t2 < TJ) ? -1 : TK;
+  }
+  V pprev[RPT], pcur[RPT], pnext[RPT];
+  uint8_t ncur[RPT], nnext[RPT];
+  uint8_t dummy;
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    pprev[u] = p_of<V, OPk>(o, r, pp, be, i_lo - 1, j0 + jl0 + u, kk, dummy);
+    pcur[u] = p_of<V, OPk>(o, r, pp, be, i_lo, j0 + jl0 + u, kk, ncur[u]);
+    pl[i_lo % 3][jl0 + u + 1][kl + 1] = pcur[u];
+  }
+  if (ring) pl[i_lo % 3][hj + 1][hk + 1] = p_of<V, OPk>(o, r, pp, be, i_lo, j0 + hj, k0 + hk, dummy);
+  const int64_t sx = (int64_t)g.ny * g.nz;
+  const V qc = qcoef<V>(o);
   Dacc a_pq = 0;
-  CELLS_OF_BLOCK(g.G) {
-    const uint8_t nb = o.nbm[idx];
-    if (!(nb & NB_UNKNOWN)) continue;
-    const V pcv = r[idx] * inv_diag<V, OPk>(o, idx, nb) + be * pp[idx];
-    pc[idx] = pcv;
-    V qv;
-    if (!(nb & NB_SLOW)) {
-      V s = V(0);
+  for (int i = i_lo; i < i_hi; ++i) {
+    const int sn = (i + 1) % 3, scur = i % 3;
 #pragma unroll
-      for (int f = 0; f < 6; ++f) {
-        if (!(nb & (1u << f))) continue;
-        const int64_t n = idx + st[f];
-        const uint8_t nn = o.nbm[n];
-        const V pn = r[n] * inv_diag<V, OPk>(o, n, nn) + be * pp[n];
-        s += pn - pcv;
-      }
-      const V bc = (OPk == OP_UNIT) ? V(1) : (V)o.il;
-      qv = (bc * s) / ((V)g.h * (V)g.h);
-    } else {
-      // slow row: the operator on p_k with neighbours rebuilt the same way
-      int i, j, kk;
-      ijk_of(g, idx, i, j, kk);
-      const int8_t m = o.mark[idx];
-      V s = V(0);
-      if (OPk == OP_HEAT) s = ((V)o.hal[m] + V(6)) * pcv;
-#pragma unroll
-      for (int f = 0; f < 6; ++f) {
-        const int8_t mn = mark_at(g, o.mark, i + c_off[f][0], j + c_off[f][1], kk + c_off[f][2]);
-        if (neumann(o, OPk, mn)) {
-          if (OPk == OP_HEAT) s -= pcv;
-          continue;
-        }
-        const int64_t n = idx + st[f];
-        const uint8_t nn = o.nbm[n];
-        const V pn = (nn & NB_UNKNOWN) ? r[n] * inv_diag<V, OPk>(o, n, nn) + be * pp[n] : V(0);
-        if (OPk == OP_HEAT) s -= pn;
-        else {
-          const V bf = (OPk == OP_UNIT) ? V(1) : face_beta<V>(o, idx, n, m, mn);
-          s += bf * (pn - pcv);
-        }
-      }
-      qv = (OPk == OP_HEAT) ? s : s / ((V)g.h * (V)g.h);
+    for (int u = 0; u < RPT; ++u) {
+      pnext[u] = p_of<V, OPk>(o, r, pp, be, i + 1, j0 + jl0 + u, kk, nnext[u]);
+      pl[sn][jl0 + u + 1][kl + 1] = pnext[u];
     }
-    q[idx] = qv;
-    a_pq += (Dacc)pcv * (Dacc)qv;
+    if (ring) pl[sn][hj + 1][hk + 1] = p_of<V, OPk>(o, r, pp, be, i + 1, j0 + hj, k0 + hk, dummy);
+    __syncthreads();
+    unsigned slow = 0;
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+      const int jl = jl0 + u;
+      const uint8_t nb = ncur[u];
+      if (!(nb & NB_UNKNOWN)) continue;  // also every cell outside the grid
+      if (nb & NB_SLOW) {
+        slow |= 1u << u;
+        continue;
+      }
+      const int64_t idx = (int64_t)i * sx + (int64_t)(j0 + jl) * g.nz + kk;
+      const V pcv = pcur[u];
+      // open faces in the order +x, +y, +z, -x, -y, -z
+      V s = V(0);
+      if (nb & 1) s += pnext[u] - pcv;
+      if (nb & 2) s += pl[scur][jl + 2][kl + 1] - pcv;
+      if (nb & 4) s += pl[scur][jl + 1][kl + 2] - pcv;
+      if (nb & 8) s += pprev[u] - pcv;
+      if (nb & 16) s += pl[scur][jl][kl + 1] - pcv;
+      if (nb & 32) s += pl[scur][jl + 1][kl] - pcv;
+      const V qv = s * qc;
+      pc[idx] = pcv;
+      q[idx] = qv;
+      a_pq += (Dacc)pcv * (Dacc)qv;
+    }
+    if (slow) {
+#pragma unroll
+      for (int u = 0; u < RPT; ++u) {
+        if (!((slow >> u) & 1u)) continue;
+        const int jl = jl0 + u, j = j0 + jl;
+        const int64_t idx = (int64_t)i * sx + (int64_t)j * g.nz + kk;
+        const V pcv = pcur[u];
+        const V qv = slow_row<V, OPk>(o, i, j, kk, idx, pcv, pnext[u], pl[scur][jl + 2][kl + 1],
+                                      pl[scur][jl + 1][kl + 2], pprev[u], pl[scur][jl][kl + 1], pl[scur][jl + 1][kl]);
+        pc[idx] = pcv;
+        q[idx] = qv;
+        a_pq += (Dacc)pcv * (Dacc)qv;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+      pprev[u] = pcur[u];
+      pcur[u] = pnext[u];
+      ncur[u] = nnext[u];
+    }
   }
   double v[1] = {(double)a_pq};
-  write_partials<1>(v, P.S);
+  write_partials<1, ST>(v, P.S);
 }
 
 // x += alpha p_k (folding x into x64 right after a reliable update);
@@ -399,10 +507,11 @@ __global__ void __launch_bounds__(PB) k_spmv(OpArgs o, Launch L, const V* __rest
 template <class V, class Dacc, int OPk>
 __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict__ x, V* __restrict__ r,
                                                const V* __restrict__ pc, const V* __restrict__ q,
-                                               double* __restrict__ x64, Parts P, SolveScalars* sc, int k, int fold) {
+                                               double* __restrict__ x64, Parts P, int nbs, SolveScalars* sc, int k,
+                                               int fold) {
   if (sc->done) return;
   double t[1];
-  read_partials<1>(P.S, L.nblk, t);
+  read_partials<1>(P.S, nbs, t);
   if (t[0] == 0.0 || bad(t[0])) {  // spray.cu:1654-1656: zp == 0 ends the solve
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->done = 1;
@@ -546,7 +655,25 @@ Launch make_launch(int64_t G) {
   return L;
 }
 
-OpArgs make_op(Ctx& c, int precond) {
+// spmv tiles: 16 x 64 (j, k) columns, planes split along i into runs so the
+// launch has about SPMV_BLOCKS blocks (two 512-thread blocks fit a CU at the
+// kernel's register use: one resident round on 256 CUs).
+constexpr int SPMV_BLOCKS = 512;
+Tiling make_tiling(const Grid& g) {
+  Tiling t;
+  t.ntj = (g.ny + TJ - 1) / TJ;
+  t.ntk = (g.nz + TK - 1) / TK;
+  const int tiles = t.ntj * t.ntk;
+  int nseg = SPMV_BLOCKS / tiles;
+  if (nseg < 1) nseg = 1;
+  if (nseg > g.nx) nseg = g.nx;
+  t.seg = (g.nx + nseg - 1) / nseg;
+  nseg = (g.nx + t.seg - 1) / t.seg;
+  t.nblk = tiles * nseg;
+  return t;
+}
+
+OpArgs make_op(Ctx& c, int precond, int opk) {
   OpArgs o;
   o.g = c.g;
   o.mark = c.mark;
@@ -560,6 +687,17 @@ OpArgs make_op(Ctx& c, int precond) {
   o.solid_c = c.cfg.solid_constraint;
   o.vac_neumann = c.cfg.heat_vacuum_neumann;
   o.precond = precond;
+  // fast-path tables in each precision's own arithmetic
+  const float hf = c.g.h, bef = (opk == OP_UNIT) ? 1.0f : (float)o.il;
+  const double hd = (double)c.g.h, bed = (opk == OP_UNIT) ? 1.0 : o.il;
+  o.dinv_f[0] = 1.0f;
+  o.dinv_d[0] = 1.0;
+  for (int n = 1; n <= 6; ++n) {
+    o.dinv_f[n] = -(hf * hf) / ((float)n * bef);
+    o.dinv_d[n] = -(hd * hd) / ((double)n * bed);
+  }
+  o.qc_f = bef / (hf * hf);
+  o.qc_d = bed / (hd * hd);
   return o;
 }
 
@@ -578,7 +716,12 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
              mpf_solve_info* info) {
   const int64_t G = c.g.G;
   const Launch L = make_launch(G);
-  const OpArgs o = make_op(c, precond);
+  const Tiling T = make_tiling(c.g);
+  const OpArgs o = make_op(c, precond, OPk);
+  if (T.nblk > 2 * MAX_BLK) {
+    c.err = "grid too large for the solver's reduction buffers";
+    return MPF_INVALID_ARG;
+  }
   V* r = (V*)c.vr;
   V* pbuf[2] = {(V*)c.vp, (V*)c.vdinv};
   V* q = (V*)c.vq;
@@ -595,10 +738,10 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
   while (k <= cap) {
     const int k_end = k + check;
     for (; k < k_end && k <= cap; ++k) {
-      launch(c, "pcg_spmv", dim3(L.nblk), dim3(PB), 0, k_spmv<V, Dacc, OPk>, o, L, (const V*)r,
+      launch(c, "pcg_spmv", dim3(T.nblk), dim3(ST), 0, k_spmv<V, Dacc, OPk>, o, T, L.nblk, (const V*)r,
              (const V*)pbuf[k & 1], pbuf[(k + 1) & 1], q, P, c.scal, k, (int)ref, eps_abs, tol, cap);
       launch(c, "pcg_update", dim3(L.nblk), dim3(PB), 0, k_update<V, Dacc, OPk>, o, L, x, r,
-             (const V*)pbuf[(k + 1) & 1], (const V*)q, mixed ? x64 : (double*)nullptr, P, c.scal, k,
+             (const V*)pbuf[(k + 1) & 1], (const V*)q, mixed ? x64 : (double*)nullptr, P, T.nblk, c.scal, k,
              (int)fold_next);
       fold_next = false;
       if (mixed && (k + 1) % R == 0) {

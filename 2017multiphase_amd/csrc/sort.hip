@@ -199,6 +199,18 @@ __global__ void k_cell_mask(int64_t G, const uint32_t* __restrict__ cstart, cons
   cmask[c] = m;
 }
 
+// Cells whose list does not index the particle array: start != undefined and
+// not start < end <= P.  Caller-supplied lists are checked before any gather
+// walks them.
+__global__ void k_cell_check(int64_t G, int64_t P, const uint32_t* __restrict__ cstart,
+                             const uint32_t* __restrict__ cend, uint32_t* __restrict__ bad) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G) return;
+  const uint32_t s = cstart[c], e = cend[c];
+  const bool ok = (s == MPF_CELL_UNDEF) || (s < e && (int64_t)e <= P);
+  if (!ok) atomicAdd(bad, 1u);
+}
+
 // ---- exclusive scan of uint32 (reduce, scan of block sums, apply) ----------
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
   const int lane = lane_id();
@@ -310,6 +322,15 @@ int64_t run_scan_u32(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n, bool 
   return (int64_t)tot;
 }
 
+int64_t run_check_cells(Ctx& c) {
+  hipMemsetAsync(c.counters + 6, 0, 4, c.stream);
+  launch(c, "cell_check", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_cell_check, c.g.G, c.P,
+         (const uint32_t*)c.cstart, (const uint32_t*)c.cend, c.counters + 6);
+  hipMemcpyAsync(c.counters_host + 6, c.counters + 6, 4, hipMemcpyDeviceToHost, c.stream);
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "cell list check")) return -1;
+  return c.counters_host[6];
+}
+
 void run_cell_mask(Ctx& c) {
   const int64_t G = c.g.G;
   launch(c, "cell_mask", dim3(nblocks(G, 256)), dim3(256), 0, k_cell_mask, G, (const uint32_t*)c.cstart,
@@ -335,6 +356,7 @@ void run_hash_sort(Ctx& c) {
          (const float*)c.gas[cur], c.pos[nxt], c.vel[nxt], c.flag[nxt], c.mass[nxt], c.temp[nxt], c.lheat[nxt],
          c.solub[nxt], c.gas[nxt]);
   c.cur = nxt;
+  c.user_cstart = c.user_cend = false;
   c.sorted = true;
   run_cell_mask(c);
 }

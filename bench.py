@@ -154,6 +154,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=64)
     ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (a kernel of known HBM "
+                         "traffic, for scaling profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the z-slab multi-GPU path is not built; run with --gpus 1",
@@ -161,6 +164,16 @@ def main():
         sys.exit(2)
     pkg = importlib.import_module("2017multiphase_amd")
     scenes = importlib.import_module("2017multiphase_amd.scenes")
+    if args.copy_calibration_mib > 0:
+        import torch
+        n = args.copy_calibration_mib << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copy calibration: %d bytes read, %d bytes written" % (n, n))
+        del src, dst
     mixed = run_form(pkg, scenes, args.scene, args.n, pkg.MIXED, args.steps, args.warmup)
     fp64 = None
     if not args.no_fp64:

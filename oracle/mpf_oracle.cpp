@@ -296,28 +296,30 @@ template <class T> struct Orc : OrcBase {
       lsair[idx] = mair;
     }
   }
-  // solvedistance, spray_k.cu:746-754
-  static inline void solvedistance(T a, T b, T cc, T& x) {
-    T d = tmin(a, tmin(b, cc)) + T(1);
+  // solvedistance, spray_k.cu:746-754 (arithmetic S)
+  template <class S>
+  static inline void solvedistance(S a, S b, S cc, S& x) {
+    S d = tmin(a, tmin(b, cc)) + S(1);
     if (d > tmax(a, tmax(b, cc)))
-      d = (a + b + cc + std::sqrt(T(3) - (a - b) * (a - b) - (a - cc) * (a - cc) - (b - cc) * (b - cc))) / T(3);
+      d = (a + b + cc + std::sqrt(S(3) - (a - b) * (a - b) - (a - cc) * (a - cc) - (b - cc) * (b - cc))) / S(3);
     if (d < x) x = d;
   }
   // sweepphibytype (spray_k.cu:771-789) as a two-buffer sweep; typeflag < 0
   // sweeps every cell (sweepphi, spray_k.cu:755-770).
-  void sweep(std::vector<T>& phi, const std::vector<int8_t>& mk_, int typeflag) {
-    std::vector<T> out(phi);
+  template <class S>
+  void sweep(std::vector<S>& phi, const std::vector<int8_t>& mk_, int typeflag) {
+    std::vector<S> out(phi);
 #pragma omp parallel for schedule(static)
     for (int64_t idx = 0; idx < G; ++idx) {
       if (typeflag >= 0 && mk_[idx] == typeflag) continue;
       int i, j, k;
       ijk(idx, i, j, k);
-      T res = phi[idx];
+      S res = phi[idx];
       for (int di = -1; di <= 1; di += 2)
         for (int dj = -1; dj <= 1; dj += 2)
           for (int dk = -1; dk <= 1; dk += 2)
             if (valid(i + di, j, k) && valid(i, j + dj, k) && valid(i, j, k + dk))
-              solvedistance(phi[cid(i + di, j, k)], phi[cid(i, j + dj, k)], phi[cid(i, j, k + dk)], res);
+              solvedistance<S>(phi[cid(i + di, j, k)], phi[cid(i, j + dj, k)], phi[cid(i, j, k + dk)], res);
       out[idx] = res;
     }
     phi.swap(out);
@@ -469,9 +471,16 @@ template <class T> struct Orc : OrcBase {
 
   // ================= a10: extrapolation and wall conditions =================
   // initphi spray_k.cu:712-723 + 3 x sweepphibytype: sweepPhi spray.cu:570-582
+  // The extrapolation's distance field and the stencil choices that read it
+  // are made in float in both instantiations, as the library makes them, so
+  // the FP64 form takes the same discrete choices (several are sign tests of
+  // sums that vanish in exact arithmetic); only the blend of velocities runs
+  // in T.
   void sweep_phi(std::vector<T>& phi, int8_t type) {
-    for (int64_t idx = 0; idx < G; ++idx) phi[idx] = (mark[idx] == type) ? T(-0.5) : (T)(NX * 3);
-    for (int it = 0; it < 3; ++it) sweep(phi, mark, type);
+    std::vector<float> pf(G);
+    for (int64_t idx = 0; idx < G; ++idx) pf[idx] = (mark[idx] == type) ? -0.5f : (float)(NX * 3);
+    for (int it = 0; it < 3; ++it) sweep<float>(pf, mark, type);
+    for (int64_t idx = 0; idx < G; ++idx) phi[idx] = (T)pf[idx];
   }
   // sweepu_k_bubble spray_k.cu:3779-3880 (two ping-pong passes, spray.cu:584-596)
   void sweep_u_pass(std::vector<T>* out, const std::vector<T>* u, const std::vector<T>& ls, int8_t sf) {
@@ -500,9 +509,9 @@ template <class T> struct Orc : OrcBase {
                 if (n[q] + dd[q] < 0 || n[q] + dd[q] > lq - 1) bad = true;
               }
               if (bad) continue;
-              T lsc = ls[cid(i, j, k)], lsb = ls[cid(bi, bj, bk)];
-              T wv[3];
-              wv[a] = (T)(-dd[a]) * (lsc - lsb);
+              const float lsc = (float)ls[cid(i, j, k)], lsb = (float)ls[cid(bi, bj, bk)];
+              float wv[3];
+              wv[a] = (float)(-dd[a]) * (lsc - lsb);
               if (wv[a] < 0) continue;
               // the two transverse weights, in the reference's order
               int t1 = (a == 0) ? 1 : (a == 1) ? 0 : 1;  // x-face: wy then wz; y-face: wx then wz; z-face: wy then wx
@@ -512,18 +521,18 @@ template <class T> struct Orc : OrcBase {
                 int q = tt == 0 ? t1 : t2;
                 int o[3] = {0, 0, 0};
                 o[q] = dd[q];
-                T lsn = ls[cid(i + o[0], j + o[1], k + o[2])];
-                T lsnb = ls[cid(bi + o[0], bj + o[1], bk + o[2])];
-                wv[q] = (lsc + lsb - lsn - lsnb) * T(0.5f);
+                const float lsn = (float)ls[cid(i + o[0], j + o[1], k + o[2])];
+                const float lsnb = (float)ls[cid(bi + o[0], bj + o[1], bk + o[2])];
+                wv[q] = (lsc + lsb - lsn - lsnb) * 0.5f;
                 if (wv[q] < 0) skip = true;
               }
               if (skip) continue;
-              T wx = wv[0], wy = wv[1], wz = wv[2];
-              T wsum = wx + wy + wz;
-              if (wsum == 0) wx = wy = wz = T(1.0f) / T(3);
+              float wx = wv[0], wy = wv[1], wz = wv[2];
+              const float wsum = wx + wy + wz;
+              if (wsum == 0) wx = wy = wz = 1.0f / 3;
               else { wx /= wsum; wy /= wsum; wz /= wsum; }
               auto U = [&](int ii, int jj, int kk) { return u[a][fid(a, ii, jj, kk)]; };
-              out[a][f] = wx * U(i + di, j, k) + wy * U(i, j + dj, k) + wz * U(i, j, k + dk);
+              out[a][f] = (T)wx * U(i + di, j, k) + (T)wy * U(i, j + dj, k) + (T)wz * U(i, j, k + dk);
             }
       }
     }

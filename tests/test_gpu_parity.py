@@ -102,6 +102,35 @@ def test_hash_sort_edge_cases(pkg, orc):
     assert np.all(sim.get_field("CELL_START") == pkg.CELL_UNDEF)
 
 
+def test_caller_cell_lists(pkg, scenes):
+    """Cell lists given through mpf_set_field take effect once both arrays are
+    set and every list indexes the particles; others are refused before any
+    kernel walks them (a fresh context's CELL_END is uninitialised)."""
+    say("caller cell lists")
+    s = scenes.make("bubble_rise", n=16)
+    ref = pkg.Simulator(s["cfg"])
+    load_scene(ref, s)
+    ref.hash_sort()
+    p = ref.get_particles()
+    cs, ce = ref.get_field("CELL_START"), ref.get_field("CELL_END")
+    ref.p2g()
+    u_ref = ref.get_field("WATER_UZ")
+    sim = pkg.Simulator(s["cfg"])
+    sim.set_particles(len(p["flag"]), **p)
+    sim.set_field("CELL_START", cs)  # CELL_END not given yet: no gather may run
+    with pytest.raises(pkg.MpfError):
+        sim.p2g()
+    bad = ce.copy()
+    bad[np.flatnonzero(cs != pkg.CELL_UNDEF)[0]] = len(p["flag"]) + 5
+    with pytest.raises(pkg.MpfError):
+        sim.set_field("CELL_END", bad)
+    with pytest.raises(pkg.MpfError):
+        sim.p2g()
+    sim.set_field("CELL_END", ce)
+    sim.p2g()
+    bitexact(sim.get_field("WATER_UZ"), u_ref, "WATER_UZ from caller cell lists")
+
+
 @pytest.mark.parametrize("name,n", SCENES)
 def test_levelset(pkg, scenes, orc, name, n):
     say("levelset", name, n)
