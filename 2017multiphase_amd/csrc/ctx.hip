@@ -114,7 +114,7 @@ void free_all(Ctx& c) {
   for (int a = 0; a < 3; ++a) dfree(c.phig[a]);
   dfree(c.phifluid); dfree(c.phiair); dfree(c.phisolid); dfree(c.ctmp);
   dfree(c.div); dfree(c.press); dfree(c.tp); dfree(c.tpchange); dfree(c.tsave);
-  dfree(c.vx); dfree(c.vr); dfree(c.vp); dfree(c.vq); dfree(c.vdinv); dfree(c.nbm);
+  dfree(c.vx); dfree(c.vr); dfree(c.vp); dfree(c.vq); dfree(c.vp2); dfree(c.nbm); dfree(c.dslow);
   dfree(c.partials); dfree(c.scal); dfree(c.counters);
   for (int a = 0; a < 3; ++a) {
     dfree(c.wu[a]); dfree(c.au[a]); dfree(c.wuo[a]); dfree(c.auo[a]); dfree(c.su[a]); dfree(c.ftmp[a]);
@@ -160,8 +160,8 @@ bool alloc_all(Ctx& c) {
   ok = ok && dalloc(c, dd, G); c.vr = dd;
   ok = ok && dalloc(c, dd, G); c.vp = dd;
   ok = ok && dalloc(c, dd, G); c.vq = dd;
-  ok = ok && dalloc(c, dd, G); c.vdinv = dd;
-  ok = ok && dalloc(c, c.nbm, G);
+  ok = ok && dalloc(c, dd, G); c.vp2 = dd;
+  ok = ok && dalloc(c, c.nbm, G) && dalloc(c, c.dslow, G);
   c.npart = 2048 * 4;
   ok = ok && dalloc(c, c.partials, c.npart) && dalloc(c, c.scal, 1) && dalloc(c, c.counters, 64);
   for (int a = 0; a < 3 && ok; ++a)

@@ -163,8 +163,9 @@ struct Ctx {
   void* press = nullptr;  // float under MPF_FP32, else double
   float *tp = nullptr, *tpchange = nullptr, *tsave = nullptr;
   // iterative solve work vectors (sized for double)
-  void *vx = nullptr, *vr = nullptr, *vp = nullptr, *vq = nullptr, *vdinv = nullptr;
+  void *vx = nullptr, *vr = nullptr, *vp = nullptr, *vq = nullptr, *vp2 = nullptr;
   uint8_t* nbm = nullptr;  // per-cell neighbour mask of the current solve
+  double* dslow = nullptr;  // 1/diag of the current solve's slow rows (flags-dependent)
   double* vx64 = nullptr;
   double* partials = nullptr;
   int npart = 0;

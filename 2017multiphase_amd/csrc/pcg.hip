@@ -46,6 +46,7 @@ struct OpArgs {
   const int8_t* mark;
   const float* ls;
   const uint8_t* nbm;
+  const double* dslow;  // 1/diag of slow rows, filled with the mask
   double il, ia;    // 1/rho_liquid, 1/rho_air
   double hal[6];    // h^2/(alpha*dt) per cell type (heat)
   int solid_c;      // SOLID cells Neumann (pressure)
@@ -107,28 +108,6 @@ __device__ __forceinline__ A face_beta(const OpArgs& o, int64_t ca, int64_t cb, 
 }
 
 // Neighbour mask of one cell.
-template <int OPk>
-__global__ void k_nbm(OpArgs o, uint8_t* __restrict__ nbm) {
-  const Grid& g = o.g;
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= g.G) return;
-  int i, j, k;
-  ijk_of(g, idx, i, j, k);
-  const int8_t m = o.mark[idx];
-  uint8_t b = 0;
-  if (is_unknown(OPk, m)) b |= NB_UNKNOWN;
-  bool slow = (OPk == OP_VARDENS) && m == AIR;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    int8_t mn = mark_at(g, o.mark, i + c_off[q][0], j + c_off[q][1], k + c_off[q][2]);
-    if (!neumann(o, OPk, mn)) b |= (uint8_t)(1u << q);
-    if (OPk == OP_VARDENS && m == FLUID && mn == AIR) slow = true;
-  }
-  if (OPk == OP_HEAT) slow = true;  // the heat operator reads the flags (diffusivity per type)
-  if (slow) b |= NB_SLOW;
-  nbm[idx] = b;
-}
-
 // Generic operator row (flags and lsmerge), arithmetic A, x stored as X.
 template <class A, int OPk, class X>
 __device__ __forceinline__ A apply_generic(const OpArgs& o, const X* __restrict__ x, int64_t idx) {
@@ -187,12 +166,35 @@ __device__ __forceinline__ A inv_diag_generic(const OpArgs& o, int64_t idx) {
   return -((A)g.h * (A)g.h) / s;
 }
 
+template <int OPk>
+__global__ void k_nbm(OpArgs o, uint8_t* __restrict__ nbm, double* __restrict__ dslow) {
+  const Grid& g = o.g;
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= g.G) return;
+  int i, j, k;
+  ijk_of(g, idx, i, j, k);
+  const int8_t m = o.mark[idx];
+  uint8_t b = 0;
+  if (is_unknown(OPk, m)) b |= NB_UNKNOWN;
+  bool slow = (OPk == OP_VARDENS) && m == AIR;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    int8_t mn = mark_at(g, o.mark, i + c_off[q][0], j + c_off[q][1], k + c_off[q][2]);
+    if (!neumann(o, OPk, mn)) b |= (uint8_t)(1u << q);
+    if (OPk == OP_VARDENS && m == FLUID && mn == AIR) slow = true;
+  }
+  if (OPk == OP_HEAT) slow = true;  // the heat operator reads the flags (diffusivity per type)
+  if (slow) b |= NB_SLOW;
+  nbm[idx] = b;
+  if (slow && (b & NB_UNKNOWN)) dslow[idx] = inv_diag_generic<double, OPk>(o, idx);
+}
+
 // 1/diag from the mask byte (fast) or the flags (slow); 1 off the unknowns or
 // without preconditioning.
 template <class A, int OPk>
 __device__ __forceinline__ A inv_diag(const OpArgs& o, int64_t idx, uint8_t b) {
   if (!o.precond || !(b & NB_UNKNOWN)) return A(1);
-  if (b & NB_SLOW) return inv_diag_generic<A, OPk>(o, idx);
+  if (b & NB_SLOW) return (A)o.dslow[idx];
   return dinv_tab<A>(o, __popc((unsigned)(b & 63)));
 }
 
@@ -285,6 +287,7 @@ struct Parts {
   double* U0;
   double* U1;
   double* S;
+  double* S2;  // p.q of the slow rows (k_slow_q)
 };
 __device__ __forceinline__ double* uslot(const Parts& P, int k) { return (k & 1) ? P.U1 : P.U0; }
 
@@ -330,25 +333,50 @@ struct Tiling {
   int ntj, ntk, seg, nblk;
 };
 
-// p_k of one cell (0 off the unknowns and outside the grid).
-template <class V, int OPk>
-__device__ __forceinline__ V p_of(const OpArgs& o, const V* __restrict__ r, const V* __restrict__ pp, V be, int i,
-                                  int j, int k, uint8_t& nbo) {
+// The inputs of p_k at one cell: r, p_{k-1} and the mask byte (nb = 0 outside
+// the grid, where nothing is loaded).
+template <class V>
+struct Raw {
+  V r, pp;
+  uint8_t nb;
+};
+template <class V>
+__device__ __forceinline__ Raw<V> load_raw(const OpArgs& o, const V* __restrict__ r, const V* __restrict__ pp, int i,
+                                           int j, int k) {
   const Grid& g = o.g;
-  nbo = 0;
-  if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return V(0);
+  Raw<V> w{V(0), V(0), 0};
+  if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return w;
   const int64_t idx = cid(g, i, j, k);
-  const uint8_t nb = o.nbm[idx];
-  nbo = nb;
-  if (!(nb & NB_UNKNOWN)) return V(0);
-  return r[idx] * inv_diag<V, OPk>(o, idx, nb) + be * pp[idx];
+  w.nb = o.nbm[idx];
+  w.r = r[idx];
+  w.pp = pp[idx];
+  return w;
+}
+// p_k from the loaded inputs; 0 off the unknowns (r and p_{k-1} are 0 there).
+// A slow row's diagonal reads the flags around it.
+template <class V, int OPk>
+__device__ __forceinline__ V p_raw(const OpArgs& o, const Raw<V>& w, V be, int i, int j, int k) {
+  if (!(w.nb & NB_UNKNOWN)) return V(0);
+  V d = V(1);
+  if (o.precond) {
+    if (!(w.nb & NB_SLOW)) d = dinv_tab<V>(o, __popc((unsigned)(w.nb & 63)));
+    else d = (V)o.dslow[cid(o.g, i, j, k)];
+  }
+  return w.r * d + be * w.pp;
+}
+// Workgroup barrier that orders LDS only: global loads issued before it stay
+// in flight (a full __syncthreads would wait for them).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // q of a slow row (flags, per-face 1/rho, heat diagonal) from the p_k of its
 // six neighbours (order +x, +y, +z, -x, -y, -z; 0 off the unknowns).
 template <class V, int OPk>
-__device__ __attribute__((noinline)) V slow_row(const OpArgs& o, int i, int j, int k, int64_t idx, V pcv, V n0, V n1,
-                                                V n2, V n3, V n4, V n5) {
+__device__ __forceinline__ V slow_row(const OpArgs& o, int i, int j, int k, int64_t idx, V pcv, V n0, V n1, V n2, V n3,
+                                      V n4, V n5) {
   const Grid& g = o.g;
   const V pn[6] = {n0, n1, n2, n3, n4, n5};
   const int8_t m = o.mark[idx];
@@ -375,7 +403,7 @@ __device__ __attribute__((noinline)) V slow_row(const OpArgs& o, int i, int j, i
 // (slot k&1) -> convergence, beta.  Then p_k = r*dinv + beta p_{k-1} for the
 // block's tile (plus its ring), stored; q = A p_k; partial p.q.
 template <class V, class Dacc, int OPk>
-__global__ void __launch_bounds__(ST) k_spmv(OpArgs o, Tiling T, int nbu, const V* __restrict__ r,
+__global__ void __launch_bounds__(ST, 4) k_spmv(OpArgs o, Tiling T, int nbu, const V* __restrict__ r,
                                              const V* __restrict__ pp, V* __restrict__ pc, V* __restrict__ q,
                                              Parts P, SolveScalars* sc, int k, int ref, double eps_abs, double tol,
                                              int cap) {
@@ -430,28 +458,49 @@ __global__ void __launch_bounds__(ST) k_spmv(OpArgs o, Tiling T, int nbu, const 
     hj = t2 % TJ;
     hk = (t2 < TJ) ? -1 : TK;
   }
+  // Software pipeline over planes: at step i the loads of plane i+2 are
+  // issued, p of plane i+1 is formed from the loads issued one step earlier
+  // and written to LDS, and q of plane i is computed.  The barrier only
+  // orders LDS (the plane i+2 loads stay in flight across it).
   V pprev[RPT], pcur[RPT], pnext[RPT];
   uint8_t ncur[RPT], nnext[RPT];
-  uint8_t dummy;
+  Raw<V> ra[RPT], rb[RPT], hra, hrb;  // raw loads of planes i+1 (a) and i+2 (b)
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
-    pprev[u] = p_of<V, OPk>(o, r, pp, be, i_lo - 1, j0 + jl0 + u, kk, dummy);
-    pcur[u] = p_of<V, OPk>(o, r, pp, be, i_lo, j0 + jl0 + u, kk, ncur[u]);
+    ra[u] = load_raw<V>(o, r, pp, i_lo - 1, j0 + jl0 + u, kk);
+    rb[u] = load_raw<V>(o, r, pp, i_lo, j0 + jl0 + u, kk);
+  }
+  hra = load_raw<V>(o, r, pp, ring ? i_lo : -1, j0 + hj, k0 + hk);
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    pprev[u] = p_raw<V, OPk>(o, ra[u], be, i_lo - 1, j0 + jl0 + u, kk);
+    pcur[u] = p_raw<V, OPk>(o, rb[u], be, i_lo, j0 + jl0 + u, kk);
+    ncur[u] = rb[u].nb;
     pl[i_lo % 3][jl0 + u + 1][kl + 1] = pcur[u];
   }
-  if (ring) pl[i_lo % 3][hj + 1][hk + 1] = p_of<V, OPk>(o, r, pp, be, i_lo, j0 + hj, k0 + hk, dummy);
+  if (ring) pl[i_lo % 3][hj + 1][hk + 1] = p_raw<V, OPk>(o, hra, be, i_lo, j0 + hj, k0 + hk);
+  // plane i_lo + 1 in flight
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) ra[u] = load_raw<V>(o, r, pp, i_lo + 1, j0 + jl0 + u, kk);
+  hra = load_raw<V>(o, r, pp, ring ? i_lo + 1 : -1, j0 + hj, k0 + hk);
   const int64_t sx = (int64_t)g.ny * g.nz;
   const V qc = qcoef<V>(o);
   Dacc a_pq = 0;
   for (int i = i_lo; i < i_hi; ++i) {
     const int sn = (i + 1) % 3, scur = i % 3;
+    // issue plane i+2
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) rb[u] = load_raw<V>(o, r, pp, i + 2, j0 + jl0 + u, kk);
+    hrb = load_raw<V>(o, r, pp, ring ? i + 2 : -1, j0 + hj, k0 + hk);
+    // form plane i+1
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
-      pnext[u] = p_of<V, OPk>(o, r, pp, be, i + 1, j0 + jl0 + u, kk, nnext[u]);
+      pnext[u] = p_raw<V, OPk>(o, ra[u], be, i + 1, j0 + jl0 + u, kk);
+      nnext[u] = ra[u].nb;
       pl[sn][jl0 + u + 1][kl + 1] = pnext[u];
     }
-    if (ring) pl[sn][hj + 1][hk + 1] = p_of<V, OPk>(o, r, pp, be, i + 1, j0 + hj, k0 + hk, dummy);
-    __syncthreads();
+    if (ring) pl[sn][hj + 1][hk + 1] = p_raw<V, OPk>(o, hra, be, i + 1, j0 + hj, k0 + hk);
+    lds_barrier();
     unsigned slow = 0;
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
@@ -477,29 +526,61 @@ __global__ void __launch_bounds__(ST) k_spmv(OpArgs o, Tiling T, int nbu, const 
       q[idx] = qv;
       a_pq += (Dacc)pcv * (Dacc)qv;
     }
-    if (slow) {
+    if (slow) {  // q of slow rows: k_slow_q
 #pragma unroll
-      for (int u = 0; u < RPT; ++u) {
-        if (!((slow >> u) & 1u)) continue;
-        const int jl = jl0 + u, j = j0 + jl;
-        const int64_t idx = (int64_t)i * sx + (int64_t)j * g.nz + kk;
-        const V pcv = pcur[u];
-        const V qv = slow_row<V, OPk>(o, i, j, kk, idx, pcv, pnext[u], pl[scur][jl + 2][kl + 1],
-                                      pl[scur][jl + 1][kl + 2], pprev[u], pl[scur][jl][kl + 1], pl[scur][jl + 1][kl]);
-        pc[idx] = pcv;
-        q[idx] = qv;
-        a_pq += (Dacc)pcv * (Dacc)qv;
-      }
+      for (int u = 0; u < RPT; ++u)
+        if ((slow >> u) & 1u) pc[(int64_t)i * sx + (int64_t)(j0 + jl0 + u) * g.nz + kk] = pcur[u];
     }
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
       pprev[u] = pcur[u];
       pcur[u] = pnext[u];
       ncur[u] = nnext[u];
+      ra[u] = rb[u];
     }
+    hra = hrb;
   }
   double v[1] = {(double)a_pq};
   write_partials<1, ST>(v, P.S);
+}
+
+// Slow rows of the current solve as a list in cell order (flag, scan, place).
+__global__ void k_slow_flag(int64_t G, const uint8_t* __restrict__ nbm, uint32_t* __restrict__ flag) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < G) flag[idx] = ((nbm[idx] & (NB_UNKNOWN | NB_SLOW)) == (NB_UNKNOWN | NB_SLOW)) ? 1u : 0u;
+}
+__global__ void k_slow_list(int64_t G, const uint8_t* __restrict__ nbm, const uint32_t* __restrict__ pos,
+                            uint32_t* __restrict__ list) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < G && (nbm[idx] & (NB_UNKNOWN | NB_SLOW)) == (NB_UNKNOWN | NB_SLOW)) list[pos[idx]] = (uint32_t)idx;
+}
+
+// q = A p_k on the slow rows (after k_spmv has stored every p_k); partial p.q
+// into S2.  A fixed grid walks the list, so the sums are deterministic.
+template <class V, class Dacc, int OPk>
+__global__ void __launch_bounds__(PB) k_slow_q(OpArgs o, const uint32_t* __restrict__ list, int64_t n,
+                                               const V* __restrict__ pc, V* __restrict__ q, double* __restrict__ S2,
+                                               SolveScalars* sc) {
+  if (sc->done) return;
+  const Grid& g = o.g;
+  Dacc a_pq = 0;
+  for (int64_t t = (int64_t)blockIdx.x * PB + threadIdx.x; t < n; t += (int64_t)gridDim.x * PB) {
+    const int64_t idx = list[t];
+    int i, j, k;
+    ijk_of(g, idx, i, j, k);
+    V pn[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      const int ii = i + c_off[f][0], jj = j + c_off[f][1], kk = k + c_off[f][2];
+      pn[f] = valid(g, ii, jj, kk) ? pc[cid(g, ii, jj, kk)] : V(0);
+    }
+    const V pcv = pc[idx];
+    const V qv = slow_row<V, OPk>(o, i, j, k, idx, pcv, pn[0], pn[1], pn[2], pn[3], pn[4], pn[5]);
+    q[idx] = qv;
+    a_pq += (Dacc)pcv * (Dacc)qv;
+  }
+  double v[1] = {(double)a_pq};
+  write_partials<1>(v, S2);
 }
 
 // x += alpha p_k (folding x into x64 right after a reliable update);
@@ -507,11 +588,13 @@ __global__ void __launch_bounds__(ST) k_spmv(OpArgs o, Tiling T, int nbu, const 
 template <class V, class Dacc, int OPk>
 __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict__ x, V* __restrict__ r,
                                                const V* __restrict__ pc, const V* __restrict__ q,
-                                               double* __restrict__ x64, Parts P, int nbs, SolveScalars* sc, int k,
-                                               int fold) {
+                                               double* __restrict__ x64, Parts P, int nbs, int nbs2, SolveScalars* sc,
+                                               int k, int fold) {
   if (sc->done) return;
-  double t[1];
+  double t[1], t2[1];
   read_partials<1>(P.S, nbs, t);
+  read_partials<1>(P.S2, nbs2, t2);
+  t[0] += t2[0];
   if (t[0] == 0.0 || bad(t[0])) {  // spray.cu:1654-1656: zp == 0 ends the solve
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       sc->done = 1;
@@ -523,21 +606,39 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
   }
   const V al = (V)(sc->rz / t[0]);
   Dacc a_rz = 0, a_rr = 0;
-  CELLS_OF_BLOCK(o.g.G) {
-    const uint8_t nb = o.nbm[idx];
-    if (!(nb & NB_UNKNOWN)) continue;
-    const V pv = pc[idx];
-    if (fold) {
-      x64[idx] += (double)(x[idx] + al * pv);
-      x[idx] = V(0);
-    } else {
-      x[idx] += al * pv;
+  // UN cells per thread per pass, all loads issued before any use
+  constexpr int UN = 4;
+  const int64_t c0 = chunk_begin(L);
+  const int64_t c1 = (o.g.G < c0 + L.chunk) ? o.g.G : c0 + L.chunk;
+  for (int64_t base = c0 + threadIdx.x; base < c1; base += (int64_t)UN * PB) {
+    uint8_t nb[UN];
+    V pv[UN], xv[UN], rv[UN], qv[UN];
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+      const int64_t idx = base + (int64_t)u * PB;
+      const bool in = idx < c1;
+      nb[u] = in ? o.nbm[idx] : (uint8_t)0;
+      pv[u] = in ? pc[idx] : V(0);
+      xv[u] = in ? x[idx] : V(0);
+      rv[u] = in ? r[idx] : V(0);
+      qv[u] = in ? q[idx] : V(0);  // q is only defined on the unknowns; used only there
     }
-    const V rv = r[idx] - al * q[idx];
-    r[idx] = rv;
-    const V z = rv * inv_diag<V, OPk>(o, idx, nb);
-    a_rz += (Dacc)rv * (Dacc)z;
-    a_rr += (Dacc)rv * (Dacc)rv;
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+      if (!(nb[u] & NB_UNKNOWN)) continue;
+      const int64_t idx = base + (int64_t)u * PB;
+      if (fold) {
+        x64[idx] += (double)(xv[u] + al * pv[u]);
+        x[idx] = V(0);
+      } else {
+        x[idx] = xv[u] + al * pv[u];
+      }
+      const V rn = rv[u] - al * qv[u];
+      r[idx] = rn;
+      const V z = rn * inv_diag<V, OPk>(o, idx, nb[u]);
+      a_rz += (Dacc)rn * (Dacc)z;
+      a_rr += (Dacc)rn * (Dacc)rn;
+    }
   }
   double v[2] = {(double)a_rz, (double)a_rr};
   write_partials<2>(v, uslot(P, k));
@@ -679,6 +780,7 @@ OpArgs make_op(Ctx& c, int precond, int opk) {
   o.mark = c.mark;
   o.ls = c.lsmerge;
   o.nbm = c.nbm;
+  o.dslow = c.dslow;
   o.il = 1.0 / (double)c.cfg.rho_liquid;
   o.ia = 1.0 / (double)c.cfg.rho_air;
   const double h2 = (double)c.g.h * (double)c.g.h;
@@ -723,14 +825,24 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
     return MPF_INVALID_ARG;
   }
   V* r = (V*)c.vr;
-  V* pbuf[2] = {(V*)c.vp, (V*)c.vdinv};
+  V* pbuf[2] = {(V*)c.vp, (V*)c.vp2};
   V* q = (V*)c.vq;
-  Parts P{c.partials, c.partials + 2 * MAX_BLK, c.partials + 4 * MAX_BLK};
+  Parts P{c.partials, c.partials + 2 * MAX_BLK, c.partials + 4 * MAX_BLK, c.partials + 6 * MAX_BLK};
   const double eps_abs = 1e-2 * (double)G;
   const int R = c.cfg.refine_every > 0 ? c.cfg.refine_every : 20;
   int check = mixed ? R : c.cfg.check_every;
   if (check <= 0) check = 16;
-  launch(c, "pcg_nbm", dim3(nblocks(G, 256)), dim3(256), 0, k_nbm<OPk>, o, c.nbm);
+  launch(c, "pcg_nbm", dim3(nblocks(G, 256)), dim3(256), 0, k_nbm<OPk>, o, c.nbm, c.dslow);
+  // slow rows: list in cell order (positions from a scan into the cell-count scratch)
+  launch(c, "pcg_slow_flag", dim3(nblocks(G, 256)), dim3(256), 0, k_slow_flag, G, (const uint8_t*)c.nbm, c.ctmp_u32);
+  const int64_t nslow = run_scan_u32(c, c.ctmp_u32, c.count, G, true);
+  if (nslow < 0) return MPF_HIP_ERROR;
+  uint32_t* slow_list = c.ctmp_u32;
+  if (nslow > 0)
+    launch(c, "pcg_slow_list", dim3(nblocks(G, 256)), dim3(256), 0, k_slow_list, G, (const uint8_t*)c.nbm,
+           (const uint32_t*)c.count, slow_list);
+  int nsb = (int)((nslow + PB - 1) / PB);
+  if (nsb > MAX_BLK) nsb = MAX_BLK;
   launch(c, "pcg_init", dim3(L.nblk), dim3(PB), 0, k_init<V, Dacc, OPk>, o, L, b, x, r, pbuf[0], pbuf[1],
          mixed ? x64 : (double*)nullptr, P, c.scal);
   int k = 0;
@@ -740,8 +852,11 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
     for (; k < k_end && k <= cap; ++k) {
       launch(c, "pcg_spmv", dim3(T.nblk), dim3(ST), 0, k_spmv<V, Dacc, OPk>, o, T, L.nblk, (const V*)r,
              (const V*)pbuf[k & 1], pbuf[(k + 1) & 1], q, P, c.scal, k, (int)ref, eps_abs, tol, cap);
+      if (nsb > 0)
+        launch(c, "pcg_slow_q", dim3(nsb), dim3(PB), 0, k_slow_q<V, Dacc, OPk>, o, (const uint32_t*)slow_list, nslow,
+               (const V*)pbuf[(k + 1) & 1], q, P.S2, c.scal);
       launch(c, "pcg_update", dim3(L.nblk), dim3(PB), 0, k_update<V, Dacc, OPk>, o, L, x, r,
-             (const V*)pbuf[(k + 1) & 1], (const V*)q, mixed ? x64 : (double*)nullptr, P, T.nblk, c.scal, k,
+             (const V*)pbuf[(k + 1) & 1], (const V*)q, mixed ? x64 : (double*)nullptr, P, T.nblk, nsb, c.scal, k,
              (int)fold_next);
       fold_next = false;
       if (mixed && (k + 1) % R == 0) {
