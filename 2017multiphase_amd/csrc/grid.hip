@@ -259,7 +259,8 @@ __global__ void __launch_bounds__(P2G_THREADS)
     k_p2g_liquid3(Grid g, const f3* __restrict__ pos, const f3* __restrict__ vel, const float* __restrict__ mass,
                   const int8_t* __restrict__ flag, const uint32_t* __restrict__ cstart,
                   const uint32_t* __restrict__ cend, const uint8_t* __restrict__ cmask, float* __restrict__ ox,
-                  float* __restrict__ oy, float* __restrict__ oz, int ntx, int nty, int ntz) {
+                  float* __restrict__ oy, float* __restrict__ oz, int ntx, int nty, int ntz,
+                  const float* __restrict__ band, float big) {
   __shared__ float accW[3][PFACE];
   __shared__ float accU[3][PFACE];
   int t = blockIdx.x;
@@ -296,7 +297,15 @@ __global__ void __launch_bounds__(P2G_THREADS)
       e = cend[c];
     }
   }
-  const int any = __syncthreads_or(e > s);
+  // Band mode (solid velocity for the collision response): only faces next to
+  // a cell reached by the solid distance field (band < big) are mapped, the
+  // others are 0; a tile with no such cell in its box skips the phases.
+  bool near = true;
+  if (band) {
+    const bool mine = b < PBOX && valid(g, ci, cj, ck) && band[cid(g, ci, cj, ck)] < big;
+    near = __syncthreads_or(mine);
+  }
+  const int any = __syncthreads_or(e > s) && near;
   if (any) {
     const float scale = 1.0f / g.h;
     const float RE = 1.4f, RE2 = RE * RE;
@@ -374,7 +383,17 @@ __global__ void __launch_bounds__(P2G_THREADS)
       int fx = q / (ly * lz), fy = (q / lz) % ly, fz = q % lz;
       int li = (fx * PFY + fy) * PFZ + fz;
       float W = accW[a][li], U = accU[a][li];
-      outs[a][fid(g, a, x0 + fx, y0 + fy, z0 + fz)] = (W > 0) ? (U / W) : 0.0f;
+      float v = (W > 0) ? (U / W) : 0.0f;
+      if (band) {
+        const int gi = x0 + fx, gj = y0 + fy, gk = z0 + fz;
+        const int n3a = (a == 0) ? g.nx : (a == 1) ? g.ny : g.nz;
+        const int na = (a == 0) ? gi : (a == 1) ? gj : gk;
+        bool in_band = false;
+        if (na < n3a) in_band = in_band || band[cid(g, gi, gj, gk)] < big;
+        if (na > 0) in_band = in_band || band[cid(g, gi - (a == 0), gj - (a == 1), gk - (a == 2))] < big;
+        if (!in_band || !near) v = 0.0f;
+      }
+      outs[a][fid(g, a, x0 + fx, y0 + fy, z0 + fz)] = v;
     }
   }
 }
@@ -890,13 +909,13 @@ static void p2g_phase(Ctx& c, float* const* out) {
          (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask, out[2], fc[2]);
 }
 
-static void p2g_liquid(Ctx& c, float* const* out) {
+static void p2g_liquid(Ctx& c, float* const* out, const float* band = nullptr, float big = 0.0f) {
   const int cur = c.cur;
   const int ntx = (c.g.nx + PT_X - 1) / PT_X, nty = (c.g.ny + PT_Y - 1) / PT_Y, ntz = (c.g.nz + PT_Z - 1) / PT_Z;
-  launch(c, "p2g_liquid", dim3((unsigned)(ntx * nty * ntz)), dim3(P2G_THREADS), 0, k_p2g_liquid3, c.g,
-         (const f3*)c.pos[cur], (const f3*)c.vel[cur], (const float*)c.mass[cur], (const int8_t*)c.flag[cur],
-         (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask, out[0], out[1], out[2], ntx,
-         nty, ntz);
+  launch(c, band ? "p2g_solid_band" : "p2g_liquid", dim3((unsigned)(ntx * nty * ntz)), dim3(P2G_THREADS), 0,
+         k_p2g_liquid3, c.g, (const f3*)c.pos[cur], (const f3*)c.vel[cur], (const float*)c.mass[cur],
+         (const int8_t*)c.flag[cur], (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask,
+         out[0], out[1], out[2], ntx, nty, ntz, band, big);
 }
 
 void run_p2g(Ctx& c) {
@@ -914,7 +933,8 @@ void run_solid_phi_and_velocity(Ctx& c) {
   launch(c, "init_solid_phi", dim3(nblocks(G, 256)), dim3(256), 0, k_init_solid_phi, G, (const uint8_t*)c.cmask,
          c.phisolid, (float)(3 * c.g.nx));
   sweep_n(c, c.phisolid, -1, 3);
-  p2g_liquid(c, c.su);
+  // solid velocity only where the collision response can read it
+  p2g_liquid(c, c.su, c.phisolid, (float)(3 * c.g.nx));
 }
 
 static void sweep_phi(Ctx& c, float* phi, int8_t type) {

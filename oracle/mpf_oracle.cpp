@@ -824,6 +824,24 @@ template <class T> struct Orc : OrcBase {
     }
     for (int it = 0; it < 3; ++it) sweep(phisolid, mark, -1);
     p2g_phase(su, false);
+    // Faces whose cells the three sweeps did not reach are never read by the
+    // collision response (it needs phi_solid <= 0.5 at the particle); they
+    // are 0 (the library maps only the band).
+    const T big = (T)(3 * NX);
+    for (int a = 0; a < 3; ++a) {
+      const int w = NX + (a == 0), hh = NY + (a == 1), d = NZ + (a == 2);
+      const int64_t fc = (int64_t)w * hh * d;
+      for (int64_t f = 0; f < fc; ++f) {
+        int i, j, k;
+        ijk3(f, w, hh, d, i, j, k);
+        const int n3a = (a == 0) ? NX : (a == 1) ? NY : NZ;
+        const int na = (a == 0) ? i : (a == 1) ? j : k;
+        bool in_band = false;
+        if (na < n3a) in_band = in_band || phisolid[cid(i, j, k)] < big;
+        if (na > 0) in_band = in_band || phisolid[cid(i - (a == 0), j - (a == 1), k - (a == 2))] < big;
+        if (!in_band) su[a][f] = T(0);
+      }
+    }
   }
   // CollisionWithSolid_k, spray_k.cu:5276-5345
   void collide_and_move() {
