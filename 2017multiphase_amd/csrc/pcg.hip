@@ -348,8 +348,10 @@ __device__ __forceinline__ Raw<V> load_raw(const OpArgs& o, const V* __restrict_
   if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return w;
   const int64_t idx = cid(g, i, j, k);
   w.nb = o.nbm[idx];
-  w.r = r[idx];
-  w.pp = pp[idx];
+  if (w.nb & NB_UNKNOWN) {  // r and p_{k-1} are 0 elsewhere: not read
+    w.r = r[idx];
+    w.pp = pp[idx];
+  }
   return w;
 }
 // p_k from the loaded inputs; 0 off the unknowns (r and p_{k-1} are 0 there).
@@ -616,12 +618,16 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
 #pragma unroll
     for (int u = 0; u < UN; ++u) {
       const int64_t idx = base + (int64_t)u * PB;
-      const bool in = idx < c1;
-      nb[u] = in ? o.nbm[idx] : (uint8_t)0;
-      pv[u] = in ? pc[idx] : V(0);
-      xv[u] = in ? x[idx] : V(0);
-      rv[u] = in ? r[idx] : V(0);
-      qv[u] = in ? q[idx] : V(0);  // q is only defined on the unknowns; used only there
+      nb[u] = (idx < c1) ? o.nbm[idx] : (uint8_t)0;
+    }
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {  // vectors are read on the unknowns only
+      const int64_t idx = base + (int64_t)u * PB;
+      const bool on = (nb[u] & NB_UNKNOWN) != 0;
+      pv[u] = on ? pc[idx] : V(0);
+      xv[u] = on ? x[idx] : V(0);
+      rv[u] = on ? r[idx] : V(0);
+      qv[u] = on ? q[idx] : V(0);
     }
 #pragma unroll
     for (int u = 0; u < UN; ++u) {

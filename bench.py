@@ -25,33 +25,54 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per launch, as (bytes per cell, bytes per particle): each
-# array element the kernel must touch, read or written once (neighbour reads
-# of a stencil come from cache).  FP32 vectors of the MIXED form; FP64_VEC
+# Algorithmic bytes per launch, as (bytes per cell, bytes per unknown cell,
+# bytes per particle): each array element the kernel must touch, read or
+# written once (neighbour reads of a stencil come from cache).  Unknown cells
+# are the cells of the pressure system (FLUID or AIR); the solver's vectors
+# are read and written there only.  FP32 vectors of the MIXED form; FP64_VEC
 # holds the FP64 form's figures.
 KERNEL_BYTES = {
-    # p_k = r/diag + beta p_{k-1} fused with q = A p_k: r 4, p_{k-1} 4, mask 1; p_k 4, q 4 written
-    "pcg_spmv": (17, 0),
-    # x += alpha p, r -= alpha q, dots: x, r read+write 16, p 4, q 4, mask 1
-    "pcg_update": (25, 0),
-    # FP64 true residual: b 4, x64 8, x 4, mask 1; r 4 written
-    "pcg_resid64": (21, 0),
+    # mask 1 per cell; r 4 + p_{k-1} 4 read, p_k 4 + q 4 written per unknown
+    "pcg_spmv": (1, 16, 0),
+    # mask 1 per cell; x, r read+write 16, p 4, q 4 per unknown
+    "pcg_update": (1, 24, 0),
+    # mask 1 per cell; b 4, x64 8, x 4 read, r 4 written per unknown
+    "pcg_resid64": (1, 20, 0),
     # start/end 8, mask 1, ls fluid+air 8 written; pos 12 + flag 1 per particle
-    "genlevelset": (17, 13),
+    "genlevelset": (17, 0, 13),
     # one launch, 3 axes: start/end 8, mask 1, 3 faces 12 written; pos 12, vel 12, mass 4, flag 1
-    "p2g_liquid": (21, 29),
+    "p2g_liquid": (21, 0, 29),
     # one launch per axis: start/end 8, mask 1, face 4 written; particles as above
-    "p2g_air": (13, 29),
+    "p2g_air": (13, 0, 29),
     # start/end 8, mask 1, T 4 written; pos 12, T 4, flag 1
-    "heat_p2g": (13, 17),
+    "heat_p2g": (13, 0, 17),
     # 45-byte record read and written + perm 4
-    "reorder": (0, 94),
+    "reorder": (0, 0, 94),
     # level-set sweep: ls 4 read, mark 1, ls 4 written
-    "sweep": (9, 0),
+    "sweep": (9, 0, 0),
     # extrapolation pass per face: u 4, out 4, ls 2x4, mark 2
-    "sweep_u": (18, 0),
+    "sweep_u": (18, 0, 0),
 }
-FP64_VEC = {"pcg_spmv": (33, 0), "pcg_update": (49, 0)}
+FP64_VEC = {"pcg_spmv": (1, 32, 0), "pcg_update": (1, 48, 0)}
+
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def measured_traffic(kernel, form):
+    """HBM bytes per launch of `kernel` in `form` from the committed rocprofv3
+    counter summary (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of
+    known size), or None."""
+    path = os.path.join(PROFILES, "hbm_traffic_melt_freeze256.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 7 and cells[0] == "`%s`" % kernel and cells[1] == form:
+            try:
+                return float(cells[6])
+            except ValueError:
+                return None
+    return None
 
 
 def log(*a):
@@ -90,19 +111,22 @@ def run_form(pkg, scenes, name, n, precision, steps, warmup, timing_step=True):
     dt = time.perf_counter() - t0
     log("[bench]   %.3f s for %d steps, PCG iterations %s" % (dt, steps, [i["project"]["iterations"] for i in infos]))
     ktimes = None
+    unknowns = None
     if timing_step:
         sim.set_kernel_timing(True)
         sim.reset_kernel_times()
         sim.step(frame, allow_no_convergence=True)
         ktimes = sim.kernel_times()
         sim.set_kernel_timing(False)
+        mark = sim.get_field("MARK")
+        unknowns = int(((mark == pkg.TYPEFLUID) | (mark == pkg.TYPEAIR)).sum())
     res = {"seconds": dt, "steps": steps, "pcg_iters": iters, "infos": infos, "ktimes": ktimes,
-           "particles": sim.particle_count(), "G": cfg.nx * cfg.ny * cfg.nz}
+           "particles": sim.particle_count(), "G": cfg.nx * cfg.ny * cfg.nz, "unknowns": unknowns}
     sim.close()
     return res
 
 
-def roofline(res, precision):
+def roofline(res, precision, form):
     kt = res["ktimes"] or {}
     if not kt:
         return None, None
@@ -111,14 +135,15 @@ def roofline(res, precision):
     table = dict(KERNEL_BYTES)
     if precision == 1:
         table.update(FP64_VEC)
-    per_cell, per_par = table.get(name, (0, 0))
-    G, P = res["G"], res["particles"]
-    bytes_per_launch = per_cell * G + per_par * P
+    per_cell, per_unk, per_par = table.get(name, (0, 0, 0))
+    G, P, U = res["G"], res["particles"], res["unknowns"] or 0
+    bytes_per_launch = per_cell * G + per_unk * U + per_par * P
     avg_ms = ms / max(cnt, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if bytes_per_launch else None
     rf = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-          "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-          "avg_launch_us": avg_ms * 1e3, "launches_per_step": cnt, "bytes_per_launch": bytes_per_launch}
+          "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": measured_traffic(name, form),
+          "avg_launch_us": avg_ms * 1e3, "launches_per_step": cnt, "bytes_per_launch": bytes_per_launch,
+          "cells": G, "unknown_cells": U, "particles": P}
     top = sorted(((k, v[1]) for k, v in kt.items()), key=lambda kv: -kv[1])[:8]
     return rf, top
 
@@ -152,8 +177,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known HBM "
                          "traffic, for scaling profiler byte counters)")
@@ -179,7 +204,7 @@ def main():
     if not args.no_fp64:
         fp64 = run_form(pkg, scenes, args.scene, args.n, pkg.FP64, args.steps, args.warmup)
     sps = mixed["steps"] / mixed["seconds"]
-    rf, top = roofline(mixed, pkg.MIXED)
+    rf, top = roofline(mixed, pkg.MIXED, "mixed")
     out = {
         "metric": "FLIP steps/s at 256^3 (mixed vs FP64 PCG, same residual); % HBM peak",
         "value": sps,
@@ -207,7 +232,7 @@ def main():
         out["fp64_steps_per_s"] = f_sps
         out["mixed_over_fp64"] = sps / f_sps
         out["fp64_pcg_iters_per_step"] = fp64["pcg_iters"] / fp64["steps"]
-        rf64, _ = roofline(fp64, pkg.FP64)
+        rf64, _ = roofline(fp64, pkg.FP64, "fp64")
         out["fp64_roofline"] = rf64
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scenes, args.scene, args.n, args.cpu_n, args.cpu_steps)
