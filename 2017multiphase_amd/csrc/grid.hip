@@ -112,11 +112,22 @@ __global__ void k_sweep(Grid g, const float* __restrict__ in, float* __restrict_
   CELL_LOOP_HEAD
   float res = in[idx];
   if (!(typeflag >= 0 && mark[idx] == typeflag)) {
-    for (int di = -1; di <= 1; di += 2)
-      for (int dj = -1; dj <= 1; dj += 2)
-        for (int dk = -1; dk <= 1; dk += 2)
-          if (valid(g, i + di, j, k) && valid(g, i, j + dj, k) && valid(g, i, j, k + dk))
-            solvedistance(in[cid(g, i + di, j, k)], in[cid(g, i, j + dj, k)], in[cid(g, i, j, k + dk)], res);
+    // the six neighbours (x-, x+, y-, y+, z-, z+); an update needs one below
+    // the current value (every candidate is at least min(a, b, c)), so cells
+    // without one keep their value without evaluating the 8 triples
+    const bool vx0 = i > 0, vx1 = i < g.nx - 1, vy0 = j > 0, vy1 = j < g.ny - 1, vz0 = k > 0, vz1 = k < g.nz - 1;
+    const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz;
+    const float nx0 = vx0 ? in[idx - sx] : INFINITY, nx1 = vx1 ? in[idx + sx] : INFINITY;
+    const float ny0 = vy0 ? in[idx - sy] : INFINITY, ny1 = vy1 ? in[idx + sy] : INFINITY;
+    const float nz0 = vz0 ? in[idx - 1] : INFINITY, nz1 = vz1 ? in[idx + 1] : INFINITY;
+    const float m = fminf(fminf(fminf(nx0, nx1), fminf(ny0, ny1)), fminf(nz0, nz1));
+    if (m < res) {
+      for (int di = -1; di <= 1; di += 2)
+        for (int dj = -1; dj <= 1; dj += 2)
+          for (int dk = -1; dk <= 1; dk += 2)
+            if ((di < 0 ? vx0 : vx1) && (dj < 0 ? vy0 : vy1) && (dk < 0 ? vz0 : vz1))
+              solvedistance(di < 0 ? nx0 : nx1, dj < 0 ? ny0 : ny1, dk < 0 ? nz0 : nz1, res);
+    }
   }
   out[idx] = res;
 }
@@ -816,12 +827,19 @@ __global__ void k_mc_tris(Grid g, const uint8_t* __restrict__ cases, const uint3
 }  // namespace
 
 // ================= host stage drivers =================
+// n two-buffer passes, ping-ponging between phi and the scratch field (one
+// copy at the end when n is odd).
 static void sweep_n(Ctx& c, float* phi, int typeflag, int n) {
+  float* src = phi;
+  float* dst = c.ctmp;
   for (int it = 0; it < n; ++it) {
-    launch(c, "sweep", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_sweep, c.g, (const float*)phi, c.ctmp,
+    launch(c, "sweep", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_sweep, c.g, (const float*)src, dst,
            (const int8_t*)c.mark, typeflag);
-    hipMemcpyAsync(phi, c.ctmp, (size_t)c.g.G * 4, hipMemcpyDeviceToDevice, c.stream);
+    float* t = src;
+    src = dst;
+    dst = t;
   }
+  if (src != phi) hipMemcpyAsync(phi, src, (size_t)c.g.G * 4, hipMemcpyDeviceToDevice, c.stream);
 }
 
 static void redistance(Ctx& c, float* ls) {
@@ -944,17 +962,18 @@ static void sweep_phi(Ctx& c, float* phi, int8_t type) {
   sweep_n(c, phi, type, 3);
 }
 
+// Two passes: u -> scratch, scratch -> u.
 static void sweep_u(Ctx& c, float* const* u, const float* phi, int8_t sf) {
   const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
   for (int it = 0; it < 2; ++it) {
-    launch(c, "sweep_u", dim3(nblocks(fc[0], 256)), dim3(256), 0, k_sweep_u<0>, c.g, (const float*)u[0], c.ftmp[0],
+    float* const* src = it == 0 ? u : c.ftmp;
+    float* const* dst = it == 0 ? c.ftmp : u;
+    launch(c, "sweep_u", dim3(nblocks(fc[0], 256)), dim3(256), 0, k_sweep_u<0>, c.g, (const float*)src[0], dst[0],
            phi, (const int8_t*)c.mark, sf, fc[0]);
-    launch(c, "sweep_u", dim3(nblocks(fc[1], 256)), dim3(256), 0, k_sweep_u<1>, c.g, (const float*)u[1], c.ftmp[1],
+    launch(c, "sweep_u", dim3(nblocks(fc[1], 256)), dim3(256), 0, k_sweep_u<1>, c.g, (const float*)src[1], dst[1],
            phi, (const int8_t*)c.mark, sf, fc[1]);
-    launch(c, "sweep_u", dim3(nblocks(fc[2], 256)), dim3(256), 0, k_sweep_u<2>, c.g, (const float*)u[2], c.ftmp[2],
+    launch(c, "sweep_u", dim3(nblocks(fc[2], 256)), dim3(256), 0, k_sweep_u<2>, c.g, (const float*)src[2], dst[2],
            phi, (const int8_t*)c.mark, sf, fc[2]);
-    for (int a = 0; a < 3; ++a)
-      hipMemcpyAsync(u[a], c.ftmp[a], (size_t)fc[a] * 4, hipMemcpyDeviceToDevice, c.stream);
   }
 }
 

@@ -340,15 +340,18 @@ struct Raw {
   V r, pp;
   uint8_t nb;
 };
-template <class V>
-__device__ __forceinline__ Raw<V> load_raw(const OpArgs& o, const V* __restrict__ r, const V* __restrict__ pp, int i,
-                                           int j, int k) {
+__device__ __forceinline__ uint8_t load_nb(const OpArgs& o, int i, int j, int k) {
   const Grid& g = o.g;
-  Raw<V> w{V(0), V(0), 0};
-  if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return w;
-  const int64_t idx = cid(g, i, j, k);
-  w.nb = o.nbm[idx];
-  if (w.nb & NB_UNKNOWN) {  // r and p_{k-1} are 0 elsewhere: not read
+  if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return 0;
+  return o.nbm[cid(g, i, j, k)];
+}
+// r and p_{k-1} of an unknown cell (0 elsewhere, where they are 0 and not read).
+template <class V>
+__device__ __forceinline__ Raw<V> load_vals(const OpArgs& o, const V* __restrict__ r, const V* __restrict__ pp,
+                                            uint8_t nb, int i, int j, int k) {
+  Raw<V> w{V(0), V(0), nb};
+  if (nb & NB_UNKNOWN) {
+    const int64_t idx = cid(o.g, i, j, k);
     w.r = r[idx];
     w.pp = pp[idx];
   }
@@ -460,19 +463,24 @@ __global__ void __launch_bounds__(ST, 4) k_spmv(OpArgs o, Tiling T, int nbu, con
     hj = t2 % TJ;
     hk = (t2 < TJ) ? -1 : TK;
   }
-  // Software pipeline over planes: at step i the loads of plane i+2 are
-  // issued, p of plane i+1 is formed from the loads issued one step earlier
-  // and written to LDS, and q of plane i is computed.  The barrier only
-  // orders LDS (the plane i+2 loads stay in flight across it).
+  // Software pipeline over planes.  At step i: the vectors of plane i+2 are
+  // loaded (on the unknowns given by its mask, loaded one step earlier), the
+  // mask of plane i+3 is loaded, p of plane i+1 is formed from the vectors
+  // loaded one step earlier and written to LDS, and q of plane i is
+  // computed.  The barrier only orders LDS, so those loads stay in flight.
   V pprev[RPT], pcur[RPT], pnext[RPT];
-  uint8_t ncur[RPT], nnext[RPT];
-  Raw<V> ra[RPT], rb[RPT], hra, hrb;  // raw loads of planes i+1 (a) and i+2 (b)
+  uint8_t ncur[RPT], nnext[RPT], mn[RPT], mnn[RPT];
+  Raw<V> ra[RPT], rb[RPT], hra, hrb;  // vectors of planes i+1 (a) and i+2 (b)
+  uint8_t hmn, hmnn;
+  const int hi_ = ring ? 0 : -100000;  // non-ring threads load nothing for the ring
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
-    ra[u] = load_raw<V>(o, r, pp, i_lo - 1, j0 + jl0 + u, kk);
-    rb[u] = load_raw<V>(o, r, pp, i_lo, j0 + jl0 + u, kk);
+    const int j = j0 + jl0 + u;
+    const uint8_t n0 = load_nb(o, i_lo - 1, j, kk), n1 = load_nb(o, i_lo, j, kk);
+    ra[u] = load_vals<V>(o, r, pp, n0, i_lo - 1, j, kk);
+    rb[u] = load_vals<V>(o, r, pp, n1, i_lo, j, kk);
   }
-  hra = load_raw<V>(o, r, pp, ring ? i_lo : -1, j0 + hj, k0 + hk);
+  hra = load_vals<V>(o, r, pp, load_nb(o, i_lo + hi_, j0 + hj, k0 + hk), i_lo, j0 + hj, k0 + hk);
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
     pprev[u] = p_raw<V, OPk>(o, ra[u], be, i_lo - 1, j0 + jl0 + u, kk);
@@ -481,19 +489,29 @@ __global__ void __launch_bounds__(ST, 4) k_spmv(OpArgs o, Tiling T, int nbu, con
     pl[i_lo % 3][jl0 + u + 1][kl + 1] = pcur[u];
   }
   if (ring) pl[i_lo % 3][hj + 1][hk + 1] = p_raw<V, OPk>(o, hra, be, i_lo, j0 + hj, k0 + hk);
-  // plane i_lo + 1 in flight
+  // plane i_lo + 1 vectors and plane i_lo + 2 mask in flight
 #pragma unroll
-  for (int u = 0; u < RPT; ++u) ra[u] = load_raw<V>(o, r, pp, i_lo + 1, j0 + jl0 + u, kk);
-  hra = load_raw<V>(o, r, pp, ring ? i_lo + 1 : -1, j0 + hj, k0 + hk);
+  for (int u = 0; u < RPT; ++u) {
+    const int j = j0 + jl0 + u;
+    ra[u] = load_vals<V>(o, r, pp, load_nb(o, i_lo + 1, j, kk), i_lo + 1, j, kk);
+    mn[u] = load_nb(o, i_lo + 2, j, kk);
+  }
+  hra = load_vals<V>(o, r, pp, load_nb(o, i_lo + 1 + hi_, j0 + hj, k0 + hk), i_lo + 1, j0 + hj, k0 + hk);
+  hmn = load_nb(o, i_lo + 2 + hi_, j0 + hj, k0 + hk);
   const int64_t sx = (int64_t)g.ny * g.nz;
   const V qc = qcoef<V>(o);
   Dacc a_pq = 0;
   for (int i = i_lo; i < i_hi; ++i) {
     const int sn = (i + 1) % 3, scur = i % 3;
-    // issue plane i+2
+    // issue plane i+2 vectors, plane i+3 mask
 #pragma unroll
-    for (int u = 0; u < RPT; ++u) rb[u] = load_raw<V>(o, r, pp, i + 2, j0 + jl0 + u, kk);
-    hrb = load_raw<V>(o, r, pp, ring ? i + 2 : -1, j0 + hj, k0 + hk);
+    for (int u = 0; u < RPT; ++u) {
+      const int j = j0 + jl0 + u;
+      rb[u] = load_vals<V>(o, r, pp, mn[u], i + 2, j, kk);
+      mnn[u] = load_nb(o, i + 3, j, kk);
+    }
+    hrb = load_vals<V>(o, r, pp, hmn, i + 2, j0 + hj, k0 + hk);
+    hmnn = load_nb(o, i + 3 + hi_, j0 + hj, k0 + hk);
     // form plane i+1
 #pragma unroll
     for (int u = 0; u < RPT; ++u) {
@@ -539,8 +557,10 @@ __global__ void __launch_bounds__(ST, 4) k_spmv(OpArgs o, Tiling T, int nbu, con
       pcur[u] = pnext[u];
       ncur[u] = nnext[u];
       ra[u] = rb[u];
+      mn[u] = mnn[u];
     }
     hra = hrb;
+    hmn = hmnn;
   }
   double v[1] = {(double)a_pq};
   write_partials<1, ST>(v, P.S);
@@ -608,42 +628,64 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
   }
   const V al = (V)(sc->rz / t[0]);
   Dacc a_rz = 0, a_rr = 0;
-  // UN cells per thread per pass, all loads issued before any use
-  constexpr int UN = 4;
+  // Batches of UN cells per thread, software-pipelined: while batch b is
+  // updated, the vectors of batch b+1 (on its unknowns) and the mask of batch
+  // b+2 are in flight.  Off the unknowns nothing is read or written.
+  constexpr int UN = 2;
   const int64_t c0 = chunk_begin(L);
   const int64_t c1 = (o.g.G < c0 + L.chunk) ? o.g.G : c0 + L.chunk;
-  for (int64_t base = c0 + threadIdx.x; base < c1; base += (int64_t)UN * PB) {
-    uint8_t nb[UN];
-    V pv[UN], xv[UN], rv[UN], qv[UN];
+  const int64_t step = (int64_t)UN * PB;
+  uint8_t nbA[UN], nbB[UN], nbC[UN];
+  V pA[UN], xA[UN], rA[UN], qA[UN], pB[UN], xB[UN], rB[UN], qB[UN];
+  auto load_mask = [&](int64_t base, uint8_t (&m)[UN]) {
 #pragma unroll
     for (int u = 0; u < UN; ++u) {
       const int64_t idx = base + (int64_t)u * PB;
-      nb[u] = (idx < c1) ? o.nbm[idx] : (uint8_t)0;
+      m[u] = (idx < c1) ? o.nbm[idx] : (uint8_t)0;
     }
+  };
+  auto load_vec = [&](int64_t base, const uint8_t (&m)[UN], V (&pv)[UN], V (&xv)[UN], V (&rv)[UN], V (&qv)[UN]) {
 #pragma unroll
-    for (int u = 0; u < UN; ++u) {  // vectors are read on the unknowns only
+    for (int u = 0; u < UN; ++u) {
       const int64_t idx = base + (int64_t)u * PB;
-      const bool on = (nb[u] & NB_UNKNOWN) != 0;
+      const bool on = (m[u] & NB_UNKNOWN) != 0;
       pv[u] = on ? pc[idx] : V(0);
       xv[u] = on ? x[idx] : V(0);
       rv[u] = on ? r[idx] : V(0);
       qv[u] = on ? q[idx] : V(0);
     }
+  };
+  int64_t base = c0 + threadIdx.x;
+  load_mask(base, nbA);
+  load_vec(base, nbA, pA, xA, rA, qA);
+  load_mask(base + step, nbB);
+  for (; base < c1; base += step) {
+    load_vec(base + step, nbB, pB, xB, rB, qB);
+    load_mask(base + 2 * step, nbC);
 #pragma unroll
     for (int u = 0; u < UN; ++u) {
-      if (!(nb[u] & NB_UNKNOWN)) continue;
+      if (!(nbA[u] & NB_UNKNOWN)) continue;
       const int64_t idx = base + (int64_t)u * PB;
       if (fold) {
-        x64[idx] += (double)(xv[u] + al * pv[u]);
+        x64[idx] += (double)(xA[u] + al * pA[u]);
         x[idx] = V(0);
       } else {
-        x[idx] = xv[u] + al * pv[u];
+        x[idx] = xA[u] + al * pA[u];
       }
-      const V rn = rv[u] - al * qv[u];
+      const V rn = rA[u] - al * qA[u];
       r[idx] = rn;
-      const V z = rn * inv_diag<V, OPk>(o, idx, nb[u]);
+      const V z = rn * inv_diag<V, OPk>(o, idx, nbA[u]);
       a_rz += (Dacc)rn * (Dacc)z;
       a_rr += (Dacc)rn * (Dacc)rn;
+    }
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+      nbA[u] = nbB[u];
+      nbB[u] = nbC[u];
+      pA[u] = pB[u];
+      xA[u] = xB[u];
+      rA[u] = rB[u];
+      qA[u] = qB[u];
     }
   }
   double v[2] = {(double)a_rz, (double)a_rr};
