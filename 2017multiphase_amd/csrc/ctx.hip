@@ -119,7 +119,7 @@ void free_all(Ctx& c) {
   for (int a = 0; a < 3; ++a) {
     dfree(c.wu[a]); dfree(c.au[a]); dfree(c.wuo[a]); dfree(c.auo[a]); dfree(c.su[a]); dfree(c.ftmp[a]);
   }
-  dfree(c.scan_blk); dfree(c.ctmp_u32);
+  dfree(c.scan_blk); dfree(c.ctmp_u32); dfree(c.p2g_parts);
   dfree(c.mc_case); dfree(c.mc_edge); dfree(c.mc_tcount); dfree(c.mc_verts); dfree(c.mc_tris);
   dfree(c.seeds);
   adj_free(c);
@@ -170,6 +170,7 @@ bool alloc_all(Ctx& c) {
   int64_t maxscan = std::max<int64_t>(3 * G, std::max<int64_t>(V, cap));
   c.scan_blk_n = maxscan / 4096 + 4;
   ok = ok && dalloc(c, c.scan_blk, c.scan_blk_n) && dalloc(c, c.ctmp_u32, G);
+  ok = ok && dalloc(c, c.p2g_parts, (size_t)p2g_parts_floats(c.g));
   if (ok && c.cfg.mc_enabled) {
     ok = ok && dalloc(c, c.mc_case, V > 0 ? V : 1) && dalloc(c, c.mc_edge, 6 * G) &&
          dalloc(c, c.mc_tcount, 2 * (V > 0 ? V : 1)) && dalloc(c, c.mc_verts, 3 * c.cfg.mc_max_verts) &&

@@ -245,19 +245,7 @@ __global__ void k_p2g(Grid g, const f3* __restrict__ pos, const f3* __restrict__
   out[f] = (weight > 0) ? (usum / weight) : 0.0f;
 }
 
-// Liquid+solid P2G for all three face axes in one pass, cell-centric.
-// A block owns the faces of a 4x8x8 cell tile (each cell's -x, -y, -z faces,
-// plus the domain-end faces for the last tile) and one thread per cell of the
-// tile grown by one cell on every side, which is exactly the set of cells
-// whose window reaches an owned face.  Each thread keeps its cell's particles
-// in registers.  The 27 window offsets o = (di,dj,dk) are visited in the
-// gather's lexicographic order; in phase o every thread adds its particles,
-// in index order, to the owned face f = c - o of each axis whose window
-// contains o (normal offset in {-1,0}), accumulating in LDS.  Within a phase
-// the map c -> f is one-to-one, so no two threads touch the same face, and a
-// face receives exactly the reference's sequence of additions
-// (mapvelp2g_k_fluidSolid, spray_k.cu:2829-2871): bit-identical results
-// without global atomics.
+// Box tiling of the phased heat P2G: a 4x8x8 tile and its one-cell ring.
 constexpr int PT_X = 4, PT_Y = 8, PT_Z = 8;
 constexpr int PB_X = PT_X + 2, PB_Y = PT_Y + 2, PB_Z = PT_Z + 2;
 constexpr int PBOX = PB_X * PB_Y * PB_Z;  // 600
@@ -266,77 +254,85 @@ constexpr int PFX = PT_X + 1, PFY = PT_Y + 1, PFZ = PT_Z + 1;
 constexpr int PFACE = PFX * PFY * PFZ;
 constexpr int PCHUNK = 8;
 
-__global__ void __launch_bounds__(P2G_THREADS)
-    k_p2g_liquid3(Grid g, const f3* __restrict__ pos, const f3* __restrict__ vel, const float* __restrict__ mass,
-                  const int8_t* __restrict__ flag, const uint32_t* __restrict__ cstart,
-                  const uint32_t* __restrict__ cend, const uint8_t* __restrict__ cmask, float* __restrict__ ox,
-                  float* __restrict__ oy, float* __restrict__ oz, int ntx, int nty, int ntz,
-                  const float* __restrict__ band, float big) {
-  __shared__ float accW[3][PFACE];
-  __shared__ float accU[3][PFACE];
+// Liquid+solid P2G, tile scatter (the hot path).  A block owns a 4x8x8 cell
+// tile with one thread per cell and the cell's particles in registers.  Its
+// cells contribute to the faces of a per-axis region (the tile's faces plus
+// the ring its windows reach); the 27 window offsets are visited as phases in
+// the gather's lexicographic order, and in phase o every thread adds its
+// particles, in index order, to face c - o of each axis whose window holds o
+// -- one-to-one within a phase, so LDS accumulates without atomics.  Each
+// tile stores its region's (sum w, sum w v); a second kernel adds, per face,
+// the regions of the (at most 8) tiles that cover it in tile order and
+// divides.  Every particle-face pair is evaluated once and every thread holds
+// a tile cell (no halo threads); sums are deterministic, associated per tile
+// (a face inside one tile's region sums exactly like the gather).
+constexpr int ST_X = 4, ST_Y = 8, ST_Z = 8;
+constexpr int ST_CELLS = ST_X * ST_Y * ST_Z;  // 256 threads
+__host__ __device__ constexpr int st_t(int d) { return d == 0 ? ST_X : (d == 1 ? ST_Y : ST_Z); }
+// region of axis a: faces [t0_d - (d != a), t0_d + T_d] along d
+__host__ __device__ constexpr int st_r(int a, int d) { return st_t(d) + (d == a ? 1 : 2); }
+__host__ __device__ constexpr int st_rsz(int a) { return st_r(a, 0) * st_r(a, 1) * st_r(a, 2); }
+__host__ __device__ constexpr int st_roff(int a) { return a == 0 ? 0 : (a == 1 ? st_rsz(0) : st_rsz(0) + st_rsz(1)); }
+constexpr int ST_RTOT = st_rsz(0) + st_rsz(1) + st_rsz(2);  // 1580 faces
+constexpr int SCH = 8;  // particles kept in registers per cell
+
+__global__ void __launch_bounds__(ST_CELLS)
+    k_p2g_tile(Grid g, const f3* __restrict__ pos, const f3* __restrict__ vel, const float* __restrict__ mass,
+               const int8_t* __restrict__ flag, const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
+               const uint8_t* __restrict__ cmask, float* __restrict__ parts, int ntx, int nty, int ntz,
+               const float* __restrict__ band, float big) {
+  __shared__ float sW[ST_RTOT];
+  __shared__ float sU[ST_RTOT];
   int t = blockIdx.x;
   const int tz = t % ntz;
   t /= ntz;
   const int ty = t % nty;
   const int tx = t / nty;
-  const int x0 = tx * PT_X, y0 = ty * PT_Y, z0 = tz * PT_Z;
-  const int n3[3] = {g.nx, g.ny, g.nz};
-  const int o3[3] = {x0, y0, z0};
-  const int T3[3] = {PT_X, PT_Y, PT_Z};
-  // owned face index ranges [lo, hi) per axis a and direction d
-  int hi[3][3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a)
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      int e = o3[d] + T3[d];
-      hi[a][d] = (d == a) ? ((e >= n3[d]) ? n3[d] + 1 : e) : min(e, n3[d]);
-    }
-  for (int q = threadIdx.x; q < 3 * PFACE; q += P2G_THREADS) {
-    (&accW[0][0])[q] = 0.0f;
-    (&accU[0][0])[q] = 0.0f;
+  const int x0 = tx * ST_X, y0 = ty * ST_Y, z0 = tz * ST_Z;
+  for (int q = threadIdx.x; q < ST_RTOT; q += ST_CELLS) {
+    sW[q] = 0.0f;
+    sU[q] = 0.0f;
   }
-  // this thread's box cell
   const int b = threadIdx.x;
-  const int bx = b / (PB_Y * PB_Z), by = (b / PB_Z) % PB_Y, bz = b % PB_Z;
-  const int ci = x0 - 1 + bx, cj = y0 - 1 + by, ck = z0 - 1 + bz;
+  const int ci = x0 + b / (ST_Y * ST_Z), cj = y0 + (b / ST_Z) % ST_Y, ck = z0 + b % ST_Z;
   uint32_t s = 0, e = 0;
-  if (b < PBOX && valid(g, ci, cj, ck)) {
-    int64_t c = cid(g, ci, cj, ck);
+  if (valid(g, ci, cj, ck)) {
+    const int64_t c = cid(g, ci, cj, ck);
     if (cmask[c] & M_FLUIDSOLID) {
       s = cstart[c];
       e = cend[c];
     }
   }
-  // Band mode (solid velocity for the collision response): only faces next to
-  // a cell reached by the solid distance field (band < big) are mapped, the
-  // others are 0; a tile with no such cell in its box skips the phases.
+  // band mode: a tile whose cells and ring hold no cell with band < big maps nothing
   bool near = true;
   if (band) {
-    const bool mine = b < PBOX && valid(g, ci, cj, ck) && band[cid(g, ci, cj, ck)] < big;
+    bool mine = false;
+    for (int q = threadIdx.x; q < (ST_X + 2) * (ST_Y + 2) * (ST_Z + 2); q += ST_CELLS) {
+      const int bi = x0 - 1 + q / ((ST_Y + 2) * (ST_Z + 2)), bj = y0 - 1 + (q / (ST_Z + 2)) % (ST_Y + 2),
+                bk = z0 - 1 + q % (ST_Z + 2);
+      if (valid(g, bi, bj, bk) && band[cid(g, bi, bj, bk)] < big) mine = true;
+    }
     near = __syncthreads_or(mine);
   }
   const int any = __syncthreads_or(e > s) && near;
   if (any) {
     const float scale = 1.0f / g.h;
     const float RE = 1.4f, RE2 = RE * RE;
-    float X[PCHUNK], Y[PCHUNK], Z[PCHUNK], VX[PCHUNK], VY[PCHUNK], VZ[PCHUNK], M[PCHUNK];
-    bool ok[PCHUNK];
+    float X[SCH], Y[SCH], Z[SCH], VX[SCH], VY[SCH], VZ[SCH], M[SCH];
     const uint32_t n = e - s;
 #pragma unroll
-    for (int q = 0; q < PCHUNK; ++q) {
-      ok[q] = false;
+    for (int q = 0; q < SCH; ++q) {
       X[q] = Y[q] = Z[q] = VX[q] = VY[q] = VZ[q] = M[q] = 0.0f;
       if ((uint32_t)q < n) {
-        uint32_t p = s + q;
-        int8_t fl = flag[p];
-        ok[q] = (fl == FLUID || fl == SOLID);
-        f3 x = pos[p], v = vel[p];
+        const uint32_t p = s + q;
+        const int8_t fl = flag[p];
+        const f3 x = pos[p], v = vel[p];
         X[q] = (x.x - g.gx0) * scale;
         Y[q] = (x.y - g.gy0) * scale;
         Z[q] = (x.z - g.gz0) * scale;
         VX[q] = v.x; VY[q] = v.y; VZ[q] = v.z;
-        M[q] = mass[p];
+        // other phases weigh 0 (w = 0 * max(...) adds +-0 to the sums)
+        M[q] = (fl == FLUID || fl == SOLID) ? mass[p] : 0.0f;
       }
     }
     for (int ph = 0; ph < 27; ++ph) {
@@ -344,40 +340,36 @@ __global__ void __launch_bounds__(P2G_THREADS)
       if (n > 0) {
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          if (o[a] > 0) continue;
+          if (o[a] > 0) continue;  // normal window is {-1, 0}
           const int f[3] = {ci - o[0], cj - o[1], ck - o[2]};
-          bool own = true;
-#pragma unroll
-          for (int d = 0; d < 3; ++d) own = own && f[d] >= o3[d] && f[d] < hi[a][d];
-          if (!own) continue;
-          const int li = ((f[0] - x0) * PFY + (f[1] - y0)) * PFZ + (f[2] - z0);
-          float W = accW[a][li], U = accU[a][li];
+          const int r0 = x0 - (a != 0), r1 = y0 - (a != 1), r2 = z0 - (a != 2);
+          const int li = st_roff(a) + ((f[0] - r0) * st_r(a, 1) + (f[1] - r1)) * st_r(a, 2) + (f[2] - r2);
+          float W = sW[li], U = sU[li];
           const float gx = (float)f[0] + (a == 0 ? 0.0f : 0.5f);
           const float gy = (float)f[1] + (a == 1 ? 0.0f : 0.5f);
           const float gz = (float)f[2] + (a == 2 ? 0.0f : 0.5f);
 #pragma unroll
-          for (int q = 0; q < PCHUNK; ++q) {
-            if (!ok[q]) continue;
-            float dx = X[q] - gx, dy = Y[q] - gy, dz = Z[q] - gz;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            float w = M[q] * fmaxf(RE2 / fmaxf(r2, 0.0001f) - 1.0f, 0.0f);
+          for (int q = 0; q < SCH; ++q) {
+            const float dx = X[q] - gx, dy = Y[q] - gy, dz = Z[q] - gz;
+            const float r2s = dx * dx + dy * dy + dz * dz;
+            const float w = M[q] * fmaxf(RE2 / fmaxf(r2s, 0.0001f) - 1.0f, 0.0f);
             W += w;
             U += w * (a == 0 ? VX[q] : (a == 1 ? VY[q] : VZ[q]));
           }
-          // particles beyond the register chunk, still in index order
-          for (uint32_t p = s + PCHUNK; p < e; ++p) {
-            int8_t fl = flag[p];
+          for (uint32_t p = s + SCH; p < e; ++p) {  // beyond the register chunk, still in index order
+            const int8_t fl = flag[p];
             if (!(fl == FLUID || fl == SOLID)) continue;
-            f3 x = pos[p];
-            float dx = (x.x - g.gx0) * scale - gx, dy = (x.y - g.gy0) * scale - gy, dz = (x.z - g.gz0) * scale - gz;
-            float r2 = dx * dx + dy * dy + dz * dz;
-            float w = mass[p] * fmaxf(RE2 / fmaxf(r2, 0.0001f) - 1.0f, 0.0f);
+            const f3 x = pos[p];
+            const float dx = (x.x - g.gx0) * scale - gx, dy = (x.y - g.gy0) * scale - gy,
+                        dz = (x.z - g.gz0) * scale - gz;
+            const float r2s = dx * dx + dy * dy + dz * dz;
+            const float w = mass[p] * fmaxf(RE2 / fmaxf(r2s, 0.0001f) - 1.0f, 0.0f);
             W += w;
             const float* vp = &vel[p].x;
             U += w * vp[a];
           }
-          accW[a][li] = W;
-          accU[a][li] = U;
+          sW[li] = W;
+          sU[li] = U;
         }
       }
       __syncthreads();
@@ -385,28 +377,58 @@ __global__ void __launch_bounds__(P2G_THREADS)
   } else {
     __syncthreads();
   }
-  // write the owned faces
-  float* outs[3] = {ox, oy, oz};
-  for (int a = 0; a < 3; ++a) {
-    const int lx = hi[a][0] - x0, ly = hi[a][1] - y0, lz = hi[a][2] - z0;
-    const int cnt = lx * ly * lz;
-    for (int q = threadIdx.x; q < cnt; q += P2G_THREADS) {
-      int fx = q / (ly * lz), fy = (q / lz) % ly, fz = q % lz;
-      int li = (fx * PFY + fy) * PFZ + fz;
-      float W = accW[a][li], U = accU[a][li];
-      float v = (W > 0) ? (U / W) : 0.0f;
-      if (band) {
-        const int gi = x0 + fx, gj = y0 + fy, gk = z0 + fz;
-        const int n3a = (a == 0) ? g.nx : (a == 1) ? g.ny : g.nz;
-        const int na = (a == 0) ? gi : (a == 1) ? gj : gk;
-        bool in_band = false;
-        if (na < n3a) in_band = in_band || band[cid(g, gi, gj, gk)] < big;
-        if (na > 0) in_band = in_band || band[cid(g, gi - (a == 0), gj - (a == 1), gk - (a == 2))] < big;
-        if (!in_band || !near) v = 0.0f;
-      }
-      outs[a][fid(g, a, x0 + fx, y0 + fy, z0 + fz)] = v;
+  float* dst = parts + (int64_t)blockIdx.x * (2 * ST_RTOT);
+  for (int q = threadIdx.x; q < ST_RTOT; q += ST_CELLS) {
+    dst[q] = sW[q];
+    dst[ST_RTOT + q] = sU[q];
+  }
+}
+
+// u = (sum w v) / (sum w) per face from the tile regions covering it, added in
+// tile order.  Band mode: faces without a cell of band < big beside them are 0.
+template <int A>
+__global__ void k_p2g_collect(Grid g, const float* __restrict__ parts, int ntx, int nty, int ntz,
+                              float* __restrict__ out, int64_t nf, const float* __restrict__ band, float big) {
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  int i, j, k;
+  ijk_face<A>(g, f, i, j, k);
+  if (band) {
+    const int n3a = (A == 0) ? g.nx : (A == 1) ? g.ny : g.nz;
+    const int na = (A == 0) ? i : (A == 1) ? j : k;
+    bool in_band = false;
+    if (na < n3a) in_band = in_band || band[cid(g, i, j, k)] < big;
+    if (na > 0) in_band = in_band || band[cid(g, i - (A == 0), j - (A == 1), k - (A == 2))] < big;
+    if (!in_band) {
+      out[f] = 0.0f;
+      return;
     }
   }
+  const int fv[3] = {i, j, k};
+  const int nt[3] = {ntx, nty, ntz};
+  int qlo[3], qhi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    // tiles q with q*T - (d != A) <= f <= q*T + T
+    const int T = st_t(d);
+    int lo = (fv[d] - T + T - 1) / T;  // ceil((f - T) / T) for f >= 0
+    if (fv[d] - T < 0) lo = 0;
+    int hi = (fv[d] + (d != A ? 1 : 0)) / T;
+    if (hi > nt[d] - 1) hi = nt[d] - 1;
+    qlo[d] = lo;
+    qhi[d] = hi;
+  }
+  float W = 0.0f, U = 0.0f;
+  for (int qx = qlo[0]; qx <= qhi[0]; ++qx)
+    for (int qy = qlo[1]; qy <= qhi[1]; ++qy)
+      for (int qz = qlo[2]; qz <= qhi[2]; ++qz) {
+        const int r0 = qx * ST_X - (A != 0), r1 = qy * ST_Y - (A != 1), r2 = qz * ST_Z - (A != 2);
+        const int li = st_roff(A) + ((fv[0] - r0) * st_r(A, 1) + (fv[1] - r1)) * st_r(A, 2) + (fv[2] - r2);
+        const float* src = parts + ((int64_t)(qx * nty + qy) * ntz + qz) * (2 * ST_RTOT);
+        W += src[li];
+        U += src[ST_RTOT + li];
+      }
+  out[f] = (W > 0) ? (U / W) : 0.0f;
 }
 
 // ---------------- extrapolation + wall conditions (a10) ----------------
@@ -929,11 +951,24 @@ static void p2g_phase(Ctx& c, float* const* out) {
 
 static void p2g_liquid(Ctx& c, float* const* out, const float* band = nullptr, float big = 0.0f) {
   const int cur = c.cur;
-  const int ntx = (c.g.nx + PT_X - 1) / PT_X, nty = (c.g.ny + PT_Y - 1) / PT_Y, ntz = (c.g.nz + PT_Z - 1) / PT_Z;
-  launch(c, band ? "p2g_solid_band" : "p2g_liquid", dim3((unsigned)(ntx * nty * ntz)), dim3(P2G_THREADS), 0,
-         k_p2g_liquid3, c.g, (const f3*)c.pos[cur], (const f3*)c.vel[cur], (const float*)c.mass[cur],
+  const int ntx = (c.g.nx + ST_X - 1) / ST_X, nty = (c.g.ny + ST_Y - 1) / ST_Y, ntz = (c.g.nz + ST_Z - 1) / ST_Z;
+  launch(c, band ? "p2g_solid_band" : "p2g_liquid", dim3((unsigned)(ntx * nty * ntz)), dim3(ST_CELLS), 0,
+         k_p2g_tile, c.g, (const f3*)c.pos[cur], (const f3*)c.vel[cur], (const float*)c.mass[cur],
          (const int8_t*)c.flag[cur], (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask,
-         out[0], out[1], out[2], ntx, nty, ntz, band, big);
+         c.p2g_parts, ntx, nty, ntz, band, big);
+  const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
+  const char* nm = band ? "p2g_solid_collect" : "p2g_collect";
+  launch(c, nm, dim3(nblocks(fc[0], 256)), dim3(256), 0, k_p2g_collect<0>, c.g, (const float*)c.p2g_parts, ntx, nty,
+         ntz, out[0], fc[0], band, big);
+  launch(c, nm, dim3(nblocks(fc[1], 256)), dim3(256), 0, k_p2g_collect<1>, c.g, (const float*)c.p2g_parts, ntx, nty,
+         ntz, out[1], fc[1], band, big);
+  launch(c, nm, dim3(nblocks(fc[2], 256)), dim3(256), 0, k_p2g_collect<2>, c.g, (const float*)c.p2g_parts, ntx, nty,
+         ntz, out[2], fc[2], band, big);
+}
+
+int64_t p2g_parts_floats(const Grid& g) {
+  const int64_t nt = (int64_t)((g.nx + ST_X - 1) / ST_X) * ((g.ny + ST_Y - 1) / ST_Y) * ((g.nz + ST_Z - 1) / ST_Z);
+  return nt * 2 * ST_RTOT;
 }
 
 void run_p2g(Ctx& c) {

@@ -181,6 +181,7 @@ struct Ctx {
   float* auo[3] = {};
   float* su[3] = {};
   float* ftmp[3] = {};
+  float* p2g_parts = nullptr;  // per-tile face sums of the liquid P2G
 
   // scan scratch
   uint32_t* scan_blk = nullptr;
@@ -254,6 +255,7 @@ void run_heat_grid_post(Ctx& c);
 void run_heat_jacobi(Ctx& c, int iters);
 int run_marching_cubes(Ctx& c, mpf_mc_info* info);
 void run_solid_phi_and_velocity(Ctx& c);
+int64_t p2g_parts_floats(const Grid& g);
 // pcg.hip
 int run_pressure_solve(Ctx& c, mpf_solve_info* info);
 int run_heat_cg(Ctx& c, mpf_solve_info* info);

@@ -628,64 +628,40 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
   }
   const V al = (V)(sc->rz / t[0]);
   Dacc a_rz = 0, a_rr = 0;
-  // Batches of UN cells per thread, software-pipelined: while batch b is
-  // updated, the vectors of batch b+1 (on its unknowns) and the mask of batch
-  // b+2 are in flight.  Off the unknowns nothing is read or written.
-  constexpr int UN = 2;
+  // UN cells per thread per pass, all loads issued before any use (the
+  // vectors are read unconditionally: off the unknowns they are 0 or unused,
+  // and a load that waits for the mask first costs more than it saves)
+  constexpr int UN = 4;
   const int64_t c0 = chunk_begin(L);
   const int64_t c1 = (o.g.G < c0 + L.chunk) ? o.g.G : c0 + L.chunk;
-  const int64_t step = (int64_t)UN * PB;
-  uint8_t nbA[UN], nbB[UN], nbC[UN];
-  V pA[UN], xA[UN], rA[UN], qA[UN], pB[UN], xB[UN], rB[UN], qB[UN];
-  auto load_mask = [&](int64_t base, uint8_t (&m)[UN]) {
+  for (int64_t base = c0 + threadIdx.x; base < c1; base += (int64_t)UN * PB) {
+    uint8_t nb[UN];
+    V pv[UN], xv[UN], rv[UN], qv[UN];
 #pragma unroll
     for (int u = 0; u < UN; ++u) {
       const int64_t idx = base + (int64_t)u * PB;
-      m[u] = (idx < c1) ? o.nbm[idx] : (uint8_t)0;
+      const bool in = idx < c1;
+      nb[u] = in ? o.nbm[idx] : (uint8_t)0;
+      pv[u] = in ? pc[idx] : V(0);
+      xv[u] = in ? x[idx] : V(0);
+      rv[u] = in ? r[idx] : V(0);
+      qv[u] = in ? q[idx] : V(0);  // q is only defined on the unknowns; used only there
     }
-  };
-  auto load_vec = [&](int64_t base, const uint8_t (&m)[UN], V (&pv)[UN], V (&xv)[UN], V (&rv)[UN], V (&qv)[UN]) {
 #pragma unroll
     for (int u = 0; u < UN; ++u) {
-      const int64_t idx = base + (int64_t)u * PB;
-      const bool on = (m[u] & NB_UNKNOWN) != 0;
-      pv[u] = on ? pc[idx] : V(0);
-      xv[u] = on ? x[idx] : V(0);
-      rv[u] = on ? r[idx] : V(0);
-      qv[u] = on ? q[idx] : V(0);
-    }
-  };
-  int64_t base = c0 + threadIdx.x;
-  load_mask(base, nbA);
-  load_vec(base, nbA, pA, xA, rA, qA);
-  load_mask(base + step, nbB);
-  for (; base < c1; base += step) {
-    load_vec(base + step, nbB, pB, xB, rB, qB);
-    load_mask(base + 2 * step, nbC);
-#pragma unroll
-    for (int u = 0; u < UN; ++u) {
-      if (!(nbA[u] & NB_UNKNOWN)) continue;
+      if (!(nb[u] & NB_UNKNOWN)) continue;
       const int64_t idx = base + (int64_t)u * PB;
       if (fold) {
-        x64[idx] += (double)(xA[u] + al * pA[u]);
+        x64[idx] += (double)(xv[u] + al * pv[u]);
         x[idx] = V(0);
       } else {
-        x[idx] = xA[u] + al * pA[u];
+        x[idx] = xv[u] + al * pv[u];
       }
-      const V rn = rA[u] - al * qA[u];
+      const V rn = rv[u] - al * qv[u];
       r[idx] = rn;
-      const V z = rn * inv_diag<V, OPk>(o, idx, nbA[u]);
+      const V z = rn * inv_diag<V, OPk>(o, idx, nb[u]);
       a_rz += (Dacc)rn * (Dacc)z;
       a_rr += (Dacc)rn * (Dacc)rn;
-    }
-#pragma unroll
-    for (int u = 0; u < UN; ++u) {
-      nbA[u] = nbB[u];
-      nbB[u] = nbC[u];
-      pA[u] = pB[u];
-      xA[u] = xB[u];
-      rA[u] = rB[u];
-      qA[u] = qB[u];
     }
   }
   double v[2] = {(double)a_rz, (double)a_rr};
