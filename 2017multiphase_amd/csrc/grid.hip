@@ -245,15 +245,6 @@ __global__ void k_p2g(Grid g, const f3* __restrict__ pos, const f3* __restrict__
   out[f] = (weight > 0) ? (usum / weight) : 0.0f;
 }
 
-// Box tiling of the phased heat P2G: a 4x8x8 tile and its one-cell ring.
-constexpr int PT_X = 4, PT_Y = 8, PT_Z = 8;
-constexpr int PB_X = PT_X + 2, PB_Y = PT_Y + 2, PB_Z = PT_Z + 2;
-constexpr int PBOX = PB_X * PB_Y * PB_Z;  // 600
-constexpr int P2G_THREADS = 640;
-constexpr int PFX = PT_X + 1, PFY = PT_Y + 1, PFZ = PT_Z + 1;
-constexpr int PFACE = PFX * PFY * PFZ;
-constexpr int PCHUNK = 8;
-
 // Liquid+solid P2G, tile scatter (the hot path).  A block owns a 4x8x8 cell
 // tile with one thread per cell and the cell's particles in registers.  Its
 // cells contribute to the faces of a per-axis region (the tile's faces plus
@@ -583,61 +574,35 @@ __global__ void k_subgrad(Grid g, float* __restrict__ u, const R* __restrict__ p
 }
 
 // ---------------- heat grid stages (a16) ----------------
-// mapHeatp2g_hash + sumHeat, spray_k.cu:4619-4653
-__global__ void k_heat_p2g(Grid g, const f3* __restrict__ pos, const float* __restrict__ temp,
-                           const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
-                           const uint8_t* __restrict__ cmask, float deft, float* __restrict__ tp) {
-  CELL_LOOP_HEAD
-  const float scale = 1.0f / g.h;
-  f3 gp = mk3((float)i + 0.5f, (float)j + 0.5f, (float)k + 0.5f);
-  const float RE = 1.4f, RE2 = RE * RE;
-  float weight = 0, hs = 0;
-  for (int di = -1; di <= 1; ++di)
-    for (int dj = -1; dj <= 1; ++dj)
-      for (int dk = -1; dk <= 1; ++dk) {
-        if (!valid(g, i + di, j + dj, k + dk)) continue;
-        int64_t c = cid(g, i + di, j + dj, k + dk);
-        if (!cmask[c]) continue;
-        for (uint32_t p = cstart[c]; p < cend[c]; ++p) {
-          f3 x = pos[p];
-          f3 dv = mk3((x.x - g.gx0) * scale - gp.x, (x.y - g.gy0) * scale - gp.y, (x.z - g.gz0) * scale - gp.z);
-          float w = fmaxf(RE2 / fmaxf(dot3(dv, dv), 0.0001f) - 1.0f, 0.0f);
-          weight += w;
-          hs += w * temp[p];
-        }
-      }
-  tp[idx] = (weight > 0) ? (hs / weight) : deft;
-}
+// Heat P2G (mapHeatp2g_hash + sumHeat, spray_k.cu:4619-4653) as a tile
+// scatter like the liquid P2G: one thread per cell of a 4x8x8 tile, 27
+// phases in the gather's order adding each particle to the cell c - o of the
+// tile's region (the tile grown by one cell), per-tile sums combined per cell
+// in tile order by k_heat_collect.
+constexpr int HR_X = ST_X + 2, HR_Y = ST_Y + 2, HR_Z = ST_Z + 2;
+constexpr int HR = HR_X * HR_Y * HR_Z;  // 600 cells
 
-// Heat P2G in the same cell-centric phased form as k_p2g_liquid3: a block
-// owns a 4x8x8 cell tile, one thread per cell of the tile grown by one cell;
-// the 27 offsets are visited in the gather's order and each target cell
-// receives its additions in the reference's sequence (mapHeatp2g_hash +
-// sumHeat, spray_k.cu:4619-4653), so results are bit-identical.
-__global__ void __launch_bounds__(P2G_THREADS)
-    k_heat_p2g3(Grid g, const f3* __restrict__ pos, const float* __restrict__ temp,
+__global__ void __launch_bounds__(ST_CELLS)
+    k_heat_tile(Grid g, const f3* __restrict__ pos, const float* __restrict__ temp,
                 const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
-                const uint8_t* __restrict__ cmask, float deft, float* __restrict__ tp, int ntx, int nty, int ntz) {
-  constexpr int TN = PT_X * PT_Y * PT_Z;
-  __shared__ float accW[TN];
-  __shared__ float accH[TN];
+                const uint8_t* __restrict__ cmask, float* __restrict__ parts, int ntx, int nty, int ntz) {
+  __shared__ float sW[HR];
+  __shared__ float sH[HR];
   int t = blockIdx.x;
   const int tz = t % ntz;
   t /= ntz;
   const int ty = t % nty;
   const int tx = t / nty;
-  const int x0 = tx * PT_X, y0 = ty * PT_Y, z0 = tz * PT_Z;
-  const int hx = min(x0 + PT_X, g.nx), hy = min(y0 + PT_Y, g.ny), hz = min(z0 + PT_Z, g.nz);
-  for (int q = threadIdx.x; q < TN; q += P2G_THREADS) {
-    accW[q] = 0.0f;
-    accH[q] = 0.0f;
+  const int x0 = tx * ST_X, y0 = ty * ST_Y, z0 = tz * ST_Z;
+  for (int q = threadIdx.x; q < HR; q += ST_CELLS) {
+    sW[q] = 0.0f;
+    sH[q] = 0.0f;
   }
   const int b = threadIdx.x;
-  const int bx = b / (PB_Y * PB_Z), by = (b / PB_Z) % PB_Y, bz = b % PB_Z;
-  const int ci = x0 - 1 + bx, cj = y0 - 1 + by, ck = z0 - 1 + bz;
+  const int ci = x0 + b / (ST_Y * ST_Z), cj = y0 + (b / ST_Z) % ST_Y, ck = z0 + b % ST_Z;
   uint32_t s = 0, e = 0;
-  if (b < PBOX && valid(g, ci, cj, ck)) {
-    int64_t c = cid(g, ci, cj, ck);
+  if (valid(g, ci, cj, ck)) {
+    const int64_t c = cid(g, ci, cj, ck);
     if (cmask[c]) {
       s = cstart[c];
       e = cend[c];
@@ -647,55 +612,81 @@ __global__ void __launch_bounds__(P2G_THREADS)
   if (any) {
     const float scale = 1.0f / g.h;
     const float RE = 1.4f, RE2 = RE * RE;
-    float X[PCHUNK], Y[PCHUNK], Z[PCHUNK], TT[PCHUNK];
+    float X[SCH], Y[SCH], Z[SCH], TT[SCH], ON[SCH];
     const uint32_t n = e - s;
 #pragma unroll
-    for (int q = 0; q < PCHUNK; ++q) {
-      X[q] = Y[q] = Z[q] = TT[q] = 0.0f;
+    for (int q = 0; q < SCH; ++q) {
+      X[q] = Y[q] = Z[q] = TT[q] = ON[q] = 0.0f;
       if ((uint32_t)q < n) {
-        f3 x = pos[s + q];
+        const f3 x = pos[s + q];
         X[q] = (x.x - g.gx0) * scale;
         Y[q] = (x.y - g.gy0) * scale;
         Z[q] = (x.z - g.gz0) * scale;
         TT[q] = temp[s + q];
+        ON[q] = 1.0f;  // empty slots weigh 0 (adds +-0)
       }
     }
     for (int ph = 0; ph < 27; ++ph) {
       const int f0 = ci - (ph / 9 - 1), f1 = cj - ((ph / 3) % 3 - 1), f2 = ck - (ph % 3 - 1);
-      if (n > 0 && f0 >= x0 && f0 < hx && f1 >= y0 && f1 < hy && f2 >= z0 && f2 < hz) {
-        const int li = ((f0 - x0) * PT_Y + (f1 - y0)) * PT_Z + (f2 - z0);
-        float W = accW[li], Hs = accH[li];
+      if (n > 0) {
+        const int li = ((f0 - (x0 - 1)) * HR_Y + (f1 - (y0 - 1))) * HR_Z + (f2 - (z0 - 1));
+        float W = sW[li], Hs = sH[li];
         const float gx = (float)f0 + 0.5f, gy = (float)f1 + 0.5f, gz = (float)f2 + 0.5f;
 #pragma unroll
-        for (int q = 0; q < PCHUNK; ++q) {
-          if ((uint32_t)q >= n) break;
-          float dx = X[q] - gx, dy = Y[q] - gy, dz = Z[q] - gz;
-          float w = fmaxf(RE2 / fmaxf(dx * dx + dy * dy + dz * dz, 0.0001f) - 1.0f, 0.0f);
+        for (int q = 0; q < SCH; ++q) {
+          const float dx = X[q] - gx, dy = Y[q] - gy, dz = Z[q] - gz;
+          const float w = ON[q] * fmaxf(RE2 / fmaxf(dx * dx + dy * dy + dz * dz, 0.0001f) - 1.0f, 0.0f);
           W += w;
           Hs += w * TT[q];
         }
-        for (uint32_t p = s + PCHUNK; p < e; ++p) {
-          f3 x = pos[p];
-          float dx = (x.x - g.gx0) * scale - gx, dy = (x.y - g.gy0) * scale - gy, dz = (x.z - g.gz0) * scale - gz;
-          float w = fmaxf(RE2 / fmaxf(dx * dx + dy * dy + dz * dz, 0.0001f) - 1.0f, 0.0f);
+        for (uint32_t p = s + SCH; p < e; ++p) {
+          const f3 x = pos[p];
+          const float dx = (x.x - g.gx0) * scale - gx, dy = (x.y - g.gy0) * scale - gy,
+                      dz = (x.z - g.gz0) * scale - gz;
+          const float w = fmaxf(RE2 / fmaxf(dx * dx + dy * dy + dz * dz, 0.0001f) - 1.0f, 0.0f);
           W += w;
           Hs += w * temp[p];
         }
-        accW[li] = W;
-        accH[li] = Hs;
+        sW[li] = W;
+        sH[li] = Hs;
       }
       __syncthreads();
     }
   } else {
     __syncthreads();
   }
-  const int lx = hx - x0, ly = hy - y0, lz = hz - z0;
-  for (int q = threadIdx.x; q < lx * ly * lz; q += P2G_THREADS) {
-    int fx = q / (ly * lz), fy = (q / lz) % ly, fz = q % lz;
-    int li = (fx * PT_Y + fy) * PT_Z + fz;
-    float W = accW[li];
-    tp[cid(g, x0 + fx, y0 + fy, z0 + fz)] = (W > 0) ? (accH[li] / W) : deft;
+  float* dst = parts + (int64_t)blockIdx.x * (2 * HR);
+  for (int q = threadIdx.x; q < HR; q += ST_CELLS) {
+    dst[q] = sW[q];
+    dst[HR + q] = sH[q];
   }
+}
+
+// T of a cell from the regions of the (at most 8) tiles covering it, in tile
+// order; cells without particle weight keep the default temperature.
+__global__ void k_heat_collect(Grid g, const float* __restrict__ parts, int ntx, int nty, int ntz, float deft,
+                               float* __restrict__ tp) {
+  CELL_LOOP_HEAD
+  const int cv[3] = {i, j, k};
+  const int nt[3] = {ntx, nty, ntz};
+  int qlo[3], qhi[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int T = st_t(d);  // region [qT - 1, qT + T]
+    qlo[d] = (cv[d] - T < 0) ? 0 : (cv[d] - 1) / T;
+    const int hi = (cv[d] + 1) / T;
+    qhi[d] = hi > nt[d] - 1 ? nt[d] - 1 : hi;
+  }
+  float W = 0.0f, Hs = 0.0f;
+  for (int qx = qlo[0]; qx <= qhi[0]; ++qx)
+    for (int qy = qlo[1]; qy <= qhi[1]; ++qy)
+      for (int qz = qlo[2]; qz <= qhi[2]; ++qz) {
+        const int li = ((i - (qx * ST_X - 1)) * HR_Y + (j - (qy * ST_Y - 1))) * HR_Z + (k - (qz * ST_Z - 1));
+        const float* src = parts + ((int64_t)(qx * nty + qy) * ntz + qz) * (2 * HR);
+        W += src[li];
+        Hs += src[HR + li];
+      }
+  tp[idx] = (W > 0) ? (Hs / W) : deft;
 }
 
 // addHeatAtBottom spray_k.cu:4492-4504, then save T and form b (compb_heat
@@ -1069,10 +1060,13 @@ void run_subgrad(Ctx& c) {
 void run_heat_grid_pre(Ctx& c) {
   const mpf_config& k = c.cfg;
   const int64_t G = c.g.G;
-  const int ntx = (c.g.nx + PT_X - 1) / PT_X, nty = (c.g.ny + PT_Y - 1) / PT_Y, ntz = (c.g.nz + PT_Z - 1) / PT_Z;
-  launch(c, "heat_p2g", dim3((unsigned)(ntx * nty * ntz)), dim3(P2G_THREADS), 0, k_heat_p2g3, c.g,
+  const int ntx = (c.g.nx + ST_X - 1) / ST_X, nty = (c.g.ny + ST_Y - 1) / ST_Y, ntz = (c.g.nz + ST_Z - 1) / ST_Z;
+  static_assert(HR <= ST_RTOT, "heat regions fit the P2G tile buffer");
+  launch(c, "heat_p2g", dim3((unsigned)(ntx * nty * ntz)), dim3(ST_CELLS), 0, k_heat_tile, c.g,
          (const f3*)c.pos[c.cur], (const float*)c.temp[c.cur], (const uint32_t*)c.cstart, (const uint32_t*)c.cend,
-         (const uint8_t*)c.cmask, k.default_temperature, c.tp, ntx, nty, ntz);
+         (const uint8_t*)c.cmask, c.p2g_parts, ntx, nty, ntz);
+  launch(c, "heat_collect", dim3(nblocks(G, 256)), dim3(256), 0, k_heat_collect, c.g, (const float*)c.p2g_parts, ntx,
+         nty, ntz, k.default_temperature, c.tp);
   launch(c, "heat_pre", dim3(nblocks(G, 256)), dim3(256), 0, k_heat_pre, c.g, c.tp, c.tsave, c.ctmp,
          (const int8_t*)c.mark, k.heat_add_bottom, k.heat_increase_bottom, k.heat_bottom_clamp, k.heat_alpha[0],
          k.heat_alpha[1], k.heat_alpha[2], k.heat_alpha[3], k.heat_alpha[4], k.heat_alpha[5], k.dt);
