@@ -61,8 +61,28 @@ struct OpArgs {
 };
 
 template <class A> __device__ __forceinline__ A dinv_tab(const OpArgs& o, int n);
-template <> __device__ __forceinline__ float dinv_tab<float>(const OpArgs& o, int n) { return o.dinv_f[n]; }
-template <> __device__ __forceinline__ double dinv_tab<double>(const OpArgs& o, int n) { return o.dinv_d[n]; }
+// (a chain of selects over register copies of the table: an indexed read of
+// the kernel arguments would be a memory load, and waiting for it waits for
+// every load issued before it; readfirstlane keeps the compiler from turning
+// the selects back into a select of addresses)
+__device__ __forceinline__ float uniform_reg(float x) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, x)));
+}
+__device__ __forceinline__ double uniform_reg(double x) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <class A>
+__device__ __forceinline__ A select7(const A (&t)[7], int n) {
+  A d = uniform_reg(t[0]);
+#pragma unroll
+  for (int c = 1; c < 7; ++c) d = (n == c) ? uniform_reg(t[c]) : d;
+  return d;
+}
+template <> __device__ __forceinline__ float dinv_tab<float>(const OpArgs& o, int n) { return select7(o.dinv_f, n); }
+template <> __device__ __forceinline__ double dinv_tab<double>(const OpArgs& o, int n) { return select7(o.dinv_d, n); }
 template <class A> __device__ __forceinline__ A qcoef(const OpArgs& o);
 template <> __device__ __forceinline__ float qcoef<float>(const OpArgs& o) { return o.qc_f; }
 template <> __device__ __forceinline__ double qcoef<double>(const OpArgs& o) { return o.qc_d; }
@@ -333,42 +353,14 @@ struct Tiling {
   int ntj, ntk, seg, nblk;
 };
 
-// The inputs of p_k at one cell: r, p_{k-1} and the mask byte (nb = 0 outside
-// the grid, where nothing is loaded).
+// The inputs of p_k at one cell: r, p_{k-1}, the slow-row diagonal and the
+// mask byte (nb = 0 outside the grid).
 template <class V>
 struct Raw {
   V r, pp;
+  double ds;
   uint8_t nb;
 };
-__device__ __forceinline__ uint8_t load_nb(const OpArgs& o, int i, int j, int k) {
-  const Grid& g = o.g;
-  if (i < 0 || j < 0 || k < 0 || i >= g.nx || j >= g.ny || k >= g.nz) return 0;
-  return o.nbm[cid(g, i, j, k)];
-}
-// r and p_{k-1} of an unknown cell (0 elsewhere, where they are 0 and not read).
-template <class V>
-__device__ __forceinline__ Raw<V> load_vals(const OpArgs& o, const V* __restrict__ r, const V* __restrict__ pp,
-                                            uint8_t nb, int i, int j, int k) {
-  Raw<V> w{V(0), V(0), nb};
-  if (nb & NB_UNKNOWN) {
-    const int64_t idx = cid(o.g, i, j, k);
-    w.r = r[idx];
-    w.pp = pp[idx];
-  }
-  return w;
-}
-// p_k from the loaded inputs; 0 off the unknowns (r and p_{k-1} are 0 there).
-// A slow row's diagonal reads the flags around it.
-template <class V, int OPk>
-__device__ __forceinline__ V p_raw(const OpArgs& o, const Raw<V>& w, V be, int i, int j, int k) {
-  if (!(w.nb & NB_UNKNOWN)) return V(0);
-  V d = V(1);
-  if (o.precond) {
-    if (!(w.nb & NB_SLOW)) d = dinv_tab<V>(o, __popc((unsigned)(w.nb & 63)));
-    else d = (V)o.dslow[cid(o.g, i, j, k)];
-  }
-  return w.r * d + be * w.pp;
-}
 // Workgroup barrier that orders LDS only: global loads issued before it stay
 // in flight (a full __syncthreads would wait for them).
 __device__ __forceinline__ void lds_barrier() {
@@ -463,104 +455,151 @@ __global__ void __launch_bounds__(ST, 4) k_spmv(OpArgs o, Tiling T, int nbu, con
     hj = t2 % TJ;
     hk = (t2 < TJ) ? -1 : TK;
   }
-  // Software pipeline over planes.  At step i: the vectors of plane i+2 are
-  // loaded (on the unknowns given by its mask, loaded one step earlier), the
-  // mask of plane i+3 is loaded, p of plane i+1 is formed from the vectors
-  // loaded one step earlier and written to LDS, and q of plane i is
-  // computed.  The barrier only orders LDS, so those loads stay in flight.
-  V pprev[RPT], pcur[RPT], pnext[RPT];
-  uint8_t ncur[RPT], nnext[RPT], mn[RPT], mnn[RPT];
-  Raw<V> ra[RPT], rb[RPT], hra, hrb;  // vectors of planes i+1 (a) and i+2 (b)
-  uint8_t hmn, hmnn;
-  const int hi_ = ring ? 0 : -100000;  // non-ring threads load nothing for the ring
+  // columns (offsets inside a plane) and whether they lie in the grid
+  const int64_t sx = (int64_t)g.ny * g.nz;
+  int64_t col[RPT];
+  bool cok[RPT];
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
     const int j = j0 + jl0 + u;
-    const uint8_t n0 = load_nb(o, i_lo - 1, j, kk), n1 = load_nb(o, i_lo, j, kk);
-    ra[u] = load_vals<V>(o, r, pp, n0, i_lo - 1, j, kk);
-    rb[u] = load_vals<V>(o, r, pp, n1, i_lo, j, kk);
+    cok[u] = j < g.ny && kk < g.nz;
+    col[u] = cok[u] ? (int64_t)j * g.nz + kk : 0;
   }
-  hra = load_vals<V>(o, r, pp, load_nb(o, i_lo + hi_, j0 + hj, k0 + hk), i_lo, j0 + hj, k0 + hk);
+  const bool hok = ring && (j0 + hj) >= 0 && (j0 + hj) < g.ny && (k0 + hk) >= 0 && (k0 + hk) < g.nz;
+  const int64_t hcol = hok ? (int64_t)(j0 + hj) * g.nz + (k0 + hk) : 0;
+  // Every load below is issued by every lane, so the number of loads in
+  // flight is the same on all paths: a lane with nothing to read reads
+  // element 0 (a cached line) and discards it.
+  // (the mask as loaded; load_v applies the grid test, so that no
+  // instruction touches the byte before it is needed)
+  auto load_m = [&](int i, bool ok, int64_t c) -> uint8_t {
+    const bool in = ok && i >= 0 && i < g.nx;
+    return o.nbm[in ? (int64_t)i * sx + c : 0];
+  };
+  // vectors of one cell given its mask: r and p_{k-1} on the unknowns, the
+  // diagonal on the slow unknowns
+  auto load_v = [&](int i, bool ok, uint8_t raw, int64_t c, Raw<V>& w) {
+    const uint8_t nb = (ok && i >= 0 && i < g.nx) ? raw : (uint8_t)0;
+    const int64_t idx = (int64_t)i * sx + c;
+    const bool un = (nb & NB_UNKNOWN) != 0;
+    const int64_t iv = un ? idx : 0;
+    w.nb = nb;
+    w.r = r[iv];
+    w.pp = pp[iv];
+    w.ds = o.dslow[(un && (nb & NB_SLOW) && o.precond) ? idx : 0];
+  };
+  auto form = [&](const Raw<V>& w) -> V {
+    if (!(w.nb & NB_UNKNOWN)) return V(0);
+    V d = V(1);
+    if (o.precond) d = (w.nb & NB_SLOW) ? (V)w.ds : dinv_tab<V>(o, __popc((unsigned)(w.nb & 63)));
+    return w.r * d + be * w.pp;
+  };
+  // Software pipeline over planes.  At step i: the vectors of plane i+2 are
+  // loaded (given its mask, loaded one step earlier), the mask of plane i+3
+  // is loaded, p of plane i+1 is formed from the vectors loaded one step
+  // earlier and written to LDS, and q of plane i is computed into registers.
+  // p and q go out in batches of SB planes: a wait for a load while a store
+  // is outstanding waits for every access, so the batch's stores are issued
+  // together and the loads stay in flight across the other planes.
+  constexpr int SB = sizeof(V) == 4 ? 8 : 4;
+  V pprev[RPT], pcur[RPT], pnext[RPT];
+  uint8_t ncur[RPT], mn[RPT], mnn[RPT], hmn, hmnn;
+  Raw<V> ra[RPT], rb[RPT], hra, hrb;  // vectors of planes i+1 (a) and i+2 (b)
+  V sp[SB][RPT], sq[SB][RPT];
+  unsigned sst = 0;  // bit d*RPT+u: row u of batch plane d is an unknown
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
-    pprev[u] = p_raw<V, OPk>(o, ra[u], be, i_lo - 1, j0 + jl0 + u, kk);
-    pcur[u] = p_raw<V, OPk>(o, rb[u], be, i_lo, j0 + jl0 + u, kk);
-    ncur[u] = rb[u].nb;
+    Raw<V> w0, w1;
+    load_v(i_lo - 1, cok[u], load_m(i_lo - 1, cok[u], col[u]), col[u], w0);
+    load_v(i_lo, cok[u], load_m(i_lo, cok[u], col[u]), col[u], w1);
+    pprev[u] = form(w0);
+    pcur[u] = form(w1);
+    ncur[u] = w1.nb;
     pl[i_lo % 3][jl0 + u + 1][kl + 1] = pcur[u];
   }
-  if (ring) pl[i_lo % 3][hj + 1][hk + 1] = p_raw<V, OPk>(o, hra, be, i_lo, j0 + hj, k0 + hk);
-  // plane i_lo + 1 vectors and plane i_lo + 2 mask in flight
+  {
+    Raw<V> w;
+    load_v(i_lo, hok, load_m(i_lo, hok, hcol), hcol, w);
+    if (ring) pl[i_lo % 3][hj + 1][hk + 1] = form(w);
+  }
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
-    const int j = j0 + jl0 + u;
-    ra[u] = load_vals<V>(o, r, pp, load_nb(o, i_lo + 1, j, kk), i_lo + 1, j, kk);
-    mn[u] = load_nb(o, i_lo + 2, j, kk);
+    load_v(i_lo + 1, cok[u], load_m(i_lo + 1, cok[u], col[u]), col[u], ra[u]);
+    mn[u] = load_m(i_lo + 2, cok[u], col[u]);
   }
-  hra = load_vals<V>(o, r, pp, load_nb(o, i_lo + 1 + hi_, j0 + hj, k0 + hk), i_lo + 1, j0 + hj, k0 + hk);
-  hmn = load_nb(o, i_lo + 2 + hi_, j0 + hj, k0 + hk);
-  const int64_t sx = (int64_t)g.ny * g.nz;
+  load_v(i_lo + 1, hok, load_m(i_lo + 1, hok, hcol), hcol, hra);
+  hmn = load_m(i_lo + 2, hok, hcol);
   const V qc = qcoef<V>(o);
   Dacc a_pq = 0;
-  for (int i = i_lo; i < i_hi; ++i) {
-    const int sn = (i + 1) % 3, scur = i % 3;
-    // issue plane i+2 vectors, plane i+3 mask
+  auto flush = [&](int ib, int nd) {
 #pragma unroll
-    for (int u = 0; u < RPT; ++u) {
-      const int j = j0 + jl0 + u;
-      rb[u] = load_vals<V>(o, r, pp, mn[u], i + 2, j, kk);
-      mnn[u] = load_nb(o, i + 3, j, kk);
-    }
-    hrb = load_vals<V>(o, r, pp, hmn, i + 2, j0 + hj, k0 + hk);
-    hmnn = load_nb(o, i + 3 + hi_, j0 + hj, k0 + hk);
-    // form plane i+1
-#pragma unroll
-    for (int u = 0; u < RPT; ++u) {
-      pnext[u] = p_raw<V, OPk>(o, ra[u], be, i + 1, j0 + jl0 + u, kk);
-      nnext[u] = ra[u].nb;
-      pl[sn][jl0 + u + 1][kl + 1] = pnext[u];
-    }
-    if (ring) pl[sn][hj + 1][hk + 1] = p_raw<V, OPk>(o, hra, be, i + 1, j0 + hj, k0 + hk);
-    lds_barrier();
-    unsigned slow = 0;
-#pragma unroll
-    for (int u = 0; u < RPT; ++u) {
-      const int jl = jl0 + u;
-      const uint8_t nb = ncur[u];
-      if (!(nb & NB_UNKNOWN)) continue;  // also every cell outside the grid
-      if (nb & NB_SLOW) {
-        slow |= 1u << u;
-        continue;
-      }
-      const int64_t idx = (int64_t)i * sx + (int64_t)(j0 + jl) * g.nz + kk;
-      const V pcv = pcur[u];
-      // open faces in the order +x, +y, +z, -x, -y, -z
-      V s = V(0);
-      if (nb & 1) s += pnext[u] - pcv;
-      if (nb & 2) s += pl[scur][jl + 2][kl + 1] - pcv;
-      if (nb & 4) s += pl[scur][jl + 1][kl + 2] - pcv;
-      if (nb & 8) s += pprev[u] - pcv;
-      if (nb & 16) s += pl[scur][jl][kl + 1] - pcv;
-      if (nb & 32) s += pl[scur][jl + 1][kl] - pcv;
-      const V qv = s * qc;
-      pc[idx] = pcv;
-      q[idx] = qv;
-      a_pq += (Dacc)pcv * (Dacc)qv;
-    }
-    if (slow) {  // q of slow rows: k_slow_q
+    for (int d = 0; d < SB; ++d)
 #pragma unroll
       for (int u = 0; u < RPT; ++u)
-        if ((slow >> u) & 1u) pc[(int64_t)i * sx + (int64_t)(j0 + jl0 + u) * g.nz + kk] = pcur[u];
-    }
+        if (d < nd && ((sst >> (d * RPT + u)) & 1u)) {
+          const int64_t idx = (int64_t)(ib + d) * sx + col[u];
+          pc[idx] = sp[d][u];
+          q[idx] = sq[d][u];
+        }
+    sst = 0;
+  };
+  for (int ib = i_lo; ib < i_hi; ib += SB) {
+    if (ib > i_lo) flush(ib - SB, SB);
 #pragma unroll
-    for (int u = 0; u < RPT; ++u) {
-      pprev[u] = pcur[u];
-      pcur[u] = pnext[u];
-      ncur[u] = nnext[u];
-      ra[u] = rb[u];
-      mn[u] = mnn[u];
+    for (int d = 0; d < SB; ++d) {
+      const int i = ib + d;
+      if (i < i_hi) {  // uniform over the block
+        const int sn = (i + 1) % 3, scur = i % 3;
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) {
+          load_v(i + 2, cok[u], mn[u], col[u], rb[u]);
+          mnn[u] = load_m(i + 3, cok[u], col[u]);
+        }
+        load_v(i + 2, hok, hmn, hcol, hrb);
+        hmnn = load_m(i + 3, hok, hcol);
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) {
+          pnext[u] = form(ra[u]);
+          pl[sn][jl0 + u + 1][kl + 1] = pnext[u];
+        }
+        if (ring) pl[sn][hj + 1][hk + 1] = form(hra);
+        lds_barrier();
+        // q of plane i (slow rows: p only, q from k_slow_q)
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) {
+          const int jl = jl0 + u;
+          const uint8_t nb = ncur[u];
+          const V pcv = pcur[u];
+          V s = V(0);
+          if (nb & 1) s += pnext[u] - pcv;
+          if (nb & 2) s += pl[scur][jl + 2][kl + 1] - pcv;
+          if (nb & 4) s += pl[scur][jl + 1][kl + 2] - pcv;
+          if (nb & 8) s += pprev[u] - pcv;
+          if (nb & 16) s += pl[scur][jl][kl + 1] - pcv;
+          if (nb & 32) s += pl[scur][jl + 1][kl] - pcv;
+          const bool fast = (nb & (NB_UNKNOWN | NB_SLOW)) == NB_UNKNOWN;
+          const V qv = fast ? s * qc : V(0);
+          if (nb & NB_UNKNOWN) sst |= 1u << (d * RPT + u);
+          sp[d][u] = pcv;
+          sq[d][u] = qv;
+          a_pq += (Dacc)pcv * (Dacc)qv;
+        }
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) {
+          pprev[u] = pcur[u];
+          pcur[u] = pnext[u];
+          ncur[u] = ra[u].nb;
+          ra[u] = rb[u];
+          mn[u] = mnn[u];
+        }
+        hra = hrb;
+        hmn = hmnn;
+      }
     }
-    hra = hrb;
-    hmn = hmnn;
+  }
+  {
+    const int last = i_lo + ((i_hi - i_lo - 1) / SB) * SB;
+    if (i_hi > i_lo) flush(last, i_hi - last);
   }
   double v[1] = {(double)a_pq};
   write_partials<1, ST>(v, P.S);
@@ -637,6 +676,7 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
   for (int64_t base = c0 + threadIdx.x; base < c1; base += (int64_t)UN * PB) {
     uint8_t nb[UN];
     V pv[UN], xv[UN], rv[UN], qv[UN];
+    double ds[UN];
 #pragma unroll
     for (int u = 0; u < UN; ++u) {
       const int64_t idx = base + (int64_t)u * PB;
@@ -646,6 +686,13 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
       xv[u] = in ? x[idx] : V(0);
       rv[u] = in ? r[idx] : V(0);
       qv[u] = in ? q[idx] : V(0);  // q is only defined on the unknowns; used only there
+    }
+    // slow-row diagonals once the masks are in (other lanes read element 0)
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+      const int64_t idx = base + (int64_t)u * PB;
+      const bool sl = o.precond && (nb[u] & (NB_UNKNOWN | NB_SLOW)) == (NB_UNKNOWN | NB_SLOW);
+      ds[u] = o.dslow[sl ? idx : 0];
     }
 #pragma unroll
     for (int u = 0; u < UN; ++u) {
@@ -659,7 +706,8 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
       }
       const V rn = rv[u] - al * qv[u];
       r[idx] = rn;
-      const V z = rn * inv_diag<V, OPk>(o, idx, nb[u]);
+      const V d = !o.precond ? V(1) : (nb[u] & NB_SLOW) ? (V)ds[u] : dinv_tab<V>(o, __popc((unsigned)(nb[u] & 63)));
+      const V z = rn * d;
       a_rz += (Dacc)rn * (Dacc)z;
       a_rr += (Dacc)rn * (Dacc)rn;
     }
@@ -681,40 +729,69 @@ __global__ void __launch_bounds__(PB) k_resid64(OpArgs o, Launch L, const float*
   int64_t st[6];
   nb_strides(g, st);
   double a_rz = 0, a_rr = 0;
-  CELLS_OF_BLOCK(g.G) {
-    const uint8_t nb = o.nbm[idx];
-    if (!(nb & NB_UNKNOWN)) continue;
-    double ax;
-    if (!(nb & NB_SLOW)) {
-      const double xc = x64[idx] + (double)x[idx];
-      double s = 0.0;
+  // UN cells per thread per pass: masks first, then every fast row's seven
+  // (x64, x) pairs for all UN cells, so the latencies overlap
+  constexpr int UN = 2;
+  const int64_t c0 = chunk_begin(L);
+  const int64_t c1 = (g.G < c0 + L.chunk) ? g.G : c0 + L.chunk;
+  const double bc = (OPk == OP_UNIT) ? 1.0 : o.il;
+  const double ih2 = 1.0 / ((double)g.h * (double)g.h);
+  for (int64_t base = c0 + threadIdx.x; base < c1; base += (int64_t)UN * PB) {
+    uint8_t nb[UN];
 #pragma unroll
-      for (int f = 0; f < 6; ++f)
-        if (nb & (1u << f)) s += (x64[idx + st[f]] + (double)x[idx + st[f]]) - xc;
-      const double bc = (OPk == OP_UNIT) ? 1.0 : o.il;
-      ax = (bc * s) / ((double)g.h * (double)g.h);
-    } else {
-      int i, j, kk;
-      ijk_of(g, idx, i, j, kk);
-      const int8_t m = o.mark[idx];
-      const double xc = x64[idx] + (double)x[idx];
-      double s = 0.0;
-#pragma unroll
-      for (int f = 0; f < 6; ++f) {
-        const int8_t mn = mark_at(g, o.mark, i + c_off[f][0], j + c_off[f][1], kk + c_off[f][2]);
-        if (neumann(o, OPk, mn)) continue;
-        const int64_t n = idx + st[f];
-        const double xn = (mn == FLUID || mn == AIR) ? x64[n] + (double)x[n] : 0.0;
-        const double bf = (OPk == OP_UNIT) ? 1.0 : face_beta<double>(o, idx, n, m, mn);
-        s += bf * (xn - xc);
-      }
-      ax = s / ((double)g.h * (double)g.h);
+    for (int u = 0; u < UN; ++u) {
+      const int64_t idx = base + (int64_t)u * PB;
+      nb[u] = (idx < c1) ? o.nbm[idx] : (uint8_t)0;
     }
-    const double rv = (double)b[idx] - ax;
-    const float rf = (float)rv;
-    r[idx] = rf;
-    a_rr += rv * rv;
-    a_rz += (double)rf * (double)(rf * inv_diag<float, OPk>(o, idx, nb));
+    double xs[UN][7];
+    float bv[UN];
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+      const int64_t idx = base + (int64_t)u * PB;
+      const bool fast = (nb[u] & (NB_UNKNOWN | NB_SLOW)) == NB_UNKNOWN;
+#pragma unroll
+      for (int f = 0; f < 7; ++f) {
+        const int64_t n = (f < 6) ? idx + st[f] : idx;
+        const bool use = fast && (f == 6 || (nb[u] & (1u << f)));
+        xs[u][f] = use ? x64[n] + (double)x[n] : 0.0;
+      }
+      bv[u] = (nb[u] & NB_UNKNOWN) ? b[idx] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < UN; ++u) {
+      if (!(nb[u] & NB_UNKNOWN)) continue;
+      const int64_t idx = base + (int64_t)u * PB;
+      double ax;
+      if (!(nb[u] & NB_SLOW)) {
+        const double xc = xs[u][6];
+        double sacc = 0.0;
+#pragma unroll
+        for (int f = 0; f < 6; ++f)
+          if (nb[u] & (1u << f)) sacc += xs[u][f] - xc;
+        ax = (bc * sacc) * ih2;
+      } else {
+        int i, j, kk;
+        ijk_of(g, idx, i, j, kk);
+        const int8_t m = o.mark[idx];
+        const double xc = x64[idx] + (double)x[idx];
+        double sacc = 0.0;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+          const int8_t mn = mark_at(g, o.mark, i + c_off[f][0], j + c_off[f][1], kk + c_off[f][2]);
+          if (neumann(o, OPk, mn)) continue;
+          const int64_t n = idx + st[f];
+          const double xn = (mn == FLUID || mn == AIR) ? x64[n] + (double)x[n] : 0.0;
+          const double bf = (OPk == OP_UNIT) ? 1.0 : face_beta<double>(o, idx, n, m, mn);
+          sacc += bf * (xn - xc);
+        }
+        ax = sacc / ((double)g.h * (double)g.h);
+      }
+      const double rv = (double)bv[u] - ax;
+      const float rf = (float)rv;
+      r[idx] = rf;
+      a_rr += rv * rv;
+      a_rz += (double)rf * (double)(rf * inv_diag<float, OPk>(o, idx, nb[u]));
+    }
   }
   double v[2] = {a_rz, a_rr};
   write_partials<2>(v, uslot(P, k));
