@@ -46,7 +46,7 @@ struct OpArgs {
   const int8_t* mark;
   const float* ls;
   const uint8_t* nbm;
-  const double* dslow;  // 1/diag of slow rows, filled with the mask
+  const void* dslow;  // 1/diag of slow rows (element type of the solve), filled with the mask
   double il, ia;    // 1/rho_liquid, 1/rho_air
   double hal[6];    // h^2/(alpha*dt) per cell type (heat)
   int solid_c;      // SOLID cells Neumann (pressure)
@@ -186,8 +186,8 @@ __device__ __forceinline__ A inv_diag_generic(const OpArgs& o, int64_t idx) {
   return -((A)g.h * (A)g.h) / s;
 }
 
-template <int OPk>
-__global__ void k_nbm(OpArgs o, uint8_t* __restrict__ nbm, double* __restrict__ dslow) {
+template <int OPk, class V>
+__global__ void k_nbm(OpArgs o, uint8_t* __restrict__ nbm, V* __restrict__ dslow) {
   const Grid& g = o.g;
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= g.G) return;
@@ -206,7 +206,13 @@ __global__ void k_nbm(OpArgs o, uint8_t* __restrict__ nbm, double* __restrict__ 
   if (OPk == OP_HEAT) slow = true;  // the heat operator reads the flags (diffusivity per type)
   if (slow) b |= NB_SLOW;
   nbm[idx] = b;
-  if (slow && (b & NB_UNKNOWN)) dslow[idx] = inv_diag_generic<double, OPk>(o, idx);
+  if (slow && (b & NB_UNKNOWN)) dslow[idx] = (V)inv_diag_generic<double, OPk>(o, idx);
+}
+
+// Slow-row diagonal as stored for a solve whose vectors are of type A.
+template <class A>
+__device__ __forceinline__ A dslow_at(const OpArgs& o, int64_t idx) {
+  return ((const A*)o.dslow)[idx];
 }
 
 // 1/diag from the mask byte (fast) or the flags (slow); 1 off the unknowns or
@@ -214,7 +220,7 @@ __global__ void k_nbm(OpArgs o, uint8_t* __restrict__ nbm, double* __restrict__ 
 template <class A, int OPk>
 __device__ __forceinline__ A inv_diag(const OpArgs& o, int64_t idx, uint8_t b) {
   if (!o.precond || !(b & NB_UNKNOWN)) return A(1);
-  if (b & NB_SLOW) return (A)o.dslow[idx];
+  if (b & NB_SLOW) return dslow_at<A>(o, idx);
   return dinv_tab<A>(o, __popc((unsigned)(b & 63)));
 }
 
@@ -357,8 +363,7 @@ struct Tiling {
 // mask byte (nb = 0 outside the grid).
 template <class V>
 struct Raw {
-  V r, pp;
-  double ds;
+  V r, pp, ds;
   uint8_t nb;
 };
 // Workgroup barrier that orders LDS only: global loads issued before it stay
@@ -405,6 +410,8 @@ __global__ void __launch_bounds__(ST, 4) k_spmv(OpArgs o, Tiling T, int nbu, con
                                              Parts P, SolveScalars* sc, int k, int ref, double eps_abs, double tol,
                                              int cap) {
   if (sc->done) return;
+  __shared__ V dtab[8];  // 1/diag of the fast rows by open-face count (ordered by read_partials' barriers)
+  if (threadIdx.x < 8) dtab[threadIdx.x] = (threadIdx.x < 7) ? dinv_tab<V>(o, threadIdx.x) : V(1);
   double cur[2], prv[2];
   read_partials<2, ST>(uslot(P, k - 1), nbu, cur);
   if (k > 0) read_partials<2, ST>(uslot(P, k), nbu, prv);
@@ -455,67 +462,69 @@ __global__ void __launch_bounds__(ST, 4) k_spmv(OpArgs o, Tiling T, int nbu, con
     hj = t2 % TJ;
     hk = (t2 < TJ) ? -1 : TK;
   }
-  // columns (offsets inside a plane) and whether they lie in the grid
+  // columns (offsets inside a plane, below 2^31) and whether they lie in the grid
   const int64_t sx = (int64_t)g.ny * g.nz;
-  int64_t col[RPT];
+  uint32_t col[RPT];
   bool cok[RPT];
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
     const int j = j0 + jl0 + u;
     cok[u] = j < g.ny && kk < g.nz;
-    col[u] = cok[u] ? (int64_t)j * g.nz + kk : 0;
+    col[u] = cok[u] ? (uint32_t)(j * g.nz + kk) : 0u;
   }
   const bool hok = ring && (j0 + hj) >= 0 && (j0 + hj) < g.ny && (k0 + hk) >= 0 && (k0 + hk) < g.nz;
-  const int64_t hcol = hok ? (int64_t)(j0 + hj) * g.nz + (k0 + hk) : 0;
-  // Every load below is issued by every lane, so the number of loads in
-  // flight is the same on all paths: a lane with nothing to read reads
-  // element 0 (a cached line) and discards it.
-  // (the mask as loaded; load_v applies the grid test, so that no
-  // instruction touches the byte before it is needed)
-  auto load_m = [&](int i, bool ok, int64_t c) -> uint8_t {
-    const bool in = ok && i >= 0 && i < g.nx;
-    return o.nbm[in ? (int64_t)i * sx + c : 0];
+  const uint32_t hcol = hok ? (uint32_t)((j0 + hj) * g.nz + (k0 + hk)) : 0u;
+  // Loads address a plane (a uniform base, clamped into the grid) plus a
+  // 32-bit column offset.  Every lane issues every load, so the number of
+  // loads in flight is the same on all paths: a lane with nothing to read
+  // reads its plane's first element and discards it.  (The mask is kept as
+  // loaded; load_v applies the grid test, so that no instruction touches the
+  // byte before it is needed.)
+  auto plane = [&](int i) -> int64_t { return (int64_t)min(max(i, 0), g.nx - 1) * sx; };
+  auto load_m = [&](int i, bool ok, uint32_t c) -> uint8_t {
+    const uint8_t* mp = o.nbm + plane(i);
+    return mp[(ok && i >= 0 && i < g.nx) ? c : 0u];
   };
   // vectors of one cell given its mask: r and p_{k-1} on the unknowns, the
   // diagonal on the slow unknowns
-  auto load_v = [&](int i, bool ok, uint8_t raw, int64_t c, Raw<V>& w) {
+  auto load_v = [&](int i, bool ok, uint8_t raw, uint32_t c, Raw<V>& w) {
     const uint8_t nb = (ok && i >= 0 && i < g.nx) ? raw : (uint8_t)0;
-    const int64_t idx = (int64_t)i * sx + c;
+    const int64_t pb = plane(i);
     const bool un = (nb & NB_UNKNOWN) != 0;
-    const int64_t iv = un ? idx : 0;
+    const uint32_t cv = un ? c : 0u;
     w.nb = nb;
-    w.r = r[iv];
-    w.pp = pp[iv];
-    w.ds = o.dslow[(un && (nb & NB_SLOW) && o.precond) ? idx : 0];
+    w.r = (r + pb)[cv];
+    w.pp = (pp + pb)[cv];
+    w.ds = ((const V*)o.dslow + pb)[(un && (nb & NB_SLOW) && o.precond) ? c : 0u];
   };
   auto form = [&](const Raw<V>& w) -> V {
-    if (!(w.nb & NB_UNKNOWN)) return V(0);
-    V d = V(1);
-    if (o.precond) d = (w.nb & NB_SLOW) ? (V)w.ds : dinv_tab<V>(o, __popc((unsigned)(w.nb & 63)));
-    return w.r * d + be * w.pp;
+    const V dt = dtab[__popc((unsigned)(w.nb & 63))];
+    const V d = !o.precond ? V(1) : (w.nb & NB_SLOW) ? w.ds : dt;
+    return (w.nb & NB_UNKNOWN) ? w.r * d + be * w.pp : V(0);
   };
-  // Software pipeline over planes.  At step i: the vectors of plane i+2 are
-  // loaded (given its mask, loaded one step earlier), the mask of plane i+3
-  // is loaded, p of plane i+1 is formed from the vectors loaded one step
-  // earlier and written to LDS, and q of plane i is computed into registers.
-  // p and q go out in batches of SB planes: a wait for a load while a store
-  // is outstanding waits for every access, so the batch's stores are issued
-  // together and the loads stay in flight across the other planes.
-  constexpr int SB = sizeof(V) == 4 ? 8 : 4;
-  V pprev[RPT], pcur[RPT], pnext[RPT];
-  uint8_t ncur[RPT], mn[RPT], mnn[RPT], hmn, hmnn;
-  Raw<V> ra[RPT], rb[RPT], hra, hrb;  // vectors of planes i+1 (a) and i+2 (b)
-  V sp[SB][RPT], sq[SB][RPT];
-  unsigned sst = 0;  // bit d*RPT+u: row u of batch plane d is an unknown
+  // Software pipeline over planes, three planes of loads ahead.  At step i:
+  // the mask of plane i+4 and the vectors of plane i+3 (given its mask,
+  // loaded one step earlier) are loaded; p of plane i+1 is formed from the
+  // vectors loaded two steps earlier and written to LDS; q of plane i is
+  // computed and p, q stored.  The barrier only orders LDS, so the loads of
+  // planes i+2..i+4 stay in flight.  Per-plane registers live in three slots,
+  // slot (plane - i_lo) % 3, and the loop is unrolled by a multiple of 3 so
+  // that every slot index is a constant.
+  constexpr int UNR = 6;
+  Raw<V> R[3][RPT], HR[3];   // vectors of planes i+1 .. i+3
+  uint8_t M[3][RPT], HM[3];  // masks of planes i+3, i+4
+  V PV[3][RPT];              // p of planes i-1 .. i+1
+  uint8_t NB[3][RPT];        // masks of planes i, i+1
+  // prologue: p of planes i_lo-1 and i_lo, vectors of i_lo+1 and i_lo+2, mask of i_lo+3
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
     Raw<V> w0, w1;
     load_v(i_lo - 1, cok[u], load_m(i_lo - 1, cok[u], col[u]), col[u], w0);
     load_v(i_lo, cok[u], load_m(i_lo, cok[u], col[u]), col[u], w1);
-    pprev[u] = form(w0);
-    pcur[u] = form(w1);
-    ncur[u] = w1.nb;
-    pl[i_lo % 3][jl0 + u + 1][kl + 1] = pcur[u];
+    PV[2][u] = form(w0);
+    PV[0][u] = form(w1);
+    NB[0][u] = w1.nb;
+    pl[i_lo % 3][jl0 + u + 1][kl + 1] = PV[0][u];
   }
   {
     Raw<V> w;
@@ -524,82 +533,65 @@ __global__ void __launch_bounds__(ST, 4) k_spmv(OpArgs o, Tiling T, int nbu, con
   }
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
-    load_v(i_lo + 1, cok[u], load_m(i_lo + 1, cok[u], col[u]), col[u], ra[u]);
-    mn[u] = load_m(i_lo + 2, cok[u], col[u]);
+    load_v(i_lo + 1, cok[u], load_m(i_lo + 1, cok[u], col[u]), col[u], R[1][u]);
+    load_v(i_lo + 2, cok[u], load_m(i_lo + 2, cok[u], col[u]), col[u], R[2][u]);
+    M[0][u] = load_m(i_lo + 3, cok[u], col[u]);
   }
-  load_v(i_lo + 1, hok, load_m(i_lo + 1, hok, hcol), hcol, hra);
-  hmn = load_m(i_lo + 2, hok, hcol);
+  load_v(i_lo + 1, hok, load_m(i_lo + 1, hok, hcol), hcol, HR[1]);
+  load_v(i_lo + 2, hok, load_m(i_lo + 2, hok, hcol), hcol, HR[2]);
+  HM[0] = load_m(i_lo + 3, hok, hcol);
   const V qc = qcoef<V>(o);
   Dacc a_pq = 0;
-  auto flush = [&](int ib, int nd) {
+  for (int ib = i_lo; ib < i_hi; ib += UNR) {
 #pragma unroll
-    for (int d = 0; d < SB; ++d)
-#pragma unroll
-      for (int u = 0; u < RPT; ++u)
-        if (d < nd && ((sst >> (d * RPT + u)) & 1u)) {
-          const int64_t idx = (int64_t)(ib + d) * sx + col[u];
-          pc[idx] = sp[d][u];
-          q[idx] = sq[d][u];
-        }
-    sst = 0;
-  };
-  for (int ib = i_lo; ib < i_hi; ib += SB) {
-    if (ib > i_lo) flush(ib - SB, SB);
-#pragma unroll
-    for (int d = 0; d < SB; ++d) {
+    for (int d = 0; d < UNR; ++d) {
       const int i = ib + d;
       if (i < i_hi) {  // uniform over the block
+        const int s0 = d % 3, s1 = (d + 1) % 3, s2 = (d + 2) % 3;  // slots of planes i, i+1, i+2 (i+3 = i)
         const int sn = (i + 1) % 3, scur = i % 3;
+        // loads: mask of plane i+4 (slot s1), vectors of plane i+3 (slot s0)
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) M[s1][u] = load_m(i + 4, cok[u], col[u]);
+        HM[s1] = load_m(i + 4, hok, hcol);
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) load_v(i + 3, cok[u], M[s0][u], col[u], R[s0][u]);
+        load_v(i + 3, hok, HM[s0], hcol, HR[s0]);
+        // form plane i+1
 #pragma unroll
         for (int u = 0; u < RPT; ++u) {
-          load_v(i + 2, cok[u], mn[u], col[u], rb[u]);
-          mnn[u] = load_m(i + 3, cok[u], col[u]);
+          PV[s1][u] = form(R[s1][u]);
+          NB[s1][u] = R[s1][u].nb;
+          pl[sn][jl0 + u + 1][kl + 1] = PV[s1][u];
         }
-        load_v(i + 2, hok, hmn, hcol, hrb);
-        hmnn = load_m(i + 3, hok, hcol);
-#pragma unroll
-        for (int u = 0; u < RPT; ++u) {
-          pnext[u] = form(ra[u]);
-          pl[sn][jl0 + u + 1][kl + 1] = pnext[u];
-        }
-        if (ring) pl[sn][hj + 1][hk + 1] = form(hra);
+        if (ring) pl[sn][hj + 1][hk + 1] = form(HR[s1]);
         lds_barrier();
-        // q of plane i (slow rows: p only, q from k_slow_q)
+        // q of plane i: the open faces in the order +x, +y, +z, -x, -y, -z
+        // (slow rows: p only, q from k_slow_q)
+        const int64_t pb = (int64_t)i * sx;
 #pragma unroll
         for (int u = 0; u < RPT; ++u) {
           const int jl = jl0 + u;
-          const uint8_t nb = ncur[u];
-          const V pcv = pcur[u];
-          V s = V(0);
-          if (nb & 1) s += pnext[u] - pcv;
-          if (nb & 2) s += pl[scur][jl + 2][kl + 1] - pcv;
-          if (nb & 4) s += pl[scur][jl + 1][kl + 2] - pcv;
-          if (nb & 8) s += pprev[u] - pcv;
-          if (nb & 16) s += pl[scur][jl][kl + 1] - pcv;
-          if (nb & 32) s += pl[scur][jl + 1][kl] - pcv;
+          const uint8_t nb = NB[s0][u];
+          const V pcv = PV[s0][u];
+          const V nyp = pl[scur][jl + 2][kl + 1], nzp = pl[scur][jl + 1][kl + 2];
+          const V nym = pl[scur][jl][kl + 1], nzm = pl[scur][jl + 1][kl];
+          V sum = V(0);
+          sum = (nb & 1) ? sum + (PV[s1][u] - pcv) : sum;
+          sum = (nb & 2) ? sum + (nyp - pcv) : sum;
+          sum = (nb & 4) ? sum + (nzp - pcv) : sum;
+          sum = (nb & 8) ? sum + (PV[s2][u] - pcv) : sum;
+          sum = (nb & 16) ? sum + (nym - pcv) : sum;
+          sum = (nb & 32) ? sum + (nzm - pcv) : sum;
           const bool fast = (nb & (NB_UNKNOWN | NB_SLOW)) == NB_UNKNOWN;
-          const V qv = fast ? s * qc : V(0);
-          if (nb & NB_UNKNOWN) sst |= 1u << (d * RPT + u);
-          sp[d][u] = pcv;
-          sq[d][u] = qv;
+          const V qv = fast ? sum * qc : V(0);
           a_pq += (Dacc)pcv * (Dacc)qv;
+          if (nb & NB_UNKNOWN) {
+            (pc + pb)[col[u]] = pcv;
+            if (fast) (q + pb)[col[u]] = qv;
+          }
         }
-#pragma unroll
-        for (int u = 0; u < RPT; ++u) {
-          pprev[u] = pcur[u];
-          pcur[u] = pnext[u];
-          ncur[u] = ra[u].nb;
-          ra[u] = rb[u];
-          mn[u] = mnn[u];
-        }
-        hra = hrb;
-        hmn = hmnn;
       }
     }
-  }
-  {
-    const int last = i_lo + ((i_hi - i_lo - 1) / SB) * SB;
-    if (i_hi > i_lo) flush(last, i_hi - last);
   }
   double v[1] = {(double)a_pq};
   write_partials<1, ST>(v, P.S);
@@ -667,49 +659,118 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
   }
   const V al = (V)(sc->rz / t[0]);
   Dacc a_rz = 0, a_rr = 0;
-  // UN cells per thread per pass, all loads issued before any use (the
-  // vectors are read unconditionally: off the unknowns they are 0 or unused,
-  // and a load that waits for the mask first costs more than it saves)
-  constexpr int UN = 4;
+  // one cell: x += al p (or x64 += x + al p, x = 0 when folding), r -= al q,
+  // partial r.z and r.r
+  auto cell = [&](uint8_t nb, V pv, V xv, V rv, V qv, V ds, double x64v, V& xo, V& ro, double& x64o) {
+    xo = fold ? V(0) : xv + al * pv;
+    if (fold) x64o = x64v + (double)(xv + al * pv);
+    const V rn = rv - al * qv;
+    ro = rn;
+    const V d = !o.precond ? V(1) : (nb & NB_SLOW) ? ds : dinv_tab<V>(o, __popc((unsigned)(nb & 63)));
+    const V z = rn * d;
+    a_rz += (Dacc)rn * (Dacc)z;
+    a_rr += (Dacc)rn * (Dacc)rn;
+  };
+  // Quads of 4 consecutive cells per thread (one 16-byte access per FP32
+  // vector, two per FP64 vector), UQ quads per pass.  The masks of the next
+  // pass are loaded one pass ahead; the vectors of a quad are read only if it
+  // holds an unknown, and a quad of unknowns is stored whole, others cell by
+  // cell.
+  constexpr int UQ = 2;
+  struct alignas(4 * sizeof(V)) Q4 { V e[4]; };
+  struct alignas(32) D4 { double e[4]; };
   const int64_t c0 = chunk_begin(L);
-  const int64_t c1 = (o.g.G < c0 + L.chunk) ? o.g.G : c0 + L.chunk;
-  for (int64_t base = c0 + threadIdx.x; base < c1; base += (int64_t)UN * PB) {
-    uint8_t nb[UN];
-    V pv[UN], xv[UN], rv[UN], qv[UN];
-    double ds[UN];
+  const int64_t c1 = max(c0, (o.g.G < c0 + L.chunk) ? o.g.G : c0 + L.chunk);  // blocks past the grid: empty
+  const int64_t cq = c0 + ((c1 - c0) & ~(int64_t)3);                            // end of the whole quads
+  constexpr int64_t PASS = (int64_t)4 * UQ * PB;
+  auto load_masks = [&](int64_t base, uint32_t (&m)[UQ]) {
 #pragma unroll
-    for (int u = 0; u < UN; ++u) {
-      const int64_t idx = base + (int64_t)u * PB;
-      const bool in = idx < c1;
-      nb[u] = in ? o.nbm[idx] : (uint8_t)0;
-      pv[u] = in ? pc[idx] : V(0);
-      xv[u] = in ? x[idx] : V(0);
-      rv[u] = in ? r[idx] : V(0);
-      qv[u] = in ? q[idx] : V(0);  // q is only defined on the unknowns; used only there
+    for (int u = 0; u < UQ; ++u) {
+      const int64_t b4 = base + (int64_t)u * 4 * PB;
+      m[u] = (b4 < cq) ? *(const uint32_t*)(o.nbm + b4) : 0u;
     }
-    // slow-row diagonals once the masks are in (other lanes read element 0)
+  };
+  uint32_t mn[UQ];
+  load_masks(c0 + 4 * (int64_t)threadIdx.x, mn);
+  for (int64_t base = c0 + 4 * (int64_t)threadIdx.x; base < cq; base += PASS) {
+    uint32_t m4[UQ];
 #pragma unroll
-    for (int u = 0; u < UN; ++u) {
-      const int64_t idx = base + (int64_t)u * PB;
-      const bool sl = o.precond && (nb[u] & (NB_UNKNOWN | NB_SLOW)) == (NB_UNKNOWN | NB_SLOW);
-      ds[u] = o.dslow[sl ? idx : 0];
-    }
+    for (int u = 0; u < UQ; ++u) m4[u] = mn[u];
+    Q4 pv[UQ], xv[UQ], rv[UQ], qv[UQ], ds[UQ];
+    D4 x64v[UQ];
 #pragma unroll
-    for (int u = 0; u < UN; ++u) {
-      if (!(nb[u] & NB_UNKNOWN)) continue;
-      const int64_t idx = base + (int64_t)u * PB;
-      if (fold) {
-        x64[idx] += (double)(xv[u] + al * pv[u]);
-        x[idx] = V(0);
-      } else {
-        x[idx] = xv[u] + al * pv[u];
+    for (int u = 0; u < UQ; ++u) {
+      const int64_t b4 = base + (int64_t)u * 4 * PB;
+      if (m4[u] & 0x40404040u) {
+        pv[u] = *(const Q4*)(pc + b4);
+        xv[u] = *(const Q4*)(x + b4);
+        rv[u] = *(const Q4*)(r + b4);
+        qv[u] = *(const Q4*)(q + b4);  // q is only defined on the unknowns; used only there
+        if (fold) x64v[u] = *(const D4*)(x64 + b4);
       }
-      const V rn = rv[u] - al * qv[u];
-      r[idx] = rn;
-      const V d = !o.precond ? V(1) : (nb[u] & NB_SLOW) ? (V)ds[u] : dinv_tab<V>(o, __popc((unsigned)(nb[u] & 63)));
-      const V z = rn * d;
-      a_rz += (Dacc)rn * (Dacc)z;
-      a_rr += (Dacc)rn * (Dacc)rn;
+    }
+    load_masks(base + PASS, mn);
+    // slow-row diagonals, for quads holding a slow row (the quad's other
+    // cells read its first element)
+#pragma unroll
+    for (int u = 0; u < UQ; ++u) {
+      const int64_t b4 = base + (int64_t)u * 4 * PB;
+      if (o.precond && (m4[u] & 0x80808080u)) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const uint8_t nb = (uint8_t)(m4[u] >> (8 * e));
+          const bool sl = (nb & (NB_UNKNOWN | NB_SLOW)) == (NB_UNKNOWN | NB_SLOW);
+          ds[u].e[e] = dslow_at<V>(o, sl ? b4 + e : b4);
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) ds[u].e[e] = V(1);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UQ; ++u) {
+      const uint32_t un4 = m4[u] & 0x40404040u;
+      if (!un4) continue;
+      const int64_t b4 = base + (int64_t)u * 4 * PB;
+      Q4 xo, ro;
+      D4 x64o;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const uint8_t nb = (uint8_t)(m4[u] >> (8 * e));
+        xo.e[e] = xv[u].e[e];
+        ro.e[e] = rv[u].e[e];
+        x64o.e[e] = fold ? x64v[u].e[e] : 0.0;
+        if (nb & NB_UNKNOWN)
+          cell(nb, pv[u].e[e], xv[u].e[e], rv[u].e[e], qv[u].e[e], ds[u].e[e], x64o.e[e], xo.e[e], ro.e[e], x64o.e[e]);
+      }
+      if (un4 == 0x40404040u) {
+        *(Q4*)(x + b4) = xo;
+        *(Q4*)(r + b4) = ro;
+        if (fold) *(D4*)(x64 + b4) = x64o;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+          if ((m4[u] >> (8 * e)) & NB_UNKNOWN) {
+            x[b4 + e] = xo.e[e];
+            r[b4 + e] = ro.e[e];
+            if (fold) x64[b4 + e] = x64o.e[e];
+          }
+      }
+    }
+  }
+  // cells after the last whole quad (grids whose size is not a multiple of 4)
+  if (threadIdx.x < c1 - cq) {
+    const int64_t idx = cq + threadIdx.x;
+    const uint8_t nb = o.nbm[idx];
+    if (nb & NB_UNKNOWN) {
+      const bool sl = o.precond && (nb & NB_SLOW);
+      V xo, ro;
+      double x64o = 0.0;
+      cell(nb, pc[idx], x[idx], r[idx], q[idx], sl ? dslow_at<V>(o, idx) : V(1), fold ? x64[idx] : 0.0, xo, ro,
+           x64o);
+      x[idx] = xo;
+      r[idx] = ro;
+      if (fold) x64[idx] = x64o;
     }
   }
   double v[2] = {(double)a_rz, (double)a_rr};
@@ -933,7 +994,7 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
   const int R = c.cfg.refine_every > 0 ? c.cfg.refine_every : 20;
   int check = mixed ? R : c.cfg.check_every;
   if (check <= 0) check = 16;
-  launch(c, "pcg_nbm", dim3(nblocks(G, 256)), dim3(256), 0, k_nbm<OPk>, o, c.nbm, c.dslow);
+  launch(c, "pcg_nbm", dim3(nblocks(G, 256)), dim3(256), 0, k_nbm<OPk, V>, o, c.nbm, (V*)c.dslow);
   // slow rows: list in cell order (positions from a scan into the cell-count scratch)
   launch(c, "pcg_slow_flag", dim3(nblocks(G, 256)), dim3(256), 0, k_slow_flag, G, (const uint8_t*)c.nbm, c.ctmp_u32);
   const int64_t nslow = run_scan_u32(c, c.ctmp_u32, c.count, G, true);
