@@ -348,12 +348,13 @@ __global__ void __launch_bounds__(PB) k_init(OpArgs o, Launch L, const V* __rest
 }
 
 // ---- spmv tiling ----
-constexpr int TJ = 16, TK = 64;               // (j, k) column tile
+constexpr int TJ = 8, TK = 128;               // (j, k) column tile
 constexpr int ST = 512;                       // threads per spmv block
 constexpr int RPT = TJ * TK / ST;             // tile rows per thread (2)
 constexpr int LJ = TJ + 2, LK = TK + 2;       // LDS plane with a one-cell ring
 constexpr int NRING = 2 * TK + 2 * TJ;        // ring cells per plane (160)
 static_assert(NRING <= ST, "one ring cell per thread");
+static_assert((TK & (TK - 1)) == 0 && (ST / TK) * RPT == TJ, "threads cover the tile in rows of TK");
 
 struct Tiling {
   int ntj, ntk, seg, nblk;
@@ -404,8 +405,8 @@ __device__ __forceinline__ V slow_row(const OpArgs& o, int i, int j, int k, int6
 // Iteration k.  From update(k-1): rz_k, rr_k (slot (k-1)&1) and rz_{k-1}
 // (slot k&1) -> convergence, beta.  Then p_k = r*dinv + beta p_{k-1} for the
 // block's tile (plus its ring), stored; q = A p_k; partial p.q.
-template <class V, class Dacc, int OPk>
-__global__ void __launch_bounds__(ST, 4) k_spmv(OpArgs o, Tiling T, int nbu, const V* __restrict__ r,
+template <class V, class Dacc, int OPk, bool SLOW>
+__global__ void __launch_bounds__(ST, (sizeof(V) == 4 && !SLOW) ? 6 : 4) k_spmv(OpArgs o, Tiling T, int nbu, const V* __restrict__ r,
                                              const V* __restrict__ pp, V* __restrict__ pc, V* __restrict__ q,
                                              Parts P, SolveScalars* sc, int k, int ref, double eps_abs, double tol,
                                              int cap) {
@@ -449,7 +450,7 @@ __global__ void __launch_bounds__(ST, 4) k_spmv(OpArgs o, Tiling T, int nbu, con
   const int i_hi = min(i_lo + T.seg, g.nx);
   const int j0 = tj * TJ, k0 = tk * TK;
   const int tid = threadIdx.x;
-  const int kl = tid & (TK - 1), jl0 = (tid >> 6) * RPT;
+  const int kl = tid & (TK - 1), jl0 = (tid / TK) * RPT;
   const int kk = k0 + kl;
   // this thread's ring cell (local coords in [-1, TJ] x [-1, TK])
   int hj = 0, hk = 0;
@@ -495,7 +496,8 @@ __global__ void __launch_bounds__(ST, 4) k_spmv(OpArgs o, Tiling T, int nbu, con
     w.nb = nb;
     w.r = (r + pb)[cv];
     w.pp = (pp + pb)[cv];
-    w.ds = ((const V*)o.dslow + pb)[(un && (nb & NB_SLOW) && o.precond) ? c : 0u];
+    // (a solve without slow rows reads no diagonals)
+    w.ds = SLOW ? ((const V*)o.dslow + pb)[(un && (nb & NB_SLOW) && o.precond) ? c : 0u] : V(1);
   };
   auto form = [&](const Raw<V>& w) -> V {
     const V dt = dtab[__popc((unsigned)(w.nb & 63))];
@@ -565,8 +567,11 @@ __global__ void __launch_bounds__(ST, 4) k_spmv(OpArgs o, Tiling T, int nbu, con
         }
         if (ring) pl[sn][hj + 1][hk + 1] = form(HR[s1]);
         lds_barrier();
-        // q of plane i: the open faces in the order +x, +y, +z, -x, -y, -z
-        // (slow rows: p only, q from k_slow_q)
+        // q of plane i (slow rows: p only, q from k_slow_q).  A fast row's q
+        // is qc * sum over its open faces of (p_n - p_c).  A closed face's
+        // neighbour is never an unknown, so its p in LDS is 0 (cells outside
+        // the grid included) and the sum equals (sum of all six p_n) minus
+        // (open faces) * p_c.
         const int64_t pb = (int64_t)i * sx;
 #pragma unroll
         for (int u = 0; u < RPT; ++u) {
@@ -575,13 +580,8 @@ __global__ void __launch_bounds__(ST, 4) k_spmv(OpArgs o, Tiling T, int nbu, con
           const V pcv = PV[s0][u];
           const V nyp = pl[scur][jl + 2][kl + 1], nzp = pl[scur][jl + 1][kl + 2];
           const V nym = pl[scur][jl][kl + 1], nzm = pl[scur][jl + 1][kl];
-          V sum = V(0);
-          sum = (nb & 1) ? sum + (PV[s1][u] - pcv) : sum;
-          sum = (nb & 2) ? sum + (nyp - pcv) : sum;
-          sum = (nb & 4) ? sum + (nzp - pcv) : sum;
-          sum = (nb & 8) ? sum + (PV[s2][u] - pcv) : sum;
-          sum = (nb & 16) ? sum + (nym - pcv) : sum;
-          sum = (nb & 32) ? sum + (nzm - pcv) : sum;
+          const V sn6 = ((PV[s1][u] + nyp) + nzp) + ((PV[s2][u] + nym) + nzm);
+          const V sum = sn6 - (V)__popc((unsigned)(nb & 63)) * pcv;
           const bool fast = (nb & (NB_UNKNOWN | NB_SLOW)) == NB_UNKNOWN;
           const V qv = fast ? sum * qc : V(0);
           a_pq += (Dacc)pcv * (Dacc)qv;
@@ -919,15 +919,15 @@ Launch make_launch(int64_t G) {
 }
 
 // spmv tiles: 16 x 64 (j, k) columns, planes split along i into runs so the
-// launch has about SPMV_BLOCKS blocks (two 512-thread blocks fit a CU at the
-// kernel's register use: one resident round on 256 CUs).
-constexpr int SPMV_BLOCKS = 512;
-Tiling make_tiling(const Grid& g) {
+// launch has about `blocks` blocks: one resident round on 256 CUs, i.e. as
+// many 512-thread blocks as fit a CU at the kernel's register use (three for
+// FP32 vectors without slow rows, two otherwise).
+Tiling make_tiling(const Grid& g, int blocks) {
   Tiling t;
   t.ntj = (g.ny + TJ - 1) / TJ;
   t.ntk = (g.nz + TK - 1) / TK;
   const int tiles = t.ntj * t.ntk;
-  int nseg = SPMV_BLOCKS / tiles;
+  int nseg = blocks / tiles;
   if (nseg < 1) nseg = 1;
   if (nseg > g.nx) nseg = g.nx;
   t.seg = (g.nx + nseg - 1) / nseg;
@@ -980,12 +980,7 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
              mpf_solve_info* info) {
   const int64_t G = c.g.G;
   const Launch L = make_launch(G);
-  const Tiling T = make_tiling(c.g);
   const OpArgs o = make_op(c, precond, OPk);
-  if (T.nblk > 2 * MAX_BLK) {
-    c.err = "grid too large for the solver's reduction buffers";
-    return MPF_INVALID_ARG;
-  }
   V* r = (V*)c.vr;
   V* pbuf[2] = {(V*)c.vp, (V*)c.vp2};
   V* q = (V*)c.vq;
@@ -1004,6 +999,12 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
     launch(c, "pcg_slow_list", dim3(nblocks(G, 256)), dim3(256), 0, k_slow_list, G, (const uint8_t*)c.nbm,
            (const uint32_t*)c.count, slow_list);
   int nsb = (int)((nslow + PB - 1) / PB);
+  auto spmv = nslow > 0 ? k_spmv<V, Dacc, OPk, true> : k_spmv<V, Dacc, OPk, false>;
+  const Tiling T = make_tiling(c.g, (sizeof(V) == 4 && nslow == 0) ? 768 : 512);
+  if (T.nblk > 2 * MAX_BLK) {
+    c.err = "grid too large for the solver's reduction buffers";
+    return MPF_INVALID_ARG;
+  }
   if (nsb > MAX_BLK) nsb = MAX_BLK;
   launch(c, "pcg_init", dim3(L.nblk), dim3(PB), 0, k_init<V, Dacc, OPk>, o, L, b, x, r, pbuf[0], pbuf[1],
          mixed ? x64 : (double*)nullptr, P, c.scal);
@@ -1012,8 +1013,8 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
   while (k <= cap) {
     const int k_end = k + check;
     for (; k < k_end && k <= cap; ++k) {
-      launch(c, "pcg_spmv", dim3(T.nblk), dim3(ST), 0, k_spmv<V, Dacc, OPk>, o, T, L.nblk, (const V*)r,
-             (const V*)pbuf[k & 1], pbuf[(k + 1) & 1], q, P, c.scal, k, (int)ref, eps_abs, tol, cap);
+      launch(c, "pcg_spmv", dim3(T.nblk), dim3(ST), 0, spmv, o, T, L.nblk, (const V*)r, (const V*)pbuf[k & 1],
+             pbuf[(k + 1) & 1], q, P, c.scal, k, (int)ref, eps_abs, tol, cap);
       if (nsb > 0)
         launch(c, "pcg_slow_q", dim3(nsb), dim3(PB), 0, k_slow_q<V, Dacc, OPk>, o, (const uint32_t*)slow_list, nslow,
                (const V*)pbuf[(k + 1) & 1], q, P.S2, c.scal);
