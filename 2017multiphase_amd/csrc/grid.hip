@@ -18,6 +18,16 @@ __device__ __forceinline__ f3 mk3(float x, float y, float z) { return f3{x, y, z
 __device__ __forceinline__ f3 sub3(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
 __device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
+// a / d for the sharp kernel of the tile scatters: the hardware reciprocal
+// and one Newton-Raphson correction of the quotient (Markstein's step,
+// residual by fma), within an ulp of the IEEE quotient for d >= 1e-4.
+__device__ __forceinline__ float div_nr(float a, float d) {
+  const float y = __builtin_amdgcn_rcpf(d);
+  const float q = a * y;
+  const float r = fmaf(-d, q, a);
+  return fmaf(r, y, q);
+}
+
 #define CELL_LOOP_HEAD                                           \
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; \
   if (idx >= g.G) return;                                        \
@@ -343,7 +353,7 @@ __global__ void __launch_bounds__(ST_CELLS)
           for (int q = 0; q < SCH; ++q) {
             const float dx = X[q] - gx, dy = Y[q] - gy, dz = Z[q] - gz;
             const float r2s = dx * dx + dy * dy + dz * dz;
-            const float w = M[q] * fmaxf(RE2 / fmaxf(r2s, 0.0001f) - 1.0f, 0.0f);
+            const float w = M[q] * fmaxf(div_nr(RE2, fmaxf(r2s, 0.0001f)) - 1.0f, 0.0f);
             W += w;
             U += w * (a == 0 ? VX[q] : (a == 1 ? VY[q] : VZ[q]));
           }
@@ -354,7 +364,7 @@ __global__ void __launch_bounds__(ST_CELLS)
             const float dx = (x.x - g.gx0) * scale - gx, dy = (x.y - g.gy0) * scale - gy,
                         dz = (x.z - g.gz0) * scale - gz;
             const float r2s = dx * dx + dy * dy + dz * dz;
-            const float w = mass[p] * fmaxf(RE2 / fmaxf(r2s, 0.0001f) - 1.0f, 0.0f);
+            const float w = mass[p] * fmaxf(div_nr(RE2, fmaxf(r2s, 0.0001f)) - 1.0f, 0.0f);
             W += w;
             const float* vp = &vel[p].x;
             U += w * vp[a];
@@ -435,7 +445,6 @@ __global__ void k_sweep_u(Grid g, const float* __restrict__ u, float* __restrict
                           const int8_t* __restrict__ mark, int8_t sf, int64_t nf) {
   int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
-  const int hh = g.ny + (A == 1), d = g.nz + (A == 2);
   int i, j, k;
   ijk_face<A>(g, f, i, j, k);
   float res = u[f];
@@ -448,29 +457,44 @@ __global__ void k_sweep_u(Grid g, const float* __restrict__ u, float* __restrict
       constexpr int t1 = (A == 0) ? 1 : (A == 1) ? 0 : 1;
       constexpr int t2 = (A == 0) ? 2 : (A == 1) ? 2 : 0;
       const int lims[3] = {g.nx, g.ny, g.nz};
-      for (int di = -1; di <= 1; di += 2)
-        for (int dj = -1; dj <= 1; dj += 2)
-          for (int dk = -1; dk <= 1; dk += 2) {
-            const int dd[3] = {di, dj, dk};
-            if (n[t1] + dd[t1] < 0 || n[t1] + dd[t1] > lims[t1] - 1) continue;
-            if (n[t2] + dd[t2] < 0 || n[t2] + dd[t2] > lims[t2] - 1) continue;
-            float wv[3];
-            wv[A] = (float)(-dd[A]) * (lsc - lsb);
-            if (wv[A] < 0) continue;
-            int o1[3] = {0, 0, 0};
-            o1[t1] = dd[t1];
-            wv[t1] = (lsc + lsb - ls[cid(g, i + o1[0], j + o1[1], k + o1[2])] - ls[cid(g, bi + o1[0], bj + o1[1], bk + o1[2])]) * 0.5f;
-            if (wv[t1] < 0) continue;
-            int o2[3] = {0, 0, 0};
-            o2[t2] = dd[t2];
-            wv[t2] = (lsc + lsb - ls[cid(g, i + o2[0], j + o2[1], k + o2[2])] - ls[cid(g, bi + o2[0], bj + o2[1], bk + o2[2])]) * 0.5f;
-            if (wv[t2] < 0) continue;
-            float wx = wv[0], wy = wv[1], wz = wv[2];
-            float wsum = wx + wy + wz;
-            if (wsum == 0) wx = wy = wz = 1.0f / 3;
-            else { wx /= wsum; wy /= wsum; wz /= wsum; }
-            res = wx * u[fid(g, A, i + di, j, k)] + wy * u[fid(g, A, i, j + dj, k)] + wz * u[fid(g, A, i, j, k + dk)];
-          }
+      // The reference visits the 8 octants (di, dj, dk in {-1, +1}, di
+      // outermost) and the last one whose three weights are all >= 0 sets
+      // the value.  Each weight depends on its own axis' sign only, so that
+      // octant takes, per axis, +1 if its weight passes there, else -1.
+      const float dl = lsc - lsb;
+      const float wa[2] = {(float)(1) * dl, (float)(-1) * dl};  // sign -1, +1
+      float wt[2][2];
+      bool ok[3][2];
+      ok[A][0] = !(wa[0] < 0);
+      ok[A][1] = !(wa[1] < 0);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int ax = t == 0 ? t1 : t2;
+#pragma unroll
+        for (int sg = 0; sg < 2; ++sg) {
+          const int dd = sg ? 1 : -1;
+          int o[3] = {0, 0, 0};
+          o[ax] = dd;
+          const bool in = n[ax] + dd >= 0 && n[ax] + dd <= lims[ax] - 1;
+          float w = 0.0f;
+          if (in) w = (lsc + lsb - ls[cid(g, i + o[0], j + o[1], k + o[2])] - ls[cid(g, bi + o[0], bj + o[1], bk + o[2])]) * 0.5f;
+          wt[t][sg] = w;
+          ok[ax][sg] = in && !(w < 0);
+        }
+      }
+      if ((ok[0][0] || ok[0][1]) && (ok[1][0] || ok[1][1]) && (ok[2][0] || ok[2][1])) {
+        const int sgn[3] = {ok[0][1] ? 1 : 0, ok[1][1] ? 1 : 0, ok[2][1] ? 1 : 0};
+        float wv[3];
+        wv[A] = wa[sgn[A]];
+        wv[t1] = wt[0][sgn[t1]];
+        wv[t2] = wt[1][sgn[t2]];
+        const int di = sgn[0] ? 1 : -1, dj = sgn[1] ? 1 : -1, dk = sgn[2] ? 1 : -1;
+        float wx = wv[0], wy = wv[1], wz = wv[2];
+        float wsum = wx + wy + wz;
+        if (wsum == 0) wx = wy = wz = 1.0f / 3;
+        else { wx /= wsum; wy /= wsum; wz /= wsum; }
+        res = wx * u[fid(g, A, i + di, j, k)] + wy * u[fid(g, A, i, j + dj, k)] + wz * u[fid(g, A, i, j, k + dk)];
+      }
     }
   }
   out[f] = res;
@@ -635,7 +659,7 @@ __global__ void __launch_bounds__(ST_CELLS)
 #pragma unroll
         for (int q = 0; q < SCH; ++q) {
           const float dx = X[q] - gx, dy = Y[q] - gy, dz = Z[q] - gz;
-          const float w = ON[q] * fmaxf(RE2 / fmaxf(dx * dx + dy * dy + dz * dz, 0.0001f) - 1.0f, 0.0f);
+          const float w = ON[q] * fmaxf(div_nr(RE2, fmaxf(dx * dx + dy * dy + dz * dz, 0.0001f)) - 1.0f, 0.0f);
           W += w;
           Hs += w * TT[q];
         }
@@ -643,7 +667,7 @@ __global__ void __launch_bounds__(ST_CELLS)
           const f3 x = pos[p];
           const float dx = (x.x - g.gx0) * scale - gx, dy = (x.y - g.gy0) * scale - gy,
                       dz = (x.z - g.gz0) * scale - gz;
-          const float w = fmaxf(RE2 / fmaxf(dx * dx + dy * dy + dz * dz, 0.0001f) - 1.0f, 0.0f);
+          const float w = fmaxf(div_nr(RE2, fmaxf(dx * dx + dy * dy + dz * dz, 0.0001f)) - 1.0f, 0.0f);
           W += w;
           Hs += w * temp[p];
         }

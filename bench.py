@@ -144,7 +144,10 @@ def roofline(res, precision, form):
           "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": measured_traffic(name, form),
           "avg_launch_us": avg_ms * 1e3, "launches_per_step": cnt, "bytes_per_launch": bytes_per_launch,
           "cells": G, "unknown_cells": U, "particles": P}
-    top = sorted(((k, v[1]) for k, v in kt.items()), key=lambda kv: -kv[1])[:8]
+    ranked = sorted(((k, v[0], v[1]) for k, v in kt.items()), key=lambda kv: -kv[2])
+    log("[bench]   kernel times of one step (name, launches, ms): " +
+        ", ".join("%s %d %.3f" % r for r in ranked))
+    top = [(k, ms) for k, _, ms in ranked[:8]]
     return rf, top
 
 
