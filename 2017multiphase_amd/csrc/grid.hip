@@ -47,10 +47,43 @@ __constant__ signed char c_ring_off[117][3];
 __constant__ unsigned char c_ring_start[10];  // ring r = offsets [start[r], start[r+1])
 __constant__ float c_ring_lb[9];              // lower bound of ring r, in h^2 units
 
+// Phases present in each 4x4x4 brick of cells (OR of the occupancy masks),
+// then, per brick, in the 3x3x3 bricks around it: a superset of the phases
+// in the 5x5x5 window of any cell of the brick.
+__global__ void k_brick_or(Grid g, const uint8_t* __restrict__ cmask, uint8_t* __restrict__ bm, int bx, int by,
+                           int bz) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= (int64_t)bx * by * bz) return;
+  const int bk = (int)(b % bz), bj = (int)((b / bz) % by), bi = (int)(b / ((int64_t)by * bz));
+  uint8_t m = 0;
+  for (int di = 0; di < 4; ++di)
+    for (int dj = 0; dj < 4; ++dj)
+      for (int dk = 0; dk < 4; ++dk) {
+        const int ci = 4 * bi + di, cj = 4 * bj + dj, ck = 4 * bk + dk;
+        if (valid(g, ci, cj, ck)) m |= cmask[cid(g, ci, cj, ck)];
+      }
+  bm[b] = m;
+}
+__global__ void k_brick_dilate(const uint8_t* __restrict__ bm, uint8_t* __restrict__ dm, int bx, int by, int bz) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= (int64_t)bx * by * bz) return;
+  const int bk = (int)(b % bz), bj = (int)((b / bz) % by), bi = (int)(b / ((int64_t)by * bz));
+  uint8_t m = 0;
+  for (int di = -1; di <= 1; ++di)
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int dk = -1; dk <= 1; ++dk) {
+        const int i = bi + di, j = bj + dj, k = bk + dk;
+        if (i >= 0 && i < bx && j >= 0 && j < by && k >= 0 && k < bz) m |= bm[((int64_t)i * by + j) * bz + k];
+      }
+  dm[b] = m;
+}
+
+// A phase absent from the cell's brick neighbourhood is absent from its
+// window: its search is skipped, with the same result (the 2.5h cap).
 __global__ void k_genlevelset(Grid g, const f3* __restrict__ pos, const int8_t* __restrict__ flag,
                               const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
-                              const uint8_t* __restrict__ cmask, float* __restrict__ lsfluid,
-                              float* __restrict__ lsair, int8_t* __restrict__ mark) {
+                              const uint8_t* __restrict__ cmask, const uint8_t* __restrict__ dm,
+                              float* __restrict__ lsfluid, float* __restrict__ lsair, int8_t* __restrict__ mark) {
   CELL_LOOP_HEAD
   const float h = g.h;
   mark[idx] = VACUUM;
@@ -58,9 +91,12 @@ __global__ void k_genlevelset(Grid g, const f3* __restrict__ pos, const int8_t* 
   f3 gp = mk3(((float)i + 0.5f) * h + g.gx0, ((float)j + 0.5f) * h + g.gy0, ((float)k + 0.5f) * h + g.gz0);
   float ma2 = INFINITY, mf2 = INFINITY;
   const float h2 = h * h;
+  const int by = (g.ny + 3) / 4, bz = (g.nz + 3) / 4;
+  const uint8_t near = dm[((int64_t)(i >> 2) * by + (j >> 2)) * bz + (k >> 2)];
+  const bool has_f = (near & M_FLUIDSOLID) != 0, has_a = (near & (M_AIR | M_AIRSOLO)) != 0;
   for (int ring = 0; ring < 9; ++ring) {
     const float lb = 0.999f * c_ring_lb[ring] * h2;
-    const bool need_f = mf2 > lb, need_a = ma2 > lb;
+    const bool need_f = has_f && mf2 > lb, need_a = has_a && ma2 > lb;
     if (!need_f && !need_a) break;
     const uint8_t want = (need_f ? M_FLUIDSOLID : 0) | (need_a ? (M_AIR | M_AIRSOLO) : 0);
     for (int q = c_ring_start[ring]; q < c_ring_start[ring + 1]; ++q) {
@@ -218,17 +254,26 @@ __global__ void k_mark_solid(Grid g, const uint32_t* __restrict__ cstart, const 
 template <int A, bool AIRPH>
 __global__ void k_p2g(Grid g, const f3* __restrict__ pos, const f3* __restrict__ vel, const float* __restrict__ mass,
                       const int8_t* __restrict__ flag, const uint32_t* __restrict__ cstart,
-                      const uint32_t* __restrict__ cend, const uint8_t* __restrict__ cmask, float* __restrict__ out,
-                      int64_t nf) {
+                      const uint32_t* __restrict__ cend, const uint8_t* __restrict__ cmask,
+                      const uint8_t* __restrict__ dm, float* __restrict__ out, int64_t nf) {
   int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
-  const int hh = g.ny + (A == 1), d = g.nz + (A == 2);
   int i, j, k;
   ijk_face<A>(g, f, i, j, k);
+  constexpr uint8_t need = AIRPH ? M_AIR : M_FLUIDSOLID;
+  {
+    // the window (offsets -2..2 from cell (i,j,k)) lies in the 3x3x3 bricks
+    // around that cell's brick: no particle of the phase there, no weight
+    const int by = (g.ny + 3) / 4, bz = (g.nz + 3) / 4;
+    const int64_t b = ((int64_t)(min(i, g.nx - 1) >> 2) * by + (min(j, g.ny - 1) >> 2)) * bz + (min(k, g.nz - 1) >> 2);
+    if (!(dm[b] & need)) {
+      out[f] = 0.0f;
+      return;
+    }
+  }
   const float scale = 1.0f / g.h;
   f3 gp = mk3((float)i + (A == 0 ? 0.0f : 0.5f), (float)j + (A == 1 ? 0.0f : 0.5f), (float)k + (A == 2 ? 0.0f : 0.5f));
   constexpr int lo = AIRPH ? -2 : -1, hin = AIRPH ? 1 : 0, lot = AIRPH ? -2 : -1, hit = AIRPH ? 2 : 1;
-  constexpr uint8_t need = AIRPH ? M_AIR : M_FLUIDSOLID;
   float weight = 0, usum = 0;
   const float RE = 1.4f;
   const float RE2 = RE * RE;
@@ -926,13 +971,24 @@ static void upload_ring_tables() {
   done = true;
 }
 
+// Brick summary of the occupancy masks (after every change of the masks).
+void run_bricks(Ctx& c) {
+  const int bx = (c.g.nx + 3) / 4, by = (c.g.ny + 3) / 4, bz = (c.g.nz + 3) / 4;
+  const int64_t NB = nbricks(c.g);
+  launch(c, "bricks", dim3(nblocks(NB, 256)), dim3(256), 0, k_brick_or, c.g, (const uint8_t*)c.cmask, c.bmask, bx, by,
+         bz);
+  launch(c, "bricks", dim3(nblocks(NB, 256)), dim3(256), 0, k_brick_dilate, (const uint8_t*)c.bmask, c.bmask + NB, bx,
+         by, bz);
+}
+
 void run_levelset(Ctx& c) {
   upload_ring_tables();
   const int64_t G = c.g.G;
   const int cur = c.cur;
+  const int64_t NB = nbricks(c.g);
   launch(c, "genlevelset", dim3(nblocks(G, 256)), dim3(256), 0, k_genlevelset, c.g, (const f3*)c.pos[cur],
          (const int8_t*)c.flag[cur], (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask,
-         c.lsfluid, c.lsair, c.mark);
+         (const uint8_t*)(c.bmask + NB), c.lsfluid, c.lsair, c.mark);
   redistance(c, c.lsfluid);
   redistance(c, c.lsair);
   launch(c, "merge_mark", dim3(nblocks(G, 256)), dim3(256), 0, k_merge, c.g, (const float*)c.lsfluid,
@@ -957,11 +1013,14 @@ static void p2g_phase(Ctx& c, float* const* out) {
   const int8_t* flag = c.flag[cur];
   const char* nm = AIRPH ? "p2g_air" : "p2g_liquid";
   launch(c, nm, dim3(nblocks(fc[0], 256)), dim3(256), 0, k_p2g<0, AIRPH>, c.g, pos, vel, mass, flag,
-         (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask, out[0], fc[0]);
+         (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask,
+         (const uint8_t*)(c.bmask + nbricks(c.g)), out[0], fc[0]);
   launch(c, nm, dim3(nblocks(fc[1], 256)), dim3(256), 0, k_p2g<1, AIRPH>, c.g, pos, vel, mass, flag,
-         (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask, out[1], fc[1]);
+         (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask,
+         (const uint8_t*)(c.bmask + nbricks(c.g)), out[1], fc[1]);
   launch(c, nm, dim3(nblocks(fc[2], 256)), dim3(256), 0, k_p2g<2, AIRPH>, c.g, pos, vel, mass, flag,
-         (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask, out[2], fc[2]);
+         (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask,
+         (const uint8_t*)(c.bmask + nbricks(c.g)), out[2], fc[2]);
 }
 
 static void p2g_liquid(Ctx& c, float* const* out, const float* band = nullptr, float big = 0.0f) {

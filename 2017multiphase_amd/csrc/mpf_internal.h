@@ -154,6 +154,7 @@ struct Ctx {
   uint32_t* cstart = nullptr;
   uint32_t* cend = nullptr;
   uint8_t* cmask = nullptr;
+  uint8_t* bmask = nullptr;  // per 4x4x4 brick: OR of cmask, then the same over the 3x3x3 bricks around it
   int8_t* mark = nullptr;
   float *lsfluid = nullptr, *lsair = nullptr, *lsmerge = nullptr;
   float *phig[3] = {};
@@ -236,6 +237,7 @@ inline unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); 
 // sort.hip
 void run_hash_sort(Ctx& c);
 void run_cell_mask(Ctx& c);
+void run_bricks(Ctx& c);
 int64_t run_check_cells(Ctx& c);  // cells whose list does not index the particles; -1 on a HIP error
 int64_t run_scan_u32(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n, bool want_total);
 // particles.hip
@@ -256,6 +258,10 @@ void run_heat_jacobi(Ctx& c, int iters);
 int run_marching_cubes(Ctx& c, mpf_mc_info* info);
 void run_solid_phi_and_velocity(Ctx& c);
 int64_t p2g_parts_floats(const Grid& g);
+// 4x4x4 cell bricks of the level-set search summary
+inline int64_t nbricks(const Grid& g) {
+  return (int64_t)((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4);
+}
 // pcg.hip
 int run_pressure_solve(Ctx& c, mpf_solve_info* info);
 int run_heat_cg(Ctx& c, mpf_solve_info* info);

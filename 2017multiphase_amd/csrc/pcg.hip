@@ -292,6 +292,17 @@ __device__ __forceinline__ void read_partials(const double* __restrict__ P, int 
   block_sum_all<N, NT>(out);
 }
 
+// Zeros for the (r.z, r.r) partials of blocks [nblk, npad) of a U slot, so
+// that kernels launched with fewer blocks than npad leave npad summands.
+template <int NT = PB>
+__device__ __forceinline__ void pad_partials2(double* __restrict__ P, int nblk, int npad) {
+  if (blockIdx.x == 0)
+    for (int q = nblk + threadIdx.x; q < npad; q += NT) {
+      P[2 * q] = 0.0;
+      P[2 * q + 1] = 0.0;
+    }
+}
+
 __device__ __forceinline__ int64_t chunk_begin(const Launch& L) {
   const int nb = L.nblk;
   const int b = blockIdx.x;
@@ -321,7 +332,7 @@ __device__ __forceinline__ double* uslot(const Parts& P, int k) { return (k & 1)
 template <class V, class Dacc, int OPk>
 __global__ void __launch_bounds__(PB) k_init(OpArgs o, Launch L, const V* __restrict__ b, V* __restrict__ x,
                                              V* __restrict__ r, V* __restrict__ pa, V* __restrict__ pb,
-                                             double* __restrict__ x64, Parts P, SolveScalars* sc) {
+                                             double* __restrict__ x64, Parts P, SolveScalars* sc, int npad) {
   Dacc a_rz = 0, a_rr = 0;
   CELLS_OF_BLOCK(o.g.G) {
     const uint8_t nb = o.nbm[idx];
@@ -337,6 +348,7 @@ __global__ void __launch_bounds__(PB) k_init(OpArgs o, Launch L, const V* __rest
   }
   double v[2] = {(double)a_rz, (double)a_rr};
   write_partials<2>(v, uslot(P, 1));
+  pad_partials2(uslot(P, 1), L.nblk, npad);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc->done = 0;
     sc->converged = 0;
@@ -567,11 +579,9 @@ __global__ void __launch_bounds__(ST, (sizeof(V) == 4 && !SLOW) ? 6 : 4) k_spmv(
         }
         if (ring) pl[sn][hj + 1][hk + 1] = form(HR[s1]);
         lds_barrier();
-        // q of plane i (slow rows: p only, q from k_slow_q).  A fast row's q
-        // is qc * sum over its open faces of (p_n - p_c).  A closed face's
-        // neighbour is never an unknown, so its p in LDS is 0 (cells outside
-        // the grid included) and the sum equals (sum of all six p_n) minus
-        // (open faces) * p_c.
+        // q of plane i: qc times the sum over the open faces, in the order
+        // +x, +y, +z, -x, -y, -z, of (p_n - p_c) (each difference exact
+        // where p is smooth); slow rows: p only, q from k_slow_q
         const int64_t pb = (int64_t)i * sx;
 #pragma unroll
         for (int u = 0; u < RPT; ++u) {
@@ -580,8 +590,13 @@ __global__ void __launch_bounds__(ST, (sizeof(V) == 4 && !SLOW) ? 6 : 4) k_spmv(
           const V pcv = PV[s0][u];
           const V nyp = pl[scur][jl + 2][kl + 1], nzp = pl[scur][jl + 1][kl + 2];
           const V nym = pl[scur][jl][kl + 1], nzm = pl[scur][jl + 1][kl];
-          const V sn6 = ((PV[s1][u] + nyp) + nzp) + ((PV[s2][u] + nym) + nzm);
-          const V sum = sn6 - (V)__popc((unsigned)(nb & 63)) * pcv;
+          V sum = V(0);
+          sum = (nb & 1) ? sum + (PV[s1][u] - pcv) : sum;
+          sum = (nb & 2) ? sum + (nyp - pcv) : sum;
+          sum = (nb & 4) ? sum + (nzp - pcv) : sum;
+          sum = (nb & 8) ? sum + (PV[s2][u] - pcv) : sum;
+          sum = (nb & 16) ? sum + (nym - pcv) : sum;
+          sum = (nb & 32) ? sum + (nzm - pcv) : sum;
           const bool fast = (nb & (NB_UNKNOWN | NB_SLOW)) == NB_UNKNOWN;
           const V qv = fast ? sum * qc : V(0);
           a_pq += (Dacc)pcv * (Dacc)qv;
@@ -642,7 +657,7 @@ template <class V, class Dacc, int OPk>
 __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict__ x, V* __restrict__ r,
                                                const V* __restrict__ pc, const V* __restrict__ q,
                                                double* __restrict__ x64, Parts P, int nbs, int nbs2, SolveScalars* sc,
-                                               int k, int fold) {
+                                               int k, int fold, int npad) {
   if (sc->done) return;
   double t[1], t2[1];
   read_partials<1>(P.S, nbs, t);
@@ -775,87 +790,184 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
   }
   double v[2] = {(double)a_rz, (double)a_rr};
   write_partials<2>(v, uslot(P, k));
+  pad_partials2(uslot(P, k), L.nblk, npad);
   if (blockIdx.x == 0 && threadIdx.x == 0) sc->iter = k + 1;
 }
 
 // MIXED reliable update after update(k): r = b - A (x64 + x) in FP64, stored
 // to the FP32 residual; (r.z, |r|^2) overwrite U slot k&1, so spmv(k+1)
-// takes beta and the convergence test from the true residual.
-template <int OPk>
-__global__ void __launch_bounds__(PB) k_resid64(OpArgs o, Launch L, const float* __restrict__ b,
+// takes beta and the convergence test from the true residual.  The spmv's
+// tiles and plane pipeline: x64 + x of each plane goes to LDS, fast rows sum
+// their open faces from there, slow rows (flags and level set) also take
+// their neighbours from LDS (0 off the unknowns).  Partials of npad blocks
+// (zeros past the launch), the count the spmv reads.
+template <int OPk, bool SLOW>
+__global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_resid64(OpArgs o, Tiling T, const float* __restrict__ b,
                                                 const double* __restrict__ x64, const float* __restrict__ x,
-                                                float* __restrict__ r, Parts P, SolveScalars* sc, int k) {
+                                                float* __restrict__ r, Parts P, SolveScalars* sc, int k, int npad) {
   if (sc->done) return;
+  __shared__ float dtab[8];
+  if (threadIdx.x < 8) dtab[threadIdx.x] = (threadIdx.x < 7) ? dinv_tab<float>(o, threadIdx.x) : 1.0f;
+  __shared__ double pl[3][LJ][LK];
   const Grid& g = o.g;
-  int64_t st[6];
-  nb_strides(g, st);
-  double a_rz = 0, a_rr = 0;
-  // UN cells per thread per pass: masks first, then every fast row's seven
-  // (x64, x) pairs for all UN cells, so the latencies overlap
-  constexpr int UN = 2;
-  const int64_t c0 = chunk_begin(L);
-  const int64_t c1 = (g.G < c0 + L.chunk) ? g.G : c0 + L.chunk;
+  int lb = blockIdx.x;
+  if ((T.nblk & 7) == 0) lb = (lb & 7) * (T.nblk >> 3) + (lb >> 3);  // XCD-contiguous tiles
+  const int tk = lb % T.ntk;
+  const int rest = lb / T.ntk;
+  const int tj = rest % T.ntj;
+  const int i_lo = (rest / T.ntj) * T.seg;
+  const int i_hi = min(i_lo + T.seg, g.nx);
+  const int j0 = tj * TJ, k0 = tk * TK;
+  const int tid = threadIdx.x;
+  const int kl = tid & (TK - 1), jl0 = (tid / TK) * RPT;
+  const int kk = k0 + kl;
+  int hj = 0, hk = 0;
+  const bool ring = tid < NRING;
+  if (tid < 2 * TK) {
+    hj = (tid < TK) ? -1 : TJ;
+    hk = tid & (TK - 1);
+  } else if (ring) {
+    const int t2 = tid - 2 * TK;
+    hj = t2 % TJ;
+    hk = (t2 < TJ) ? -1 : TK;
+  }
+  const int64_t sx = (int64_t)g.ny * g.nz;
+  uint32_t col[RPT];
+  bool cok[RPT];
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    const int j = j0 + jl0 + u;
+    cok[u] = j < g.ny && kk < g.nz;
+    col[u] = cok[u] ? (uint32_t)(j * g.nz + kk) : 0u;
+  }
+  const bool hok = ring && (j0 + hj) >= 0 && (j0 + hj) < g.ny && (k0 + hk) >= 0 && (k0 + hk) < g.nz;
+  const uint32_t hcol = hok ? (uint32_t)((j0 + hj) * g.nz + (k0 + hk)) : 0u;
+  // loads as in the spmv: uniform plane base, 32-bit column, every lane
+  // issues every load (element 0 of the plane when there is nothing to read)
+  auto plane = [&](int i) -> int64_t { return (int64_t)min(max(i, 0), g.nx - 1) * sx; };
+  auto load_m = [&](int i, bool ok, uint32_t c) -> uint8_t {
+    const uint8_t* mp = o.nbm + plane(i);
+    return mp[(ok && i >= 0 && i < g.nx) ? c : 0u];
+  };
+  struct Rw {
+    double x64;
+    float x, b;
+    uint8_t nb;
+  };
+  auto load_v = [&](int i, bool ok, uint8_t raw, uint32_t c, Rw& w) {
+    const uint8_t nb = (ok && i >= 0 && i < g.nx) ? raw : (uint8_t)0;
+    const int64_t pb = plane(i);
+    const uint32_t cv = (nb & NB_UNKNOWN) ? c : 0u;
+    w.nb = nb;
+    w.x64 = (x64 + pb)[cv];
+    w.x = (x + pb)[cv];
+    w.b = (b + pb)[cv];
+  };
+  auto form = [&](const Rw& w) -> double { return (w.nb & NB_UNKNOWN) ? w.x64 + (double)w.x : 0.0; };
+  constexpr int UNR = 6;
+  Rw R[3][RPT], HR[3];
+  uint8_t M[3][RPT], HM[3];
+  double XV[3][RPT];
+  float BV[3][RPT];
+  uint8_t NB[3][RPT];
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    Rw w0, w1;
+    load_v(i_lo - 1, cok[u], load_m(i_lo - 1, cok[u], col[u]), col[u], w0);
+    load_v(i_lo, cok[u], load_m(i_lo, cok[u], col[u]), col[u], w1);
+    XV[2][u] = form(w0);
+    XV[0][u] = form(w1);
+    NB[0][u] = w1.nb;
+    BV[0][u] = w1.b;
+    pl[i_lo % 3][jl0 + u + 1][kl + 1] = XV[0][u];
+  }
+  {
+    Rw w;
+    load_v(i_lo, hok, load_m(i_lo, hok, hcol), hcol, w);
+    if (ring) pl[i_lo % 3][hj + 1][hk + 1] = form(w);
+  }
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    load_v(i_lo + 1, cok[u], load_m(i_lo + 1, cok[u], col[u]), col[u], R[1][u]);
+    load_v(i_lo + 2, cok[u], load_m(i_lo + 2, cok[u], col[u]), col[u], R[2][u]);
+    M[0][u] = load_m(i_lo + 3, cok[u], col[u]);
+  }
+  load_v(i_lo + 1, hok, load_m(i_lo + 1, hok, hcol), hcol, HR[1]);
+  load_v(i_lo + 2, hok, load_m(i_lo + 2, hok, hcol), hcol, HR[2]);
+  HM[0] = load_m(i_lo + 3, hok, hcol);
   const double bc = (OPk == OP_UNIT) ? 1.0 : o.il;
   const double ih2 = 1.0 / ((double)g.h * (double)g.h);
-  for (int64_t base = c0 + threadIdx.x; base < c1; base += (int64_t)UN * PB) {
-    uint8_t nb[UN];
+  double a_rz = 0, a_rr = 0;
+  for (int ib = i_lo; ib < i_hi; ib += UNR) {
 #pragma unroll
-    for (int u = 0; u < UN; ++u) {
-      const int64_t idx = base + (int64_t)u * PB;
-      nb[u] = (idx < c1) ? o.nbm[idx] : (uint8_t)0;
-    }
-    double xs[UN][7];
-    float bv[UN];
+    for (int d = 0; d < UNR; ++d) {
+      const int i = ib + d;
+      if (i < i_hi) {  // uniform over the block
+        const int s0 = d % 3, s1 = (d + 1) % 3, s2 = (d + 2) % 3;
+        const int sn = (i + 1) % 3, scur = i % 3;
 #pragma unroll
-    for (int u = 0; u < UN; ++u) {
-      const int64_t idx = base + (int64_t)u * PB;
-      const bool fast = (nb[u] & (NB_UNKNOWN | NB_SLOW)) == NB_UNKNOWN;
+        for (int u = 0; u < RPT; ++u) M[s1][u] = load_m(i + 4, cok[u], col[u]);
+        HM[s1] = load_m(i + 4, hok, hcol);
 #pragma unroll
-      for (int f = 0; f < 7; ++f) {
-        const int64_t n = (f < 6) ? idx + st[f] : idx;
-        const bool use = fast && (f == 6 || (nb[u] & (1u << f)));
-        xs[u][f] = use ? x64[n] + (double)x[n] : 0.0;
-      }
-      bv[u] = (nb[u] & NB_UNKNOWN) ? b[idx] : 0.0f;
-    }
+        for (int u = 0; u < RPT; ++u) load_v(i + 3, cok[u], M[s0][u], col[u], R[s0][u]);
+        load_v(i + 3, hok, HM[s0], hcol, HR[s0]);
 #pragma unroll
-    for (int u = 0; u < UN; ++u) {
-      if (!(nb[u] & NB_UNKNOWN)) continue;
-      const int64_t idx = base + (int64_t)u * PB;
-      double ax;
-      if (!(nb[u] & NB_SLOW)) {
-        const double xc = xs[u][6];
-        double sacc = 0.0;
-#pragma unroll
-        for (int f = 0; f < 6; ++f)
-          if (nb[u] & (1u << f)) sacc += xs[u][f] - xc;
-        ax = (bc * sacc) * ih2;
-      } else {
-        int i, j, kk;
-        ijk_of(g, idx, i, j, kk);
-        const int8_t m = o.mark[idx];
-        const double xc = x64[idx] + (double)x[idx];
-        double sacc = 0.0;
-#pragma unroll
-        for (int f = 0; f < 6; ++f) {
-          const int8_t mn = mark_at(g, o.mark, i + c_off[f][0], j + c_off[f][1], kk + c_off[f][2]);
-          if (neumann(o, OPk, mn)) continue;
-          const int64_t n = idx + st[f];
-          const double xn = (mn == FLUID || mn == AIR) ? x64[n] + (double)x[n] : 0.0;
-          const double bf = (OPk == OP_UNIT) ? 1.0 : face_beta<double>(o, idx, n, m, mn);
-          sacc += bf * (xn - xc);
+        for (int u = 0; u < RPT; ++u) {
+          XV[s1][u] = form(R[s1][u]);
+          NB[s1][u] = R[s1][u].nb;
+          BV[s1][u] = R[s1][u].b;
+          pl[sn][jl0 + u + 1][kl + 1] = XV[s1][u];
         }
-        ax = sacc / ((double)g.h * (double)g.h);
+        if (ring) pl[sn][hj + 1][hk + 1] = form(HR[s1]);
+        lds_barrier();
+        const int64_t pb = (int64_t)i * sx;
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) {
+          const int jl = jl0 + u;
+          const uint8_t nb = NB[s0][u];
+          if (!(nb & NB_UNKNOWN)) continue;
+          const double xc = XV[s0][u];
+          // neighbours in the order +x, +y, +z, -x, -y, -z
+          const double xn[6] = {XV[s1][u], pl[scur][jl + 2][kl + 1], pl[scur][jl + 1][kl + 2],
+                                XV[s2][u], pl[scur][jl][kl + 1], pl[scur][jl + 1][kl]};
+          const int64_t idx = pb + col[u];
+          double ax;
+          if (!SLOW || !(nb & NB_SLOW)) {
+            double sacc = 0.0;
+#pragma unroll
+            for (int f = 0; f < 6; ++f)
+              if (nb & (1u << f)) sacc += xn[f] - xc;
+            ax = (bc * sacc) * ih2;
+          } else {
+            int ci, cj, ck;
+            ijk_of(g, idx, ci, cj, ck);
+            const int8_t m = o.mark[idx];
+            double sacc = 0.0;
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+              const int8_t mn = mark_at(g, o.mark, ci + c_off[f][0], cj + c_off[f][1], ck + c_off[f][2]);
+              if (neumann(o, OPk, mn)) continue;
+              const int64_t n = cid(g, ci + c_off[f][0], cj + c_off[f][1], ck + c_off[f][2]);
+              const double bf = (OPk == OP_UNIT) ? 1.0 : face_beta<double>(o, idx, n, m, mn);
+              sacc += bf * (xn[f] - xc);
+            }
+            ax = sacc / ((double)g.h * (double)g.h);
+          }
+          const double rv = (double)BV[s0][u] - ax;
+          const float rf = (float)rv;
+          (r + pb)[col[u]] = rf;
+          a_rr += rv * rv;
+          const float dv = !o.precond ? 1.0f
+                           : (SLOW && (nb & NB_SLOW)) ? dslow_at<float>(o, idx)
+                                                      : dtab[__popc((unsigned)(nb & 63))];
+          a_rz += (double)rf * (double)(rf * dv);
+        }
       }
-      const double rv = (double)bv[u] - ax;
-      const float rf = (float)rv;
-      r[idx] = rf;
-      a_rr += rv * rv;
-      a_rz += (double)rf * (double)(rf * inv_diag<float, OPk>(o, idx, nb[u]));
     }
   }
   double v[2] = {a_rz, a_rr};
-  write_partials<2>(v, uslot(P, k));
+  write_partials<2, ST>(v, uslot(P, k));
+  pad_partials2<ST>(uslot(P, k), T.nblk, npad);
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc->refinements += 1;
     sc->last_true = k;
@@ -1000,31 +1112,35 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
            (const uint32_t*)c.count, slow_list);
   int nsb = (int)((nslow + PB - 1) / PB);
   auto spmv = nslow > 0 ? k_spmv<V, Dacc, OPk, true> : k_spmv<V, Dacc, OPk, false>;
+  auto resid64 = nslow > 0 ? k_resid64<OPk, true> : k_resid64<OPk, false>;
   const Tiling T = make_tiling(c.g, (sizeof(V) == 4 && nslow == 0) ? 768 : 512);
-  if (T.nblk > 2 * MAX_BLK) {
+  // U-slot partials: update and init write L.nblk, the residual pass T.nblk;
+  // both pad to npad, the count the spmv sums
+  const int npad = L.nblk > T.nblk ? L.nblk : T.nblk;
+  if (npad > MAX_BLK) {
     c.err = "grid too large for the solver's reduction buffers";
     return MPF_INVALID_ARG;
   }
   if (nsb > MAX_BLK) nsb = MAX_BLK;
   launch(c, "pcg_init", dim3(L.nblk), dim3(PB), 0, k_init<V, Dacc, OPk>, o, L, b, x, r, pbuf[0], pbuf[1],
-         mixed ? x64 : (double*)nullptr, P, c.scal);
+         mixed ? x64 : (double*)nullptr, P, c.scal, npad);
   int k = 0;
   bool fold_next = false;
   while (k <= cap) {
     const int k_end = k + check;
     for (; k < k_end && k <= cap; ++k) {
-      launch(c, "pcg_spmv", dim3(T.nblk), dim3(ST), 0, spmv, o, T, L.nblk, (const V*)r, (const V*)pbuf[k & 1],
+      launch(c, "pcg_spmv", dim3(T.nblk), dim3(ST), 0, spmv, o, T, npad, (const V*)r, (const V*)pbuf[k & 1],
              pbuf[(k + 1) & 1], q, P, c.scal, k, (int)ref, eps_abs, tol, cap);
       if (nsb > 0)
         launch(c, "pcg_slow_q", dim3(nsb), dim3(PB), 0, k_slow_q<V, Dacc, OPk>, o, (const uint32_t*)slow_list, nslow,
                (const V*)pbuf[(k + 1) & 1], q, P.S2, c.scal);
       launch(c, "pcg_update", dim3(L.nblk), dim3(PB), 0, k_update<V, Dacc, OPk>, o, L, x, r,
              (const V*)pbuf[(k + 1) & 1], (const V*)q, mixed ? x64 : (double*)nullptr, P, T.nblk, nsb, c.scal, k,
-             (int)fold_next);
+             (int)fold_next, npad);
       fold_next = false;
       if (mixed && (k + 1) % R == 0) {
-        launch(c, "pcg_resid64", dim3(L.nblk), dim3(PB), 0, k_resid64<OPk>, o, L, (const float*)b, (const double*)x64,
-               (const float*)x, (float*)r, P, c.scal, k);
+        launch(c, "pcg_resid64", dim3(T.nblk), dim3(ST), 0, resid64, o, T, (const float*)b,
+               (const double*)x64, (const float*)x, (float*)r, P, c.scal, k, npad);
         fold_next = true;
       }
     }
@@ -1039,8 +1155,8 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
     if (kd == 0 || s.last_true == kd - 1) break;
     // resid64 returns early while done is set: clear it first
     launch(c, "pcg_clear", dim3(1), dim3(64), 0, k_clear_done, c.scal);
-    launch(c, "pcg_resid64", dim3(L.nblk), dim3(PB), 0, k_resid64<OPk>, o, L, (const float*)b, (const double*)x64,
-           (const float*)x, (float*)r, P, c.scal, kd - 1);
+    launch(c, "pcg_resid64", dim3(T.nblk), dim3(ST), 0, resid64, o, T, (const float*)b, (const double*)x64,
+           (const float*)x, (float*)r, P, c.scal, kd - 1, npad);
     fold_next = true;
     k = kd;
   }

@@ -335,6 +335,7 @@ void run_cell_mask(Ctx& c) {
   const int64_t G = c.g.G;
   launch(c, "cell_mask", dim3(nblocks(G, 256)), dim3(256), 0, k_cell_mask, G, (const uint32_t*)c.cstart,
          (const uint32_t*)c.cend, (const int8_t*)c.flag[c.cur], c.cmask);
+  run_bricks(c);
 }
 
 void run_hash_sort(Ctx& c) {
