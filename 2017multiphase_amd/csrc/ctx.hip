@@ -164,7 +164,7 @@ bool alloc_all(Ctx& c) {
   ok = ok && dalloc(c, dd, G); c.vp2 = dd;
   ok = ok && dalloc(c, c.nbm, G) && dalloc(c, c.dslow, G);
   c.npart = 2048 * 4;
-  ok = ok && dalloc(c, c.partials, c.npart) && dalloc(c, c.scal, 1) && dalloc(c, c.counters, 64);
+  ok = ok && dalloc(c, c.partials, c.npart) && dalloc(c, c.scal, 1) && dalloc(c, c.counters, NCOUNTERS);
   for (int a = 0; a < 3 && ok; ++a)
     ok = ok && dalloc(c, c.wu[a], fc[a]) && dalloc(c, c.au[a], fc[a]) && dalloc(c, c.wuo[a], fc[a]) &&
          dalloc(c, c.auo[a], fc[a]) && dalloc(c, c.su[a], fc[a]) && dalloc(c, c.ftmp[a], fc[a]);
@@ -179,11 +179,11 @@ bool alloc_all(Ctx& c) {
   }
   if (!ok) return false;
   if (hipHostMalloc((void**)&c.scal_host, sizeof(SolveScalars), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c.counters_host, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c.counters_host, NCOUNTERS * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
     c.err = "hipHostMalloc failed";
     return false;
   }
-  hipMemsetAsync(c.counters, 0, 64 * 4, c.stream);
+  hipMemsetAsync(c.counters, 0, NCOUNTERS * 4, c.stream);
   hipMemsetAsync(c.scal, 0, sizeof(SolveScalars), c.stream);
   // zero-initialise the fields a caller may read before they are computed
   hipMemsetAsync(c.mark, VACUUM, (size_t)G, c.stream);

@@ -384,24 +384,44 @@ __global__ void __launch_bounds__(ST_CELLS)
     for (int ph = 0; ph < 27; ++ph) {
       const int o[3] = {ph / 9 - 1, (ph / 3) % 3 - 1, ph % 3 - 1};
       if (n > 0) {
+        // The faces of the three axes in this phase share the cell c - o:
+        // axis a's face sits at that cell's corner along a and at its centre
+        // along the others, so each particle's six coordinate differences
+        // (corner and centre, per direction) and their squares serve all
+        // three distances, summed in the gather's order.
+        const int f[3] = {ci - o[0], cj - o[1], ck - o[2]};
+        int li[3];
+        float W[3], U[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          if (o[a] > 0) continue;  // normal window is {-1, 0}
-          const int f[3] = {ci - o[0], cj - o[1], ck - o[2]};
           const int r0 = x0 - (a != 0), r1 = y0 - (a != 1), r2 = z0 - (a != 2);
-          const int li = st_roff(a) + ((f[0] - r0) * st_r(a, 1) + (f[1] - r1)) * st_r(a, 2) + (f[2] - r2);
-          float W = sW[li], U = sU[li];
+          li[a] = st_roff(a) + ((f[0] - r0) * st_r(a, 1) + (f[1] - r1)) * st_r(a, 2) + (f[2] - r2);
+          W[a] = (o[a] > 0) ? 0.0f : sW[li[a]];  // normal window is {-1, 0}
+          U[a] = (o[a] > 0) ? 0.0f : sU[li[a]];
+        }
+        const float fx = (float)f[0], fy = (float)f[1], fz = (float)f[2];
+        const float hx = fx + 0.5f, hy = fy + 0.5f, hz = fz + 0.5f;
+#pragma unroll
+        for (int q = 0; q < SCH; ++q) {
+          const float ax = X[q] - fx, bx = X[q] - hx, ay = Y[q] - fy, by = Y[q] - hy, az = Z[q] - fz, bz = Z[q] - hz;
+          const float ax2 = ax * ax, bx2 = bx * bx, ay2 = ay * ay, by2 = by * by, az2 = az * az, bz2 = bz * bz;
+          const float d2[3] = {ax2 + by2 + bz2, bx2 + ay2 + bz2, bx2 + by2 + az2};
+          const float vq[3] = {VX[q], VY[q], VZ[q]};
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            if (o[a] > 0) continue;
+            const float w = M[q] * fmaxf(div_nr(RE2, fmaxf(d2[a], 0.0001f)) - 1.0f, 0.0f);
+            W[a] += w;
+            U[a] += w * vq[a];
+          }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          if (o[a] > 0) continue;
           const float gx = (float)f[0] + (a == 0 ? 0.0f : 0.5f);
           const float gy = (float)f[1] + (a == 1 ? 0.0f : 0.5f);
           const float gz = (float)f[2] + (a == 2 ? 0.0f : 0.5f);
-#pragma unroll
-          for (int q = 0; q < SCH; ++q) {
-            const float dx = X[q] - gx, dy = Y[q] - gy, dz = Z[q] - gz;
-            const float r2s = dx * dx + dy * dy + dz * dz;
-            const float w = M[q] * fmaxf(div_nr(RE2, fmaxf(r2s, 0.0001f)) - 1.0f, 0.0f);
-            W += w;
-            U += w * (a == 0 ? VX[q] : (a == 1 ? VY[q] : VZ[q]));
-          }
+          float Wa = W[a], Ua = U[a];
           for (uint32_t p = s + SCH; p < e; ++p) {  // beyond the register chunk, still in index order
             const int8_t fl = flag[p];
             if (!(fl == FLUID || fl == SOLID)) continue;
@@ -410,12 +430,12 @@ __global__ void __launch_bounds__(ST_CELLS)
                         dz = (x.z - g.gz0) * scale - gz;
             const float r2s = dx * dx + dy * dy + dz * dz;
             const float w = mass[p] * fmaxf(div_nr(RE2, fmaxf(r2s, 0.0001f)) - 1.0f, 0.0f);
-            W += w;
+            Wa += w;
             const float* vp = &vel[p].x;
-            U += w * vp[a];
+            Ua += w * vp[a];
           }
-          sW[li] = W;
-          sU[li] = U;
+          sW[li[a]] = Wa;
+          sU[li[a]] = Ua;
         }
       }
       __syncthreads();
@@ -862,8 +882,9 @@ __global__ void k_mc_classify(Grid g, const float* __restrict__ f, uint8_t* __re
       edge_used[3 * cid(g, i + c_eo[e][0], j + c_eo[e][1], k + c_eo[e][2]) + c_eo[e][3]] = 1u;
     }
   }
-  unsigned long long m = __ballot(nv > 0);
-  if (m && __lane_id() == (unsigned)(__ffsll((long long)m) - 1)) atomicAdd(active, (uint32_t)__popcll(m));
+  // active voxels: one add per block, spread over NSPREAD counters
+  const int cnt = __syncthreads_count(nv > 0);
+  if (threadIdx.x == 0 && cnt) atomicAdd(active + (blockIdx.x % NSPREAD) * SPREAD_STRIDE, (uint32_t)cnt);
 }
 
 // vertexInterp spray_k.cu:1905-1910 along each used edge; edge_id holds the
@@ -1189,16 +1210,17 @@ int run_marching_cubes(Ctx& c, mpf_mc_info* info) {
   const int64_t V = (int64_t)(c.g.nx - 1) * (c.g.ny - 1) * (c.g.nz - 1);
   c.mc_nvox = V;
   hipMemsetAsync(c.mc_edge, 0, (size_t)3 * G * 4, c.stream);
-  hipMemsetAsync(c.counters + 3, 0, 4, c.stream);
+  uint32_t* spread = c.counters + 64;
+  hipMemsetAsync(spread, 0, NSPREAD * SPREAD_STRIDE * 4, c.stream);
   launch(c, "mc_classify", dim3(nblocks(V, 256)), dim3(256), 0, k_mc_classify, c.g, (const float*)c.lsfluid, c.mc_case,
-         c.mc_tcount, c.mc_edge, V, c.counters + 3);
+         c.mc_tcount, c.mc_edge, V, spread);
   // vertex ids = exclusive scan of the used-edge flags (second half of mc_edge)
   uint32_t* vid = c.mc_edge + 3 * G;
   int64_t nv = run_scan_u32(c, c.mc_edge, vid, 3 * G, true);
   // triangle offsets = exclusive scan of per-voxel triangle counts (second half of mc_tcount)
   int64_t nt = run_scan_u32(c, c.mc_tcount, c.mc_tcount + V, V, true);
   if (nv < 0 || nt < 0) return MPF_HIP_ERROR;
-  hipMemcpyAsync(c.counters_host + 3, c.counters + 3, 4, hipMemcpyDeviceToHost, c.stream);
+  hipMemcpyAsync(c.counters_host + 64, spread, NSPREAD * SPREAD_STRIDE * 4, hipMemcpyDeviceToHost, c.stream);
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "marching_cubes")) return MPF_HIP_ERROR;
   if (nv > c.cfg.mc_max_verts || nt > c.cfg.mc_max_tris) {
     c.err = "marching cubes mesh exceeds mc_max_verts/mc_max_tris";
@@ -1213,7 +1235,9 @@ int run_marching_cubes(Ctx& c, mpf_mc_info* info) {
   if (info) {
     info->verts = nv;
     info->tris = nt;
-    info->active_voxels = (int64_t)c.counters_host[3];
+    int64_t act = 0;
+    for (int q = 0; q < NSPREAD; ++q) act += c.counters_host[64 + q * SPREAD_STRIDE];
+    info->active_voxels = act;
   }
   return MPF_OK;
 }

@@ -172,7 +172,7 @@ struct Ctx {
   int npart = 0;
   SolveScalars* scal = nullptr;
   SolveScalars* scal_host = nullptr;  // pinned
-  uint32_t* counters = nullptr;       // [0] reduction arrival, [1..] misc counters
+  uint32_t* counters = nullptr;       // [0] reduction arrival, [1..63] misc counters, then spread counters
   uint32_t* counters_host = nullptr;  // pinned
 
   // faces
@@ -258,6 +258,11 @@ void run_heat_jacobi(Ctx& c, int iters);
 int run_marching_cubes(Ctx& c, mpf_mc_info* info);
 void run_solid_phi_and_velocity(Ctx& c);
 int64_t p2g_parts_floats(const Grid& g);
+// Device counters: 64 single counters, then SPREAD counters 64 bytes apart
+// (a total that many blocks add to, split over cache lines)
+constexpr int NSPREAD = 64;
+constexpr int SPREAD_STRIDE = 16;
+constexpr int NCOUNTERS = 64 + NSPREAD * SPREAD_STRIDE;
 // 4x4x4 cell bricks of the level-set search summary
 inline int64_t nbricks(const Grid& g) {
   return (int64_t)((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4);
