@@ -797,22 +797,55 @@ __global__ void k_heat_pre(Grid g, float* __restrict__ tp, float* __restrict__ t
 }
 
 // One Jacobi sweep of the heat operator (computeAx_heat, spray_k.cu:4526-4567).
-__global__ void k_heat_jacobi(Grid g, const float* __restrict__ x, float* __restrict__ xn, const float* __restrict__ b,
-                              const int8_t* __restrict__ mark, float a0, float a1, float a2, float a3, float a4,
-                              float a5, float dt, int vac_neumann) {
+// Per cell, once per Jacobi solve: its type (7 = BOUNDARY) and the 6-bit
+// mask of neighbours that are not Neumann, in the order +x, +y, +z, -x, -y, -z.
+__global__ void k_heat_code(Grid g, const int8_t* __restrict__ mark, uint16_t* __restrict__ code, int vac_neumann) {
   CELL_LOOP_HEAD
-  int8_t m = mark[idx];
-  if (m == BOUNDARY) { xn[idx] = 0.0f; return; }
-  const float al[6] = {a0, a1, a2, a3, a4, a5};
-  double dg = (double)g.h * (double)g.h / (double)al[m] / (double)dt + 6.0, s = (double)b[idx];
+  const int8_t m = mark[idx];
+  if (m == BOUNDARY) {
+    code[idx] = 7;
+    return;
+  }
   const int off[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+  unsigned mask = 0;
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
-    int8_t mn = mark_at(g, mark, i + off[q][0], j + off[q][1], k + off[q][2]);
-    if (mn == BOUNDARY || (vac_neumann && mn == VACUUM)) dg -= 1.0;
-    else s += (double)x[cid(g, i + off[q][0], j + off[q][1], k + off[q][2])];
+    const int8_t mn = mark_at(g, mark, i + off[q][0], j + off[q][1], k + off[q][2]);
+    if (!(mn == BOUNDARY || (vac_neumann && mn == VACUUM))) mask |= 1u << q;
   }
-  xn[idx] = (float)(s / dg);
+  code[idx] = (uint16_t)((unsigned)m | (mask << 3));
+}
+
+// One Jacobi pass: the diagonal h^2/(alpha_m dt) + 6, less one per Neumann
+// neighbour, from a per-block table of the 6 x 7 (type, Neumann count)
+// values (each subtraction of 1 is exact, so the table equals the
+// one-by-one form); x of the open neighbours added to b in face order.
+__global__ void k_heat_jacobi(Grid g, const float* __restrict__ x, float* __restrict__ xn, const float* __restrict__ b,
+                              const uint16_t* __restrict__ code, float a0, float a1, float a2, float a3, float a4,
+                              float a5, float dt) {
+  __shared__ double dgt[6][7];
+  if (threadIdx.x < 42) {
+    const float al[6] = {a0, a1, a2, a3, a4, a5};
+    const int m = threadIdx.x / 7, nn = threadIdx.x % 7;
+    double d = (double)g.h * (double)g.h / (double)al[m] / (double)dt + 6.0;
+    for (int q = 0; q < nn; ++q) d -= 1.0;
+    dgt[m][nn] = d;
+  }
+  __syncthreads();
+  CELL_LOOP_HEAD
+  const unsigned cd = code[idx];
+  if ((cd & 7u) == 7u) {
+    xn[idx] = 0.0f;
+    return;
+  }
+  const unsigned mask = cd >> 3;
+  const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz;
+  const int64_t st[6] = {sx, sy, 1, -sx, -sy, -1};
+  double s = (double)b[idx];
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+    if (mask & (1u << q)) s += (double)x[idx + st[q]];
+  xn[idx] = (float)(s / dgt[cd & 7u][6 - __popc(mask)]);
 }
 
 __global__ void k_heat_start(int64_t G, const float* __restrict__ tsave, const int8_t* __restrict__ mark,
@@ -1181,12 +1214,15 @@ void run_heat_jacobi(Ctx& c, int iters) {
   const int64_t G = c.g.G;
   float* x = (float*)c.vx;
   float* xn = (float*)c.vr;
+  uint16_t* code = (uint16_t*)c.ctmp_u32;
   launch(c, "heat_start", dim3(nblocks(G, 256)), dim3(256), 0, k_heat_start, G, (const float*)c.tsave,
          (const int8_t*)c.mark, x);
+  launch(c, "heat_code", dim3(nblocks(G, 256)), dim3(256), 0, k_heat_code, c.g, (const int8_t*)c.mark, code,
+         k.heat_vacuum_neumann);
   for (int it = 0; it < iters; ++it) {
     launch(c, "heat_jacobi", dim3(nblocks(G, 256)), dim3(256), 0, k_heat_jacobi, c.g, (const float*)x, xn,
-           (const float*)c.ctmp, (const int8_t*)c.mark, k.heat_alpha[0], k.heat_alpha[1], k.heat_alpha[2],
-           k.heat_alpha[3], k.heat_alpha[4], k.heat_alpha[5], k.dt, k.heat_vacuum_neumann);
+           (const float*)c.ctmp, (const uint16_t*)code, k.heat_alpha[0], k.heat_alpha[1], k.heat_alpha[2],
+           k.heat_alpha[3], k.heat_alpha[4], k.heat_alpha[5], k.dt);
     float* t = x; x = xn; xn = t;
   }
   hipMemcpyAsync(c.tp, x, (size_t)G * 4, hipMemcpyDeviceToDevice, c.stream);
