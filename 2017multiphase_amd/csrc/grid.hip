@@ -104,13 +104,26 @@ __global__ void k_genlevelset(Grid g, const f3* __restrict__ pos, const int8_t* 
       if (!valid(g, ci, cj, ck)) continue;
       int64_t c = cid(g, ci, cj, ck);
       if (!(cmask[c] & want)) continue;
-      uint32_t s = cstart[c], e = cend[c];
-      for (uint32_t p = s; p < e; ++p) {
-        f3 d = sub3(pos[p], gp);
-        float d2 = dot3(d, d);
-        int8_t f = flag[p];
-        if (f == AIR || f == AIRSOLO) ma2 = fminf(ma2, d2);
-        else if (f == FLUID || f == SOLID) mf2 = fminf(mf2, d2);
+      const uint32_t s = cstart[c], e = cend[c];
+      // four particles' loads issued together (the minimum does not depend
+      // on the order); lanes past the range re-read the first particle
+      for (uint32_t p = s; p < e; p += 4) {
+        f3 x[4];
+        int8_t fl[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const bool in = p + u < e;
+          x[u] = pos[in ? p + u : s];
+          fl[u] = in ? flag[p + u] : (int8_t)-1;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          f3 d = sub3(x[u], gp);
+          float d2 = dot3(d, d);
+          const int8_t f = fl[u];
+          if (f == AIR || f == AIRSOLO) ma2 = fminf(ma2, d2);
+          else if (f == FLUID || f == SOLID) mf2 = fminf(mf2, d2);
+        }
       }
     }
   }
