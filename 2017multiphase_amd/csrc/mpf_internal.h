@@ -110,6 +110,7 @@ __host__ __device__ inline int64_t fid(const Grid& g, int a, int i, int j, int k
 // recurrence runs without host round trips.
 struct SolveScalars {
   double rz, rz_prev, pq, rr, bb, alpha, beta, rr_true;
+  double al[2];  // alpha of update k in slot k&1 (x may take it one iteration later)
   int iter, done, converged, nonfinite, refinements, rec_conv, fold_pending;
   int last_true;  // MIXED: update index whose residual the last executed true-residual pass replaced (-1 none)
 };

@@ -652,12 +652,15 @@ __global__ void __launch_bounds__(PB) k_slow_q(OpArgs o, const uint32_t* __restr
 }
 
 // x += alpha p_k (folding x into x64 right after a reliable update);
-// r -= alpha q; partials r.z, r.r into U slot k&1.
+// r -= alpha q; partials r.z, r.r into U slot k&1.  x advances every other
+// iteration: mode 0 leaves x (the host keeps alpha_k p_k pending), mode 1
+// adds alpha_k p_k, mode 2 adds the pending alpha_{k-1} p_{k-1} and then
+// alpha_k p_k, rounding after each add exactly like two updates.
 template <class V, class Dacc, int OPk>
 __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict__ x, V* __restrict__ r,
-                                               const V* __restrict__ pc, const V* __restrict__ q,
-                                               double* __restrict__ x64, Parts P, int nbs, int nbs2, SolveScalars* sc,
-                                               int k, int fold, int npad) {
+                                               const V* __restrict__ pc, const V* __restrict__ ppv,
+                                               const V* __restrict__ q, double* __restrict__ x64, Parts P, int nbs,
+                                               int nbs2, SolveScalars* sc, int k, int fold, int mode, int npad) {
   if (sc->done) return;
   double t[1], t2[1];
   read_partials<1>(P.S, nbs, t);
@@ -672,13 +675,19 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
     }
     return;
   }
-  const V al = (V)(sc->rz / t[0]);
+  const double ald = sc->rz / t[0];
+  const V al = (V)ald;
+  const V al1 = (mode == 2) ? (V)sc->al[(k - 1) & 1] : V(0);
+  if (blockIdx.x == 0 && threadIdx.x == 0) sc->al[k & 1] = ald;
   Dacc a_rz = 0, a_rr = 0;
-  // one cell: x += al p (or x64 += x + al p, x = 0 when folding), r -= al q,
-  // partial r.z and r.r
-  auto cell = [&](uint8_t nb, V pv, V xv, V rv, V qv, V ds, double x64v, V& xo, V& ro, double& x64o) {
-    xo = fold ? V(0) : xv + al * pv;
-    if (fold) x64o = x64v + (double)(xv + al * pv);
+  // one cell: x += al p (or x64 += x + al p, x = 0 when folding), after
+  // x += al1 p_prev in mode 2; r -= al q; partial r.z and r.r
+  auto cell = [&](uint8_t nb, V pv, V p1v, V xv, V rv, V qv, V ds, double x64v, V& xo, V& ro, double& x64o) {
+    if (mode != 0) {
+      const V xa = (mode == 2) ? xv + al1 * p1v : xv;
+      xo = fold ? V(0) : xa + al * pv;
+      if (fold) x64o = x64v + (double)(xa + al * pv);
+    }
     const V rn = rv - al * qv;
     ro = rn;
     const V d = !o.precond ? V(1) : (nb & NB_SLOW) ? ds : dinv_tab<V>(o, __popc((unsigned)(nb & 63)));
@@ -711,17 +720,20 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
     uint32_t m4[UQ];
 #pragma unroll
     for (int u = 0; u < UQ; ++u) m4[u] = mn[u];
-    Q4 pv[UQ], xv[UQ], rv[UQ], qv[UQ], ds[UQ];
-    D4 x64v[UQ];
+    Q4 pv[UQ] = {}, p1[UQ] = {}, xv[UQ] = {}, rv[UQ], qv[UQ], ds[UQ];
+    D4 x64v[UQ] = {};
 #pragma unroll
     for (int u = 0; u < UQ; ++u) {
       const int64_t b4 = base + (int64_t)u * 4 * PB;
       if (m4[u] & 0x40404040u) {
-        pv[u] = *(const Q4*)(pc + b4);
-        xv[u] = *(const Q4*)(x + b4);
         rv[u] = *(const Q4*)(r + b4);
         qv[u] = *(const Q4*)(q + b4);  // q is only defined on the unknowns; used only there
-        if (fold) x64v[u] = *(const D4*)(x64 + b4);
+        if (mode != 0) {
+          pv[u] = *(const Q4*)(pc + b4);
+          xv[u] = *(const Q4*)(x + b4);
+          if (mode == 2) p1[u] = *(const Q4*)(ppv + b4);
+          if (fold) x64v[u] = *(const D4*)(x64 + b4);
+        }
       }
     }
     load_masks(base + PASS, mn);
@@ -756,18 +768,19 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
         ro.e[e] = rv[u].e[e];
         x64o.e[e] = fold ? x64v[u].e[e] : 0.0;
         if (nb & NB_UNKNOWN)
-          cell(nb, pv[u].e[e], xv[u].e[e], rv[u].e[e], qv[u].e[e], ds[u].e[e], x64o.e[e], xo.e[e], ro.e[e], x64o.e[e]);
+          cell(nb, pv[u].e[e], p1[u].e[e], xv[u].e[e], rv[u].e[e], qv[u].e[e], ds[u].e[e], x64o.e[e], xo.e[e],
+               ro.e[e], x64o.e[e]);
       }
       if (un4 == 0x40404040u) {
-        *(Q4*)(x + b4) = xo;
         *(Q4*)(r + b4) = ro;
+        if (mode != 0) *(Q4*)(x + b4) = xo;
         if (fold) *(D4*)(x64 + b4) = x64o;
       } else {
 #pragma unroll
         for (int e = 0; e < 4; ++e)
           if ((m4[u] >> (8 * e)) & NB_UNKNOWN) {
-            x[b4 + e] = xo.e[e];
             r[b4 + e] = ro.e[e];
+            if (mode != 0) x[b4 + e] = xo.e[e];
             if (fold) x64[b4 + e] = x64o.e[e];
           }
       }
@@ -779,12 +792,12 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
     const uint8_t nb = o.nbm[idx];
     if (nb & NB_UNKNOWN) {
       const bool sl = o.precond && (nb & NB_SLOW);
-      V xo, ro;
+      V xo = V(0), ro;
       double x64o = 0.0;
-      cell(nb, pc[idx], x[idx], r[idx], q[idx], sl ? dslow_at<V>(o, idx) : V(1), fold ? x64[idx] : 0.0, xo, ro,
-           x64o);
-      x[idx] = xo;
+      cell(nb, mode != 0 ? pc[idx] : V(0), mode == 2 ? ppv[idx] : V(0), mode != 0 ? x[idx] : V(0), r[idx], q[idx],
+           sl ? dslow_at<V>(o, idx) : V(1), fold ? x64[idx] : 0.0, xo, ro, x64o);
       r[idx] = ro;
+      if (mode != 0) x[idx] = xo;
       if (fold) x64[idx] = x64o;
     }
   }
@@ -975,6 +988,18 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_resid64(OpArgs o, Tiling T
 }
 
 // MIXED: x64 += x at exit.
+// x += alpha_{k} p_{k} that update k left pending (alpha from its slot).
+template <class V>
+__global__ void __launch_bounds__(PB) k_flush_x(Launch L, int64_t G, const uint8_t* __restrict__ nbm,
+                                                V* __restrict__ x, const V* __restrict__ p,
+                                                const SolveScalars* __restrict__ sc, int slot) {
+  const V al = (V)sc->al[slot];
+  CELLS_OF_BLOCK(G) {
+    if (!(nbm[idx] & NB_UNKNOWN)) continue;
+    x[idx] = x[idx] + al * p[idx];
+  }
+}
+
 __global__ void __launch_bounds__(PB) k_final_fold(Launch L, int64_t G, const uint8_t* __restrict__ nbm,
                                                    float* __restrict__ x, double* __restrict__ x64) {
   CELLS_OF_BLOCK(G) {
@@ -1126,6 +1151,15 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
          mixed ? x64 : (double*)nullptr, P, c.scal, npad);
   int k = 0;
   bool fold_next = false;
+  // x advances every other iteration (update modes: see k_update); half[k]
+  // records that update k left alpha_k p_k pending in x
+  std::vector<char> half((size_t)cap + 2, 0);
+  auto flush = [&](int kd) {  // x += alpha_{kd-1} p_{kd-1} if update kd-1 left it pending
+    if (kd < 1 || !half[kd - 1]) return;
+    launch(c, "pcg_flush_x", dim3(L.nblk), dim3(PB), 0, k_flush_x<V>, L, G, (const uint8_t*)c.nbm, x,
+           (const V*)pbuf[kd & 1], (const SolveScalars*)c.scal, (kd - 1) & 1);
+    half[kd - 1] = 0;
+  };
   while (k <= cap) {
     const int k_end = k + check;
     for (; k < k_end && k <= cap; ++k) {
@@ -1134,9 +1168,13 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
       if (nsb > 0)
         launch(c, "pcg_slow_q", dim3(nsb), dim3(PB), 0, k_slow_q<V, Dacc, OPk>, o, (const uint32_t*)slow_list, nslow,
                (const V*)pbuf[(k + 1) & 1], q, P.S2, c.scal);
+      const bool resid_next = mixed && (k + 1) % R == 0;
+      const bool h = (k % 2 == 0) && !resid_next && !fold_next;
+      const int mode = h ? 0 : ((k > 0 && half[k - 1]) ? 2 : 1);
+      half[k] = h;
       launch(c, "pcg_update", dim3(L.nblk), dim3(PB), 0, k_update<V, Dacc, OPk>, o, L, x, r,
-             (const V*)pbuf[(k + 1) & 1], (const V*)q, mixed ? x64 : (double*)nullptr, P, T.nblk, nsb, c.scal, k,
-             (int)fold_next, npad);
+             (const V*)pbuf[(k + 1) & 1], (const V*)pbuf[k & 1], (const V*)q, mixed ? x64 : (double*)nullptr, P,
+             T.nblk, nsb, c.scal, k, (int)fold_next, mode, npad);
       fold_next = false;
       if (mixed && (k + 1) % R == 0) {
         launch(c, "pcg_resid64", dim3(T.nblk), dim3(ST), 0, resid64, o, T, (const float*)b,
@@ -1147,6 +1185,9 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
     if (!read_scalars(c)) return MPF_HIP_ERROR;
     const SolveScalars& s = *c.scal_host;
     if (!s.done) continue;
+    // updates after s.iter - 1 returned without running: x takes what the
+    // last one that ran left pending
+    flush(s.iter);
     if (!mixed || !s.converged || s.nonfinite) break;
     // spmv(kd) met the criterion on the residual of update(kd-1).  Which
     // true-residual pass last ran is read from the device: passes launched

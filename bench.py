@@ -34,8 +34,10 @@ HBM_PEAK_GBS = 8000.0
 KERNEL_BYTES = {
     # mask 1 per cell; r 4 + p_{k-1} 4 read, p_k 4 + q 4 written per unknown
     "pcg_spmv": (1, 16, 0),
-    # mask 1 per cell; x, r read+write 16, p 4, q 4 per unknown
-    "pcg_update": (1, 24, 0),
+    # mask 1 per cell; per unknown, averaged over the alternating launches: one
+    # reads r, q and writes r (12), the next also reads x, p_k, p_{k-1} and
+    # writes x (28)
+    "pcg_update": (1, 20, 0),
     # mask 1 per cell; b 4, x64 8, x 4 read, r 4 written per unknown
     "pcg_resid64": (1, 20, 0),
     # start/end 8, mask 1, ls fluid+air 8 written; pos 12 + flag 1 per particle
@@ -53,7 +55,7 @@ KERNEL_BYTES = {
     # extrapolation pass per face: u 4, out 4, ls 2x4, mark 2
     "sweep_u": (18, 0, 0),
 }
-FP64_VEC = {"pcg_spmv": (1, 32, 0), "pcg_update": (1, 48, 0)}
+FP64_VEC = {"pcg_spmv": (1, 32, 0), "pcg_update": (1, 40, 0)}
 
 PROFILES = os.path.join(ROOT, "profiles")
 
