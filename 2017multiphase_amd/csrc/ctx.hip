@@ -417,6 +417,7 @@ mpf_status mpf_set_particles(mpf_ctx* ctx, int64_t n, const float* pos, const fl
   put(c.gas[b], gas, (size_t)n * 4, 0);
   c.P = n;
   c.sorted = false;
+  c.own_sort = false;
   c.user_cstart = c.user_cend = false;
   return hip_ok(c, hipStreamSynchronize(c.stream), "set_particles") ? MPF_OK : MPF_HIP_ERROR;
 }
@@ -488,6 +489,7 @@ mpf_status mpf_set_field(mpf_ctx* ctx, int32_t field, const void* src, int32_t w
     // Caller-supplied cell lists (particles assumed in that order) take effect
     // once both arrays have been given and every list indexes the particles.
     c.sorted = false;
+    c.own_sort = false;
     if (field == MPF_F_CELL_START) c.user_cstart = true;
     else c.user_cend = true;
     if (!hip_ok(c, hipStreamSynchronize(c.stream), "set_field")) return MPF_HIP_ERROR;

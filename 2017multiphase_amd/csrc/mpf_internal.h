@@ -149,6 +149,9 @@ struct Ctx {
   uint32_t* ghash = nullptr;  // sorted keys (gridHash)
   bool sorted = false;
   bool user_cstart = false, user_cend = false;  // cell lists set by the caller since the last sort
+  // the cell lists and particle order come from this library's last sort and
+  // no particle moved or was added since (a new sort would be the identity)
+  bool own_sort = false;
 
   // cells
   uint32_t* count = nullptr;  // per-cell counts, then scatter cursors

@@ -292,7 +292,7 @@ __global__ void k_melt(int64_t P, int8_t* __restrict__ flag, float* __restrict__
 __global__ void k_freeze(Grid g, int64_t Pn, const int8_t* __restrict__ f0, const f3* __restrict__ p0,
                          int8_t* __restrict__ flag, f3* __restrict__ pos, float* __restrict__ lheat,
                          const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend, float ss,
-                         uint32_t* __restrict__ counter) {
+                         uint32_t* __restrict__ counter, uint32_t* __restrict__ moved) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool ch = false;
   if (p < Pn && f0[p] == FLUID && lheat[p] < 0) {
@@ -326,7 +326,15 @@ __global__ void k_freeze(Grid g, int64_t Pn, const int8_t* __restrict__ f0, cons
         f3 y = p0[mi];
         f3 dd = sub3(x, y);
         float l = len3(dd);
-        pos[p] = add3(mul3(mul3(dd, 1.0f / l), ss), y);
+        const f3 xn = add3(mul3(mul3(dd, 1.0f / l), ss), y);
+        pos[p] = xn;
+        // the snapped particle left its cell: the cell lists no longer hold
+        // (same cell rule as the hash key)
+        const float rx = (xn.x - g.gx0) / g.h, ry = (xn.y - g.gy0) / g.h, rz = (xn.z - g.gz0) / g.h;
+        const int ni = (rx >= 0 && rx < (float)g.nx) ? (int)rx : 0;
+        const int nj = (ry >= 0 && ry < (float)g.ny) ? (int)ry : 0;
+        const int nk = (rz >= 0 && rz < (float)g.nz) ? (int)rz : 0;
+        if (ni != i || nj != j || nk != k) atomicOr(moved, 1u);
       }
     }
   }
@@ -372,6 +380,7 @@ void run_advect(Ctx& c) {
            c.cfg.bounce_pos);
   }
   c.sorted = false;
+  c.own_sort = false;
 }
 
 void run_heat_g2p(Ctx& c) {
@@ -386,6 +395,7 @@ int run_phase_change(Ctx& c, int frame, mpf_phase_info* info) {
   const int cur = c.cur;
   int added = 0;
   hipMemsetAsync(c.counters + 1, 0, 8, c.stream);
+  hipMemsetAsync(c.counters + 7, 0, 4, c.stream);
   if (k.gas_enabled && k.gas_every > 0 && frame % k.gas_every == 0) {
     launch(c, "solubility", dim3(nblocks(c.P, 256)), dim3(256), 0, k_solubility, c.P, (const int8_t*)c.flag[cur],
            (const float*)c.temp[cur], c.solub[cur], k.dissolve_gas_rate * k.airm0, k.temperature0);
@@ -407,7 +417,7 @@ int run_phase_change(Ctx& c, int frame, mpf_phase_info* info) {
   }
   const int64_t Pn = c.P;  // particles covered by the cell lists
   c.P += added;
-  if (added) c.sorted = false;
+  if (added) c.sorted = c.own_sort = false;
   if (k.melt_enabled)
     launch(c, "melt", dim3(nblocks(c.P, 256)), dim3(256), 0, k_melt, c.P, c.flag[cur], c.lheat[cur], k.liquid_heat_th,
            c.counters + 1);
@@ -417,10 +427,11 @@ int run_phase_change(Ctx& c, int frame, mpf_phase_info* info) {
     hipMemcpyAsync(c.pos[alt], c.pos[cur], (size_t)Pn * sizeof(f3), hipMemcpyDeviceToDevice, c.stream);
     launch(c, "freeze", dim3(nblocks(Pn, 256)), dim3(256), 0, k_freeze, c.g, Pn, (const int8_t*)c.flag[alt],
            (const f3*)c.pos[alt], c.flag[cur], c.pos[cur], c.lheat[cur], (const uint32_t*)c.cstart,
-           (const uint32_t*)c.cend, k.samplespace, c.counters + 2);
+           (const uint32_t*)c.cend, k.samplespace, c.counters + 2, c.counters + 7);
   }
-  hipMemcpyAsync(c.counters_host + 1, c.counters + 1, 8, hipMemcpyDeviceToHost, c.stream);
+  hipMemcpyAsync(c.counters_host + 1, c.counters + 1, 7 * 4, hipMemcpyDeviceToHost, c.stream);
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "phase_change")) return MPF_HIP_ERROR;
+  if (k.freeze_enabled && c.counters_host[7]) c.own_sort = false;  // a frozen particle snapped into another cell
   if (info) {
     info->melted = (int32_t)c.counters_host[1];
     info->frozen = (int32_t)c.counters_host[2];

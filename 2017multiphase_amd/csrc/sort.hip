@@ -303,6 +303,11 @@ __global__ void k_scan_apply(const uint32_t* __restrict__ in, int64_t n, const u
 
 }  // namespace
 
+__global__ void k_iota(int64_t n, uint32_t* __restrict__ v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (uint32_t)i;
+}
+
 int64_t run_scan_u32(Ctx& c, const uint32_t* in, uint32_t* out, int64_t n, bool want_total) {
   if (n <= 0) return 0;
   int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -341,6 +346,14 @@ void run_cell_mask(Ctx& c) {
 void run_hash_sort(Ctx& c) {
   const int64_t P = c.P, G = c.g.G;
   const int cur = c.cur, nxt = 1 - c.cur;
+  if (c.sorted && c.own_sort) {
+    // nothing moved since this library's last sort: the stable sort is the
+    // identity (keys, cell ranges and order stand); flags may have changed,
+    // so the occupancy masks are rebuilt
+    launch(c, "sort_identity", dim3(nblocks(P, 256)), dim3(256), 0, k_iota, P, c.perm);
+    run_cell_mask(c);
+    return;
+  }
   hipMemsetAsync(c.count, 0, (size_t)G * 4, c.stream);
   launch(c, "hash_count", dim3(nblocks(P, 256)), dim3(256), 0, k_hash_count, c.g, (const f3*)c.pos[cur], P,
          c.key, c.count);
@@ -359,6 +372,7 @@ void run_hash_sort(Ctx& c) {
   c.cur = nxt;
   c.user_cstart = c.user_cend = false;
   c.sorted = true;
+  c.own_sort = true;
   run_cell_mask(c);
 }
 
