@@ -136,7 +136,8 @@ __global__ void k_genlevelset(Grid g, const f3* __restrict__ pos, const int8_t* 
 }
 
 // markLS_bigpositive spray_k.cu:3683-3697 (mode 0) / preparels 3708-3722 (mode 1)
-__global__ void k_ls_prepare(int64_t G, float* __restrict__ ls, int8_t* __restrict__ mark, float h, int mode) {
+__global__ void k_ls_prepare(int64_t G, const float* __restrict__ ls, float* __restrict__ out,
+                             int8_t* __restrict__ mark, float h, int mode) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= G) return;
   float v = ls[idx];
@@ -149,7 +150,7 @@ __global__ void k_ls_prepare(int64_t G, float* __restrict__ ls, int8_t* __restri
     if (v > 0) { v = 5.0f; mark[idx] = AIR; }
     else mark[idx] = FLUID;
   }
-  ls[idx] = v;
+  out[idx] = v;
 }
 // setLSback_bigpositive spray_k.cu:3699-3706 (mode 0) / setLSback 3724-3731 (mode 1)
 __global__ void k_ls_back(int64_t G, float* __restrict__ ls, float h, int mode) {
@@ -268,7 +269,8 @@ template <int A, bool AIRPH>
 __global__ void k_p2g(Grid g, const f3* __restrict__ pos, const f3* __restrict__ vel, const float* __restrict__ mass,
                       const int8_t* __restrict__ flag, const uint32_t* __restrict__ cstart,
                       const uint32_t* __restrict__ cend, const uint8_t* __restrict__ cmask,
-                      const uint8_t* __restrict__ dm, float* __restrict__ out, int64_t nf) {
+                      const uint8_t* __restrict__ dm, float* __restrict__ out, float* __restrict__ out2,
+                      int64_t nf) {
   int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
   int i, j, k;
@@ -281,6 +283,7 @@ __global__ void k_p2g(Grid g, const f3* __restrict__ pos, const f3* __restrict__
     const int64_t b = ((int64_t)(min(i, g.nx - 1) >> 2) * by + (min(j, g.ny - 1) >> 2)) * bz + (min(k, g.nz - 1) >> 2);
     if (!(dm[b] & need)) {
       out[f] = 0.0f;
+      if (out2) out2[f] = 0.0f;
       return;
     }
   }
@@ -310,7 +313,9 @@ __global__ void k_p2g(Grid g, const f3* __restrict__ pos, const f3* __restrict__
           usum += w * vp[A];
         }
       }
-  out[f] = (weight > 0) ? (usum / weight) : 0.0f;
+  const float u = (weight > 0) ? (usum / weight) : 0.0f;
+  out[f] = u;
+  if (out2) out2[f] = u;
 }
 
 // Liquid+solid P2G, tile scatter (the hot path).  A block owns a 4x8x8 cell
@@ -464,10 +469,12 @@ __global__ void __launch_bounds__(ST_CELLS)
 }
 
 // u = (sum w v) / (sum w) per face from the tile regions covering it, added in
-// tile order.  Band mode: faces without a cell of band < big beside them are 0.
+// tile order (also into out2 when given: the copy the FLIP update keeps).
+// Band mode: faces without a cell of band < big beside them are 0.
 template <int A>
 __global__ void k_p2g_collect(Grid g, const float* __restrict__ parts, int ntx, int nty, int ntz,
-                              float* __restrict__ out, int64_t nf, const float* __restrict__ band, float big) {
+                              float* __restrict__ out, float* __restrict__ out2, int64_t nf,
+                              const float* __restrict__ band, float big) {
   int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nf) return;
   int i, j, k;
@@ -507,7 +514,9 @@ __global__ void k_p2g_collect(Grid g, const float* __restrict__ parts, int ntx, 
         W += src[li];
         U += src[ST_RTOT + li];
       }
-  out[f] = (W > 0) ? (U / W) : 0.0f;
+  const float u = (W > 0) ? (U / W) : 0.0f;
+  out[f] = u;
+  if (out2) out2[f] = u;
 }
 
 // ---------------- extrapolation + wall conditions (a10) ----------------
@@ -978,9 +987,11 @@ __global__ void k_mc_tris(Grid g, const uint8_t* __restrict__ cases, const uint3
 // ================= host stage drivers =================
 // n two-buffer passes, ping-ponging between phi and the scratch field (one
 // copy at the end when n is odd).
-static void sweep_n(Ctx& c, float* phi, int typeflag, int n) {
-  float* src = phi;
-  float* dst = c.ctmp;
+// n passes ending in phi; the initial field is in c.ctmp when in_tmp (the
+// preparing kernel wrote it there, so an odd n needs no final copy)
+static void sweep_n(Ctx& c, float* phi, int typeflag, int n, bool in_tmp = false) {
+  float* src = in_tmp ? c.ctmp : phi;
+  float* dst = in_tmp ? phi : c.ctmp;
   for (int it = 0; it < n; ++it) {
     launch(c, "sweep", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_sweep, c.g, (const float*)src, dst,
            (const int8_t*)c.mark, typeflag);
@@ -993,11 +1004,13 @@ static void sweep_n(Ctx& c, float* phi, int typeflag, int n) {
 
 static void redistance(Ctx& c, float* ls) {
   const int64_t G = c.g.G;
-  launch(c, "ls_prepare", dim3(nblocks(G, 256)), dim3(256), 0, k_ls_prepare, G, ls, c.mark, c.g.h, 0);
-  sweep_n(c, ls, FLUID, 5);
+  launch(c, "ls_prepare", dim3(nblocks(G, 256)), dim3(256), 0, k_ls_prepare, G, (const float*)ls, c.ctmp, c.mark,
+         c.g.h, 0);
+  sweep_n(c, ls, FLUID, 5, true);
   launch(c, "ls_back", dim3(nblocks(G, 256)), dim3(256), 0, k_ls_back, G, ls, c.g.h, 0);
-  launch(c, "ls_prepare", dim3(nblocks(G, 256)), dim3(256), 0, k_ls_prepare, G, ls, c.mark, c.g.h, 1);
-  sweep_n(c, ls, FLUID, 5);
+  launch(c, "ls_prepare", dim3(nblocks(G, 256)), dim3(256), 0, k_ls_prepare, G, (const float*)ls, c.ctmp, c.mark,
+         c.g.h, 1);
+  sweep_n(c, ls, FLUID, 5, true);
   launch(c, "ls_back", dim3(nblocks(G, 256)), dim3(256), 0, k_ls_back, G, ls, c.g.h, 1);
 }
 
@@ -1071,7 +1084,7 @@ void run_levelset(Ctx& c) {
 }
 
 template <bool AIRPH>
-static void p2g_phase(Ctx& c, float* const* out) {
+static void p2g_phase(Ctx& c, float* const* out, float* const* out2) {
   const int cur = c.cur;
   const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
   const f3* pos = c.pos[cur];
@@ -1081,16 +1094,17 @@ static void p2g_phase(Ctx& c, float* const* out) {
   const char* nm = AIRPH ? "p2g_air" : "p2g_liquid";
   launch(c, nm, dim3(nblocks(fc[0], 256)), dim3(256), 0, k_p2g<0, AIRPH>, c.g, pos, vel, mass, flag,
          (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask,
-         (const uint8_t*)(c.bmask + nbricks(c.g)), out[0], fc[0]);
+         (const uint8_t*)(c.bmask + nbricks(c.g)), out[0], out2 ? out2[0] : (float*)nullptr, fc[0]);
   launch(c, nm, dim3(nblocks(fc[1], 256)), dim3(256), 0, k_p2g<1, AIRPH>, c.g, pos, vel, mass, flag,
          (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask,
-         (const uint8_t*)(c.bmask + nbricks(c.g)), out[1], fc[1]);
+         (const uint8_t*)(c.bmask + nbricks(c.g)), out[1], out2 ? out2[1] : (float*)nullptr, fc[1]);
   launch(c, nm, dim3(nblocks(fc[2], 256)), dim3(256), 0, k_p2g<2, AIRPH>, c.g, pos, vel, mass, flag,
          (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask,
-         (const uint8_t*)(c.bmask + nbricks(c.g)), out[2], fc[2]);
+         (const uint8_t*)(c.bmask + nbricks(c.g)), out[2], out2 ? out2[2] : (float*)nullptr, fc[2]);
 }
 
-static void p2g_liquid(Ctx& c, float* const* out, const float* band = nullptr, float big = 0.0f) {
+static void p2g_liquid(Ctx& c, float* const* out, float* const* out2 = nullptr, const float* band = nullptr,
+                       float big = 0.0f) {
   const int cur = c.cur;
   const int ntx = (c.g.nx + ST_X - 1) / ST_X, nty = (c.g.ny + ST_Y - 1) / ST_Y, ntz = (c.g.nz + ST_Z - 1) / ST_Z;
   launch(c, band ? "p2g_solid_band" : "p2g_liquid", dim3((unsigned)(ntx * nty * ntz)), dim3(ST_CELLS), 0,
@@ -1100,11 +1114,11 @@ static void p2g_liquid(Ctx& c, float* const* out, const float* band = nullptr, f
   const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
   const char* nm = band ? "p2g_solid_collect" : "p2g_collect";
   launch(c, nm, dim3(nblocks(fc[0], 256)), dim3(256), 0, k_p2g_collect<0>, c.g, (const float*)c.p2g_parts, ntx, nty,
-         ntz, out[0], fc[0], band, big);
+         ntz, out[0], out2 ? out2[0] : (float*)nullptr, fc[0], band, big);
   launch(c, nm, dim3(nblocks(fc[1], 256)), dim3(256), 0, k_p2g_collect<1>, c.g, (const float*)c.p2g_parts, ntx, nty,
-         ntz, out[1], fc[1], band, big);
+         ntz, out[1], out2 ? out2[1] : (float*)nullptr, fc[1], band, big);
   launch(c, nm, dim3(nblocks(fc[2], 256)), dim3(256), 0, k_p2g_collect<2>, c.g, (const float*)c.p2g_parts, ntx, nty,
-         ntz, out[2], fc[2], band, big);
+         ntz, out[2], out2 ? out2[2] : (float*)nullptr, fc[2], band, big);
 }
 
 int64_t p2g_parts_floats(const Grid& g) {
@@ -1113,29 +1127,24 @@ int64_t p2g_parts_floats(const Grid& g) {
 }
 
 void run_p2g(Ctx& c) {
-  p2g_liquid(c, c.wu);
-  p2g_phase<true>(c, c.au);
-  const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
-  for (int a = 0; a < 3; ++a) {
-    hipMemcpyAsync(c.wuo[a], c.wu[a], (size_t)fc[a] * 4, hipMemcpyDeviceToDevice, c.stream);
-    hipMemcpyAsync(c.auo[a], c.au[a], (size_t)fc[a] * 4, hipMemcpyDeviceToDevice, c.stream);
-  }
+  p2g_liquid(c, c.wu, c.wuo);
+  p2g_phase<true>(c, c.au, c.auo);
 }
 
 void run_solid_phi_and_velocity(Ctx& c) {
   const int64_t G = c.g.G;
   launch(c, "init_solid_phi", dim3(nblocks(G, 256)), dim3(256), 0, k_init_solid_phi, G, (const uint8_t*)c.cmask,
-         c.phisolid, (float)(3 * c.g.nx));
-  sweep_n(c, c.phisolid, -1, 3);
+         c.ctmp, (float)(3 * c.g.nx));
+  sweep_n(c, c.phisolid, -1, 3, true);
   // solid velocity only where the collision response can read it
-  p2g_liquid(c, c.su, c.phisolid, (float)(3 * c.g.nx));
+  p2g_liquid(c, c.su, nullptr, c.phisolid, (float)(3 * c.g.nx));
 }
 
 static void sweep_phi(Ctx& c, float* phi, int8_t type) {
   const int64_t G = c.g.G;
-  launch(c, "initphi", dim3(nblocks(G, 256)), dim3(256), 0, k_initphi, G, (const int8_t*)c.mark, phi, type,
+  launch(c, "initphi", dim3(nblocks(G, 256)), dim3(256), 0, k_initphi, G, (const int8_t*)c.mark, c.ctmp, type,
          (float)(c.g.nx * 3));
-  sweep_n(c, phi, type, 3);
+  sweep_n(c, phi, type, 3, true);
 }
 
 // Two passes: u -> scratch, scratch -> u.
