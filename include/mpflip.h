@@ -270,7 +270,7 @@ mpf_status mpf_get_mesh(mpf_ctx* ctx, float* verts, uint32_t* tris, int32_t wher
 /* Per-voxel case index (uint8, (nx-1)*(ny-1)*(nz-1) voxels over cell centres). */
 mpf_status mpf_get_mc_cases(mpf_ctx* ctx, uint8_t* cases, int32_t where);
 
-/* One full step in bubblesim order (spray.cpp:979-1233, north-star subset). */
+/* One full step in bubblesim order (spray.cpp:979-1233), the stages above only. */
 mpf_status mpf_step(mpf_ctx* ctx, int32_t frame, mpf_step_info* info);
 
 /* ---- adjoints ----------------------------------------------------------

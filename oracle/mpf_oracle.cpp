@@ -1274,7 +1274,7 @@ template <class T> struct Orc : OrcBase {
     return MPF_OK;
   }
 
-  // ================= composite step (spray.cpp:979-1233, north-star subset) =================
+  // ================= composite step (spray.cpp:979-1233, the stages of this path) =================
   int step(int frame, mpf_step_info* info) {
     mpf_step_info loc;
     std::memset(&loc, 0, sizeof(loc));
