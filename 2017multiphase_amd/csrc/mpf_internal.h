@@ -169,6 +169,7 @@ struct Ctx {
   float *tp = nullptr, *tpchange = nullptr, *tsave = nullptr;
   // iterative solve work vectors (sized for double)
   void *vx = nullptr, *vr = nullptr, *vp = nullptr, *vq = nullptr, *vp2 = nullptr;
+  void *vr2 = nullptr, *vq2 = nullptr;  // second r and q of the single-kernel iteration
   uint8_t* nbm = nullptr;  // per-cell neighbour mask of the current solve
   double* dslow = nullptr;  // 1/diag of the current solve's slow rows (flags-dependent)
   double* vx64 = nullptr;

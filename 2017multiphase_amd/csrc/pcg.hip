@@ -32,6 +32,8 @@
 // are dispatched round-robin over the 8 XCDs) walks one contiguous slab.
 #include "mpf_internal.h"
 
+#include <type_traits>
+
 namespace mpf {
 
 namespace {
@@ -814,10 +816,14 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
 // their open faces from there, slow rows (flags and level set) also take
 // their neighbours from LDS (0 off the unknowns).  Partials of npad blocks
 // (zeros past the launch), the count the spmv reads.
+//
+// For the single-kernel iteration (q given): also q.z and q.r with the true
+// residual, and the four sums (r.z, r.r, q.z, q.r) per block go to Pf.
 template <int OPk, bool SLOW>
 __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_resid64(OpArgs o, Tiling T, const float* __restrict__ b,
                                                 const double* __restrict__ x64, const float* __restrict__ x,
-                                                float* __restrict__ r, Parts P, SolveScalars* sc, int k, int npad) {
+                                                float* __restrict__ r, Parts P, SolveScalars* sc, int k, int npad,
+                                                const float* __restrict__ qf, double* __restrict__ Pf) {
   if (sc->done) return;
   __shared__ float dtab[8];
   if (threadIdx.x < 8) dtab[threadIdx.x] = (threadIdx.x < 7) ? dinv_tab<float>(o, threadIdx.x) : 1.0f;
@@ -910,7 +916,7 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_resid64(OpArgs o, Tiling T
   HM[0] = load_m(i_lo + 3, hok, hcol);
   const double bc = (OPk == OP_UNIT) ? 1.0 : o.il;
   const double ih2 = 1.0 / ((double)g.h * (double)g.h);
-  double a_rz = 0, a_rr = 0;
+  double a_rz = 0, a_rr = 0, a_qz = 0, a_qr = 0;
   for (int ib = i_lo; ib < i_hi; ib += UNR) {
 #pragma unroll
     for (int d = 0; d < UNR; ++d) {
@@ -974,17 +980,293 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_resid64(OpArgs o, Tiling T
                            : (SLOW && (nb & NB_SLOW)) ? dslow_at<float>(o, idx)
                                                       : dtab[__popc((unsigned)(nb & 63))];
           a_rz += (double)rf * (double)(rf * dv);
+          if (qf) {
+            const float qv = (qf + pb)[col[u]];  // every 20th iteration: a plain load
+            a_qz += (double)qv * (double)(rf * dv);
+            a_qr += (double)qv * (double)rf;
+          }
         }
       }
     }
   }
-  double v[2] = {a_rz, a_rr};
-  write_partials<2, ST>(v, uslot(P, k));
-  pad_partials2<ST>(uslot(P, k), T.nblk, npad);
+  if (qf) {
+    double v[4] = {a_rz, a_rr, a_qz, a_qr};
+    write_partials<4, ST>(v, Pf);
+  } else {
+    double v[2] = {a_rz, a_rr};
+    write_partials<2, ST>(v, uslot(P, k));
+    pad_partials2<ST>(uslot(P, k), T.nblk, npad);
+  }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc->refinements += 1;
     sc->last_true = k;
   }
+}
+
+// ---- single-kernel iteration (MIXED form, solves without slow rows) ----
+// One launch per iteration k does what update(k-1) and spmv(k) do: with
+// alpha_{k-1} = (r.z)_{k-1} / (p.q)_{k-1} it forms r_k = r_{k-1} - alpha q_{k-1},
+// advances x by alpha p_{k-1}, and forms p_k = z_k + beta p_{k-1} and
+// q_k = A p_k.  beta needs (r.z)_k before any r_k exists, so it is taken from
+// the sums of the previous launch, which hold every term of
+//   (r - a q).D(r - a q) = r.z - 2 a q.z + a^2 q.Dq   (and the same for r.r):
+// each launch sums r.z, r.r of the r_k it formed (exact for its vectors) and
+// p.q, q.z, q.Dq, q.r, q.q, so the expansion spans one iteration only.  r, q
+// and p alternate between two buffers (ring cells read the previous values
+// of a neighbouring tile).  first = 1: k = 0 (sums of the init kernel, U slot
+// 1); first = 2: a restart from a true residual (sums of the residual pass).
+// Pres (when not first): a true-residual pass ran after the previous launch;
+// its r.z, r.r, q.z, q.r replace those of the previous launch.
+constexpr int FDOT = 7;  // r.z, r.r, p.q, q.z, q.Dq, q.r, q.q
+template <int OPk>
+__global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, int nbu, const double* __restrict__ Pprev,
+                                               const double* __restrict__ Pres, double* __restrict__ Pout,
+                                               const float* __restrict__ rin, const float* __restrict__ qin,
+                                               const float* __restrict__ pin, float* __restrict__ rout,
+                                               float* __restrict__ qout, float* __restrict__ pout,
+                                               float* __restrict__ x, double* __restrict__ x64, int fold,
+                                               SolveScalars* sc, int k, double tol, int cap) {
+  using V = float;
+  if (sc->done) return;
+  __shared__ V dtab[8];
+  if (threadIdx.x < 8) dtab[threadIdx.x] = (threadIdx.x < 7) ? dinv_tab<V>(o, threadIdx.x) : V(1);
+  double ald = 0.0, rz, rr, be = 0.0, rz_prev = 0.0;
+  bool zero_pq = false, bad_pq = false;
+  if (first == 1) {
+    double v[2];
+    read_partials<2, ST>(Pres, nbu, v);
+    rz = v[0];
+    rr = v[1];
+  } else if (first == 2) {
+    double v[4];
+    read_partials<4, ST>(Pres, T.nblk, v);
+    rz = v[0];
+    rr = v[1];
+  } else {
+    double d[FDOT];
+    read_partials<FDOT, ST>(Pprev, T.nblk, d);
+    if (Pres) {
+      double e[4];
+      read_partials<4, ST>(Pres, T.nblk, e);
+      d[0] = e[0];
+      d[1] = e[1];
+      d[3] = e[2];
+      d[5] = e[3];
+    }
+    zero_pq = d[2] == 0.0;
+    bad_pq = bad(d[2]);
+    ald = zero_pq || bad_pq ? 0.0 : d[0] / d[2];
+    rz_prev = d[0];
+    rz = (d[0] - (2.0 * ald) * d[3]) + (ald * ald) * d[4];
+    rr = (d[1] - (2.0 * ald) * d[5]) + (ald * ald) * d[6];
+  }
+  const double bb = (first == 1 && k == 0) ? rr : sc->bb;
+  const bool nonfin = bad(rz) || bad(rr) || bad_pq;
+  const double rrc = rr > 0.0 ? rr : 0.0;
+  const bool conv = zero_pq || rrc == 0.0 || sqrt(rrc) <= tol * sqrt(bb);
+  const bool stop = conv || nonfin || k >= cap;
+  if (!first && !stop) be = rz / rz_prev;
+  const V al = (V)ald, bev = (V)be;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc->rz = rz;
+    sc->rr = rr;
+    sc->iter = k;
+    if (first == 1 && k == 0) sc->bb = bb;
+    if (stop) {
+      sc->done = 1;
+      sc->converged = conv && !nonfin;
+      sc->nonfinite = nonfin;
+    }
+  }
+  __shared__ V pl[3][LJ][LK];
+  const Grid& g = o.g;
+  int lb = blockIdx.x;
+  if ((T.nblk & 7) == 0) lb = (lb & 7) * (T.nblk >> 3) + (lb >> 3);  // XCD-contiguous tiles
+  const int tk = lb % T.ntk;
+  const int rest = lb / T.ntk;
+  const int tj = rest % T.ntj;
+  const int i_lo = (rest / T.ntj) * T.seg;
+  const int i_hi = min(i_lo + T.seg, g.nx);
+  const int j0 = tj * TJ, k0 = tk * TK;
+  const int tid = threadIdx.x;
+  const int kl = tid & (TK - 1), jl0 = (tid / TK) * RPT;
+  const int kk = k0 + kl;
+  const int64_t sx = (int64_t)g.ny * g.nz;
+  uint32_t col[RPT];
+  bool cok[RPT];
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    const int j = j0 + jl0 + u;
+    cok[u] = j < g.ny && kk < g.nz;
+    col[u] = cok[u] ? (uint32_t)(j * g.nz + kk) : 0u;
+  }
+  auto plane = [&](int i) -> int64_t { return (int64_t)min(max(i, 0), g.nx - 1) * sx; };
+  // x += alpha p_{k-1} on the unknowns (x64 += x + alpha p, x = 0 when folding)
+  auto advance_x = [&](int64_t idx, V pv) {
+    const V xv = x[idx];
+    if (fold) {
+      x64[idx] = x64[idx] + (double)(xv + al * pv);
+      x[idx] = V(0);
+    } else if (!first) {
+      x[idx] = xv + al * pv;
+    }
+  };
+  if (stop) {
+    if (nonfin || zero_pq) return;
+    for (int i = i_lo; i < i_hi; ++i)
+#pragma unroll
+      for (int u = 0; u < RPT; ++u) {
+        if (!cok[u]) continue;
+        const int64_t idx = (int64_t)i * sx + col[u];
+        if (o.nbm[idx] & NB_UNKNOWN) advance_x(idx, pin[idx]);
+      }
+    return;
+  }
+  int hj = 0, hk = 0;
+  const bool ring = tid < NRING;
+  if (tid < 2 * TK) {
+    hj = (tid < TK) ? -1 : TJ;
+    hk = tid & (TK - 1);
+  } else if (ring) {
+    const int t2 = tid - 2 * TK;
+    hj = t2 % TJ;
+    hk = (t2 < TJ) ? -1 : TK;
+  }
+  const bool hok = ring && (j0 + hj) >= 0 && (j0 + hj) < g.ny && (k0 + hk) >= 0 && (k0 + hk) < g.nz;
+  const uint32_t hcol = hok ? (uint32_t)((j0 + hj) * g.nz + (k0 + hk)) : 0u;
+  // loads as in the spmv: uniform plane base, 32-bit column, every lane
+  // issues every load (element 0 of the plane when there is nothing to read)
+  struct Rw {
+    V r, q, p, x;
+    uint8_t nb;
+  };
+  auto load_m = [&](int i, bool ok, uint32_t c) -> uint8_t {
+    const uint8_t* mp = o.nbm + plane(i);
+    return mp[(ok && i >= 0 && i < g.nx) ? c : 0u];
+  };
+  auto load_v = [&](int i, bool ok, uint8_t raw, uint32_t c, Rw& w, bool with_x) {
+    const uint8_t nb = (ok && i >= 0 && i < g.nx) ? raw : (uint8_t)0;
+    const int64_t pb = plane(i);
+    const uint32_t cv = (nb & NB_UNKNOWN) ? c : 0u;
+    w.nb = nb;
+    w.r = (rin + pb)[cv];
+    w.q = (qin + pb)[cv];
+    w.p = (pin + pb)[cv];
+    w.x = with_x ? (x + pb)[cv] : V(0);
+  };
+  // r_k, and p_k = z_k + beta p_{k-1} (0 off the unknowns)
+  auto r_of = [&](const Rw& w) -> V { return first ? w.r : w.r - al * w.q; };
+  auto d_of = [&](uint8_t nb) -> V { return o.precond ? dtab[__popc((unsigned)(nb & 63))] : V(1); };
+  auto p_of = [&](const Rw& w, V rk) -> V {
+    if (!(w.nb & NB_UNKNOWN)) return V(0);
+    return first ? rk * d_of(w.nb) : rk * d_of(w.nb) + bev * w.p;
+  };
+  constexpr int UNR = 6;
+  Rw R[3][RPT], HR[3];
+  uint8_t M[3][RPT], HM[3];
+  V PV[3][RPT], RV[3][RPT];  // p_k, r_k of planes i-1 .. i+1
+  uint8_t NB[3][RPT];
+  double a[FDOT] = {0, 0, 0, 0, 0, 0, 0};
+  // a cell of this block's planes once its r_k is formed: r_k and x stored,
+  // r.z and r.r summed
+  auto own = [&](int i, int u, const Rw& w, V rk) {
+    if (!(w.nb & NB_UNKNOWN)) return;
+    const int64_t pb = (int64_t)i * sx;
+    (rout + pb)[col[u]] = rk;
+    if (fold) {
+      const int64_t idx = pb + col[u];
+      x64[idx] = x64[idx] + (double)(w.x + al * w.p);
+      (x + pb)[col[u]] = V(0);
+    } else if (!first) {
+      (x + pb)[col[u]] = w.x + al * w.p;
+    }
+    const V zv = rk * d_of(w.nb);
+    a[0] += (double)rk * (double)zv;
+    a[1] += (double)rk * (double)rk;
+  };
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    Rw w0, w1;
+    load_v(i_lo - 1, cok[u], load_m(i_lo - 1, cok[u], col[u]), col[u], w0, false);
+    load_v(i_lo, cok[u], load_m(i_lo, cok[u], col[u]), col[u], w1, true);
+    PV[2][u] = p_of(w0, r_of(w0));
+    RV[0][u] = r_of(w1);
+    PV[0][u] = p_of(w1, RV[0][u]);
+    NB[0][u] = w1.nb;
+    pl[i_lo % 3][jl0 + u + 1][kl + 1] = PV[0][u];
+    if (i_lo < i_hi) own(i_lo, u, w1, RV[0][u]);
+  }
+  {
+    Rw w;
+    load_v(i_lo, hok, load_m(i_lo, hok, hcol), hcol, w, false);
+    if (ring) pl[i_lo % 3][hj + 1][hk + 1] = p_of(w, r_of(w));
+  }
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    load_v(i_lo + 1, cok[u], load_m(i_lo + 1, cok[u], col[u]), col[u], R[1][u], true);
+    load_v(i_lo + 2, cok[u], load_m(i_lo + 2, cok[u], col[u]), col[u], R[2][u], true);
+    M[0][u] = load_m(i_lo + 3, cok[u], col[u]);
+  }
+  load_v(i_lo + 1, hok, load_m(i_lo + 1, hok, hcol), hcol, HR[1], false);
+  load_v(i_lo + 2, hok, load_m(i_lo + 2, hok, hcol), hcol, HR[2], false);
+  HM[0] = load_m(i_lo + 3, hok, hcol);
+  const V qc = qcoef<V>(o);
+  for (int ib = i_lo; ib < i_hi; ib += UNR) {
+#pragma unroll
+    for (int d = 0; d < UNR; ++d) {
+      const int i = ib + d;
+      if (i < i_hi) {  // uniform over the block
+        const int s0 = d % 3, s1 = (d + 1) % 3, s2 = (d + 2) % 3;
+        const int sn = (i + 1) % 3, scur = i % 3;
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) M[s1][u] = load_m(i + 4, cok[u], col[u]);
+        HM[s1] = load_m(i + 4, hok, hcol);
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) load_v(i + 3, cok[u], M[s0][u], col[u], R[s0][u], true);
+        load_v(i + 3, hok, HM[s0], hcol, HR[s0], false);
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) {
+          RV[s1][u] = r_of(R[s1][u]);
+          PV[s1][u] = p_of(R[s1][u], RV[s1][u]);
+          NB[s1][u] = R[s1][u].nb;
+          pl[sn][jl0 + u + 1][kl + 1] = PV[s1][u];
+          if (i + 1 < i_hi) own(i + 1, u, R[s1][u], RV[s1][u]);
+        }
+        if (ring) pl[sn][hj + 1][hk + 1] = p_of(HR[s1], r_of(HR[s1]));
+        lds_barrier();
+        const int64_t pb = (int64_t)i * sx;
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) {
+          const int jl = jl0 + u;
+          const uint8_t nb = NB[s0][u];
+          const V pcv = PV[s0][u];
+          const V nyp = pl[scur][jl + 2][kl + 1], nzp = pl[scur][jl + 1][kl + 2];
+          const V nym = pl[scur][jl][kl + 1], nzm = pl[scur][jl + 1][kl];
+          V sum = V(0);
+          sum = (nb & 1) ? sum + (PV[s1][u] - pcv) : sum;
+          sum = (nb & 2) ? sum + (nyp - pcv) : sum;
+          sum = (nb & 4) ? sum + (nzp - pcv) : sum;
+          sum = (nb & 8) ? sum + (PV[s2][u] - pcv) : sum;
+          sum = (nb & 16) ? sum + (nym - pcv) : sum;
+          sum = (nb & 32) ? sum + (nzm - pcv) : sum;
+          if (nb & NB_UNKNOWN) {
+            const V qv = sum * qc;
+            const V rk = RV[s0][u];
+            const V dv = d_of(nb);
+            const V zv = rk * dv;
+            (pout + pb)[col[u]] = pcv;
+            (qout + pb)[col[u]] = qv;
+            a[2] += (double)pcv * (double)qv;
+            a[3] += (double)qv * (double)zv;
+            a[4] += (double)qv * (double)(dv * qv);
+            a[5] += (double)qv * (double)rk;
+            a[6] += (double)qv * (double)qv;
+          }
+        }
+      }
+    }
+  }
+  write_partials<FDOT, ST>(a, Pout);
 }
 
 // MIXED: x64 += x at exit.
@@ -1029,6 +1311,13 @@ __global__ void __launch_bounds__(PB) k_collect_rr(const double* __restrict__ S,
   double t[1];
   read_partials<1>(S, nb, t);
   if (threadIdx.x == 0) sc->rr_true = t[0];
+}
+
+// r.r of a residual pass with four sums per block (the single-kernel form).
+__global__ void __launch_bounds__(PB) k_collect_rr4(const double* __restrict__ S, int nb, SolveScalars* sc) {
+  double t[4];
+  read_partials<4>(S, nb, t);
+  if (threadIdx.x == 0) sc->rr_true = t[1];
 }
 
 __global__ void k_clear_done(SolveScalars* sc) {
@@ -1138,7 +1427,9 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
   int nsb = (int)((nslow + PB - 1) / PB);
   auto spmv = nslow > 0 ? k_spmv<V, Dacc, OPk, true> : k_spmv<V, Dacc, OPk, false>;
   auto resid64 = nslow > 0 ? k_resid64<OPk, true> : k_resid64<OPk, false>;
-  const Tiling T = make_tiling(c.g, (sizeof(V) == 4 && nslow == 0) ? 768 : 512);
+  // the MIXED form without slow rows runs one kernel per iteration (k_cgf)
+  const bool fused = std::is_same<V, float>::value && mixed && !ref && nslow == 0;
+  const Tiling T = make_tiling(c.g, fused ? 512 : ((sizeof(V) == 4 && nslow == 0) ? 768 : 512));
   // U-slot partials: update and init write L.nblk, the residual pass T.nblk;
   // both pad to npad, the count the spmv sums
   const int npad = L.nblk > T.nblk ? L.nblk : T.nblk;
@@ -1151,55 +1442,105 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
          mixed ? x64 : (double*)nullptr, P, c.scal, npad);
   int k = 0;
   bool fold_next = false;
-  // x advances every other iteration (update modes: see k_update); half[k]
-  // records that update k left alpha_k p_k pending in x
-  std::vector<char> half((size_t)cap + 2, 0);
-  auto flush = [&](int kd) {  // x += alpha_{kd-1} p_{kd-1} if update kd-1 left it pending
-    if (kd < 1 || !half[kd - 1]) return;
-    launch(c, "pcg_flush_x", dim3(L.nblk), dim3(PB), 0, k_flush_x<V>, L, G, (const uint8_t*)c.nbm, x,
-           (const V*)pbuf[kd & 1], (const SolveScalars*)c.scal, (kd - 1) & 1);
-    half[kd - 1] = 0;
-  };
-  while (k <= cap) {
-    const int k_end = k + check;
-    for (; k < k_end && k <= cap; ++k) {
-      launch(c, "pcg_spmv", dim3(T.nblk), dim3(ST), 0, spmv, o, T, npad, (const V*)r, (const V*)pbuf[k & 1],
-             pbuf[(k + 1) & 1], q, P, c.scal, k, (int)ref, eps_abs, tol, cap);
-      if (nsb > 0)
-        launch(c, "pcg_slow_q", dim3(nsb), dim3(PB), 0, k_slow_q<V, Dacc, OPk>, o, (const uint32_t*)slow_list, nslow,
-               (const V*)pbuf[(k + 1) & 1], q, P.S2, c.scal);
-      const bool resid_next = mixed && (k + 1) % R == 0;
-      const bool h = (k % 2 == 0) && !resid_next && !fold_next;
-      const int mode = h ? 0 : ((k > 0 && half[k - 1]) ? 2 : 1);
-      half[k] = h;
-      launch(c, "pcg_update", dim3(L.nblk), dim3(PB), 0, k_update<V, Dacc, OPk>, o, L, x, r,
-             (const V*)pbuf[(k + 1) & 1], (const V*)pbuf[k & 1], (const V*)q, mixed ? x64 : (double*)nullptr, P,
-             T.nblk, nsb, c.scal, k, (int)fold_next, mode, npad);
-      fold_next = false;
-      if (mixed && (k + 1) % R == 0) {
-        launch(c, "pcg_resid64", dim3(T.nblk), dim3(ST), 0, resid64, o, T, (const float*)b,
-               (const double*)x64, (const float*)x, (float*)r, P, c.scal, k, npad);
-        fold_next = true;
+  if constexpr (std::is_same<V, float>::value) {
+    if (fused) {
+      float* rb[2] = {(float*)c.vr, (float*)c.vr2};
+      float* qb[2] = {(float*)c.vq, (float*)c.vq2};
+      float* pf[2] = {(float*)c.vp, (float*)c.vp2};
+      double* S7[2] = {c.partials + 8 * MAX_BLK, c.partials + 15 * MAX_BLK};
+      double* Rs = c.partials + 22 * MAX_BLK;
+      int first = 1, fold = 0;
+      bool use_res = false;
+      while (k <= cap) {
+        const int k_end = k + check;
+        for (; k < k_end && k <= cap; ++k) {
+          const double* pres = first == 1 ? (const double*)P.U1 : (first == 2 || use_res) ? Rs : nullptr;
+          launch(c, "pcg_cg_iter", dim3(T.nblk), dim3(ST), 0, k_cgf<OPk>, o, T, first, npad,
+                 (const double*)S7[(k + 1) & 1], pres, S7[k & 1], (const float*)rb[k & 1], (const float*)qb[k & 1],
+                 (const float*)pf[k & 1], rb[(k + 1) & 1], qb[(k + 1) & 1], pf[(k + 1) & 1], (float*)x, x64, fold,
+                 c.scal, k, tol, cap);
+          first = 0;
+          fold = 0;
+          use_res = false;
+          if ((k + 1) % R == 0) {
+            launch(c, "pcg_resid64", dim3(T.nblk), dim3(ST), 0, resid64, o, T, (const float*)b, (const double*)x64,
+                   (const float*)x, rb[(k + 1) & 1], P, c.scal, k, npad, (const float*)qb[(k + 1) & 1], Rs);
+            use_res = true;
+            fold = 1;
+          }
+        }
+        if (!read_scalars(c)) return MPF_HIP_ERROR;
+        const SolveScalars& s = *c.scal_host;
+        if (!s.done) continue;
+        if (!s.converged || s.nonfinite) break;
+        // launch kd stopped and advanced x; convergence stands only on the
+        // FP64 true residual of that x, else the iteration restarts from it
+        const int kd = s.iter;
+        launch(c, "pcg_clear", dim3(1), dim3(64), 0, k_clear_done, c.scal);
+        launch(c, "pcg_resid64", dim3(T.nblk), dim3(ST), 0, resid64, o, T, (const float*)b, (const double*)x64,
+               (const float*)x, rb[kd & 1], P, c.scal, kd, npad, (const float*)qb[kd & 1], Rs);
+        launch(c, "pcg_collect", dim3(1), dim3(PB), 0, k_collect_rr4, (const double*)Rs, T.nblk, c.scal);
+        if (!read_scalars(c)) return MPF_HIP_ERROR;
+        const SolveScalars& t = *c.scal_host;
+        if (t.rr_true == 0.0 || sqrt(t.rr_true) <= tol * sqrt(t.bb) || kd >= cap) break;
+        first = 2;
+        fold = 1;
+        k = kd;
       }
     }
-    if (!read_scalars(c)) return MPF_HIP_ERROR;
-    const SolveScalars& s = *c.scal_host;
-    if (!s.done) continue;
-    // updates after s.iter - 1 returned without running: x takes what the
-    // last one that ran left pending
-    flush(s.iter);
-    if (!mixed || !s.converged || s.nonfinite) break;
-    // spmv(kd) met the criterion on the residual of update(kd-1).  Which
-    // true-residual pass last ran is read from the device: passes launched
-    // after the solve finished returned without running.
-    const int kd = s.iter;
-    if (kd == 0 || s.last_true == kd - 1) break;
-    // resid64 returns early while done is set: clear it first
-    launch(c, "pcg_clear", dim3(1), dim3(64), 0, k_clear_done, c.scal);
-    launch(c, "pcg_resid64", dim3(T.nblk), dim3(ST), 0, resid64, o, T, (const float*)b, (const double*)x64,
-           (const float*)x, (float*)r, P, c.scal, kd - 1, npad);
-    fold_next = true;
-    k = kd;
+  }
+  if (!fused) {
+    // x advances every other iteration (update modes: see k_update); half[k]
+    // records that update k left alpha_k p_k pending in x
+    std::vector<char> half((size_t)cap + 2, 0);
+    auto flush = [&](int kd) {  // x += alpha_{kd-1} p_{kd-1} if update kd-1 left it pending
+      if (kd < 1 || !half[kd - 1]) return;
+      launch(c, "pcg_flush_x", dim3(L.nblk), dim3(PB), 0, k_flush_x<V>, L, G, (const uint8_t*)c.nbm, x,
+             (const V*)pbuf[kd & 1], (const SolveScalars*)c.scal, (kd - 1) & 1);
+      half[kd - 1] = 0;
+    };
+    while (k <= cap) {
+      const int k_end = k + check;
+      for (; k < k_end && k <= cap; ++k) {
+        launch(c, "pcg_spmv", dim3(T.nblk), dim3(ST), 0, spmv, o, T, npad, (const V*)r, (const V*)pbuf[k & 1],
+               pbuf[(k + 1) & 1], q, P, c.scal, k, (int)ref, eps_abs, tol, cap);
+        if (nsb > 0)
+          launch(c, "pcg_slow_q", dim3(nsb), dim3(PB), 0, k_slow_q<V, Dacc, OPk>, o, (const uint32_t*)slow_list, nslow,
+                 (const V*)pbuf[(k + 1) & 1], q, P.S2, c.scal);
+        const bool resid_next = mixed && (k + 1) % R == 0;
+        const bool h = (k % 2 == 0) && !resid_next && !fold_next;
+        const int mode = h ? 0 : ((k > 0 && half[k - 1]) ? 2 : 1);
+        half[k] = h;
+        launch(c, "pcg_update", dim3(L.nblk), dim3(PB), 0, k_update<V, Dacc, OPk>, o, L, x, r,
+               (const V*)pbuf[(k + 1) & 1], (const V*)pbuf[k & 1], (const V*)q, mixed ? x64 : (double*)nullptr, P,
+               T.nblk, nsb, c.scal, k, (int)fold_next, mode, npad);
+        fold_next = false;
+        if (mixed && (k + 1) % R == 0) {
+          launch(c, "pcg_resid64", dim3(T.nblk), dim3(ST), 0, resid64, o, T, (const float*)b,
+                 (const double*)x64, (const float*)x, (float*)r, P, c.scal, k, npad, (const float*)nullptr,
+                 (double*)nullptr);
+          fold_next = true;
+        }
+      }
+      if (!read_scalars(c)) return MPF_HIP_ERROR;
+      const SolveScalars& s = *c.scal_host;
+      if (!s.done) continue;
+      // updates after s.iter - 1 returned without running: x takes what the
+      // last one that ran left pending
+      flush(s.iter);
+      if (!mixed || !s.converged || s.nonfinite) break;
+      // spmv(kd) met the criterion on the residual of update(kd-1).  Which
+      // true-residual pass last ran is read from the device: passes launched
+      // after the solve finished returned without running.
+      const int kd = s.iter;
+      if (kd == 0 || s.last_true == kd - 1) break;
+      // resid64 returns early while done is set: clear it first
+      launch(c, "pcg_clear", dim3(1), dim3(64), 0, k_clear_done, c.scal);
+      launch(c, "pcg_resid64", dim3(T.nblk), dim3(ST), 0, resid64, o, T, (const float*)b, (const double*)x64,
+             (const float*)x, (float*)r, P, c.scal, kd - 1, npad, (const float*)nullptr, (double*)nullptr);
+      fold_next = true;
+      k = kd;
+    }
   }
   if (mixed) {
     launch(c, "pcg_fold", dim3(L.nblk), dim3(PB), 0, k_final_fold, L, G, (const uint8_t*)c.nbm, (float*)x, x64);
