@@ -40,6 +40,9 @@ KERNEL_BYTES = {
     "pcg_update": (1, 20, 0),
     # mask 1 per cell; b 4, x64 8, x 4 read, r 4 written per unknown
     "pcg_resid64": (1, 20, 0),
+    # one-kernel iteration (MIXED form without slow rows): mask 1 per cell;
+    # r, q, p_{k-1}, x read and r, q, p_k, x written per unknown
+    "pcg_cg_iter": (1, 32, 0),
     # start/end 8, mask 1, ls fluid+air 8 written; pos 12 + flag 1 per particle
     "genlevelset": (17, 0, 13),
     # one launch, 3 axes: start/end 8, mask 1, 3 faces 12 written; pos 12, vel 12, mass 4, flag 1
