@@ -170,18 +170,25 @@ __device__ __forceinline__ void solvedistance(float a, float b, float c, float& 
 __global__ void k_sweep(Grid g, const float* __restrict__ in, float* __restrict__ out, const int8_t* __restrict__ mark,
                         int typeflag) {
   CELL_LOOP_HEAD
+  // the cell, its mark and its six neighbours (x-, x+, y-, y+, z-, z+) are
+  // loaded together (a neighbour outside the grid reads the cell itself and
+  // is replaced by infinity)
+  const bool vx0 = i > 0, vx1 = i < g.nx - 1, vy0 = j > 0, vy1 = j < g.ny - 1, vz0 = k > 0, vz1 = k < g.nz - 1;
+  const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz;
   float res = in[idx];
-  if (!(typeflag >= 0 && mark[idx] == typeflag)) {
-    // the six neighbours (x-, x+, y-, y+, z-, z+); an update needs one below
-    // the current value (every candidate is at least min(a, b, c)), so cells
-    // without one keep their value without evaluating the 8 triples
-    const bool vx0 = i > 0, vx1 = i < g.nx - 1, vy0 = j > 0, vy1 = j < g.ny - 1, vz0 = k > 0, vz1 = k < g.nz - 1;
-    const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz;
-    const float nx0 = vx0 ? in[idx - sx] : INFINITY, nx1 = vx1 ? in[idx + sx] : INFINITY;
-    const float ny0 = vy0 ? in[idx - sy] : INFINITY, ny1 = vy1 ? in[idx + sy] : INFINITY;
-    const float nz0 = vz0 ? in[idx - 1] : INFINITY, nz1 = vz1 ? in[idx + 1] : INFINITY;
-    const float m = fminf(fminf(fminf(nx0, nx1), fminf(ny0, ny1)), fminf(nz0, nz1));
-    if (m < res) {
+  const int8_t m = (typeflag >= 0) ? mark[idx] : (int8_t)0;
+  const float lx0 = in[vx0 ? idx - sx : idx], lx1 = in[vx1 ? idx + sx : idx];
+  const float ly0 = in[vy0 ? idx - sy : idx], ly1 = in[vy1 ? idx + sy : idx];
+  const float lz0 = in[vz0 ? idx - 1 : idx], lz1 = in[vz1 ? idx + 1 : idx];
+  if (!(typeflag >= 0 && m == typeflag)) {
+    const float nx0 = vx0 ? lx0 : INFINITY, nx1 = vx1 ? lx1 : INFINITY;
+    const float ny0 = vy0 ? ly0 : INFINITY, ny1 = vy1 ? ly1 : INFINITY;
+    const float nz0 = vz0 ? lz0 : INFINITY, nz1 = vz1 ? lz1 : INFINITY;
+    // an update needs a neighbour below the current value (every candidate
+    // is at least min(a, b, c)), so cells without one keep their value
+    // without evaluating the 8 triples
+    const float mn = fminf(fminf(fminf(nx0, nx1), fminf(ny0, ny1)), fminf(nz0, nz1));
+    if (mn < res) {
       for (int di = -1; di <= 1; di += 2)
         for (int dj = -1; dj <= 1; dj += 2)
           for (int dk = -1; dk <= 1; dk += 2)
@@ -534,54 +541,80 @@ __global__ void k_sweep_u(Grid g, const float* __restrict__ u, float* __restrict
   if (f >= nf) return;
   int i, j, k;
   ijk_face<A>(g, f, i, j, k);
-  float res = u[f];
   const int n[3] = {i, j, k};
   const int lim = (A == 0) ? g.nx : (A == 1) ? g.ny : g.nz;
-  if (n[A] > 1 && n[A] < lim - 1) {
-    const int bi = i - (A == 0), bj = j - (A == 1), bk = k - (A == 2);
-    if (mark[cid(g, i, j, k)] != sf && mark[cid(g, bi, bj, bk)] != sf) {
-      const float lsc = ls[cid(g, i, j, k)], lsb = ls[cid(g, bi, bj, bk)];
-      constexpr int t1 = (A == 0) ? 1 : (A == 1) ? 0 : 1;
-      constexpr int t2 = (A == 0) ? 2 : (A == 1) ? 2 : 0;
-      const int lims[3] = {g.nx, g.ny, g.nz};
-      // The reference visits the 8 octants (di, dj, dk in {-1, +1}, di
-      // outermost) and the last one whose three weights are all >= 0 sets
-      // the value.  Each weight depends on its own axis' sign only, so that
-      // octant takes, per axis, +1 if its weight passes there, else -1.
-      const float dl = lsc - lsb;
-      const float wa[2] = {(float)(1) * dl, (float)(-1) * dl};  // sign -1, +1
-      float wt[2][2];
-      bool ok[3][2];
-      ok[A][0] = !(wa[0] < 0);
-      ok[A][1] = !(wa[1] < 0);
+  const bool inner = n[A] > 1 && n[A] < lim - 1;
+  // first round of loads: the face, the marks and level sets of its two cells
+  // (cell indices clamped for the faces the rule leaves as they are)
+  const int ci = min(i, g.nx - 1), cj = min(j, g.ny - 1), ck = min(k, g.nz - 1);
+  const int bi = max(i - (A == 0), 0), bj = max(j - (A == 1), 0), bk = max(k - (A == 2), 0);
+  float res = u[f];
+  const int8_t mc = mark[cid(g, ci, cj, ck)], mb = mark[cid(g, bi, bj, bk)];
+  const float lsc = ls[cid(g, ci, cj, ck)], lsb = ls[cid(g, bi, bj, bk)];
+  if (inner && mc != sf && mb != sf) {
+    constexpr int t1 = (A == 0) ? 1 : (A == 1) ? 0 : 1;
+    constexpr int t2 = (A == 0) ? 2 : (A == 1) ? 2 : 0;
+    const int lims[3] = {g.nx, g.ny, g.nz};
+    // second round: the level sets beside both cells across each transverse
+    // axis and the face's six neighbours, all issued together (indices
+    // clamped; a value outside the grid is not used)
+    float lt[2][2][2];  // [transverse t][sign][cell c / cell b]
+    bool in[2][2];
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int ax = t == 0 ? t1 : t2;
+    for (int t = 0; t < 2; ++t) {
+      const int ax = t == 0 ? t1 : t2;
 #pragma unroll
-        for (int sg = 0; sg < 2; ++sg) {
-          const int dd = sg ? 1 : -1;
-          int o[3] = {0, 0, 0};
-          o[ax] = dd;
-          const bool in = n[ax] + dd >= 0 && n[ax] + dd <= lims[ax] - 1;
-          float w = 0.0f;
-          if (in) w = (lsc + lsb - ls[cid(g, i + o[0], j + o[1], k + o[2])] - ls[cid(g, bi + o[0], bj + o[1], bk + o[2])]) * 0.5f;
-          wt[t][sg] = w;
-          ok[ax][sg] = in && !(w < 0);
-        }
+      for (int sg = 0; sg < 2; ++sg) {
+        const int dd = sg ? 1 : -1;
+        in[t][sg] = n[ax] + dd >= 0 && n[ax] + dd <= lims[ax] - 1;
+        int o[3] = {0, 0, 0};
+        o[ax] = in[t][sg] ? dd : 0;
+        lt[t][sg][0] = ls[cid(g, i + o[0], j + o[1], k + o[2])];
+        lt[t][sg][1] = ls[cid(g, i - (A == 0) + o[0], j - (A == 1) + o[1], k - (A == 2) + o[2])];
       }
-      if ((ok[0][0] || ok[0][1]) && (ok[1][0] || ok[1][1]) && (ok[2][0] || ok[2][1])) {
-        const int sgn[3] = {ok[0][1] ? 1 : 0, ok[1][1] ? 1 : 0, ok[2][1] ? 1 : 0};
-        float wv[3];
-        wv[A] = wa[sgn[A]];
-        wv[t1] = wt[0][sgn[t1]];
-        wv[t2] = wt[1][sgn[t2]];
-        const int di = sgn[0] ? 1 : -1, dj = sgn[1] ? 1 : -1, dk = sgn[2] ? 1 : -1;
-        float wx = wv[0], wy = wv[1], wz = wv[2];
-        float wsum = wx + wy + wz;
-        if (wsum == 0) wx = wy = wz = 1.0f / 3;
-        else { wx /= wsum; wy /= wsum; wz /= wsum; }
-        res = wx * u[fid(g, A, i + di, j, k)] + wy * u[fid(g, A, i, j + dj, k)] + wz * u[fid(g, A, i, j, k + dk)];
+    }
+    float un[3][2];  // u of the face's neighbours along x, y, z (sign -1, +1)
+    const int fl[3] = {(A == 0) ? g.nx : g.nx - 1, (A == 1) ? g.ny : g.ny - 1, (A == 2) ? g.nz : g.nz - 1};
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int sg = 0; sg < 2; ++sg) {
+        int q[3] = {i, j, k};
+        q[d] = min(max(q[d] + (sg ? 1 : -1), 0), fl[d]);
+        un[d][sg] = u[fid(g, A, q[0], q[1], q[2])];
       }
+    // The reference visits the 8 octants (di, dj, dk in {-1, +1}, di
+    // outermost) and the last one whose three weights are all >= 0 sets the
+    // value.  Each weight depends on its own axis' sign only, so that octant
+    // takes, per axis, +1 if its weight passes there, else -1.
+    const float dl = lsc - lsb;
+    const float wa[2] = {(float)(1) * dl, (float)(-1) * dl};  // sign -1, +1
+    float wt[2][2];
+    bool ok[3][2];
+    ok[A][0] = !(wa[0] < 0);
+    ok[A][1] = !(wa[1] < 0);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int ax = t == 0 ? t1 : t2;
+#pragma unroll
+      for (int sg = 0; sg < 2; ++sg) {
+        float w = 0.0f;
+        if (in[t][sg]) w = (lsc + lsb - lt[t][sg][0] - lt[t][sg][1]) * 0.5f;
+        wt[t][sg] = w;
+        ok[ax][sg] = in[t][sg] && !(w < 0);
+      }
+    }
+    if ((ok[0][0] || ok[0][1]) && (ok[1][0] || ok[1][1]) && (ok[2][0] || ok[2][1])) {
+      const int sgn[3] = {ok[0][1] ? 1 : 0, ok[1][1] ? 1 : 0, ok[2][1] ? 1 : 0};
+      float wv[3];
+      wv[A] = wa[sgn[A]];
+      wv[t1] = wt[0][sgn[t1]];
+      wv[t2] = wt[1][sgn[t2]];
+      float wx = wv[0], wy = wv[1], wz = wv[2];
+      float wsum = wx + wy + wz;
+      if (wsum == 0) wx = wy = wz = 1.0f / 3;
+      else { wx /= wsum; wy /= wsum; wz /= wsum; }
+      res = wx * un[0][sgn[0]] + wy * un[1][sgn[1]] + wz * un[2][sgn[2]];
     }
   }
   out[f] = res;
