@@ -896,10 +896,14 @@ __global__ void k_heat_jacobi(Grid g, const float* __restrict__ x, float* __rest
   const unsigned mask = cd >> 3;
   const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz;
   const int64_t st[6] = {sx, sy, 1, -sx, -sy, -1};
+  // the six neighbour loads issued together (a closed face reads the cell)
+  float xv[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) xv[q] = x[(mask & (1u << q)) ? idx + st[q] : idx];
   double s = (double)b[idx];
 #pragma unroll
   for (int q = 0; q < 6; ++q)
-    if (mask & (1u << q)) s += (double)x[idx + st[q]];
+    if (mask & (1u << q)) s += (double)xv[q];
   xn[idx] = (float)(s / dgt[cd & 7u][6 - __popc(mask)]);
 }
 
