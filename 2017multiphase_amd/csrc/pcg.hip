@@ -888,6 +888,7 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_resid64(OpArgs o, Tiling T
   uint8_t M[3][RPT], HM[3];
   double XV[3][RPT];
   float BV[3][RPT];
+  float QN[2][RPT];  // q of plane i (slot d&1), loaded one step ahead (single-kernel form only)
   uint8_t NB[3][RPT];
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
@@ -917,6 +918,8 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_resid64(OpArgs o, Tiling T
   const double bc = (OPk == OP_UNIT) ? 1.0 : o.il;
   const double ih2 = 1.0 / ((double)g.h * (double)g.h);
   double a_rz = 0, a_rr = 0, a_qz = 0, a_qr = 0;
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) QN[0][u] = qf ? (qf + plane(i_lo))[col[u]] : 0.0f;
   for (int ib = i_lo; ib < i_hi; ib += UNR) {
 #pragma unroll
     for (int d = 0; d < UNR; ++d) {
@@ -930,6 +933,8 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_resid64(OpArgs o, Tiling T
 #pragma unroll
         for (int u = 0; u < RPT; ++u) load_v(i + 3, cok[u], M[s0][u], col[u], R[s0][u]);
         load_v(i + 3, hok, HM[s0], hcol, HR[s0]);
+#pragma unroll
+        for (int u = 0; u < RPT; ++u) QN[(d + 1) & 1][u] = qf ? (qf + plane(i + 1))[col[u]] : 0.0f;
 #pragma unroll
         for (int u = 0; u < RPT; ++u) {
           XV[s1][u] = form(R[s1][u]);
@@ -981,7 +986,7 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_resid64(OpArgs o, Tiling T
                                                       : dtab[__popc((unsigned)(nb & 63))];
           a_rz += (double)rf * (double)(rf * dv);
           if (qf) {
-            const float qv = (qf + pb)[col[u]];  // every 20th iteration: a plain load
+            const float qv = QN[d & 1][u];
             a_qz += (double)qv * (double)(rf * dv);
             a_qr += (double)qv * (double)rf;
           }
