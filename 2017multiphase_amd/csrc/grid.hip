@@ -159,11 +159,21 @@ __global__ void k_ls_back(int64_t G, float* __restrict__ ls, float h, int mode) 
   ls[idx] = (mode == 0) ? ls[idx] * h : -ls[idx] * h;
 }
 
-// solvedistance spray_k.cu:746-754
-__device__ __forceinline__ void solvedistance(float a, float b, float c, float& x) {
-  float d = fminf(a, fminf(b, c)) + 1.0f;
+// One triple of solvedistance (spray_k.cu:746-754: d = min(a,b,c) + 1, or
+// (a + b + c + sqrt(3 - (a-b)^2 - (a-c)^2 - (b-c)^2)) / 3 when that d exceeds
+// max(a,b,c); x = min(x, d)) with the division of its three-axis branch left
+// out: those candidates enter x only through a minimum and x/3 rounded
+// to nearest is non-decreasing in x, so the smallest numerator s divided once
+// at the end (solvedistance_end) gives the same float as dividing each.
+__device__ __forceinline__ void solvedistance_acc(float a, float b, float c, float& x, float& s) {
+  const float d = fminf(a, fminf(b, c)) + 1.0f;
   if (d > fmaxf(a, fmaxf(b, c)))
-    d = (a + b + c + sqrtf(3.0f - (a - b) * (a - b) - (a - c) * (a - c) - (b - c) * (b - c))) / 3.0f;
+    s = fminf(s, a + b + c + sqrtf(3.0f - (a - b) * (a - b) - (a - c) * (a - c) - (b - c) * (b - c)));
+  else if (d < x)
+    x = d;
+}
+__device__ __forceinline__ void solvedistance_end(float s, float& x) {
+  const float d = s / 3.0f;  // infinity when no triple took the branch
   if (d < x) x = d;
 }
 // sweepphibytype (spray_k.cu:771-789) / sweepphi (755-770, typeflag < 0), two buffers
@@ -189,11 +199,13 @@ __global__ void k_sweep(Grid g, const float* __restrict__ in, float* __restrict_
     // without evaluating the 8 triples
     const float mn = fminf(fminf(fminf(nx0, nx1), fminf(ny0, ny1)), fminf(nz0, nz1));
     if (mn < res) {
+      float s3 = INFINITY;
       for (int di = -1; di <= 1; di += 2)
         for (int dj = -1; dj <= 1; dj += 2)
           for (int dk = -1; dk <= 1; dk += 2)
             if ((di < 0 ? vx0 : vx1) && (dj < 0 ? vy0 : vy1) && (dk < 0 ? vz0 : vz1))
-              solvedistance(di < 0 ? nx0 : nx1, dj < 0 ? ny0 : ny1, dk < 0 ? nz0 : nz1, res);
+              solvedistance_acc(di < 0 ? nx0 : nx1, dj < 0 ? ny0 : ny1, dk < 0 ? nz0 : nz1, res, s3);
+      solvedistance_end(s3, res);
     }
   }
   out[idx] = res;
