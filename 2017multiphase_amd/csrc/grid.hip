@@ -403,20 +403,21 @@ __global__ void __launch_bounds__(ST_CELLS)
     const float RE = 1.4f, RE2 = RE * RE;
     float X[SCH], Y[SCH], Z[SCH], VX[SCH], VY[SCH], VZ[SCH], M[SCH];
     const uint32_t n = e - s;
+    // every slot loads (an empty slot its cell's last particle, or particle
+    // 0 in an empty cell of a tile that has particles), so the loads of all
+    // slots are in flight together; empty slots and other phases weigh 0
+    // (w = 0 * max(...) adds +-0 to the sums)
 #pragma unroll
     for (int q = 0; q < SCH; ++q) {
-      X[q] = Y[q] = Z[q] = VX[q] = VY[q] = VZ[q] = M[q] = 0.0f;
-      if ((uint32_t)q < n) {
-        const uint32_t p = s + q;
-        const int8_t fl = flag[p];
-        const f3 x = pos[p], v = vel[p];
-        X[q] = (x.x - g.gx0) * scale;
-        Y[q] = (x.y - g.gy0) * scale;
-        Z[q] = (x.z - g.gz0) * scale;
-        VX[q] = v.x; VY[q] = v.y; VZ[q] = v.z;
-        // other phases weigh 0 (w = 0 * max(...) adds +-0 to the sums)
-        M[q] = (fl == FLUID || fl == SOLID) ? mass[p] : 0.0f;
-      }
+      const uint32_t p = (n > 0) ? s + min((uint32_t)q, n - 1) : 0u;
+      const int8_t fl = flag[p];
+      const f3 x = pos[p], v = vel[p];
+      const float m = mass[p];
+      X[q] = (x.x - g.gx0) * scale;
+      Y[q] = (x.y - g.gy0) * scale;
+      Z[q] = (x.z - g.gz0) * scale;
+      VX[q] = v.x; VY[q] = v.y; VZ[q] = v.z;
+      M[q] = ((uint32_t)q < n && (fl == FLUID || fl == SOLID)) ? m : 0.0f;
     }
     for (int ph = 0; ph < 27; ++ph) {
       const int o[3] = {ph / 9 - 1, (ph / 3) % 3 - 1, ph % 3 - 1};
@@ -770,17 +771,25 @@ __global__ void __launch_bounds__(ST_CELLS)
     const float RE = 1.4f, RE2 = RE * RE;
     float X[SCH], Y[SCH], Z[SCH], TT[SCH], ON[SCH];
     const uint32_t n = e - s;
+    // every slot loads (an empty slot its cell's last particle, or particle
+    // 0 in an empty cell of a tile that has particles), so the loads of all
+    // slots are in flight together; empty slots weigh 0 (adds +-0)
+    f3 xr[SCH];
 #pragma unroll
     for (int q = 0; q < SCH; ++q) {
-      X[q] = Y[q] = Z[q] = TT[q] = ON[q] = 0.0f;
-      if ((uint32_t)q < n) {
-        const f3 x = pos[s + q];
-        X[q] = (x.x - g.gx0) * scale;
-        Y[q] = (x.y - g.gy0) * scale;
-        Z[q] = (x.z - g.gz0) * scale;
-        TT[q] = temp[s + q];
-        ON[q] = 1.0f;  // empty slots weigh 0 (adds +-0)
-      }
+      const uint32_t p = (n > 0) ? s + min((uint32_t)q, n - 1) : 0u;
+      xr[q] = pos[p];
+      TT[q] = temp[p];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of their uses
+#pragma unroll
+    for (int q = 0; q < SCH; ++q) {
+      const bool on = (uint32_t)q < n;
+      X[q] = on ? (xr[q].x - g.gx0) * scale : 0.0f;
+      Y[q] = on ? (xr[q].y - g.gy0) * scale : 0.0f;
+      Z[q] = on ? (xr[q].z - g.gz0) * scale : 0.0f;
+      TT[q] = on ? TT[q] : 0.0f;
+      ON[q] = on ? 1.0f : 0.0f;
     }
     for (int ph = 0; ph < 27; ++ph) {
       const int f0 = ci - (ph / 9 - 1), f1 = cj - ((ph / 3) % 3 - 1), f2 = ck - (ph % 3 - 1);
