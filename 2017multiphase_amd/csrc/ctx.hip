@@ -109,7 +109,7 @@ void free_all(Ctx& c) {
     dfree(c.temp[b]); dfree(c.lheat[b]); dfree(c.solub[b]); dfree(c.gas[b]);
   }
   dfree(c.key); dfree(c.perm); dfree(c.ghash);
-  dfree(c.count); dfree(c.cstart); dfree(c.cend); dfree(c.cmask); dfree(c.bmask); dfree(c.mark);
+  dfree(c.count); dfree(c.cstart); dfree(c.cend); dfree(c.cmask); dfree(c.bmask); dfree(c.sflag); dfree(c.mark);
   dfree(c.lsfluid); dfree(c.lsair); dfree(c.lsmerge);
   for (int a = 0; a < 3; ++a) dfree(c.phig[a]);
   dfree(c.phifluid); dfree(c.phiair); dfree(c.phisolid); dfree(c.ctmp);
@@ -148,7 +148,7 @@ bool alloc_all(Ctx& c) {
   ok = ok && dalloc(c, c.key, cap) && dalloc(c, c.perm, cap) && dalloc(c, c.ghash, cap);
   ok = ok && dalloc(c, c.count, G) && dalloc(c, c.cstart, G) && dalloc(c, c.cend, G) && dalloc(c, c.cmask, G) &&
        dalloc(c, c.mark, G) && dalloc(c, c.lsfluid, G) && dalloc(c, c.lsair, G) && dalloc(c, c.lsmerge, G) &&
-       dalloc(c, c.bmask, 2 * nbricks(c.g));
+       dalloc(c, c.bmask, 2 * nbricks(c.g)) && dalloc(c, c.sflag, 2 * sweep_tiles(c.g));
   for (int a = 0; a < 3; ++a) ok = ok && dalloc(c, c.phig[a], G);
   ok = ok && dalloc(c, c.phifluid, G) && dalloc(c, c.phiair, G) && dalloc(c, c.phisolid, G) && dalloc(c, c.ctmp, G);
   double* dd = nullptr;

@@ -176,39 +176,98 @@ __device__ __forceinline__ void solvedistance_end(float s, float& x) {
   const float d = s / 3.0f;  // infinity when no triple took the branch
   if (d < x) x = d;
 }
-// sweepphibytype (spray_k.cu:771-789) / sweepphi (755-770, typeflag < 0), two buffers
-__global__ void k_sweep(Grid g, const float* __restrict__ in, float* __restrict__ out, const int8_t* __restrict__ mark,
-                        int typeflag) {
-  CELL_LOOP_HEAD
-  // the cell, its mark and its six neighbours (x-, x+, y-, y+, z-, z+) are
-  // loaded together (a neighbour outside the grid reads the cell itself and
-  // is replaced by infinity)
-  const bool vx0 = i > 0, vx1 = i < g.nx - 1, vy0 = j > 0, vy1 = j < g.ny - 1, vz0 = k > 0, vz1 = k < g.nz - 1;
-  const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz;
-  float res = in[idx];
-  const int8_t m = (typeflag >= 0) ? mark[idx] : (int8_t)0;
-  const float lx0 = in[vx0 ? idx - sx : idx], lx1 = in[vx1 ? idx + sx : idx];
-  const float ly0 = in[vy0 ? idx - sy : idx], ly1 = in[vy1 ? idx + sy : idx];
-  const float lz0 = in[vz0 ? idx - 1 : idx], lz1 = in[vz1 ? idx + 1 : idx];
-  if (!(typeflag >= 0 && m == typeflag)) {
-    const float nx0 = vx0 ? lx0 : INFINITY, nx1 = vx1 ? lx1 : INFINITY;
-    const float ny0 = vy0 ? ly0 : INFINITY, ny1 = vy1 ? ly1 : INFINITY;
-    const float nz0 = vz0 ? lz0 : INFINITY, nz1 = vz1 ? lz1 : INFINITY;
-    // an update needs a neighbour below the current value (every candidate
-    // is at least min(a, b, c)), so cells without one keep their value
-    // without evaluating the 8 triples
-    const float mn = fminf(fminf(fminf(nx0, nx1), fminf(ny0, ny1)), fminf(nz0, nz1));
-    if (mn < res) {
-      float s3 = INFINITY;
-      for (int di = -1; di <= 1; di += 2)
-        for (int dj = -1; dj <= 1; dj += 2)
-          for (int dk = -1; dk <= 1; dk += 2)
-            if ((di < 0 ? vx0 : vx1) && (dj < 0 ? vy0 : vy1) && (dk < 0 ? vz0 : vz1))
-              solvedistance_acc(di < 0 ? nx0 : nx1, dj < 0 ? ny0 : ny1, dk < 0 ? nz0 : nz1, res, s3);
-      solvedistance_end(s3, res);
-    }
+// sweepphibytype (spray_k.cu:771-789) / sweepphi (755-770, typeflag < 0), two
+// buffers, on 1 x SW_J x SW_K cell tiles of one wave's cells (lane = row *
+// SW_K + column), with change flags.  A Jacobi pass changes a cell only when
+// a value of its 7-point neighbourhood changed in the pass before.  cur[tile]
+// records which cells of the tile got a smaller value: SF_ANY for any, and
+// the j-low, j-high, k-low and k-high bits for a change on that face of the
+// tile.  A tile whose own flags, those of its x neighbours and the facing bits
+// of its y and z neighbours of the previous pass (prev) are clear would
+// recompute its previous values, which the output buffer (the previous pass's
+// input) already holds, so it is skipped and gets a clear flag.  The first
+// pass of a series (prev == nullptr) computes every tile.  A wave owns
+// SW_L consecutive tiles: its first SW_L lanes read their flags together and
+// the wave then visits the active ones.
+constexpr uint8_t SF_ANY = 1, SF_JLO = 2, SF_JHI = 4, SF_KLO = 8, SF_KHI = 16;
+constexpr int SW_L = 16;
+static_assert(SW_J * SW_K == 64 && SW_L <= 64, "one wave's cells per eikonal tile");
+__global__ void __launch_bounds__(256)
+    k_sweep(Grid g, const float* __restrict__ in, float* __restrict__ out, const int8_t* __restrict__ mark,
+            int typeflag, const uint8_t* __restrict__ prev, uint8_t* __restrict__ cur, uint32_t ntj, uint32_t ntk,
+            uint32_t ntiles) {
+  const uint32_t t0 = (blockIdx.x * 4u + threadIdx.x / 64u) * SW_L;
+  if (t0 >= ntiles) return;
+  const int lane = threadIdx.x & 63;
+  const uint32_t sxt = ntj * ntk;
+  const uint32_t tl = t0 + lane;
+  const bool mine = lane < SW_L && tl < ntiles;
+  bool act = mine;
+  if (prev && mine) {
+    // the seven flags are read together (indices clamped, then masked)
+    const uint32_t tk = tl % ntk, tj = (tl / ntk) % ntj, i = tl / sxt;
+    const uint8_t f0 = prev[tl];
+    const uint8_t fx0 = prev[i > 0 ? tl - sxt : tl], fx1 = prev[i + 1 < (uint32_t)g.nx ? tl + sxt : tl];
+    const uint8_t fy0 = prev[tj > 0 ? tl - ntk : tl], fy1 = prev[tj + 1 < ntj ? tl + ntk : tl];
+    const uint8_t fz0 = prev[tk > 0 ? tl - 1 : tl], fz1 = prev[tk + 1 < ntk ? tl + 1 : tl];
+    act = (f0 & SF_ANY) || (i > 0 && (fx0 & SF_ANY)) || (i + 1 < (uint32_t)g.nx && (fx1 & SF_ANY)) ||
+          (tj > 0 && (fy0 & SF_JHI)) || (tj + 1 < ntj && (fy1 & SF_JLO)) || (tk > 0 && (fz0 & SF_KHI)) ||
+          (tk + 1 < ntk && (fz1 & SF_KLO));
   }
-  out[idx] = res;
+  uint64_t todo = __ballot(act);
+  uint8_t myflag = 0;
+  const int jl = lane / SW_K, kl = lane % SW_K;
+  const int64_t sx = (int64_t)g.ny * g.nz, sy = g.nz;
+  while (todo) {
+    const int q = __builtin_ctzll(todo);
+    todo &= todo - 1;
+    const uint32_t t = t0 + q;
+    const int tk = (int)(t % ntk), tj = (int)((t / ntk) % ntj), i = (int)(t / sxt);
+    const int j = tj * SW_J + jl, k = tk * SW_K + kl;
+    bool changed = false;
+    if (j < g.ny && k < g.nz) {
+      const int64_t idx = ((int64_t)i * g.ny + j) * g.nz + k;
+      // the cell, its mark and its six neighbours (x-, x+, y-, y+, z-, z+)
+      // are loaded together (a neighbour outside the grid reads the cell
+      // itself and is replaced by infinity)
+      const bool vx0 = i > 0, vx1 = i < g.nx - 1, vy0 = j > 0, vy1 = j < g.ny - 1, vz0 = k > 0, vz1 = k < g.nz - 1;
+      const float c0 = in[idx];
+      float res = c0;
+      const int8_t m = (typeflag >= 0) ? mark[idx] : (int8_t)0;
+      const float lx0 = in[vx0 ? idx - sx : idx], lx1 = in[vx1 ? idx + sx : idx];
+      const float ly0 = in[vy0 ? idx - sy : idx], ly1 = in[vy1 ? idx + sy : idx];
+      const float lz0 = in[vz0 ? idx - 1 : idx], lz1 = in[vz1 ? idx + 1 : idx];
+      if (!(typeflag >= 0 && m == typeflag)) {
+        const float nx0 = vx0 ? lx0 : INFINITY, nx1 = vx1 ? lx1 : INFINITY;
+        const float ny0 = vy0 ? ly0 : INFINITY, ny1 = vy1 ? ly1 : INFINITY;
+        const float nz0 = vz0 ? lz0 : INFINITY, nz1 = vz1 ? lz1 : INFINITY;
+        // an update needs a neighbour below the current value (every
+        // candidate is at least min(a, b, c)), so cells without one keep
+        // their value without evaluating the 8 triples
+        const float mn = fminf(fminf(fminf(nx0, nx1), fminf(ny0, ny1)), fminf(nz0, nz1));
+        if (mn < res) {
+          float s3 = INFINITY;
+          for (int di = -1; di <= 1; di += 2)
+            for (int dj = -1; dj <= 1; dj += 2)
+              for (int dk = -1; dk <= 1; dk += 2)
+                if ((di < 0 ? vx0 : vx1) && (dj < 0 ? vy0 : vy1) && (dk < 0 ? vz0 : vz1))
+                  solvedistance_acc(di < 0 ? nx0 : nx1, dj < 0 ? ny0 : ny1, dk < 0 ? nz0 : nz1, res, s3);
+          solvedistance_end(s3, res);
+        }
+      }
+      // res only ever takes strictly smaller values: unchanged means bitwise equal
+      changed = res < c0;
+      out[idx] = res;
+    }
+    const uint64_t bal = __ballot(changed);
+    constexpr uint64_t JLO = (1ull << SW_K) - 1, JHI = JLO << (64 - SW_K);
+    constexpr uint64_t KLO = 0x0001000100010001ull, KHI = KLO << (SW_K - 1);
+    static_assert(SW_K == 16 && SW_J == 4, "face masks of a 4 x 16 tile");
+    const uint8_t f = (uint8_t)((bal ? SF_ANY : 0) | ((bal & JLO) ? SF_JLO : 0) | ((bal & JHI) ? SF_JHI : 0) |
+                                ((bal & KLO) ? SF_KLO : 0) | ((bal & KHI) ? SF_KHI : 0));
+    if (lane == q) myflag = f;
+  }
+  if (mine) cur[tl] = myflag;
 }
 
 // mergeLSAndMarkGrid spray_k.cu:3733-3777
@@ -1050,9 +1109,13 @@ __global__ void k_mc_tris(Grid g, const uint8_t* __restrict__ cases, const uint3
 static void sweep_n(Ctx& c, float* phi, int typeflag, int n, bool in_tmp = false) {
   float* src = in_tmp ? c.ctmp : phi;
   float* dst = in_tmp ? phi : c.ctmp;
+  const uint32_t ntj = (uint32_t)((c.g.ny + SW_J - 1) / SW_J), ntk = (uint32_t)((c.g.nz + SW_K - 1) / SW_K);
+  const int64_t nt = sweep_tiles(c.g);
+  const int64_t nwaves = (nt + SW_L - 1) / SW_L;
   for (int it = 0; it < n; ++it) {
-    launch(c, "sweep", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_sweep, c.g, (const float*)src, dst,
-           (const int8_t*)c.mark, typeflag);
+    const uint8_t* prev = it == 0 ? nullptr : c.sflag + ((it - 1) & 1) * nt;
+    launch(c, "sweep", dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, k_sweep, c.g, (const float*)src, dst,
+           (const int8_t*)c.mark, typeflag, prev, c.sflag + (it & 1) * nt, ntj, ntk, (uint32_t)nt);
     float* t = src;
     src = dst;
     dst = t;

@@ -164,6 +164,7 @@ struct Ctx {
   float *phig[3] = {};
   float *phifluid = nullptr, *phiair = nullptr, *phisolid = nullptr;
   float* ctmp = nullptr;  // scratch cell field for two-buffer sweeps
+  uint8_t* sflag = nullptr;  // 2 x sweep_tiles: per-tile change flags of the last two eikonal passes
   void* div = nullptr;    // float, or double under MPF_FP64
   void* press = nullptr;  // float under MPF_FP32, else double
   float *tp = nullptr, *tpchange = nullptr, *tsave = nullptr;
@@ -268,6 +269,12 @@ int64_t p2g_parts_floats(const Grid& g);
 constexpr int NSPREAD = 64;
 constexpr int SPREAD_STRIDE = 16;
 constexpr int NCOUNTERS = 64 + NSPREAD * SPREAD_STRIDE;
+// 1x4x16 cell tiles of the eikonal passes, one wave's cells each (change
+// flags per tile and pass)
+constexpr int SW_J = 4, SW_K = 16;
+inline int64_t sweep_tiles(const Grid& g) {
+  return (int64_t)g.nx * ((g.ny + SW_J - 1) / SW_J) * ((g.nz + SW_K - 1) / SW_K);
+}
 // 4x4x4 cell bricks of the level-set search summary
 inline int64_t nbricks(const Grid& g) {
   return (int64_t)((g.nx + 3) / 4) * ((g.ny + 3) / 4) * ((g.nz + 3) / 4);
