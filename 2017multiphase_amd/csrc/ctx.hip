@@ -148,7 +148,8 @@ bool alloc_all(Ctx& c) {
   ok = ok && dalloc(c, c.key, cap) && dalloc(c, c.perm, cap) && dalloc(c, c.ghash, cap);
   ok = ok && dalloc(c, c.count, G) && dalloc(c, c.cstart, G) && dalloc(c, c.cend, G) && dalloc(c, c.cmask, G) &&
        dalloc(c, c.mark, G) && dalloc(c, c.lsfluid, G) && dalloc(c, c.lsair, G) && dalloc(c, c.lsmerge, G) &&
-       dalloc(c, c.bmask, 2 * nbricks(c.g)) && dalloc(c, c.sflag, 2 * sweep_tiles(c.g));
+       dalloc(c, c.bmask, 2 * nbricks(c.g)) &&
+       dalloc(c, c.sflag, 2 * sweep_tiles(c.g) + face_tiles(c.g, 0) + face_tiles(c.g, 1) + face_tiles(c.g, 2));
   for (int a = 0; a < 3; ++a) ok = ok && dalloc(c, c.phig[a], G);
   ok = ok && dalloc(c, c.phifluid, G) && dalloc(c, c.phiair, G) && dalloc(c, c.phisolid, G) && dalloc(c, c.ctmp, G);
   double* dd = nullptr;

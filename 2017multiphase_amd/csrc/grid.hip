@@ -191,7 +191,36 @@ __device__ __forceinline__ void solvedistance_end(float s, float& x) {
 // the wave then visits the active ones.
 constexpr uint8_t SF_ANY = 1, SF_JLO = 2, SF_JHI = 4, SF_KLO = 8, SF_KHI = 16;
 constexpr int SW_L = 16;
+constexpr int SW_U1 = 4;
 static_assert(SW_J * SW_K == 64 && SW_L <= 64, "one wave's cells per eikonal tile");
+// whether tile t (of a ntx x ntj x ntk tile grid, x slowest) must be computed:
+// always without previous flags, else when it or an x neighbour changed or a
+// y or z neighbour changed on the facing side (the seven flags are read
+// together: indices clamped, then masked)
+__device__ __forceinline__ bool tile_active(const uint8_t* __restrict__ prev, uint32_t t, bool mine, uint32_t ntj,
+                                            uint32_t ntk, uint32_t ntx) {
+  if (!mine) return false;
+  if (!prev) return true;
+  const uint32_t sxt = ntj * ntk;
+  const uint32_t tk = t % ntk, tj = (t / ntk) % ntj, i = t / sxt;
+  const uint8_t f0 = prev[t];
+  const uint8_t fx0 = prev[i > 0 ? t - sxt : t], fx1 = prev[i + 1 < ntx ? t + sxt : t];
+  const uint8_t fy0 = prev[tj > 0 ? t - ntk : t], fy1 = prev[tj + 1 < ntj ? t + ntk : t];
+  const uint8_t fz0 = prev[tk > 0 ? t - 1 : t], fz1 = prev[tk + 1 < ntk ? t + 1 : t];
+  const int a = (f0 & SF_ANY) | ((i > 0) ? (fx0 & SF_ANY) : 0) | ((i + 1 < ntx) ? (fx1 & SF_ANY) : 0) |
+                ((tj > 0) ? (fy0 & SF_JHI) : 0) | ((tj + 1 < ntj) ? (fy1 & SF_JLO) : 0) |
+                ((tk > 0) ? (fz0 & SF_KHI) : 0) | ((tk + 1 < ntk) ? (fz1 & SF_KLO) : 0);
+  return a != 0;
+}
+// the flags of a tile from its lanes' change bits (lane = row * SW_K + column)
+__device__ __forceinline__ uint8_t tile_flags(bool changed) {
+  const uint64_t bal = __ballot(changed);
+  constexpr uint64_t JLO = (1ull << SW_K) - 1, JHI = JLO << (64 - SW_K);
+  constexpr uint64_t KLO = 0x0001000100010001ull, KHI = KLO << (SW_K - 1);
+  static_assert(SW_K == 16 && SW_J == 4, "face masks of a 4 x 16 tile");
+  return (uint8_t)((bal ? SF_ANY : 0) | ((bal & JLO) ? SF_JLO : 0) | ((bal & JHI) ? SF_JHI : 0) |
+                   ((bal & KLO) ? SF_KLO : 0) | ((bal & KHI) ? SF_KHI : 0));
+}
 __global__ void __launch_bounds__(256)
     k_sweep(Grid g, const float* __restrict__ in, float* __restrict__ out, const int8_t* __restrict__ mark,
             int typeflag, const uint8_t* __restrict__ prev, uint8_t* __restrict__ cur, uint32_t ntj, uint32_t ntk,
@@ -202,18 +231,7 @@ __global__ void __launch_bounds__(256)
   const uint32_t sxt = ntj * ntk;
   const uint32_t tl = t0 + lane;
   const bool mine = lane < SW_L && tl < ntiles;
-  bool act = mine;
-  if (prev && mine) {
-    // the seven flags are read together (indices clamped, then masked)
-    const uint32_t tk = tl % ntk, tj = (tl / ntk) % ntj, i = tl / sxt;
-    const uint8_t f0 = prev[tl];
-    const uint8_t fx0 = prev[i > 0 ? tl - sxt : tl], fx1 = prev[i + 1 < (uint32_t)g.nx ? tl + sxt : tl];
-    const uint8_t fy0 = prev[tj > 0 ? tl - ntk : tl], fy1 = prev[tj + 1 < ntj ? tl + ntk : tl];
-    const uint8_t fz0 = prev[tk > 0 ? tl - 1 : tl], fz1 = prev[tk + 1 < ntk ? tl + 1 : tl];
-    act = (f0 & SF_ANY) || (i > 0 && (fx0 & SF_ANY)) || (i + 1 < (uint32_t)g.nx && (fx1 & SF_ANY)) ||
-          (tj > 0 && (fy0 & SF_JHI)) || (tj + 1 < ntj && (fy1 & SF_JLO)) || (tk > 0 && (fz0 & SF_KHI)) ||
-          (tk + 1 < ntk && (fz1 & SF_KLO));
-  }
+  const bool act = tile_active(prev, tl, mine, ntj, ntk, (uint32_t)g.nx);
   uint64_t todo = __ballot(act);
   uint8_t myflag = 0;
   const int jl = lane / SW_K, kl = lane % SW_K;
@@ -259,12 +277,7 @@ __global__ void __launch_bounds__(256)
       changed = res < c0;
       out[idx] = res;
     }
-    const uint64_t bal = __ballot(changed);
-    constexpr uint64_t JLO = (1ull << SW_K) - 1, JHI = JLO << (64 - SW_K);
-    constexpr uint64_t KLO = 0x0001000100010001ull, KHI = KLO << (SW_K - 1);
-    static_assert(SW_K == 16 && SW_J == 4, "face masks of a 4 x 16 tile");
-    const uint8_t f = (uint8_t)((bal ? SF_ANY : 0) | ((bal & JLO) ? SF_JLO : 0) | ((bal & JHI) ? SF_JHI : 0) |
-                                ((bal & KLO) ? SF_KLO : 0) | ((bal & KHI) ? SF_KHI : 0));
+    const uint8_t f = tile_flags(changed);
     if (lane == q) myflag = f;
   }
   if (mine) cur[tl] = myflag;
@@ -605,14 +618,12 @@ __global__ void k_initphi(int64_t G, const int8_t* __restrict__ mark, float* __r
   if (idx < G) phi[idx] = (mark[idx] == type) ? -0.5f : big;
 }
 
-// sweepu_k_bubble spray_k.cu:3779-3880 (one axis, out-of-place)
+// sweepu_k_bubble spray_k.cu:3779-3880 (one axis, out-of-place): the new
+// value of face (i, j, k) of axis A (index f); u0 returns its old value
 template <int A>
-__global__ void k_sweep_u(Grid g, const float* __restrict__ u, float* __restrict__ out, const float* __restrict__ ls,
-                          const int8_t* __restrict__ mark, int8_t sf, int64_t nf) {
-  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
-  int i, j, k;
-  ijk_face<A>(g, f, i, j, k);
+__device__ __forceinline__ float sweep_u_face(const Grid& g, const float* __restrict__ u,
+                                              const float* __restrict__ ls, const int8_t* __restrict__ mark, int8_t sf,
+                                              int i, int j, int k, int64_t f, float& u0) {
   const int n[3] = {i, j, k};
   const int lim = (A == 0) ? g.nx : (A == 1) ? g.ny : g.nz;
   const bool inner = n[A] > 1 && n[A] < lim - 1;
@@ -621,6 +632,7 @@ __global__ void k_sweep_u(Grid g, const float* __restrict__ u, float* __restrict
   const int ci = min(i, g.nx - 1), cj = min(j, g.ny - 1), ck = min(k, g.nz - 1);
   const int bi = max(i - (A == 0), 0), bj = max(j - (A == 1), 0), bk = max(k - (A == 2), 0);
   float res = u[f];
+  u0 = res;
   const int8_t mc = mark[cid(g, ci, cj, ck)], mb = mark[cid(g, bi, bj, bk)];
   const float lsc = ls[cid(g, ci, cj, ck)], lsb = ls[cid(g, bi, bj, bk)];
   if (inner && mc != sf && mb != sf) {
@@ -689,7 +701,48 @@ __global__ void k_sweep_u(Grid g, const float* __restrict__ u, float* __restrict
       res = wx * un[0][sgn[0]] + wy * un[1][sgn[1]] + wz * un[2][sgn[2]];
     }
   }
-  out[f] = res;
+  return res;
+}
+
+// One extrapolation pass of axis A on 1 x SW_J x SW_K tiles of its faces,
+// with the change flags of the eikonal passes: a face's new value reads its
+// own and its six neighbours' values (level sets and marks are fixed over the
+// passes), so a tile whose neighbourhood did not change in the previous pass
+// (prev) is skipped, its values being in the output buffer already.  cur (may
+// be null) records the changes of this pass; prev == nullptr computes every
+// tile.  A change is a different bit pattern.
+template <int A>
+__global__ void __launch_bounds__(256)
+    k_sweep_u(Grid g, const float* __restrict__ u, float* __restrict__ out, const float* __restrict__ ls,
+              const int8_t* __restrict__ mark, int8_t sf, const uint8_t* __restrict__ prev, uint8_t* __restrict__ cur,
+              uint32_t ntj, uint32_t ntk, uint32_t ntiles, int per_wave) {
+  const uint32_t t0 = (blockIdx.x * 4u + threadIdx.x / 64u) * (uint32_t)per_wave;
+  if (t0 >= ntiles) return;
+  const int lane = threadIdx.x & 63;
+  const int nfx = g.nx + (A == 0), nfy = g.ny + (A == 1), nfz = g.nz + (A == 2);
+  const bool act = tile_active(prev, t0 + lane, lane < per_wave && t0 + lane < ntiles, ntj, ntk, (uint32_t)nfx);
+  uint64_t todo = __ballot(act);
+  uint8_t myflag = 0;
+  const int jl = lane / SW_K, kl = lane % SW_K;
+  const uint32_t sxt = ntj * ntk;
+  while (todo) {
+    const int q = __builtin_ctzll(todo);
+    todo &= todo - 1;
+    const uint32_t t = t0 + q;
+    const int tk = (int)(t % ntk), tj = (int)((t / ntk) % ntj), i = (int)(t / sxt);
+    const int j = tj * SW_J + jl, k = tk * SW_K + kl;
+    bool changed = false;
+    if (j < nfy && k < nfz) {
+      const int64_t f = fid(g, A, i, j, k);
+      float u0;
+      const float res = sweep_u_face<A>(g, u, ls, mark, sf, i, j, k, f, u0);
+      changed = __float_as_uint(res) != __float_as_uint(u0);
+      out[f] = res;
+    }
+    const uint8_t fl = tile_flags(changed);
+    if (lane == q) myflag = fl;
+  }
+  if (cur && lane < per_wave && t0 + lane < ntiles) cur[t0 + lane] = myflag;
 }
 
 // setWaterBoundaryU_k spray_k.cu:956-995 (in place).  The ceiling rule reads
@@ -1269,17 +1322,33 @@ static void sweep_phi(Ctx& c, float* phi, int8_t type) {
 }
 
 // Two passes: u -> scratch, scratch -> u.
+template <int A>
+static void sweep_u_pass(Ctx& c, const float* src, float* dst, const float* phi, int8_t sf, const uint8_t* prev,
+                         uint8_t* cur) {
+  const int64_t nfy = c.g.ny + (A == 1), nfz = c.g.nz + (A == 2);
+  const uint32_t ntj = (uint32_t)((nfy + SW_J - 1) / SW_J), ntk = (uint32_t)((nfz + SW_K - 1) / SW_K);
+  const int64_t nt = face_tiles(c.g, A);
+  // a pass that computes every tile gives each wave SW_U1 tiles (more waves
+  // in flight for its two rounds of loads); a pass that skips reads the
+  // flags of SW_L tiles per wave
+  const int per_wave = prev ? SW_L : SW_U1;
+  const int64_t nwaves = (nt + per_wave - 1) / per_wave;
+  launch(c, "sweep_u", dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, k_sweep_u<A>, c.g, src, dst, phi,
+         (const int8_t*)c.mark, sf, prev, cur, ntj, ntk, (uint32_t)nt, per_wave);
+}
 static void sweep_u(Ctx& c, float* const* u, const float* phi, int8_t sf) {
-  const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
+  // the first pass records per-tile changes (one flag set per axis), the
+  // second skips the tiles they leave unchanged
+  uint8_t* fl[3] = {c.sflag + 2 * sweep_tiles(c.g), nullptr, nullptr};
+  fl[1] = fl[0] + face_tiles(c.g, 0);
+  fl[2] = fl[1] + face_tiles(c.g, 1);
   for (int it = 0; it < 2; ++it) {
     float* const* src = it == 0 ? u : c.ftmp;
     float* const* dst = it == 0 ? c.ftmp : u;
-    launch(c, "sweep_u", dim3(nblocks(fc[0], 256)), dim3(256), 0, k_sweep_u<0>, c.g, (const float*)src[0], dst[0],
-           phi, (const int8_t*)c.mark, sf, fc[0]);
-    launch(c, "sweep_u", dim3(nblocks(fc[1], 256)), dim3(256), 0, k_sweep_u<1>, c.g, (const float*)src[1], dst[1],
-           phi, (const int8_t*)c.mark, sf, fc[1]);
-    launch(c, "sweep_u", dim3(nblocks(fc[2], 256)), dim3(256), 0, k_sweep_u<2>, c.g, (const float*)src[2], dst[2],
-           phi, (const int8_t*)c.mark, sf, fc[2]);
+    const bool first = it == 0;
+    sweep_u_pass<0>(c, src[0], dst[0], phi, sf, first ? nullptr : fl[0], first ? fl[0] : nullptr);
+    sweep_u_pass<1>(c, src[1], dst[1], phi, sf, first ? nullptr : fl[1], first ? fl[1] : nullptr);
+    sweep_u_pass<2>(c, src[2], dst[2], phi, sf, first ? nullptr : fl[2], first ? fl[2] : nullptr);
   }
 }
 

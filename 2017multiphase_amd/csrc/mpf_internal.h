@@ -164,7 +164,8 @@ struct Ctx {
   float *phig[3] = {};
   float *phifluid = nullptr, *phiair = nullptr, *phisolid = nullptr;
   float* ctmp = nullptr;  // scratch cell field for two-buffer sweeps
-  uint8_t* sflag = nullptr;  // 2 x sweep_tiles: per-tile change flags of the last two eikonal passes
+  uint8_t* sflag = nullptr;  // per-tile change flags: 2 x sweep_tiles (eikonal passes), then face_tiles of
+                             // each axis (first extrapolation pass)
   void* div = nullptr;    // float, or double under MPF_FP64
   void* press = nullptr;  // float under MPF_FP32, else double
   float *tp = nullptr, *tpchange = nullptr, *tsave = nullptr;
@@ -274,6 +275,10 @@ constexpr int NCOUNTERS = 64 + NSPREAD * SPREAD_STRIDE;
 constexpr int SW_J = 4, SW_K = 16;
 inline int64_t sweep_tiles(const Grid& g) {
   return (int64_t)g.nx * ((g.ny + SW_J - 1) / SW_J) * ((g.nz + SW_K - 1) / SW_K);
+}
+// the same tiles over the faces of axis a (extrapolation passes)
+inline int64_t face_tiles(const Grid& g, int a) {
+  return (int64_t)(g.nx + (a == 0)) * ((g.ny + (a == 1) + SW_J - 1) / SW_J) * ((g.nz + (a == 2) + SW_K - 1) / SW_K);
 }
 // 4x4x4 cell bricks of the level-set search summary
 inline int64_t nbricks(const Grid& g) {
