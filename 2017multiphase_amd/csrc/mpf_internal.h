@@ -113,6 +113,7 @@ struct SolveScalars {
   double al[2];  // alpha of update k in slot k&1 (x may take it one iteration later)
   int iter, done, converged, nonfinite, refinements, rec_conv, fold_pending;
   int last_true;  // MIXED: update index whose residual the last executed true-residual pass replaced (-1 none)
+  int done_k;     // single-kernel iteration: the k of the launch that set done
 };
 
 struct KernelStat {

@@ -1032,7 +1032,9 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
                                                float* __restrict__ x, double* __restrict__ x64, int fold,
                                                SolveScalars* sc, int k, double tol, int cap) {
   using V = float;
-  if (sc->done) return;
+  // a launch after the one that stopped the solve does nothing (the
+  // stopping launch's own late blocks still finish its x update)
+  if (sc->done && sc->done_k != k) return;
   __shared__ V dtab[8];
   if (threadIdx.x < 8) dtab[threadIdx.x] = (threadIdx.x < 7) ? dinv_tab<V>(o, threadIdx.x) : V(1);
   double ald = 0.0, rz, rr, be = 0.0, rz_prev = 0.0;
@@ -1078,6 +1080,8 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
     sc->iter = k;
     if (first == 1 && k == 0) sc->bb = bb;
     if (stop) {
+      sc->done_k = k;
+      __threadfence();  // done_k is visible before done
       sc->done = 1;
       sc->converged = conv && !nonfin;
       sc->nonfinite = nonfin;
