@@ -48,20 +48,27 @@ __global__ void k_g2p(Grid g, int64_t P, const f3* __restrict__ pos, f3* __restr
                       const int8_t* __restrict__ flag, Faces3 wdu, Faces3 adu, Faces3 wu, Faces3 au, float alpha) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
-  int8_t fl = flag[p];
-  const Faces3* du;
-  const Faces3* un;
-  if (fl == FLUID || fl == SOLID) { du = &wdu; un = &wu; }
-  else if (fl == AIR) { du = &adu; un = &au; }
-  else return;
-  f3 x = pos[p], v = vel[p];
-  f3 gv = vel_from_grid(g, x, du->u[0], du->u[1], du->u[2]);
+  // the flag, position and velocity are read together, and the phase's face
+  // arrays are chosen by value (a pointer into the kernel arguments would be
+  // a per-lane load of the pointers after the flag arrives)
+  const int8_t fl = flag[p];
+  const f3 x = pos[p];
+  f3 v = vel[p];
+  const bool air = fl == AIR;
+  const bool mapped = fl == FLUID || fl == SOLID || air;
+  const float* d0 = air ? adu.u[0] : wdu.u[0];
+  const float* d1 = air ? adu.u[1] : wdu.u[1];
+  const float* d2 = air ? adu.u[2] : wdu.u[2];
+  const f3 gv = vel_from_grid(g, x, d0, d1, d2);
   if (alpha == 1.0f) v = add3(v, gv);
   else {
-    f3 u = vel_from_grid(g, x, un->u[0], un->u[1], un->u[2]);
+    const float* n0 = air ? au.u[0] : wu.u[0];
+    const float* n1 = air ? au.u[1] : wu.u[1];
+    const float* n2 = air ? au.u[2] : wu.u[2];
+    const f3 u = vel_from_grid(g, x, n0, n1, n2);
     v = add3(mul3(u, 1.0f - alpha), mul3(add3(v, gv), alpha));
   }
-  vel[p] = v;
+  if (mapped) vel[p] = v;
 }
 
 // setVelZeroSolid_k spray_k.cu:5213-5218
@@ -76,12 +83,13 @@ __global__ void k_collide_move(Grid g, int64_t P, f3* __restrict__ pos, f3* __re
                                float bounce_vel, float bounce_pos) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
-  int8_t fl = flag[p];
-  if (fl == SOLID) return;
+  // flag, position and velocity are read together; solid particles are
+  // computed like the others and not stored
+  const int8_t fl = flag[p];
   f3 x = pos[p], v = vel[p];
   const float h = g.h;
   float iphi = scalar_from_grid(g, x, phis);
-  if (iphi <= 0.5f) {
+  if (iphi <= 0.5f && fl != SOLID) {
     f3 sv = vel_from_grid(g, x, su.u[0], su.u[1], su.u[2]);
     f3 rv = sub3(v, sv);
     float d = h * 0.5f;
@@ -107,6 +115,7 @@ __global__ void k_collide_move(Grid g, int64_t P, f3* __restrict__ pos, f3* __re
   if (x.x >= hi.x) { x.x = hi.x; v.x = 0; }
   if (x.y >= hi.y) { x.y = hi.y; v.y = 0; }
   if (x.z >= hi.z) { x.z = hi.z; v.z = 0; }
+  if (fl == SOLID) return;
   vel[p] = v;
   pos[p] = x;
 }
@@ -118,13 +127,17 @@ __global__ void k_advect_rk2(Grid g, int64_t P, f3* __restrict__ pos, f3* __rest
   if (p >= P) return;
   int8_t fl = flag[p];
   if (fl == AIRSOLO) return;
-  const Faces3& u = (fl == AIR) ? au : wu;
+  // the phase's face arrays chosen by value (no per-lane pointer load)
+  const bool air = fl == AIR;
+  const float* u0 = air ? au.u[0] : wu.u[0];
+  const float* u1 = air ? au.u[1] : wu.u[1];
+  const float* u2 = air ? au.u[2] : wu.u[2];
   f3 x = pos[p], v = vel[p];
   float m = g.h + 0.5f * ss;
   f3 lo = mk3(g.gx0 + m, g.gy0 + m, g.gz0 + m), hi = mk3(g.gx1 - m, g.gy1 - m, g.gz1 - m);
-  f3 gv = vel_from_grid(g, x, u.u[0], u.u[1], u.u[2]);
+  f3 gv = vel_from_grid(g, x, u0, u1, u2);
   f3 mid = add3(x, mul3(mul3(gv, dt), 0.5f));
-  f3 gm = vel_from_grid(g, mid, u.u[0], u.u[1], u.u[2]);
+  f3 gm = vel_from_grid(g, mid, u0, u1, u2);
   x = add3(x, mul3(gm, dt));
   if (x.x <= lo.x) { x.x = lo.x; v.x = 0; }
   if (x.y <= lo.y) { x.y = lo.y; v.y = 0; }
