@@ -43,8 +43,11 @@ __device__ __forceinline__ float div_nr(float a, float d) {
 // h^2); a phase stops once its minimum is below the next ring's bound (with a
 // 0.1% margin for rounding), and the 8 corner cells (bound 6.75 h^2 > 2.5^2)
 // can never beat the initial 2.5h.  The result equals the full-window minimum.
-__constant__ signed char c_ring_off[117][3];
-__constant__ unsigned char c_ring_start[10];  // ring r = offsets [start[r], start[r+1])
+// offsets (di + 4) | (dj + 4) << 8 | (dk + 4) << 16, one dword each so the
+// uniform-indexed reads are scalar loads (byte tables would be vector loads
+// waited on before every neighbour's address)
+__constant__ int c_ring_off[120];
+__constant__ int c_ring_start[10];  // ring r = offsets [start[r], start[r+1])
 __constant__ float c_ring_lb[9];              // lower bound of ring r, in h^2 units
 
 // Phases present in each 4x4x4 brick of cells (OR of the occupancy masks),
@@ -99,30 +102,48 @@ __global__ void k_genlevelset(Grid g, const f3* __restrict__ pos, const int8_t* 
     const bool need_f = has_f && mf2 > lb, need_a = has_a && ma2 > lb;
     if (!need_f && !need_a) break;
     const uint8_t want = (need_f ? M_FLUIDSOLID : 0) | (need_a ? (M_AIR | M_AIRSOLO) : 0);
-    for (int q = c_ring_start[ring]; q < c_ring_start[ring + 1]; ++q) {
-      int ci = i + c_ring_off[q][0], cj = j + c_ring_off[q][1], ck = k + c_ring_off[q][2];
-      if (!valid(g, ci, cj, ck)) continue;
-      int64_t c = cid(g, ci, cj, ck);
-      if (!(cmask[c] & want)) continue;
-      const uint32_t s = cstart[c], e = cend[c];
-      // four particles' loads issued together (the minimum does not depend
-      // on the order); lanes past the range re-read the first particle
-      for (uint32_t p = s; p < e; p += 4) {
-        f3 x[4];
-        int8_t fl[4];
+    // the ring's cells four at a time: their masks and particle ranges are
+    // read together (cells outside the grid or past the ring read cell 0
+    // and are dropped), then each cell's particles are visited
+    const int q1 = c_ring_start[ring + 1];
+    for (int q0 = c_ring_start[ring]; q0 < q1; q0 += 4) {
+      uint32_t S4[4], E4[4];
+      bool take[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const bool in = p + u < e;
-          x[u] = pos[in ? p + u : s];
-          fl[u] = in ? flag[p + u] : (int8_t)-1;
-        }
+      for (int w = 0; w < 4; ++w) {
+        const int q = (q0 + w < q1) ? q0 + w : q0;
+        const int ow = c_ring_off[q];
+        const int ci = i + (ow & 0xff) - 4, cj = j + ((ow >> 8) & 0xff) - 4, ck = k + ((ow >> 16) & 0xff) - 4;
+        const bool v = q0 + w < q1 && valid(g, ci, cj, ck);
+        const int64_t c = v ? cid(g, ci, cj, ck) : 0;
+        const uint8_t m = cmask[c];
+        S4[w] = cstart[c];
+        E4[w] = cend[c];
+        take[w] = v && (m & want);
+      }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          f3 d = sub3(x[u], gp);
-          float d2 = dot3(d, d);
-          const int8_t f = fl[u];
-          if (f == AIR || f == AIRSOLO) ma2 = fminf(ma2, d2);
-          else if (f == FLUID || f == SOLID) mf2 = fminf(mf2, d2);
+      for (int w = 0; w < 4; ++w) {
+        if (!take[w]) continue;
+        const uint32_t s = S4[w], e = E4[w];
+        // eight particles' loads issued together (the minimum does not
+        // depend on the order); lanes past the range re-read the first one
+        for (uint32_t p = s; p < e; p += 8) {
+          f3 x[8];
+          int8_t fl[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const bool in = p + u < e;
+            x[u] = pos[in ? p + u : s];
+            fl[u] = in ? flag[p + u] : (int8_t)-1;
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            f3 d = sub3(x[u], gp);
+            float d2 = dot3(d, d);
+            const int8_t f = fl[u];
+            if (f == AIR || f == AIRSOLO) ma2 = fminf(ma2, d2);
+            else if (f == FLUID || f == SOLID) mf2 = fminf(mf2, d2);
+          }
         }
       }
     }
@@ -1206,19 +1227,19 @@ static void upload_ring_tables() {
         offs.push_back(O{{di, dj, dk}, lb});
       }
   std::stable_sort(offs.begin(), offs.end(), [](const O& x, const O& y) { return x.lb < y.lb; });
-  signed char tab[117][3];
-  unsigned char start[10];
+  int tab[120] = {0};
+  int start[10];
   float lbs[9];
   int nr = 0;
   for (size_t q = 0; q < offs.size(); ++q) {
     if (q == 0 || offs[q].lb != offs[q - 1].lb) {
-      start[nr] = (unsigned char)q;
+      start[nr] = (int)q;
       lbs[nr] = offs[q].lb;
       ++nr;
     }
-    for (int a = 0; a < 3; ++a) tab[q][a] = (signed char)offs[q].d[a];
+    tab[q] = (offs[q].d[0] + 4) | ((offs[q].d[1] + 4) << 8) | ((offs[q].d[2] + 4) << 16);
   }
-  start[nr] = (unsigned char)offs.size();
+  start[nr] = (int)offs.size();
   hipMemcpyToSymbol(HIP_SYMBOL(c_ring_off), tab, sizeof(tab));
   hipMemcpyToSymbol(HIP_SYMBOL(c_ring_start), start, sizeof(start));
   hipMemcpyToSymbol(HIP_SYMBOL(c_ring_lb), lbs, sizeof(lbs));
