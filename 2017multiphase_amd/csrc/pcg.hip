@@ -373,6 +373,23 @@ static_assert((TK & (TK - 1)) == 0 && (ST / TK) * RPT == TJ, "threads cover the 
 struct Tiling {
   int ntj, ntk, seg, nblk;
 };
+// Block -> tile run.  Blocks are dispatched round-robin over the 8 XCDs, so
+// each XCD takes a contiguous run of tiles (its L2 sees neighbouring tiles);
+// inside a run the second half swaps neighbouring tiles, so the two blocks a
+// CU holds (run positions m and m + half) lie in different k tiles: a liquid
+// surface across k makes k tiles unequal in work, and pairing a full with a
+// partly filled tile evens the CUs' loads.  Any block order is correct.
+__device__ __forceinline__ int tile_of_block(const Tiling& T) {
+  const int b = blockIdx.x;
+  if ((T.nblk & 15) == 0) {
+    int m = b >> 3;
+    if (m >= (T.nblk >> 4)) m ^= 1;
+    return (b & 7) * (T.nblk >> 3) + m;
+  }
+  if ((T.nblk & 7) == 0) return (b & 7) * (T.nblk >> 3) + (b >> 3);
+  return b;
+}
+
 
 // The inputs of p_k at one cell: r, p_{k-1}, the slow-row diagonal and the
 // mask byte (nb = 0 outside the grid).
@@ -455,8 +472,7 @@ __global__ void __launch_bounds__(ST, (sizeof(V) == 4 && !SLOW) ? 6 : 4) k_spmv(
   }
   __shared__ V pl[3][LJ][LK];
   const Grid& g = o.g;
-  int lb = blockIdx.x;
-  if ((T.nblk & 7) == 0) lb = (lb & 7) * (T.nblk >> 3) + (lb >> 3);  // XCD-contiguous tiles
+  const int lb = tile_of_block(T);
   const int tk = lb % T.ntk;
   const int rest = lb / T.ntk;
   const int tj = rest % T.ntj;
@@ -829,8 +845,7 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_resid64(OpArgs o, Tiling T
   if (threadIdx.x < 8) dtab[threadIdx.x] = (threadIdx.x < 7) ? dinv_tab<float>(o, threadIdx.x) : 1.0f;
   __shared__ double pl[3][LJ][LK];
   const Grid& g = o.g;
-  int lb = blockIdx.x;
-  if ((T.nblk & 7) == 0) lb = (lb & 7) * (T.nblk >> 3) + (lb >> 3);  // XCD-contiguous tiles
+  const int lb = tile_of_block(T);
   const int tk = lb % T.ntk;
   const int rest = lb / T.ntk;
   const int tj = rest % T.ntj;
@@ -1089,8 +1104,7 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
   }
   __shared__ V pl[3][LJ][LK];
   const Grid& g = o.g;
-  int lb = blockIdx.x;
-  if ((T.nblk & 7) == 0) lb = (lb & 7) * (T.nblk >> 3) + (lb >> 3);  // XCD-contiguous tiles
+  const int lb = tile_of_block(T);
   const int tk = lb % T.ntk;
   const int rest = lb / T.ntk;
   const int tj = rest % T.ntj;
