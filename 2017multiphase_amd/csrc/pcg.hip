@@ -381,7 +381,7 @@ struct Tiling {
 // partly filled tile evens the CUs' loads.  Any block order is correct.
 __device__ __forceinline__ int tile_of_block(const Tiling& T) {
   const int b = blockIdx.x;
-  if ((T.nblk & 15) == 0) {
+  if ((T.nblk & 31) == 0) {  // an even half, so the swap stays inside it
     int m = b >> 3;
     if (m >= (T.nblk >> 4)) m ^= 1;
     return (b & 7) * (T.nblk >> 3) + m;

@@ -284,10 +284,11 @@ def test_project(pkg, scenes, orc, name, n, over):
             assert e <= (1e-3 if f == "PRESSURE" else 1e-4)
 
 
-def _odd_grid_sim(pkg, with_air, precision):
-    """A 13 x 17 x 23 box (no size a multiple of the solver's tile or quad
-    widths, 5083 cells): liquid below k = 14, optionally an air ball in it."""
-    nx, ny, nz = 13, 17, 23
+def _odd_grid_sim(pkg, with_air, precision, dims=(13, 17, 23)):
+    """A 13 x 17 x 23 box by default (no size a multiple of the solver's tile
+    or quad widths, 5083 cells): liquid below k = 14, optionally an air ball
+    in it."""
+    nx, ny, nz = dims
     I, J, K = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
     I, J, K = I.ravel(), J.ravel(), K.ravel()
     inner = (I >= 1) & (I <= nx - 2) & (J >= 1) & (J <= ny - 2) & (K >= 1) & (K <= nz - 2)
@@ -336,6 +337,31 @@ def test_project_odd_grid(pkg, orc, with_air, precision):
     copy_fields(sim, o64, ["MARK", "LSMERGE"] + FACE)
     mark = sim.get_field("MARK")
     assert (mark == pkg.TYPEAIR).any() == with_air
+    gi = sim.project()
+    oi = o64.project()
+    say("  gpu", gi)
+    say("  fp64 oracle", oi)
+    assert gi["converged"] == 1
+    assert gi["rel_residual"] <= 1e-9
+    for f in ["PRESSURE"] + FACE:
+        e = rel_l2(sim.get_field(f), o64.get_field(f))
+        say("  %s rel L2 vs FP64 = %.2e" % (f, e))
+        assert e <= TOL
+
+
+@pytest.mark.parametrize("with_air", [False, True])
+@pytest.mark.parametrize("precision", [1, 2])
+def test_project_tile_block_order(pkg, orc, with_air, precision):
+    """16 x 24 x 24: the solver's tile kernels run 3 tiles x 16 runs = 48
+    blocks, a count whose half-run per XCD is odd (the tile order that pairs
+    k tiles on a CU must stay a permutation of the tiles)."""
+    say("project 16x24x24", "air" if with_air else "liquid only", "precision", precision)
+    cfg, sim = _odd_grid_sim(pkg, with_air, precision, dims=(16, 24, 24))
+    tight = pkg.Config.from_buffer_copy(cfg)
+    tight.tol = 1e-12
+    tight.max_iters = 20000
+    o64 = orc.Oracle(tight, use_double=True)
+    copy_fields(sim, o64, ["MARK", "LSMERGE"] + FACE)
     gi = sim.project()
     oi = o64.project()
     say("  gpu", gi)
