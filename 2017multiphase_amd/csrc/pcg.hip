@@ -1367,7 +1367,7 @@ Launch make_launch(int64_t G) {
   return L;
 }
 
-// spmv tiles: 16 x 64 (j, k) columns, planes split along i into runs so the
+// spmv tiles: TJ x TK (j, k) columns, planes split along i into runs so the
 // launch has about `blocks` blocks: one resident round on 256 CUs, i.e. as
 // many 512-thread blocks as fit a CU at the kernel's register use (three for
 // FP32 vectors without slow rows, two otherwise).
