@@ -349,6 +349,30 @@ def test_project_odd_grid(pkg, orc, with_air, precision):
         assert e <= TOL
 
 
+@pytest.mark.parametrize("dims", [(13, 17, 23), (16, 24, 24)])
+def test_levelset_extrapolate_odd_grid(pkg, orc, dims):
+    """Level set and extrapolation on grids whose sizes are not multiples of
+    the 1x4x16 tiles of the flagged passes (partial tiles in j and k), with
+    an air ball: bit-exact against the oracle."""
+    say("levelset + extrapolate_bc odd grid", dims)
+    cfg, sim = _odd_grid_sim(pkg, True, 2, dims=dims)
+    o = orc.Oracle(cfg)
+    copy_particles(sim, o)
+    copy_fields(sim, o, ["CELL_START", "CELL_END"])
+    sim.levelset_update()
+    o.levelset_update()
+    for f in ("LSFLUID", "LSAIR", "LSMERGE"):
+        bitexact(sim.get_field(f), o.get_field(f).astype(np.float32), f)
+    bitexact(sim.get_field("MARK"), o.get_field("MARK"), "MARK")
+    sim.p2g()
+    copy_fields(sim, o, FACE)
+    for recompute in (True, False):
+        sim.extrapolate_bc(recompute)
+        o.extrapolate_bc(recompute)
+        for f in FACE + ["PHIFLUID", "PHIAIR"]:
+            bitexact(sim.get_field(f), o.get_field(f).astype(np.float32), f)
+
+
 @pytest.mark.parametrize("with_air", [False, True])
 @pytest.mark.parametrize("precision", [1, 2])
 def test_project_tile_block_order(pkg, orc, with_air, precision):
