@@ -110,11 +110,20 @@ __device__ __forceinline__ void bitonic(uint32_t (&a)[N]) {
   }
 }
 
+// A slice of n <= N entries: read together (past the end: the last entry),
+// checked for order (runs usually arrive in order) and sorted in registers
+// when not.
 template <int N>
-__device__ __forceinline__ void sort_small(uint32_t* __restrict__ v, uint32_t n) {
+__device__ __forceinline__ void seg_small(uint32_t* __restrict__ v, uint32_t n) {
   uint32_t a[N];
 #pragma unroll
-  for (int q = 0; q < N; ++q) a[q] = ((uint32_t)q < n) ? v[q] : 0xffffffffu;
+  for (int q = 0; q < N; ++q) a[q] = v[(uint32_t)q < n ? q : n - 1];
+  bool ordered = true;
+#pragma unroll
+  for (int q = 1; q < N; ++q) ordered = ordered && ((uint32_t)q >= n || a[q - 1] < a[q]);
+  if (ordered) return;
+#pragma unroll
+  for (int q = 0; q < N; ++q) a[q] = ((uint32_t)q < n) ? a[q] : 0xffffffffu;
   bitonic<N>(a);
 #pragma unroll
   for (int q = 0; q < N; ++q)
@@ -131,13 +140,9 @@ __global__ void k_segsort(int64_t G, const uint32_t* __restrict__ cstart, const 
   uint32_t n = cend[c] - s;
   if (n < 2) return;
   uint32_t* v = perm + s;
-  // already ordered? (runs usually arrive in order)
-  bool ordered = true;
-  for (uint32_t q = 1; q < n && ordered; ++q) ordered = v[q - 1] < v[q];
-  if (ordered) return;
-  if (n <= 8) sort_small<8>(v, n);
-  else if (n <= 16) sort_small<16>(v, n);
-  else if (n <= 32) sort_small<32>(v, n);
+  if (n <= 8) seg_small<8>(v, n);
+  else if (n <= 16) seg_small<16>(v, n);
+  else if (n <= 32) seg_small<32>(v, n);
   else {
     for (uint32_t q = 1; q < n; ++q) {
       uint32_t x = v[q];
