@@ -189,16 +189,24 @@ __global__ void k_cell_mask(int64_t G, const uint32_t* __restrict__ cstart, cons
                             const int8_t* __restrict__ flag, uint8_t* __restrict__ cmask) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= G) return;
-  uint32_t s = cstart[c];
+  const uint32_t s = cstart[c], e = cend[c];
   uint8_t m = 0;
   if (s != MPF_CELL_UNDEF) {
     m = M_ANY;
-    for (uint32_t p = s; p < cend[c]; ++p) {
-      int8_t f = flag[p];
-      if (f == FLUID || f == SOLID) m |= M_FLUIDSOLID;
-      if (f == SOLID) m |= M_SOLID;
-      if (f == AIR) m |= M_AIR;
-      if (f == AIRSOLO) m |= M_AIRSOLO;
+    // eight flags read together (past the end: the first again; the mask is
+    // an OR, so repeats change nothing)
+    for (uint32_t p = s; p < e; p += 8) {
+      int8_t f8[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) f8[u] = flag[p + u < e ? p + u : s];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int8_t f = f8[u];
+        if (f == FLUID || f == SOLID) m |= M_FLUIDSOLID;
+        if (f == SOLID) m |= M_SOLID;
+        if (f == AIR) m |= M_AIR;
+        if (f == AIRSOLO) m |= M_AIRSOLO;
+      }
     }
   }
   cmask[c] = m;
