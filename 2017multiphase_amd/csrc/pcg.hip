@@ -366,7 +366,7 @@ constexpr int TJ = 8, TK = 128;               // (j, k) column tile
 constexpr int ST = 512;                       // threads per spmv block
 constexpr int RPT = TJ * TK / ST;             // tile rows per thread (2)
 constexpr int LJ = TJ + 2, LK = TK + 2;       // LDS plane with a one-cell ring
-constexpr int NRING = 2 * TK + 2 * TJ;        // ring cells per plane (160)
+constexpr int NRING = 2 * TK + 2 * TJ;        // ring cells per plane
 static_assert(NRING <= ST, "one ring cell per thread");
 static_assert((TK & (TK - 1)) == 0 && (ST / TK) * RPT == TJ, "threads cover the tile in rows of TK");
 
