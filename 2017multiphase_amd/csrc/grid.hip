@@ -456,7 +456,7 @@ __global__ void __launch_bounds__(ST_CELLS)
     k_p2g_tile(Grid g, const f3* __restrict__ pos, const f3* __restrict__ vel, const float* __restrict__ mass,
                const int8_t* __restrict__ flag, const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
                const uint8_t* __restrict__ cmask, float* __restrict__ parts, int ntx, int nty, int ntz,
-               const float* __restrict__ band, float big) {
+               const float* __restrict__ band, float big, const uint8_t* __restrict__ dmask) {
   __shared__ float sW[ST_RTOT];
   __shared__ float sU[ST_RTOT];
   int t = blockIdx.x;
@@ -482,6 +482,23 @@ __global__ void __launch_bounds__(ST_CELLS)
   // band mode: a tile whose cells and ring hold no cell with band < big maps nothing
   bool near = true;
   if (band) {
+    // band cells lie within 3 cells of a solid cell, and the region's cells
+    // within one cell of the tile, so a tile whose bricks (1 x 2 x 2 of the
+    // 4x4x4 bricks, each with its 3x3x3 neighbourhood in dmask) hold no solid
+    // cell has none in its region: its band scan is skipped
+    static_assert(ST_X == 4 && ST_Y == 8 && ST_Z == 8, "a tile spans 1 x 2 x 2 bricks");
+    const int nby = (g.ny + 3) / 4, nbz = (g.nz + 3) / 4;
+    uint8_t bm = 0;
+#pragma unroll
+    for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+      for (int dk = 0; dk < 2; ++dk) {
+        const int bj = (y0 >> 2) + dj, bk = (z0 >> 2) + dk;
+        if (bj < nby && bk < nbz) bm |= dmask[((int64_t)(x0 >> 2) * nby + bj) * nbz + bk];
+      }
+    near = (bm & M_SOLID) != 0;
+  }
+  if (band && near) {
     bool mine = false;
     for (int q = threadIdx.x; q < (ST_X + 2) * (ST_Y + 2) * (ST_Z + 2); q += ST_CELLS) {
       const int bi = x0 - 1 + q / ((ST_Y + 2) * (ST_Z + 2)), bj = y0 - 1 + (q / (ST_Z + 2)) % (ST_Y + 2),
@@ -1305,7 +1322,7 @@ static void p2g_liquid(Ctx& c, float* const* out, float* const* out2 = nullptr, 
   launch(c, band ? "p2g_solid_band" : "p2g_liquid", dim3((unsigned)(ntx * nty * ntz)), dim3(ST_CELLS), 0,
          k_p2g_tile, c.g, (const f3*)c.pos[cur], (const f3*)c.vel[cur], (const float*)c.mass[cur],
          (const int8_t*)c.flag[cur], (const uint32_t*)c.cstart, (const uint32_t*)c.cend, (const uint8_t*)c.cmask,
-         c.p2g_parts, ntx, nty, ntz, band, big);
+         c.p2g_parts, ntx, nty, ntz, band, big, (const uint8_t*)(c.bmask + nbricks(c.g)));
   const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
   const char* nm = band ? "p2g_solid_collect" : "p2g_collect";
   launch(c, nm, dim3(nblocks(fc[0], 256)), dim3(256), 0, k_p2g_collect<0>, c.g, (const float*)c.p2g_parts, ntx, nty,
