@@ -349,6 +349,33 @@ def test_project_odd_grid(pkg, orc, with_air, precision):
         assert e <= TOL
 
 
+@pytest.mark.parametrize("precision", [1, 2])
+def test_project_iteration_cap(pkg, orc, precision):
+    """A solve stopped by its iteration cap (tolerance out of reach): the
+    library reports no convergence, stops at the cap with a finite pressure
+    and a residual below the initial one, and the stopping launch still
+    advances the solution (its later launches do nothing)."""
+    say("project at the iteration cap", "precision", precision)
+    cfg, sim = _odd_grid_sim(pkg, False, precision)
+    cfg2 = pkg.Config.from_buffer_copy(cfg)
+    cfg2.tol = 1e-14
+    cfg2.max_iters = 7
+    s2 = pkg.Simulator(cfg2)
+    p = sim.get_particles()
+    s2.set_particles(len(p["flag"]), **p)
+    s2.hash_sort()
+    s2.levelset_update()
+    s2.p2g()
+    s2.extrapolate_bc(True)
+    gi = s2.project(allow_no_convergence=True)
+    say("  gpu", gi)
+    assert gi["converged"] == 0
+    assert 6 <= gi["iterations"] <= 8
+    assert np.isfinite(gi["rel_residual"]) and 0.0 < gi["rel_residual"] < 1.0
+    pr = s2.get_field("PRESSURE")
+    assert np.all(np.isfinite(pr)) and np.any(pr != 0)
+
+
 @pytest.mark.parametrize("dims", [(13, 17, 23), (16, 24, 24)])
 def test_levelset_extrapolate_odd_grid(pkg, orc, dims):
     """Level set and extrapolation on grids whose sizes are not multiples of
