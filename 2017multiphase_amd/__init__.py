@@ -89,6 +89,7 @@ class Config(ctypes.Structure):
         ("dissolve_gas_rate", ctypes.c_float), ("vapor_gen_rate", ctypes.c_float),
         ("mc_enabled", ctypes.c_int32), ("mc_max_verts", ctypes.c_int64), ("mc_max_tris", ctypes.c_int64),
         ("seed", ctypes.c_uint64),
+        ("adjoint_enabled", ctypes.c_int32), ("pad2_", ctypes.c_int32),
     ]
 
 

@@ -458,18 +458,30 @@ static int64_t face_count(const Ctx& c, int q) {
   return fc[q % 3];
 }
 
-static bool adj_alloc(Ctx& c) {
-  if (c.adj_ready) return true;
-  for (int q = 0; q < 6; ++q) {
-    if (hipMalloc((void**)&c.adj_fin[q], (size_t)face_count(c, q) * 8) != hipSuccess) return false;
-    if (hipMalloc((void**)&c.adj_fout[q], (size_t)face_count(c, q) * 8) != hipSuccess) return false;
-  }
+// Adjoint scratch, allocated by mpf_create when cfg.adjoint_enabled is set
+// (a failure part-way leaves the pointers set so that adj_free releases them).
+bool adj_alloc_all(Ctx& c) {
+  auto get = [&](double*& p, size_t bytes) {
+    void* q = nullptr;
+    if (hipMalloc(&q, bytes) != hipSuccess) return false;
+    p = (double*)q;
+    return true;
+  };
+  for (int q = 0; q < 6; ++q)
+    if (!get(c.adj_fin[q], (size_t)face_count(c, q) * 8) || !get(c.adj_fout[q], (size_t)face_count(c, q) * 8))
+      return false;
   for (int q = 0; q < 2; ++q)
-    if (hipMalloc((void**)&c.adj_cell[q], (size_t)c.g.G * 8) != hipSuccess) return false;
+    if (!get(c.adj_cell[q], (size_t)c.g.G * 8)) return false;
   for (int q = 0; q < 3; ++q)
-    if (hipMalloc((void**)&c.adj_part[q], (size_t)(c.cap > 0 ? c.cap : 1) * 24) != hipSuccess) return false;
+    if (!get(c.adj_part[q], (size_t)(c.cap > 0 ? c.cap : 1) * 24)) return false;
   c.adj_ready = true;
   return true;
+}
+
+static bool adj_alloc(Ctx& c) {
+  if (c.adj_ready) return true;
+  c.err = "adjoint entry points need mpf_config.adjoint_enabled = 1 at mpf_create";
+  return false;
 }
 
 void adj_free(Ctx& c) {
@@ -503,10 +515,7 @@ static void zero(Ctx& c, double* a, int64_t n) {
 }
 
 int run_project_adjoint(Ctx& c, const double* const* gout, double* const* gin, int where, mpf_solve_info* info) {
-  if (!adj_alloc(c)) {
-    c.err = "adjoint scratch allocation failed";
-    return MPF_CAPACITY;
-  }
+  if (!adj_alloc(c)) return MPF_INVALID_ARG;
   for (int q = 0; q < 6; ++q) copy_in(c, c.adj_fin[q], gout ? gout[q] : nullptr, face_count(c, q), where);
   const bool vd = c.cfg.op == MPF_OPERATOR_VARDENS;
   const double bw = vd ? 1.0 / (double)c.cfg.rho_liquid : 1.0, ba = vd ? 1.0 / (double)c.cfg.rho_air : 1.0;
@@ -533,10 +542,7 @@ int run_project_adjoint(Ctx& c, const double* const* gout, double* const* gin, i
 }
 
 int run_extrapolate_adjoint(Ctx& c, const double* const* gout, double* const* gin, int where) {
-  if (!adj_alloc(c)) {
-    c.err = "adjoint scratch allocation failed";
-    return MPF_CAPACITY;
-  }
+  if (!adj_alloc(c)) return MPF_INVALID_ARG;
   for (int q = 0; q < 6; ++q) copy_in(c, c.adj_fin[q], gout ? gout[q] : nullptr, face_count(c, q), where);
   const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
   for (int ph = 0; ph < 2; ++ph) {
@@ -568,10 +574,7 @@ int run_extrapolate_adjoint(Ctx& c, const double* const* gout, double* const* gi
 }
 
 int run_p2g_adjoint(Ctx& c, const double* const* gfaces, double* gvel, double* gpos, int where) {
-  if (!adj_alloc(c)) {
-    c.err = "adjoint scratch allocation failed";
-    return MPF_CAPACITY;
-  }
+  if (!adj_alloc(c)) return MPF_INVALID_ARG;
   const int cur = c.cur;
   const int64_t P = c.P;
   for (int q = 0; q < 6; ++q) copy_in(c, c.adj_fin[q], gfaces ? gfaces[q] : nullptr, face_count(c, q), where);
@@ -618,10 +621,7 @@ int run_p2g_adjoint(Ctx& c, const double* const* gfaces, double* gvel, double* g
 
 int run_g2p_adjoint(Ctx& c, const double* gvel_out, double* gvel_in, double* const* gu, double* const* guo,
                     double* gpos, int where) {
-  if (!adj_alloc(c)) {
-    c.err = "adjoint scratch allocation failed";
-    return MPF_CAPACITY;
-  }
+  if (!adj_alloc(c)) return MPF_INVALID_ARG;
   const int cur = c.cur;
   const int64_t P = c.P;
   copy_in(c, c.adj_part[0], gvel_out, 3 * P, where);
@@ -656,10 +656,7 @@ int run_g2p_adjoint(Ctx& c, const double* gvel_out, double* gvel_in, double* con
 
 int run_advect_adjoint(Ctx& c, const double* gpos_out, const double* gvel_out, double* gpos_in, double* gvel_in,
                        int where, int64_t* in_band) {
-  if (!adj_alloc(c)) {
-    c.err = "adjoint scratch allocation failed";
-    return MPF_CAPACITY;
-  }
+  if (!adj_alloc(c)) return MPF_INVALID_ARG;
   if (c.cfg.integrator != MPF_ADVECT_EULER_COLLIDE) {
     c.err = "advection adjoint is implemented for the Euler/collision integrator";
     return MPF_INVALID_ARG;
@@ -668,8 +665,9 @@ int run_advect_adjoint(Ctx& c, const double* gpos_out, const double* gvel_out, d
   const int64_t P = c.P;
   copy_in(c, c.adj_part[0], gvel_out, 3 * P, where);
   copy_in(c, c.adj_part[2], gpos_out, 3 * P, where);
-  // the solid level set of this step's forward call (recomputed from the cell lists)
-  if (c.sorted) run_solid_phi_and_velocity(c);
+  // the solid level set of this step's forward call (recomputed from the cell
+  // lists, which the entry point requires to be current)
+  run_solid_phi_and_velocity(c);
   hipMemsetAsync(c.counters + 4, 0, 4, c.stream);
   // g_x -> adj_part[1]; g_v is updated in place in adj_part[0] (each thread
   // reads its own entries before writing them)

@@ -180,7 +180,11 @@ bool alloc_all(Ctx& c) {
          dalloc(c, c.mc_tcount, 2 * (V > 0 ? V : 1)) && dalloc(c, c.mc_verts, 3 * c.cfg.mc_max_verts) &&
          dalloc(c, c.mc_tris, 3 * c.cfg.mc_max_tris);
   }
-  if (!ok) return false;
+  if (ok && c.cfg.adjoint_enabled) ok = adj_alloc_all(c);
+  if (!ok) {
+    if (c.err.empty()) c.err = "hipMalloc failed (adjoint scratch)";
+    return false;
+  }
   if (hipHostMalloc((void**)&c.scal_host, sizeof(SolveScalars), hipHostMallocDefault) != hipSuccess ||
       hipHostMalloc((void**)&c.counters_host, NCOUNTERS * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
     c.err = "hipHostMalloc failed";
@@ -575,11 +579,13 @@ mpf_status mpf_project(mpf_ctx* ctx, mpf_solve_info* info) {
   c.err.clear();
   run_divergence(c);
   int st = run_pressure_solve(c, info);
-  if (st == MPF_HIP_ERROR || st == MPF_NONFINITE) return (mpf_status)st;
+  // any failure other than stopping at the cap leaves the pressure unusable:
+  // return its status before the gradient step touches the face velocities
+  if (st != MPF_OK && st != MPF_NO_CONVERGENCE) return (mpf_status)st;
   run_subgrad(c);
   mpf_status ls = check_launch(c);
   if (ls != MPF_OK) return ls;
-  if (st == MPF_NO_CONVERGENCE) c.err = "pressure solve stopped at its iteration cap";
+  if (st == MPF_NO_CONVERGENCE && c.err.empty()) c.err = "pressure solve stopped at its iteration cap";
   return (mpf_status)st;
 }
 
@@ -637,6 +643,7 @@ mpf_status mpf_g2p_adjoint(mpf_ctx* ctx, const double* g_vel_out, double* g_vel_
 mpf_status mpf_advect_adjoint(mpf_ctx* ctx, const double* g_pos_out, const double* g_vel_out, double* g_pos_in,
                               double* g_vel_in, int32_t where, int64_t* in_band) {
   ADJ_PRELUDE(ctx)
+  REQUIRE_SORTED(c);
   return (mpf_status)run_advect_adjoint(c, g_pos_out, g_vel_out, g_pos_in, g_vel_in, where, in_band);
 }
 
@@ -649,7 +656,7 @@ mpf_status mpf_heat(mpf_ctx* ctx, mpf_solve_info* info) {
   int st = MPF_OK;
   if (c.cfg.heat_solver == MPF_HEAT_CG) {
     st = run_heat_cg(c, info);
-    if (st == MPF_HIP_ERROR || st == MPF_NONFINITE) return (mpf_status)st;
+    if (st != MPF_OK && st != MPF_NO_CONVERGENCE) return (mpf_status)st;
   } else {
     run_heat_jacobi(c, c.cfg.heat_jacobi_iters);
     if (info) {
@@ -690,6 +697,7 @@ mpf_status mpf_get_mesh(mpf_ctx* ctx, float* verts, uint32_t* tris, int32_t wher
   if (!ctx) return MPF_INVALID_ARG;
   Ctx& c = ctx->c;
   c.err.clear();
+  if (where != MPF_HOST && where != MPF_DEVICE) return fail(c, MPF_INVALID_ARG, "bad memory location");
   hipMemcpyKind kind = where == MPF_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
   if (verts && c.mc_nv) hipMemcpyAsync(verts, c.mc_verts, (size_t)c.mc_nv * 12, kind, c.stream);
   if (tris && c.mc_nt) hipMemcpyAsync(tris, c.mc_tris, (size_t)c.mc_nt * 12, kind, c.stream);
@@ -700,6 +708,7 @@ mpf_status mpf_get_mc_cases(mpf_ctx* ctx, uint8_t* cases, int32_t where) {
   if (!ctx || !cases) return MPF_INVALID_ARG;
   Ctx& c = ctx->c;
   c.err.clear();
+  if (where != MPF_HOST && where != MPF_DEVICE) return fail(c, MPF_INVALID_ARG, "bad memory location");
   if (!c.mc_case) return fail(c, MPF_INVALID_ARG, "marching cubes disabled in the configuration");
   hipMemcpyKind kind = where == MPF_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
   if (c.mc_nvox) hipMemcpyAsync(cases, c.mc_case, (size_t)c.mc_nvox, kind, c.stream);
@@ -720,6 +729,7 @@ mpf_status mpf_step(mpf_ctx* ctx, int32_t frame, mpf_step_info* info) {
   if ((st = mpf_extrapolate_bc(ctx, 1)) != MPF_OK) return st;
   mpf_status pst = mpf_project(ctx, &loc.project);
   if (pst != MPF_OK && pst != MPF_NO_CONVERGENCE) return pst;
+  const std::string perr = c.err;
   if ((st = mpf_extrapolate_bc(ctx, 0)) != MPF_OK) return st;
   if ((st = mpf_g2p(ctx)) != MPF_OK) return st;
   if ((st = mpf_advect(ctx)) != MPF_OK) return st;
@@ -730,7 +740,7 @@ mpf_status mpf_step(mpf_ctx* ctx, int32_t frame, mpf_step_info* info) {
   if (c.cfg.mc_enabled && (st = mpf_marching_cubes(ctx, &loc.mc)) != MPF_OK) return st;
   loc.particles = c.P;
   if (info) *info = loc;
-  if (pst == MPF_NO_CONVERGENCE) c.err = "pressure solve stopped at its iteration cap";
+  if (pst == MPF_NO_CONVERGENCE) c.err = perr;
   return pst;
 }
 

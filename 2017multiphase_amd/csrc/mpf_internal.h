@@ -290,6 +290,7 @@ int run_pressure_solve(Ctx& c, mpf_solve_info* info);
 int run_heat_cg(Ctx& c, mpf_solve_info* info);
 int run_adjoint_solve(Ctx& c, const double* rhs, double* sol, mpf_solve_info* info);
 // adjoint.hip
+bool adj_alloc_all(Ctx& c);
 void adj_free(Ctx& c);
 int run_project_adjoint(Ctx& c, const double* const* gout, double* const* gin, int where, mpf_solve_info* info);
 int run_extrapolate_adjoint(Ctx& c, const double* const* gout, double* const* gin, int where);

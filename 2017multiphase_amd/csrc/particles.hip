@@ -120,20 +120,25 @@ __global__ void k_collide_move(Grid g, int64_t P, f3* __restrict__ pos, f3* __re
   pos[p] = x;
 }
 
-// advectparticle_RK2 spray_k.cu:409-459 (positions; the blend is in G2P)
+// advectparticle_RK2_bubble spray_k.cu:3463-3523: FLUID particles on the water
+// grid, AIR particles on the air grid, every other type untouched; wall
+// margin h + 0.5h; clamped components of the velocity zeroed (the FLIP/PIC
+// blend of that kernel is done by G2P).  The position is advanced with the
+// midpoint step, as advectparticle_RK2 (spray_k.cu:409-459) writes it (the
+// bubble kernel's position write is commented out, spray_k.cu:3520-3521).
 __global__ void k_advect_rk2(Grid g, int64_t P, f3* __restrict__ pos, f3* __restrict__ vel,
-                             const int8_t* __restrict__ flag, Faces3 wu, Faces3 au, float dt, float ss) {
+                             const int8_t* __restrict__ flag, Faces3 wu, Faces3 au, float dt) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
   int8_t fl = flag[p];
-  if (fl == AIRSOLO) return;
+  if (fl != FLUID && fl != AIR) return;
   // the phase's face arrays chosen by value (no per-lane pointer load)
   const bool air = fl == AIR;
   const float* u0 = air ? au.u[0] : wu.u[0];
   const float* u1 = air ? au.u[1] : wu.u[1];
   const float* u2 = air ? au.u[2] : wu.u[2];
   f3 x = pos[p], v = vel[p];
-  float m = g.h + 0.5f * ss;
+  float m = g.h + 0.5f * g.h;
   f3 lo = mk3(g.gx0 + m, g.gy0 + m, g.gz0 + m), hi = mk3(g.gx1 - m, g.gy1 - m, g.gz1 - m);
   f3 gv = vel_from_grid(g, x, u0, u1, u2);
   f3 mid = add3(x, mul3(mul3(gv, dt), 0.5f));
@@ -145,7 +150,7 @@ __global__ void k_advect_rk2(Grid g, int64_t P, f3* __restrict__ pos, f3* __rest
   if (x.x >= hi.x) { x.x = hi.x; v.x = 0; }
   if (x.y >= hi.y) { x.y = hi.y; v.y = 0; }
   if (x.z >= hi.z) { x.z = hi.z; v.z = 0; }
-  if (fl != SOLID) pos[p] = x;
+  pos[p] = x;
   vel[p] = v;
 }
 
@@ -384,7 +389,7 @@ void run_advect(Ctx& c) {
   if (c.cfg.integrator == MPF_ADVECT_RK2) {
     Faces3 wu{{c.wu[0], c.wu[1], c.wu[2]}}, au{{c.au[0], c.au[1], c.au[2]}};
     launch(c, "advect_rk2", dim3(nblocks(c.P, 256)), dim3(256), 0, k_advect_rk2, c.g, c.P, c.pos[cur], c.vel[cur],
-           (const int8_t*)c.flag[cur], wu, au, c.cfg.dt, c.cfg.samplespace);
+           (const int8_t*)c.flag[cur], wu, au, c.cfg.dt);
   } else {
     run_solid_phi_and_velocity(c);
     Faces3 su{{c.su[0], c.su[1], c.su[2]}};

@@ -40,6 +40,7 @@ namespace {
 
 constexpr int PB = 512;        // threads per block
 constexpr int MAX_BLK = 1024;  // blocks per solver launch
+constexpr int MAX_RESTARTS = 50;  // MIXED: restarts from the FP64 true residual per solve
 constexpr int OP_UNIT = 0, OP_VARDENS = 1, OP_HEAT = 2;
 constexpr uint8_t NB_UNKNOWN = 64, NB_SLOW = 128;
 
@@ -1465,6 +1466,11 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
          mixed ? x64 : (double*)nullptr, P, c.scal, npad);
   int k = 0;
   bool fold_next = false;
+  // MIXED restarts (the recurrence met the criterion, the FP64 true residual
+  // did not): bounded, so a solve that cannot reach the tolerance in float
+  // vectors stops with MPF_NO_CONVERGENCE instead of cycling
+  int restarts = 0;
+  bool true_missed = false;
   if constexpr (std::is_same<V, float>::value) {
     if (fused) {
       float* rb[2] = {(float*)c.vr, (float*)c.vr2};
@@ -1505,7 +1511,11 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
         launch(c, "pcg_collect", dim3(1), dim3(PB), 0, k_collect_rr4, (const double*)Rs, T.nblk, c.scal);
         if (!read_scalars(c)) return MPF_HIP_ERROR;
         const SolveScalars& t = *c.scal_host;
-        if (t.rr_true == 0.0 || sqrt(t.rr_true) <= tol * sqrt(t.bb) || kd >= cap) break;
+        if (t.rr_true == 0.0 || sqrt(t.rr_true) <= tol * sqrt(t.bb)) break;
+        if (kd >= cap || ++restarts > MAX_RESTARTS) {
+          true_missed = true;
+          break;
+        }
         first = 2;
         fold = 1;
         k = kd;
@@ -1557,6 +1567,10 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
       // after the solve finished returned without running.
       const int kd = s.iter;
       if (kd == 0 || s.last_true == kd - 1) break;
+      if (kd >= cap || ++restarts > MAX_RESTARTS) {
+        true_missed = true;
+        break;
+      }
       // resid64 returns early while done is set: clear it first
       launch(c, "pcg_clear", dim3(1), dim3(64), 0, k_clear_done, c.scal);
       launch(c, "pcg_resid64", dim3(T.nblk), dim3(ST), 0, resid64, o, T, (const float*)b, (const double*)x64,
@@ -1575,10 +1589,11 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
   launch(c, "pcg_collect", dim3(1), dim3(PB), 0, k_collect_rr, (const double*)P.S, L.nblk, c.scal);
   if (!read_scalars(c)) return MPF_HIP_ERROR;
   const SolveScalars& s = *c.scal_host;
+  const bool converged = s.converged && !true_missed;
   if (info) {
     info->iterations = s.iter;
     info->refinements = s.refinements;
-    info->converged = s.converged;
+    info->converged = converged ? 1 : 0;
     info->rel_residual = s.bb > 0 ? sqrt(s.rr_true) / sqrt(s.bb) : 0.0;
     info->abs_rr = s.rr;
   }
@@ -1586,7 +1601,10 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
     c.err = "non-finite value in the solver";
     return MPF_NONFINITE;
   }
-  return s.converged ? MPF_OK : MPF_NO_CONVERGENCE;
+  if (true_missed)
+    c.err = "mixed-precision solve: the FP64 true residual missed the tolerance (" + std::to_string(restarts) +
+            " restarts, iteration " + std::to_string(s.iter) + ")";
+  return converged ? MPF_OK : MPF_NO_CONVERGENCE;
 }
 
 }  // namespace

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define MPF_ABI_VERSION 1
+#define MPF_ABI_VERSION 2
 #define MPF_CELL_UNDEF 0xffffffffu
 
 /* Particle / cell type codes: utility.h:13-19. */
@@ -149,6 +149,11 @@ typedef struct mpf_config {
   int64_t mc_max_verts, mc_max_tris;
 
   uint64_t seed;
+
+  /* ---- appended in ABI version 2 ---- */
+  int32_t adjoint_enabled;   /* allocate the adjoint entry points' scratch at
+                                mpf_create (the adjoints refuse otherwise)    */
+  int32_t pad2_;
 } mpf_config;
 
 /* Opaque context. */

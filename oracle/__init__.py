@@ -49,6 +49,10 @@ def lib():
     L.orc_extrapolate_bc.argtypes = [vp, c.c_int]
     L.orc_project.argtypes = [vp, c.POINTER(pkg.SolveInfo)]
     L.orc_heat.argtypes = [vp, c.POINTER(pkg.SolveInfo)]
+    L.orc_heat_grid.argtypes = [vp, c.POINTER(pkg.SolveInfo)]
+    L.orc_heat_g2p.argtypes = [vp]
+    for n in ("orc_phigrad", "orc_mark_from_counts", "orc_mark_solid_cells"):
+        getattr(L, n).argtypes = [vp]
     L.orc_phase_change.argtypes = [vp, c.c_int, c.POINTER(pkg.PhaseInfo)]
     L.orc_marching_cubes.argtypes = [vp, c.POINTER(pkg.McInfo)]
     L.orc_get_mesh.argtypes = [vp, vp, vp]
@@ -187,6 +191,25 @@ class Oracle:
         info = self.pkg.SolveInfo()
         self._ok(self.L.orc_heat(self.h, ctypes.byref(info)))
         return info.as_dict()
+
+    def heat_grid(self):
+        """Grid step of the heat update on the current TEMPERATURE field."""
+        info = self.pkg.SolveInfo()
+        self._ok(self.L.orc_heat_grid(self.h, ctypes.byref(info)))
+        return info.as_dict()
+
+    def heat_g2p(self):
+        """Grid-to-particle temperature map and latent-heat rules."""
+        self._ok(self.L.orc_heat_g2p(self.h))
+
+    def phigrad(self):
+        self._ok(self.L.orc_phigrad(self.h))
+
+    def mark_from_counts(self):
+        self._ok(self.L.orc_mark_from_counts(self.h))
+
+    def mark_solid_cells(self):
+        self._ok(self.L.orc_mark_solid_cells(self.h))
 
     def phase_change(self, frame):
         info = self.pkg.PhaseInfo()

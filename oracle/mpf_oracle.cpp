@@ -882,19 +882,21 @@ template <class T> struct Orc : OrcBase {
       S3(pos, p, x);
     }
   }
-  // advectparticle_RK2 spray_k.cu:409-459 (position part; velocity blend is in g2p),
-  // two-phase grids as advectparticle_RK2_bubble spray_k.cu:3463-3523.
+  // advectparticle_RK2_bubble spray_k.cu:3463-3523: FLUID on the water grid,
+  // AIR on the air grid, other types untouched; wall margin h + 0.5h; the
+  // clamped velocity components zeroed (the FLIP/PIC blend is in g2p).  The
+  // position takes the midpoint step as advectparticle_RK2 (409-459) writes
+  // it; the bubble kernel's own position write is commented out (3520-3521).
   void advect_rk2() {
     const T dt = (T)c.dt;
-    const T ss = (T)c.samplespace;
 #pragma omp parallel for schedule(static)
     for (int64_t p = 0; p < P; ++p) {
       int8_t fl = flag[p];
-      if (fl == AIRSOLO) continue;
+      if (fl != FLUID && fl != AIR) continue;
       const std::vector<T>* g = (fl == AIR) ? au : wu;
       V3<T> x = P3(pos, p), v = P3(vel, p);
-      V3<T> lo = gmin() + mk(h + T(0.5f) * ss, h + T(0.5f) * ss, h + T(0.5f) * ss);
-      V3<T> hi = gmax() - mk(h + T(0.5f) * ss, h + T(0.5f) * ss, h + T(0.5f) * ss);
+      V3<T> lo = gmin() + mk(h + T(0.5f) * h, h + T(0.5f) * h, h + T(0.5f) * h);
+      V3<T> hi = gmax() - mk(h + T(0.5f) * h, h + T(0.5f) * h, h + T(0.5f) * h);
       V3<T> gv = vel_from_grid(x, g);
       V3<T> mid = x + (gv * dt) * T(0.5);
       V3<T> gm = vel_from_grid(mid, g);
@@ -905,7 +907,7 @@ template <class T> struct Orc : OrcBase {
       if (x.x >= hi.x) { x.x = hi.x; v.x = 0; }
       if (x.y >= hi.y) { x.y = hi.y; v.y = 0; }
       if (x.z >= hi.z) { x.z = hi.z; v.z = 0; }
-      if (fl != SOLID) S3(pos, p, x);
+      S3(pos, p, x);
       S3(vel, p, v);
     }
   }
@@ -967,6 +969,13 @@ template <class T> struct Orc : OrcBase {
   }
   int heat(mpf_solve_info* info) {
     heat_p2g();
+    heat_grid(info);
+    heat_g2p();
+    return MPF_OK;
+  }
+  // Grid part of updateTemperature (spray.cu:1897-1929) on the current tp:
+  // heater, implicit diffusion, boundary copy, tpchange.
+  void heat_grid(mpf_solve_info* info) {
     if (c.heat_add_bottom) {  // addHeatAtBottom spray_k.cu:4492-4504
       for (int64_t idx = 0; idx < G; ++idx) {
         int i, j, k;
@@ -1054,7 +1063,15 @@ template <class T> struct Orc : OrcBase {
     }
     // compTpChange spray_k.cu:4607-4617
     for (int64_t i = 0; i < G; ++i) tpchange[i] = (mark[i] != BOUNDARY) ? tp[i] - tsave[i] : T(0);
-    // mapHeatg2p / mapHeatg2p_MeltAndBoil spray_k.cu:4655-4679 ; updateLatentHeat_k 5546-5573
+    if (info) {
+      info->iterations = it; info->refinements = 0; info->converged = conv ? 1 : 0;
+      double bn = std::sqrt(rsum(G, [&](int64_t i) { return b[i] * b[i]; }));
+      info->rel_residual = bn > 0 ? std::sqrt(rr) / bn : 0.0;
+      info->abs_rr = rr;
+    }
+  }
+  // mapHeatg2p / mapHeatg2p_MeltAndBoil spray_k.cu:4655-4679 ; updateLatentHeat_k 5546-5573
+  void heat_g2p() {
     const T at = (T)c.alpha_temp_trans;
     const T mp = (T)c.melting_point, bp = (T)c.boiling_point, th = (T)c.liquid_heat_th;
 #pragma omp parallel for schedule(static)
@@ -1070,13 +1087,6 @@ template <class T> struct Orc : OrcBase {
         else lheat[p] = th;
       }
     }
-    if (info) {
-      info->iterations = it; info->refinements = 0; info->converged = conv ? 1 : 0;
-      double bn = std::sqrt(rsum(G, [&](int64_t i) { return b[i] * b[i]; }));
-      info->rel_residual = bn > 0 ? std::sqrt(rr) / bn : 0.0;
-      info->abs_rr = rr;
-    }
-    return MPF_OK;
   }
 
   // ================= a17: phase change =================
@@ -1460,6 +1470,13 @@ int orc_project(orc_state* s, mpf_solve_info* info) { return dispatch(s, [&](aut
 int orc_g2p(orc_state* s) { return dispatch(s, [](auto& o) { o.g2p(); return 0; }); }
 int orc_advect(orc_state* s) { return dispatch(s, [](auto& o) { o.advect(); return 0; }); }
 int orc_heat(orc_state* s, mpf_solve_info* info) { return dispatch(s, [&](auto& o) { return o.heat(info); }); }
+int orc_heat_grid(orc_state* s, mpf_solve_info* info) {
+  return dispatch(s, [&](auto& o) { o.heat_grid(info); return 0; });
+}
+int orc_heat_g2p(orc_state* s) { return dispatch(s, [&](auto& o) { o.heat_g2p(); return 0; }); }
+int orc_phigrad(orc_state* s) { return dispatch(s, [&](auto& o) { o.phigrad(); return 0; }); }
+int orc_mark_from_counts(orc_state* s) { return dispatch(s, [&](auto& o) { o.mark_from_counts(); return 0; }); }
+int orc_mark_solid_cells(orc_state* s) { return dispatch(s, [&](auto& o) { o.mark_solid_cells(); return 0; }); }
 int orc_phase_change(orc_state* s, int frame, mpf_phase_info* info) {
   return dispatch(s, [&](auto& o) { return o.phase_change(frame, info); });
 }

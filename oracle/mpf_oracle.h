@@ -59,6 +59,17 @@ int orc_project(orc_state* s, mpf_solve_info* info);
 int orc_g2p(orc_state* s);
 int orc_advect(orc_state* s);
 int orc_heat(orc_state* s, mpf_solve_info* info);
+/* The two halves of orc_heat after the particle-to-grid map: the grid step on
+   the state's TEMPERATURE (heater, implicit diffusion, boundary copy,
+   TEMPERATURE_CHANGE), and the grid-to-particle map with latent heat. */
+int orc_heat_grid(orc_state* s, mpf_solve_info* info);
+int orc_heat_g2p(orc_state* s);
+/* Single pieces of orc_levelset_update on the state's fields: the level-set
+   gradient of LSMERGE, and the count-based and solid cell flags (MARK) from
+   the current cell lists. */
+int orc_phigrad(orc_state* s);
+int orc_mark_from_counts(orc_state* s);
+int orc_mark_solid_cells(orc_state* s);
 int orc_phase_change(orc_state* s, int frame, mpf_phase_info* info);
 int orc_marching_cubes(orc_state* s, mpf_mc_info* info);
 int orc_get_mesh(orc_state* s, double* verts, uint32_t* tris);

@@ -315,6 +315,7 @@ def adjoint_gradient(pkg, scene, precision, steps, checkpoint, u_init, s, target
     cfg.precision = precision
     cfg.tol = tol
     cfg.max_iters = 20000
+    cfg.adjoint_enabled = 1
     sim = pkg.Simulator(cfg)
     icfg = pkg.Config.from_buffer_copy(cfg)
     icfg.flip_alpha = 0.0

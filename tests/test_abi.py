@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol(pkg):
 
 
 def test_abi_version(pkg):
-    assert pkg.lib().mpf_abi_version() == 1
+    assert pkg.lib().mpf_abi_version() == 2
 
 
 def test_config_default_without_gpu(pkg):

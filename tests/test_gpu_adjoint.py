@@ -33,6 +33,7 @@ FD_EPS = 1e-7  # in cell sizes
 
 def _make(pkg, scenes, name, n, **over):
     s = scenes.make(name, n=n)
+    s["cfg"].adjoint_enabled = 1
     for k, v in over.items():
         setattr(s["cfg"], k, v)
     sim = pkg.Simulator(s["cfg"])
