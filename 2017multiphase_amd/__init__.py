@@ -7,8 +7,9 @@
 mapvelp2g_bubble, sweepU/setWaterBoundaryU, project_CG_bubble,
 mapvelg2p_bubble, CollisionSolid, updateTemperature, MeltSolid/Freezing/
 GenerateGasParticle, runMC_smooth) and `step()` for bubblesim
-(spray.cpp:979-1233).  Host buffers are numpy arrays; device buffers may be
-passed as torch tensors (anything with .data_ptr()).
+(spray.cpp:979-1233).  Host buffers are numpy arrays; device buffers are
+anything with .data_ptr() that the library's HIP runtime allocated
+(hipmem.DeviceArray).
 
 The compute path is the HIP library `libmpflip.so` built next to this file;
 importing fails loudly when it is missing — there is no CPU fallback.
@@ -54,7 +55,8 @@ ABI_FUNCTIONS = [
     "mpf_marching_cubes", "mpf_get_mesh", "mpf_get_mc_cases", "mpf_step",
     "mpf_set_kernel_timing", "mpf_kernel_times", "mpf_reset_kernel_times",
     "mpf_project_adjoint", "mpf_extrapolate_adjoint", "mpf_p2g_adjoint", "mpf_g2p_adjoint",
-    "mpf_advect_adjoint",
+    "mpf_advect_adjoint", "mpf_hash_sort_noalloc", "mpf_update_seed_cells", "mpf_set_pour_source",
+    "mpf_delete_particles", "mpf_pour",
 ]
 
 
@@ -89,7 +91,10 @@ class Config(ctypes.Structure):
         ("dissolve_gas_rate", ctypes.c_float), ("vapor_gen_rate", ctypes.c_float),
         ("mc_enabled", ctypes.c_int32), ("mc_max_verts", ctypes.c_int64), ("mc_max_tris", ctypes.c_int64),
         ("seed", ctypes.c_uint64),
-        ("adjoint_enabled", ctypes.c_int32), ("pad2_", ctypes.c_int32),
+        ("adjoint_enabled", ctypes.c_int32),
+        ("delete_enabled", ctypes.c_int32), ("delete_vacuum_sweeps", ctypes.c_int32),
+        ("pour_every", ctypes.c_int32), ("pour_pos_rand", ctypes.c_float), ("pour_vel_rand", ctypes.c_float),
+        ("pour_temperature", ctypes.c_float), ("seed_update_every", ctypes.c_int32),
     ]
 
 
@@ -104,7 +109,7 @@ class SolveInfo(ctypes.Structure):
 
 class PhaseInfo(ctypes.Structure):
     _fields_ = [("melted", ctypes.c_int32), ("frozen", ctypes.c_int32), ("gas_added", ctypes.c_int32),
-                ("pad_", ctypes.c_int32), ("particles", ctypes.c_int64)]
+                ("deleted", ctypes.c_int32), ("particles", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad_"}
@@ -119,7 +124,7 @@ class McInfo(ctypes.Structure):
 
 class StepInfo(ctypes.Structure):
     _fields_ = [("project", SolveInfo), ("heat", SolveInfo), ("phase", PhaseInfo), ("mc", McInfo),
-                ("particles", ctypes.c_int64)]
+                ("particles", ctypes.c_int64), ("poured", ctypes.c_int64)]
 
 
 _lib_handle = None
@@ -172,6 +177,11 @@ def lib():
     L.mpf_p2g_adjoint.argtypes = [vp, vp, vp, vp, i32]
     L.mpf_g2p_adjoint.argtypes = [vp, vp, vp, vp, vp, vp, i32]
     L.mpf_advect_adjoint.argtypes = [vp, vp, vp, vp, vp, i32, c.POINTER(i64)]
+    L.mpf_hash_sort_noalloc.argtypes = [c.POINTER(Config), i64, vp, vp, vp, vp, vp, vp]
+    L.mpf_update_seed_cells.argtypes = [vp, i32]
+    L.mpf_set_pour_source.argtypes = [vp, i64, vp, vp, c.c_int8]
+    L.mpf_delete_particles.argtypes = [vp, c.POINTER(i64)]
+    L.mpf_pour.argtypes = [vp, i32, c.POINTER(i64)]
     _lib_handle = L
     return L
 
@@ -190,6 +200,22 @@ def default_config(nx, ny, nz, **overrides):
         else:
             setattr(cfg, k, v)
     return cfg
+
+
+def hash_sort_noalloc(cfg, n, pos, grid_hash, grid_index, cell_start, cell_end, stream=None):
+    """mpf_hash_sort_noalloc on device buffers (hipmem.DeviceArray or raw
+    device addresses as int): no allocation, asynchronous on `stream` (a raw
+    hipStream_t as int, or None for the null stream)."""
+    def addr(a):
+        if a is None:
+            return None
+        return ctypes.c_void_p(a.data_ptr() if hasattr(a, "data_ptr") else int(a))
+    # device buffers: hipmem.DeviceArray (the library's own HIP runtime) or raw addresses
+    L = lib()
+    st = L.mpf_hash_sort_noalloc(ctypes.byref(cfg), int(n), addr(pos), addr(grid_hash), addr(grid_index),
+                                 addr(cell_start), addr(cell_end), ctypes.c_void_p(stream) if stream else None)
+    if st != OK:
+        raise MpfError(st, L.mpf_last_error(None).decode())
 
 
 class MpfError(RuntimeError):
@@ -366,12 +392,32 @@ class Simulator:
         self._check(self.L.mpf_get_mc_cases(self.h, _ptr(a), HOST))
         return a
 
+    def update_seed_cells(self, frame):
+        self._check(self.L.mpf_update_seed_cells(self.h, frame))
+
+    def set_pour_source(self, pos, vel, flag=TYPEFLUID):
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, dtype=np.float32).reshape(-1, 3)
+        assert pos.shape == vel.shape
+        self._check(self.L.mpf_set_pour_source(self.h, len(pos), _ptr(pos), _ptr(vel), flag))
+
+    def delete_particles(self):
+        n = ctypes.c_int64()
+        self._check(self.L.mpf_delete_particles(self.h, ctypes.byref(n)))
+        return int(n.value)
+
+    def pour(self, frame):
+        n = ctypes.c_int64()
+        self._check(self.L.mpf_pour(self.h, frame, ctypes.byref(n)))
+        return int(n.value)
+
     def step(self, frame, allow_no_convergence=False):
         info = StepInfo()
         allow = (OK, NO_CONVERGENCE) if allow_no_convergence else (OK,)
         self._check(self.L.mpf_step(self.h, frame, ctypes.byref(info)), allow)
         return {"project": info.project.as_dict(), "heat": info.heat.as_dict(),
-                "phase": info.phase.as_dict(), "mc": info.mc.as_dict(), "particles": info.particles}
+                "phase": info.phase.as_dict(), "mc": info.mc.as_dict(), "particles": info.particles,
+                "poured": info.poured}
 
     # -- adjoints (FP64 cotangents in host memory) --
     def face_sizes(self):

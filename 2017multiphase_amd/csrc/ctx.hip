@@ -79,9 +79,13 @@ struct mpf_ctx {
   Ctx c;
 };
 
-namespace {
+namespace mpf {
+// message of the last failing call made without a context (mpf_create,
+// mpf_hash_sort_noalloc), returned by mpf_last_error(NULL)
+thread_local std::string g_noctx_err;
+}  // namespace mpf
 
-thread_local std::string g_create_err;
+namespace {
 
 template <class T>
 bool dalloc(Ctx& c, T*& p, size_t n) {
@@ -122,6 +126,8 @@ void free_all(Ctx& c) {
   dfree(c.scan_blk); dfree(c.ctmp_u32); dfree(c.p2g_parts);
   dfree(c.mc_case); dfree(c.mc_edge); dfree(c.mc_tcount); dfree(c.mc_verts); dfree(c.mc_tris);
   dfree(c.seeds);
+  dfree(c.pour_pos);
+  dfree(c.pour_vel);
   adj_free(c);
   if (c.scal_host) hipHostFree(c.scal_host);
   if (c.counters_host) hipHostFree(c.counters_host);
@@ -323,12 +329,20 @@ mpf_status mpf_config_default(mpf_config* cfg, int32_t nx, int32_t ny, int32_t n
   cfg->mc_max_verts = G > (1 << 20) ? G : (1 << 20);
   cfg->mc_max_tris = cfg->mc_max_verts;
   cfg->seed = 0;
+  cfg->adjoint_enabled = 0;
+  cfg->delete_enabled = 0;
+  cfg->delete_vacuum_sweeps = 20;            // spray.cu:2205-2206
+  cfg->pour_every = 4;                       // spray.cu:3014
+  cfg->pour_pos_rand = 0.1f;                 // spray.cpp:55
+  cfg->pour_vel_rand = 0.0f;
+  cfg->pour_temperature = 293.15f;           // spray.cpp:54
+  cfg->seed_update_every = 0;
   return MPF_OK;
 }
 
 mpf_status mpf_create(const mpf_config* cfg, mpf_ctx** out) {
   if (!cfg || !out) {
-    g_create_err = "null argument";
+    g_noctx_err = "null argument";
     return MPF_INVALID_ARG;
   }
   *out = nullptr;
@@ -337,7 +351,7 @@ mpf_status mpf_create(const mpf_config* cfg, mpf_ctx** out) {
       cfg->precision < 0 || cfg->precision > 2 || cfg->solver < 0 || cfg->solver > 1 || cfg->op < 0 || cfg->op > 1 ||
       (cfg->op == MPF_OPERATOR_VARDENS && !(cfg->rho_liquid > 0 && cfg->rho_air > 0)) ||
       (cfg->heat_enabled && cfg->heat_solver == MPF_HEAT_CG && cfg->heat_alpha[0] <= 0)) {
-    g_create_err = "invalid configuration";
+    g_noctx_err = "invalid configuration";
     return MPF_INVALID_ARG;
   }
   mpf_ctx* h = new (std::nothrow) mpf_ctx;
@@ -363,12 +377,12 @@ mpf_status mpf_create(const mpf_config* cfg, mpf_ctx** out) {
   c.cap = cfg->particle_capacity;
   hipGetDevice(&c.device);
   if (!hip_ok(c, hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "stream")) {
-    g_create_err = c.err;
+    g_noctx_err = c.err;
     delete h;
     return MPF_HIP_ERROR;
   }
   if (!alloc_all(c)) {
-    g_create_err = c.err;
+    g_noctx_err = c.err;
     free_all(c);
     delete h;
     return MPF_CAPACITY;
@@ -386,7 +400,7 @@ mpf_status mpf_destroy(mpf_ctx* ctx) {
 }
 
 const char* mpf_last_error(mpf_ctx* ctx) {
-  if (!ctx) return g_create_err.c_str();
+  if (!ctx) return g_noctx_err.c_str();
   return ctx->c.err.c_str();
 }
 
@@ -519,13 +533,91 @@ mpf_status mpf_set_seed_cells(mpf_ctx* ctx, const int32_t* cells, int32_t n) {
   c.err.clear();
   for (int32_t i = 0; i < n; ++i)
     if (cells[i] < 0 || cells[i] >= c.g.G) return fail(c, MPF_INVALID_ARG, "seed cell out of range");
+  hipStreamSynchronize(c.stream);
   dfree(c.seeds);
   c.nseeds = 0;
+  c.seed_ring = 0;
+  c.seeds_host.assign(cells, cells + n);
   if (n == 0) return MPF_OK;
   if (!dalloc(c, c.seeds, (size_t)n)) return MPF_CAPACITY;
   hipMemcpyAsync(c.seeds, cells, (size_t)n * 4, hipMemcpyHostToDevice, c.stream);
   c.nseeds = n;
   return hip_ok(c, hipStreamSynchronize(c.stream), "set_seed_cells") ? MPF_OK : MPF_HIP_ERROR;
+}
+
+// Counter hash of the nucleation and pour random numbers (SplitMix64
+// finalizer, Steele, Lea, Flood, OOPSLA 2014), host side.
+static uint64_t mix64_host(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+mpf_status mpf_update_seed_cells(mpf_ctx* ctx, int32_t frame) {
+  if (!ctx) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  const int every = c.cfg.seed_update_every;
+  if (every <= 0 || c.nseeds <= 0 || frame % every != 0) return MPF_OK;
+  const int bound = 3, z = 2;
+  if (c.g.nx <= 2 * bound || c.g.ny <= 2 * bound || c.g.nz <= z) return MPF_OK;
+  // four replacements per refresh (the reference's `while (cnt++ <= 3)`)
+  const uint64_t tag = 0x73656564ull << 32;
+  for (int q = 0; q < 4; ++q) {
+    const uint64_t hx = mix64_host(c.cfg.seed ^ mix64_host(tag ^ mix64_host((uint64_t)frame ^ mix64_host(2 * q))));
+    const uint64_t hy = mix64_host(c.cfg.seed ^ mix64_host(tag ^ mix64_host((uint64_t)frame ^ mix64_host(2 * q + 1))));
+    const int x = bound + (int)((hx >> 11) % (uint64_t)(c.g.nx - 2 * bound));
+    const int y = bound + (int)((hy >> 11) % (uint64_t)(c.g.ny - 2 * bound));
+    c.seeds_host[c.seed_ring] = (int32_t)cid(c.g, x, y, z);
+    c.seed_ring = (c.seed_ring + 1) % c.nseeds;
+  }
+  hipMemcpyAsync(c.seeds, c.seeds_host.data(), (size_t)c.nseeds * 4, hipMemcpyHostToDevice, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "update_seed_cells") ? MPF_OK : MPF_HIP_ERROR;
+}
+
+mpf_status mpf_set_pour_source(mpf_ctx* ctx, int64_t n, const float* pos, const float* vel, int8_t flag) {
+  if (!ctx || n < 0 || (n > 0 && (!pos || !vel))) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  if (flag != MPF_TYPEFLUID && flag != MPF_TYPEAIRSOLO && flag != MPF_TYPEAIR && flag != MPF_TYPESOLID)
+    return fail(c, MPF_INVALID_ARG, "pour source type must be a particle type");
+  if (n > c.cap) return fail(c, MPF_CAPACITY, "pour source larger than the particle capacity");
+  hipStreamSynchronize(c.stream);
+  dfree(c.pour_pos);
+  dfree(c.pour_vel);
+  c.npour = 0;
+  if (n == 0) return MPF_OK;
+  if (!dalloc(c, c.pour_pos, (size_t)n) || !dalloc(c, c.pour_vel, (size_t)n)) return MPF_CAPACITY;
+  hipMemcpyAsync(c.pour_pos, pos, (size_t)n * 12, hipMemcpyHostToDevice, c.stream);
+  hipMemcpyAsync(c.pour_vel, vel, (size_t)n * 12, hipMemcpyHostToDevice, c.stream);
+  c.npour = n;
+  c.pour_flag = flag;
+  return hip_ok(c, hipStreamSynchronize(c.stream), "set_pour_source") ? MPF_OK : MPF_HIP_ERROR;
+}
+
+mpf_status mpf_delete_particles(mpf_ctx* ctx, int64_t* deleted) {
+  if (!ctx) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  int64_t d = 0, fd = 0;
+  int st = run_delete_particles(c, &d, &fd);
+  if (deleted) *deleted = d;
+  if (st != MPF_OK) return (mpf_status)st;
+  return check_launch(c);
+}
+
+mpf_status mpf_pour(mpf_ctx* ctx, int32_t frame, int64_t* added) {
+  if (!ctx) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  int64_t a = 0;
+  int st = run_pour(c, frame, &a);
+  if (added) *added = a;
+  if (st != MPF_OK) return (mpf_status)st;
+  mpf_status ls = check_launch(c);
+  if (ls != MPF_OK) return ls;
+  return hip_ok(c, hipStreamSynchronize(c.stream), "pour") ? MPF_OK : MPF_HIP_ERROR;
 }
 
 mpf_status mpf_hash_sort(mpf_ctx* ctx) {
@@ -722,6 +814,8 @@ mpf_status mpf_step(mpf_ctx* ctx, int32_t frame, mpf_step_info* info) {
   mpf_step_info loc;
   std::memset(&loc, 0, sizeof(loc));
   mpf_status st;
+  if ((st = mpf_update_seed_cells(ctx, frame)) != MPF_OK) return st;
+  if ((st = mpf_pour(ctx, frame, &loc.poured)) != MPF_OK) return st;
   if ((st = mpf_hash_sort(ctx)) != MPF_OK) return st;
   if ((st = mpf_apply_forces(ctx)) != MPF_OK) return st;
   if ((st = mpf_levelset_update(ctx)) != MPF_OK) return st;
@@ -736,7 +830,16 @@ mpf_status mpf_step(mpf_ctx* ctx, int32_t frame, mpf_step_info* info) {
   if ((st = mpf_hash_sort(ctx)) != MPF_OK) return st;
   if ((st = mpf_levelset_update(ctx)) != MPF_OK) return st;
   if (c.cfg.heat_enabled && (st = mpf_heat(ctx, &loc.heat)) != MPF_OK) return st;
-  if ((st = mpf_phase_change(ctx, frame, &loc.phase)) != MPF_OK) return st;
+  {
+    // nucleation, deletion, melting and freezing in the reference's order
+    c.err.clear();
+    REQUIRE_SORTED(c);
+    int64_t deleted = 0;
+    int pst2 = run_phase_change_with_delete(c, frame, &loc.phase, &deleted);
+    if (pst2 != MPF_OK) return (mpf_status)pst2;
+    if ((st = check_launch(c)) != MPF_OK) return st;
+    loc.phase.deleted = (int32_t)deleted;
+  }
   if (c.cfg.mc_enabled && (st = mpf_marching_cubes(ctx, &loc.mc)) != MPF_OK) return st;
   loc.particles = c.P;
   if (info) *info = loc;

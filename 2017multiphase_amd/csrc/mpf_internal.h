@@ -210,9 +210,16 @@ struct Ctx {
   double* adj_cell[2] = {};
   double* adj_part[3] = {};  // vel out / vel in / pos, 3P each
   bool adj_ready = false;
-  // nucleation seeds
+  // nucleation seeds (ring position of the next refresh)
   int32_t* seeds = nullptr;
   int32_t nseeds = 0;
+  int32_t seed_ring = 0;
+  std::vector<int32_t> seeds_host;
+  // pour source (positions, velocities, particle type)
+  f3* pour_pos = nullptr;
+  f3* pour_vel = nullptr;
+  int64_t npour = 0;
+  int8_t pour_flag = MPF_TYPEFLUID;
 
   // timing
   bool timing = false;
@@ -254,6 +261,9 @@ void run_g2p(Ctx& c);
 void run_advect(Ctx& c);
 void run_heat_g2p(Ctx& c);
 int run_phase_change(Ctx& c, int frame, mpf_phase_info* info);
+int run_phase_change_with_delete(Ctx& c, int frame, mpf_phase_info* info, int64_t* deleted);
+int run_delete_particles(Ctx& c, int64_t* deleted, int64_t* fluid_deleted);
+int run_pour(Ctx& c, int frame, int64_t* added);
 // grid.hip
 void run_levelset(Ctx& c);
 void run_p2g(Ctx& c);
@@ -301,5 +311,6 @@ int run_advect_adjoint(Ctx& c, const double* gpos_out, const double* gvel_out, d
                        int where, int64_t* in_band);
 
 bool hip_ok(Ctx& c, hipError_t e, const char* what);
+extern thread_local std::string g_noctx_err;
 
 }  // namespace mpf

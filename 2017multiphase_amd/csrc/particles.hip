@@ -360,6 +360,130 @@ __global__ void k_freeze(Grid g, int64_t Pn, const int8_t* __restrict__ f0, cons
   if (m && __lane_id() == (unsigned)(__ffsll((long long)m) - 1)) atomicAdd(counter, (uint32_t)__popcll(m));
 }
 
+
+// ---- particle deletion and pouring (f1) ----
+// sweepVacuum, spray_k.cu:4117-4132, as a two-buffer pass: an AIR cell with a
+// VACUUM cell at one of its eight corner-diagonal neighbours becomes VACUUM.
+__global__ void k_sweep_vacuum(Grid g, const int8_t* __restrict__ in, int8_t* __restrict__ out) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= g.G) return;
+  int8_t m = in[idx];
+  if (m == AIR) {
+    int i, j, k;
+    ijk_of(g, idx, i, j, k);
+    for (int di = -1; di <= 1; di += 2)
+      for (int dj = -1; dj <= 1; dj += 2)
+        for (int dk = -1; dk <= 1; dk += 2)
+          if (valid(g, i + di, j + dj, k + dk) && in[cid(g, i + di, j + dj, k + dk)] == VACUUM) m = VACUUM;
+  }
+  out[idx] = m;
+}
+
+// markDeleteAirParticle, spray_k.cu:4134-4169: keep[p] = 1 to preserve.
+// SOLID stays; FLUID goes when the air level set at it is below -h; AIR and
+// AIRSOLO go when a cell of their 3x3x3 neighbourhood is VACUUM or their mass
+// is at most 1e-6.
+__global__ void k_delete_mark(Grid g, int64_t P, const f3* __restrict__ pos, const int8_t* __restrict__ flag,
+                              const float* __restrict__ mass, const int8_t* __restrict__ mark,
+                              const float* __restrict__ lsair, uint32_t* __restrict__ keep,
+                              uint32_t* __restrict__ fluid_deleted) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool fdel = false;
+  if (p < P) {
+    const int8_t fl = flag[p];
+    uint32_t kp = 1;
+    if (fl != SOLID) {
+      const f3 x = pos[p];
+      if (fl == FLUID) {
+        const float lsa = scalar_from_grid(g, x, lsair);
+        if (lsa < -1.0f * g.h) {
+          kp = 0;
+          fdel = true;
+        }
+      } else {
+        const float qx = (x.x - g.gx0) / g.h, qy = (x.y - g.gy0) / g.h, qz = (x.z - g.gz0) / g.h;
+        const int i = (qx >= 0 && qx < (float)g.nx) ? (int)qx : 0;
+        const int j = (qy >= 0 && qy < (float)g.ny) ? (int)qy : 0;
+        const int k = (qz >= 0 && qz < (float)g.nz) ? (int)qz : 0;
+        int cnt = 0;
+        for (int di = -1; di <= 1; ++di)
+          for (int dj = -1; dj <= 1; ++dj)
+            for (int dk = -1; dk <= 1; ++dk)
+              if (valid(g, i + di, j + dj, k + dk) && mark[cid(g, i + di, j + dj, k + dk)] == VACUUM) cnt++;
+        kp = (cnt == 0 && mass[p] > 0.000001f) ? 1u : 0u;
+      }
+    }
+    keep[p] = kp;
+  }
+  unsigned long long m = __ballot(fdel);
+  if (m && __lane_id() == (unsigned)(__ffsll((long long)m) - 1)) atomicAdd(fluid_deleted, (uint32_t)__popcll(m));
+}
+
+struct PartPtrs {
+  f3 *pos, *vel;
+  int8_t* flag;
+  float *mass, *temp, *lheat, *solub, *gas;
+};
+
+// deleteparticles, spray_k.cu:4172-4194: kept particles to their scan slot
+// in the other buffer (order preserved).  off: exclusive scan of keep.
+__global__ void k_compact(int64_t P, const uint32_t* __restrict__ off, uint32_t total, PartPtrs in, PartPtrs out) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const uint32_t o = off[p];
+  const uint32_t nxt = (p + 1 < P) ? off[p + 1] : total;
+  if (nxt == o) return;
+  out.pos[o] = in.pos[p];
+  out.vel[o] = in.vel[p];
+  out.flag[o] = in.flag[p];
+  out.mass[o] = in.mass[p];
+  out.temp[o] = in.temp[p];
+  out.lheat[o] = in.lheat[p];
+  out.solub[o] = in.solub[p];
+  out.gas[o] = in.gas[p];
+}
+
+// Cell lists after the compaction: a cell's range [s, e) maps to
+// [off[s], off[e]) (order is preserved), empty when nothing of it was kept.
+__global__ void k_cells_remap(int64_t G, int64_t P, uint32_t total, const uint32_t* __restrict__ off,
+                              uint32_t* __restrict__ cstart, uint32_t* __restrict__ cend) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G) return;
+  const uint32_t s = cstart[c];
+  if (s == MPF_CELL_UNDEF) return;
+  const uint32_t e = cend[c];
+  const uint32_t ns = off[s], ne = ((int64_t)e < P) ? off[e] : total;
+  cstart[c] = (ne > ns) ? ns : MPF_CELL_UNDEF;
+  cend[c] = (ne > ns) ? ne : MPF_CELL_UNDEF;
+}
+
+// pouringwater, spray_k.cu:5575-5596: n particles appended at the source
+// positions with a random offset (pos_rand * samplespace per component, in
+// [-1, 1)) and a random velocity perturbation; random numbers from the
+// counter hash of (seed, stream tag, source index, component), the same at
+// every pour (the reference passes frame 0, spray.cu:3020).
+constexpr uint64_t POUR_TAG = 0x706f7572ull << 32;
+__global__ void k_pour(int64_t P, int64_t n, const f3* __restrict__ spos, const f3* __restrict__ svel, int8_t fl,
+                       PartPtrs out, float pos_rand_ss, float vel_rand, float m0, float T, float th, uint64_t seed) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const f3 rv = mk3(uniform24(seed, POUR_TAG, (uint64_t)i, 0) * 2.0f - 1.0f,
+                    uniform24(seed, POUR_TAG, (uint64_t)i, 1) * 2.0f - 1.0f,
+                    uniform24(seed, POUR_TAG, (uint64_t)i, 2) * 2.0f - 1.0f);
+  const f3 rp = mk3(uniform24(seed, POUR_TAG, (uint64_t)i, 3) * 2.0f - 1.0f,
+                    uniform24(seed, POUR_TAG, (uint64_t)i, 4) * 2.0f - 1.0f,
+                    uniform24(seed, POUR_TAG, (uint64_t)i, 5) * 2.0f - 1.0f);
+  const int64_t q = P + i;
+  out.pos[q] = add3(spos[i], mul3(rp, pos_rand_ss));
+  out.vel[q] = add3(svel[i], mul3(rv, vel_rand));
+  out.mass[q] = m0;
+  out.flag[q] = fl;
+  out.temp[q] = T;
+  out.lheat[q] = th;
+  out.gas[q] = 0.0f;
+  out.solub[q] = 0.0f;
+}
+
 }  // namespace
 
 void run_apply_forces(Ctx& c) {
@@ -408,38 +532,42 @@ void run_heat_g2p(Ctx& c) {
          k.alpha_temp_trans, k.heat_melt_and_boil, k.melting_point, k.boiling_point, k.liquid_heat_th);
 }
 
-int run_phase_change(Ctx& c, int frame, mpf_phase_info* info) {
+// GenerateGasParticle, spray.cu:2653-2683: updatesolubility + nucleation with
+// slots from a scan over cells.  Returns the particles appended (-1 on error).
+static int64_t run_nucleation(Ctx& c, int frame) {
   const mpf_config& k = c.cfg;
   const int cur = c.cur;
-  int added = 0;
-  hipMemsetAsync(c.counters + 1, 0, 8, c.stream);
-  hipMemsetAsync(c.counters + 7, 0, 4, c.stream);
-  if (k.gas_enabled && k.gas_every > 0 && frame % k.gas_every == 0) {
-    launch(c, "solubility", dim3(nblocks(c.P, 256)), dim3(256), 0, k_solubility, c.P, (const int8_t*)c.flag[cur],
-           (const float*)c.temp[cur], c.solub[cur], k.dissolve_gas_rate * k.airm0, k.temperature0);
-    GasArgs a;
-    a.pos = c.pos[cur]; a.pos_w = c.pos[cur]; a.vel_w = c.vel[cur]; a.flag = c.flag[cur]; a.mass = c.mass[cur];
-    a.temp = c.temp[cur]; a.lheat = c.lheat[cur]; a.solub = c.solub[cur]; a.gas = c.gas[cur];
-    a.cstart = c.cstart; a.cend = c.cend; a.seeds = c.seeds; a.nseeds = c.nseeds; a.tgrid = c.tp;
-    a.m0 = k.airm0; a.th = k.liquid_heat_th; a.vr = k.vapor_gen_rate; a.seed = k.seed; a.frame = frame;
-    launch(c, "gas_count", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_gas_count, c.g, a, c.ctmp_u32);
-    int64_t na = run_scan_u32(c, c.ctmp_u32, c.count, c.g.G, true);
-    if (na < 0) return MPF_HIP_ERROR;
-    if (c.P + na > c.cap) {
-      c.err = "particle capacity exceeded by nucleation";
-      return MPF_CAPACITY;
-    }
-    launch(c, "gas_emit", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_gas_emit, c.g, a, (const uint32_t*)c.ctmp_u32,
-           (const uint32_t*)c.count, c.P);
-    added = (int)na;
+  if (!(k.gas_enabled && k.gas_every > 0 && frame % k.gas_every == 0)) return 0;
+  launch(c, "solubility", dim3(nblocks(c.P, 256)), dim3(256), 0, k_solubility, c.P, (const int8_t*)c.flag[cur],
+         (const float*)c.temp[cur], c.solub[cur], k.dissolve_gas_rate * k.airm0, k.temperature0);
+  GasArgs a;
+  a.pos = c.pos[cur]; a.pos_w = c.pos[cur]; a.vel_w = c.vel[cur]; a.flag = c.flag[cur]; a.mass = c.mass[cur];
+  a.temp = c.temp[cur]; a.lheat = c.lheat[cur]; a.solub = c.solub[cur]; a.gas = c.gas[cur];
+  a.cstart = c.cstart; a.cend = c.cend; a.seeds = c.seeds; a.nseeds = c.nseeds; a.tgrid = c.tp;
+  a.m0 = k.airm0; a.th = k.liquid_heat_th; a.vr = k.vapor_gen_rate; a.seed = k.seed; a.frame = frame;
+  launch(c, "gas_count", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_gas_count, c.g, a, c.ctmp_u32);
+  const int64_t na = run_scan_u32(c, c.ctmp_u32, c.count, c.g.G, true);
+  if (na < 0) return -1;
+  if (c.P + na > c.cap) {
+    c.err = "particle capacity exceeded by nucleation";
+    return -2;
   }
-  const int64_t Pn = c.P;  // particles covered by the cell lists
-  c.P += added;
-  if (added) c.sorted = c.own_sort = false;
+  launch(c, "gas_emit", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_gas_emit, c.g, a, (const uint32_t*)c.ctmp_u32,
+         (const uint32_t*)c.count, c.P);
+  c.P += na;
+  if (na) c.sorted = c.own_sort = false;
+  return na;
+}
+
+// MeltSolid / Freezing, spray.cu:2816-2875, on the particles [0, Pn) the cell
+// lists cover (counters 1 and 2: melted, frozen; 7: frozen and moved).
+static void run_melt_freeze(Ctx& c, int64_t Pn) {
+  const mpf_config& k = c.cfg;
+  const int cur = c.cur;
   if (k.melt_enabled)
     launch(c, "melt", dim3(nblocks(c.P, 256)), dim3(256), 0, k_melt, c.P, c.flag[cur], c.lheat[cur], k.liquid_heat_th,
            c.counters + 1);
-  if (k.freeze_enabled) {
+  if (k.freeze_enabled && Pn > 0) {
     const int alt = 1 - cur;
     hipMemcpyAsync(c.flag[alt], c.flag[cur], (size_t)Pn, hipMemcpyDeviceToDevice, c.stream);
     hipMemcpyAsync(c.pos[alt], c.pos[cur], (size_t)Pn * sizeof(f3), hipMemcpyDeviceToDevice, c.stream);
@@ -447,15 +575,133 @@ int run_phase_change(Ctx& c, int frame, mpf_phase_info* info) {
            (const f3*)c.pos[alt], c.flag[cur], c.pos[cur], c.lheat[cur], (const uint32_t*)c.cstart,
            (const uint32_t*)c.cend, k.samplespace, c.counters + 2, c.counters + 7);
   }
+}
+
+static PartPtrs part_ptrs(Ctx& c, int b) {
+  return PartPtrs{c.pos[b], c.vel[b], c.flag[b], c.mass[b], c.temp[b], c.lheat[b], c.solub[b], c.gas[b]};
+}
+
+// deleteAirFluidParticle, spray.cu:2194-2248, on the cell flags and air level
+// set of the last level-set update.  *pPn: particles the cell lists cover
+// (their ranges are remapped through the compaction and *pPn updated), or -1
+// when the lists are not kept.  Counter 9: FLUID particles deleted.
+static int run_delete(Ctx& c, int64_t* pPn, int64_t* deleted, int64_t* fluid_deleted) {
+  const mpf_config& k = c.cfg;
+  const int64_t P = c.P;
+  const int64_t Pn = *pPn;
+  *deleted = 0;
+  *fluid_deleted = 0;
+  // sweepVacuum passes ping-pong between the flags and the solver's per-cell
+  // scratch; an odd count ends with a copy back
+  int8_t* m0 = c.mark;
+  int8_t* m1 = (int8_t*)c.nbm;
+  for (int t = 0; t < k.delete_vacuum_sweeps; ++t) {
+    launch(c, "sweep_vacuum", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_sweep_vacuum, c.g, (const int8_t*)m0, m1);
+    std::swap(m0, m1);
+  }
+  if (m0 != c.mark) hipMemcpyAsync(c.mark, m0, (size_t)c.g.G, hipMemcpyDeviceToDevice, c.stream);
+  if (P == 0) return MPF_OK;
+  const int cur = c.cur, nxt = 1 - c.cur;
+  hipMemsetAsync(c.counters + 9, 0, 4, c.stream);
+  launch(c, "delete_mark", dim3(nblocks(P, 256)), dim3(256), 0, k_delete_mark, c.g, P, (const f3*)c.pos[cur],
+         (const int8_t*)c.flag[cur], (const float*)c.mass[cur], (const int8_t*)c.mark, (const float*)c.lsair, c.key,
+         c.counters + 9);
+  // in-place exclusive scan: keep[p] = off[p+1] - off[p]
+  const int64_t total = run_scan_u32(c, c.key, c.key, P, true);
+  if (total < 0) return MPF_HIP_ERROR;
+  hipMemcpyAsync(c.counters_host + 9, c.counters + 9, 4, hipMemcpyDeviceToHost, c.stream);
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "delete")) return MPF_HIP_ERROR;
+  *fluid_deleted = c.counters_host[9];
+  if (total == P) return MPF_OK;
+  *deleted = P - total;
+  launch(c, "compact", dim3(nblocks(P, 256)), dim3(256), 0, k_compact, P, (const uint32_t*)c.key, (uint32_t)total,
+         part_ptrs(c, cur), part_ptrs(c, nxt));
+  if (Pn >= 0) {
+    launch(c, "cells_remap", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_cells_remap, c.g.G, P, (uint32_t)total,
+           (const uint32_t*)c.key, c.cstart, c.cend);
+    if (Pn < P) {
+      hipMemcpyAsync(&c.counters_host[10], c.key + Pn, 4, hipMemcpyDeviceToHost, c.stream);
+      if (!hip_ok(c, hipStreamSynchronize(c.stream), "delete")) return MPF_HIP_ERROR;
+      *pPn = c.counters_host[10];
+    } else {
+      *pPn = total;
+    }
+  }
+  c.cur = nxt;
+  c.P = total;
+  c.own_sort = false;
+  if (Pn >= 0) run_cell_mask(c);
+  return MPF_OK;
+}
+
+int run_phase_change(Ctx& c, int frame, mpf_phase_info* info) {
+  const int64_t Pn = c.P;  // particles covered by the cell lists
+  hipMemsetAsync(c.counters + 1, 0, 8, c.stream);
+  hipMemsetAsync(c.counters + 7, 0, 4, c.stream);
+  const int64_t added = run_nucleation(c, frame);
+  if (added == -1) return MPF_HIP_ERROR;
+  if (added == -2) return MPF_CAPACITY;
+  run_melt_freeze(c, Pn);
   hipMemcpyAsync(c.counters_host + 1, c.counters + 1, 7 * 4, hipMemcpyDeviceToHost, c.stream);
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "phase_change")) return MPF_HIP_ERROR;
-  if (k.freeze_enabled && c.counters_host[7]) c.own_sort = false;  // a frozen particle snapped into another cell
+  if (c.cfg.freeze_enabled && c.counters_host[7]) c.own_sort = false;  // a frozen particle snapped into another cell
   if (info) {
     info->melted = (int32_t)c.counters_host[1];
     info->frozen = (int32_t)c.counters_host[2];
-    info->gas_added = added;
+    info->gas_added = (int32_t)added;
+    info->deleted = 0;
     info->particles = c.P;
   }
+  return MPF_OK;
+}
+
+// The step's order (spray.cpp:1155-1180): nucleation, deletion, melting and
+// freezing.  Deletion keeps the cell lists valid for the particles they cover.
+int run_phase_change_with_delete(Ctx& c, int frame, mpf_phase_info* info, int64_t* deleted) {
+  int64_t Pn = c.P;
+  hipMemsetAsync(c.counters + 1, 0, 8, c.stream);
+  hipMemsetAsync(c.counters + 7, 0, 4, c.stream);
+  const int64_t added = run_nucleation(c, frame);
+  if (added == -1) return MPF_HIP_ERROR;
+  if (added == -2) return MPF_CAPACITY;
+  int64_t fl_del = 0;
+  if (c.cfg.delete_enabled) {
+    const int st = run_delete(c, &Pn, deleted, &fl_del);
+    if (st != MPF_OK) return st;
+  } else {
+    *deleted = 0;
+  }
+  run_melt_freeze(c, Pn);
+  hipMemcpyAsync(c.counters_host + 1, c.counters + 1, 7 * 4, hipMemcpyDeviceToHost, c.stream);
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "phase_change")) return MPF_HIP_ERROR;
+  if (info) {
+    info->melted = (int32_t)c.counters_host[1];
+    info->frozen = (int32_t)c.counters_host[2];
+    info->gas_added = (int32_t)added;
+    info->deleted = (int32_t)*deleted;
+    info->particles = c.P;
+  }
+  return MPF_OK;
+}
+
+// mpf_delete_particles: the cell lists are remapped when they index every particle.
+int run_delete_particles(Ctx& c, int64_t* deleted, int64_t* fluid_deleted) {
+  int64_t Pn = c.sorted ? c.P : -1;
+  return run_delete(c, &Pn, deleted, fluid_deleted);
+}
+
+// pouring / pouringgas, spray.cu:3012-3043.
+int run_pour(Ctx& c, int frame, int64_t* added) {
+  const mpf_config& k = c.cfg;
+  *added = 0;
+  if (c.npour <= 0 || k.pour_every <= 0 || frame % k.pour_every != 0) return MPF_OK;
+  if (c.P + c.npour > c.cap) return MPF_OK;  // the reference skips the pour when it would not fit
+  launch(c, "pour", dim3(nblocks(c.npour, 256)), dim3(256), 0, k_pour, c.P, c.npour, (const f3*)c.pour_pos,
+         (const f3*)c.pour_vel, c.pour_flag, part_ptrs(c, c.cur), k.pour_pos_rand * k.samplespace, k.pour_vel_rand,
+         k.m0, k.pour_temperature, k.liquid_heat_th, k.seed);
+  c.P += c.npour;
+  *added = c.npour;
+  c.sorted = c.own_sort = false;
   return MPF_OK;
 }
 

@@ -17,6 +17,8 @@
 //  4. per-cell segment sort of the permutation (runs from different waves can
 //     interleave within a cell): a register sorting network for <=16 entries;
 //  5. gather of the 45-byte particle record into the other buffer.
+#include <mutex>
+
 #include "mpf_internal.h"
 
 namespace mpf {
@@ -87,6 +89,53 @@ __global__ void k_scatter(int64_t P, const uint32_t* __restrict__ key, uint32_t*
     perm[slot] = (uint32_t)p;
     skey[slot] = k;
   }
+}
+
+// Keyless variants for mpf_hash_sort_noalloc: the count goes straight into
+// the caller's cell_end, and the scatter recomputes each key from the
+// position (no per-particle key array).
+__global__ void k_hash_count_nokey(Grid g, const f3* __restrict__ pos, int64_t P, uint32_t* __restrict__ count) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool act = p < P;
+  uint32_t k = act ? hash_key(g, pos[p]) : 0xffffffffu;
+  bool head;
+  int runlen, headlane;
+  wave_runs(act, k, head, runlen, headlane);
+  if (head) atomicAdd(&count[k], (uint32_t)runlen);
+}
+
+__global__ void k_scatter_rehash(Grid g, const f3* __restrict__ pos, int64_t P, uint32_t* __restrict__ cursor,
+                                 uint32_t* __restrict__ perm, uint32_t* __restrict__ skey) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool act = p < P;
+  uint32_t k = act ? hash_key(g, pos[p]) : 0xffffffffu;
+  bool head;
+  int runlen, headlane;
+  wave_runs(act, k, head, runlen, headlane);
+  uint32_t base = 0;
+  if (head) base = atomicAdd(&cursor[k], (uint32_t)runlen);
+  base = __shfl(base, headlane);
+  if (act) {
+    uint32_t slot = base + (uint32_t)(lane_id() - headlane);
+    perm[slot] = (uint32_t)p;
+    skey[slot] = k;
+  }
+}
+
+__global__ void k_fill_u32(int64_t n, uint32_t* __restrict__ v, uint32_t x) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) v[i] = x;
+}
+
+// counts (in cend) + exclusive scan (in cstart) -> cstart = start or UNDEF,
+// cend = start as the scatter cursor (UNDEF when empty); the scatter then
+// advances each non-empty cell's cursor to its end.
+__global__ void k_cells_cursor(int64_t G, uint32_t* __restrict__ cstart, uint32_t* __restrict__ cend) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= G) return;
+  const uint32_t n = cend[c], s = cstart[c];
+  cstart[c] = n ? s : MPF_CELL_UNDEF;
+  cend[c] = n ? s : MPF_CELL_UNDEF;
 }
 
 template <int N>
@@ -389,4 +438,94 @@ void run_hash_sort(Ctx& c) {
   run_cell_mask(c);
 }
 
+// ---- mpf_hash_sort_noalloc: the sort without a context -------------------
+// Device scratch of the scan's block sums: a file-scope array, one copy per
+// device, held until the process ends.  Host code never passes the array's
+// name to a kernel (on the host it is only a stand-in); it looks up the
+// device address once per device and passes that.
+constexpr int64_t NOALLOC_SCAN_BLOCKS = 65536;
+constexpr int64_t NOALLOC_MAX_CELLS = (NOALLOC_SCAN_BLOCKS - 1) * SCAN_TILE;
+static __device__ __align__(256) uint32_t scratch[NOALLOC_SCAN_BLOCKS];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// Device address of `scratch` on the current device (looked up once per device).
+uint32_t* noalloc_scratch(int dev, std::string& err) {
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) {
+    err = "device index out of range for the no-allocation scratch";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    g_scratch_ptr[dev] = (uint32_t*)ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace
+
 }  // namespace mpf
+
+using namespace mpf;
+
+extern "C" mpf_status mpf_hash_sort_noalloc(const mpf_config* cfg, int64_t n, const float* pos, uint32_t* grid_hash,
+                                            uint32_t* grid_index, uint32_t* cell_start, uint32_t* cell_end,
+                                            void* stream) {
+  auto fail = [](mpf_status st, const std::string& m) {
+    g_noctx_err = m;
+    return st;
+  };
+  if (!cfg || !cell_start || !cell_end) return fail(MPF_INVALID_ARG, "null argument");
+  if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1 || !(cfg->dx > 0))
+    return fail(MPF_INVALID_ARG, "invalid grid");
+  const int64_t G = (int64_t)cfg->nx * cfg->ny * cfg->nz;
+  if (G > NOALLOC_MAX_CELLS)
+    return fail(MPF_INVALID_ARG, "grid of " + std::to_string(G) + " cells exceeds the no-allocation limit of " +
+                                     std::to_string(NOALLOC_MAX_CELLS));
+  if (n < 0 || n > 0xfffffff0ll) return fail(MPF_INVALID_ARG, "particle count out of range");
+  if (n > 0 && (!pos || !grid_hash || !grid_index)) return fail(MPF_INVALID_ARG, "null particle array");
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return fail(MPF_HIP_ERROR, "hipGetDevice failed");
+  std::string err;
+  uint32_t* blk = noalloc_scratch(dev, err);
+  if (!blk) return fail(MPF_HIP_ERROR, err);
+  Grid g{};
+  g.nx = cfg->nx; g.ny = cfg->ny; g.nz = cfg->nz;
+  g.G = G;
+  g.h = cfg->dx;
+  g.gx0 = cfg->gmin[0]; g.gy0 = cfg->gmin[1]; g.gz0 = cfg->gmin[2];
+  g.dnz = make_fastdiv((uint32_t)g.nz);
+  g.dny = make_fastdiv((uint32_t)g.ny);
+  hipStream_t st = (hipStream_t)stream;
+  const f3* p3 = (const f3*)pos;
+  if (n == 0) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblocks(G, 256)), dim3(256), 0, st, G, cell_start, MPF_CELL_UNDEF);
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblocks(G, 256)), dim3(256), 0, st, G, cell_end, MPF_CELL_UNDEF);
+  } else {
+    const int64_t nb = (G + SCAN_TILE - 1) / SCAN_TILE;
+    // counts in cell_end (cleared by a kernel, not a memset: graph-capture safe)
+    hipLaunchKernelGGL(k_fill_u32, dim3(nblocks(G, 256)), dim3(256), 0, st, G, cell_end, 0u);
+    hipLaunchKernelGGL(k_hash_count_nokey, dim3(nblocks(n, 256)), dim3(256), 0, st, g, p3, n, cell_end);
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, (const uint32_t*)cell_end, G,
+                       blk);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(SCAN_THREADS), 0, st, blk, nb);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, (const uint32_t*)cell_end, G,
+                       (const uint32_t*)blk, cell_start);
+    hipLaunchKernelGGL(k_cells_cursor, dim3(nblocks(G, 256)), dim3(256), 0, st, G, cell_start, cell_end);
+    hipLaunchKernelGGL(k_scatter_rehash, dim3(nblocks(n, 256)), dim3(256), 0, st, g, p3, n, cell_end, grid_index,
+                       grid_hash);
+    hipLaunchKernelGGL(k_segsort, dim3(nblocks(G, 256)), dim3(256), 0, st, G, (const uint32_t*)cell_start,
+                       (const uint32_t*)cell_end, grid_index);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(MPF_HIP_ERROR, std::string("kernel launch: ") + hipGetErrorString(e));
+  return MPF_OK;
+}

@@ -99,9 +99,6 @@ def run_form(pkg, scenes, name, n, precision, steps, warmup, timing_step=True):
         sim.step(frame, allow_no_convergence=True)
         frame += 1
     sim.synchronize()
-    import torch
-    if torch.cuda.is_available():
-        torch.cuda.synchronize()
     iters = 0
     t0 = time.perf_counter()
     infos = []
@@ -111,8 +108,6 @@ def run_form(pkg, scenes, name, n, precision, steps, warmup, timing_step=True):
         iters += info["project"]["iterations"]
         frame += 1
     sim.synchronize()
-    if torch.cuda.is_available():
-        torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     log("[bench]   %.3f s for %d steps, PCG iterations %s" % (dt, steps, [i["project"]["iterations"] for i in infos]))
     ktimes = None
@@ -198,13 +193,14 @@ def main():
     pkg = importlib.import_module("2017multiphase_amd")
     scenes = importlib.import_module("2017multiphase_amd.scenes")
     if args.copy_calibration_mib > 0:
-        import torch
+        hm = importlib.import_module("2017multiphase_amd.hipmem")
         n = args.copy_calibration_mib << 20
-        src = torch.ones(n, dtype=torch.uint8, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        src = hm.DeviceArray(n, "u1")
+        dst = hm.DeviceArray(n, "u1")
+        src.fill_bytes(1)
+        hm.synchronize()
+        dst.copy_from(src, n)
+        hm.synchronize()
         log("[bench] copy calibration: %d bytes read, %d bytes written" % (n, n))
         del src, dst
     mixed = run_form(pkg, scenes, args.scene, args.n, pkg.MIXED, args.steps, args.warmup)

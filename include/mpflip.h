@@ -153,7 +153,16 @@ typedef struct mpf_config {
   /* ---- appended in ABI version 2 ---- */
   int32_t adjoint_enabled;   /* allocate the adjoint entry points' scratch at
                                 mpf_create (the adjoints refuse otherwise)    */
-  int32_t pad2_;
+  /* particle deletion (deleteAirFluidParticle, spray.cu:2194-2248) */
+  int32_t delete_enabled;
+  int32_t delete_vacuum_sweeps; /* sweepVacuum passes (20)                    */
+  /* pouring (pouring / pouringgas, spray.cu:3012-3043) */
+  int32_t pour_every;        /* frames between pours (4; 6 for gas)           */
+  float pour_pos_rand;       /* posrandparam (0.1, spray.cpp:55), x samplespace */
+  float pour_vel_rand;       /* velrandparam (0.0, spray.cpp:55)              */
+  float pour_temperature;    /* defaultLiquidT (293.15, spray.cpp:54)         */
+  /* nucleation seed refresh (updateSeedCell, spray.cu:2933-2950) */
+  int32_t seed_update_every; /* frames between refreshes, 0 = never (15)      */
 } mpf_config;
 
 /* Opaque context. */
@@ -191,7 +200,8 @@ typedef struct mpf_solve_info {
 } mpf_solve_info;
 
 typedef struct mpf_phase_info {
-  int32_t melted, frozen, gas_added, pad_;
+  int32_t melted, frozen, gas_added;
+  int32_t deleted;         /* mpf_step with delete_enabled: particles deleted */
   int64_t particles;
 } mpf_phase_info;
 
@@ -205,6 +215,7 @@ typedef struct mpf_step_info {
   mpf_phase_info phase;
   mpf_mc_info mc;
   int64_t particles;
+  int64_t poured;          /* particles appended by the pour source           */
 } mpf_step_info;
 
 /* ---- lifecycle --------------------------------------------------------- */
@@ -215,7 +226,8 @@ mpf_status mpf_config_default(mpf_config* cfg, int32_t nx, int32_t ny, int32_t n
 /* Replaces cspray::init()/initmem_bubble() (spray.cpp:628-670, spray.cu:25-218). */
 mpf_status mpf_create(const mpf_config* cfg, mpf_ctx** out);
 mpf_status mpf_destroy(mpf_ctx* ctx);
-/* Message of the last failing call on ctx (or of mpf_create when ctx is NULL). */
+/* Message of the last failing call on ctx (or, when ctx is NULL, of the last
+   failing context-free call: mpf_create, mpf_hash_sort_noalloc). */
 const char* mpf_last_error(mpf_ctx* ctx);
 int32_t mpf_abi_version(void);
 /* The context's HIP stream (hipStream_t), for callers that time or order work. */
@@ -240,6 +252,16 @@ mpf_status mpf_get_field(mpf_ctx* ctx, int32_t field, void* dst, int32_t where);
 mpf_status mpf_set_field(mpf_ctx* ctx, int32_t field, const void* src, int32_t where);
 /* Nucleation seed cells (cell indices), spray.cu:2907-2951. */
 mpf_status mpf_set_seed_cells(mpf_ctx* ctx, const int32_t* cells, int32_t n);
+/* cspray::updateSeedCell, spray.cu:2933-2950: when frame is a multiple of
+   cfg.seed_update_every, four seed cells (a ring over the list) are replaced
+   by random cells at k = 2, 3 <= i < nx-3, 3 <= j < ny-3 (counter-hash random
+   numbers of (seed, frame, slot)).  mpf_step calls it first. */
+mpf_status mpf_update_seed_cells(mpf_ctx* ctx, int32_t frame);
+/* Pour source of pouring/pouringgas (spray.cu:3012-3043, positions built by
+   CompPouringParam_*, spray.cpp:1415-1545): n positions and velocities
+   (float[3n], host memory) and the type of the poured particles (FLUID, or
+   AIRSOLO for pouringgas).  n = 0 removes the source. */
+mpf_status mpf_set_pour_source(mpf_ctx* ctx, int64_t n, const float* pos, const float* vel, int8_t flag);
 
 /* ---- step entry points (one per reference host step method) ------------ */
 /* cspray::hashAndSortParticles, spray.cu:614-637: calcHashD + stable sort by
@@ -268,6 +290,20 @@ mpf_status mpf_advect(mpf_ctx* ctx);
 mpf_status mpf_heat(mpf_ctx* ctx, mpf_solve_info* info);
 /* GenerateGasParticle, MeltSolid, Freezing: spray.cu:2653-2683, 2816-2875. */
 mpf_status mpf_phase_change(mpf_ctx* ctx, int32_t frame, mpf_phase_info* info);
+/* cspray::deleteAirFluidParticle, spray.cu:2194-2248, on the cell flags and
+   air level set of the last mpf_levelset_update: cfg.delete_vacuum_sweeps
+   passes turn AIR cells with a VACUUM corner-diagonal neighbour into VACUUM
+   (two-buffer passes), then FLUID particles with lsair < -h and AIR/AIRSOLO
+   particles with a VACUUM cell in their 3x3x3 neighbourhood (or mass <= 1e-6)
+   are removed, order preserved (stable compaction).  Cell lists that indexed
+   every particle are remapped and stay valid.  *deleted (may be NULL): the
+   number removed. */
+mpf_status mpf_delete_particles(mpf_ctx* ctx, int64_t* deleted);
+/* cspray::pouring / pouringgas: when frame is a multiple of cfg.pour_every
+   and the particles fit, append the pour source's particles (random offsets
+   cfg.pour_pos_rand * samplespace and cfg.pour_vel_rand per component).
+   *added (may be NULL): particles appended. */
+mpf_status mpf_pour(mpf_ctx* ctx, int32_t frame, int64_t* added);
 /* runMC_smooth stages 2-7 (spray.cu:804-979) on the liquid level set. */
 mpf_status mpf_marching_cubes(mpf_ctx* ctx, mpf_mc_info* info);
 /* Copy the last mesh out: verts float[3*nv], tri indices uint32[3*nt]. */
@@ -275,7 +311,10 @@ mpf_status mpf_get_mesh(mpf_ctx* ctx, float* verts, uint32_t* tris, int32_t wher
 /* Per-voxel case index (uint8, (nx-1)*(ny-1)*(nz-1) voxels over cell centres). */
 mpf_status mpf_get_mc_cases(mpf_ctx* ctx, uint8_t* cases, int32_t where);
 
-/* One full step in bubblesim order (spray.cpp:979-1233), the stages above only. */
+/* One full step in bubblesim order (spray.cpp:979-1233), the stages above
+   only: seed refresh, pour, sort, forces, level set, P2G, extrapolation,
+   projection, extrapolation, G2P, advection, sort, level set, heat,
+   nucleation, deletion (cfg.delete_enabled), melting/freezing, marching cubes. */
 mpf_status mpf_step(mpf_ctx* ctx, int32_t frame, mpf_step_info* info);
 
 /* ---- adjoints ----------------------------------------------------------
@@ -309,6 +348,25 @@ mpf_status mpf_g2p_adjoint(mpf_ctx* ctx, const double* g_vel_out, double* g_vel_
    *in_band (may be NULL) returns how many particles were inside its band. */
 mpf_status mpf_advect_adjoint(mpf_ctx* ctx, const double* g_pos_out, const double* g_vel_out, double* g_pos_in,
                               double* g_vel_in, int32_t where, int64_t* in_band);
+
+/* ---- context-free sort without allocation -------------------------------
+   No counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc): the cell hash and stable sort of mpf_hash_sort on the
+   grid of `cfg` (nx, ny, nz, dx, gmin; nothing else is read), with the
+   caller's device buffers.  In: pos float[3n].  Out: grid_hash, grid_index
+   uint32[n] (sorted keys, permutation) and cell_start, cell_end uint32[G]
+   (MPF_CELL_UNDEF for empty cells), bit-identical to mpf_hash_sort's
+   MPF_F_GRID_HASH, MPF_F_GRID_INDEX, MPF_F_CELL_START, MPF_F_CELL_END; the
+   particle arrays are not reordered.  Launches on `stream` (a hipStream_t,
+   NULL = the null stream) without synchronising; calls no hipMalloc/hipFree.
+   Scratch: a fixed device array of 65536 uint32 (256 KiB) per GPU, held
+   until the process ends, so G <= 65535 * 4096 cells; larger grids return
+   MPF_INVALID_ARG.  Every call on one GPU shares that array: calls that
+   overlap on one GPU (different streams or threads) race.  Errors: message
+   through mpf_last_error(NULL). */
+mpf_status mpf_hash_sort_noalloc(const mpf_config* cfg, int64_t n, const float* pos, uint32_t* grid_hash,
+                                 uint32_t* grid_index, uint32_t* cell_start, uint32_t* cell_end, void* stream);
 
 /* ---- introspection ------------------------------------------------------ */
 /* Kernel timing with HIP events around every launch on the context stream. */

@@ -44,6 +44,13 @@ def lib():
     L.orc_get_field.argtypes = [vp, c.c_int, vp]
     L.orc_set_field.argtypes = [vp, c.c_int, vp]
     L.orc_set_seed_cells.argtypes = [vp, vp, i32]
+    L.orc_get_seed_cells.argtypes = [vp, vp]
+    L.orc_update_seed_cells.argtypes = [vp, c.c_int]
+    L.orc_set_pour_source.argtypes = [vp, i64, vp, vp, c.c_int8]
+    L.orc_pour.argtypes = [vp, c.c_int]
+    L.orc_pour.restype = i64
+    L.orc_delete_particles.argtypes = [vp, c.c_int]
+    L.orc_delete_particles.restype = i64
     for n in ("orc_hash_sort", "orc_apply_forces", "orc_levelset_update", "orc_p2g", "orc_g2p", "orc_advect"):
         getattr(L, n).argtypes = [vp]
     L.orc_extrapolate_bc.argtypes = [vp, c.c_int]
@@ -161,6 +168,25 @@ class Oracle:
         a = np.ascontiguousarray(cells, np.int32)
         self._ok(self.L.orc_set_seed_cells(self.h, _p(a), a.size))
 
+    def get_seed_cells(self, cap=4096):
+        a = np.empty(cap, np.int32)
+        n = self.L.orc_get_seed_cells(self.h, _p(a))
+        return a[:n].copy()
+
+    def update_seed_cells(self, frame):
+        self._ok(self.L.orc_update_seed_cells(self.h, frame))
+
+    def set_pour_source(self, pos, vel, flag=0):
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1)
+        vel = np.ascontiguousarray(vel, np.float32).reshape(-1)
+        self._ok(self.L.orc_set_pour_source(self.h, pos.size // 3, _p(pos), _p(vel), flag))
+
+    def pour(self, frame):
+        return int(self.L.orc_pour(self.h, frame))
+
+    def delete_particles(self, keep_lists=True):
+        return int(self.L.orc_delete_particles(self.h, 1 if keep_lists else 0))
+
     def hash_sort(self):
         self._ok(self.L.orc_hash_sort(self.h))
 
@@ -237,7 +263,7 @@ class Oracle:
         info = self.pkg.StepInfo()
         self._ok(self.L.orc_step(self.h, frame, ctypes.byref(info)), (0, 3) if allow_no_convergence else (0,))
         return {"project": info.project.as_dict(), "heat": info.heat.as_dict(), "phase": info.phase.as_dict(),
-                "mc": info.mc.as_dict(), "particles": info.particles}
+                "mc": info.mc.as_dict(), "particles": info.particles, "poured": info.poured}
 
     def apply_operator(self, x):
         x = np.ascontiguousarray(x, np.float64)

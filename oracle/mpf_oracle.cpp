@@ -52,10 +52,14 @@ inline uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
   return z ^ (z >> 31);
 }
-inline float uniform24(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
-  uint64_t h = mix64(seed ^ mix64(a ^ mix64(b ^ mix64(c))));
-  return (float)(h >> 40) * (1.0f / 16777216.0f);
+inline uint64_t hash64(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
+  return mix64(seed ^ mix64(a ^ mix64(b ^ mix64(c))));
 }
+inline float uniform24(uint64_t seed, uint64_t a, uint64_t b, uint64_t c) {
+  return (float)(hash64(seed, a, b, c) >> 40) * (1.0f / 16777216.0f);
+}
+constexpr uint64_t POUR_TAG = 0x706f7572ull << 32;
+constexpr uint64_t SEED_TAG = 0x73656564ull << 32;
 
 template <class T> struct V3 { T x, y, z; };
 template <class T> inline V3<T> mk(T x, T y, T z) { return V3<T>{x, y, z}; }
@@ -1097,8 +1101,10 @@ template <class T> struct Orc : OrcBase {
     const int level = 2;
     return x <= level || x >= NX_ - 1 - level || y <= level || y >= NY_ - 1 - level;
   }
-  int phase_change(int frame, mpf_phase_info* info) {
-    int added = 0, melted = 0, frozen = 0;
+  // GenerateGasParticle (spray.cu:2653-2683): returns the particles appended
+  // (-1 when they would not fit).
+  int64_t nucleation(int frame) {
+    int64_t added = 0;
     if (c.gas_enabled && c.gas_every > 0 && frame % c.gas_every == 0) {
       const T T0 = (T)c.temperature0;
       for (int64_t p = 0; p < P; ++p)
@@ -1143,7 +1149,7 @@ template <class T> struct Orc : OrcBase {
       }
       int64_t na = 0;
       for (int64_t idx = 0; idx < G; ++idx) na += add[idx];
-      if (P + na > c.particle_capacity) return MPF_CAPACITY;
+      if (P + na > c.particle_capacity) return -1;
       int64_t slot = P;
       pos.resize(3 * (P + na)); vel.resize(3 * (P + na)); flag.resize(P + na); mass.resize(P + na);
       temp.resize(P + na); lheat.resize(P + na); solub.resize(P + na); gas.resize(P + na);
@@ -1157,9 +1163,14 @@ template <class T> struct Orc : OrcBase {
         ++slot;
       }
       P += na;
-      added = (int)na;
+      added = na;
     }
-    const int64_t Pn = P - added;  // particles covered by the cell lists
+    return added;
+  }
+  // MeltingSolidByHeat / FreezingSolidByHeat on the particles [0, Pn) the cell
+  // lists cover; returns melted and frozen counts.
+  void melt_freeze(int64_t Pn, int& melted, int& frozen) {
+    melted = frozen = 0;
     if (c.melt_enabled) {
       const T th = (T)c.liquid_heat_th;
       for (int64_t p = 0; p < P; ++p)
@@ -1208,8 +1219,134 @@ template <class T> struct Orc : OrcBase {
       }
       frozen = cnt;
     }
-    if (info) { info->melted = melted; info->frozen = frozen; info->gas_added = added; info->particles = P; }
+  }
+  int phase_change(int frame, mpf_phase_info* info) {
+    const int64_t Pn = P;
+    const int64_t added = nucleation(frame);
+    if (added < 0) return MPF_CAPACITY;
+    int melted = 0, frozen = 0;
+    melt_freeze(Pn, melted, frozen);
+    if (info) {
+      info->melted = melted; info->frozen = frozen; info->gas_added = (int)added; info->deleted = 0;
+      info->particles = P;
+    }
     return MPF_OK;
+  }
+  // Step order (spray.cpp:1155-1180): nucleation, deletion, melting/freezing.
+  int phase_change_with_delete(int frame, mpf_phase_info* info) {
+    int64_t Pn = P;
+    const int64_t added = nucleation(frame);
+    if (added < 0) return MPF_CAPACITY;
+    int64_t deleted = c.delete_enabled ? delete_particles(&Pn) : 0;
+    int melted = 0, frozen = 0;
+    melt_freeze(Pn, melted, frozen);
+    if (info) {
+      info->melted = melted; info->frozen = frozen; info->gas_added = (int)added; info->deleted = (int)deleted;
+      info->particles = P;
+    }
+    return MPF_OK;
+  }
+
+  // ================= f1: particle deletion, pouring, seed refresh =================
+  // sweepVacuum (spray_k.cu:4117-4132) as two-buffer passes;
+  // markDeleteAirParticle (4134-4169); deleteparticles (4172-4194) as a
+  // stable compaction; deleteAirFluidParticle (spray.cu:2194-2248).  *Pn:
+  // particles the cell lists cover (remapped), or -1 when they are not kept.
+  int64_t delete_particles(int64_t* Pn) {
+    for (int t = 0; t < c.delete_vacuum_sweeps; ++t) {
+      std::vector<int8_t> in = mark;
+      for (int64_t idx = 0; idx < G; ++idx) {
+        if (in[idx] != AIR) continue;
+        int i, j, k;
+        ijk(idx, i, j, k);
+        for (int di = -1; di <= 1; di += 2)
+          for (int dj = -1; dj <= 1; dj += 2)
+            for (int dk = -1; dk <= 1; dk += 2)
+              if (valid(i + di, j + dj, k + dk) && in[cid(i + di, j + dj, k + dk)] == VACUUM) mark[idx] = VACUUM;
+      }
+    }
+    std::vector<uint32_t> off(P + 1, 0);
+    for (int64_t p = 0; p < P; ++p) {
+      uint32_t keep = 1;
+      if (flag[p] == FLUID) {
+        const T lsa = scalar_from_grid(P3(pos, p), lsair);
+        if (lsa < (T)-1.0f * h) keep = 0;
+      } else if (flag[p] != SOLID) {
+        int i, j, k;
+        ijk_of_pos(P3(pos, p), i, j, k);
+        int cnt = 0;
+        for (int di = -1; di <= 1; ++di)
+          for (int dj = -1; dj <= 1; ++dj)
+            for (int dk = -1; dk <= 1; ++dk)
+              if (valid(i + di, j + dj, k + dk) && mark[cid(i + di, j + dj, k + dk)] == VACUUM) cnt++;
+        keep = (cnt == 0 && mass[p] > (T)0.000001f) ? 1u : 0u;
+      }
+      off[p + 1] = off[p] + keep;
+    }
+    const int64_t total = off[P];
+    if (total == P) return 0;
+    auto compact = [&](auto& a, int w) {
+      for (int64_t p = 0; p < P; ++p)
+        if (off[p + 1] != off[p])
+          for (int q = 0; q < w; ++q) a[(size_t)w * off[p] + q] = a[(size_t)w * p + q];
+      a.resize((size_t)w * total);
+    };
+    compact(pos, 3); compact(vel, 3); compact(flag, 1); compact(mass, 1); compact(temp, 1);
+    compact(lheat, 1); compact(solub, 1); compact(gas, 1);
+    if (Pn && *Pn >= 0) {
+      for (int64_t g = 0; g < G; ++g) {
+        if (cstart[g] == MPF_CELL_UNDEF) continue;
+        const uint32_t ns = off[cstart[g]], ne = off[cend[g]];
+        cstart[g] = ne > ns ? ns : MPF_CELL_UNDEF;
+        cend[g] = ne > ns ? ne : MPF_CELL_UNDEF;
+      }
+      *Pn = off[*Pn];
+    }
+    const int64_t deleted = P - total;
+    P = total;
+    return deleted;
+  }
+  // pouringwater (spray_k.cu:5575-5596), pouring (spray.cu:3012-3027)
+  std::vector<T> pour_pos, pour_vel;
+  int8_t pour_flag = FLUID;
+  int64_t pour(int frame) {
+    const int64_t n = (int64_t)pour_pos.size() / 3;
+    if (n == 0 || c.pour_every <= 0 || frame % c.pour_every != 0 || P + n > c.particle_capacity) return 0;
+    const T pr = (T)(c.pour_pos_rand * c.samplespace), vr = (T)c.pour_vel_rand;
+    auto grow = [&](auto& a, int w) { a.resize((size_t)w * (P + n)); };
+    grow(pos, 3); grow(vel, 3); grow(flag, 1); grow(mass, 1); grow(temp, 1); grow(lheat, 1); grow(solub, 1);
+    grow(gas, 1);
+    for (int64_t i = 0; i < n; ++i) {
+      V3<T> rv, rp;
+      rv.x = (T)(uniform24(c.seed, POUR_TAG, (uint64_t)i, 0) * 2.0f - 1.0f);
+      rv.y = (T)(uniform24(c.seed, POUR_TAG, (uint64_t)i, 1) * 2.0f - 1.0f);
+      rv.z = (T)(uniform24(c.seed, POUR_TAG, (uint64_t)i, 2) * 2.0f - 1.0f);
+      rp.x = (T)(uniform24(c.seed, POUR_TAG, (uint64_t)i, 3) * 2.0f - 1.0f);
+      rp.y = (T)(uniform24(c.seed, POUR_TAG, (uint64_t)i, 4) * 2.0f - 1.0f);
+      rp.z = (T)(uniform24(c.seed, POUR_TAG, (uint64_t)i, 5) * 2.0f - 1.0f);
+      const int64_t q = P + i;
+      S3(pos, q, P3(pour_pos, i) + rp * pr);
+      S3(vel, q, P3(pour_vel, i) + rv * vr);
+      mass[q] = (T)c.m0; flag[q] = pour_flag; temp[q] = (T)c.pour_temperature; lheat[q] = (T)c.liquid_heat_th;
+      gas[q] = 0; solub[q] = 0;
+    }
+    P += n;
+    return n;
+  }
+  // updateSeedCell (spray.cu:2933-2950) with counter-hash random cells
+  int seed_ring = 0;
+  void update_seeds(int frame) {
+    const int every = c.seed_update_every, bound = 3, z = 2;
+    if (every <= 0 || seeds.empty() || frame % every != 0) return;
+    if (NX <= 2 * bound || NY <= 2 * bound || NZ <= z) return;
+    for (int q = 0; q < 4; ++q) {
+      const uint64_t hx = hash64(c.seed, SEED_TAG, (uint64_t)frame, (uint64_t)(2 * q));
+      const uint64_t hy = hash64(c.seed, SEED_TAG, (uint64_t)frame, (uint64_t)(2 * q + 1));
+      const int x = bound + (int)((hx >> 11) % (uint64_t)(NX - 2 * bound));
+      const int y = bound + (int)((hy >> 11) % (uint64_t)(NY - 2 * bound));
+      seeds[seed_ring] = (int32_t)cid(x, y, z);
+      seed_ring = (seed_ring + 1) % (int)seeds.size();
+    }
   }
 
   // ================= a18: marching cubes on the liquid level set =================
@@ -1288,6 +1425,8 @@ template <class T> struct Orc : OrcBase {
   int step(int frame, mpf_step_info* info) {
     mpf_step_info loc;
     std::memset(&loc, 0, sizeof(loc));
+    update_seeds(frame);
+    loc.poured = pour(frame);
     hash_sort();
     apply_forces();
     levelset_update();
@@ -1300,7 +1439,7 @@ template <class T> struct Orc : OrcBase {
     hash_sort();
     levelset_update();
     if (c.heat_enabled) heat(&loc.heat);
-    int st2 = phase_change(frame, &loc.phase);
+    int st2 = phase_change_with_delete(frame, &loc.phase);
     if (st2 != MPF_OK) return st2;
     if (c.mc_enabled) {
       int st3 = marching_cubes(&loc.mc);
@@ -1459,7 +1598,29 @@ int64_t orc_field_count(orc_state* s, int f) {
 int orc_get_field(orc_state* s, int f, void* dst) { return dispatch(s, [&](auto& o) { return o.get_field(f, dst); }); }
 int orc_set_field(orc_state* s, int f, const void* src) { return dispatch(s, [&](auto& o) { return o.set_field(f, src); }); }
 int orc_set_seed_cells(orc_state* s, const int32_t* cells, int32_t n) {
-  return dispatch(s, [&](auto& o) { o.seeds.assign(cells, cells + n); return 0; });
+  return dispatch(s, [&](auto& o) { o.seeds.assign(cells, cells + n); o.seed_ring = 0; return 0; });
+}
+int orc_get_seed_cells(orc_state* s, int32_t* cells) {
+  return dispatch(s, [&](auto& o) { std::copy(o.seeds.begin(), o.seeds.end(), cells); return (int)o.seeds.size(); });
+}
+int orc_update_seed_cells(orc_state* s, int frame) { return dispatch(s, [&](auto& o) { o.update_seeds(frame); return 0; }); }
+int orc_set_pour_source(orc_state* s, int64_t n, const float* pos, const float* vel, int8_t flag) {
+  return dispatch(s, [&](auto& o) {
+    using R = typename std::remove_reference<decltype(o.pos)>::type::value_type;
+    o.pour_pos.assign(pos, pos + 3 * n);
+    o.pour_vel.assign(vel, vel + 3 * n);
+    (void)sizeof(R);
+    o.pour_flag = flag;
+    return 0;
+  });
+}
+int64_t orc_pour(orc_state* s, int frame) { return s->dbl ? static_cast<Orc<double>*>(s->impl)->pour(frame)
+                                                          : static_cast<Orc<float>*>(s->impl)->pour(frame); }
+int64_t orc_delete_particles(orc_state* s, int keep_lists) {
+  return dispatch(s, [&](auto& o) {
+    int64_t Pn = keep_lists ? o.P : -1;
+    return (int)o.delete_particles(&Pn);
+  });
 }
 int orc_hash_sort(orc_state* s) { return dispatch(s, [](auto& o) { o.hash_sort(); return 0; }); }
 int orc_apply_forces(orc_state* s) { return dispatch(s, [](auto& o) { o.apply_forces(); return 0; }); }

@@ -49,6 +49,12 @@ int64_t orc_field_count(orc_state* s, int field);
 int orc_get_field(orc_state* s, int field, void* dst);
 int orc_set_field(orc_state* s, int field, const void* src);
 int orc_set_seed_cells(orc_state* s, const int32_t* cells, int32_t n);
+int orc_get_seed_cells(orc_state* s, int32_t* cells); /* returns the count */
+int orc_update_seed_cells(orc_state* s, int frame);
+int orc_set_pour_source(orc_state* s, int64_t n, const float* pos, const float* vel, int8_t flag);
+int64_t orc_pour(orc_state* s, int frame);
+/* keep_lists: the cell lists index every particle and are remapped. */
+int64_t orc_delete_particles(orc_state* s, int keep_lists);
 
 int orc_hash_sort(orc_state* s);
 int orc_apply_forces(orc_state* s);

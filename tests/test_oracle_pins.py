@@ -734,3 +734,144 @@ def test_phigrad_linear_field(pkg, orc):
     np.testing.assert_allclose(gy[:, [0, -1]], b / 2, rtol=1e-12)
     np.testing.assert_allclose(gz[:, :, 1:-1], c, rtol=1e-12)
     np.testing.assert_allclose(gz[:, :, [0, -1]], c / 2, rtol=1e-12)
+
+
+# ------------------------------------------------------------- deletion, pouring, seeds
+def test_sweep_vacuum_diagonal_closed_form(pkg, orc):
+    """sweepVacuum (spray_k.cu:4117-4132) as two-buffer passes: an AIR cell
+    turns VACUUM when one of its eight corner-diagonal neighbours is VACUUM.
+    From one VACUUM cell in AIR, T passes reach exactly the offsets whose
+    three components share a parity and whose largest magnitude is <= T
+    (each diagonal step flips every component's parity); BOUNDARY cells
+    neither change nor spread."""
+    n = 14
+    for T in (1, 2, 3):
+        cfg = small_cfg(pkg, n, delete_vacuum_sweeps=T)
+        o = orc.Oracle(cfg)
+        mark = boxed_marks(n, 1)
+        c0 = (7, 6, 7)
+        mark[c0] = 3
+        o.set_field("MARK", mark)
+        o.set_particles(0)
+        assert o.delete_particles() == 0
+        got = o.get_field("MARK").reshape(n, n, n)
+        I, J, K = np.meshgrid(*[np.arange(n)] * 3, indexing="ij")
+        a, b, c = I - c0[0], J - c0[1], K - c0[2]
+        reach = ((a % 2) == (b % 2)) & ((b % 2) == (c % 2)) & (np.maximum(np.maximum(abs(a), abs(b)), abs(c)) <= T)
+        exp = np.where(mark == 2, 2, np.where(reach, 3, 1))
+        assert np.array_equal(got, exp), T
+
+
+def test_delete_particle_rules_and_compaction(pkg, orc):
+    """markDeleteAirParticle (spray_k.cu:4134-4169) + deleteparticles
+    (4172-4194): SOLID always stays; FLUID goes where the air level set is
+    below -h; AIR and AIRSOLO go with a VACUUM cell in their 3x3x3
+    neighbourhood or with mass <= 1e-6.  Survivors keep their order and the
+    cell lists are remapped to the compacted array (equal to a fresh sort)."""
+    n = 12
+    cfg = small_cfg(pkg, n, delete_vacuum_sweeps=0)
+    o = orc.Oracle(cfg)
+    rng = np.random.default_rng(29)
+    P = 4000
+    pos = ((rng.random((P, 3)) * (n - 2) + 1) * H).astype(np.float32)
+    flag = rng.choice(np.array([0, 1, 4, 5], np.int8), P)
+    mass = np.full(P, 1e-3, np.float32)
+    mass[rng.random(P) < 0.05] = 1e-7
+    o.set_particles(P, pos=pos, flag=flag, mass=mass)
+    o.hash_sort()
+    sp = o.get_particles()
+    mark = boxed_marks(n, 1)
+    vac = [(3, 3, 3), (8, 4, 6), (5, 9, 2)]
+    for v in vac:
+        mark[v] = 3
+    o.set_field("MARK", mark)
+    # lsair = (x - 6h) along x: below -h for x < 5h
+    I = np.meshgrid(*[np.arange(n)] * 3, indexing="ij")[0]
+    o.set_field("LSAIR", ((I + 0.5 - 6.0) * H).ravel())
+    deleted = o.delete_particles(keep_lists=True)
+    x = sp["pos"] / H
+    cell = np.floor(x).astype(int)
+    near = np.zeros(len(x), bool)
+    for v in vac:
+        near |= np.all(np.abs(cell - np.array(v)) <= 1, axis=1)
+    keep = np.ones(len(x), bool)
+    fl = sp["flag"]
+    keep[(fl == 0) & (x[:, 0] - 6.0 < -1.0)] = False
+    air = (fl == 1) | (fl == 4)
+    keep[air & (near | (sp["mass"] <= 1e-6))] = False
+    assert deleted == (~keep).sum() and 0 < deleted < P
+    assert (keep[fl == 5]).all()
+    got = o.get_particles()
+    for k in ("pos", "flag", "mass"):
+        assert np.array_equal(got[k], sp[k][keep]), k
+    cs, ce = o.get_field("CELL_START"), o.get_field("CELL_END")
+    ref = orc.Oracle(cfg)
+    ref.set_particles(int(keep.sum()), pos=sp["pos"][keep].astype(np.float32), flag=sp["flag"][keep])
+    ref.hash_sort()
+    assert np.array_equal(ref.get_field("GRID_INDEX"), np.arange(keep.sum()))  # order already sorted
+    assert np.array_equal(cs, ref.get_field("CELL_START"))
+    ne = cs != pkg.CELL_UNDEF
+    assert np.array_equal(ce[ne], ref.get_field("CELL_END")[ne])
+
+
+def test_pour_known_answers(pkg, orc):
+    """pouringwater (spray_k.cu:5575-5596): every pour_every frames the source
+    particles are appended at src + (2u - 1) pos_rand samplespace with
+    velocity src_v + (2u - 1) vel_rand, mass m0, the source's type, the pour
+    temperature and latent heat = threshold; u from the counter hash of
+    (seed, pour tag, source index, component 3..5 / 0..2); no pour when it
+    would exceed the capacity."""
+    n = 10
+    cfg = small_cfg(pkg, n, pour_every=4, pour_pos_rand=0.1, pour_vel_rand=0.05, pour_temperature=300.0,
+                    liquid_heat_th=10.0, seed=11, particle_capacity=30)
+    o = orc.Oracle(cfg, use_double=True)
+    src = np.array([[0.3, 0.4, 0.5], [0.31, 0.41, 0.5], [0.32, 0.42, 0.5]], np.float32) * 0.2
+    sv = np.array([[0.0, 0.0, -1.0], [0.1, 0.0, -1.0], [0.0, 0.2, -1.0]], np.float32)
+    o.set_pour_source(src, sv, flag=4)
+    o.set_particles(2, pos=np.full((2, 3), 0.05, np.float32))
+    assert o.pour(1) == 0 and o.particle_count() == 2
+    assert o.pour(8) == 3 and o.particle_count() == 5
+    got = o.get_particles()
+    tag = 0x706f7572 << 32
+    f = lambda v: float(np.float32(v))
+    for i in range(3):
+        u = [orc.uniform(11, tag, i, q) for q in range(6)]
+        rv = np.array(u[:3]) * 2 - 1
+        rp = np.array(u[3:]) * 2 - 1
+        np.testing.assert_allclose(got["pos"][2 + i], src[i].astype(float) + rp * f(f(0.1) * f(cfg.samplespace)),
+                                   rtol=0, atol=1e-12)
+        np.testing.assert_allclose(got["vel"][2 + i], sv[i].astype(float) + rv * f(0.05), rtol=0, atol=1e-12)
+    assert list(got["flag"][2:]) == [4, 4, 4]
+    assert np.allclose(got["mass"][2:], f(cfg.m0)) and np.all(got["T"][2:] == 300.0)
+    assert np.all(got["lheat"][2:] == 10.0)
+    assert o.pour(12) == 3 and o.particle_count() == 8
+    for _ in range(7):
+        o.pour(16)
+    assert o.particle_count() == 29  # 29 + 3 > 30: the next pour is skipped
+    assert o.pour(20) == 0 and o.particle_count() == 29
+
+
+def test_seed_refresh_ring(pkg, orc):
+    """updateSeedCell (spray.cu:2933-2950): on frames that are multiples of
+    seed_update_every, four seed slots (a ring over the list) get cells at
+    k = 2 with 3 <= i, j < N - 3; other frames change nothing."""
+    n = 16
+    cfg = small_cfg(pkg, n, seed_update_every=5, seed=3)
+    o = orc.Oracle(cfg)
+    seeds0 = np.array([1, 2, 3, 4, 5, 6], np.int32)
+    o.set_seed_cells(seeds0)
+    o.update_seed_cells(3)
+    assert np.array_equal(o.get_seed_cells(), seeds0)
+    o.update_seed_cells(5)
+    s1 = o.get_seed_cells()
+    assert np.array_equal(s1[4:], seeds0[4:]) and not np.array_equal(s1[:4], seeds0[:4])
+    o.update_seed_cells(10)
+    s2 = o.get_seed_cells()
+    assert np.array_equal(s2[2:4], s1[2:4])  # slots 4, 5, 0, 1 replaced
+    for c in np.concatenate([s1[:4], s2[[4, 5, 0, 1]]]):
+        i, j, k = c // (n * n), (c // n) % n, c % n
+        assert k == 2 and 3 <= i < n - 3 and 3 <= j < n - 3
+    o2 = orc.Oracle(cfg)
+    o2.set_seed_cells(seeds0)
+    o2.update_seed_cells(5)
+    assert np.array_equal(o2.get_seed_cells(), s1)
