@@ -9,9 +9,13 @@ marching cubes — on inputs already resident in HBM.  The mixed-precision form
 PCG form on the same initial state; both solve to the same residual.
 
 Prints ONE JSON line (rank 0, N=1).  Extra fields: fp64 steps/s and the ratio,
-particle-steps/s, PCG iterations/s, a roofline record for the dominant kernel
-(per-launch duration from HIP events in this process) and a CPU baseline (the
-oracle, OpenMP, on a bounded sample of the same workload).
+particle-steps/s, PCG iterations/s, whether every timed solve reached its
+tolerance on the FP64 true residual (the run exits 1 otherwise), a roofline
+record for the dominant kernel (average launch duration from HIP events on
+the context stream over the timed steps), the same mixed/FP64 comparison on a
+scene with air (the bubble column at 256^3: the solver's slow-row path) and a
+CPU baseline (the oracle with OpenMP, FP32 solve, one step of the same 256^3
+workload on this host's cores).
 """
 import argparse
 import importlib
@@ -84,9 +88,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(pkg, scenes, name, n, precision, steps, warmup, timing_step=True):
+def run_form(pkg, scenes, name, n, precision, steps, warmup, kernel_timing=True, **overrides):
+    """W warm-up + K timed mpf_step calls of one form.  With kernel_timing the
+    timed steps record a HIP-event pair around every launch on the context
+    stream, so the per-kernel table and the roofline come from those steps."""
     s = scenes.make(name, n=n, precision=precision)
     cfg = s["cfg"]
+    for k, v in overrides.items():
+        setattr(cfg, k, v)
     sim = pkg.Simulator(cfg)
     sim.set_particles(s["n_particles"], **s["particles"])
     if len(s["seeds"]):
@@ -109,27 +118,42 @@ def run_form(pkg, scenes, name, n, precision, steps, warmup, timing_step=True):
         frame += 1
     sim.synchronize()
     dt = time.perf_counter() - t0
-    log("[bench]   %.3f s for %d steps, PCG iterations %s" % (dt, steps, [i["project"]["iterations"] for i in infos]))
+    solves = [i["project"] for i in infos]
+    log("[bench]   %.3f s for %d steps, PCG iterations %s, true rel. residuals %s" % (
+        dt, steps, [p["iterations"] for p in solves], ["%.2e" % p["rel_residual"] for p in solves]))
     ktimes = None
-    unknowns = None
-    if timing_step:
-        sim.set_kernel_timing(True)
+    kt_iters = None
+    if kernel_timing:
+        # per-launch HIP events add ~7% to a step, so they run in a second
+        # window of the same length right after the timed one
         sim.reset_kernel_times()
-        sim.step(frame, allow_no_convergence=True)
-        ktimes = sim.kernel_times()
+        sim.set_kernel_timing(True)
+        kt_iters = []
+        for _ in range(steps):
+            info = sim.step(frame, allow_no_convergence=True)
+            kt_iters.append(info["project"]["iterations"])
+            solves.append(info["project"])
+            frame += 1
+        ktimes = sim.kernel_times()  # waits for the recorded events
         sim.set_kernel_timing(False)
-        mark = sim.get_field("MARK")
-        unknowns = int(((mark == pkg.TYPEFLUID) | (mark == pkg.TYPEAIR)).sum())
-    res = {"seconds": dt, "steps": steps, "pcg_iters": iters, "infos": infos, "ktimes": ktimes,
-           "particles": sim.particle_count(), "G": cfg.nx * cfg.ny * cfg.nz, "unknowns": unknowns}
+        log("[bench]   kernel-timing window: PCG iterations %s" % kt_iters)
+    mark = sim.get_field("MARK")
+    unknowns = int(((mark == pkg.TYPEFLUID) | (mark == pkg.TYPEAIR)).sum())
+    res = {"seconds": dt, "steps": steps, "pcg_iters": iters, "infos": infos, "ktimes": ktimes, "kt_iters": kt_iters,
+           "particles": sim.particle_count(), "G": cfg.nx * cfg.ny * cfg.nz, "unknowns": unknowns,
+           "converged": all(p["converged"] == 1 for p in solves),
+           "max_rel_residual": max(p["rel_residual"] for p in solves), "tol": cfg.tol}
     sim.close()
     return res
 
 
 def roofline(res, precision, form):
+    """Roofline record of the kernel with the largest total time over the
+    timed steps (HIP events on the context stream)."""
     kt = res["ktimes"] or {}
     if not kt:
         return None, None
+    K = res["steps"]
     dom = max(kt.items(), key=lambda kv: kv[1][1])
     name, (cnt, ms) = dom
     table = dict(KERNEL_BYTES)
@@ -142,46 +166,95 @@ def roofline(res, precision, form):
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if bytes_per_launch else None
     rf = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
           "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": measured_traffic(name, form),
-          "avg_launch_us": avg_ms * 1e3, "launches_per_step": cnt, "bytes_per_launch": bytes_per_launch,
-          "cells": G, "unknown_cells": U, "particles": P}
-    ranked = sorted(((k, v[0], v[1]) for k, v in kt.items()), key=lambda kv: -kv[2])
-    log("[bench]   kernel times of one step (name, launches, ms): " +
-        ", ".join("%s %d %.3f" % r for r in ranked))
-    top = [(k, ms) for k, _, ms in ranked[:8]]
+          "avg_launch_us": avg_ms * 1e3, "launches_per_step": cnt / K, "bytes_per_launch": bytes_per_launch,
+          "cells": G, "unknown_cells": U, "particles": P,
+          "measured_over": "HIP events on the context stream over the %d steps right after the timed ones "
+                           "(PCG iterations per step %.1f there, %.1f in the timed steps)" % (
+                               K, sum(res["kt_iters"]) / K, res["pcg_iters"] / K)}
+    ranked = sorted(((k, v[0] / K, v[1] / K) for k, v in kt.items()), key=lambda kv: -kv[2])
+    log("[bench]   kernel times per step of the kernel-timing window (name, launches, ms): " +
+        ", ".join("%s %.1f %.3f" % r for r in ranked))
+    top = [(k, ms_) for k, _, ms_ in ranked[:8]]
     return rf, top
 
 
-def cpu_baseline(scenes, name, n_full, n_sample, steps):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(scenes, name, n, steps, warm):
+    """The oracle (the reference's kernel bodies as OpenMP loops, FP32 fields
+    and an FP32 pressure solve as the reference's, dots summed in double) on
+    the same workload at full size, timed on this host."""
     import oracle
-    s = scenes.make(name, n=n_sample)
-    threads = min(os.cpu_count() or 1, 16)
-    oracle.set_threads(threads)
+    s = scenes.make(name, n=n)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    threads = oracle.set_threads(threads)
     o = oracle.Oracle(s["cfg"])
+    o.set_solve_float(True)
     o.set_particles(s["n_particles"], **s["particles"])
     if len(s["seeds"]):
         o.set_seed_cells(s["seeds"])
-    t0 = time.perf_counter()
-    for f in range(steps):
+    for f in range(warm):
         o.step(f, allow_no_convergence=True)
+    t0 = time.perf_counter()
+    its = []
+    for f in range(warm, warm + steps):
+        its.append(o.step(f, allow_no_convergence=True)["project"]["iterations"])
     dt = time.perf_counter() - t0
-    scale = (n_sample / float(n_full)) ** 3
-    return {"value": (steps / dt) * scale, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": "%s at %d^3 (%d particles), %d steps in %.1f s on %d threads, scaled by the cell ratio "
-                      "(%d/%d)^3 to %d^3 (PCG iteration growth with size not modelled)" % (
-                          name, n_sample, s["n_particles"], steps, dt, threads, n_sample, n_full, n_full)}
+    o.close()
+    return {"value": steps / dt, "unit": "steps/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "sample": "%s %d^3 (%d particles, the measured workload itself), %d step(s) after %d untimed, "
+                      "%.1f s on %d OpenMP threads, FP32 fields and FP32 PCG (PCG iterations %s)" % (
+                          name, n, s["n_particles"], steps, warm, dt, threads, its)}
+
+
+def air_scene(pkg, scenes, args):
+    """Mixed over FP64 on a scene with air (slow rows in the operator: the
+    two-kernel iteration plus the slow-row kernel)."""
+    mixed = run_form(pkg, scenes, args.air_scene, args.air_n, pkg.MIXED, args.air_steps, args.air_warmup,
+                     kernel_timing=False, tol=args.air_tol, max_iters=args.air_max_iters)
+    fp64 = run_form(pkg, scenes, args.air_scene, args.air_n, pkg.FP64, args.air_steps, args.air_warmup,
+                    kernel_timing=False, tol=args.air_tol, max_iters=args.air_max_iters)
+    m = mixed["steps"] / mixed["seconds"]
+    f = fp64["steps"] / fp64["seconds"]
+    return {"workload": "%s %d^3, %d particles, PCG to %.0e rel. residual in both forms" % (
+                args.air_scene, args.air_n, mixed["particles"], args.air_tol),
+            "steps": args.air_steps, "warmup": args.air_warmup, "mixed_steps_per_s": m, "fp64_steps_per_s": f,
+            "mixed_over_fp64": m / f, "mixed_pcg_iters_per_step": mixed["pcg_iters"] / mixed["steps"],
+            "fp64_pcg_iters_per_step": fp64["pcg_iters"] / fp64["steps"],
+            "converged": mixed["converged"] and fp64["converged"],
+            "max_rel_residual": max(mixed["max_rel_residual"], fp64["max_rel_residual"])}, [mixed, fp64]
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--scene", default="melt_freeze")
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-air", action="store_true")
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="time the steps without the per-launch HIP events (no kernel table, no roofline)")
+    ap.add_argument("--air-scene", default="bubble_column")
+    ap.add_argument("--air-tol", type=float, default=1e-6)
+    ap.add_argument("--air-max-iters", type=int, default=30000)
+    ap.add_argument("--air-n", type=int, default=256)
+    ap.add_argument("--air-steps", type=int, default=5)
+    ap.add_argument("--air-warmup", type=int, default=2)
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-warmup", type=int, default=0)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known HBM "
                          "traffic, for scaling profiler byte counters)")
@@ -203,10 +276,11 @@ def main():
         hm.synchronize()
         log("[bench] copy calibration: %d bytes read, %d bytes written" % (n, n))
         del src, dst
-    mixed = run_form(pkg, scenes, args.scene, args.n, pkg.MIXED, args.steps, args.warmup)
+    kt = not args.no_kernel_timing
+    mixed = run_form(pkg, scenes, args.scene, args.n, pkg.MIXED, args.steps, args.warmup, kernel_timing=kt)
     fp64 = None
     if not args.no_fp64:
-        fp64 = run_form(pkg, scenes, args.scene, args.n, pkg.FP64, args.steps, args.warmup)
+        fp64 = run_form(pkg, scenes, args.scene, args.n, pkg.FP64, args.steps, args.warmup, kernel_timing=kt)
     sps = mixed["steps"] / mixed["seconds"]
     rf, top = roofline(mixed, pkg.MIXED, "mixed")
     out = {
@@ -228,19 +302,33 @@ def main():
         "particle_steps_per_s": sps * mixed["particles"],
         "pcg_iters_per_s": mixed["pcg_iters"] / mixed["seconds"],
         "pcg_iters_per_step": mixed["pcg_iters"] / mixed["steps"],
+        "converged": mixed["converged"],
+        "max_true_rel_residual": mixed["max_rel_residual"],
+
         "roofline": rf,
-        "top_kernels_ms": top,
+        "top_kernels_ms_per_step": top,
     }
+    forms = [mixed]
     if fp64:
         f_sps = fp64["steps"] / fp64["seconds"]
         out["fp64_steps_per_s"] = f_sps
         out["mixed_over_fp64"] = sps / f_sps
         out["fp64_pcg_iters_per_step"] = fp64["pcg_iters"] / fp64["steps"]
+        out["fp64_converged"] = fp64["converged"]
+        out["fp64_max_true_rel_residual"] = fp64["max_rel_residual"]
         rf64, _ = roofline(fp64, pkg.FP64, "fp64")
         out["fp64_roofline"] = rf64
+        forms.append(fp64)
+    if not args.no_air:
+        out["air_scene"], more = air_scene(pkg, scenes, args)
+        forms += more
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scenes, args.scene, args.n, args.cpu_n, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(scenes, args.scene, args.cpu_n, args.cpu_steps, args.cpu_warmup)
     print(json.dumps(out), flush=True)
+    bad = [f for f in forms if not f["converged"] or f["max_rel_residual"] > f["tol"] * (1 + 1e-9)]
+    if bad:
+        log("[bench] a timed solve did not reach its tolerance: %s" % [(f["max_rel_residual"], f["tol"]) for f in bad])
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -58,6 +58,7 @@ def lib():
     L.orc_heat.argtypes = [vp, c.POINTER(pkg.SolveInfo)]
     L.orc_heat_grid.argtypes = [vp, c.POINTER(pkg.SolveInfo)]
     L.orc_heat_g2p.argtypes = [vp]
+    L.orc_set_solve_float.argtypes = [vp, c.c_int]
     for n in ("orc_phigrad", "orc_mark_from_counts", "orc_mark_solid_cells"):
         getattr(L, n).argtypes = [vp]
     L.orc_phase_change.argtypes = [vp, c.c_int, c.POINTER(pkg.PhaseInfo)]
@@ -227,6 +228,9 @@ class Oracle:
     def heat_g2p(self):
         """Grid-to-particle temperature map and latent-heat rules."""
         self._ok(self.L.orc_heat_g2p(self.h))
+
+    def set_solve_float(self, on=True):
+        self._ok(self.L.orc_set_solve_float(self.h, 1 if on else 0))
 
     def phigrad(self):
         self._ok(self.L.orc_phigrad(self.h))

@@ -242,6 +242,7 @@ template <class T> struct Orc : OrcBase {
   }
   template <class V> void gather(V& a, const std::vector<uint32_t>& idx, int w) {
     V o(a.size());
+#pragma omp parallel for schedule(static)
     for (int64_t s = 0; s < P; ++s)
       for (int q = 0; q < w; ++q) o[w * s + q] = a[w * (int64_t)idx[s] + q];
     a.swap(o);
@@ -657,34 +658,38 @@ template <class T> struct Orc : OrcBase {
     return bhi + th * (blo - bhi);
   }
   // A x: computeAx_bubble (spray_k.cu:3423-3447) for UNIT, 1/rho weights for VARDENS.
-  void apply_A(const std::vector<double>& x, std::vector<double>& y) const {
-    const double h2 = (double)h * (double)h;
+  // V: arithmetic of the vectors (double for the oracle; float for the
+  // CPU-baseline form that solves in FP32 as the reference does)
+  template <class V>
+  void apply_A(const std::vector<V>& x, std::vector<V>& y) const {
+    const V h2 = (V)((double)h * (double)h);
 #pragma omp parallel for schedule(static)
     for (int64_t idx = 0; idx < G; ++idx) {
-      if (!unknown(mark[idx])) { y[idx] = 0.0; continue; }
+      if (!unknown(mark[idx])) { y[idx] = V(0); continue; }
       int i, j, k;
       ijk(idx, i, j, k);
-      double xc = x[idx], s = 0.0;
+      V xc = x[idx], s = V(0);
       const int off[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
       if (c.op == MPF_OPERATOR_UNIT) {
-        s = -6.0 * xc;
+        s = V(-6) * xc;
         for (auto& o : off) {
           int8_t mn = mark_at(i + o[0], j + o[1], k + o[2]);
-          s += neumann(mn) ? xc : (unknown(mn) ? x[cid(i + o[0], j + o[1], k + o[2])] : 0.0);
+          s += neumann(mn) ? xc : (unknown(mn) ? x[cid(i + o[0], j + o[1], k + o[2])] : V(0));
         }
       } else {
         for (auto& o : off) {
           int8_t mn = mark_at(i + o[0], j + o[1], k + o[2]);
           if (neumann(mn)) continue;
           int64_t nb = cid(i + o[0], j + o[1], k + o[2]);
-          double xn = unknown(mn) ? x[nb] : 0.0;
-          s += face_beta(idx, nb) * (xn - xc);
+          V xn = unknown(mn) ? x[nb] : V(0);
+          s += (V)face_beta(idx, nb) * (xn - xc);
         }
       }
       y[idx] = s / h2;
     }
   }
-  void diag_A(std::vector<double>& d) const {
+  template <class V>
+  void diag_A(std::vector<V>& d) const {
     const double h2 = (double)h * (double)h;
 #pragma omp parallel for schedule(static)
     for (int64_t idx = 0; idx < G; ++idx) {
@@ -698,17 +703,23 @@ template <class T> struct Orc : OrcBase {
         if (neumann(mn)) continue;
         s += (c.op == MPF_OPERATOR_UNIT) ? 1.0 : face_beta(idx, cid(i + o[0], j + o[1], k + o[2]));
       }
-      d[idx] = s > 0 ? -s / h2 : 1.0;
+      d[idx] = (V)(s > 0 ? -s / h2 : 1.0);
     }
   }
-  // solver_cg_bubble (spray.cu:1613-1687) and the Jacobi-PCG form, in double.
+  bool solve_float = false;  // CPU baseline: FP32 vectors (dots still summed in double)
   int solve_pressure(mpf_solve_info* info) {
-    std::vector<double> x(G, 0.0), r(G), z(G), p(G), q(G), d(G), b(G);
-    for (int64_t i = 0; i < G; ++i) b[i] = unknown(mark[i]) ? (double)div[i] : 0.0;
+    return solve_float ? solve_pressure_t<float>(info) : solve_pressure_t<double>(info);
+  }
+  // solver_cg_bubble (spray.cu:1613-1687) and the Jacobi-PCG form; vectors
+  // in V (double for the oracle).
+  template <class V>
+  int solve_pressure_t(mpf_solve_info* info) {
+    std::vector<V> x(G, V(0)), r(G), z(G), p(G), q(G), d(G), b(G);
+    for (int64_t i = 0; i < G; ++i) b[i] = unknown(mark[i]) ? (V)div[i] : V(0);
     const bool ref = c.solver == MPF_SOLVER_REFERENCE_CG;
     if (!ref) diag_A(d);
-    auto dotv = [&](const std::vector<double>& u, const std::vector<double>& v) {
-      return rsum(G, [&](int64_t i) { return u[i] * v[i]; });
+    auto dotv = [&](const std::vector<V>& u, const std::vector<V>& v) {
+      return rsum(G, [&](int64_t i) { return (double)u[i] * (double)v[i]; });
     };
     // x = 0: r = b
     for (int64_t i = 0; i < G; ++i) r[i] = b[i];
@@ -720,7 +731,8 @@ template <class T> struct Orc : OrcBase {
     double rr = dotv(r, r);
     if (ref ? (rr < eps) : (bnorm == 0.0 || std::sqrt(rr) <= tol * bnorm)) conv = true;
     if (!conv) {
-      for (int64_t i = 0; i < G; ++i) z[i] = ref ? r[i] : (unknown(mark[i]) ? r[i] / d[i] : 0.0);
+#pragma omp parallel for schedule(static)
+      for (int64_t i = 0; i < G; ++i) z[i] = ref ? r[i] : (unknown(mark[i]) ? r[i] / d[i] : V(0));
       p = z;
       double rz = dotv(z, r);
       const int cap = ref ? c.ref_max_iters : c.max_iters;
@@ -729,23 +741,28 @@ template <class T> struct Orc : OrcBase {
         double pq = dotv(q, p);
         if (pq == 0.0) { conv = true; break; }
         double alpha = rz / pq;
+#pragma omp parallel for schedule(static)
         for (int64_t i = 0; i < G; ++i) {
           if (!unknown(mark[i])) continue;
-          x[i] += alpha * p[i];
-          r[i] -= alpha * q[i];
+          x[i] += (V)(alpha * p[i]);
+          r[i] -= (V)(alpha * q[i]);
         }
         ++it;
         rr = dotv(r, r);
         if (ref ? (rr < eps) : (std::sqrt(rr) <= tol * bnorm)) { conv = true; break; }
-        for (int64_t i = 0; i < G; ++i) z[i] = ref ? r[i] : (unknown(mark[i]) ? r[i] / d[i] : 0.0);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < G; ++i) z[i] = ref ? r[i] : (unknown(mark[i]) ? r[i] / d[i] : V(0));
         double rz2 = dotv(z, r);
         double beta = rz2 / rz;
-        for (int64_t i = 0; i < G; ++i) p[i] = unknown(mark[i]) ? z[i] + beta * p[i] : 0.0;
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < G; ++i) p[i] = unknown(mark[i]) ? (V)(z[i] + beta * p[i]) : V(0);
         rz = rz2;
       }
     }
-    apply_A(x, q);
-    double tr = rsum(G, [&](int64_t i) { double e = b[i] - q[i]; return e * e; });
+    // the true residual of the solution, in double
+    std::vector<double> xd(x.begin(), x.end()), qd(G), bd(b.begin(), b.end());
+    apply_A(xd, qd);
+    double tr = rsum(G, [&](int64_t i) { double e = bd[i] - qd[i]; return e * e; });
     for (int64_t i = 0; i < G; ++i) press[i] = (T)x[i];
     if (info) {
       info->iterations = it;
@@ -1635,6 +1652,9 @@ int orc_heat_grid(orc_state* s, mpf_solve_info* info) {
   return dispatch(s, [&](auto& o) { o.heat_grid(info); return 0; });
 }
 int orc_heat_g2p(orc_state* s) { return dispatch(s, [&](auto& o) { o.heat_g2p(); return 0; }); }
+int orc_set_solve_float(orc_state* s, int on) {
+  return dispatch(s, [&](auto& o) { o.solve_float = on != 0; return 0; });
+}
 int orc_phigrad(orc_state* s) { return dispatch(s, [&](auto& o) { o.phigrad(); return 0; }); }
 int orc_mark_from_counts(orc_state* s) { return dispatch(s, [&](auto& o) { o.mark_from_counts(); return 0; }); }
 int orc_mark_solid_cells(orc_state* s) { return dispatch(s, [&](auto& o) { o.mark_solid_cells(); return 0; }); }

@@ -73,6 +73,9 @@ int orc_heat_g2p(orc_state* s);
 /* Single pieces of orc_levelset_update on the state's fields: the level-set
    gradient of LSMERGE, and the count-based and solid cell flags (MARK) from
    the current cell lists. */
+/* Pressure solve with FP32 vectors (the reference's arithmetic; dot
+   products still summed in double): the CPU baseline's form. */
+int orc_set_solve_float(orc_state* s, int on);
 int orc_phigrad(orc_state* s);
 int orc_mark_from_counts(orc_state* s);
 int orc_mark_solid_cells(orc_state* s);
