@@ -172,7 +172,7 @@ bool alloc_all(Ctx& c) {
   ok = ok && dalloc(c, dd, G); c.vr2 = dd;
   ok = ok && dalloc(c, dd, G); c.vq2 = dd;
   ok = ok && dalloc(c, c.nbm, G) && dalloc(c, c.dslow, G);
-  c.npart = 2048 * 16;
+  c.npart = 1024 * 48;  // solver partial sums (pcg.hip lays out regions of 1024-block slots)
   ok = ok && dalloc(c, c.partials, c.npart) && dalloc(c, c.scal, 1) && dalloc(c, c.counters, NCOUNTERS);
   for (int a = 0; a < 3 && ok; ++a)
     ok = ok && dalloc(c, c.wu[a], fc[a]) && dalloc(c, c.au[a], fc[a]) && dalloc(c, c.wuo[a], fc[a]) &&
@@ -669,6 +669,7 @@ mpf_status mpf_project(mpf_ctx* ctx, mpf_solve_info* info) {
   if (!ctx) return MPF_INVALID_ARG;
   Ctx& c = ctx->c;
   c.err.clear();
+  c.note.clear();
   run_divergence(c);
   int st = run_pressure_solve(c, info);
   // any failure other than stopping at the cap leaves the pressure unusable:
@@ -677,7 +678,7 @@ mpf_status mpf_project(mpf_ctx* ctx, mpf_solve_info* info) {
   run_subgrad(c);
   mpf_status ls = check_launch(c);
   if (ls != MPF_OK) return ls;
-  if (st == MPF_NO_CONVERGENCE && c.err.empty()) c.err = "pressure solve stopped at its iteration cap";
+  if (st == MPF_NO_CONVERGENCE) c.err = c.note.empty() ? "pressure solve stopped at its iteration cap" : c.note;
   return (mpf_status)st;
 }
 

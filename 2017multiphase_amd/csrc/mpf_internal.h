@@ -132,6 +132,7 @@ struct Ctx {
   Grid g;
   hipStream_t stream = nullptr;
   std::string err;
+  std::string note;  // why the last solve stopped without converging (not an error)
   int device = 0;
 
   int64_t P = 0, cap = 0;

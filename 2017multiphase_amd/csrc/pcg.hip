@@ -373,6 +373,7 @@ static_assert((TK & (TK - 1)) == 0 && (ST / TK) * RPT == TJ, "threads cover the 
 
 struct Tiling {
   int ntj, ntk, seg, nblk;
+  int ns5;  // single-kernel iteration: blocks of slow-row partials after the nblk tile blocks
 };
 // Block -> tile run.  Blocks are dispatched round-robin over the 8 XCDs, so
 // each XCD takes a contiguous run of tiles (its L2 sees neighbouring tiles);
@@ -836,8 +837,11 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
 //
 // For the single-kernel iteration (q given): also q.z and q.r with the true
 // residual, and the four sums (r.z, r.r, q.z, q.r) per block go to Pf.
-template <int OPk, bool SLOW>
-__global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_resid64(OpArgs o, Tiling T, const float* __restrict__ b,
+// MODE 0: no slow rows; 1: slow rows inline (flags and level set read per
+// row); 2: slow rows skipped (their residual and sums come from
+// k_slow_resid64, whose partials follow this kernel's T.nblk blocks).
+template <int OPk, int MODE>
+__global__ void __launch_bounds__(ST, MODE == 1 ? 2 : 4) k_resid64(OpArgs o, Tiling T, const float* __restrict__ b,
                                                 const double* __restrict__ x64, const float* __restrict__ x,
                                                 float* __restrict__ r, Parts P, SolveScalars* sc, int k, int npad,
                                                 const float* __restrict__ qf, double* __restrict__ Pf) {
@@ -966,13 +970,14 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_resid64(OpArgs o, Tiling T
           const int jl = jl0 + u;
           const uint8_t nb = NB[s0][u];
           if (!(nb & NB_UNKNOWN)) continue;
+          if (MODE == 2 && (nb & NB_SLOW)) continue;
           const double xc = XV[s0][u];
           // neighbours in the order +x, +y, +z, -x, -y, -z
           const double xn[6] = {XV[s1][u], pl[scur][jl + 2][kl + 1], pl[scur][jl + 1][kl + 2],
                                 XV[s2][u], pl[scur][jl][kl + 1], pl[scur][jl + 1][kl]};
           const int64_t idx = pb + col[u];
           double ax;
-          if (!SLOW || !(nb & NB_SLOW)) {
+          if (MODE != 1 || !(nb & NB_SLOW)) {
             double sacc = 0.0;
 #pragma unroll
             for (int f = 0; f < 6; ++f)
@@ -998,8 +1003,8 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_resid64(OpArgs o, Tiling T
           (r + pb)[col[u]] = rf;
           a_rr += rv * rv;
           const float dv = !o.precond ? 1.0f
-                           : (SLOW && (nb & NB_SLOW)) ? dslow_at<float>(o, idx)
-                                                      : dtab[__popc((unsigned)(nb & 63))];
+                           : (MODE == 1 && (nb & NB_SLOW)) ? dslow_at<float>(o, idx)
+                                                           : dtab[__popc((unsigned)(nb & 63))];
           a_rz += (double)rf * (double)(rf * dv);
           if (qf) {
             const float qv = QN[d & 1][u];
@@ -1038,9 +1043,15 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_resid64(OpArgs o, Tiling T
 // 1); first = 2: a restart from a true residual (sums of the residual pass).
 // Pres (when not first): a true-residual pass ran after the previous launch;
 // its r.z, r.r, q.z, q.r replace those of the previous launch.
+//
+// Solves with slow rows (SLOW): p_k of a slow row takes its own diagonal
+// (loaded with the vectors); its q_k and its share of the q sums come from
+// k_slow_q5 right after this launch, which writes them as T.ns5 more blocks
+// after this launch's T.nblk partials, so the next launch sums all of them
+// in one fixed order (likewise the true-residual sums of k_slow_resid64).
 constexpr int FDOT = 7;  // r.z, r.r, p.q, q.z, q.Dq, q.r, q.q
-template <int OPk>
-__global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, int nbu, const double* __restrict__ Pprev,
+template <int OPk, bool SLOW>
+__global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_cgf(OpArgs o, Tiling T, int first, int nbu, const double* __restrict__ Pprev,
                                                const double* __restrict__ Pres, double* __restrict__ Pout,
                                                const float* __restrict__ rin, const float* __restrict__ qin,
                                                const float* __restrict__ pin, float* __restrict__ rout,
@@ -1062,15 +1073,15 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
     rr = v[1];
   } else if (first == 2) {
     double v[4];
-    read_partials<4, ST>(Pres, T.nblk, v);
+    read_partials<4, ST>(Pres, T.nblk + T.ns5, v);
     rz = v[0];
     rr = v[1];
   } else {
     double d[FDOT];
-    read_partials<FDOT, ST>(Pprev, T.nblk, d);
+    read_partials<FDOT, ST>(Pprev, T.nblk + T.ns5, d);
     if (Pres) {
       double e[4];
-      read_partials<4, ST>(Pres, T.nblk, e);
+      read_partials<4, ST>(Pres, T.nblk + T.ns5, e);
       d[0] = e[0];
       d[1] = e[1];
       d[3] = e[2];
@@ -1086,7 +1097,15 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
   const double bb = (first == 1 && k == 0) ? rr : sc->bb;
   const bool nonfin = bad(rz) || bad(rr) || bad_pq;
   const double rrc = rr > 0.0 ? rr : 0.0;
-  const bool conv = zero_pq || rrc == 0.0 || sqrt(rrc) <= tol * sqrt(bb);
+  // an expanded r.z whose sign differs from the previous one (r.z has the
+  // sign of the preconditioner for every nonzero r) has lost every
+  // significant digit: the recurrence is spent, so the solve stops and the
+  // host's FP64 true-residual check decides (it restarts from the true
+  // residual with exact sums, or accepts x)
+  // (also a step length that is not positive, or a recurrence residual grown
+  // far past the right-hand side: conjugacy lost at the FP32 floor)
+  const bool spent = !first && (!(rz * rz_prev > 0.0) || (!zero_pq && !(ald > 0.0)) || rr > 1e12 * bb);
+  const bool conv = zero_pq || rrc == 0.0 || spent || sqrt(rrc) <= tol * sqrt(bb);
   const bool stop = conv || nonfin || k >= cap;
   if (!first && !stop) be = rz / rz_prev;
   const V al = (V)ald, bev = (V)be;
@@ -1160,10 +1179,17 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
   const uint32_t hcol = hok ? (uint32_t)((j0 + hj) * g.nz + (k0 + hk)) : 0u;
   // loads as in the spmv: uniform plane base, 32-bit column, every lane
   // issues every load (element 0 of the plane when there is nothing to read)
-  struct Rw {
+  // (the slow-row diagonal exists only in the SLOW form: a field that is
+  // constant there still shifts the register allocation of the fast form)
+  struct RwS {
+    V r, q, p, x, ds;
+    uint8_t nb;
+  };
+  struct RwF {
     V r, q, p, x;
     uint8_t nb;
   };
+  using Rw = typename std::conditional<SLOW, RwS, RwF>::type;
   auto load_m = [&](int i, bool ok, uint32_t c) -> uint8_t {
     const uint8_t* mp = o.nbm + plane(i);
     return mp[(ok && i >= 0 && i < g.nx) ? c : 0u];
@@ -1171,19 +1197,27 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
   auto load_v = [&](int i, bool ok, uint8_t raw, uint32_t c, Rw& w, bool with_x) {
     const uint8_t nb = (ok && i >= 0 && i < g.nx) ? raw : (uint8_t)0;
     const int64_t pb = plane(i);
-    const uint32_t cv = (nb & NB_UNKNOWN) ? c : 0u;
+    const bool un = (nb & NB_UNKNOWN) != 0;
+    const uint32_t cv = un ? c : 0u;
     w.nb = nb;
     w.r = (rin + pb)[cv];
     w.q = (qin + pb)[cv];
     w.p = (pin + pb)[cv];
     w.x = with_x ? (x + pb)[cv] : V(0);
+    if constexpr (SLOW) w.ds = ((const V*)o.dslow + pb)[(un && (nb & NB_SLOW) && o.precond) ? c : 0u];
   };
   // r_k, and p_k = z_k + beta p_{k-1} (0 off the unknowns)
   auto r_of = [&](const Rw& w) -> V { return first ? w.r : w.r - al * w.q; };
   auto d_of = [&](uint8_t nb) -> V { return o.precond ? dtab[__popc((unsigned)(nb & 63))] : V(1); };
+  auto dw_of = [&](const Rw& w) -> V {
+    if constexpr (SLOW) {
+      if (w.nb & NB_SLOW) return o.precond ? w.ds : V(1);
+    }
+    return d_of(w.nb);
+  };
   auto p_of = [&](const Rw& w, V rk) -> V {
     if (!(w.nb & NB_UNKNOWN)) return V(0);
-    return first ? rk * d_of(w.nb) : rk * d_of(w.nb) + bev * w.p;
+    return first ? rk * dw_of(w) : rk * dw_of(w) + bev * w.p;
   };
   constexpr int UNR = 6;
   Rw R[3][RPT], HR[3];
@@ -1204,7 +1238,7 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
     } else if (!first) {
       (x + pb)[col[u]] = w.x + al * w.p;
     }
-    const V zv = rk * d_of(w.nb);
+    const V zv = rk * dw_of(w);
     a[0] += (double)rk * (double)zv;
     a[1] += (double)rk * (double)rk;
   };
@@ -1273,7 +1307,10 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
           sum = (nb & 8) ? sum + (PV[s2][u] - pcv) : sum;
           sum = (nb & 16) ? sum + (nym - pcv) : sum;
           sum = (nb & 32) ? sum + (nzm - pcv) : sum;
-          if (nb & NB_UNKNOWN) {
+          if (SLOW && (nb & NB_SLOW)) {
+            // slow row: p_k only, q_k from k_slow_q5
+            if (nb & NB_UNKNOWN) (pout + pb)[col[u]] = pcv;
+          } else if (nb & NB_UNKNOWN) {
             const V qv = sum * qc;
             const V rk = RV[s0][u];
             const V dv = d_of(nb);
@@ -1291,6 +1328,88 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
     }
   }
   write_partials<FDOT, ST>(a, Pout);
+}
+
+// q_k = A p_k on the slow rows after k_cgf stored every p_k and r_k, with
+// their share of the sums p.q, q.z, q.Dq, q.r, q.q (z = D r, D the stored
+// slow-row diagonal); a fixed grid walks the list, so the sums are
+// deterministic.
+template <int OPk>
+__global__ void __launch_bounds__(PB) k_slow_q5(OpArgs o, const uint32_t* __restrict__ list, int64_t n,
+                                                const float* __restrict__ pc, const float* __restrict__ rk,
+                                                float* __restrict__ q, double* __restrict__ S7, SolveScalars* sc,
+                                                int k) {
+  using V = float;
+  if (sc->done) return;  // the solve stopped at launch k: no later launch reads these sums
+  double a[FDOT] = {0, 0, 0, 0, 0, 0, 0};  // r.z, r.r are k_cgf's
+  {
+    const Grid& g = o.g;
+    for (int64_t t = (int64_t)blockIdx.x * PB + threadIdx.x; t < n; t += (int64_t)gridDim.x * PB) {
+      const int64_t idx = list[t];
+      int i, j, kk;
+      ijk_of(g, idx, i, j, kk);
+      V pn[6];
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        const int ii = i + c_off[f][0], jj = j + c_off[f][1], k2 = kk + c_off[f][2];
+        pn[f] = valid(g, ii, jj, k2) ? pc[cid(g, ii, jj, k2)] : V(0);
+      }
+      const V pcv = pc[idx];
+      const V qv = slow_row<V, OPk>(o, i, j, kk, idx, pcv, pn[0], pn[1], pn[2], pn[3], pn[4], pn[5]);
+      q[idx] = qv;
+      const V r = rk[idx];
+      const V dv = o.precond ? dslow_at<V>(o, idx) : V(1);
+      const V zv = r * dv;
+      a[2] += (double)pcv * (double)qv;
+      a[3] += (double)qv * (double)zv;
+      a[4] += (double)qv * (double)(dv * qv);
+      a[5] += (double)qv * (double)r;
+      a[6] += (double)qv * (double)qv;
+    }
+  }
+  write_partials<FDOT>(a, S7);
+}
+
+// FP64 true residual of the slow rows (the tile pass skipped them), with the
+// sums r.z, r.r, q.z, q.r written as blocks after the tile pass's partials.
+template <int OPk>
+__global__ void __launch_bounds__(PB) k_slow_resid64(OpArgs o, const uint32_t* __restrict__ list, int64_t n,
+                                                     const float* __restrict__ b, const double* __restrict__ x64,
+                                                     const float* __restrict__ x, float* __restrict__ r,
+                                                     const float* __restrict__ qf, double* __restrict__ R4,
+                                                     SolveScalars* sc) {
+  if (sc->done) return;
+  const Grid& g = o.g;
+  double a[4] = {0, 0, 0, 0};
+  for (int64_t t = (int64_t)blockIdx.x * PB + threadIdx.x; t < n; t += (int64_t)gridDim.x * PB) {
+    const int64_t idx = list[t];
+    int ci, cj, ck;
+    ijk_of(g, idx, ci, cj, ck);
+    const int8_t m = o.mark[idx];
+    const double xc = x64[idx] + (double)x[idx];
+    double sacc = 0.0;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      const int ii = ci + c_off[f][0], jj = cj + c_off[f][1], kk = ck + c_off[f][2];
+      const int8_t mn = mark_at(g, o.mark, ii, jj, kk);
+      if (neumann(o, OPk, mn)) continue;
+      const int64_t nbi = cid(g, ii, jj, kk);
+      const double xn = is_unknown(OPk, mn) ? x64[nbi] + (double)x[nbi] : 0.0;
+      const double bf = (OPk == OP_UNIT) ? 1.0 : face_beta<double>(o, idx, nbi, m, mn);
+      sacc += bf * (xn - xc);
+    }
+    const double ax = sacc / ((double)g.h * (double)g.h);
+    const double rv = (double)b[idx] - ax;
+    const float rf = (float)rv;
+    r[idx] = rf;
+    const float dv = o.precond ? dslow_at<float>(o, idx) : 1.0f;
+    a[0] += (double)rf * (double)(rf * dv);
+    a[1] += rv * rv;
+    const float qv = qf[idx];
+    a[2] += (double)qv * (double)(rf * dv);
+    a[3] += (double)qv * (double)rf;
+  }
+  write_partials<4>(a, R4);
 }
 
 // MIXED: x64 += x at exit.
@@ -1450,10 +1569,11 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
            (const uint32_t*)c.count, slow_list);
   int nsb = (int)((nslow + PB - 1) / PB);
   auto spmv = nslow > 0 ? k_spmv<V, Dacc, OPk, true> : k_spmv<V, Dacc, OPk, false>;
-  auto resid64 = nslow > 0 ? k_resid64<OPk, true> : k_resid64<OPk, false>;
+  auto resid64 = nslow > 0 ? k_resid64<OPk, 1> : k_resid64<OPk, 0>;
   // the MIXED form without slow rows runs one kernel per iteration (k_cgf)
-  const bool fused = std::is_same<V, float>::value && mixed && !ref && nslow == 0;
-  const Tiling T = make_tiling(c.g, fused ? 512 : ((sizeof(V) == 4 && nslow == 0) ? 768 : 512));
+  const bool fused = std::is_same<V, float>::value && mixed && !ref;
+  Tiling T = make_tiling(c.g, fused ? 512 : ((sizeof(V) == 4 && nslow == 0) ? 768 : 512));
+  T.ns5 = fused ? nsb : 0;
   // U-slot partials: update and init write L.nblk, the residual pass T.nblk;
   // both pad to npad, the count the spmv sums
   const int npad = L.nblk > T.nblk ? L.nblk : T.nblk;
@@ -1471,29 +1591,44 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
   // vectors stops with MPF_NO_CONVERGENCE instead of cycling
   int restarts = 0;
   bool true_missed = false;
+  double prev_true_rr = -1.0;  // FP64 true r.r at the last restart (stagnation test)
   if constexpr (std::is_same<V, float>::value) {
     if (fused) {
       float* rb[2] = {(float*)c.vr, (float*)c.vr2};
       float* qb[2] = {(float*)c.vq, (float*)c.vq2};
       float* pf[2] = {(float*)c.vp, (float*)c.vp2};
-      double* S7[2] = {c.partials + 8 * MAX_BLK, c.partials + 15 * MAX_BLK};
-      double* Rs = c.partials + 22 * MAX_BLK;
+      // partial sums: per launch T.nblk tile blocks then T.ns5 slow-row blocks
+      double* S7[2] = {c.partials + 8 * MAX_BLK, c.partials + 22 * MAX_BLK};
+      double* Rs = c.partials + 36 * MAX_BLK;
+      auto cg_iter = nslow > 0 ? k_cgf<OPk, true> : k_cgf<OPk, false>;
+      auto res_tiles = nslow > 0 ? k_resid64<OPk, 2> : k_resid64<OPk, 0>;
+      // FP64 true residual: the tile pass, then the slow rows' list pass
+      auto true_resid = [&](int kk, float* rr_, const float* qq_) {
+        launch(c, "pcg_resid64", dim3(T.nblk), dim3(ST), 0, res_tiles, o, T, (const float*)b, (const double*)x64,
+               (const float*)x, rr_, P, c.scal, kk, npad, qq_, Rs);
+        if (nslow > 0)
+          launch(c, "pcg_slow_resid64", dim3(nsb), dim3(PB), 0, k_slow_resid64<OPk>, o, (const uint32_t*)slow_list,
+                 nslow, (const float*)b, (const double*)x64, (const float*)x, rr_, qq_, Rs + 4 * T.nblk, c.scal);
+      };
       int first = 1, fold = 0;
       bool use_res = false;
       while (k <= cap) {
         const int k_end = k + check;
         for (; k < k_end && k <= cap; ++k) {
           const double* pres = first == 1 ? (const double*)P.U1 : (first == 2 || use_res) ? Rs : nullptr;
-          launch(c, "pcg_cg_iter", dim3(T.nblk), dim3(ST), 0, k_cgf<OPk>, o, T, first, npad,
-                 (const double*)S7[(k + 1) & 1], pres, S7[k & 1], (const float*)rb[k & 1], (const float*)qb[k & 1],
-                 (const float*)pf[k & 1], rb[(k + 1) & 1], qb[(k + 1) & 1], pf[(k + 1) & 1], (float*)x, x64, fold,
-                 c.scal, k, tol, cap);
+          launch(c, "pcg_cg_iter", dim3(T.nblk), dim3(ST), 0, cg_iter, o, T, first, npad,
+                 (const double*)S7[(k + 1) & 1], pres, S7[k & 1], (const float*)rb[k & 1],
+                 (const float*)qb[k & 1], (const float*)pf[k & 1], rb[(k + 1) & 1], qb[(k + 1) & 1],
+                 pf[(k + 1) & 1], (float*)x, x64, fold, c.scal, k, tol, cap);
+          if (nslow > 0)
+            launch(c, "pcg_slow_q5", dim3(nsb), dim3(PB), 0, k_slow_q5<OPk>, o, (const uint32_t*)slow_list, nslow,
+                   (const float*)pf[(k + 1) & 1], (const float*)rb[(k + 1) & 1], qb[(k + 1) & 1],
+                   S7[k & 1] + FDOT * T.nblk, c.scal, k);
           first = 0;
           fold = 0;
           use_res = false;
           if ((k + 1) % R == 0) {
-            launch(c, "pcg_resid64", dim3(T.nblk), dim3(ST), 0, resid64, o, T, (const float*)b, (const double*)x64,
-                   (const float*)x, rb[(k + 1) & 1], P, c.scal, k, npad, (const float*)qb[(k + 1) & 1], Rs);
+            true_resid(k, rb[(k + 1) & 1], (const float*)qb[(k + 1) & 1]);
             use_res = true;
             fold = 1;
           }
@@ -1506,13 +1641,16 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
         // FP64 true residual of that x, else the iteration restarts from it
         const int kd = s.iter;
         launch(c, "pcg_clear", dim3(1), dim3(64), 0, k_clear_done, c.scal);
-        launch(c, "pcg_resid64", dim3(T.nblk), dim3(ST), 0, resid64, o, T, (const float*)b, (const double*)x64,
-               (const float*)x, rb[kd & 1], P, c.scal, kd, npad, (const float*)qb[kd & 1], Rs);
-        launch(c, "pcg_collect", dim3(1), dim3(PB), 0, k_collect_rr4, (const double*)Rs, T.nblk, c.scal);
+        true_resid(kd, rb[kd & 1], (const float*)qb[kd & 1]);
+        launch(c, "pcg_collect", dim3(1), dim3(PB), 0, k_collect_rr4, (const double*)Rs, T.nblk + T.ns5, c.scal);
         if (!read_scalars(c)) return MPF_HIP_ERROR;
         const SolveScalars& t = *c.scal_host;
         if (t.rr_true == 0.0 || sqrt(t.rr_true) <= tol * sqrt(t.bb)) break;
-        if (kd >= cap || ++restarts > MAX_RESTARTS) {
+        // stop at the cap, after too many restarts, or when a restart did
+        // not at least halve the true residual (FP32 vectors at their limit)
+        const bool stagnant = prev_true_rr >= 0.0 && t.rr_true > 0.25 * prev_true_rr;
+        prev_true_rr = t.rr_true;
+        if (kd >= cap || ++restarts > MAX_RESTARTS || stagnant) {
           true_missed = true;
           break;
         }
@@ -1602,8 +1740,8 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
     return MPF_NONFINITE;
   }
   if (true_missed)
-    c.err = "mixed-precision solve: the FP64 true residual missed the tolerance (" + std::to_string(restarts) +
-            " restarts, iteration " + std::to_string(s.iter) + ")";
+    c.note = "mixed-precision solve: the FP64 true residual missed the tolerance (" + std::to_string(restarts) +
+             " restarts, iteration " + std::to_string(s.iter) + ")";
   return converged ? MPF_OK : MPF_NO_CONVERGENCE;
 }
 
