@@ -31,6 +31,7 @@ SOLVER_REFERENCE_CG, SOLVER_JACOBI_PCG = 0, 1
 OPERATOR_UNIT, OPERATOR_VARDENS = 0, 1
 ADVECT_EULER_COLLIDE, ADVECT_RK2 = 0, 1
 HEAT_CG, HEAT_JACOBI = 0, 1
+MC_LEVELSET, MC_LIQUID_GAS, MC_GAS, MC_SOLID = 0, 1, 2, 3
 
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "NONFINITE", 3: "NO_CONVERGENCE", 4: "CAPACITY",
           5: "HIP_ERROR", 6: "IO"}
@@ -56,7 +57,7 @@ ABI_FUNCTIONS = [
     "mpf_set_kernel_timing", "mpf_kernel_times", "mpf_reset_kernel_times",
     "mpf_project_adjoint", "mpf_extrapolate_adjoint", "mpf_p2g_adjoint", "mpf_g2p_adjoint",
     "mpf_advect_adjoint", "mpf_hash_sort_noalloc", "mpf_update_seed_cells", "mpf_set_pour_source",
-    "mpf_delete_particles", "mpf_pour",
+    "mpf_delete_particles", "mpf_pour", "mpf_mc_hash_sort", "mpf_get_mc_field", "mpf_get_mc_cells",
 ]
 
 
@@ -95,6 +96,7 @@ class Config(ctypes.Structure):
         ("delete_enabled", ctypes.c_int32), ("delete_vacuum_sweeps", ctypes.c_int32),
         ("pour_every", ctypes.c_int32), ("pour_pos_rand", ctypes.c_float), ("pour_vel_rand", ctypes.c_float),
         ("pour_temperature", ctypes.c_float), ("seed_update_every", ctypes.c_int32),
+        ("mc_source", ctypes.c_int32), ("mc_gas_airsolo", ctypes.c_int32), ("mc_every", ctypes.c_int32),
     ]
 
 
@@ -167,6 +169,9 @@ def lib():
     L.mpf_marching_cubes.argtypes = [vp, c.POINTER(McInfo)]
     L.mpf_get_mesh.argtypes = [vp, vp, vp, i32]
     L.mpf_get_mc_cases.argtypes = [vp, vp, i32]
+    L.mpf_mc_hash_sort.argtypes = [vp]
+    L.mpf_get_mc_field.argtypes = [vp, vp, i32]
+    L.mpf_get_mc_cells.argtypes = [vp, vp, vp, i32]
     L.mpf_step.argtypes = [vp, i32, c.POINTER(StepInfo)]
     L.mpf_set_kernel_timing.argtypes = [vp, i32]
     L.mpf_kernel_times.argtypes = [vp, vp, vp, vp, i32]
@@ -200,6 +205,13 @@ def default_config(nx, ny, nz, **overrides):
         else:
             setattr(cfg, k, v)
     return cfg
+
+
+def mc_voxels(cfg):
+    """Voxels of the marching-cubes lattice of cfg.mc_source."""
+    if cfg.mc_source == MC_LEVELSET:
+        return (cfg.nx - 1) * (cfg.ny - 1) * (cfg.nz - 1)
+    return 8 * cfg.nx * cfg.ny * cfg.nz
 
 
 def hash_sort_noalloc(cfg, n, pos, grid_hash, grid_index, cell_start, cell_end, stream=None):
@@ -387,10 +399,28 @@ class Simulator:
         return v, t
 
     def get_mc_cases(self):
-        n = (self.cfg.nx - 1) * (self.cfg.ny - 1) * (self.cfg.nz - 1)
+        n = mc_voxels(self.cfg)
         a = np.empty(n, dtype=np.uint8)
         self._check(self.L.mpf_get_mc_cases(self.h, _ptr(a), HOST))
         return a
+
+    def mc_hash_sort(self):
+        """preMC (spray.cu:759-765): hash, stable sort and reorder at 2N."""
+        self._check(self.L.mpf_mc_hash_sort(self.h))
+
+    def get_mc_field(self):
+        c = self.cfg
+        a = np.empty((2 * c.nx + 1) * (2 * c.ny + 1) * (2 * c.nz + 1), dtype=np.float32)
+        self._check(self.L.mpf_get_mc_field(self.h, _ptr(a), HOST))
+        return a
+
+    def get_mc_cells(self):
+        c = self.cfg
+        n = 8 * c.nx * c.ny * c.nz
+        s0 = np.empty(n, dtype=np.uint32)
+        s1 = np.empty(n, dtype=np.uint32)
+        self._check(self.L.mpf_get_mc_cells(self.h, _ptr(s0), _ptr(s1), HOST))
+        return s0, s1
 
     def update_seed_cells(self, frame):
         self._check(self.L.mpf_update_seed_cells(self.h, frame))

@@ -125,6 +125,7 @@ void free_all(Ctx& c) {
   }
   dfree(c.scan_blk); dfree(c.ctmp_u32); dfree(c.p2g_parts);
   dfree(c.mc_case); dfree(c.mc_edge); dfree(c.mc_tcount); dfree(c.mc_verts); dfree(c.mc_tris);
+  dfree(c.mc_count); dfree(c.mc_cstart); dfree(c.mc_cend); dfree(c.mc_field);
   dfree(c.seeds);
   dfree(c.pour_pos);
   dfree(c.pour_vel);
@@ -141,9 +142,21 @@ void free_all(Ctx& c) {
   c.stream = nullptr;
 }
 
+// Nodes and voxels of the marching-cubes lattice of the configured source.
+void mc_lattice(const Ctx& c, int64_t& nodes, int64_t& voxels) {
+  if (c.cfg.mc_source == MPF_MC_LEVELSET) {
+    nodes = c.g.G;
+    voxels = (int64_t)(c.g.nx - 1) * (c.g.ny - 1) * (c.g.nz - 1);
+  } else {
+    nodes = (int64_t)(c.g2.nx + 1) * (c.g2.ny + 1) * (c.g2.nz + 1);
+    voxels = c.g2.G;
+  }
+}
+
 bool alloc_all(Ctx& c) {
   const int64_t G = c.g.G, cap = c.cap;
-  const int64_t V = (int64_t)(c.g.nx - 1) * (c.g.ny - 1) * (c.g.nz - 1);
+  int64_t MN = 0, V = 0;
+  mc_lattice(c, MN, V);
   const int64_t fc[3] = {c.g.FX, c.g.FY, c.g.FZ};
   bool ok = true;
   for (int b = 0; b < 2 && ok; ++b) {
@@ -178,13 +191,17 @@ bool alloc_all(Ctx& c) {
     ok = ok && dalloc(c, c.wu[a], fc[a]) && dalloc(c, c.au[a], fc[a]) && dalloc(c, c.wuo[a], fc[a]) &&
          dalloc(c, c.auo[a], fc[a]) && dalloc(c, c.su[a], fc[a]) && dalloc(c, c.ftmp[a], fc[a]);
   int64_t maxscan = std::max<int64_t>(3 * G, std::max<int64_t>(V, cap));
+  if (c.cfg.mc_enabled) maxscan = std::max<int64_t>(maxscan, std::max<int64_t>(3 * MN, c.g2.G));
   c.scan_blk_n = maxscan / 4096 + 4;
   ok = ok && dalloc(c, c.scan_blk, c.scan_blk_n) && dalloc(c, c.ctmp_u32, G);
   ok = ok && dalloc(c, c.p2g_parts, (size_t)p2g_parts_floats(c.g));
   if (ok && c.cfg.mc_enabled) {
-    ok = ok && dalloc(c, c.mc_case, V > 0 ? V : 1) && dalloc(c, c.mc_edge, 6 * G) &&
+    ok = ok && dalloc(c, c.mc_case, V > 0 ? V : 1) && dalloc(c, c.mc_edge, 6 * MN) &&
          dalloc(c, c.mc_tcount, 2 * (V > 0 ? V : 1)) && dalloc(c, c.mc_verts, 3 * c.cfg.mc_max_verts) &&
          dalloc(c, c.mc_tris, 3 * c.cfg.mc_max_tris);
+    if (c.cfg.mc_source != MPF_MC_LEVELSET)
+      ok = ok && dalloc(c, c.mc_count, c.g2.G) && dalloc(c, c.mc_cstart, c.g2.G) && dalloc(c, c.mc_cend, c.g2.G) &&
+           dalloc(c, c.mc_field, MN);
   }
   if (ok && c.cfg.adjoint_enabled) ok = adj_alloc_all(c);
   if (!ok) {
@@ -337,6 +354,9 @@ mpf_status mpf_config_default(mpf_config* cfg, int32_t nx, int32_t ny, int32_t n
   cfg->pour_vel_rand = 0.0f;
   cfg->pour_temperature = 293.15f;           // spray.cpp:54
   cfg->seed_update_every = 0;
+  cfg->mc_source = MPF_MC_LEVELSET;
+  cfg->mc_gas_airsolo = 1;
+  cfg->mc_every = 1;
   return MPF_OK;
 }
 
@@ -352,6 +372,15 @@ mpf_status mpf_create(const mpf_config* cfg, mpf_ctx** out) {
       (cfg->op == MPF_OPERATOR_VARDENS && !(cfg->rho_liquid > 0 && cfg->rho_air > 0)) ||
       (cfg->heat_enabled && cfg->heat_solver == MPF_HEAT_CG && cfg->heat_alpha[0] <= 0)) {
     g_noctx_err = "invalid configuration";
+    return MPF_INVALID_ARG;
+  }
+  if (cfg->mc_source < MPF_MC_LEVELSET || cfg->mc_source > MPF_MC_SOLID || cfg->mc_every < 1) {
+    g_noctx_err = "invalid marching cubes configuration (mc_source, mc_every)";
+    return MPF_INVALID_ARG;
+  }
+  if (cfg->mc_enabled && cfg->mc_source != MPF_MC_LEVELSET &&
+      (int64_t)(2 * cfg->nx + 1) * (2 * cfg->ny + 1) * (2 * cfg->nz + 1) * 3 > 0xffffffffll) {
+    g_noctx_err = "grid too large for the 2N marching cubes lattice (3 edges per node must fit 32 bits)";
     return MPF_INVALID_ARG;
   }
   mpf_ctx* h = new (std::nothrow) mpf_ctx;
@@ -374,6 +403,17 @@ mpf_status mpf_create(const mpf_config* cfg, mpf_ctx** out) {
   g.dny = make_fastdiv((uint32_t)g.ny);
   g.dnz1 = make_fastdiv((uint32_t)g.nz + 1);
   g.dny1 = make_fastdiv((uint32_t)g.ny + 1);
+  // 2N hash grid of preMC (calcHashD_MC, spray_k.cu:1038-1042): cell size
+  // cellsize.x / NXMC * NX evaluated in float
+  Grid& g2 = c.g2;
+  g2.nx = 2 * g.nx; g2.ny = 2 * g.ny; g2.nz = 2 * g.nz;
+  g2.G = (int64_t)g2.nx * g2.ny * g2.nz;
+  g2.h = (cfg->dx / (float)g2.nx) * (float)g.nx;
+  g2.gx0 = g.gx0; g2.gy0 = g.gy0; g2.gz0 = g.gz0;
+  g2.dnz = make_fastdiv((uint32_t)g2.nz);
+  g2.dny = make_fastdiv((uint32_t)g2.ny);
+  g2.dnz1 = make_fastdiv((uint32_t)g2.nz + 1);
+  g2.dny1 = make_fastdiv((uint32_t)g2.ny + 1);
   c.cap = cfg->particle_capacity;
   hipGetDevice(&c.device);
   if (!hip_ok(c, hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "stream")) {
@@ -439,6 +479,7 @@ mpf_status mpf_set_particles(mpf_ctx* ctx, int64_t n, const float* pos, const fl
   c.P = n;
   c.sorted = false;
   c.own_sort = false;
+  c.mc_sorted = false;
   c.user_cstart = c.user_cend = false;
   return hip_ok(c, hipStreamSynchronize(c.stream), "set_particles") ? MPF_OK : MPF_HIP_ERROR;
 }
@@ -776,11 +817,23 @@ mpf_status mpf_phase_change(mpf_ctx* ctx, int32_t frame, mpf_phase_info* info) {
   return check_launch(c);
 }
 
+mpf_status mpf_mc_hash_sort(mpf_ctx* ctx) {
+  if (!ctx) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  if (!c.mc_cstart)
+    return fail(c, MPF_INVALID_ARG, "mpf_mc_hash_sort needs mc_enabled and a particle mc_source at mpf_create");
+  run_hash_sort_mc(c);
+  return check_launch(c);
+}
+
 mpf_status mpf_marching_cubes(mpf_ctx* ctx, mpf_mc_info* info) {
   if (!ctx) return MPF_INVALID_ARG;
   Ctx& c = ctx->c;
   c.err.clear();
   if (!c.cfg.mc_enabled) return fail(c, MPF_INVALID_ARG, "marching cubes disabled in the configuration");
+  if (c.cfg.mc_source != MPF_MC_LEVELSET && !c.mc_sorted)
+    return fail(c, MPF_INVALID_ARG, "particles not in 2N order: call mpf_mc_hash_sort first");
   int st = run_marching_cubes(c, info);
   if (st != MPF_OK) return (mpf_status)st;
   return check_launch(c);
@@ -806,6 +859,32 @@ mpf_status mpf_get_mc_cases(mpf_ctx* ctx, uint8_t* cases, int32_t where) {
   hipMemcpyKind kind = where == MPF_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
   if (c.mc_nvox) hipMemcpyAsync(cases, c.mc_case, (size_t)c.mc_nvox, kind, c.stream);
   return hip_ok(c, hipStreamSynchronize(c.stream), "get_mc_cases") ? MPF_OK : MPF_HIP_ERROR;
+}
+
+mpf_status mpf_get_mc_field(mpf_ctx* ctx, float* field, int32_t where) {
+  if (!ctx || !field) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  if (where != MPF_HOST && where != MPF_DEVICE) return fail(c, MPF_INVALID_ARG, "bad memory location");
+  if (!c.mc_field) return fail(c, MPF_INVALID_ARG, "no particle marching cubes source in the configuration");
+  if (!c.mc_field_valid) return fail(c, MPF_INVALID_ARG, "no particle-source mesh built yet");
+  int64_t n = 0, v = 0;
+  mc_lattice(c, n, v);
+  hipMemcpyAsync(field, c.mc_field, (size_t)n * 4, where == MPF_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice,
+                 c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "get_mc_field") ? MPF_OK : MPF_HIP_ERROR;
+}
+
+mpf_status mpf_get_mc_cells(mpf_ctx* ctx, uint32_t* cell_start, uint32_t* cell_end, int32_t where) {
+  if (!ctx) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  if (where != MPF_HOST && where != MPF_DEVICE) return fail(c, MPF_INVALID_ARG, "bad memory location");
+  if (!c.mc_cstart) return fail(c, MPF_INVALID_ARG, "no particle marching cubes source in the configuration");
+  hipMemcpyKind kind = where == MPF_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  if (cell_start) hipMemcpyAsync(cell_start, c.mc_cstart, (size_t)c.g2.G * 4, kind, c.stream);
+  if (cell_end) hipMemcpyAsync(cell_end, c.mc_cend, (size_t)c.g2.G * 4, kind, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "get_mc_cells") ? MPF_OK : MPF_HIP_ERROR;
 }
 
 mpf_status mpf_step(mpf_ctx* ctx, int32_t frame, mpf_step_info* info) {
@@ -841,7 +920,11 @@ mpf_status mpf_step(mpf_ctx* ctx, int32_t frame, mpf_step_info* info) {
     if ((st = check_launch(c)) != MPF_OK) return st;
     loc.phase.deleted = (int32_t)deleted;
   }
-  if (c.cfg.mc_enabled && (st = mpf_marching_cubes(ctx, &loc.mc)) != MPF_OK) return st;
+  if (c.cfg.mc_enabled && frame % c.cfg.mc_every == 0) {
+    // bubblesim's rendering stage (spray.cpp:1192-1201): preMC, then the mesh
+    if (c.cfg.mc_source != MPF_MC_LEVELSET && (st = mpf_mc_hash_sort(ctx)) != MPF_OK) return st;
+    if ((st = mpf_marching_cubes(ctx, &loc.mc)) != MPF_OK) return st;
+  }
   loc.particles = c.P;
   if (info) *info = loc;
   if (pst == MPF_NO_CONVERGENCE) c.err = perr;

@@ -1110,16 +1110,31 @@ __global__ void k_init_solid_phi(int64_t G, const uint8_t* __restrict__ cmask, f
 }
 
 // ---------------- marching cubes (a18) ----------------
-// classifyVoxel spray_k.cu:1853-1888 on lsfluid at cell centres; marks the
-// edges its triangles use (markActiveEdge_MC 2172-2220).
-__constant__ int c_eo[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
-                                {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
+// A node lattice: nodes (nx, ny, nz), voxels one fewer per axis; node
+// (i,j,k) at index (i*ny + j)*nz + k, position (i + off) * h + origin.  The
+// level-set source uses cell centres of the N grid (off 0.5); the particle
+// sources the (2N+1)^3 nodes of spacing dx/2 (off 0, spray_k.cu:1649).
+struct McGrid {
+  int nx, ny, nz;
+  int64_t N, V;
+  float h, off, x0, y0, z0;
+};
+__device__ __forceinline__ int64_t nid(const McGrid& m, int i, int j, int k) {
+  return ((int64_t)i * m.ny + j) * m.nz + k;
+}
+
+// Cube edges as (di, dj, dk, slot): the edge starts at node (i+di, j+dj, k+dk)
+// and runs along +x (slot 1), +y (slot 0) or +z (slot 2); its id is
+// 3 * node + slot, the numbering of GetEdgeID (spray_k.cu:1920-1957).
+__constant__ int c_eo[12][4] = {{0, 0, 0, 1}, {1, 0, 0, 0}, {0, 1, 0, 1}, {0, 0, 0, 0}, {0, 0, 1, 1}, {1, 0, 1, 0},
+                                {0, 1, 1, 1}, {0, 0, 1, 0}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
 __constant__ signed char c_tri[256][16];
 __constant__ unsigned char c_nv[256];
 
-__device__ __forceinline__ uint32_t mc_case_of(const Grid& g, const float* f, int i, int j, int k) {
-  const int64_t si = (int64_t)g.ny * g.nz, sj = g.nz;
-  const float* b = f + cid(g, i, j, k);
+// classifyVoxel, spray_k.cu:1853-1888 (isoValue 0)
+__device__ __forceinline__ uint32_t mc_case_of(const McGrid& m, const float* f, int i, int j, int k) {
+  const int64_t si = (int64_t)m.ny * m.nz, sj = m.nz;
+  const float* b = f + nid(m, i, j, k);
   float v[8] = {b[0], b[si], b[si + sj], b[sj], b[1], b[si + 1], b[si + sj + 1], b[sj + 1]};
   uint32_t ci = 0;
 #pragma unroll
@@ -1127,22 +1142,23 @@ __device__ __forceinline__ uint32_t mc_case_of(const Grid& g, const float* f, in
   return ci;
 }
 
-__global__ void k_mc_classify(Grid g, const float* __restrict__ f, uint8_t* __restrict__ cases,
-                              uint32_t* __restrict__ tcount, uint32_t* __restrict__ edge_used, int64_t V,
+// Case index, triangle count and the edges its triangles use
+// (markActiveEdge_MC, spray_k.cu:2172-2220).
+__global__ void k_mc_classify(McGrid m, const float* __restrict__ f, uint8_t* __restrict__ cases,
+                              uint32_t* __restrict__ tcount, uint32_t* __restrict__ edge_used,
                               uint32_t* __restrict__ active) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int nv = 0;
-  if (v < V) {
-    const int VY = g.ny - 1, VZ = g.nz - 1;
+  if (v < m.V) {
     int i, j, k;
-    ijk3(v, VY, VZ, i, j, k);
-    uint32_t ci = mc_case_of(g, f, i, j, k);
+    ijk3(v, m.ny - 1, m.nz - 1, i, j, k);
+    uint32_t ci = mc_case_of(m, f, i, j, k);
     cases[v] = (uint8_t)ci;
     nv = c_nv[ci];
     tcount[v] = (uint32_t)(nv / 3);
     for (int t = 0; t < nv; ++t) {
       int e = c_tri[ci][t];
-      edge_used[3 * cid(g, i + c_eo[e][0], j + c_eo[e][1], k + c_eo[e][2]) + c_eo[e][3]] = 1u;
+      edge_used[3 * nid(m, i + c_eo[e][0], j + c_eo[e][1], k + c_eo[e][2]) + c_eo[e][3]] = 1u;
     }
   }
   // active voxels: one add per block, spread over NSPREAD counters
@@ -1150,44 +1166,142 @@ __global__ void k_mc_classify(Grid g, const float* __restrict__ f, uint8_t* __re
   if (threadIdx.x == 0 && cnt) atomicAdd(active + (blockIdx.x % NSPREAD) * SPREAD_STRIDE, (uint32_t)cnt);
 }
 
-// vertexInterp spray_k.cu:1905-1910 along each used edge; edge_id holds the
-// exclusive scan of the used flags (= vertex index).
-__global__ void k_mc_verts(Grid g, const float* __restrict__ f, const uint32_t* __restrict__ used,
+// vertexInterp (spray_k.cu:1903-1908) along each used edge, from its start
+// node p0 to p1 = p0 + h along the edge (generateTriangles_indices,
+// spray_k.cu:2092-2117); vid holds the exclusive scan of the used flags.  The
+// reference writes a shared edge's vertex from whichever voxel comes last;
+// here every edge is interpolated once, in the + direction.
+__global__ void k_mc_verts(McGrid m, const float* __restrict__ f, const uint32_t* __restrict__ used,
                            const uint32_t* __restrict__ vid, float* __restrict__ verts) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= 3 * g.G || !used[e]) return;
-  int64_t n = e / 3;
-  int a = (int)(e % 3);
+  if (e >= 3 * m.N || !used[e]) return;
+  const int64_t n = e / 3;
+  const int slot = (int)(e % 3);
+  const int a = slot == 1 ? 0 : (slot == 0 ? 1 : 2);
   int i, j, k;
-  ijk_of(g, n, i, j, k);
-  int i2 = i + (a == 0), j2 = j + (a == 1), k2 = k + (a == 2);
-  float f0 = f[n], f1 = f[cid(g, i2, j2, k2)];
-  float t = (0.0f - f0) / (f1 - f0);
-  const float h = g.h;
-  float p0x = ((float)i + 0.5f) * h + g.gx0, p0y = ((float)j + 0.5f) * h + g.gy0, p0z = ((float)k + 0.5f) * h + g.gz0;
-  float p1x = ((float)i2 + 0.5f) * h + g.gx0, p1y = ((float)j2 + 0.5f) * h + g.gy0, p1z = ((float)k2 + 0.5f) * h + g.gz0;
-  uint32_t o = vid[e];
-  verts[3 * (int64_t)o + 0] = p0x + (p1x - p0x) * t;
-  verts[3 * (int64_t)o + 1] = p0y + (p1y - p0y) * t;
-  verts[3 * (int64_t)o + 2] = p0z + (p1z - p0z) * t;
+  ijk3(n, m.ny, m.nz, i, j, k);
+  const int i2 = i + (a == 0), j2 = j + (a == 1), k2 = k + (a == 2);
+  const float f0 = f[n], f1 = f[nid(m, i2, j2, k2)];
+  const float t = (0.0f - f0) / (f1 - f0);
+  const float h = m.h;
+  const float p0x = ((float)i + m.off) * h + m.x0, p0y = ((float)j + m.off) * h + m.y0,
+              p0z = ((float)k + m.off) * h + m.z0;
+  const float p1x = a == 0 ? p0x + h : p0x, p1y = a == 1 ? p0y + h : p0y, p1z = a == 2 ? p0z + h : p0z;
+  const uint32_t o = vid[e];
+  verts[3 * (int64_t)o + 0] = p0x + t * (p1x - p0x);
+  verts[3 * (int64_t)o + 1] = p0y + t * (p1y - p0y);
+  verts[3 * (int64_t)o + 2] = p0z + t * (p1z - p0z);
 }
 
 // generateTriangles_indices spray_k.cu:2074-2170 (indices = scanned edge ids)
-__global__ void k_mc_tris(Grid g, const uint8_t* __restrict__ cases, const uint32_t* __restrict__ toff,
-                          const uint32_t* __restrict__ vid, uint32_t* __restrict__ tris, int64_t V) {
+__global__ void k_mc_tris(McGrid m, const uint8_t* __restrict__ cases, const uint32_t* __restrict__ toff,
+                          const uint32_t* __restrict__ vid, uint32_t* __restrict__ tris) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= V) return;
+  if (v >= m.V) return;
   uint32_t ci = cases[v];
   int nv = c_nv[ci];
   if (!nv) return;
-  const int VY = g.ny - 1, VZ = g.nz - 1;
   int i, j, k;
-  ijk3(v, VY, VZ, i, j, k);
+  ijk3(v, m.ny - 1, m.nz - 1, i, j, k);
   int64_t w = 3 * (int64_t)toff[v];
   for (int t = 0; t < nv; ++t) {
     int e = c_tri[ci][t];
-    tris[w + t] = vid[3 * cid(g, i + c_eo[e][0], j + c_eo[e][1], k + c_eo[e][2]) + c_eo[e][3]];
+    tris[w + t] = vid[3 * nid(m, i + c_eo[e][0], j + c_eo[e][1], k + c_eo[e][2]) + c_eo[e][3]];
   }
+}
+
+// Node field of a particle source, one thread per node of the (2N+1)^3
+// lattice, reading the 4x4x4 cells of the 2N lists around it (di, dj, dk in
+// [-2, 1]):
+//  MPF_MC_LIQUID_GAS  genWaterDensfield_liquidAndGas + sumcelldens_liquidAndGas
+//                     (spray_k.cu:1595-1686): FLUID/AIR/AIRSOLO within R = 2h,
+//                     w = (R^2 - d^2)^3, phi = h/2 - |x - centre| (-h/2 when
+//                     no particle), lattice faces -1000;
+//  MPF_MC_GAS         genWaterDensfield_Gas + sumcelldens_Gas (1528-1593): AIR
+//                     and, when airsolo, AIRSOLO; r = 0.8h;
+//  MPF_MC_SOLID       genWaterDensfield_GY + sumcelldens3 (1688-1772): SOLID
+//                     within R = h, w = max((1 - (d/4h)^2)^3, 0), phi =
+//                     |x - centre| - 0.75h f(Ev) with the centre measured
+//                     from the origin (the reference's centertmp is zero),
+//                     lattice faces min(phi, -10).
+// Per cell the weighted positions are summed from zero and added to the
+// centre, the weights accumulate directly (the reference's order).
+template <int SRC>
+__global__ void k_mc_density(McGrid m, Grid g2, const f3* __restrict__ pos, const int8_t* __restrict__ flag,
+                             const uint32_t* __restrict__ cs, const uint32_t* __restrict__ ce,
+                             float* __restrict__ out, int airsolo) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= m.N) return;
+  int i, j, k;
+  ijk3(idx, m.ny, m.nz, i, j, k);
+  const float h = m.h;
+  const float px = ((float)i + m.off) * h + m.x0, py = ((float)j + m.off) * h + m.y0,
+              pz = ((float)k + m.off) * h + m.z0;
+  const float R = (SRC == MPF_MC_SOLID) ? h : h * 2.0f;
+  float cx = 0.0f, cy = 0.0f, cz = 0.0f, wsum = 0.0f;
+  for (int di = -2; di <= 1; ++di)
+    for (int dj = -2; dj <= 1; ++dj)
+      for (int dk = -2; dk <= 1; ++dk) {
+        const int a = i + di, b = j + dj, q = k + dk;
+        if (a < 0 || a > g2.nx - 1 || b < 0 || b > g2.ny - 1 || q < 0 || q > g2.nz - 1) continue;
+        const int64_t cell = ((int64_t)a * g2.ny + b) * g2.nz + q;
+        const uint32_t s = cs[cell];
+        if (s == MPF_CELL_UNDEF) continue;
+        const uint32_t e = ce[cell];
+        float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+        for (uint32_t p = s; p < e; ++p) {
+          const int8_t fl = flag[p];
+          bool take;
+          if (SRC == MPF_MC_LIQUID_GAS) take = fl == AIR || fl == AIRSOLO || fl == FLUID;
+          else if (SRC == MPF_MC_GAS) take = fl == AIR || (fl == AIRSOLO && airsolo);
+          else take = fl == SOLID;
+          if (!take) continue;
+          const f3 x = pos[p];
+          const float dx = x.x - px, dy = x.y - py, dz = x.z - pz;
+          const float dis = sqrtf(dx * dx + dy * dy + dz * dz);
+          if (dis < R) {
+            float w;
+            if (SRC == MPF_MC_SOLID) {
+              w = dis / (4.0f * h);
+              w = 1.0f - w * w;
+              w = fmaxf(w * w * w, 0.0f);
+            } else {
+              w = R * R - dis * dis;
+              w = w * w * w;
+            }
+            rx = rx + x.x * w;
+            ry = ry + x.y * w;
+            rz = rz + x.z * w;
+            wsum = wsum + w;
+          }
+        }
+        cx = cx + rx;
+        cy = cy + ry;
+        cz = cz + rz;
+      }
+  const float r = (SRC == MPF_MC_LIQUID_GAS) ? 0.5f * h : (SRC == MPF_MC_GAS ? 0.8f * h : 0.75f * h);
+  float phi;
+  if (wsum > 0.0f) {
+    cx = cx / wsum;
+    cy = cy / wsum;
+    cz = cz / wsum;
+    const float ex = px - cx, ey = py - cy, ez = pz - cz;
+    const float len = sqrtf(ex * ex + ey * ey + ez * ez);
+    if (SRC == MPF_MC_SOLID) {
+      const float thigh = 0.51f, tlow = 0.49f;
+      const float Ev = fmaxf(cx, fmaxf(cy, cz)) / (4.0f * h);
+      const float gamma = (thigh - Ev) / (thigh - tlow);
+      const float fr = (Ev < tlow) ? 1.0f : gamma * gamma * gamma - 3.0f * gamma * gamma + 3.0f * gamma;
+      phi = len - r * fr;
+    } else {
+      phi = r - len;
+    }
+  } else {
+    phi = -r;
+  }
+  const bool face = i == 0 || j == 0 || k == 0 || i == m.nx - 1 || j == m.ny - 1 || k == m.nz - 1;
+  if (face) phi = (SRC == MPF_MC_SOLID) ? fminf(phi, -10.0f) : -1000.0f;
+  out[idx] = phi;
 }
 
 }  // namespace
@@ -1486,23 +1600,59 @@ void run_heat_grid_post(Ctx& c) {
 }
 
 int run_marching_cubes(Ctx& c, mpf_mc_info* info) {
-  static bool tables_loaded = false;  // per process; the tables are constants
-  if (!tables_loaded) {
-    hipMemcpyToSymbol(HIP_SYMBOL(c_tri), MC_TRI_TABLE, sizeof(MC_TRI_TABLE));
-    hipMemcpyToSymbol(HIP_SYMBOL(c_nv), MC_NUM_VERTS, sizeof(MC_NUM_VERTS));
-    tables_loaded = true;
+  static bool tables_loaded[64] = {};  // per device; the tables are constants
+  const int dslot = c.device >= 0 && c.device < 64 ? c.device : 0;
+  if (!tables_loaded[dslot]) {
+    if (hipMemcpyToSymbol(HIP_SYMBOL(c_tri), MC_TRI_TABLE, sizeof(MC_TRI_TABLE)) != hipSuccess ||
+        hipMemcpyToSymbol(HIP_SYMBOL(c_nv), MC_NUM_VERTS, sizeof(MC_NUM_VERTS)) != hipSuccess) {
+      c.err = "marching cubes tables: hipMemcpyToSymbol failed";
+      return MPF_HIP_ERROR;
+    }
+    tables_loaded[dslot] = true;
   }
-  const int64_t G = c.g.G;
-  const int64_t V = (int64_t)(c.g.nx - 1) * (c.g.ny - 1) * (c.g.nz - 1);
+  const int src = c.cfg.mc_source;
+  McGrid m{};
+  const float* f;
+  if (src == MPF_MC_LEVELSET) {
+    m.nx = c.g.nx; m.ny = c.g.ny; m.nz = c.g.nz;
+    m.h = c.g.h;
+    m.off = 0.5f;
+    f = c.lsfluid;
+  } else {
+    m.nx = c.g2.nx + 1; m.ny = c.g2.ny + 1; m.nz = c.g2.nz + 1;
+    m.h = c.cfg.dx / 2.0f;  // cellsize.x / (NXMC / NX), spray_k.cu:1640
+    m.off = 0.0f;
+    f = c.mc_field;
+  }
+  m.x0 = c.g.gx0; m.y0 = c.g.gy0; m.z0 = c.g.gz0;
+  m.N = (int64_t)m.nx * m.ny * m.nz;
+  m.V = (int64_t)(m.nx - 1) * (m.ny - 1) * (m.nz - 1);
+  if (src != MPF_MC_LEVELSET) {
+    const f3* pos = (const f3*)c.pos[c.cur];
+    const int8_t* fl = (const int8_t*)c.flag[c.cur];
+    const uint32_t *cs = c.mc_cstart, *ce = c.mc_cend;
+    const dim3 gd(nblocks(m.N, 256));
+    if (src == MPF_MC_LIQUID_GAS)
+      launch(c, "mc_density_liquid_gas", gd, dim3(256), 0, k_mc_density<MPF_MC_LIQUID_GAS>, m, c.g2, pos, fl, cs, ce,
+             c.mc_field, c.cfg.mc_gas_airsolo);
+    else if (src == MPF_MC_GAS)
+      launch(c, "mc_density_gas", gd, dim3(256), 0, k_mc_density<MPF_MC_GAS>, m, c.g2, pos, fl, cs, ce, c.mc_field,
+             c.cfg.mc_gas_airsolo);
+    else
+      launch(c, "mc_density_solid", gd, dim3(256), 0, k_mc_density<MPF_MC_SOLID>, m, c.g2, pos, fl, cs, ce,
+             c.mc_field, c.cfg.mc_gas_airsolo);
+    c.mc_field_valid = true;
+  }
+  const int64_t V = m.V, N = m.N;
   c.mc_nvox = V;
-  hipMemsetAsync(c.mc_edge, 0, (size_t)3 * G * 4, c.stream);
+  hipMemsetAsync(c.mc_edge, 0, (size_t)3 * N * 4, c.stream);
   uint32_t* spread = c.counters + 64;
   hipMemsetAsync(spread, 0, NSPREAD * SPREAD_STRIDE * 4, c.stream);
-  launch(c, "mc_classify", dim3(nblocks(V, 256)), dim3(256), 0, k_mc_classify, c.g, (const float*)c.lsfluid, c.mc_case,
-         c.mc_tcount, c.mc_edge, V, spread);
+  launch(c, "mc_classify", dim3(nblocks(V, 256)), dim3(256), 0, k_mc_classify, m, f, c.mc_case, c.mc_tcount,
+         c.mc_edge, spread);
   // vertex ids = exclusive scan of the used-edge flags (second half of mc_edge)
-  uint32_t* vid = c.mc_edge + 3 * G;
-  int64_t nv = run_scan_u32(c, c.mc_edge, vid, 3 * G, true);
+  uint32_t* vid = c.mc_edge + 3 * N;
+  int64_t nv = run_scan_u32(c, c.mc_edge, vid, 3 * N, true);
   // triangle offsets = exclusive scan of per-voxel triangle counts (second half of mc_tcount)
   int64_t nt = run_scan_u32(c, c.mc_tcount, c.mc_tcount + V, V, true);
   if (nv < 0 || nt < 0) return MPF_HIP_ERROR;
@@ -1512,10 +1662,10 @@ int run_marching_cubes(Ctx& c, mpf_mc_info* info) {
     c.err = "marching cubes mesh exceeds mc_max_verts/mc_max_tris";
     return MPF_CAPACITY;
   }
-  launch(c, "mc_verts", dim3(nblocks(3 * G, 256)), dim3(256), 0, k_mc_verts, c.g, (const float*)c.lsfluid,
-         (const uint32_t*)c.mc_edge, (const uint32_t*)vid, c.mc_verts);
-  launch(c, "mc_tris", dim3(nblocks(V, 256)), dim3(256), 0, k_mc_tris, c.g, (const uint8_t*)c.mc_case,
-         (const uint32_t*)(c.mc_tcount + V), (const uint32_t*)vid, c.mc_tris, V);
+  launch(c, "mc_verts", dim3(nblocks(3 * N, 256)), dim3(256), 0, k_mc_verts, m, f, (const uint32_t*)c.mc_edge,
+         (const uint32_t*)vid, c.mc_verts);
+  launch(c, "mc_tris", dim3(nblocks(V, 256)), dim3(256), 0, k_mc_tris, m, (const uint8_t*)c.mc_case,
+         (const uint32_t*)(c.mc_tcount + V), (const uint32_t*)vid, c.mc_tris);
   c.mc_nv = nv;
   c.mc_nt = nt;
   if (info) {

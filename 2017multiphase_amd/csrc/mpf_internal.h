@@ -204,6 +204,13 @@ struct Ctx {
   float* mc_verts = nullptr;
   uint32_t* mc_tris = nullptr;
   int64_t mc_nv = 0, mc_nt = 0, mc_nvox = 0;
+  // particle-source marching cubes (cfg.mc_source != MPF_MC_LEVELSET): the
+  // 2N hash grid, its cell lists and the node field
+  Grid g2{};
+  uint32_t *mc_count = nullptr, *mc_cstart = nullptr, *mc_cend = nullptr;  // g2.G each
+  float* mc_field = nullptr;  // (2nx+1)(2ny+1)(2nz+1) nodes
+  bool mc_sorted = false;     // particles are in the order of the last 2N sort
+  bool mc_field_valid = false;
   // adjoint scratch (allocated on first use): 6 face cotangent arrays in and
   // out, 2 cell fields, per-particle cotangents
   double* adj_fin[6] = {};
@@ -275,6 +282,7 @@ void run_heat_grid_pre(Ctx& c);
 void run_heat_grid_post(Ctx& c);
 void run_heat_jacobi(Ctx& c, int iters);
 int run_marching_cubes(Ctx& c, mpf_mc_info* info);
+void run_hash_sort_mc(Ctx& c);
 void run_solid_phi_and_velocity(Ctx& c);
 int64_t p2g_parts_floats(const Grid& g);
 // Device counters: 64 single counters, then SPREAD counters 64 bytes apart

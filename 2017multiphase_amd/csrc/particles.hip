@@ -523,6 +523,7 @@ void run_advect(Ctx& c) {
   }
   c.sorted = false;
   c.own_sort = false;
+  c.mc_sorted = false;
 }
 
 void run_heat_g2p(Ctx& c) {
@@ -555,7 +556,7 @@ static int64_t run_nucleation(Ctx& c, int frame) {
   launch(c, "gas_emit", dim3(nblocks(c.g.G, 256)), dim3(256), 0, k_gas_emit, c.g, a, (const uint32_t*)c.ctmp_u32,
          (const uint32_t*)c.count, c.P);
   c.P += na;
-  if (na) c.sorted = c.own_sort = false;
+  if (na) c.sorted = c.own_sort = c.mc_sorted = false;
   return na;
 }
 
@@ -630,6 +631,7 @@ static int run_delete(Ctx& c, int64_t* pPn, int64_t* deleted, int64_t* fluid_del
   c.cur = nxt;
   c.P = total;
   c.own_sort = false;
+  c.mc_sorted = false;
   if (Pn >= 0) run_cell_mask(c);
   return MPF_OK;
 }
@@ -644,7 +646,7 @@ int run_phase_change(Ctx& c, int frame, mpf_phase_info* info) {
   run_melt_freeze(c, Pn);
   hipMemcpyAsync(c.counters_host + 1, c.counters + 1, 7 * 4, hipMemcpyDeviceToHost, c.stream);
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "phase_change")) return MPF_HIP_ERROR;
-  if (c.cfg.freeze_enabled && c.counters_host[7]) c.own_sort = false;  // a frozen particle snapped into another cell
+  if (c.cfg.freeze_enabled && c.counters_host[7]) c.own_sort = c.mc_sorted = false;  // a frozen particle snapped into another cell
   if (info) {
     info->melted = (int32_t)c.counters_host[1];
     info->frozen = (int32_t)c.counters_host[2];
@@ -701,7 +703,7 @@ int run_pour(Ctx& c, int frame, int64_t* added) {
          k.m0, k.pour_temperature, k.liquid_heat_th, k.seed);
   c.P += c.npour;
   *added = c.npour;
-  c.sorted = c.own_sort = false;
+  c.sorted = c.own_sort = c.mc_sorted = false;
   return MPF_OK;
 }
 

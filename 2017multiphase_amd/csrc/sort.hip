@@ -435,7 +435,37 @@ void run_hash_sort(Ctx& c) {
   c.user_cstart = c.user_cend = false;
   c.sorted = true;
   c.own_sort = true;
+  c.mc_sorted = false;
   run_cell_mask(c);
+}
+
+// cspray::preMC / hashAndSortParticles_MC (spray.cu:639-662, 759-765): the
+// same hash, stable sort and reorder on the 2N grid (calcHashD_MC,
+// spray_k.cu:1030-1048, cell size (dx / 2nx) * nx); its cell lists go to the
+// mc_* arrays, and the N-grid lists are stale afterwards.
+void run_hash_sort_mc(Ctx& c) {
+  const int64_t P = c.P, G2 = c.g2.G;
+  const int cur = c.cur, nxt = 1 - c.cur;
+  hipMemsetAsync(c.mc_count, 0, (size_t)G2 * 4, c.stream);
+  launch(c, "mc_hash_count", dim3(nblocks(P, 256)), dim3(256), 0, k_hash_count, c.g2, (const f3*)c.pos[cur], P,
+         c.key, c.mc_count);
+  run_scan_u32(c, c.mc_count, c.mc_cstart, G2, false);
+  launch(c, "mc_cells_from_scan", dim3(nblocks(G2, 256)), dim3(256), 0, k_cells_from_scan, G2, c.mc_count,
+         c.mc_cstart, c.mc_cend);
+  launch(c, "mc_scatter", dim3(nblocks(P, 256)), dim3(256), 0, k_scatter, P, (const uint32_t*)c.key, c.mc_count,
+         c.perm, c.ghash);
+  launch(c, "mc_segsort", dim3(nblocks(G2, 256)), dim3(256), 0, k_segsort, G2, (const uint32_t*)c.mc_cstart,
+         (const uint32_t*)c.mc_cend, c.perm);
+  launch(c, "mc_reorder", dim3(nblocks(P, 256)), dim3(256), 0, k_reorder, P, (const uint32_t*)c.perm,
+         (const f3*)c.pos[cur], (const f3*)c.vel[cur], (const int8_t*)c.flag[cur], (const float*)c.mass[cur],
+         (const float*)c.temp[cur], (const float*)c.lheat[cur], (const float*)c.solub[cur],
+         (const float*)c.gas[cur], c.pos[nxt], c.vel[nxt], c.flag[nxt], c.mass[nxt], c.temp[nxt], c.lheat[nxt],
+         c.solub[nxt], c.gas[nxt]);
+  c.cur = nxt;
+  c.user_cstart = c.user_cend = false;
+  c.sorted = false;
+  c.own_sort = false;
+  c.mc_sorted = true;
 }
 
 // ---- mpf_hash_sort_noalloc: the sort without a context -------------------

@@ -84,6 +84,18 @@ enum {
   MPF_ADVECT_RK2 = 1            /* advectparticle_RK2,   spray_k.cu:409-459    */
 };
 
+/* Field that marching cubes meshes (mpf_config.mc_source; runMC_smooth,
+   spray.cu:802-826). */
+enum {
+  MPF_MC_LEVELSET = 0,   /* lsfluid at cell centres                           */
+  MPF_MC_LIQUID_GAS = 1, /* runMC_fluid: genWaterDensfield_liquidAndGas,
+                            spray_k.cu:1595-1686, FLUID/AIR/AIRSOLO particles  */
+  MPF_MC_GAS = 2,        /* runMC_gas: genWaterDensfield_Gas,
+                            spray_k.cu:1528-1593, AIR (+AIRSOLO) particles     */
+  MPF_MC_SOLID = 3       /* runMC_solid: genWaterDensfield_GY,
+                            spray_k.cu:1688-1772, SOLID particles              */
+};
+
 /* Heat diffusion solver. */
 enum {
   MPF_HEAT_CG = 0,    /* solver_cg_heat, spray.cu:1780-1852                  */
@@ -163,6 +175,14 @@ typedef struct mpf_config {
   float pour_temperature;    /* defaultLiquidT (293.15, spray.cpp:54)         */
   /* nucleation seed refresh (updateSeedCell, spray.cu:2933-2950) */
   int32_t seed_update_every; /* frames between refreshes, 0 = never (15)      */
+  /* marching cubes source: MPF_MC_LEVELSET, or a particle field on the
+     (2nx+1)(2ny+1)(2nz+1) node lattice of spacing dx/2 (preMC + runMC_*,
+     spray.cu:759-826), whose buffers are allocated at mpf_create */
+  int32_t mc_source;
+  int32_t mc_gas_airsolo;    /* MPF_MC_GAS counts AIRSOLO particles (1; the
+                                reference's SCENE_INTERACTION uses 0)         */
+  int32_t mc_every;          /* mpf_step meshes frames that are multiples of
+                                this (1; the reference's outputframeDelta is 2) */
 } mpf_config;
 
 /* Opaque context. */
@@ -304,17 +324,39 @@ mpf_status mpf_delete_particles(mpf_ctx* ctx, int64_t* deleted);
    cfg.pour_pos_rand * samplespace and cfg.pour_vel_rand per component).
    *added (may be NULL): particles appended. */
 mpf_status mpf_pour(mpf_ctx* ctx, int32_t frame, int64_t* added);
-/* runMC_smooth stages 2-7 (spray.cu:804-979) on the liquid level set. */
+/* cspray::preMC, spray.cu:759-765 (hashAndSortParticles_MC, spray.cu:639-662;
+   calcHashD_MC, spray_k.cu:1030-1048): the hash, stable sort and particle
+   reorder of mpf_hash_sort on the 2nx x 2ny x 2nz grid of cell size
+   (dx / 2nx) * nx.  Particles come back in that order (MPF_F_GRID_HASH and
+   MPF_F_GRID_INDEX hold its keys and permutation); the cell lists of
+   mpf_hash_sort are stale until it runs again.  Needs a particle mc_source. */
+mpf_status mpf_mc_hash_sort(mpf_ctx* ctx);
+/* runMC_smooth (spray.cu:802-979) on cfg.mc_source: for MPF_MC_LEVELSET the
+   lattice is the cell centres of lsfluid; for a particle source the node
+   field of genWaterDensfield_* is built first from the lists of the last
+   mpf_mc_hash_sort, which must be current (MPF_INVALID_ARG otherwise).
+   Voxels are classified (classifyVoxel, spray_k.cu:1853-1888), active edges
+   numbered in the order of GetEdgeID (spray_k.cu:1925-1957), so vertex and
+   triangle indices follow the reference's numbering. */
 mpf_status mpf_marching_cubes(mpf_ctx* ctx, mpf_mc_info* info);
 /* Copy the last mesh out: verts float[3*nv], tri indices uint32[3*nt]. */
 mpf_status mpf_get_mesh(mpf_ctx* ctx, float* verts, uint32_t* tris, int32_t where);
-/* Per-voxel case index (uint8, (nx-1)*(ny-1)*(nz-1) voxels over cell centres). */
+/* Per-voxel case index of the last mesh (uint8): (nx-1)(ny-1)(nz-1) voxels
+   over cell centres, or 2nx*2ny*2nz for a particle source. */
 mpf_status mpf_get_mc_cases(mpf_ctx* ctx, uint8_t* cases, int32_t where);
+/* Node field of the last particle-source mesh, float[(2nx+1)(2ny+1)(2nz+1)]
+   (node (i,j,k) at index (i*(2ny+1) + j)*(2nz+1) + k). */
+mpf_status mpf_get_mc_field(mpf_ctx* ctx, float* field, int32_t where);
+/* Cell lists of the last mpf_mc_hash_sort, uint32[2nx*2ny*2nz] each (either
+   pointer may be NULL). */
+mpf_status mpf_get_mc_cells(mpf_ctx* ctx, uint32_t* cell_start, uint32_t* cell_end, int32_t where);
 
 /* One full step in bubblesim order (spray.cpp:979-1233), the stages above
    only: seed refresh, pour, sort, forces, level set, P2G, extrapolation,
    projection, extrapolation, G2P, advection, sort, level set, heat,
-   nucleation, deletion (cfg.delete_enabled), melting/freezing, marching cubes. */
+   nucleation, deletion (cfg.delete_enabled), melting/freezing, and on frames
+   that are multiples of cfg.mc_every marching cubes (after mpf_mc_hash_sort
+   for a particle source, as bubblesim calls preMC). */
 mpf_status mpf_step(mpf_ctx* ctx, int32_t frame, mpf_step_info* info);
 
 /* ---- adjoints ----------------------------------------------------------

@@ -65,6 +65,9 @@ def lib():
     L.orc_marching_cubes.argtypes = [vp, c.POINTER(pkg.McInfo)]
     L.orc_get_mesh.argtypes = [vp, vp, vp]
     L.orc_get_mc_cases.argtypes = [vp, vp]
+    L.orc_mc_hash_sort.argtypes = [vp]
+    L.orc_get_mc_field.argtypes = [vp, vp]
+    L.orc_get_mc_cells.argtypes = [vp, vp, vp]
     L.orc_step.argtypes = [vp, c.c_int, c.POINTER(pkg.StepInfo)]
     L.orc_hash_key.argtypes = [c.POINTER(pkg.Config), c.c_float, c.c_float, c.c_float]
     L.orc_hash_key.restype = c.c_uint32
@@ -258,10 +261,26 @@ class Oracle:
         return v, t
 
     def get_mc_cases(self):
-        n = (self.cfg.nx - 1) * (self.cfg.ny - 1) * (self.cfg.nz - 1)
-        a = np.empty(n, np.uint8)
+        a = np.empty(self.pkg.mc_voxels(self.cfg), np.uint8)
         self._ok(self.L.orc_get_mc_cases(self.h, _p(a)))
         return a
+
+    def mc_hash_sort(self):
+        self._ok(self.L.orc_mc_hash_sort(self.h))
+
+    def get_mc_field(self):
+        c = self.cfg
+        a = np.empty((2 * c.nx + 1) * (2 * c.ny + 1) * (2 * c.nz + 1))
+        self._ok(self.L.orc_get_mc_field(self.h, _p(a)))
+        return a
+
+    def get_mc_cells(self):
+        c = self.cfg
+        n = 8 * c.nx * c.ny * c.nz
+        s0 = np.empty(n, np.uint32)
+        s1 = np.empty(n, np.uint32)
+        self._ok(self.L.orc_get_mc_cells(self.h, _p(s0), _p(s1)))
+        return s0, s1
 
     def step(self, frame, allow_no_convergence=False):
         info = self.pkg.StepInfo()

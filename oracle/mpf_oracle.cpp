@@ -109,6 +109,9 @@ template <class T> struct Orc : OrcBase {
   // face fields
   std::vector<T> wu[3], au[3], wuo[3], auo[3], su[3];
   std::vector<int32_t> seeds;
+  // 2N lists and node field of the particle sources of marching cubes
+  std::vector<uint32_t> mc_cstart, mc_cend;
+  std::vector<T> mc_field;
   // marching cubes output
   std::vector<uint8_t> mc_case;
   std::vector<T> mc_verts;
@@ -215,7 +218,7 @@ template <class T> struct Orc : OrcBase {
       key[p] = (uint32_t)cid(i, j, k);
       idx[p] = (uint32_t)p;
     }
-    stable_counting_sort(key, idx);
+    stable_counting_sort(key, idx, G);
     std::fill(cstart.begin(), cstart.end(), MPF_CELL_UNDEF);
     std::fill(cend.begin(), cend.end(), MPF_CELL_UNDEF);
     for (int64_t s = 0; s < P; ++s) {
@@ -231,11 +234,11 @@ template <class T> struct Orc : OrcBase {
   }
   // Counting sort: count keys, exclusive scan, scatter in increasing index
   // order.  Stable by construction, so the result is ordered by (key, idx).
-  void stable_counting_sort(std::vector<uint32_t>& key, std::vector<uint32_t>& idx) {
-    std::vector<uint32_t> cnt(G + 1, 0);
+  void stable_counting_sort(std::vector<uint32_t>& key, std::vector<uint32_t>& idx, int64_t ncells) {
+    std::vector<uint32_t> cnt(ncells + 1, 0);
     for (int64_t p = 0; p < P; ++p) cnt[key[p]]++;
     uint32_t run = 0;
-    for (int64_t g = 0; g < G; ++g) { uint32_t t = cnt[g]; cnt[g] = run; run += t; }
+    for (int64_t g = 0; g < ncells; ++g) { uint32_t t = cnt[g]; cnt[g] = run; run += t; }
     std::vector<uint32_t> k2(P), i2(P);
     for (int64_t p = 0; p < P; ++p) { uint32_t s = cnt[key[p]]++; k2[s] = key[p]; i2[s] = idx[p]; }
     key.swap(k2); idx.swap(i2);
@@ -246,6 +249,38 @@ template <class T> struct Orc : OrcBase {
     for (int64_t s = 0; s < P; ++s)
       for (int q = 0; q < w; ++q) o[w * s + q] = a[w * (int64_t)idx[s] + q];
     a.swap(o);
+  }
+  // preMC / hashAndSortParticles_MC spray.cu:639-662, 759-765 and
+  // calcHashD_MC spray_k.cu:1030-1048: the same sort on the 2N grid of cell
+  // size cellsize.x / NXMC * NX, with its own cell lists.
+  void mc_hash_sort() {
+    const int X2 = 2 * NX, Y2 = 2 * NY, Z2 = 2 * NZ;
+    const int64_t G2 = (int64_t)X2 * Y2 * Z2;
+    const T d2 = ((T)c.dx / (T)X2) * (T)NX;
+    std::vector<uint32_t> key(P), idx(P);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < P; ++p) {
+      V3<T> q = P3(pos, p);
+      q = mk((q.x - (T)c.gmin[0]) / d2, (q.y - (T)c.gmin[1]) / d2, (q.z - (T)c.gmin[2]) / d2);
+      int i = (q.x >= 0 && q.x < (T)X2) ? (int)q.x : 0;
+      int j = (q.y >= 0 && q.y < (T)Y2) ? (int)q.y : 0;
+      int k = (q.z >= 0 && q.z < (T)Z2) ? (int)q.z : 0;
+      key[p] = (uint32_t)(((int64_t)i * Y2 + j) * Z2 + k);
+      idx[p] = (uint32_t)p;
+    }
+    stable_counting_sort(key, idx, G2);
+    mc_cstart.assign(G2, MPF_CELL_UNDEF);
+    mc_cend.assign(G2, MPF_CELL_UNDEF);
+    for (int64_t s = 0; s < P; ++s) {
+      if (s == 0 || key[s] != key[s - 1]) {
+        mc_cstart[key[s]] = (uint32_t)s;
+        if (s > 0) mc_cend[key[s - 1]] = (uint32_t)s;
+      }
+      if (s == P - 1) mc_cend[key[s]] = (uint32_t)(s + 1);
+    }
+    permute(idx);
+    ghash = key;
+    gindex = idx;
   }
   void permute(const std::vector<uint32_t>& idx) {
     gather(pos, idx, 3); gather(vel, idx, 3); gather(flag, idx, 1); gather(mass, idx, 1);
@@ -1366,17 +1401,120 @@ template <class T> struct Orc : OrcBase {
     }
   }
 
-  // ================= a18: marching cubes on the liquid level set =================
+  // ================= a4: node fields of the particle sources =================
+  // genWaterDensfield_liquidAndGas + sumcelldens_liquidAndGas spray_k.cu:1595-1686,
+  // genWaterDensfield_Gas + sumcelldens_Gas 1528-1593, genWaterDensfield_GY +
+  // sumcelldens3 1688-1772, on the (2N+1)^3 lattice of spacing h/2 from the 2N
+  // lists: per node, the 4x4x4 cells (di,dj,dk in [-2,1]) of the 2N grid; per
+  // cell the weighted positions summed from zero and added to the centre, the
+  // weights accumulated directly.  GY's centertmp is the host's zero vector.
+  void mc_density(int src, int MX, int MY, int MZ, T mh) {
+    const int X2 = 2 * NX, Y2 = 2 * NY, Z2 = 2 * NZ;
+    const int64_t MN = (int64_t)MX * MY * MZ;
+    mc_field.assign(MN, T(0));
+    const T R = src == MPF_MC_SOLID ? mh : mh * T(2);
+    const int airsolo = c.mc_gas_airsolo;
+#pragma omp parallel for schedule(static)
+    for (int64_t idx = 0; idx < MN; ++idx) {
+      int i, j, k;
+      ijk3(idx, MX, MY, MZ, i, j, k);
+      const T px = ((T)i + T(0)) * mh + (T)c.gmin[0], py = ((T)j + T(0)) * mh + (T)c.gmin[1],
+              pz = ((T)k + T(0)) * mh + (T)c.gmin[2];
+      T cx = 0, cy = 0, cz = 0, wsum = 0;
+      for (int di = -2; di <= 1; ++di)
+        for (int dj = -2; dj <= 1; ++dj)
+          for (int dk = -2; dk <= 1; ++dk) {
+            const int a = i + di, b = j + dj, q = k + dk;
+            if (a < 0 || a > X2 - 1 || b < 0 || b > Y2 - 1 || q < 0 || q > Z2 - 1) continue;
+            const int64_t cell = ((int64_t)a * Y2 + b) * Z2 + q;
+            if (mc_cstart[cell] == MPF_CELL_UNDEF) continue;
+            T rx = 0, ry = 0, rz = 0;
+            for (uint32_t p = mc_cstart[cell]; p < mc_cend[cell]; ++p) {
+              const int8_t fl = flag[p];
+              bool take;
+              if (src == MPF_MC_LIQUID_GAS) take = fl == AIR || fl == AIRSOLO || fl == FLUID;
+              else if (src == MPF_MC_GAS) take = fl == AIR || (fl == AIRSOLO && airsolo);
+              else take = fl == SOLID;
+              if (!take) continue;
+              const V3<T> x = P3(pos, p);
+              const T dx = x.x - px, dy = x.y - py, dz = x.z - pz;
+              const T dis = std::sqrt(dx * dx + dy * dy + dz * dz);
+              if (dis < R) {
+                T w;
+                if (src == MPF_MC_SOLID) {
+                  w = dis / (T(4) * mh);
+                  w = T(1) - w * w;
+                  w = tmax(w * w * w, T(0));
+                } else {
+                  w = R * R - dis * dis;
+                  w = w * w * w;
+                }
+                rx = rx + x.x * w;
+                ry = ry + x.y * w;
+                rz = rz + x.z * w;
+                wsum = wsum + w;
+              }
+            }
+            cx = cx + rx;
+            cy = cy + ry;
+            cz = cz + rz;
+          }
+      const T r = src == MPF_MC_LIQUID_GAS ? T(0.5f) * mh : (src == MPF_MC_GAS ? (T)0.8f * mh : (T)0.75f * mh);
+      T phi;
+      if (wsum > 0) {
+        cx = cx / wsum;
+        cy = cy / wsum;
+        cz = cz / wsum;
+        const T ex = px - cx, ey = py - cy, ez = pz - cz;
+        const T len = std::sqrt(ex * ex + ey * ey + ez * ez);
+        if (src == MPF_MC_SOLID) {
+          const T thigh = (T)0.51f, tlow = (T)0.49f;
+          const T Ev = tmax(cx, tmax(cy, cz)) / (T(4) * mh);
+          const T gamma = (thigh - Ev) / (thigh - tlow);
+          const T fr = (Ev < tlow) ? T(1) : gamma * gamma * gamma - T(3) * gamma * gamma + T(3) * gamma;
+          phi = len - r * fr;
+        } else {
+          phi = r - len;
+        }
+      } else {
+        phi = -r;
+      }
+      if (i == 0 || j == 0 || k == 0 || i == MX - 1 || j == MY - 1 || k == MZ - 1)
+        phi = src == MPF_MC_SOLID ? tmin(phi, (T)-10.0f) : (T)-1000.0f;
+      mc_field[idx] = phi;
+    }
+  }
+
+  // ================= a18: marching cubes =================
   // classifyVoxel spray_k.cu:1853-1888 (case index), scans + compactVoxels
   // 1891-1900, markActiveEdge_MC 2172-2220, generateTriangles_indices
-  // 2074-2170 (shared vertices indexed by the scan of active edges).  Nodes
-  // are cell centres of lsfluid; voxel (i,j,k) spans cells i..i+1 etc.
+  // 2074-2170 (shared vertices indexed by the scan of active edges, edge ids
+  // 3*node + slot in GetEdgeID's order 1920-1957: slot 0 +y, 1 +x, 2 +z).
+  // The lattice is the cell centres of lsfluid (MPF_MC_LEVELSET) or the
+  // (2N+1)^3 nodes of a particle source's field.  Each edge's vertex is
+  // interpolated once, from its start node p0 to p0 + h along the edge.
   int marching_cubes(mpf_mc_info* info) {
-    const int VX = NX - 1, VY = NY - 1, VZ = NZ - 1;
-    const int64_t V = (int64_t)VX * VY * VZ;
+    const int src = c.mc_source;
+    int MX, MY, MZ;
+    T mh, off;
+    const std::vector<T>* fp;
+    if (src == MPF_MC_LEVELSET) {
+      MX = NX; MY = NY; MZ = NZ; mh = h; off = T(0.5); fp = &lsfluid;
+    } else {
+      MX = 2 * NX + 1; MY = 2 * NY + 1; MZ = 2 * NZ + 1;
+      mh = (T)c.dx / T(2);
+      off = T(0);
+      if (mc_cstart.empty()) return MPF_INVALID_ARG;
+      mc_density(src, MX, MY, MZ, mh);
+      fp = &mc_field;
+    }
+    const std::vector<T>& fld = *fp;
+    const int VX = MX - 1, VY = MY - 1, VZ = MZ - 1;
+    const int64_t V = (int64_t)VX * VY * VZ, MN = (int64_t)MX * MY * MZ;
+    auto nid = [&](int i, int j, int k) { return ((int64_t)i * MY + j) * MZ + k; };
     mc_case.assign(V > 0 ? V : 0, 0);
     const T iso = 0;
-    auto F = [&](int i, int j, int k) { return lsfluid[cid(i, j, k)]; };
+    auto F = [&](int i, int j, int k) { return fld[nid(i, j, k)]; };
     int64_t active = 0, nverts_tri = 0;
     for (int64_t v = 0; v < V; ++v) {
       int i, j, k;
@@ -1388,11 +1526,10 @@ template <class T> struct Orc : OrcBase {
       mc_case[v] = (uint8_t)ci;
       if (MC_NUM_VERTS[ci] > 0) { active++; nverts_tri += MC_NUM_VERTS[ci]; }
     }
-    // Edge ids: node n = cid(i,j,k); 3*n + axis for the edge from node n to n+e_axis.
-    std::vector<uint8_t> used((size_t)3 * G, 0);
-    const int eo[12][4] = {  // (di,dj,dk,axis) of the cube edges
-        {0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
-        {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
+    std::vector<uint8_t> used((size_t)3 * MN, 0);
+    const int eo[12][4] = {  // (di,dj,dk,slot) of the cube edges
+        {0, 0, 0, 1}, {1, 0, 0, 0}, {0, 1, 0, 1}, {0, 0, 0, 0}, {0, 0, 1, 1}, {1, 0, 1, 0},
+        {0, 1, 1, 1}, {0, 0, 1, 0}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
     for (int64_t v = 0; v < V; ++v) {
       uint32_t ci = mc_case[v];
       if (!MC_NUM_VERTS[ci]) continue;
@@ -1400,28 +1537,30 @@ template <class T> struct Orc : OrcBase {
       ijk3(v, VX, VY, VZ, i, j, k);
       for (int t = 0; t < MC_NUM_VERTS[ci]; ++t) {
         int e = MC_TRI_TABLE[ci][t];
-        used[3 * cid(i + eo[e][0], j + eo[e][1], k + eo[e][2]) + eo[e][3]] = 1;
+        used[3 * nid(i + eo[e][0], j + eo[e][1], k + eo[e][2]) + eo[e][3]] = 1;
       }
     }
-    std::vector<uint32_t> vid((size_t)3 * G, 0);
+    std::vector<uint32_t> vid((size_t)3 * MN, 0);
     uint32_t nv = 0;
     for (size_t e = 0; e < used.size(); ++e) { vid[e] = nv; nv += used[e]; }
     if ((int64_t)nv > c.mc_max_verts || nverts_tri / 3 > c.mc_max_tris) return MPF_CAPACITY;
     mc_verts.assign((size_t)3 * nv, T(0));
     mc_tris.assign((size_t)nverts_tri, 0);
-    // vertex positions: vertexInterp spray_k.cu:1905-1910 along each active edge
-    for (int64_t n = 0; n < G; ++n)
-      for (int a = 0; a < 3; ++a) {
-        if (!used[3 * n + a]) continue;
+    // vertex positions: vertexInterp spray_k.cu:1903-1908 along each active edge
+    for (int64_t n = 0; n < MN; ++n)
+      for (int slot = 0; slot < 3; ++slot) {
+        if (!used[3 * n + slot]) continue;
+        const int a = slot == 1 ? 0 : (slot == 0 ? 1 : 2);
         int i, j, k;
-        ijk(n, i, j, k);
-        int i2 = i + (a == 0), j2 = j + (a == 1), k2 = k + (a == 2);
-        T f0 = F(i, j, k), f1 = F(i2, j2, k2);
-        T t = (iso - f0) / (f1 - f0);
-        V3<T> p0 = mk(((T)i + T(0.5)) * h, ((T)j + T(0.5)) * h, ((T)k + T(0.5)) * h) + gmin();
-        V3<T> p1 = mk(((T)i2 + T(0.5)) * h, ((T)j2 + T(0.5)) * h, ((T)k2 + T(0.5)) * h) + gmin();
-        V3<T> pv = p0 + (p1 - p0) * t;
-        S3(mc_verts, vid[3 * n + a], pv);
+        ijk3(n, MX, MY, MZ, i, j, k);
+        const int i2 = i + (a == 0), j2 = j + (a == 1), k2 = k + (a == 2);
+        const T f0 = F(i, j, k), f1 = F(i2, j2, k2);
+        const T t = (iso - f0) / (f1 - f0);
+        V3<T> p0 = mk(((T)i + off) * mh + (T)c.gmin[0], ((T)j + off) * mh + (T)c.gmin[1],
+                      ((T)k + off) * mh + (T)c.gmin[2]);
+        V3<T> p1 = mk(a == 0 ? p0.x + mh : p0.x, a == 1 ? p0.y + mh : p0.y, a == 2 ? p0.z + mh : p0.z);
+        V3<T> pv = mk(p0.x + t * (p1.x - p0.x), p0.y + t * (p1.y - p0.y), p0.z + t * (p1.z - p0.z));
+        S3(mc_verts, vid[3 * n + slot], pv);
       }
     int64_t w = 0;
     for (int64_t v = 0; v < V; ++v) {
@@ -1431,7 +1570,7 @@ template <class T> struct Orc : OrcBase {
       ijk3(v, VX, VY, VZ, i, j, k);
       for (int t = 0; t < MC_NUM_VERTS[ci]; ++t) {
         int e = MC_TRI_TABLE[ci][t];
-        mc_tris[w++] = vid[3 * cid(i + eo[e][0], j + eo[e][1], k + eo[e][2]) + eo[e][3]];
+        mc_tris[w++] = vid[3 * nid(i + eo[e][0], j + eo[e][1], k + eo[e][2]) + eo[e][3]];
       }
     }
     if (info) { info->active_voxels = active; info->verts = nv; info->tris = nverts_tri / 3; }
@@ -1458,7 +1597,9 @@ template <class T> struct Orc : OrcBase {
     if (c.heat_enabled) heat(&loc.heat);
     int st2 = phase_change_with_delete(frame, &loc.phase);
     if (st2 != MPF_OK) return st2;
-    if (c.mc_enabled) {
+    if (c.mc_enabled && frame % c.mc_every == 0) {
+      // bubblesim's rendering stage (spray.cpp:1192-1201): preMC, then the mesh
+      if (c.mc_source != MPF_MC_LEVELSET) mc_hash_sort();
       int st3 = marching_cubes(&loc.mc);
       if (st3 != MPF_OK) return st3;
     }
@@ -1666,6 +1807,20 @@ int orc_get_mesh(orc_state* s, double* verts, uint32_t* tris) {
   return dispatch(s, [&](auto& o) {
     for (size_t i = 0; i < o.mc_verts.size(); ++i) verts[i] = (double)o.mc_verts[i];
     std::memcpy(tris, o.mc_tris.data(), o.mc_tris.size() * 4);
+    return 0;
+  });
+}
+int orc_mc_hash_sort(orc_state* s) { return dispatch(s, [&](auto& o) { o.mc_hash_sort(); return 0; }); }
+int orc_get_mc_field(orc_state* s, double* out) {
+  return dispatch(s, [&](auto& o) {
+    for (size_t i = 0; i < o.mc_field.size(); ++i) out[i] = (double)o.mc_field[i];
+    return 0;
+  });
+}
+int orc_get_mc_cells(orc_state* s, uint32_t* start, uint32_t* end) {
+  return dispatch(s, [&](auto& o) {
+    if (start) std::memcpy(start, o.mc_cstart.data(), o.mc_cstart.size() * 4);
+    if (end) std::memcpy(end, o.mc_cend.data(), o.mc_cend.size() * 4);
     return 0;
   });
 }

@@ -57,6 +57,12 @@ int64_t orc_pour(orc_state* s, int frame);
 int64_t orc_delete_particles(orc_state* s, int keep_lists);
 
 int orc_hash_sort(orc_state* s);
+/* preMC: the sort on the 2N grid with its own cell lists (2nx*2ny*2nz). */
+int orc_mc_hash_sort(orc_state* s);
+/* Node field of the last particle-source mesh ((2nx+1)(2ny+1)(2nz+1) doubles)
+   and the 2N cell lists (either pointer may be NULL). */
+int orc_get_mc_field(orc_state* s, double* out);
+int orc_get_mc_cells(orc_state* s, uint32_t* start, uint32_t* end);
 int orc_apply_forces(orc_state* s);
 int orc_levelset_update(orc_state* s);
 int orc_p2g(orc_state* s);
