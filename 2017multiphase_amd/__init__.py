@@ -43,7 +43,7 @@ FIELDS = [
     "CELL_START", "CELL_END",
     "WATER_UX", "WATER_UY", "WATER_UZ", "AIR_UX", "AIR_UY", "AIR_UZ",
     "WATER_UX_OLD", "WATER_UY_OLD", "WATER_UZ_OLD", "AIR_UX_OLD", "AIR_UY_OLD", "AIR_UZ_OLD",
-    "SOLID_UX", "SOLID_UY", "SOLID_UZ", "GRID_HASH", "GRID_INDEX",
+    "SOLID_UX", "SOLID_UY", "SOLID_UZ", "GRID_HASH", "GRID_INDEX", "SURFACE_TENSION",
 ]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 
@@ -58,6 +58,8 @@ ABI_FUNCTIONS = [
     "mpf_project_adjoint", "mpf_extrapolate_adjoint", "mpf_p2g_adjoint", "mpf_g2p_adjoint",
     "mpf_advect_adjoint", "mpf_hash_sort_noalloc", "mpf_update_seed_cells", "mpf_set_pour_source",
     "mpf_delete_particles", "mpf_pour", "mpf_mc_hash_sort", "mpf_get_mc_field", "mpf_get_mc_cells",
+    "mpf_mark_solo_air", "mpf_surface_tension", "mpf_drag_force", "mpf_update_solo_air",
+    "mpf_correct_positions", "mpf_correct_bubble_positions",
 ]
 
 
@@ -97,6 +99,12 @@ class Config(ctypes.Structure):
         ("pour_every", ctypes.c_int32), ("pour_pos_rand", ctypes.c_float), ("pour_vel_rand", ctypes.c_float),
         ("pour_temperature", ctypes.c_float), ("seed_update_every", ctypes.c_int32),
         ("mc_source", ctypes.c_int32), ("mc_gas_airsolo", ctypes.c_int32), ("mc_every", ctypes.c_int32),
+        ("solo_air_enabled", ctypes.c_int32), ("solo_tol_in", ctypes.c_float), ("solo_tol_out", ctypes.c_float),
+        ("sph_viscosity", ctypes.c_float), ("surface_tension_sigma", ctypes.c_float),
+        ("drag_enabled", ctypes.c_int32), ("drag_solo", ctypes.c_float), ("drag_grid", ctypes.c_float),
+        ("drag_particle_update", ctypes.c_int32), ("correct_position", ctypes.c_int32),
+        ("correction_spring", ctypes.c_float), ("correction_radius", ctypes.c_float),
+        ("correct_bubble_air_ls", ctypes.c_int32),
     ]
 
 
@@ -160,7 +168,9 @@ def lib():
     L.mpf_get_field.argtypes = [vp, i32, vp, i32]
     L.mpf_set_field.argtypes = [vp, i32, vp, i32]
     L.mpf_set_seed_cells.argtypes = [vp, vp, i32]
-    for n in ("mpf_hash_sort", "mpf_apply_forces", "mpf_levelset_update", "mpf_p2g", "mpf_g2p", "mpf_advect"):
+    for n in ("mpf_hash_sort", "mpf_apply_forces", "mpf_levelset_update", "mpf_p2g", "mpf_g2p", "mpf_advect",
+              "mpf_mark_solo_air", "mpf_surface_tension", "mpf_drag_force", "mpf_update_solo_air",
+              "mpf_correct_positions", "mpf_correct_bubble_positions"):
         getattr(L, n).argtypes = [vp]
     L.mpf_extrapolate_bc.argtypes = [vp, i32]
     L.mpf_project.argtypes = [vp, c.POINTER(SolveInfo)]
@@ -376,6 +386,24 @@ class Simulator:
 
     def advect(self):
         self._check(self.L.mpf_advect(self.h))
+
+    def mark_solo_air(self):
+        self._check(self.L.mpf_mark_solo_air(self.h))
+
+    def surface_tension(self):
+        self._check(self.L.mpf_surface_tension(self.h))
+
+    def drag_force(self):
+        self._check(self.L.mpf_drag_force(self.h))
+
+    def update_solo_air(self):
+        self._check(self.L.mpf_update_solo_air(self.h))
+
+    def correct_positions(self):
+        self._check(self.L.mpf_correct_positions(self.h))
+
+    def correct_bubble_positions(self):
+        self._check(self.L.mpf_correct_bubble_positions(self.h))
 
     def heat(self):
         info = SolveInfo()

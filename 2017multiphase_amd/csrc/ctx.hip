@@ -116,7 +116,7 @@ void free_all(Ctx& c) {
   dfree(c.count); dfree(c.cstart); dfree(c.cend); dfree(c.cmask); dfree(c.bmask); dfree(c.sflag); dfree(c.mark);
   dfree(c.lsfluid); dfree(c.lsair); dfree(c.lsmerge);
   for (int a = 0; a < 3; ++a) dfree(c.phig[a]);
-  dfree(c.phifluid); dfree(c.phiair); dfree(c.phisolid); dfree(c.ctmp);
+  dfree(c.phifluid); dfree(c.phiair); dfree(c.phisolid); dfree(c.ctmp); dfree(c.sf);
   dfree(c.div); dfree(c.press); dfree(c.tp); dfree(c.tpchange); dfree(c.tsave);
   dfree(c.vx); dfree(c.vr); dfree(c.vp); dfree(c.vq); dfree(c.vp2); dfree(c.vr2); dfree(c.vq2); dfree(c.nbm); dfree(c.dslow);
   dfree(c.partials); dfree(c.scal); dfree(c.counters);
@@ -170,7 +170,8 @@ bool alloc_all(Ctx& c) {
        dalloc(c, c.bmask, 2 * nbricks(c.g)) &&
        dalloc(c, c.sflag, 2 * sweep_tiles(c.g) + face_tiles(c.g, 0) + face_tiles(c.g, 1) + face_tiles(c.g, 2));
   for (int a = 0; a < 3; ++a) ok = ok && dalloc(c, c.phig[a], G);
-  ok = ok && dalloc(c, c.phifluid, G) && dalloc(c, c.phiair, G) && dalloc(c, c.phisolid, G) && dalloc(c, c.ctmp, G);
+  ok = ok && dalloc(c, c.phifluid, G) && dalloc(c, c.phiair, G) && dalloc(c, c.phisolid, G) && dalloc(c, c.ctmp, G) &&
+       dalloc(c, c.sf, G);
   double* dd = nullptr;
   ok = ok && dalloc(c, dd, G);
   c.div = dd;
@@ -221,7 +222,7 @@ bool alloc_all(Ctx& c) {
   hipMemsetAsync(c.cstart, 0xff, (size_t)G * 4, c.stream);
   hipMemsetAsync(c.cend, 0xff, (size_t)G * 4, c.stream);
   for (float* f : {c.lsfluid, c.lsair, c.lsmerge, c.phig[0], c.phig[1], c.phig[2], c.phifluid, c.phiair, c.phisolid,
-                   c.tp, c.tpchange, c.tsave})
+                   c.tp, c.tpchange, c.tsave, c.sf})
     hipMemsetAsync(f, 0, (size_t)G * 4, c.stream);
   hipMemsetAsync(c.div, 0, (size_t)G * 8, c.stream);
   hipMemsetAsync(c.press, 0, (size_t)G * 8, c.stream);
@@ -265,6 +266,7 @@ bool field_ref(Ctx& c, int f, void** ptr, int64_t* n, int* es) {
     case MPF_F_CELL_END: *ptr = c.cend; return true;
     case MPF_F_GRID_HASH: *ptr = c.ghash; *n = c.P; return true;
     case MPF_F_GRID_INDEX: *ptr = c.perm; *n = c.P; return true;
+    case MPF_F_SURFACE_TENSION: *ptr = c.sf; return true;
     default: break;
   }
   float** groups[5] = {c.wu, c.au, c.wuo, c.auo, c.su};
@@ -357,6 +359,19 @@ mpf_status mpf_config_default(mpf_config* cfg, int32_t nx, int32_t ny, int32_t n
   cfg->mc_source = MPF_MC_LEVELSET;
   cfg->mc_gas_airsolo = 1;
   cfg->mc_every = 1;
+  cfg->solo_air_enabled = 1;
+  cfg->solo_tol_in = -1.45f;                 // spray_k.cu:4245
+  cfg->solo_tol_out = -0.5f;
+  cfg->sph_viscosity = 1.8f;                 // spray.cpp:40
+  cfg->surface_tension_sigma = 1.0f;         // spray.cpp:34
+  cfg->drag_enabled = 1;
+  cfg->drag_solo = 10.0f;                    // spray.cpp:39
+  cfg->drag_grid = 0.05f;
+  cfg->drag_particle_update = 0;
+  cfg->correct_position = 1;                 // spray.cpp:50
+  cfg->correction_spring = 500000.0f;        // spray.cpp:617-618
+  cfg->correction_radius = 0.5f;
+  cfg->correct_bubble_air_ls = 0;
   return MPF_OK;
 }
 
@@ -680,6 +695,58 @@ mpf_status mpf_apply_forces(mpf_ctx* ctx) {
 #define REQUIRE_SORTED(c)                                                                  \
   if (!(c).sorted) return fail((c), MPF_INVALID_ARG, "particles not sorted: call mpf_hash_sort first");
 
+mpf_status mpf_mark_solo_air(mpf_ctx* ctx) {
+  if (!ctx) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  REQUIRE_SORTED(c);
+  run_mark_solo_air(c);
+  return check_launch(c);
+}
+
+mpf_status mpf_surface_tension(mpf_ctx* ctx) {
+  if (!ctx) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  run_surface_tension(c);
+  return check_launch(c);
+}
+
+mpf_status mpf_drag_force(mpf_ctx* ctx) {
+  if (!ctx) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  REQUIRE_SORTED(c);
+  run_drag(c);
+  return check_launch(c);
+}
+
+mpf_status mpf_update_solo_air(mpf_ctx* ctx) {
+  if (!ctx) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  REQUIRE_SORTED(c);
+  run_solo_air_sph(c);
+  return check_launch(c);
+}
+
+mpf_status mpf_correct_positions(mpf_ctx* ctx) {
+  if (!ctx) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  REQUIRE_SORTED(c);
+  run_correct_pos(c);
+  return check_launch(c);
+}
+
+mpf_status mpf_correct_bubble_positions(mpf_ctx* ctx) {
+  if (!ctx) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  run_correct_bubble_pos(c);
+  return check_launch(c);
+}
+
 mpf_status mpf_levelset_update(mpf_ctx* ctx) {
   if (!ctx) return MPF_INVALID_ARG;
   Ctx& c = ctx->c;
@@ -899,16 +966,31 @@ mpf_status mpf_step(mpf_ctx* ctx, int32_t frame, mpf_step_info* info) {
   if ((st = mpf_hash_sort(ctx)) != MPF_OK) return st;
   if ((st = mpf_apply_forces(ctx)) != MPF_OK) return st;
   if ((st = mpf_levelset_update(ctx)) != MPF_OK) return st;
+  const mpf_config& k = c.cfg;
+  if (k.solo_air_enabled && (st = mpf_mark_solo_air(ctx)) != MPF_OK) return st;
+  if (k.surface_tension_sigma != 0.0f && (st = mpf_surface_tension(ctx)) != MPF_OK) return st;
   if ((st = mpf_p2g(ctx)) != MPF_OK) return st;
+  if (k.drag_enabled && (st = mpf_drag_force(ctx)) != MPF_OK) return st;
   if ((st = mpf_extrapolate_bc(ctx, 1)) != MPF_OK) return st;
   mpf_status pst = mpf_project(ctx, &loc.project);
   if (pst != MPF_OK && pst != MPF_NO_CONVERGENCE) return pst;
   const std::string perr = c.err;
   if ((st = mpf_extrapolate_bc(ctx, 0)) != MPF_OK) return st;
   if ((st = mpf_g2p(ctx)) != MPF_OK) return st;
+  if (k.solo_air_enabled && (st = mpf_update_solo_air(ctx)) != MPF_OK) return st;
   if ((st = mpf_advect(ctx)) != MPF_OK) return st;
   if ((st = mpf_hash_sort(ctx)) != MPF_OK) return st;
-  if ((st = mpf_levelset_update(ctx)) != MPF_OK) return st;
+  if (k.correct_position) {
+    // spray.cpp:1130-1142: spring correction, level set of the corrected
+    // particles, bubble-side correction, and the sort for the heat stage
+    if ((st = mpf_correct_positions(ctx)) != MPF_OK) return st;
+    if ((st = mpf_hash_sort(ctx)) != MPF_OK) return st;
+    if ((st = mpf_levelset_update(ctx)) != MPF_OK) return st;
+    if ((st = mpf_correct_bubble_positions(ctx)) != MPF_OK) return st;
+    if ((st = mpf_hash_sort(ctx)) != MPF_OK) return st;
+  } else {
+    if ((st = mpf_levelset_update(ctx)) != MPF_OK) return st;
+  }
   if (c.cfg.heat_enabled && (st = mpf_heat(ctx, &loc.heat)) != MPF_OK) return st;
   {
     // nucleation, deletion, melting and freezing in the reference's order

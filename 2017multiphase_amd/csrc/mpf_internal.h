@@ -166,6 +166,7 @@ struct Ctx {
   float *phig[3] = {};
   float *phifluid = nullptr, *phiair = nullptr, *phisolid = nullptr;
   float* ctmp = nullptr;  // scratch cell field for two-buffer sweeps
+  float* sf = nullptr;    // surface tension force per cell
   uint8_t* sflag = nullptr;  // per-tile change flags: 2 x sweep_tiles (eikonal passes), then face_tiles of
                              // each axis (first extrapolation pass)
   void* div = nullptr;    // float, or double under MPF_FP64
@@ -283,6 +284,14 @@ void run_heat_grid_post(Ctx& c);
 void run_heat_jacobi(Ctx& c, int iters);
 int run_marching_cubes(Ctx& c, mpf_mc_info* info);
 void run_hash_sort_mc(Ctx& c);
+// bubble.hip
+void run_mark_solo_air(Ctx& c);
+void run_surface_tension(Ctx& c);
+void run_drag(Ctx& c);
+void run_solo_air_sph(Ctx& c);
+void run_correct_pos(Ctx& c);
+void run_correct_bubble_pos(Ctx& c);
+void sph_constants(float h, float& poly6, float& spiky, float& lap);
 void run_solid_phi_and_velocity(Ctx& c);
 int64_t p2g_parts_floats(const Grid& g);
 // Device counters: 64 single counters, then SPREAD counters 64 bytes apart

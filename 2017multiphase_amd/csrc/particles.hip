@@ -676,6 +676,7 @@ int run_phase_change_with_delete(Ctx& c, int frame, mpf_phase_info* info, int64_
   run_melt_freeze(c, Pn);
   hipMemcpyAsync(c.counters_host + 1, c.counters + 1, 7 * 4, hipMemcpyDeviceToHost, c.stream);
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "phase_change")) return MPF_HIP_ERROR;
+  if (c.cfg.freeze_enabled && c.counters_host[7]) c.own_sort = c.mc_sorted = false;  // a frozen particle snapped into another cell
   if (info) {
     info->melted = (int32_t)c.counters_host[1];
     info->frozen = (int32_t)c.counters_host[2];

@@ -183,6 +183,20 @@ typedef struct mpf_config {
                                 reference's SCENE_INTERACTION uses 0)         */
   int32_t mc_every;          /* mpf_step meshes frames that are multiples of
                                 this (1; the reference's outputframeDelta is 2) */
+  /* remaining bubblesim stages (spray.cpp:1059-1150), reference defaults */
+  int32_t solo_air_enabled;  /* markSoloAirParticle + updateSoloAirParticle (1) */
+  float solo_tol_in, solo_tol_out; /* verifySoloAirParticle tol1, tol2 in cells
+                                (-1.45, -0.5; MELTANDBOIL 0.05, -0.8)         */
+  float sph_viscosity;       /* viscosiySPH (1.8)                             */
+  float surface_tension_sigma; /* surfacetensionsigma (1.0; 0 skips the stage) */
+  int32_t drag_enabled;      /* enforceDragForce (1)                          */
+  float drag_solo, drag_grid; /* dragParamSolo, dragParamGrid (10, 0.05)      */
+  int32_t drag_particle_update; /* the drag also accelerates the solo air
+                                particle (SCENE_INTERACTION; 0)               */
+  int32_t correct_position;  /* m_bCorrectPosition (1)                        */
+  float correction_spring, correction_radius; /* (500000, 0.5 cells)          */
+  int32_t correct_bubble_air_ls; /* correctbubblepos_air (the MELTANDBOIL,
+                                INTERACTION and ALL scenes; 0)                */
 } mpf_config;
 
 /* Opaque context. */
@@ -207,6 +221,7 @@ typedef enum {
   MPF_F_SOLID_UX, MPF_F_SOLID_UY, MPF_F_SOLID_UZ,
   /* particle fields (P = current count) */
   MPF_F_GRID_HASH, MPF_F_GRID_INDEX, /* uint32, from the last mpf_hash_sort  */
+  MPF_F_SURFACE_TENSION,             /* cell float, compsurfacetension_k      */
   MPF_F_COUNT
 } mpf_field;
 
@@ -351,10 +366,44 @@ mpf_status mpf_get_mc_field(mpf_ctx* ctx, float* field, int32_t where);
    pointer may be NULL). */
 mpf_status mpf_get_mc_cells(mpf_ctx* ctx, uint32_t* cell_start, uint32_t* cell_end, int32_t where);
 
+/* ---- the remaining bubblesim stages ------------------------------------ */
+/* cspray::markSoloAirParticle, spray.cu:2614-2618 (verifySoloAirParticle,
+   spray_k.cu:4223-4261): AIR particles with few air neighbours, deep in
+   the liquid or near the x/y walls become AIRSOLO (velocity from the air
+   grid); AIRSOLO particles with many air neighbours or near the air become
+   AIR.  Needs the cell lists and mpf_levelset_update's LSMERGE. */
+mpf_status mpf_mark_solo_air(mpf_ctx* ctx);
+/* cspray::computesurfacetension, spray.cu:2001-2009: the curvature force
+   field sigma * div(n) (compsurfacetension_k, spray_k.cu:3543-3637) and its
+   velocity change on AIR particles within one cell of the interface
+   (enforcesurfacetension_p, 3639-3680). */
+mpf_status mpf_surface_tension(mpf_ctx* ctx);
+/* cspray::enforceDragForce, spray.cu:2953-2957 (calDragForce, spray_k.cu:
+   5127-5165): each AIRSOLO particle's drag on the liquid faces of its cell,
+   added in particle order (the reference adds with atomics).  After mpf_p2g. */
+mpf_status mpf_drag_force(mpf_ctx* ctx);
+/* cspray::updateSoloAirParticle, spray.cu:2620-2651: SPH density, pressure
+   and viscosity of AIR/AIRSOLO particles (calcDensPress_Air,
+   enforceForceSoloAirP, spray_k.cu:4284-4373); neighbour velocities are
+   the pass's input (the reference reads them while they are rewritten). */
+mpf_status mpf_update_solo_air(mpf_ctx* ctx);
+/* cspray::correctpos, spray.cu:328-339 (correctparticlepos, spray_k.cu:
+   1357-1411): spring separation of FLUID/AIR particles, clamped to the
+   domain.  Leaves the particles unsorted. */
+mpf_status mpf_correct_positions(mpf_ctx* ctx);
+/* cspray::correctpos_bubble, spray.cu:2067-2098 (correctbubblepos,
+   spray_k.cu:3884-3931, or correctbubblepos_air, 3934-3991, when
+   cfg.correct_bubble_air_ls): particles on the wrong side of the merged
+   level set are pushed across it.  Uses mpf_levelset_update's fields. */
+mpf_status mpf_correct_bubble_positions(mpf_ctx* ctx);
+
 /* One full step in bubblesim order (spray.cpp:979-1233), the stages above
-   only: seed refresh, pour, sort, forces, level set, P2G, extrapolation,
-   projection, extrapolation, G2P, advection, sort, level set, heat,
-   nucleation, deletion (cfg.delete_enabled), melting/freezing, and on frames
+   only: seed refresh, pour, sort, forces, level set, solo-air marking,
+   surface tension, P2G, drag, extrapolation, projection, extrapolation,
+   G2P, solo-air SPH, advection, then either (cfg.correct_position) sort,
+   position correction, sort, level set, bubble position correction, sort,
+   or sort and level set; heat, nucleation, deletion (cfg.delete_enabled),
+   melting/freezing, and on frames
    that are multiples of cfg.mc_every marching cubes (after mpf_mc_hash_sort
    for a particle source, as bubblesim calls preMC). */
 mpf_status mpf_step(mpf_ctx* ctx, int32_t frame, mpf_step_info* info);

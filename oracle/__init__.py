@@ -66,6 +66,9 @@ def lib():
     L.orc_get_mesh.argtypes = [vp, vp, vp]
     L.orc_get_mc_cases.argtypes = [vp, vp]
     L.orc_mc_hash_sort.argtypes = [vp]
+    for n in ("mark_solo_air", "surface_tension", "drag_force", "update_solo_air", "correct_positions",
+              "correct_bubble_positions"):
+        getattr(L, "orc_" + n).argtypes = [vp]
     L.orc_get_mc_field.argtypes = [vp, vp]
     L.orc_get_mc_cells.argtypes = [vp, vp, vp]
     L.orc_step.argtypes = [vp, c.c_int, c.POINTER(pkg.StepInfo)]
@@ -216,6 +219,24 @@ class Oracle:
 
     def advect(self):
         self._ok(self.L.orc_advect(self.h))
+
+    def mark_solo_air(self):
+        self._ok(self.L.orc_mark_solo_air(self.h))
+
+    def surface_tension(self):
+        self._ok(self.L.orc_surface_tension(self.h))
+
+    def drag_force(self):
+        self._ok(self.L.orc_drag_force(self.h))
+
+    def update_solo_air(self):
+        self._ok(self.L.orc_update_solo_air(self.h))
+
+    def correct_positions(self):
+        self._ok(self.L.orc_correct_positions(self.h))
+
+    def correct_bubble_positions(self):
+        self._ok(self.L.orc_correct_bubble_positions(self.h))
 
     def heat(self):
         info = self.pkg.SolveInfo()

@@ -105,7 +105,7 @@ template <class T> struct Orc : OrcBase {
   // cell fields
   std::vector<int8_t> mark;
   std::vector<T> lsfluid, lsair, lsmerge, phigx, phigy, phigz, phifluid, phiair, phisolid, div, press,
-      tp, tpchange;
+      tp, tpchange, sf;
   // face fields
   std::vector<T> wu[3], au[3], wuo[3], auo[3], su[3];
   std::vector<int32_t> seeds;
@@ -127,7 +127,7 @@ template <class T> struct Orc : OrcBase {
     h = (T)c.dx;
     mark.assign(G, VACUUM);
     for (auto* v : {&lsfluid, &lsair, &lsmerge, &phigx, &phigy, &phigz, &phifluid, &phiair, &phisolid,
-                    &div, &press, &tp, &tpchange})
+                    &div, &press, &tp, &tpchange, &sf})
       v->assign(G, T(0));
     cstart.assign(G, MPF_CELL_UNDEF);
     cend.assign(G, MPF_CELL_UNDEF);
@@ -1401,6 +1401,318 @@ template <class T> struct Orc : OrcBase {
     }
   }
 
+  // ================= f2: the remaining bubblesim stages =================
+  inline void cell_of(V3<T> x, int& i, int& j, int& k) const { ijk_of_pos(x, i, j, k); }
+  V3<T> vec_from_grid(V3<T> x, const std::vector<T>& gx, const std::vector<T>& gy, const std::vector<T>& gz) const {
+    return mk(scalar_from_grid(x, gx), scalar_from_grid(x, gy), scalar_from_grid(x, gz));
+  }
+  // the particles of the cells around (i,j,k) within `lv`, cell by cell
+  template <class F> void for_cells(int i, int j, int k, int lv, F f) const {
+    for (int di = -lv; di <= lv; ++di)
+      for (int dj = -lv; dj <= lv; ++dj)
+        for (int dk = -lv; dk <= lv; ++dk) {
+          if (!valid(i + di, j + dj, k + dk)) continue;
+          f(cid(i + di, j + dj, k + dk));
+        }
+  }
+
+  // markSoloAirParticle: verifySoloAirParticle + cntairparticle + isInBoundaryCell,
+  // spray_k.cu:4195-4261 (flags flip only between AIR and AIRSOLO, both counted)
+  void mark_solo_air() {
+    const T tol1 = (T)c.solo_tol_in, tol2 = (T)c.solo_tol_out;
+    const std::vector<int8_t> f0(flag);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < P; ++p) {
+      const int8_t fl = f0[p];
+      if (fl == FLUID || fl == SOLID) continue;
+      const V3<T> x = P3(pos, p);
+      const T ls = scalar_from_grid(x, lsmerge);
+      int i, j, k;
+      cell_of(x, i, j, k);
+      int cnt = 0;
+      for_cells(i, j, k, 1, [&](int64_t cc) {
+        if (cstart[cc] == MPF_CELL_UNDEF) return;
+        for (uint32_t q = cstart[cc]; q < cend[cc]; ++q) {
+          const T d = len3(P3(pos, q) - x);
+          if (d < h && (f0[q] == AIR || f0[q] == AIRSOLO)) ++cnt;
+        }
+      });
+      const bool band = i <= 2 || i >= NX - 3 || j <= 2 || j >= NY - 3;
+      if ((cnt >= 10 || ls > tol1 * h) && fl == AIRSOLO && !band) {
+        if (cnt >= 3) flag[p] = AIR;
+      } else if (fl == AIR && (band || ls < tol2 * h || cnt <= 1)) {
+        S3(vel, p, vel_from_grid(x, au));
+        flag[p] = AIRSOLO;
+      }
+    }
+  }
+
+  // computesurfacetension: compsurfacetension_k + enforcesurfacetension_p,
+  // spray_k.cu:3543-3680 (only AIR particles pass the scene test)
+  void surface_tension() {
+    const T sigma = (T)c.surface_tension_sigma;
+#pragma omp parallel for schedule(static)
+    for (int64_t idx = 0; idx < G; ++idx) {
+      if (mark[idx] == BOUNDARY) { sf[idx] = 0; continue; }
+      int i, j, k;
+      ijk(idx, i, j, k);
+      V3<T> cen = mk(phigx[idx], phigy[idx], phigz[idx]);
+      T len = len3(cen), res;
+      if (len == 0) {
+        res = 0;
+      } else {
+        cen = mk(cen.x / len, cen.y / len, cen.z / len);
+        auto comp = [&](int a, int b, int q, int axis, T own) {
+          if (!valid(a, b, q)) return own;
+          const int64_t n = cid(a, b, q);
+          const T l = len3(mk(phigx[n], phigy[n], phigz[n]));
+          if (l == 0) return own;
+          return (axis == 0 ? phigx[n] : (axis == 1 ? phigy[n] : phigz[n])) / l;
+        };
+        const T x1 = comp(i + 1, j, k, 0, cen.x), x0 = comp(i - 1, j, k, 0, cen.x);
+        const T y1 = comp(i, j + 1, k, 1, cen.y), y0 = comp(i, j - 1, k, 1, cen.y);
+        const T z1 = comp(i, j, k + 1, 2, cen.z), z0 = comp(i, j, k - 1, 2, cen.z);
+        res = (x1 - x0 + y1 - y0 + z1 - z0) / h * (T)0.5f;
+      }
+      sf[idx] = res * sigma;
+    }
+    const T dt = (T)c.dt;
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < P; ++p) {
+      const int8_t fl = flag[p];
+      if (fl == SOLID || fl == FLUID || fl == AIRSOLO) continue;
+      const V3<T> x = P3(pos, p);
+      T ils = scalar_from_grid(x, lsmerge);
+      const T isf = scalar_from_grid(x, sf);
+      V3<T> dir = vec_from_grid(x, phigx, phigy, phigz);
+      const T lendir = len3(dir);
+      if (lendir == 0) continue;
+      dir = mk(dir.x / lendir, dir.y / lendir, dir.z / lendir);
+      ils = ils / lendir;
+      int i, j, k;
+      cell_of(x, i, j, k);
+      int cnt = mark[cid(i, j, k)] == AIR ? 1 : 0;
+      for (int di = -1; di <= 1; di += 2)
+        for (int dj = -1; dj <= 1; dj += 2)
+          for (int dk = -1; dk <= 1; dk += 2)
+            if (valid(i + di, j + dj, k + dk) && mark[cid(i + di, j + dj, k + dk)] == AIR) cnt++;
+      if (cnt == 0) continue;
+      if (std::fabs(ils) < h) {
+        const V3<T> f = mk(-isf * dir.x, -isf * dir.y, -isf * dir.z);
+        S3(vel, p, P3(vel, p) + f * dt);
+      }
+    }
+  }
+
+  // enforceDragForce: calDragForce, spray_k.cu:5127-5165: every particle
+  // reads the liquid faces as they were before the stage, and the face
+  // atomics are taken in particle order
+  void drag() {
+    const T dt = (T)c.dt, solo = (T)c.drag_solo, grid = (T)c.drag_grid;
+    const std::vector<T> w0[3] = {wu[0], wu[1], wu[2]};
+    for (int64_t p = 0; p < P; ++p) {
+      if (flag[p] != AIRSOLO) continue;
+      const V3<T> x = P3(pos, p), v = P3(vel, p);
+      int i, j, k;
+      cell_of(x, i, j, k);
+      const V3<T> gv = vel_from_grid(x, w0);
+      const V3<T> dv = gv - v;
+      const T s = solo * len3(dv);
+      const V3<T> fb = mk(s * dv.x, s * dv.y, s * dv.z);
+      const V3<T> ch = mk(-fb.x, -fb.y, -fb.z) * grid * dt;
+      const V3<T> w = mk(x.x / h - (T)i, x.y / h - (T)j, x.z / h - (T)k);
+      wu[0][fid(0, i, j, k)] += ch.x * (T(1) - w.x);
+      wu[0][fid(0, i + 1, j, k)] += ch.x * w.x;
+      wu[1][fid(1, i, j, k)] += ch.y * (T(1) - w.y);
+      wu[1][fid(1, i, j + 1, k)] += ch.y * w.y;
+      wu[2][fid(2, i, j, k)] += ch.z * (T(1) - w.z);
+      wu[2][fid(2, i, j, k + 1)] += ch.z * w.z;
+      if (c.drag_particle_update) S3(vel, p, v + fb * dt);
+    }
+  }
+
+  // updateSoloAirParticle: calcDensPress_Air + enforceForceSoloAirP,
+  // spray_k.cu:4264-4373; (h2-d2)^3 as a product, neighbour velocities of
+  // the pass's input
+  void solo_air_sph() {
+    const float hf = (float)h;
+    const float poly6 = 315.0f / (64.0f * 3.141592f * powf(hf, 9.0f));
+    const float spiky = -45.0f / (3.141592f * powf(hf, 6.0f));
+    const float lap = 45.0f / (3.141592f * powf(hf, 6.0f));
+    const T m0poly6 = (T)(c.airm0 * poly6), vterm = (T)(lap * c.sph_viscosity), sp = (T)spiky;
+    const T m0 = (T)c.airm0, dt = (T)c.dt, rho = (T)c.rho_liquid;
+    std::vector<T> dens(P, T(0)), press(P, T(0));
+    const T h2 = h * h;
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < P; ++p) {
+      if (flag[p] != AIR && flag[p] != AIRSOLO) continue;
+      const V3<T> x = P3(pos, p);
+      int i, j, k;
+      cell_of(x, i, j, k);
+      T d = 0;
+      for_cells(i, j, k, 1, [&](int64_t cc) {
+        T res = 0;
+        if (cstart[cc] != MPF_CELL_UNDEF)
+          for (uint32_t q = cstart[cc]; q < cend[cc]; ++q) {
+            if (flag[q] != AIR && flag[q] != AIRSOLO) continue;
+            const V3<T> dd = P3(pos, q) - x;
+            const T d2 = dot3(dd, dd);
+            if (d2 < h2) { const T t = h2 - d2; res = res + t * t * t; }
+          }
+        d = d + res;
+      });
+      d = d * m0poly6;
+      if (d == 0) d = T(1);
+      dens[p] = T(1) / d;
+      press[p] = (T)1.5f * (d - rho * (T)0.5f);
+    }
+    std::vector<T> vout(vel);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < P; ++p) {
+      if (flag[p] != AIRSOLO && flag[p] != AIR) continue;
+      const V3<T> x = P3(pos, p);
+      V3<T> iv = P3(vel, p);
+      const T ip = press[p], id = dens[p];
+      int i, j, k;
+      cell_of(x, i, j, k);
+      V3<T> force = mk(T(0), T(0), T(0));
+      for_cells(i, j, k, 1, [&](int64_t cc) {
+        V3<T> res = mk(T(0), T(0), T(0));
+        if (cstart[cc] != MPF_CELL_UNDEF)
+          for (uint32_t q = cstart[cc]; q < cend[cc]; ++q) {
+            const V3<T> dir = x - P3(pos, q);
+            const T dis = len3(dir);
+            if (dis > 0 && dis < h && (flag[q] == AIRSOLO || flag[q] == AIR)) {
+              const T cc2 = h - dis;
+              const T pterm = (T)-0.5f * cc2 * sp * (ip + press[q]) / dis;
+              const T dterm = cc2 * id * dens[q];
+              const V3<T> vq = P3(vel, q);
+              const V3<T> t = mk(pterm * dir.x, pterm * dir.y, pterm * dir.z) +
+                              mk(vterm * (vq.x - iv.x), vterm * (vq.y - iv.y), vterm * (vq.z - iv.z));
+              res = res + t * dterm;
+            }
+          }
+        force = force + res;
+      });
+      force = force * m0;
+      iv = iv + force * dt;
+      S3(vout, p, iv);
+    }
+    vel.swap(vout);
+  }
+
+  // correctpos: correctparticlepos + sumcellspring + smooth_kernel,
+  // spray_k.cu:1330-1411 (reads the input positions, writes new ones)
+  void correct_pos() {
+    const T dt = (T)c.dt, ks = (T)c.correction_spring, re = (T)c.correction_radius * h;
+    const T m = h + (T)0.5f * (T)c.samplespace;
+    const V3<T> lo = gmin() + mk(m, m, m), hi = gmax() - mk(m, m, m);
+    std::vector<T> out(pos);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < P; ++p) {
+      if (flag[p] == SOLID || flag[p] == AIRSOLO) continue;
+      V3<T> x = P3(pos, p);
+      int i, j, k;
+      cell_of(x, i, j, k);
+      V3<T> spring = mk(T(0), T(0), T(0));
+      for_cells(i, j, k, 1, [&](int64_t cc) {
+        V3<T> sc = mk(T(0), T(0), T(0));
+        if (cstart[cc] != MPF_CELL_UNDEF)
+          for (uint32_t q = cstart[cc]; q < cend[cc]; ++q) {
+            const V3<T> y = P3(pos, q);
+            const T dist = len3(y - x);
+            const T w = mass[q] * tmax(T(1) - (dist * dist) / (re * re), T(0));
+            if (dist > (T)0.1f * re) {
+              const V3<T> d = x - y;
+              sc = sc + mk((w * d.x) / dist, (w * d.y) / dist, (w * d.z) / dist);
+            }
+          }
+        spring = spring + sc;
+      });
+      spring = spring * (ks * re);
+      if (len3(mk(dt * spring.x, dt * spring.y, dt * spring.z)) > (T)0.3f * h) {
+        const T a = h * (T)0.3f, l = len3(spring);
+        x = x + mk((a * spring.x) / l, (a * spring.y) / l, (a * spring.z) / l);
+      } else {
+        x = x + mk(dt * spring.x, dt * spring.y, dt * spring.z);
+      }
+      x.x = tmax(lo.x, tmin(hi.x, x.x));
+      x.y = tmax(lo.y, tmin(hi.y, x.y));
+      x.z = tmax(lo.z, tmin(hi.z, x.z));
+      S3(out, p, x);
+    }
+    pos.swap(out);
+  }
+
+  // correctpos_bubble: correctbubblepos (spray_k.cu:3884-3931) or, with
+  // correct_bubble_air_ls, correctbubblepos_air (3934-3991) on the gradient of
+  // the air level set (computePhigra, 3994-4019)
+  void correct_bubble_pos() {
+    const bool airls = c.correct_bubble_air_ls != 0;
+    std::vector<T> ax, ay, az;
+    if (airls) {
+      ax.assign(G, T(0)); ay.assign(G, T(0)); az.assign(G, T(0));
+      for (int64_t idx = 0; idx < G; ++idx) {
+        int i, j, k;
+        ijk(idx, i, j, k);
+        const T lc = lsair[idx];
+        auto L = [&](int a, int b, int q) { return valid(a, b, q) ? lsair[cid(a, b, q)] : lc; };
+        ax[idx] = ((L(i + 1, j, k) - L(i - 1, j, k)) * (T)0.5f) / h;
+        ay[idx] = ((L(i, j + 1, k) - L(i, j - 1, k)) * (T)0.5f) / h;
+        az[idx] = ((L(i, j, k + 1) - L(i, j, k - 1)) * (T)0.5f) / h;
+      }
+    }
+    const T rs = (T)0.5f * h;
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < P; ++p) {
+      const int8_t fl = flag[p];
+      if (fl == AIRSOLO || fl == SOLID) continue;
+      V3<T> x = P3(pos, p);
+      const T s = fl == FLUID ? T(-1) : T(1);
+      auto unit = [&](const std::vector<T>& gx, const std::vector<T>& gy, const std::vector<T>& gz, V3<T>& dir,
+                      T& dl) {
+        dir = vec_from_grid(x, gx, gy, gz);
+        dl = len3(dir);
+        if (dl == 0) return false;
+        const T inv = T(1) / dl;
+        dir = dir * inv;
+        return true;
+      };
+      V3<T> dir;
+      T dl;
+      if (!unit(phigx, phigy, phigz, dir, dl)) continue;
+      T d = scalar_from_grid(x, lsmerge) / dl;
+      if (!airls) {
+        if (s * d < 0 && std::fabs(d) < (T)0.5f * h) {
+          if (fl == AIR && std::fabs(d) > (T)0.3f * h) {
+            x = x - mk(d * dir.x, d * dir.y, d * dir.z);
+          } else if (fl == FLUID) {
+            x = x - mk(d * dir.x, d * dir.y, d * dir.z);
+            if (!unit(phigx, phigy, phigz, dir, dl)) continue;
+            d = scalar_from_grid(x, lsmerge) / dl;
+            const T a = s * (rs - s * d);
+            x = x + mk(a * dir.x, a * dir.y, a * dir.z);
+          }
+        } else if (fl == FLUID && s * d < rs * (T)0.5f && s * d >= 0) {
+          const T a = s * (rs - s * d);
+          x = x + mk(a * dir.x, a * dir.y, a * dir.z);
+        }
+      } else {
+        if (s * d < 0 && std::fabs(d) < (T)0.5f * h && fl == AIR && std::fabs(d) > (T)0.3f * h)
+          x = x - mk(d * dir.x, d * dir.y, d * dir.z);
+        if (fl == FLUID) {
+          if (!unit(ax, ay, az, dir, dl)) continue;
+          d = scalar_from_grid(x, lsair) / dl;
+          if (d < (T)-1.3f * rs) {
+            const T a = d - rs;
+            x = x - mk(a * dir.x, a * dir.y, a * dir.z);
+          }
+        }
+      }
+      S3(pos, p, x);
+    }
+  }
+
   // ================= a4: node fields of the particle sources =================
   // genWaterDensfield_liquidAndGas + sumcelldens_liquidAndGas spray_k.cu:1595-1686,
   // genWaterDensfield_Gas + sumcelldens_Gas 1528-1593, genWaterDensfield_GY +
@@ -1586,14 +1898,26 @@ template <class T> struct Orc : OrcBase {
     hash_sort();
     apply_forces();
     levelset_update();
+    if (c.solo_air_enabled) mark_solo_air();
+    if (c.surface_tension_sigma != 0.0f) surface_tension();
     p2g();
+    if (c.drag_enabled) drag();
     extrapolate_bc(true);
     int st = project(&loc.project);
     extrapolate_bc(false);
     g2p();
+    if (c.solo_air_enabled) solo_air_sph();
     advect();
     hash_sort();
-    levelset_update();
+    if (c.correct_position) {
+      correct_pos();
+      hash_sort();
+      levelset_update();
+      correct_bubble_pos();
+      hash_sort();
+    } else {
+      levelset_update();
+    }
     if (c.heat_enabled) heat(&loc.heat);
     int st2 = phase_change_with_delete(frame, &loc.phase);
     if (st2 != MPF_OK) return st2;
@@ -1621,6 +1945,7 @@ template <class T> struct Orc : OrcBase {
       case MPF_F_WATER_UX_OLD: return &wuo[0]; case MPF_F_WATER_UY_OLD: return &wuo[1]; case MPF_F_WATER_UZ_OLD: return &wuo[2];
       case MPF_F_AIR_UX_OLD: return &auo[0]; case MPF_F_AIR_UY_OLD: return &auo[1]; case MPF_F_AIR_UZ_OLD: return &auo[2];
       case MPF_F_SOLID_UX: return &su[0]; case MPF_F_SOLID_UY: return &su[1]; case MPF_F_SOLID_UZ: return &su[2];
+      case MPF_F_SURFACE_TENSION: return &sf;
       default: return nullptr;
     }
   }
@@ -1809,6 +2134,14 @@ int orc_get_mesh(orc_state* s, double* verts, uint32_t* tris) {
     std::memcpy(tris, o.mc_tris.data(), o.mc_tris.size() * 4);
     return 0;
   });
+}
+int orc_mark_solo_air(orc_state* s) { return dispatch(s, [&](auto& o) { o.mark_solo_air(); return 0; }); }
+int orc_surface_tension(orc_state* s) { return dispatch(s, [&](auto& o) { o.surface_tension(); return 0; }); }
+int orc_drag_force(orc_state* s) { return dispatch(s, [&](auto& o) { o.drag(); return 0; }); }
+int orc_update_solo_air(orc_state* s) { return dispatch(s, [&](auto& o) { o.solo_air_sph(); return 0; }); }
+int orc_correct_positions(orc_state* s) { return dispatch(s, [&](auto& o) { o.correct_pos(); return 0; }); }
+int orc_correct_bubble_positions(orc_state* s) {
+  return dispatch(s, [&](auto& o) { o.correct_bubble_pos(); return 0; });
 }
 int orc_mc_hash_sort(orc_state* s) { return dispatch(s, [&](auto& o) { o.mc_hash_sort(); return 0; }); }
 int orc_get_mc_field(orc_state* s, double* out) {

@@ -57,6 +57,13 @@ int64_t orc_pour(orc_state* s, int frame);
 int64_t orc_delete_particles(orc_state* s, int keep_lists);
 
 int orc_hash_sort(orc_state* s);
+/* The remaining bubblesim stages (mpf_mark_solo_air ... mpf_correct_bubble_positions). */
+int orc_mark_solo_air(orc_state* s);
+int orc_surface_tension(orc_state* s);
+int orc_drag_force(orc_state* s);
+int orc_update_solo_air(orc_state* s);
+int orc_correct_positions(orc_state* s);
+int orc_correct_bubble_positions(orc_state* s);
 /* preMC: the sort on the 2N grid with its own cell lists (2nx*2ny*2nz). */
 int orc_mc_hash_sort(orc_state* s);
 /* Node field of the last particle-source mesh ((2nx+1)(2ny+1)(2nz+1) doubles)
