@@ -59,7 +59,7 @@ ABI_FUNCTIONS = [
     "mpf_advect_adjoint", "mpf_hash_sort_noalloc", "mpf_update_seed_cells", "mpf_set_pour_source",
     "mpf_delete_particles", "mpf_pour", "mpf_mc_hash_sort", "mpf_get_mc_field", "mpf_get_mc_cells",
     "mpf_mark_solo_air", "mpf_surface_tension", "mpf_drag_force", "mpf_update_solo_air",
-    "mpf_correct_positions", "mpf_correct_bubble_positions",
+    "mpf_correct_positions", "mpf_correct_bubble_positions", "mpf_solid_motion",
 ]
 
 
@@ -105,6 +105,7 @@ class Config(ctypes.Structure):
         ("drag_particle_update", ctypes.c_int32), ("correct_position", ctypes.c_int32),
         ("correction_spring", ctypes.c_float), ("correction_radius", ctypes.c_float),
         ("correct_bubble_air_ls", ctypes.c_int32),
+        ("solid_bounce", ctypes.c_float), ("rigid_lock_y", ctypes.c_int32),
     ]
 
 
@@ -170,7 +171,7 @@ def lib():
     L.mpf_set_seed_cells.argtypes = [vp, vp, i32]
     for n in ("mpf_hash_sort", "mpf_apply_forces", "mpf_levelset_update", "mpf_p2g", "mpf_g2p", "mpf_advect",
               "mpf_mark_solo_air", "mpf_surface_tension", "mpf_drag_force", "mpf_update_solo_air",
-              "mpf_correct_positions", "mpf_correct_bubble_positions"):
+              "mpf_correct_positions", "mpf_correct_bubble_positions", "mpf_solid_motion"):
         getattr(L, n).argtypes = [vp]
     L.mpf_extrapolate_bc.argtypes = [vp, i32]
     L.mpf_project.argtypes = [vp, c.POINTER(SolveInfo)]
@@ -386,6 +387,9 @@ class Simulator:
 
     def advect(self):
         self._check(self.L.mpf_advect(self.h))
+
+    def solid_motion(self):
+        self._check(self.L.mpf_solid_motion(self.h))
 
     def mark_solo_air(self):
         self._check(self.L.mpf_mark_solo_air(self.h))

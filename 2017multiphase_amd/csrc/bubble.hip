@@ -317,6 +317,12 @@ __global__ void k_sph_force(Grid g, int64_t P, const f3* __restrict__ pos, const
 }
 
 // ---- correctpos: correctparticlepos + sumcellspring, spray_k.cu:1330-1411 ----
+// The spring weight m (1 - d^2/re^2) is 0 from d >= re on (re = radius h,
+// 0.5 h by default), and a zero-weight pair adds +-0 to the spring, which
+// changes no position: cells farther than re from the particle are skipped
+// (the bound keeps a margin against the rounding of the cell boundaries), and
+// a pair is evaluated only when its squared distance is below re^2 (1 + 2^-19),
+// a bound under which the rounded quotient d^2/re^2 is always >= 1.
 __global__ void k_correct_pos(Grid g, int64_t P, const f3* __restrict__ pin, f3* __restrict__ pout,
                               const float* __restrict__ mass, const int8_t* __restrict__ flag,
                               const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend, float spring_k,
@@ -337,24 +343,73 @@ __global__ void k_correct_pos(Grid g, int64_t P, const f3* __restrict__ pin, f3*
   const f3 tmin = mk3(g.gx0 + m, g.gy0 + m, g.gz0 + m);
   const f3 tmax = mk3(g.gx1 - m, g.gy1 - m, g.gz1 - m);
   const float re = radius * hh;
-  for (int di = -1; di <= 1; ++di)
-    for (int dj = -1; dj <= 1; ++dj)
-      for (int dk = -1; dk <= 1; ++dk) {
-        if (!valid(g, i + di, j + dj, k + dk)) continue;
-        const int64_t c = cid(g, i + di, j + dj, k + dk);
-        const uint32_t s = cstart[c];
-        f3 sc = mk3(0.0f, 0.0f, 0.0f);
-        if (s != MPF_CELL_UNDEF) {
-          const uint32_t e = cend[c];
-          for (uint32_t q = s; q < e; ++q) {
-            const f3 y = pin[q];
-            const float dist = len3(sub3(y, x));
-            const float w = mass[q] * fmaxf(1.0f - (dist * dist) / (re * re), 0.0f);  // smooth_kernel
+  const float re2 = re * re;
+  const float cut = re2 * (1.0f + 1.0f / 524288.0f);
+  // distance from the particle to the low and high faces of its cell, per
+  // axis, less a margin: a neighbour cell on that side is reachable only
+  // when that gap is below re
+  const float mg = 1e-3f * hh;
+  const float lx = (x.x - g.gx0) - (float)i * hh, ly = (x.y - g.gy0) - (float)j * hh, lz = (x.z - g.gz0) - (float)k * hh;
+  const float gl[3] = {lx - mg, ly - mg, lz - mg};
+  const float gh[3] = {hh - lx - mg, hh - ly - mg, hh - lz - mg};
+  auto gap = [&](int a, int d) -> float {
+    const float v = d < 0 ? gl[a] : (d > 0 ? gh[a] : 0.0f);
+    return v > 0.0f ? v : 0.0f;
+  };
+  // the reachable cells, as bits of the 27 offsets in the reference's order
+  uint32_t cand = 0;
+  for (int o = 0; o < 27; ++o) {
+    const int di = o / 9 - 1, dj = (o / 3) % 3 - 1, dk = o % 3 - 1;
+    if (!valid(g, i + di, j + dj, k + dk)) continue;
+    const float gx = gap(0, di), gy = gap(1, dj), gz = gap(2, dk);
+    if (gx * gx + gy * gy + gz * gz <= cut) cand |= 1u << o;
+  }
+  // up to 8 cells at a time: their ranges read together, then each cell's
+  // particles four at a time (positions in flight together); the sums keep
+  // the reference's order (cells by offset, particles by index)
+  constexpr int NCB = 8;
+  while (cand) {
+    uint32_t cs[NCB], ce[NCB];
+    int nc = 0;
+#pragma unroll
+    for (int t = 0; t < NCB; ++t) {
+      int64_t c = 0;
+      if (cand) {
+        const int o = __ffs(cand) - 1;
+        cand &= cand - 1u;
+        c = cid(g, i + o / 9 - 1, j + (o / 3) % 3 - 1, k + o % 3 - 1);
+        nc = t + 1;
+      }
+      cs[t] = cstart[c];
+      ce[t] = cend[c];
+    }
+#pragma unroll
+    for (int t = 0; t < NCB; ++t) {
+      if (t >= nc) break;
+      const uint32_t s0 = cs[t];
+      f3 sc = mk3(0.0f, 0.0f, 0.0f);
+      if (s0 != MPF_CELL_UNDEF) {
+        const uint32_t e0 = ce[t];
+        for (uint32_t q0 = s0; q0 < e0; q0 += 4) {
+          f3 y4[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) y4[u] = pin[min(q0 + (uint32_t)u, e0 - 1u)];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (q0 + (uint32_t)u >= e0) break;
+            const f3 y = y4[u];
+            const f3 dd = sub3(y, x);
+            const float d2 = dot3(dd, dd);
+            if (!(d2 < cut)) continue;
+            const float dist = sqrtf(d2);
+            const float w = mass[q0 + u] * fmaxf(1.0f - (dist * dist) / (re * re), 0.0f);
             if (dist > 0.1f * re) sc = add3(sc, div3(smul3(w, sub3(x, y)), dist));
           }
         }
-        spring = add3(spring, sc);
       }
+      spring = add3(spring, sc);
+    }
+  }
   spring = mul3(spring, spring_k * re);
   if (len3(smul3(dt, spring)) > 0.3f * hh)
     x = add3(x, div3(smul3(hh * 0.3f, spring), len3(spring)));

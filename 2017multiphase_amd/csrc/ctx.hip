@@ -132,8 +132,10 @@ void free_all(Ctx& c) {
   adj_free(c);
   if (c.scal_host) hipHostFree(c.scal_host);
   if (c.counters_host) hipHostFree(c.counters_host);
+  if (c.red_host) hipHostFree(c.red_host);
   c.scal_host = nullptr;
   c.counters_host = nullptr;
+  c.red_host = nullptr;
   for (auto& pe : c.pending) { hipEventDestroy(pe.a); hipEventDestroy(pe.b); }
   for (auto e : c.event_pool) hipEventDestroy(e);
   c.pending.clear();
@@ -210,7 +212,8 @@ bool alloc_all(Ctx& c) {
     return false;
   }
   if (hipHostMalloc((void**)&c.scal_host, sizeof(SolveScalars), hipHostMallocDefault) != hipSuccess ||
-      hipHostMalloc((void**)&c.counters_host, NCOUNTERS * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void**)&c.counters_host, NCOUNTERS * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c.red_host, 16 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     c.err = "hipHostMalloc failed";
     return false;
   }
@@ -372,6 +375,8 @@ mpf_status mpf_config_default(mpf_config* cfg, int32_t nx, int32_t ny, int32_t n
   cfg->correction_spring = 500000.0f;        // spray.cpp:617-618
   cfg->correction_radius = 0.5f;
   cfg->correct_bubble_air_ls = 0;
+  cfg->solid_bounce = 0.5f;                  // SolidbounceParam, spray.cpp:52
+  cfg->rigid_lock_y = 0;
   return MPF_OK;
 }
 
@@ -747,6 +752,15 @@ mpf_status mpf_correct_bubble_positions(mpf_ctx* ctx) {
   return check_launch(c);
 }
 
+mpf_status mpf_solid_motion(mpf_ctx* ctx) {
+  if (!ctx) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  int st = run_solid_motion(c);
+  if (st != MPF_OK) return (mpf_status)st;
+  return check_launch(c);
+}
+
 mpf_status mpf_levelset_update(mpf_ctx* ctx) {
   if (!ctx) return MPF_INVALID_ARG;
   Ctx& c = ctx->c;
@@ -977,7 +991,14 @@ mpf_status mpf_step(mpf_ctx* ctx, int32_t frame, mpf_step_info* info) {
   const std::string perr = c.err;
   if ((st = mpf_extrapolate_bc(ctx, 0)) != MPF_OK) return st;
   if ((st = mpf_g2p(ctx)) != MPF_OK) return st;
+  // bubblesim runs solidmotion before updateSoloAirParticle; the SPH reads
+  // only air particles, which the solid motion does not touch, so running it
+  // first gives the same result while the cell lists still hold
   if (k.solo_air_enabled && (st = mpf_update_solo_air(ctx)) != MPF_OK) return st;
+  if ((st = mpf_solid_motion(ctx)) != MPF_OK) return st;
+  // bubblesim sorts before CollisionSolid (spray.cpp:1122); only a moving
+  // solid has moved particles since the step's first sort
+  if (!c.sorted && (st = mpf_hash_sort(ctx)) != MPF_OK) return st;
   if ((st = mpf_advect(ctx)) != MPF_OK) return st;
   if ((st = mpf_hash_sort(ctx)) != MPF_OK) return st;
   if (k.correct_position) {

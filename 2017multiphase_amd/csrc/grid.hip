@@ -450,7 +450,113 @@ __host__ __device__ constexpr int st_r(int a, int d) { return st_t(d) + (d == a 
 __host__ __device__ constexpr int st_rsz(int a) { return st_r(a, 0) * st_r(a, 1) * st_r(a, 2); }
 __host__ __device__ constexpr int st_roff(int a) { return a == 0 ? 0 : (a == 1 ? st_rsz(0) : st_rsz(0) + st_rsz(1)); }
 constexpr int ST_RTOT = st_rsz(0) + st_rsz(1) + st_rsz(2);  // 1580 faces
-constexpr int SCH = 8;  // particles kept in registers per cell
+constexpr int SCH = 8;   // particles of a cell in registers in the first pass
+constexpr int SCH2 = 4;  // particles per cell in each overflow pass
+
+// A cell's next ns particles [base, base + cnt) in registers (an empty slot
+// loads the chunk's last particle, or particle 0 when the chunk is empty, so
+// all loads are in flight together; its weight is 0).
+template <int NS>
+__device__ __forceinline__ void p2g_load(const f3* __restrict__ pos, const f3* __restrict__ vel,
+                                         const float* __restrict__ mass, const int8_t* __restrict__ flag,
+                                         const Grid& g, float scale, uint32_t base, uint32_t cnt, float (&X)[NS],
+                                         float (&Y)[NS], float (&Z)[NS], float (&VX)[NS], float (&VY)[NS],
+                                         float (&VZ)[NS], float (&M)[NS]) {
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    const uint32_t p = (cnt > 0) ? base + min((uint32_t)q, cnt - 1) : 0u;
+    const int8_t fl = flag[p];
+    const f3 x = pos[p], v = vel[p];
+    const float m = mass[p];
+    X[q] = (x.x - g.gx0) * scale;
+    Y[q] = (x.y - g.gy0) * scale;
+    Z[q] = (x.z - g.gz0) * scale;
+    VX[q] = v.x; VY[q] = v.y; VZ[q] = v.z;
+    M[q] = ((uint32_t)q < cnt && (fl == FLUID || fl == SOLID)) ? m : 0.0f;
+  }
+}
+
+// The 27 phases of one pass: in phase o every active thread adds its NS
+// register particles (of cell (ci, cj, ck)) to face c - o of each axis whose
+// window holds o.  Within a pass the active cells are distinct, so a face
+// takes at most one thread per phase (no atomics).  Inactive threads only
+// meet the barriers.
+template <int NS>
+__device__ __forceinline__ void p2g_phases(int x0, int y0, int z0, int ci, int cj, int ck, bool act,
+                                           const float (&X)[NS], const float (&Y)[NS], const float (&Z)[NS],
+                                           const float (&VX)[NS], const float (&VY)[NS], const float (&VZ)[NS],
+                                           const float (&M)[NS], float* sW, float* sU) {
+  const float RE = 1.4f, RE2 = RE * RE;
+  for (int ph = 0; ph < 27; ++ph) {
+    const int o[3] = {ph / 9 - 1, (ph / 3) % 3 - 1, ph % 3 - 1};
+    if (act) {
+      // The faces of the three axes in this phase share the cell c - o:
+      // axis a's face sits at that cell's corner along a and at its centre
+      // along the others, so each particle's six coordinate differences
+      // (corner and centre, per direction) and their squares serve all
+      // three distances, summed in the gather's order.
+      const int f[3] = {ci - o[0], cj - o[1], ck - o[2]};
+      int li[3];
+      float W[3], U[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const int r0 = x0 - (a != 0), r1 = y0 - (a != 1), r2 = z0 - (a != 2);
+        li[a] = st_roff(a) + ((f[0] - r0) * st_r(a, 1) + (f[1] - r1)) * st_r(a, 2) + (f[2] - r2);
+        W[a] = (o[a] > 0) ? 0.0f : sW[li[a]];  // normal window is {-1, 0}
+        U[a] = (o[a] > 0) ? 0.0f : sU[li[a]];
+      }
+      const float fx = (float)f[0], fy = (float)f[1], fz = (float)f[2];
+      const float hx = fx + 0.5f, hy = fy + 0.5f, hz = fz + 0.5f;
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        const float ax = X[q] - fx, bx = X[q] - hx, ay = Y[q] - fy, by = Y[q] - hy, az = Z[q] - fz, bz = Z[q] - hz;
+        const float ax2 = ax * ax, bx2 = bx * bx, ay2 = ay * ay, by2 = by * by, az2 = az * az, bz2 = bz * bz;
+        const float d2[3] = {ax2 + by2 + bz2, bx2 + ay2 + bz2, bx2 + by2 + az2};
+        const float vq[3] = {VX[q], VY[q], VZ[q]};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          if (o[a] > 0) continue;
+          const float w = M[q] * fmaxf(div_nr(RE2, fmaxf(d2[a], 0.0001f)) - 1.0f, 0.0f);
+          W[a] += w;
+          U[a] += w * vq[a];
+        }
+      }
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        if (o[a] > 0) continue;
+        sW[li[a]] = W[a];
+        sU[li[a]] = U[a];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Overflow passes of a tile: pass p takes particles [thr, thr + SCH2) of
+// every cell holding more than thr = SCH + (p-1) SCH2, the cells compacted
+// onto the first threads (a wave without such a cell skips the arithmetic).
+// Returns after the pass that finds no cell left; block-uniform.
+template <class PassFn>
+__device__ __forceinline__ void overflow_passes(uint32_t n, uint32_t* sList, int* sWc, PassFn pass) {
+  const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+  for (uint32_t thr = SCH;; thr += SCH2) {
+    const bool pred = n > thr;
+    const unsigned long long bal = __ballot(pred);
+    if (ln == 0) sWc[wv] = __popcll(bal);
+    __syncthreads();
+    int off = 0, tot = 0;
+#pragma unroll
+    for (int w2 = 0; w2 < ST_CELLS / 64; ++w2) {
+      off += (w2 < wv) ? sWc[w2] : 0;
+      tot += sWc[w2];
+    }
+    if (pred) sList[off + __popcll(bal & ((1ull << ln) - 1ull))] = threadIdx.x;
+    __syncthreads();
+    if (tot == 0) return;
+    const bool act = (int)threadIdx.x < tot;
+    pass(act, act ? sList[threadIdx.x] : 0u, thr);
+  }
+}
 
 __global__ void __launch_bounds__(ST_CELLS)
     k_p2g_tile(Grid g, const f3* __restrict__ pos, const f3* __restrict__ vel, const float* __restrict__ mass,
@@ -459,6 +565,8 @@ __global__ void __launch_bounds__(ST_CELLS)
                const float* __restrict__ band, float big, const uint8_t* __restrict__ dmask) {
   __shared__ float sW[ST_RTOT];
   __shared__ float sU[ST_RTOT];
+  __shared__ uint32_t sS[ST_CELLS], sN[ST_CELLS], sList[ST_CELLS];
+  __shared__ int sWc[ST_CELLS / 64];
   int t = blockIdx.x;
   const int tz = t % ntz;
   t /= ntz;
@@ -510,84 +618,23 @@ __global__ void __launch_bounds__(ST_CELLS)
   const int any = __syncthreads_or(e > s) && near;
   if (any) {
     const float scale = 1.0f / g.h;
-    const float RE = 1.4f, RE2 = RE * RE;
-    float X[SCH], Y[SCH], Z[SCH], VX[SCH], VY[SCH], VZ[SCH], M[SCH];
     const uint32_t n = e - s;
-    // every slot loads (an empty slot its cell's last particle, or particle
-    // 0 in an empty cell of a tile that has particles), so the loads of all
-    // slots are in flight together; empty slots and other phases weigh 0
-    // (w = 0 * max(...) adds +-0 to the sums)
-#pragma unroll
-    for (int q = 0; q < SCH; ++q) {
-      const uint32_t p = (n > 0) ? s + min((uint32_t)q, n - 1) : 0u;
-      const int8_t fl = flag[p];
-      const f3 x = pos[p], v = vel[p];
-      const float m = mass[p];
-      X[q] = (x.x - g.gx0) * scale;
-      Y[q] = (x.y - g.gy0) * scale;
-      Z[q] = (x.z - g.gz0) * scale;
-      VX[q] = v.x; VY[q] = v.y; VZ[q] = v.z;
-      M[q] = ((uint32_t)q < n && (fl == FLUID || fl == SOLID)) ? m : 0.0f;
+    sS[b] = s;
+    sN[b] = n;
+    {
+      float X[SCH], Y[SCH], Z[SCH], VX[SCH], VY[SCH], VZ[SCH], M[SCH];
+      p2g_load<SCH>(pos, vel, mass, flag, g, scale, s, min(n, (uint32_t)SCH), X, Y, Z, VX, VY, VZ, M);
+      p2g_phases<SCH>(x0, y0, z0, ci, cj, ck, n > 0, X, Y, Z, VX, VY, VZ, M, sW, sU);
     }
-    for (int ph = 0; ph < 27; ++ph) {
-      const int o[3] = {ph / 9 - 1, (ph / 3) % 3 - 1, ph % 3 - 1};
-      if (n > 0) {
-        // The faces of the three axes in this phase share the cell c - o:
-        // axis a's face sits at that cell's corner along a and at its centre
-        // along the others, so each particle's six coordinate differences
-        // (corner and centre, per direction) and their squares serve all
-        // three distances, summed in the gather's order.
-        const int f[3] = {ci - o[0], cj - o[1], ck - o[2]};
-        int li[3];
-        float W[3], U[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          const int r0 = x0 - (a != 0), r1 = y0 - (a != 1), r2 = z0 - (a != 2);
-          li[a] = st_roff(a) + ((f[0] - r0) * st_r(a, 1) + (f[1] - r1)) * st_r(a, 2) + (f[2] - r2);
-          W[a] = (o[a] > 0) ? 0.0f : sW[li[a]];  // normal window is {-1, 0}
-          U[a] = (o[a] > 0) ? 0.0f : sU[li[a]];
-        }
-        const float fx = (float)f[0], fy = (float)f[1], fz = (float)f[2];
-        const float hx = fx + 0.5f, hy = fy + 0.5f, hz = fz + 0.5f;
-#pragma unroll
-        for (int q = 0; q < SCH; ++q) {
-          const float ax = X[q] - fx, bx = X[q] - hx, ay = Y[q] - fy, by = Y[q] - hy, az = Z[q] - fz, bz = Z[q] - hz;
-          const float ax2 = ax * ax, bx2 = bx * bx, ay2 = ay * ay, by2 = by * by, az2 = az * az, bz2 = bz * bz;
-          const float d2[3] = {ax2 + by2 + bz2, bx2 + ay2 + bz2, bx2 + by2 + az2};
-          const float vq[3] = {VX[q], VY[q], VZ[q]};
-#pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            if (o[a] > 0) continue;
-            const float w = M[q] * fmaxf(div_nr(RE2, fmaxf(d2[a], 0.0001f)) - 1.0f, 0.0f);
-            W[a] += w;
-            U[a] += w * vq[a];
-          }
-        }
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          if (o[a] > 0) continue;
-          const float gx = (float)f[0] + (a == 0 ? 0.0f : 0.5f);
-          const float gy = (float)f[1] + (a == 1 ? 0.0f : 0.5f);
-          const float gz = (float)f[2] + (a == 2 ? 0.0f : 0.5f);
-          float Wa = W[a], Ua = U[a];
-          for (uint32_t p = s + SCH; p < e; ++p) {  // beyond the register chunk, still in index order
-            const int8_t fl = flag[p];
-            if (!(fl == FLUID || fl == SOLID)) continue;
-            const f3 x = pos[p];
-            const float dx = (x.x - g.gx0) * scale - gx, dy = (x.y - g.gy0) * scale - gy,
-                        dz = (x.z - g.gz0) * scale - gz;
-            const float r2s = dx * dx + dy * dy + dz * dz;
-            const float w = mass[p] * fmaxf(div_nr(RE2, fmaxf(r2s, 0.0001f)) - 1.0f, 0.0f);
-            Wa += w;
-            const float* vp = &vel[p].x;
-            Ua += w * vp[a];
-          }
-          sW[li[a]] = Wa;
-          sU[li[a]] = Ua;
-        }
-      }
-      __syncthreads();
-    }
+    // cells with more than SCH particles: the rest in compacted passes
+    overflow_passes(n, sList, sWc, [&](bool act, uint32_t cb, uint32_t thr) {
+      const int oi = x0 + (int)cb / (ST_Y * ST_Z), oj = y0 + ((int)cb / ST_Z) % ST_Y, ok = z0 + (int)cb % ST_Z;
+      const uint32_t os = act ? sS[cb] + thr : 0u;
+      const uint32_t oc = act ? min(sN[cb] - thr, (uint32_t)SCH2) : 0u;
+      float X[SCH2], Y[SCH2], Z[SCH2], VX[SCH2], VY[SCH2], VZ[SCH2], M[SCH2];
+      if (act) p2g_load<SCH2>(pos, vel, mass, flag, g, scale, os, oc, X, Y, Z, VX, VY, VZ, M);
+      p2g_phases<SCH2>(x0, y0, z0, oi, oj, ok, act, X, Y, Z, VX, VY, VZ, M, sW, sU);
+    });
   } else {
     __syncthreads();
   }
@@ -889,12 +936,64 @@ __global__ void k_subgrad(Grid g, float* __restrict__ u, const R* __restrict__ p
 constexpr int HR_X = ST_X + 2, HR_Y = ST_Y + 2, HR_Z = ST_Z + 2;
 constexpr int HR = HR_X * HR_Y * HR_Z;  // 600 cells
 
+// NS particles of cell (ci, cj, ck) in registers, and the phases that add
+// them to cell c - o in phase o (the liquid P2G's pass structure).
+template <int NS>
+__device__ __forceinline__ void heat_load(const f3* __restrict__ pos, const float* __restrict__ temp, const Grid& g,
+                                          float scale, uint32_t base, uint32_t cnt, float (&X)[NS], float (&Y)[NS],
+                                          float (&Z)[NS], float (&TT)[NS], float (&ON)[NS]) {
+  f3 xr[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    const uint32_t p = (cnt > 0) ? base + min((uint32_t)q, cnt - 1) : 0u;
+    xr[q] = pos[p];
+    TT[q] = temp[p];
+  }
+  __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of their uses
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    const bool on = (uint32_t)q < cnt;
+    X[q] = on ? (xr[q].x - g.gx0) * scale : 0.0f;
+    Y[q] = on ? (xr[q].y - g.gy0) * scale : 0.0f;
+    Z[q] = on ? (xr[q].z - g.gz0) * scale : 0.0f;
+    TT[q] = on ? TT[q] : 0.0f;
+    ON[q] = on ? 1.0f : 0.0f;
+  }
+}
+
+template <int NS>
+__device__ __forceinline__ void heat_phases(int x0, int y0, int z0, int ci, int cj, int ck, bool act,
+                                            const float (&X)[NS], const float (&Y)[NS], const float (&Z)[NS],
+                                            const float (&TT)[NS], const float (&ON)[NS], float* sW, float* sH) {
+  const float RE = 1.4f, RE2 = RE * RE;
+  for (int ph = 0; ph < 27; ++ph) {
+    const int f0 = ci - (ph / 9 - 1), f1 = cj - ((ph / 3) % 3 - 1), f2 = ck - (ph % 3 - 1);
+    if (act) {
+      const int li = ((f0 - (x0 - 1)) * HR_Y + (f1 - (y0 - 1))) * HR_Z + (f2 - (z0 - 1));
+      float W = sW[li], Hs = sH[li];
+      const float gx = (float)f0 + 0.5f, gy = (float)f1 + 0.5f, gz = (float)f2 + 0.5f;
+#pragma unroll
+      for (int q = 0; q < NS; ++q) {
+        const float dx = X[q] - gx, dy = Y[q] - gy, dz = Z[q] - gz;
+        const float w = ON[q] * fmaxf(div_nr(RE2, fmaxf(dx * dx + dy * dy + dz * dz, 0.0001f)) - 1.0f, 0.0f);
+        W += w;
+        Hs += w * TT[q];
+      }
+      sW[li] = W;
+      sH[li] = Hs;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ void __launch_bounds__(ST_CELLS)
     k_heat_tile(Grid g, const f3* __restrict__ pos, const float* __restrict__ temp,
                 const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
                 const uint8_t* __restrict__ cmask, float* __restrict__ parts, int ntx, int nty, int ntz) {
   __shared__ float sW[HR];
   __shared__ float sH[HR];
+  __shared__ uint32_t sS[ST_CELLS], sN[ST_CELLS], sList[ST_CELLS];
+  __shared__ int sWc[ST_CELLS / 64];
   int t = blockIdx.x;
   const int tz = t % ntz;
   t /= ntz;
@@ -918,55 +1017,22 @@ __global__ void __launch_bounds__(ST_CELLS)
   const int any = __syncthreads_or(e > s);
   if (any) {
     const float scale = 1.0f / g.h;
-    const float RE = 1.4f, RE2 = RE * RE;
-    float X[SCH], Y[SCH], Z[SCH], TT[SCH], ON[SCH];
     const uint32_t n = e - s;
-    // every slot loads (an empty slot its cell's last particle, or particle
-    // 0 in an empty cell of a tile that has particles), so the loads of all
-    // slots are in flight together; empty slots weigh 0 (adds +-0)
-    f3 xr[SCH];
-#pragma unroll
-    for (int q = 0; q < SCH; ++q) {
-      const uint32_t p = (n > 0) ? s + min((uint32_t)q, n - 1) : 0u;
-      xr[q] = pos[p];
-      TT[q] = temp[p];
+    sS[b] = s;
+    sN[b] = n;
+    {
+      float X[SCH], Y[SCH], Z[SCH], TT[SCH], ON[SCH];
+      heat_load<SCH>(pos, temp, g, scale, s, min(n, (uint32_t)SCH), X, Y, Z, TT, ON);
+      heat_phases<SCH>(x0, y0, z0, ci, cj, ck, n > 0, X, Y, Z, TT, ON, sW, sH);
     }
-    __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of their uses
-#pragma unroll
-    for (int q = 0; q < SCH; ++q) {
-      const bool on = (uint32_t)q < n;
-      X[q] = on ? (xr[q].x - g.gx0) * scale : 0.0f;
-      Y[q] = on ? (xr[q].y - g.gy0) * scale : 0.0f;
-      Z[q] = on ? (xr[q].z - g.gz0) * scale : 0.0f;
-      TT[q] = on ? TT[q] : 0.0f;
-      ON[q] = on ? 1.0f : 0.0f;
-    }
-    for (int ph = 0; ph < 27; ++ph) {
-      const int f0 = ci - (ph / 9 - 1), f1 = cj - ((ph / 3) % 3 - 1), f2 = ck - (ph % 3 - 1);
-      if (n > 0) {
-        const int li = ((f0 - (x0 - 1)) * HR_Y + (f1 - (y0 - 1))) * HR_Z + (f2 - (z0 - 1));
-        float W = sW[li], Hs = sH[li];
-        const float gx = (float)f0 + 0.5f, gy = (float)f1 + 0.5f, gz = (float)f2 + 0.5f;
-#pragma unroll
-        for (int q = 0; q < SCH; ++q) {
-          const float dx = X[q] - gx, dy = Y[q] - gy, dz = Z[q] - gz;
-          const float w = ON[q] * fmaxf(div_nr(RE2, fmaxf(dx * dx + dy * dy + dz * dz, 0.0001f)) - 1.0f, 0.0f);
-          W += w;
-          Hs += w * TT[q];
-        }
-        for (uint32_t p = s + SCH; p < e; ++p) {
-          const f3 x = pos[p];
-          const float dx = (x.x - g.gx0) * scale - gx, dy = (x.y - g.gy0) * scale - gy,
-                      dz = (x.z - g.gz0) * scale - gz;
-          const float w = fmaxf(div_nr(RE2, fmaxf(dx * dx + dy * dy + dz * dz, 0.0001f)) - 1.0f, 0.0f);
-          W += w;
-          Hs += w * temp[p];
-        }
-        sW[li] = W;
-        sH[li] = Hs;
-      }
-      __syncthreads();
-    }
+    overflow_passes(n, sList, sWc, [&](bool act, uint32_t cb, uint32_t thr) {
+      const int oi = x0 + (int)cb / (ST_Y * ST_Z), oj = y0 + ((int)cb / ST_Z) % ST_Y, ok = z0 + (int)cb % ST_Z;
+      const uint32_t os = act ? sS[cb] + thr : 0u;
+      const uint32_t oc = act ? min(sN[cb] - thr, (uint32_t)SCH2) : 0u;
+      float X[SCH2], Y[SCH2], Z[SCH2], TT[SCH2], ON[SCH2];
+      if (act) heat_load<SCH2>(pos, temp, g, scale, os, oc, X, Y, Z, TT, ON);
+      heat_phases<SCH2>(x0, y0, z0, oi, oj, ok, act, X, Y, Z, TT, ON, sW, sH);
+    });
   } else {
     __syncthreads();
   }

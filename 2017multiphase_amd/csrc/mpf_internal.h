@@ -184,6 +184,7 @@ struct Ctx {
   SolveScalars* scal_host = nullptr;  // pinned
   uint32_t* counters = nullptr;       // [0] reduction arrival, [1..63] misc counters, then spread counters
   uint32_t* counters_host = nullptr;  // pinned
+  double* red_host = nullptr;         // pinned, 16 doubles: reduced totals read by the host
 
   // faces
   float* wu[3] = {};
@@ -292,6 +293,8 @@ void run_solo_air_sph(Ctx& c);
 void run_correct_pos(Ctx& c);
 void run_correct_bubble_pos(Ctx& c);
 void sph_constants(float h, float& poly6, float& spiky, float& lap);
+// solid.hip
+int run_solid_motion(Ctx& c);
 void run_solid_phi_and_velocity(Ctx& c);
 int64_t p2g_parts_floats(const Grid& g);
 // Device counters: 64 single counters, then SPREAD counters 64 bytes apart

@@ -197,6 +197,9 @@ typedef struct mpf_config {
   float correction_spring, correction_radius; /* (500000, 0.5 cells)          */
   int32_t correct_bubble_air_ls; /* correctbubblepos_air (the MELTANDBOIL,
                                 INTERACTION and ALL scenes; 0)                */
+  /* rigid solid motion (solidmotion, spray.cu:2691-2774; fix_solids = 0) */
+  float solid_bounce;        /* SolidbounceParam (0.5, spray.cpp:52)          */
+  int32_t rigid_lock_y;      /* SCENE_INTERACTION: no translation along y (0) */
 } mpf_config;
 
 /* Opaque context. */
@@ -387,6 +390,16 @@ mpf_status mpf_drag_force(mpf_ctx* ctx);
    enforceForceSoloAirP, spray_k.cu:4284-4373); neighbour velocities are
    the pass's input (the reference reads them while they are rewritten). */
 mpf_status mpf_update_solo_air(mpf_ctx* ctx);
+/* cspray::solidmotion_fixed / solidmotion, spray.cu:2685-2774: with
+   cfg.fix_solids the SOLID particles' velocities are zeroed; otherwise the
+   solid moves rigidly: wall push (solidCollisionWithBound, spray_k.cu:
+   5478-5508), translation = mean velocity, centre = mean position, rotation
+   = the reference's estimate sum(r x v) / mean|r|^2 / n (compute_cI_k), each
+   particle's velocity averaged with the rigid one and its position rotated
+   and translated (computeVelSolid_k, computePosSolid_k).  Sums are taken in
+   double in a fixed order (the reference adds float block sums).  Moves
+   particles: a sort must follow. */
+mpf_status mpf_solid_motion(mpf_ctx* ctx);
 /* cspray::correctpos, spray.cu:328-339 (correctparticlepos, spray_k.cu:
    1357-1411): spring separation of FLUID/AIR particles, clamped to the
    domain.  Leaves the particles unsorted. */
@@ -400,7 +413,7 @@ mpf_status mpf_correct_bubble_positions(mpf_ctx* ctx);
 /* One full step in bubblesim order (spray.cpp:979-1233), the stages above
    only: seed refresh, pour, sort, forces, level set, solo-air marking,
    surface tension, P2G, drag, extrapolation, projection, extrapolation,
-   G2P, solo-air SPH, advection, then either (cfg.correct_position) sort,
+   G2P, solid motion, solo-air SPH, advection, then either (cfg.correct_position) sort,
    position correction, sort, level set, bubble position correction, sort,
    or sort and level set; heat, nucleation, deletion (cfg.delete_enabled),
    melting/freezing, and on frames

@@ -67,7 +67,7 @@ def lib():
     L.orc_get_mc_cases.argtypes = [vp, vp]
     L.orc_mc_hash_sort.argtypes = [vp]
     for n in ("mark_solo_air", "surface_tension", "drag_force", "update_solo_air", "correct_positions",
-              "correct_bubble_positions"):
+              "correct_bubble_positions", "solid_motion"):
         getattr(L, "orc_" + n).argtypes = [vp]
     L.orc_get_mc_field.argtypes = [vp, vp]
     L.orc_get_mc_cells.argtypes = [vp, vp, vp]
@@ -219,6 +219,9 @@ class Oracle:
 
     def advect(self):
         self._ok(self.L.orc_advect(self.h))
+
+    def solid_motion(self):
+        self._ok(self.L.orc_solid_motion(self.h))
 
     def mark_solo_air(self):
         self._ok(self.L.orc_mark_solo_air(self.h))

@@ -1713,6 +1713,100 @@ template <class T> struct Orc : OrcBase {
     }
   }
 
+  // ================= f4: rigid solid motion =================
+  // solidmotion_fixed / solidmotion, spray.cu:2685-2774 (solidCollisionWithBound,
+  // compute_cI_k, computeVelSolid_k, computePosSolid_k, spray_k.cu:5188-5252,
+  // 5478-5508); sums in double, the host arithmetic in float as the reference's
+  void solid_motion() {
+    if (c.fix_solids) {
+      for (int64_t p = 0; p < P; ++p)
+        if (flag[p] == SOLID) S3(vel, p, mk(T(0), T(0), T(0)));
+      return;
+    }
+    double A[4] = {0, 0, 0, 0};
+    for (int64_t p = 0; p < P; ++p) {
+      if (flag[p] != SOLID) continue;
+      A[0] += 1.0;
+      A[1] += (double)pos[3 * p]; A[2] += (double)pos[3 * p + 1]; A[3] += (double)pos[3 * p + 2];
+    }
+    const int n = (int)A[0];
+    if (n <= 0) return;
+    const float fn = (float)n;
+    const T bounce = (T)c.solid_bounce;
+    const T mgn = h + (T)0.3f * (T)c.samplespace;
+    const V3<T> lo = gmin() + mk(mgn, mgn, mgn), hi = gmax() - mk(mgn, mgn, mgn);
+    for (int64_t p = 0; p < P; ++p) {
+      if (flag[p] != SOLID) continue;
+      const V3<T> x = P3(pos, p);
+      V3<T> v = P3(vel, p);
+      const T tn = (T)fn;
+      if (x.x < lo.x) v.x += (lo.x - x.x) * bounce * tn;
+      if (x.x > hi.x) v.x -= (x.x - hi.x) * bounce * tn;
+      if (x.y < lo.y) v.y += (lo.y - x.y) * bounce * tn;
+      if (x.y > hi.y) v.y -= (x.y - hi.y) * bounce * tn;
+      if (x.z < lo.z) v.z += (lo.z - x.z) * bounce * tn;
+      if (x.z > hi.z) v.z -= (x.z - hi.z) * bounce * tn;
+      S3(vel, p, v);
+    }
+    const V3<T> rg0 = mk((T)((float)A[1] / fn), (T)((float)A[2] / fn), (T)((float)A[3] / fn));
+    double B[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int64_t p = 0; p < P; ++p) {
+      if (flag[p] != SOLID) continue;
+      const V3<T> u = P3(vel, p);
+      B[0] += (double)u.x; B[1] += (double)u.y; B[2] += (double)u.z;
+      const V3<T> r = P3(pos, p) - rg0;
+      const T dis = len3(r);
+      if (dis > (T)1e-6f) {
+        B[3] += (double)(r.y * u.z - r.z * u.y);
+        B[4] += (double)(r.z * u.x - r.x * u.z);
+        B[5] += (double)(r.x * u.y - r.y * u.x);
+        B[6] += (double)dis;
+      }
+    }
+    // host arithmetic of spray.cu:2714-2766, in float
+    float Tv[3] = {(float)B[0] / fn, (float)B[1] / fn, (float)B[2] / fn};
+    if (c.rigid_lock_y) Tv[1] = 0.0f;
+    const float dt = c.dt;
+    float g0[3], g1[3], R[3];
+    for (int a = 0; a < 3; ++a) {
+      g0[a] = (float)A[1 + a] / fn;
+      g1[a] = g0[a] + dt * Tv[a];
+    }
+    float ti = (float)B[6];
+    if (ti <= 0) ti = 1.0f;
+    ti /= fn;
+    for (int a = 0; a < 3; ++a) R[a] = (float)B[3 + a] / ti / ti / fn;
+    float ax[3] = {R[0], R[1], R[2]};
+    float theta = -std::sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]) * dt;
+    if (std::fabs(theta) > 1e-6) {
+      const float inv = 1.0f / std::sqrt(ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2]);
+      for (int a = 0; a < 3; ++a) ax[a] = ax[a] * inv;
+    } else {
+      ax[0] = 1.0f, ax[1] = 0.0f, ax[2] = 0.0f;
+      theta = 0.0f;
+    }
+    const float x = ax[0], y = ax[1], z = ax[2];
+    const float cs = cosf(theta), sn = sinf(theta);
+    const float rm[9] = {cs + (1 - cs) * x * x, (1 - cs) * x * y - sn * z, (1 - cs) * x * z + sn * y,
+                         (1 - cs) * y * x + sn * z, cs + (1 - cs) * y * y, (1 - cs) * y * z - sn * x,
+                         (1 - cs) * z * x - sn * y, (1 - cs) * z * y + sn * x, cs + (1 - cs) * z * z};
+    const V3<T> Rv = mk((T)R[0], (T)R[1], (T)R[2]), Tt = mk((T)Tv[0], (T)Tv[1], (T)Tv[2]);
+    const V3<T> rg = mk((T)g1[0], (T)g1[1], (T)g1[2]);
+    const V3<T> r0 = mk((T)g0[0], (T)g0[1], (T)g0[2]);
+    for (int64_t p = 0; p < P; ++p) {
+      if (flag[p] != SOLID) continue;
+      const V3<T> xp = P3(pos, p), u = P3(vel, p);
+      const V3<T> d = xp - rg;
+      const V3<T> vh = mk(Rv.y * d.z - Rv.z * d.y, Rv.z * d.x - Rv.x * d.z, Rv.x * d.y - Rv.y * d.x) + Tt;
+      S3(vel, p, mk((T)0.5f * (u.x + vh.x), (T)0.5f * (u.y + vh.y), (T)0.5f * (u.z + vh.z)));
+      const V3<T> t = xp - r0;
+      const V3<T> q = mk(t.x * (T)rm[0] + t.y * (T)rm[3] + t.z * (T)rm[6],
+                         t.x * (T)rm[1] + t.y * (T)rm[4] + t.z * (T)rm[7],
+                         t.x * (T)rm[2] + t.y * (T)rm[5] + t.z * (T)rm[8]);
+      S3(pos, p, q + rg);
+    }
+  }
+
   // ================= a4: node fields of the particle sources =================
   // genWaterDensfield_liquidAndGas + sumcelldens_liquidAndGas spray_k.cu:1595-1686,
   // genWaterDensfield_Gas + sumcelldens_Gas 1528-1593, genWaterDensfield_GY +
@@ -1906,7 +2000,9 @@ template <class T> struct Orc : OrcBase {
     int st = project(&loc.project);
     extrapolate_bc(false);
     g2p();
+    solid_motion();
     if (c.solo_air_enabled) solo_air_sph();
+    if (!c.fix_solids) hash_sort();  // bubblesim sorts before CollisionSolid
     advect();
     hash_sort();
     if (c.correct_position) {
@@ -2143,6 +2239,7 @@ int orc_correct_positions(orc_state* s) { return dispatch(s, [&](auto& o) { o.co
 int orc_correct_bubble_positions(orc_state* s) {
   return dispatch(s, [&](auto& o) { o.correct_bubble_pos(); return 0; });
 }
+int orc_solid_motion(orc_state* s) { return dispatch(s, [&](auto& o) { o.solid_motion(); return 0; }); }
 int orc_mc_hash_sort(orc_state* s) { return dispatch(s, [&](auto& o) { o.mc_hash_sort(); return 0; }); }
 int orc_get_mc_field(orc_state* s, double* out) {
   return dispatch(s, [&](auto& o) {

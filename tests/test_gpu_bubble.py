@@ -134,3 +134,66 @@ def test_correct_bubble_positions(pkg, scenes, orc, name, n, air_ls):
     o.correct_bubble_positions()
     same_particles(sim, o, "correct_bubble_positions")
     say("  particles moved:", int(np.count_nonzero(np.any(sim.get_particles()["pos"] != p0, axis=1))))
+
+
+@pytest.mark.parametrize("lock_y", [0, 1])
+def test_solid_motion(pkg, scenes, orc, lock_y):
+    """Rigid motion of the solid blocks (fix_solids = 0) with random solid
+    velocities: the GPU's double sums are added in another order than the
+    oracle's, so positions and velocities agree to float rounding (1e-6 of
+    their scale) rather than bit for bit; other particles bit-exact."""
+    say("solid_motion", lock_y)
+    s = scenes.make("melt_freeze", n=24)
+    cfg = s["cfg"]
+    cfg.fix_solids = 0
+    cfg.rigid_lock_y = lock_y
+    fl = s["particles"]["flag"]
+    rng = np.random.default_rng(4)
+    v = s["particles"]["vel"].copy()
+    v[fl == pkg.TYPESOLID] = rng.normal(0, 0.2, (int((fl == pkg.TYPESOLID).sum()), 3)).astype(np.float32)
+    s["particles"]["vel"] = v
+    sim = pkg.Simulator(cfg)
+    load_scene(sim, s)
+    o = orc.Oracle(cfg)
+    load_scene(o, s)
+    sim.hash_sort()
+    o.hash_sort()
+    sim.solid_motion()
+    o.solid_motion()
+    pg, po = sim.get_particles(), o.get_particles()
+    sol = pg["flag"] == pkg.TYPESOLID
+    assert sol.sum() > 100
+    bitexact(pg["pos"][~sol], po["pos"][~sol].astype(np.float32), "non-solid pos")
+    bitexact(pg["vel"][~sol], po["vel"][~sol].astype(np.float32), "non-solid vel")
+    dp = np.abs(pg["pos"][sol] - po["pos"][sol]).max()
+    dv = np.abs(pg["vel"][sol] - po["vel"][sol]).max()
+    say("  max |dpos| %.2e  max |dvel| %.2e" % (dp, dv))
+    assert dp <= 1e-6 and dv <= 1e-6
+    with pytest.raises(RuntimeError):  # the solid moved: the cell lists are stale
+        sim.mark_solo_air()
+
+
+def test_step_with_moving_solid(pkg, scenes, orc):
+    """Two steps of the melt/freeze scene with the solid blocks free to move:
+    particle counts equal, solid positions within 1e-5 of the float oracle
+    (float rounding of the rigid sums), all positions within 1e-4 rel L2."""
+    say("step with moving solid")
+    s = scenes.make("melt_freeze", n=24)
+    cfg = s["cfg"]
+    cfg.fix_solids = 0
+    cfg.tol = 1e-9
+    sim = pkg.Simulator(cfg)
+    load_scene(sim, s)
+    o = orc.Oracle(cfg)
+    load_scene(o, s)
+    for f in range(2):
+        gi = sim.step(f)
+        oi = o.step(f, allow_no_convergence=True)
+        assert gi["particles"] == oi["particles"]
+    pg, po = sim.get_particles(), o.get_particles()
+    sol = (pg["flag"] == pkg.TYPESOLID) & (po["flag"] == pkg.TYPESOLID)
+    d = np.abs(pg["pos"][sol] - po["pos"][sol]).max()
+    from helpers import rel_l2
+    e = rel_l2(pg["pos"], po["pos"])
+    say("  solid max |dpos| %.2e, positions rel L2 %.2e" % (d, e))
+    assert d <= 1e-5 and e <= 1e-4

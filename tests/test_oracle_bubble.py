@@ -345,3 +345,80 @@ def test_correct_bubble_air_levelset_variant(pkg, orc):
     assert np.isclose(z[0], za + rs, rtol=0, atol=1e-12)
     assert np.isclose(z[1], float(pos[1, 2]), rtol=0, atol=1e-15)
     assert np.isclose(z[2], z0, rtol=0, atol=1e-12)
+
+
+# --------------------------------------------------------------- rigid solid motion
+def solid_oracle(pkg, orc, pos, vel, flag, **kw):
+    n = 16
+    cfg = cfg_of(pkg, n, fix_solids=0, **kw)
+    o = orc.Oracle(cfg, use_double=True)
+    o.set_particles(len(pos), pos=np.array(pos, F32), vel=np.array(vel, F32), flag=np.array(flag, np.int8))
+    o.solid_motion()
+    p = o.get_particles()
+    return cfg, p["pos"], p["vel"]
+
+
+def test_solid_motion_translation(pkg, orc):
+    """A solid whose particles all move with T0 (away from the walls):
+    the centre moves by dt T0, the rotation estimate is 0 (sum r = 0), so
+    every particle moves by dt T0 and keeps T0; other particles are
+    untouched."""
+    rng = np.random.default_rng(8)
+    pos = (np.array([8.0, 8.0, 8.0]) + rng.uniform(-2, 2, (40, 3))) * H
+    T0 = np.array([0.3, -0.2, 0.1])
+    vel = np.tile(T0, (40, 1))
+    flag = [pkg.TYPESOLID] * 39 + [pkg.TYPEFLUID]
+    cfg, p, v = solid_oracle(pkg, orc, pos, vel, flag)
+    x0 = np.array(pos, F32).astype(float)
+    t0 = np.float32(T0).astype(float)
+    assert np.allclose(p[:39], x0[:39] + cfg.dt * t0, rtol=0, atol=1e-7)
+    assert np.allclose(v[:39], t0, rtol=0, atol=1e-7)
+    assert np.array_equal(p[39], x0[39]) and np.array_equal(v[39], t0)
+
+
+def test_solid_motion_ring_rotation(pkg, orc):
+    """Particles on a ring of radius a about the z axis through c, moving
+    with v = w z x r: sum r x v = n a^2 w and the mean |r| is a, so the
+    rotation estimate is exactly w; positions turn by w dt about c and
+    velocities stay the rigid ones."""
+    a, w = 3.0 * H, 2.5
+    c = np.array([8.0, 8.0, 8.0]) * H
+    ph = np.linspace(0, 2 * np.pi, 24, endpoint=False)
+    r = np.stack([a * np.cos(ph), a * np.sin(ph), np.zeros_like(ph)], 1)
+    pos = c + r
+    vel = np.stack([-w * r[:, 1], w * r[:, 0], np.zeros_like(ph)], 1)
+    cfg, p, v = solid_oracle(pkg, orc, pos, vel, [pkg.TYPESOLID] * 24)
+    x0 = np.array(pos, F32).astype(float)
+    cen = x0.mean(0)
+    th = w * float(F32(cfg.dt))
+    rot = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    want = (x0 - cen) @ rot.T + cen
+    assert np.max(np.abs(p - want)) < 2e-6 * a  # float host arithmetic of the reference
+    v0 = np.array(vel, F32).astype(float)
+    assert np.max(np.abs(v - v0)) < 1e-5 * w * a
+
+
+def test_solid_motion_wall_push_and_fixed(pkg, orc):
+    """A solid particle below gmin + h + 0.3 ss takes (tmin - x) bounce n
+    on that velocity component before the sums (n solid particles), which
+    moves the body's translation by that over n; with fix_solids the solid
+    velocities are zeroed and nothing moves; rigid_lock_y drops T.y."""
+    n_in = 4
+    base = (np.array([8.0, 8.0, 8.0]) + np.array([[0, 0, 0], [0.5, 0, 0], [0, 0.5, 0], [0, 0, 0.5]])) * H
+    low = np.array([[8.0 * H, 8.0 * H, 0.5 * H]])
+    pos = np.concatenate([base, low])
+    vel = np.zeros((5, 3))
+    cfg, p, v = solid_oracle(pkg, orc, pos, vel, [pkg.TYPESOLID] * 5)
+    tmin = float(F32(cfg.dx + 0.3 * cfg.samplespace))
+    push = (tmin - float(F32(0.5 * H))) * float(F32(cfg.solid_bounce)) * 5
+    Tz = push / 5
+    assert np.isclose(p[:, 2].mean() - np.array(pos, F32)[:, 2].astype(float).mean(), cfg.dt * Tz, rtol=1e-4)
+    cfg2, p2, v2 = solid_oracle(pkg, orc, pos, np.ones((5, 3)), [pkg.TYPESOLID] * 4 + [pkg.TYPEFLUID], rigid_lock_y=1)
+    assert np.allclose(p2[:4, 1], np.array(pos, F32)[:4, 1], rtol=0, atol=1e-7)  # no y translation
+    n = 16
+    cfg3 = cfg_of(pkg, n, fix_solids=1)
+    o = orc.Oracle(cfg3, use_double=True)
+    o.set_particles(5, pos=np.array(pos, F32), vel=np.ones((5, 3), F32), flag=np.array([pkg.TYPESOLID] * 5, np.int8))
+    o.solid_motion()
+    q = o.get_particles()
+    assert np.all(q["vel"] == 0) and np.array_equal(q["pos"], np.array(pos, F32).astype(float))
