@@ -60,6 +60,7 @@ ABI_FUNCTIONS = [
     "mpf_delete_particles", "mpf_pour", "mpf_mc_hash_sort", "mpf_get_mc_field", "mpf_get_mc_cells",
     "mpf_mark_solo_air", "mpf_surface_tension", "mpf_drag_force", "mpf_update_solo_air",
     "mpf_correct_positions", "mpf_correct_bubble_positions", "mpf_solid_motion",
+    "mpf_get_mesh_normals", "mpf_write_mesh", "mpf_read_point_cloud",
 ]
 
 
@@ -99,6 +100,7 @@ class Config(ctypes.Structure):
         ("pour_every", ctypes.c_int32), ("pour_pos_rand", ctypes.c_float), ("pour_vel_rand", ctypes.c_float),
         ("pour_temperature", ctypes.c_float), ("seed_update_every", ctypes.c_int32),
         ("mc_source", ctypes.c_int32), ("mc_gas_airsolo", ctypes.c_int32), ("mc_every", ctypes.c_int32),
+        ("mc_smooth_iters", ctypes.c_int32),
         ("solo_air_enabled", ctypes.c_int32), ("solo_tol_in", ctypes.c_float), ("solo_tol_out", ctypes.c_float),
         ("sph_viscosity", ctypes.c_float), ("surface_tension_sigma", ctypes.c_float),
         ("drag_enabled", ctypes.c_int32), ("drag_solo", ctypes.c_float), ("drag_grid", ctypes.c_float),
@@ -180,6 +182,9 @@ def lib():
     L.mpf_marching_cubes.argtypes = [vp, c.POINTER(McInfo)]
     L.mpf_get_mesh.argtypes = [vp, vp, vp, i32]
     L.mpf_get_mc_cases.argtypes = [vp, vp, i32]
+    L.mpf_get_mesh_normals.argtypes = [vp, vp, i32]
+    L.mpf_write_mesh.argtypes = [vp, c.c_char_p, i32, c.c_char_p]
+    L.mpf_read_point_cloud.argtypes = [c.c_char_p, i64, vp, c.POINTER(i64)]
     L.mpf_mc_hash_sort.argtypes = [vp]
     L.mpf_get_mc_field.argtypes = [vp, vp, i32]
     L.mpf_get_mc_cells.argtypes = [vp, vp, vp, i32]
@@ -216,6 +221,23 @@ def default_config(nx, ny, nz, **overrides):
         else:
             setattr(cfg, k, v)
     return cfg
+
+
+MESH_POVRAY, MESH_OBJ, MESH_BOBJ_GZ = 0, 1, 2
+
+
+def read_point_cloud(path, cap=None):
+    """readdata_solid (LoadSolidData.cpp:12-59): the points of an .asc file, float64[n, 3]."""
+    L = lib()
+    n = ctypes.c_int64(0)
+    st = L.mpf_read_point_cloud(str(path).encode(), 0, None, ctypes.byref(n))
+    if st != OK:
+        raise MpfError(st, L.mpf_last_error(None).decode())
+    out = np.empty((n.value if cap is None else min(cap, n.value), 3))
+    st = L.mpf_read_point_cloud(str(path).encode(), len(out), _ptr(out), ctypes.byref(n))
+    if st != OK:
+        raise MpfError(st, L.mpf_last_error(None).decode())
+    return out
 
 
 def mc_voxels(cfg):
@@ -429,6 +451,15 @@ class Simulator:
         t = np.empty((ntris, 3), dtype=np.uint32)
         self._check(self.L.mpf_get_mesh(self.h, _ptr(v), _ptr(t), HOST))
         return v, t
+
+    def get_mesh_normals(self, nverts):
+        a = np.empty((nverts, 3), dtype=np.float32)
+        self._check(self.L.mpf_get_mesh_normals(self.h, _ptr(a), HOST))
+        return a
+
+    def write_mesh(self, path, fmt, object_name="water"):
+        """outputPovRaywater / outputOBJwater / outputBOBJwater of the last mesh."""
+        self._check(self.L.mpf_write_mesh(self.h, str(path).encode(), fmt, object_name.encode()))
 
     def get_mc_cases(self):
         n = mc_voxels(self.cfg)

@@ -125,6 +125,7 @@ void free_all(Ctx& c) {
   }
   dfree(c.scan_blk); dfree(c.ctmp_u32); dfree(c.p2g_parts);
   dfree(c.mc_case); dfree(c.mc_edge); dfree(c.mc_tcount); dfree(c.mc_verts); dfree(c.mc_tris);
+  dfree(c.mc_normals); dfree(c.mc_vedge);
   dfree(c.mc_count); dfree(c.mc_cstart); dfree(c.mc_cend); dfree(c.mc_field);
   dfree(c.seeds);
   dfree(c.pour_pos);
@@ -201,7 +202,8 @@ bool alloc_all(Ctx& c) {
   if (ok && c.cfg.mc_enabled) {
     ok = ok && dalloc(c, c.mc_case, V > 0 ? V : 1) && dalloc(c, c.mc_edge, 6 * MN) &&
          dalloc(c, c.mc_tcount, 2 * (V > 0 ? V : 1)) && dalloc(c, c.mc_verts, 3 * c.cfg.mc_max_verts) &&
-         dalloc(c, c.mc_tris, 3 * c.cfg.mc_max_tris);
+         dalloc(c, c.mc_tris, 3 * c.cfg.mc_max_tris) && dalloc(c, c.mc_normals, 3 * c.cfg.mc_max_verts) &&
+         dalloc(c, c.mc_vedge, c.cfg.mc_max_verts);
     if (c.cfg.mc_source != MPF_MC_LEVELSET)
       ok = ok && dalloc(c, c.mc_count, c.g2.G) && dalloc(c, c.mc_cstart, c.g2.G) && dalloc(c, c.mc_cend, c.g2.G) &&
            dalloc(c, c.mc_field, MN);
@@ -362,6 +364,7 @@ mpf_status mpf_config_default(mpf_config* cfg, int32_t nx, int32_t ny, int32_t n
   cfg->mc_source = MPF_MC_LEVELSET;
   cfg->mc_gas_airsolo = 1;
   cfg->mc_every = 1;
+  cfg->mc_smooth_iters = 5;                  // m_bSmoothMC, smoothIterTimes (spray.cpp:62, 610)
   cfg->solo_air_enabled = 1;
   cfg->solo_tol_in = -1.45f;                 // spray_k.cu:4245
   cfg->solo_tol_out = -0.5f;
@@ -929,6 +932,27 @@ mpf_status mpf_get_mesh(mpf_ctx* ctx, float* verts, uint32_t* tris, int32_t wher
   if (verts && c.mc_nv) hipMemcpyAsync(verts, c.mc_verts, (size_t)c.mc_nv * 12, kind, c.stream);
   if (tris && c.mc_nt) hipMemcpyAsync(tris, c.mc_tris, (size_t)c.mc_nt * 12, kind, c.stream);
   return hip_ok(c, hipStreamSynchronize(c.stream), "get_mesh") ? MPF_OK : MPF_HIP_ERROR;
+}
+
+mpf_status mpf_write_mesh(mpf_ctx* ctx, const char* path, int32_t format, const char* object_name) {
+  if (!ctx) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  if (!path) return fail(c, MPF_INVALID_ARG, "null path");
+  if (format < MPF_MESH_POVRAY || format > MPF_MESH_BOBJ_GZ) return fail(c, MPF_INVALID_ARG, "unknown mesh format");
+  if (!c.mc_verts) return fail(c, MPF_INVALID_ARG, "marching cubes disabled in the configuration");
+  return (mpf_status)run_write_mesh(c, path, format, object_name);
+}
+
+mpf_status mpf_get_mesh_normals(mpf_ctx* ctx, float* normals, int32_t where) {
+  if (!ctx || !normals) return MPF_INVALID_ARG;
+  Ctx& c = ctx->c;
+  c.err.clear();
+  if (where != MPF_HOST && where != MPF_DEVICE) return fail(c, MPF_INVALID_ARG, "bad memory location");
+  if (!c.mc_normals) return fail(c, MPF_INVALID_ARG, "marching cubes disabled in the configuration");
+  hipMemcpyKind kind = where == MPF_HOST ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  if (c.mc_nv) hipMemcpyAsync(normals, c.mc_normals, (size_t)c.mc_nv * 12, kind, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "get_mesh_normals") ? MPF_OK : MPF_HIP_ERROR;
 }
 
 mpf_status mpf_get_mc_cases(mpf_ctx* ctx, uint8_t* cases, int32_t where) {

@@ -1238,7 +1238,8 @@ __global__ void k_mc_classify(McGrid m, const float* __restrict__ f, uint8_t* __
 // reference writes a shared edge's vertex from whichever voxel comes last;
 // here every edge is interpolated once, in the + direction.
 __global__ void k_mc_verts(McGrid m, const float* __restrict__ f, const uint32_t* __restrict__ used,
-                           const uint32_t* __restrict__ vid, float* __restrict__ verts) {
+                           const uint32_t* __restrict__ vid, float* __restrict__ verts,
+                           uint32_t* __restrict__ vedge) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= 3 * m.N || !used[e]) return;
   const int64_t n = e / 3;
@@ -1254,6 +1255,7 @@ __global__ void k_mc_verts(McGrid m, const float* __restrict__ f, const uint32_t
               p0z = ((float)k + m.off) * h + m.z0;
   const float p1x = a == 0 ? p0x + h : p0x, p1y = a == 1 ? p0y + h : p0y, p1z = a == 2 ? p0z + h : p0z;
   const uint32_t o = vid[e];
+  vedge[o] = (uint32_t)e;
   verts[3 * (int64_t)o + 0] = p0x + t * (p1x - p0x);
   verts[3 * (int64_t)o + 1] = p0y + t * (p1y - p0y);
   verts[3 * (int64_t)o + 2] = p0z + t * (p1z - p0z);
@@ -1274,6 +1276,107 @@ __global__ void k_mc_tris(McGrid m, const uint8_t* __restrict__ cases, const uin
     int e = c_tri[ci][t];
     tris[w + t] = vid[3 * nid(m, i + c_eo[e][0], j + c_eo[e][1], k + c_eo[e][2]) + c_eo[e][3]];
   }
+}
+
+// The triangles around one vertex, in the order of the triangle list: the
+// vertex lies on edge e, which the (at most) 4 voxels around it share; their
+// index order is the list's order, and within a voxel its triangles follow
+// the table.  fn(i1, i2, i3) sees every triangle that holds the vertex.
+template <class Fn>
+__device__ __forceinline__ void mc_vertex_triangles(const McGrid& m, uint32_t e, const uint8_t* __restrict__ cases,
+                                                    const uint32_t* __restrict__ toff,
+                                                    const uint32_t* __restrict__ tris, uint32_t v, Fn fn) {
+  const int64_t n = e / 3;
+  const int slot = (int)(e % 3);
+  int i, j, k;
+  ijk3(n, m.ny, m.nz, i, j, k);
+  const int VX = m.nx - 1, VY = m.ny - 1, VZ = m.nz - 1;
+  for (int q = 0; q < 4; ++q) {
+    const int a = q >> 1, b = q & 1;
+    int vi, vj, vk;
+    if (slot == 1) { vi = i; vj = j - 1 + a; vk = k - 1 + b; }       // x edge
+    else if (slot == 0) { vi = i - 1 + a; vj = j; vk = k - 1 + b; }  // y edge
+    else { vi = i - 1 + a; vj = j - 1 + b; vk = k; }                  // z edge
+    if (vi < 0 || vi >= VX || vj < 0 || vj >= VY || vk < 0 || vk >= VZ) continue;
+    const int64_t vox = ((int64_t)vi * VY + vj) * VZ + vk;
+    const int nt = c_nv[cases[vox]] / 3;
+    const int64_t t0 = toff[vox];
+    for (int t = 0; t < nt; ++t) {
+      const uint32_t i1 = tris[3 * (t0 + t)], i2 = tris[3 * (t0 + t) + 1], i3 = tris[3 * (t0 + t) + 2];
+      if (i1 == v || i2 == v || i3 == v) fn(i1, i2, i3);
+    }
+  }
+}
+
+// calNormals, spray.cu:981-997: the sum of cross(p2 - p1, p3 - p1) over the
+// vertex's triangles (calnormal_k, spray_k.cu:2223-2250, adds them with
+// atomics; here in the list's order), normalised when nonzero
+// (normalizeTriangleNor_k, 2252-2261).
+__global__ void k_mc_normals(McGrid m, const uint32_t* __restrict__ vedge, const uint8_t* __restrict__ cases,
+                             const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tris,
+                             const float* __restrict__ verts, float* __restrict__ nrm, int64_t nv) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+  mc_vertex_triangles(m, vedge[v], cases, toff, tris, (uint32_t)v, [&](uint32_t i1, uint32_t i2, uint32_t i3) {
+    const float* a = verts + 3 * (int64_t)i1;
+    const float* b = verts + 3 * (int64_t)i2;
+    const float* c = verts + 3 * (int64_t)i3;
+    const float ux = b[0] - a[0], uy = b[1] - a[1], uz = b[2] - a[2];
+    const float wx = c[0] - a[0], wy = c[1] - a[1], wz = c[2] - a[2];
+    nx = nx + (uy * wz - uz * wy);
+    ny = ny + (uz * wx - ux * wz);
+    nz = nz + (ux * wy - uy * wx);
+  });
+  const float len = sqrtf(nx * nx + ny * ny + nz * nz);
+  if (len > 0.0f) {
+    const float inv = 1.0f / len;
+    nx = nx * inv;
+    ny = ny * inv;
+    nz = nz * inv;
+  }
+  nrm[3 * v] = nx;
+  nrm[3 * v + 1] = ny;
+  nrm[3 * v + 2] = nz;
+}
+
+// smoothMesh, spray.cu:1271-1288, one iteration: each vertex gathers the
+// edge vectors to the other two corners of its triangles (in the order of
+// smooth_computedisplacement, spray_k.cu:2408-2427) and moves by
+// lambda * sum / count (smooth_addDisplacement, 2429-2440).  The
+// reference's second, mu = -0.53 pass finds the sums already cleared by the
+// first and does nothing; so here.  Positions of the pass's input are read.
+__global__ void k_mc_smooth(McGrid m, const uint32_t* __restrict__ vedge, const uint8_t* __restrict__ cases,
+                            const uint32_t* __restrict__ toff, const uint32_t* __restrict__ tris,
+                            const float* __restrict__ vin, float* __restrict__ vout, int64_t nv, float lambda) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  float dx = 0.0f, dy = 0.0f, dz = 0.0f;
+  int w = 0;
+  const float* pv = vin + 3 * v;
+  mc_vertex_triangles(m, vedge[v], cases, toff, tris, (uint32_t)v, [&](uint32_t i1, uint32_t i2, uint32_t i3) {
+    const uint32_t o1 = (i1 == (uint32_t)v) ? i2 : i1;
+    const uint32_t o2 = (i3 == (uint32_t)v) ? i2 : i3;
+    const float* a = vin + 3 * (int64_t)o1;
+    const float* b = vin + 3 * (int64_t)o2;
+    dx = dx + (a[0] - pv[0]);
+    dy = dy + (a[1] - pv[1]);
+    dz = dz + (a[2] - pv[2]);
+    dx = dx + (b[0] - pv[0]);
+    dy = dy + (b[1] - pv[1]);
+    dz = dz + (b[2] - pv[2]);
+    w += 2;
+  });
+  float x = pv[0], y = pv[1], z = pv[2];
+  if (w > 0) {
+    const float fw = (float)w;
+    x = x + lambda * dx / fw;
+    y = y + lambda * dy / fw;
+    z = z + lambda * dz / fw;
+  }
+  vout[3 * v] = x;
+  vout[3 * v + 1] = y;
+  vout[3 * v + 2] = z;
 }
 
 // Node field of a particle source, one thread per node of the (2N+1)^3
@@ -1729,9 +1832,25 @@ int run_marching_cubes(Ctx& c, mpf_mc_info* info) {
     return MPF_CAPACITY;
   }
   launch(c, "mc_verts", dim3(nblocks(3 * N, 256)), dim3(256), 0, k_mc_verts, m, f, (const uint32_t*)c.mc_edge,
-         (const uint32_t*)vid, c.mc_verts);
+         (const uint32_t*)vid, c.mc_verts, c.mc_vedge);
   launch(c, "mc_tris", dim3(nblocks(V, 256)), dim3(256), 0, k_mc_tris, m, (const uint8_t*)c.mc_case,
          (const uint32_t*)(c.mc_tcount + V), (const uint32_t*)vid, c.mc_tris);
+  // runMC_smooth step 8 (spray.cu:948-951): smoothing, then vertex normals
+  // (the normal buffer is the smoothing's second vertex buffer before that)
+  {
+    const uint32_t* toff = c.mc_tcount + V;
+    float* a = c.mc_verts;
+    float* b = c.mc_normals;
+    for (int it = 0; it < c.cfg.mc_smooth_iters; ++it) {
+      launch(c, "mc_smooth", dim3(nblocks(nv, 256)), dim3(256), 0, k_mc_smooth, m, (const uint32_t*)c.mc_vedge,
+             (const uint8_t*)c.mc_case, toff, (const uint32_t*)c.mc_tris, (const float*)a, b, nv, 0.5f);
+      std::swap(a, b);
+    }
+    if (a != c.mc_verts && nv > 0)
+      hipMemcpyAsync(c.mc_verts, a, (size_t)nv * 12, hipMemcpyDeviceToDevice, c.stream);
+    launch(c, "mc_normals", dim3(nblocks(nv, 256)), dim3(256), 0, k_mc_normals, m, (const uint32_t*)c.mc_vedge,
+           (const uint8_t*)c.mc_case, toff, (const uint32_t*)c.mc_tris, (const float*)c.mc_verts, c.mc_normals, nv);
+  }
   c.mc_nv = nv;
   c.mc_nt = nt;
   if (info) {

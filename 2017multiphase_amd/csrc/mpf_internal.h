@@ -204,6 +204,8 @@ struct Ctx {
   uint32_t* mc_edge = nullptr;  // 3G flags -> vertex ids
   uint32_t* mc_tcount = nullptr;  // per voxel triangle offsets
   float* mc_verts = nullptr;
+  float* mc_normals = nullptr;   // per vertex (also the smoothing's second buffer)
+  uint32_t* mc_vedge = nullptr;  // vertex -> edge id
   uint32_t* mc_tris = nullptr;
   int64_t mc_nv = 0, mc_nt = 0, mc_nvox = 0;
   // particle-source marching cubes (cfg.mc_source != MPF_MC_LEVELSET): the
@@ -295,6 +297,8 @@ void run_correct_bubble_pos(Ctx& c);
 void sph_constants(float h, float& poly6, float& spiky, float& lap);
 // solid.hip
 int run_solid_motion(Ctx& c);
+// io.hip
+int run_write_mesh(Ctx& c, const char* path, int format, const char* object_name);
 void run_solid_phi_and_velocity(Ctx& c);
 int64_t p2g_parts_floats(const Grid& g);
 // Device counters: 64 single counters, then SPREAD counters 64 bytes apart

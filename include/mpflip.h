@@ -183,6 +183,8 @@ typedef struct mpf_config {
                                 reference's SCENE_INTERACTION uses 0)         */
   int32_t mc_every;          /* mpf_step meshes frames that are multiples of
                                 this (1; the reference's outputframeDelta is 2) */
+  int32_t mc_smooth_iters;   /* smoothMesh iterations after meshing (5;
+                                m_bSmoothMC / smoothIterTimes, spray.cpp:62, 610) */
   /* remaining bubblesim stages (spray.cpp:1059-1150), reference defaults */
   int32_t solo_air_enabled;  /* markSoloAirParticle + updateSoloAirParticle (1) */
   float solo_tol_in, solo_tol_out; /* verifySoloAirParticle tol1, tol2 in cells
@@ -357,8 +359,29 @@ mpf_status mpf_mc_hash_sort(mpf_ctx* ctx);
    numbered in the order of GetEdgeID (spray_k.cu:1925-1957), so vertex and
    triangle indices follow the reference's numbering. */
 mpf_status mpf_marching_cubes(mpf_ctx* ctx, mpf_mc_info* info);
-/* Copy the last mesh out: verts float[3*nv], tri indices uint32[3*nt]. */
+/* Copy the last mesh out: verts float[3*nv], tri indices uint32[3*nt].  The
+   mesh is smoothed cfg.mc_smooth_iters times (smoothMesh, spray.cu:1271-1288)
+   and given vertex normals (calNormals, spray.cu:981-997), each vertex
+   summing its triangles in the order of the triangle list. */
 mpf_status mpf_get_mesh(mpf_ctx* ctx, float* verts, uint32_t* tris, int32_t where);
+mpf_status mpf_get_mesh_normals(mpf_ctx* ctx, float* normals, int32_t where);
+
+/* Mesh files of the last mesh (host I/O after the mesh is copied out). */
+enum {
+  MPF_MESH_POVRAY = 0,  /* outputPovRaywater, spray.cu:2250-2310: a mesh2 object */
+  MPF_MESH_OBJ = 1,     /* outputOBJwater, spray.cu:2312-2372                    */
+  MPF_MESH_BOBJ_GZ = 2  /* outputBOBJwater, spray.cu:2373-2435 (gzip level 1)    */
+};
+/* Writes the mesh (vertices, normals, faces) to path in the given format;
+   object_name names the POV-Ray material (may be NULL: "water").  Non-finite
+   vertex or normal components are written as 0 (verifyfloat3).  MPF_IO when
+   the file cannot be written. */
+mpf_status mpf_write_mesh(mpf_ctx* ctx, const char* path, int32_t format, const char* object_name);
+/* readdata_solid, LoadSolidData.cpp:12-59: an .asc point cloud (first line
+   the point count, then one line per point: an index and three coordinates).
+   Up to cap points are stored in xyz (double[3*cap]); *n gets the count the
+   file declares.  Context-free: mpf_last_error(NULL) explains a failure. */
+mpf_status mpf_read_point_cloud(const char* path, int64_t cap, double* xyz, int64_t* n);
 /* Per-voxel case index of the last mesh (uint8): (nx-1)(ny-1)(nz-1) voxels
    over cell centres, or 2nx*2ny*2nz for a particle source. */
 mpf_status mpf_get_mc_cases(mpf_ctx* ctx, uint8_t* cases, int32_t where);

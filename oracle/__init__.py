@@ -66,6 +66,7 @@ def lib():
     L.orc_get_mesh.argtypes = [vp, vp, vp]
     L.orc_get_mc_cases.argtypes = [vp, vp]
     L.orc_mc_hash_sort.argtypes = [vp]
+    L.orc_get_mesh_normals.argtypes = [vp, vp]
     for n in ("mark_solo_air", "surface_tension", "drag_force", "update_solo_air", "correct_positions",
               "correct_bubble_positions", "solid_motion"):
         getattr(L, "orc_" + n).argtypes = [vp]
@@ -283,6 +284,11 @@ class Oracle:
         t = np.empty((ntris, 3), np.uint32)
         self._ok(self.L.orc_get_mesh(self.h, _p(v), _p(t)))
         return v, t
+
+    def get_mesh_normals(self, nverts):
+        a = np.empty((nverts, 3))
+        self._ok(self.L.orc_get_mesh_normals(self.h, _p(a)))
+        return a
 
     def get_mc_cases(self):
         a = np.empty(self.pkg.mc_voxels(self.cfg), np.uint8)

@@ -114,7 +114,7 @@ template <class T> struct Orc : OrcBase {
   std::vector<T> mc_field;
   // marching cubes output
   std::vector<uint8_t> mc_case;
-  std::vector<T> mc_verts;
+  std::vector<T> mc_verts, mc_normals;
   std::vector<uint32_t> mc_tris;
 
   explicit Orc(const mpf_config& cfg) {
@@ -1979,8 +1979,47 @@ template <class T> struct Orc : OrcBase {
         mc_tris[w++] = vid[3 * nid(i + eo[e][0], j + eo[e][1], k + eo[e][2]) + eo[e][3]];
       }
     }
+    mesh_smooth_and_normals(nv, nverts_tri / 3);
     if (info) { info->active_voxels = active; info->verts = nv; info->tris = nverts_tri / 3; }
     return MPF_OK;
+  }
+
+  // runMC_smooth step 8 (spray.cu:948-951): smoothMesh (spray.cu:1271-1288,
+  // smooth_computedisplacement / smooth_addDisplacement spray_k.cu:2408-2440;
+  // the mu = -0.53 pass finds the sums cleared and does nothing) and
+  // calNormals (spray.cu:981-997, calnormal_k / normalizeTriangleNor_k
+  // spray_k.cu:2223-2261), the per-vertex sums taken in triangle order
+  void mesh_smooth_and_normals(int64_t nv, int64_t nt) {
+    const T lambda = (T)0.5f;
+    for (int it = 0; it < c.mc_smooth_iters; ++it) {
+      std::vector<T> d((size_t)3 * nv, T(0));
+      std::vector<int> w(nv, 0);
+      auto add = [&](uint32_t v, uint32_t o) {
+        for (int a = 0; a < 3; ++a) d[3 * v + a] = d[3 * v + a] + (mc_verts[3 * o + a] - mc_verts[3 * v + a]);
+      };
+      for (int64_t t = 0; t < nt; ++t) {
+        const uint32_t p1 = mc_tris[3 * t], p2 = mc_tris[3 * t + 1], p3 = mc_tris[3 * t + 2];
+        add(p1, p2); add(p1, p3); w[p1] += 2;
+        add(p2, p1); add(p2, p3); w[p2] += 2;
+        add(p3, p1); add(p3, p2); w[p3] += 2;
+      }
+      for (int64_t v = 0; v < nv; ++v)
+        if (w[v] > 0)
+          for (int a = 0; a < 3; ++a) mc_verts[3 * v + a] = mc_verts[3 * v + a] + lambda * d[3 * v + a] / (T)w[v];
+    }
+    mc_normals.assign((size_t)3 * nv, T(0));
+    for (int64_t t = 0; t < nt; ++t) {
+      const uint32_t i1 = mc_tris[3 * t], i2 = mc_tris[3 * t + 1], i3 = mc_tris[3 * t + 2];
+      const V3<T> a = P3(mc_verts, i1), b = P3(mc_verts, i2), cc = P3(mc_verts, i3);
+      const V3<T> u = b - a, w = cc - a;
+      const V3<T> n = mk(u.y * w.z - u.z * w.y, u.z * w.x - u.x * w.z, u.x * w.y - u.y * w.x);
+      for (uint32_t q : {i1, i2, i3}) S3(mc_normals, q, P3(mc_normals, q) + n);
+    }
+    for (int64_t v = 0; v < nv; ++v) {
+      const V3<T> n = P3(mc_normals, v);
+      const T len = len3(n);
+      if (len > 0) S3(mc_normals, v, n * (T(1) / len));
+    }
   }
 
   // ================= composite step (spray.cpp:979-1233, the stages of this path) =================
@@ -2251,6 +2290,12 @@ int orc_get_mc_cells(orc_state* s, uint32_t* start, uint32_t* end) {
   return dispatch(s, [&](auto& o) {
     if (start) std::memcpy(start, o.mc_cstart.data(), o.mc_cstart.size() * 4);
     if (end) std::memcpy(end, o.mc_cend.data(), o.mc_cend.size() * 4);
+    return 0;
+  });
+}
+int orc_get_mesh_normals(orc_state* s, double* normals) {
+  return dispatch(s, [&](auto& o) {
+    for (size_t i = 0; i < o.mc_normals.size(); ++i) normals[i] = (double)o.mc_normals[i];
     return 0;
   });
 }

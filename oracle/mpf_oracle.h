@@ -65,6 +65,8 @@ int orc_update_solo_air(orc_state* s);
 int orc_correct_positions(orc_state* s);
 int orc_correct_bubble_positions(orc_state* s);
 int orc_solid_motion(orc_state* s);
+/* Vertex normals of the last mesh (double[3*nv]). */
+int orc_get_mesh_normals(orc_state* s, double* normals);
 /* preMC: the sort on the 2N grid with its own cell lists (2nx*2ny*2nz). */
 int orc_mc_hash_sort(orc_state* s);
 /* Node field of the last particle-source mesh ((2nx+1)(2ny+1)(2nz+1) doubles)

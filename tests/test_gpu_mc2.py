@@ -108,3 +108,75 @@ def test_step_with_particle_source(pkg, scenes, orc, name, n):
     oi = o.marching_cubes()
     assert info["mc"] == oi
     bitexact(sim.get_mc_cases(), o.get_mc_cases(), "case index")
+
+
+@pytest.mark.parametrize("src", [0, 1])
+def test_mesh_smoothing_and_normals(pkg, scenes, orc, src):
+    """smoothMesh (5 iterations) and calNormals on the GPU against the float
+    oracle: vertices and normals bit-exact (each vertex sums its triangles in
+    the order of the triangle list on both sides)."""
+    say("mesh smoothing and normals", src)
+    s, sim = make(pkg, scenes, "bubble_rise", 16, src)
+    assert s["cfg"].mc_smooth_iters == 5
+    o = orc.Oracle(s["cfg"])
+    load_scene(o, s)
+    if src == 0:
+        sim.hash_sort()
+        sim.levelset_update()
+        o.set_field("LSFLUID", sim.get_field("LSFLUID"))
+    else:
+        sim.mc_hash_sort()
+        o.mc_hash_sort()
+    gi = sim.marching_cubes()
+    oi = o.marching_cubes()
+    assert gi == oi and gi["tris"] > 0
+    vg, tg = sim.get_mesh(gi["verts"], gi["tris"])
+    vo, to = o.get_mesh(oi["verts"], oi["tris"])
+    bitexact(tg, to, "triangles")
+    bitexact(vg, vo.astype(np.float32), "smoothed vertices")
+    bitexact(sim.get_mesh_normals(gi["verts"]), o.get_mesh_normals(oi["verts"]).astype(np.float32), "normals")
+
+
+def test_mesh_files(pkg, scenes, tmp_path):
+    """outputPovRaywater / outputOBJwater / outputBOBJwater layouts, read
+    back: counts, vertices and normals (%f text: 1e-6; bobj: bit for bit),
+    face indices (OBJ from 1)."""
+    import gzip
+    import re
+    say("mesh files")
+    s, sim = make(pkg, scenes, "bubble_rise", 16, 0)
+    sim.hash_sort()
+    sim.levelset_update()
+    info = sim.marching_cubes()
+    nv, nt = info["verts"], info["tris"]
+    v, t = sim.get_mesh(nv, nt)
+    nrm = sim.get_mesh_normals(nv)
+    sim.write_mesh(tmp_path / "w.pov", pkg.MESH_POVRAY, "water")
+    sim.write_mesh(tmp_path / "w.obj", pkg.MESH_OBJ)
+    sim.write_mesh(tmp_path / "w.bobj.gz", pkg.MESH_BOBJ_GZ)
+    txt = open(tmp_path / "w.obj").read().splitlines()
+    V = np.array([[float(x) for x in l.split()[1:]] for l in txt if l.startswith("v ")])
+    N = np.array([[float(x) for x in l.split()[1:]] for l in txt if l.startswith("vn ")])
+    F = np.array([[int(x) for x in l.split()[1:]] for l in txt if l.startswith("f ")])
+    assert V.shape == (nv, 3) and N.shape == (nv, 3) and F.shape == (nt, 3)
+    assert np.max(np.abs(V - v)) <= 1e-6 and np.max(np.abs(N - nrm)) <= 1e-6
+    assert np.array_equal(F - 1, t.astype(np.int64))
+    pov = open(tmp_path / "w.pov").read()
+    assert pov.startswith("#declare watermesh=mesh2{") and "material{water_material}" in pov
+    blocks = re.findall(r"(vertex_vectors|normal_vectors|face_indices)\{\n(\d+),\n(.*?)\}", pov, re.S)
+    assert [b[0] for b in blocks] == ["vertex_vectors", "normal_vectors", "face_indices"]
+    assert [int(b[1]) for b in blocks] == [nv, nv, nt]
+    PV = np.array([[float(x) for x in m] for m in re.findall(r"<\s*([-\d.e]+),\s*([-\d.e]+),\s*([-\d.e]+)\s*>", blocks[0][2])])
+    PF = np.array([[int(x) for x in m] for m in re.findall(r"<(\d+), (\d+), (\d+)>", blocks[2][2])])
+    assert np.max(np.abs(PV - v)) <= 1e-6 and np.array_equal(PF, t.astype(np.int64))
+    raw = gzip.open(tmp_path / "w.bobj.gz").read()
+    n1 = np.frombuffer(raw[:4], np.int32)[0]
+    BV = np.frombuffer(raw[4:4 + 12 * n1], np.float32).reshape(-1, 3)
+    o = 4 + 12 * n1
+    n2 = np.frombuffer(raw[o:o + 4], np.int32)[0]
+    BN = np.frombuffer(raw[o + 4:o + 4 + 12 * n2], np.float32).reshape(-1, 3)
+    o += 4 + 12 * n2
+    ni = np.frombuffer(raw[o:o + 4], np.int32)[0]
+    BI = np.frombuffer(raw[o + 4:o + 4 + 4 * ni], np.int32).reshape(-1, 3)
+    assert n1 == nv and n2 == nv and ni == 3 * nt and len(raw) == o + 4 + 4 * ni
+    assert np.array_equal(BV, v) and np.array_equal(BN, nrm) and np.array_equal(BI, t.astype(np.int32))

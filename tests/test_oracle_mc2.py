@@ -214,7 +214,7 @@ def test_mesh_numbering_follows_getedgeid(pkg, orc):
     zero of the linear interpolant along its edge."""
     tab = ref_edge_table()
     n = 6
-    cfg = cfg_of(pkg, n, 1)
+    cfg = cfg_of(pkg, n, 1, mc_smooth_iters=0)  # vertices on their edges
     rng = np.random.default_rng(2)
     P = 400
     pos = rng.uniform(1.0 * H, (n - 1.0) * H, (P, 3)).astype(F32)
@@ -275,3 +275,79 @@ def test_step_leaves_particles_in_2n_order(pkg, scenes, orc):
     pos = o.get_particles()["pos"].reshape(-1, 3)
     k2 = np.floor(pos.astype(F32) / F32(HH)).astype(np.int64).clip(0, 31)
     assert np.array_equal((k2[:, 0] * 32 + k2[:, 1]) * 32 + k2[:, 2], keys)
+
+
+# --------------------------------------------------------------- smoothing, normals, point clouds
+def _mesh(pkg, orc, iters, sdf_fn, n=16):
+    cfg = pkg.default_config(n, n, n, mc_enabled=1, particle_capacity=10, mc_smooth_iters=iters)
+    o = orc.Oracle(cfg, use_double=True)
+    c = (np.stack(np.meshgrid(*[np.arange(n)] * 3, indexing="ij"), -1).reshape(-1, 3) + 0.5) * H
+    o.set_field("LSFLUID", sdf_fn(c))
+    info = o.marching_cubes()
+    v, t = o.get_mesh(info["verts"], info["tris"])
+    return v, t, o.get_mesh_normals(info["verts"])
+
+
+def test_mesh_smoothing_and_normals_against_numpy(pkg, orc):
+    """smoothMesh: each iteration moves every vertex by 0.5 x the mean of the
+    edge vectors to the other corners of its triangles (the mu pass is a
+    no-op, its sums being cleared); calNormals: the normalised sum of
+    cross(p2 - p1, p3 - p1) over a vertex's triangles.  Both recomputed here
+    with numpy from the unsmoothed mesh of the same level set."""
+    cen = np.array([8.1, 7.7, 8.3]) * H
+    sdf = lambda c: np.linalg.norm(c - cen, axis=1) - 4.3 * H
+    v0, t0, _ = _mesh(pkg, orc, 0, sdf)
+    v2, t2, n2 = _mesh(pkg, orc, 2, sdf)
+    assert np.array_equal(t0, t2)
+    v = v0.copy()
+    for _ in range(2):
+        d = np.zeros_like(v)
+        w = np.zeros(len(v))
+        for a, b, c in ((0, 1, 2), (1, 0, 2), (2, 0, 1)):
+            np.add.at(d, t0[:, a], v[t0[:, b]] - v[t0[:, a]])
+            np.add.at(d, t0[:, a], v[t0[:, c]] - v[t0[:, a]])
+            np.add.at(w, t0[:, a], 2)
+        v = v + 0.5 * d / w[:, None]
+    assert np.max(np.abs(v - v2)) < 1e-13
+    fn = np.cross(v[t0[:, 1]] - v[t0[:, 0]], v[t0[:, 2]] - v[t0[:, 0]])
+    nn = np.zeros_like(v)
+    for a in range(3):
+        np.add.at(nn, t0[:, a], fn)
+    nn /= np.linalg.norm(nn, axis=1)[:, None]
+    assert np.max(np.abs(nn - n2)) < 1e-12
+    r0 = np.linalg.norm(v0 - cen, axis=1).mean()
+    r2 = np.linalg.norm(v2 - cen, axis=1).mean()
+    assert r2 < r0  # Laplacian smoothing shrinks a sphere
+    assert np.all(((v2 - cen) * n2).sum(1) > 0)  # normals point out of the liquid
+
+
+def test_mesh_normals_of_a_plane(pkg, orc):
+    """A planar level set z = z0: every vertex lies on the plane, every
+    normal is exactly (0, 0, 1) (out of the liquid below), and smoothing
+    keeps z (x and y of the normals exactly 0, z 1 to rounding)."""
+    z0 = 7.3 * H
+    v, t, nrm = _mesh(pkg, orc, 5, lambda c: c[:, 2] - z0)
+    assert len(t) > 100
+    assert np.allclose(v[:, 2], z0, rtol=0, atol=1e-15)
+    assert np.all(nrm[:, :2] == 0) and np.allclose(nrm[:, 2], 1.0, rtol=0, atol=1e-15)
+
+
+def test_read_point_cloud(pkg, tmp_path):
+    """readdata_solid's .asc layout: the count, then 'index x y z' lines;
+    separators of one character after each number; a short file fails."""
+    rng = np.random.default_rng(2)
+    pts = rng.normal(0, 1, (7, 3))
+    p = tmp_path / "cloud.asc"
+    with open(p, "w") as f:
+        f.write("7\n")
+        for i, q in enumerate(pts):
+            f.write("%d %.17g %.17g %.17g\n" % (i, q[0], q[1], q[2]))
+    got = pkg.read_point_cloud(p)
+    assert got.shape == (7, 3) and np.array_equal(got, pts)
+    assert np.array_equal(pkg.read_point_cloud(p, cap=3), pts[:3])
+    with open(p, "w") as f:
+        f.write("9\n0 1 2 3\n")
+    with pytest.raises(RuntimeError):
+        pkg.read_point_cloud(p)
+    with pytest.raises(RuntimeError):
+        pkg.read_point_cloud(tmp_path / "missing.asc")
