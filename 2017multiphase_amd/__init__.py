@@ -29,7 +29,7 @@ HOST, DEVICE = 0, 1
 FP32, FP64, MIXED = 0, 1, 2
 SOLVER_REFERENCE_CG, SOLVER_JACOBI_PCG = 0, 1
 OPERATOR_UNIT, OPERATOR_VARDENS = 0, 1
-ADVECT_EULER_COLLIDE, ADVECT_RK2 = 0, 1
+ADVECT_EULER_COLLIDE, ADVECT_RK2, ADVECT_FREEZING = 0, 1, 2
 HEAT_CG, HEAT_JACOBI = 0, 1
 MC_LEVELSET, MC_LIQUID_GAS, MC_GAS, MC_SOLID = 0, 1, 2, 3
 

@@ -397,6 +397,10 @@ mpf_status mpf_create(const mpf_config* cfg, mpf_ctx** out) {
     g_noctx_err = "invalid configuration";
     return MPF_INVALID_ARG;
   }
+  if (cfg->integrator < MPF_ADVECT_EULER_COLLIDE || cfg->integrator > MPF_ADVECT_FREEZING) {
+    g_noctx_err = "invalid configuration (integrator)";
+    return MPF_INVALID_ARG;
+  }
   if (cfg->mc_source < MPF_MC_LEVELSET || cfg->mc_source > MPF_MC_SOLID || cfg->mc_every < 1) {
     g_noctx_err = "invalid marching cubes configuration (mc_source, mc_every)";
     return MPF_INVALID_ARG;

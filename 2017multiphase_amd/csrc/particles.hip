@@ -120,6 +120,84 @@ __global__ void k_collide_move(Grid g, int64_t P, f3* __restrict__ pos, f3* __re
   pos[p] = x;
 }
 
+// CollisionWithSolid_Freezing, spray_k.cu:5348-5446 (CollisionSolid's
+// integrator for the freezing and melting-pour scenes): near a solid
+// (phisolid <= 1) a particle pushed by neighbours closer than h/2 leaves
+// the solid surface along its normal by the deepest overlap with the normal
+// velocity removed, or with neighbours within h only loses 0.1 |v| along
+// the normal; then the wall bounce (-0.5 v at gmin + h + 0.3 ss) and
+// x += v dt.  Neighbour positions are the pass's input (the reference reads
+// them while they are rewritten); the result goes to the other buffer.
+__global__ void k_collide_freezing(Grid g, int64_t P, const f3* __restrict__ pin, f3* __restrict__ pout,
+                                   f3* __restrict__ vel, const int8_t* __restrict__ flag,
+                                   const float* __restrict__ phis, const uint32_t* __restrict__ cstart,
+                                   const uint32_t* __restrict__ cend, float dt, float ss) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  f3 x = pin[p];
+  if (flag[p] == SOLID) {
+    pout[p] = x;
+    return;
+  }
+  f3 v = vel[p];
+  const float h = g.h;
+  const float qx = (x.x - g.gx0) / h, qy = (x.y - g.gy0) / h, qz = (x.z - g.gz0) / h;
+  const int i = (qx >= 0 && qx < (float)g.nx) ? (int)qx : 0;
+  const int j = (qy >= 0 && qy < (float)g.ny) ? (int)qy : 0;
+  const int k = (qz >= 0 && qz < (float)g.nz) ? (int)qz : 0;
+  const float iphi = scalar_from_grid(g, x, phis);
+  if (iphi <= 1.0f) {
+    const float r = 0.25f * h, hh = 4.0f * r;
+    float depth = 0.0f;
+    int ncol = 0, nadh = 0;
+    for (int di = -1; di <= 1; ++di)
+      for (int dj = -1; dj <= 1; ++dj)
+        for (int dk = -1; dk <= 1; ++dk) {
+          if (!valid(g, i + di, j + dj, k + dk)) continue;
+          const int64_t c = cid(g, i + di, j + dj, k + dk);
+          const uint32_t s = cstart[c];
+          if (s == MPF_CELL_UNDEF) continue;
+          const uint32_t e = cend[c];
+          for (uint32_t q = s; q < e; ++q) {
+            const float dis = len3(sub3(x, pin[q]));
+            if (dis > 0 && dis < 2 * r) {
+              depth = fmaxf(depth, 2 * r - dis);
+              ncol++;
+            } else if (dis < hh) {
+              nadh++;
+            }
+          }
+        }
+    const float d = h * 0.5f;
+    f3 n;
+    n.x = scalar_from_grid(g, add3(x, mk3(d, 0.f, 0.f)), phis) - scalar_from_grid(g, sub3(x, mk3(d, 0.f, 0.f)), phis);
+    n.y = scalar_from_grid(g, add3(x, mk3(0.f, d, 0.f)), phis) - scalar_from_grid(g, sub3(x, mk3(0.f, d, 0.f)), phis);
+    n.z = scalar_from_grid(g, add3(x, mk3(0.f, 0.f, d)), phis) - scalar_from_grid(g, sub3(x, mk3(0.f, 0.f, d)), phis);
+    const f3 v0 = v;
+    const float nl = len3(n);
+    if (nl > 0) {
+      n = mul3(n, 1.0f / nl);
+      if (ncol > 0) {
+        v = sub3(v, mul3(n, dot3(v0, n)));
+        x = add3(x, mul3(n, depth));
+      } else if (nadh > 0) {
+        v = sub3(v, mul3(mul3(n, 0.1f), len3(v)));
+      }
+    }
+  }
+  const float m = h + 0.3f * ss;
+  const f3 lo = mk3(g.gx0 + m, g.gy0 + m, g.gz0 + m), hi = mk3(g.gx1 - m, g.gy1 - m, g.gz1 - m);
+  if (x.x > hi.x) { v.x *= -0.5f; x.x = hi.x; }
+  if (x.x < lo.x) { v.x *= -0.5f; x.x = lo.x; }
+  if (x.y > hi.y) { v.y *= -0.5f; x.y = hi.y; }
+  if (x.y < lo.y) { v.y *= -0.5f; x.y = lo.y; }
+  if (x.z > hi.z) { v.z *= -0.5f; x.z = hi.z; }
+  if (x.z < lo.z) { v.z *= -0.5f; x.z = lo.z; }
+  x = add3(x, mul3(v, dt));
+  vel[p] = v;
+  pout[p] = x;
+}
+
 // advectparticle_RK2_bubble spray_k.cu:3463-3523: FLUID particles on the water
 // grid, AIR particles on the air grid, every other type untouched; wall
 // margin h + 0.5h; clamped components of the velocity zeroed (the FLIP/PIC
@@ -514,6 +592,13 @@ void run_advect(Ctx& c) {
     Faces3 wu{{c.wu[0], c.wu[1], c.wu[2]}}, au{{c.au[0], c.au[1], c.au[2]}};
     launch(c, "advect_rk2", dim3(nblocks(c.P, 256)), dim3(256), 0, k_advect_rk2, c.g, c.P, c.pos[cur], c.vel[cur],
            (const int8_t*)c.flag[cur], wu, au, c.cfg.dt);
+  } else if (c.cfg.integrator == MPF_ADVECT_FREEZING) {
+    run_solid_phi_and_velocity(c);
+    const int nxt = 1 - cur;
+    launch(c, "collide_freezing", dim3(nblocks(c.P, 256)), dim3(256), 0, k_collide_freezing, c.g, c.P,
+           (const f3*)c.pos[cur], c.pos[nxt], c.vel[cur], (const int8_t*)c.flag[cur], (const float*)c.phisolid,
+           (const uint32_t*)c.cstart, (const uint32_t*)c.cend, c.cfg.dt, c.cfg.samplespace);
+    std::swap(c.pos[cur], c.pos[nxt]);
   } else {
     run_solid_phi_and_velocity(c);
     Faces3 su{{c.su[0], c.su[1], c.su[2]}};

@@ -81,7 +81,9 @@ enum {
 /* Particle integrator. */
 enum {
   MPF_ADVECT_EULER_COLLIDE = 0, /* CollisionWithSolid_k, spray_k.cu:5276-5345 */
-  MPF_ADVECT_RK2 = 1            /* advectparticle_RK2,   spray_k.cu:409-459    */
+  MPF_ADVECT_RK2 = 1,           /* advectparticle_RK2,   spray_k.cu:409-459    */
+  MPF_ADVECT_FREEZING = 2       /* CollisionWithSolid_Freezing, spray_k.cu:5348-5446
+                                   (the freezing and melting-pour scenes)    */
 };
 
 /* Field that marching cubes meshes (mpf_config.mc_source; runMC_smooth,

@@ -967,11 +967,73 @@ template <class T> struct Orc : OrcBase {
       S3(vel, p, v);
     }
   }
+  // CollisionWithSolid_Freezing spray_k.cu:5348-5446 (neighbour positions of
+  // the pass's input)
+  void collide_freezing() {
+    const T dt = (T)c.dt;
+    const std::vector<T> p0(pos);
+    const T mgn = h + (T)0.3f * (T)c.samplespace;
+    const V3<T> lo = gmin() + mk(mgn, mgn, mgn), hi = gmax() - mk(mgn, mgn, mgn);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < P; ++p) {
+      if (flag[p] == SOLID) continue;
+      V3<T> x = P3(p0, p), v = P3(vel, p);
+      int i, j, k;
+      ijk_of_pos(x, i, j, k);
+      const T iphi = scalar_from_grid(x, phisolid);
+      if (iphi <= T(1)) {
+        const T r = (T)0.25f * h, hh = T(4) * r;
+        T depth = 0;
+        int ncol = 0, nadh = 0;
+        for (int di = -1; di <= 1; ++di)
+          for (int dj = -1; dj <= 1; ++dj)
+            for (int dk = -1; dk <= 1; ++dk) {
+              if (!valid(i + di, j + dj, k + dk)) continue;
+              const int64_t cc = cid(i + di, j + dj, k + dk);
+              if (cstart[cc] == MPF_CELL_UNDEF) continue;
+              for (uint32_t q = cstart[cc]; q < cend[cc]; ++q) {
+                const T dis = len3(x - P3(p0, q));
+                if (dis > 0 && dis < T(2) * r) {
+                  depth = tmax(depth, T(2) * r - dis);
+                  ncol++;
+                } else if (dis < hh) {
+                  nadh++;
+                }
+              }
+            }
+        const T d = h * (T)0.5f;
+        V3<T> n = mk(scalar_from_grid(x + mk(d, T(0), T(0)), phisolid) - scalar_from_grid(x - mk(d, T(0), T(0)), phisolid),
+                     scalar_from_grid(x + mk(T(0), d, T(0)), phisolid) - scalar_from_grid(x - mk(T(0), d, T(0)), phisolid),
+                     scalar_from_grid(x + mk(T(0), T(0), d), phisolid) - scalar_from_grid(x - mk(T(0), T(0), d), phisolid));
+        const V3<T> v0 = v;
+        const T nl = len3(n);
+        if (nl > 0) {
+          n = n * (T(1) / nl);
+          if (ncol > 0) {
+            v = v - n * dot3(v0, n);
+            x = x + n * depth;
+          } else if (nadh > 0) {
+            v = v - (n * (T)0.1f) * len3(v);
+          }
+        }
+      }
+      if (x.x > hi.x) { v.x = v.x * (T)-0.5f; x.x = hi.x; }
+      if (x.x < lo.x) { v.x = v.x * (T)-0.5f; x.x = lo.x; }
+      if (x.y > hi.y) { v.y = v.y * (T)-0.5f; x.y = hi.y; }
+      if (x.y < lo.y) { v.y = v.y * (T)-0.5f; x.y = lo.y; }
+      if (x.z > hi.z) { v.z = v.z * (T)-0.5f; x.z = hi.z; }
+      if (x.z < lo.z) { v.z = v.z * (T)-0.5f; x.z = lo.z; }
+      x = x + v * dt;
+      S3(vel, p, v);
+      S3(pos, p, x);
+    }
+  }
   void advect() {
     if (c.fix_solids)  // setVelZeroSolid_k spray_k.cu:5213-5218
       for (int64_t p = 0; p < P; ++p) if (flag[p] == SOLID) S3(vel, p, mk(T(0), T(0), T(0)));
     if (c.integrator == MPF_ADVECT_RK2) { advect_rk2(); return; }
     solid_phi_and_velocity();
+    if (c.integrator == MPF_ADVECT_FREEZING) { collide_freezing(); return; }
     collide_and_move();
   }
 

@@ -197,3 +197,26 @@ def test_step_with_moving_solid(pkg, scenes, orc):
     e = rel_l2(pg["pos"], po["pos"])
     say("  solid max |dpos| %.2e, positions rel L2 %.2e" % (d, e))
     assert d <= 1e-5 and e <= 1e-4
+
+
+def test_collide_freezing(pkg, scenes, orc):
+    """CollisionWithSolid_Freezing (MPF_ADVECT_FREEZING) on the melt/freeze
+    scene: positions and velocities bit-exact with the float oracle (the
+    solid distance field is bit-exact and neighbour positions are the pass's
+    input on both sides)."""
+    say("collide freezing")
+    s = scenes.make("melt_freeze", n=24)
+    cfg = s["cfg"]
+    cfg.integrator = pkg.ADVECT_FREEZING
+    rng = np.random.default_rng(9)
+    s["particles"]["vel"] = rng.normal(0, 0.3, s["particles"]["vel"].shape).astype(np.float32)
+    sim = pkg.Simulator(cfg)
+    load_scene(sim, s)
+    sim.hash_sort()
+    o = orc.Oracle(cfg)
+    sync(sim, o)
+    sim.advect()
+    o.advect()
+    same_particles(sim, o, "collide_freezing")
+    moved = np.abs(sim.get_particles()["pos"] - o.get_particles()["pos"]).max()
+    assert moved == 0

@@ -422,3 +422,52 @@ def test_solid_motion_wall_push_and_fixed(pkg, orc):
     o.solid_motion()
     q = o.get_particles()
     assert np.all(q["vel"] == 0) and np.array_equal(q["pos"], np.array(pos, F32).astype(float))
+
+
+# --------------------------------------------------------------- freezing-scene collision
+def test_collide_freezing_closed_forms(pkg, orc):
+    """CollisionWithSolid_Freezing next to a flat solid slab (cells k <= 2):
+    the solid distance grows along +z only, so the normal is (0, 0, 1).  Two
+    fluid particles 0.2 h apart at z = 3.3 h (phisolid <= 1) overlap by
+    0.5 h - 0.2 h: each rises by 0.3 h and loses its normal velocity; a
+    particle whose nearest neighbour is 0.6 h away (adhesion) loses
+    0.1 |v| along the normal; one far from the solid only moves by v dt; a
+    particle past the wall margin bounces with -0.5 v; then x += v dt."""
+    n = 12
+    cfg = cfg_of(pkg, n, integrator=pkg.ADVECT_FREEZING, fix_solids=1)
+    sol = np.array([[i + 0.5, j + 0.5, k + 0.5] for i in range(n) for j in range(n) for k in range(3)]) * H
+    v = np.array([0.1, -0.2, 0.3])
+    a = np.array([5.3, 5.5, 3.3]) * H
+    b = a + np.array([0.2, 0, 0]) * H
+    c0 = np.array([8.3, 8.5, 3.3]) * H
+    c1 = c0 + np.array([0.6, 0, 0]) * H
+    far = np.array([5.5, 5.5, 8.5]) * H
+    wall = np.array([5.5, 0.9 * 1.0, 6.5]) * H  # y below gmin + h + 0.3 ss
+    fl_pts = np.array([a, b, c0, c1, far, wall])
+    pos = np.concatenate([sol, fl_pts]).astype(F32)
+    vel = np.concatenate([np.zeros_like(sol), np.tile(v, (6, 1))]).astype(F32)
+    flag = np.array([pkg.TYPESOLID] * len(sol) + [pkg.TYPEFLUID] * 6, np.int8)
+    o = orc.Oracle(cfg, use_double=True)
+    o.set_particles(len(pos), pos=pos, vel=vel, flag=flag)
+    o.hash_sort()
+    order = o.get_field("GRID_INDEX")
+    o.advect()
+    got = o.get_particles()
+    P = np.empty((len(pos), 3)); P[order] = got["pos"]
+    V = np.empty((len(pos), 3)); V[order] = got["vel"]
+    q = len(sol)
+    x0 = pos.astype(float)
+    v0 = np.float32(v).astype(float)
+    dt = float(F32(cfg.dt))
+    for s in (q, q + 1):  # collision: pushed up by the overlap, normal velocity removed
+        vn = np.array([v0[0], v0[1], 0.0])
+        want = x0[s] + np.array([0, 0, 0.3 * H]) + vn * dt
+        assert np.allclose(V[s], vn, atol=1e-12) and np.allclose(P[s], want, atol=1e-9)
+    for s in (q + 2, q + 3):  # adhesion
+        vn = v0 - np.array([0, 0, 0.1 * np.linalg.norm(v0)])
+        assert np.allclose(V[s], vn, atol=1e-7) and np.allclose(P[s], x0[s] + vn * dt, atol=1e-9)
+    assert np.allclose(V[q + 4], v0) and np.allclose(P[q + 4], x0[q + 4] + v0 * dt, atol=1e-12)
+    lo = float(F32(cfg.dx + 0.3 * cfg.samplespace))
+    vw = v0 * np.array([1, -0.5, 1])
+    assert np.allclose(V[q + 5], vw) and np.isclose(P[q + 5][1], lo + vw[1] * dt, atol=1e-9)
+    assert np.array_equal(P[:q], x0[:q])  # solids stay
