@@ -88,6 +88,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+LOG_STEPS = False  # --log-steps: one line per step (long profiler runs stay audible)
+
+
+def step_done(frame, info):
+    if LOG_STEPS:
+        log("[bench]   step %d done, %d PCG iterations" % (frame, info["project"]["iterations"]))
+
+
 def run_form(pkg, scenes, name, n, precision, steps, warmup, kernel_timing=True, **overrides):
     """W warm-up + K timed mpf_step calls of one form.  With kernel_timing the
     timed steps record a HIP-event pair around every launch on the context
@@ -105,7 +113,7 @@ def run_form(pkg, scenes, name, n, precision, steps, warmup, kernel_timing=True,
         name, n, precision, s["n_particles"], warmup, steps))
     frame = 0
     for _ in range(warmup):
-        sim.step(frame, allow_no_convergence=True)
+        step_done(frame, sim.step(frame, allow_no_convergence=True))
         frame += 1
     sim.synchronize()
     iters = 0
@@ -115,6 +123,7 @@ def run_form(pkg, scenes, name, n, precision, steps, warmup, kernel_timing=True,
         info = sim.step(frame, allow_no_convergence=True)
         infos.append(info)
         iters += info["project"]["iterations"]
+        step_done(frame, info)
         frame += 1
     sim.synchronize()
     dt = time.perf_counter() - t0
@@ -141,6 +150,7 @@ def run_form(pkg, scenes, name, n, precision, steps, warmup, kernel_timing=True,
     unknowns = int(((mark == pkg.TYPEFLUID) | (mark == pkg.TYPEAIR)).sum())
     res = {"seconds": dt, "steps": steps, "pcg_iters": iters, "infos": infos, "ktimes": ktimes, "kt_iters": kt_iters,
            "particles": sim.particle_count(), "G": cfg.nx * cfg.ny * cfg.nz, "unknowns": unknowns,
+           "air_cells": int((mark == pkg.TYPEAIR).sum()),
            "converged": all(p["converged"] == 1 for p in solves),
            "max_rel_residual": max(p["rel_residual"] for p in solves), "tol": cfg.tol}
     sim.close()
@@ -216,18 +226,20 @@ def cpu_baseline(scenes, name, n, steps, warm):
                           name, n, s["n_particles"], steps, warm, dt, threads, its)}
 
 
-def air_scene(pkg, scenes, args):
-    """Mixed over FP64 on a scene with air (slow rows in the operator: the
-    two-kernel iteration plus the slow-row kernel)."""
-    mixed = run_form(pkg, scenes, args.air_scene, args.air_n, pkg.MIXED, args.air_steps, args.air_warmup,
-                     kernel_timing=False, tol=args.air_tol, max_iters=args.air_max_iters)
-    fp64 = run_form(pkg, scenes, args.air_scene, args.air_n, pkg.FP64, args.air_steps, args.air_warmup,
-                    kernel_timing=False, tol=args.air_tol, max_iters=args.air_max_iters)
+def other_scene(pkg, scenes, name, n, steps, warmup, tol, max_iters):
+    """Mixed over FP64 step rate on another scene (with air cells the
+    operator has slow rows: the single-kernel iteration with its slow-row
+    list in the mixed form, the two-kernel iteration in FP64)."""
+    mixed = run_form(pkg, scenes, name, n, pkg.MIXED, steps, warmup, kernel_timing=False, tol=tol,
+                     max_iters=max_iters)
+    fp64 = run_form(pkg, scenes, name, n, pkg.FP64, steps, warmup, kernel_timing=False, tol=tol,
+                    max_iters=max_iters)
     m = mixed["steps"] / mixed["seconds"]
     f = fp64["steps"] / fp64["seconds"]
     return {"workload": "%s %d^3, %d particles, PCG to %.0e rel. residual in both forms" % (
-                args.air_scene, args.air_n, mixed["particles"], args.air_tol),
-            "steps": args.air_steps, "warmup": args.air_warmup, "mixed_steps_per_s": m, "fp64_steps_per_s": f,
+                name, n, mixed["particles"], tol),
+            "air_cells_after": mixed["air_cells"],
+            "steps": steps, "warmup": warmup, "mixed_steps_per_s": m, "fp64_steps_per_s": f,
             "mixed_over_fp64": m / f, "mixed_pcg_iters_per_step": mixed["pcg_iters"] / mixed["steps"],
             "fp64_pcg_iters_per_step": fp64["pcg_iters"] / fp64["steps"],
             "converged": mixed["converged"] and fp64["converged"],
@@ -252,13 +264,20 @@ def main():
     ap.add_argument("--air-n", type=int, default=256)
     ap.add_argument("--air-steps", type=int, default=5)
     ap.add_argument("--air-warmup", type=int, default=2)
+    ap.add_argument("--no-boiling", action="store_true")
+    ap.add_argument("--boiling-n", type=int, default=256)
+    ap.add_argument("--boiling-steps", type=int, default=5)
+    ap.add_argument("--boiling-warmup", type=int, default=2)
     ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-warmup", type=int, default=0)
+    ap.add_argument("--log-steps", action="store_true", help="log every step's end (for profiler runs)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (a kernel of known HBM "
                          "traffic, for scaling profiler byte counters)")
     args = ap.parse_args()
+    global LOG_STEPS
+    LOG_STEPS = args.log_steps
     if args.gpus != 1:
         print("bench.py measures one GPU only: the z-slab multi-GPU path is not built; run with --gpus 1",
               file=sys.stderr)
@@ -320,7 +339,12 @@ def main():
         out["fp64_roofline"] = rf64
         forms.append(fp64)
     if not args.no_air:
-        out["air_scene"], more = air_scene(pkg, scenes, args)
+        out["air_scene"], more = other_scene(pkg, scenes, args.air_scene, args.air_n, args.air_steps,
+                                             args.air_warmup, args.air_tol, args.air_max_iters)
+        forms += more
+    if not args.no_boiling:
+        out["boiling_scene"], more = other_scene(pkg, scenes, "boiling", args.boiling_n, args.boiling_steps,
+                                                 args.boiling_warmup, 1e-6, args.air_max_iters)
         forms += more
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scenes, args.scene, args.cpu_n, args.cpu_steps, args.cpu_warmup)

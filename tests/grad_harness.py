@@ -112,11 +112,15 @@ def forces_vjp(cfg, g_v, v_before, flag):
 class Runner:
     """Drives one backend (GPU Simulator or FP64 oracle) through the step sequence."""
 
-    def __init__(self, backend, cfg, oracle=False):
+    def __init__(self, backend, cfg, oracle=False, records=None):
         self.b = backend
         self.cfg = cfg
         self.oracle = oracle
         self.dt = float(cfg.dt)
+        self.records = records  # discrete decisions to replay, one per step (None: the backend's own)
+
+    def fz(self, k):
+        return self.records[k] if self.records is not None else None
 
     def load(self, part):
         n = len(part["flag"])
@@ -178,10 +182,10 @@ class Runner:
         b.advect()
         return self.particles(), u_final, rec
 
-    def backward_step(self, part, s, gx_out, gv_out, gu_final=None):
+    def backward_step(self, part, s, gx_out, gv_out, gu_final=None, frozen=None):
         """Cotangents of one step's inputs (particles in input order, s)."""
         b = self.b
-        _, _, rec = self.step(part, s, stop="g2p")
+        _, _, rec = self.step(part, s, frozen=frozen, stop="g2p")
         gxa, gva, _ = b.advect_adjoint(gx_out, gv_out)
         r = b.g2p_adjoint(gva)
         gu = [a.copy() for a in r["u"]]
@@ -190,7 +194,7 @@ class Runner:
                 gu[q] += gu_final[q]
         gu = b.extrapolate_adjoint(gu)
         gu, _ = b.project_adjoint(gu, allow_no_convergence=True)
-        self.step(part, s, stop="extrap1")
+        self.step(part, s, frozen=frozen, stop="extrap1")
         gu = b.extrapolate_adjoint(gu)
         gs = source_vjp(self.cfg, gu[2], self.dt) if s is not None else None
         gp = [a + c for a, c in zip(gu, r["u_old"])]
@@ -260,18 +264,18 @@ def make_functions(torch):
         """K steps from a checkpoint; outputs (pos, vel, u_final of the last step)."""
 
         @staticmethod
-        def forward(ctx, x, v, s, runner, static, nsteps, last, side):
+        def forward(ctx, x, v, s, runner, static, nsteps, last, side, first):
             part = dict(static)
             part["pos"] = x.detach().cpu().numpy()
             part["vel"] = v.detach().cpu().numpy()
             s_np = s.detach().cpu().numpy()
             u_final = None
             for k in range(nsteps):
-                part, u, _ = runner.step(part, s_np, want_u=last and k == nsteps - 1)
+                part, u, _ = runner.step(part, s_np, frozen=runner.fz(first + k), want_u=last and k == nsteps - 1)
                 if u is not None:
                     u_final = u
             side["static"] = {k: part[k] for k in PART_KEYS if k not in ("pos", "vel")}
-            ctx.runner, ctx.static, ctx.nsteps, ctx.last = runner, static, nsteps, last
+            ctx.runner, ctx.static, ctx.nsteps, ctx.last, ctx.first = runner, static, nsteps, last, first
             ctx.save_for_backward(x, v, s)
             uf = torch.from_numpy(np.concatenate(u_final)) if u_final is not None else torch.zeros(0,
                                                                                                    dtype=torch.float64)
@@ -290,7 +294,7 @@ def make_functions(torch):
             for k in range(ctx.nsteps):
                 inputs.append(part)
                 if k + 1 < ctx.nsteps:
-                    part, _, _ = runner.step(part, s_np)
+                    part, _, _ = runner.step(part, s_np, frozen=runner.fz(ctx.first + k))
             gx = gx.detach().cpu().numpy().reshape(-1, 3)
             gv = gv.detach().cpu().numpy().reshape(-1, 3)
             gu_final = None
@@ -300,16 +304,20 @@ def make_functions(torch):
             gs = np.zeros_like(s_np)
             for k in reversed(range(ctx.nsteps)):
                 gx, gv, gsk = runner.backward_step(inputs[k], s_np, gx, gv,
-                                                   gu_final if k == ctx.nsteps - 1 else None)
+                                                   gu_final if k == ctx.nsteps - 1 else None,
+                                                   frozen=runner.fz(ctx.first + k))
                 gs += gsk
-            return torch.from_numpy(gx), torch.from_numpy(gv), torch.from_numpy(gs), None, None, None, None, None
+            return (torch.from_numpy(gx), torch.from_numpy(gv), torch.from_numpy(gs), None, None, None, None, None,
+                    None)
 
     return InitFn, SegmentFn
 
 
-def adjoint_gradient(pkg, scene, precision, steps, checkpoint, u_init, s, target, tol=1e-10):
+def adjoint_gradient(pkg, scene, precision, steps, checkpoint, u_init, s, target, tol=1e-10, records=None):
     """Loss and its gradient w.r.t. (u_init, s) through the GPU adjoints; also the
-    discrete records of every step (for the frozen-decision check)."""
+    discrete records of every step (for the frozen-decision check).  With
+    `records` the GPU replays those decisions (e.g. the FP64 oracle's own, from
+    oracle_records) instead of taking its own."""
     import torch
     cfg = pkg.Config.from_buffer_copy(scene["cfg"])
     cfg.precision = precision
@@ -320,7 +328,7 @@ def adjoint_gradient(pkg, scene, precision, steps, checkpoint, u_init, s, target
     icfg = pkg.Config.from_buffer_copy(cfg)
     icfg.flip_alpha = 0.0
     isim = pkg.Simulator(icfg)
-    runner = Runner(sim, cfg)
+    runner = Runner(sim, cfg, records=records)
     init = InitTransfer(isim, icfg)
     InitFn, SegmentFn = make_functions(torch)
     part0 = {k: np.asarray(scene["particles"][k]) for k in PART_KEYS}
@@ -337,20 +345,21 @@ def adjoint_gradient(pkg, scene, precision, steps, checkpoint, u_init, s, target
     while done < steps:
         k = min(checkpoint, steps - done)
         side = {}
-        x, v, uf = SegmentFn.apply(x, v, st, runner, static, k, done + k == steps, side)
+        x, v, uf = SegmentFn.apply(x, v, st, runner, static, k, done + k == steps, side, done)
         static = side["static"]
         done += k
     tt = torch.from_numpy(np.concatenate(target))
     loss = ((uf - tt) ** 2).sum()
     loss.backward()
-    # discrete records of the run, for the frozen-decision reference
-    records = []
-    v0 = init.forward(part0, u_init)
-    part = dict(part0)
-    part["vel"] = v0
-    for k in range(steps):
-        part, _, rec = runner.step(part, np.asarray(s, np.float64))
-        records.append(rec)
+    if records is None:
+        # discrete records of the run, for the frozen-decision reference
+        records = []
+        v0 = init.forward(part0, u_init)
+        part = dict(part0)
+        part["vel"] = v0
+        for k in range(steps):
+            part, _, rec = runner.step(part, np.asarray(s, np.float64))
+            records.append(rec)
     out = {"loss": float(loss.item()), "g_u": ut.grad.numpy().copy(), "g_s": st.grad.numpy().copy(),
            "records": records, "part0": part0}
     sim.close()
@@ -383,28 +392,48 @@ def frozen_loss(orc, pkg, scene, records, u_init, s, target, tol=1e-12):
     return loss_of(u_final, target)
 
 
-def fd_check(orc, pkg, scene, grad, u_init, s, target, ndirs, eps, seed=0, say=print):
-    """Central differences of the frozen-decision FP64 loss along random
-    directions vs the adjoint's directional derivative.  Directions are scaled
-    to the RMS of each parameter; eps is relative to that scale."""
+def fd_directions(u_init, s, ndirs, seed=0):
+    """Random directions (du, ds), scaled to the RMS of each parameter; the
+    same seed gives the same directions on every machine."""
     rng = np.random.default_rng(seed)
     su = max(float(np.sqrt(np.mean(np.concatenate(u_init) ** 2))), 1e-3)
     ss = max(float(np.sqrt(np.mean(np.asarray(s) ** 2))), 1e-3)
-    sizes = [len(a) for a in u_init]
-    errs = []
-    for d in range(ndirs):
-        du = rng.standard_normal(sum(sizes)) * su
+    n_u = sum(len(a) for a in u_init)
+    for _ in range(ndirs):
+        du = rng.standard_normal(n_u) * su
         ds = rng.standard_normal(len(s)) * ss
-        dus = np.split(du, np.cumsum(sizes)[:-1])
-        lp = frozen_loss(orc, pkg, scene, grad["records"], [a + eps * b for a, b in zip(u_init, dus)],
-                         np.asarray(s) + eps * ds, target)
-        lm = frozen_loss(orc, pkg, scene, grad["records"], [a - eps * b for a, b in zip(u_init, dus)],
-                         np.asarray(s) - eps * ds, target)
-        fd = (lp - lm) / (2 * eps)
-        ad = float(np.dot(grad["g_u"], du) + np.dot(grad["g_s"], ds))
-        e = abs(fd - ad) / max(abs(fd), abs(ad), 1e-300)
-        errs.append(e)
-        say("  direction %d: FD %.10e adjoint %.10e rel %.2e" % (d, fd, ad, e))
+        yield du, ds
+
+
+def directional(grad, du, ds):
+    return float(np.dot(grad["g_u"], du) + np.dot(grad["g_s"], ds))
+
+
+def fd_value(orc, pkg, scene, records, u_init, s, target, du, ds, eps):
+    """Central difference of the frozen-decision FP64 loss along (du, ds)."""
+    sizes = [len(a) for a in u_init]
+    dus = np.split(du, np.cumsum(sizes)[:-1])
+    lp = frozen_loss(orc, pkg, scene, records, [a + eps * b for a, b in zip(u_init, dus)],
+                     np.asarray(s) + eps * ds, target)
+    lm = frozen_loss(orc, pkg, scene, records, [a - eps * b for a, b in zip(u_init, dus)],
+                     np.asarray(s) - eps * ds, target)
+    return (lp - lm) / (2 * eps)
+
+
+def rel_err(a, b):
+    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+
+
+def fd_check(orc, pkg, scene, grad, u_init, s, target, ndirs, eps, seed=0, say=print):
+    """Central differences of the frozen-decision FP64 loss along random
+    directions vs the adjoint's directional derivative.  eps is relative to
+    the directions' scale."""
+    errs = []
+    for d, (du, ds) in enumerate(fd_directions(u_init, s, ndirs, seed)):
+        fd = fd_value(orc, pkg, scene, grad["records"], u_init, s, target, du, ds, eps)
+        ad = directional(grad, du, ds)
+        errs.append(rel_err(fd, ad))
+        say("  direction %d: FD %.10e adjoint %.10e rel %.2e" % (d, fd, ad, errs[-1]))
     return errs
 
 
@@ -418,6 +447,47 @@ def parameters(scene, seed):
     return u_init, s, target
 
 
+def oracle_records(orc, pkg, scene, steps, u_init, s):
+    """The FP64 oracle's own discrete decisions for the run (permutation, cell
+    lists, flags, level sets of every step)."""
+    cfg = pkg.Config.from_buffer_copy(scene["cfg"])
+    cfg.precision = 1
+    cfg.tol = 1e-12
+    cfg.max_iters = 20000
+    icfg = pkg.Config.from_buffer_copy(cfg)
+    icfg.flip_alpha = 0.0
+    o = orc.Oracle(cfg, use_double=True)
+    oi = orc.Oracle(icfg, use_double=True)
+    part0 = {k: np.asarray(scene["particles"][k]) for k in PART_KEYS}
+    part0["pos"] = part0["pos"].astype(np.float64).reshape(-1, 3)
+    part0["vel"] = np.zeros_like(part0["pos"])
+    part = dict(part0)
+    part["vel"] = InitTransfer(oi, icfg, oracle=True).forward(part0, u_init)
+    runner = Runner(o, cfg, oracle=True)
+    records = []
+    for k in range(steps):
+        part, _, rec = runner.step(part, np.asarray(s, np.float64))
+        for key in ("v_sorted", "flag_sorted"):
+            rec.pop(key, None)
+        records.append(rec)
+    return records
+
+
+def record_digest(records):
+    """sha256 per step over the integer decisions and over the level-set values."""
+    import hashlib
+    out = []
+    for rec in records:
+        hi = hashlib.sha256()
+        for key in ("perm", "CELL_START", "CELL_END", "MARK"):
+            hi.update(np.ascontiguousarray(rec[key]).astype(np.int64).tobytes())
+        hf = hashlib.sha256()
+        for key in LEVELSET[1:]:
+            hf.update(np.ascontiguousarray(rec[key]).tobytes())
+        out.append((hi.hexdigest()[:16], hf.hexdigest()[:16]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--n", type=int, default=96)
@@ -426,28 +496,72 @@ def main():
     ap.add_argument("--dirs", type=int, default=8)
     ap.add_argument("--eps", type=float, default=1e-6)
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--records", choices=["gpu", "oracle"], default="gpu",
+                    help="whose discrete decisions the gradients and the finite differences hold fixed")
+    ap.add_argument("--save", help="write each form's directional derivatives along the check's directions "
+                                   "and the record digests here (.npz)")
+    ap.add_argument("--save-dirs", type=int, default=64)
+    ap.add_argument("--fd-from", help="no GPU: finite differences against a gradient saved with --save "
+                                      "(records: the oracle's, rebuilt here and checked against the digests)")
+    ap.add_argument("--threads", type=int, default=16)
     args = ap.parse_args()
     pkg = importlib.import_module("2017multiphase_amd")
     scenes = importlib.import_module("2017multiphase_amd.scenes")
     import oracle as orc
-    orc.set_threads(min(os.cpu_count() or 1, 16))
+    orc.set_threads(min(os.cpu_count() or 1, args.threads))
     scene = scenes.make("diff_slab", n=args.n)
     u_init, s, target = parameters(scene, args.seed)
     say = lambda *a: print(*a, flush=True)  # noqa: E731
-    say("diff_slab n=%d particles=%d steps=%d checkpoint=%d" % (args.n, scene["n_particles"], args.steps,
-                                                                  args.checkpoint))
-    g64 = adjoint_gradient(pkg, scene, pkg.FP64, args.steps, args.checkpoint, u_init, s, target)
-    g32 = adjoint_gradient(pkg, scene, pkg.FP32, args.steps, args.checkpoint, u_init, s, target, tol=1e-7)
-    gm = adjoint_gradient(pkg, scene, pkg.MIXED, args.steps, args.checkpoint, u_init, s, target)
+    say("diff_slab n=%d particles=%d steps=%d checkpoint=%d records=%s" % (
+        args.n, scene["n_particles"], args.steps, args.checkpoint, "oracle" if args.fd_from else args.records))
+    if args.fd_from:
+        z = np.load(args.fd_from)
+        records = oracle_records(orc, pkg, scene, args.steps, u_init, s)
+        dig = record_digest(records)
+        saved = [tuple(d) for d in z["digest"].tolist()]
+        same_int = all(a[0] == b[0] for a, b in zip(dig, saved)) and len(dig) == len(saved)
+        same_ls = all(a[1] == b[1] for a, b in zip(dig, saved))
+        say("oracle records match the gradient's: decisions %s, level-set values %s" % (same_int, same_ls))
+        if not same_int:
+            sys.exit("the saved gradient was taken with other decisions")
+        forms = [k[3:] for k in z.files if k.startswith("ad_")]
+        errs = {f: [] for f in forms}
+        for d, (du, ds) in enumerate(fd_directions(u_init, s, min(args.dirs, len(z["ad_fp64"])))):
+            fd = fd_value(orc, pkg, scene, records, u_init, s, target, du, ds, args.eps)
+            for f in forms:
+                errs[f].append(rel_err(fd, float(z["ad_" + f][d])))
+            say("  direction %d: FD %.10e adjoint %s" % (d, fd, " ".join(
+                "%s %.10e (rel %.2e)" % (f, z["ad_" + f][d], errs[f][-1]) for f in forms)))
+        for f in forms:
+            say("FD vs %s adjoint: max rel %.2e over %d directions (eps %g)" % (f, max(errs[f]), len(errs[f]), args.eps))
+        return
+    records = None
+    if args.records == "oracle":
+        records = oracle_records(orc, pkg, scene, args.steps, u_init, s)
+        say("oracle records built")
+    g64 = adjoint_gradient(pkg, scene, pkg.FP64, args.steps, args.checkpoint, u_init, s, target, records=records)
+    say("FP64 loss %.12e |g_u| %.6e |g_s| %.6e" % (g64["loss"], np.linalg.norm(g64["g_u"]), np.linalg.norm(g64["g_s"])))
+    g32 = adjoint_gradient(pkg, scene, pkg.FP32, args.steps, args.checkpoint, u_init, s, target, tol=1e-7,
+                           records=records)
+    gm = adjoint_gradient(pkg, scene, pkg.MIXED, args.steps, args.checkpoint, u_init, s, target, records=records)
+    if args.save:
+        # the three forms' directional derivatives along the check's directions
+        # (the finite differences can then run elsewhere, from the same records)
+        dirs = list(fd_directions(u_init, s, args.save_dirs))
+        np.savez(args.save, loss=g64["loss"], digest=np.array(record_digest(g64["records"])),
+                 **{"ad_" + nm: np.array([directional(g, du, ds) for du, ds in dirs])
+                    for nm, g in (("fp64", g64), ("fp32", g32), ("mixed", gm))})
+        say("saved %d directional derivatives per form to %s" % (len(dirs), args.save))
     for nm, g in (("FP32", g32), ("MIXED", gm)):
         for key in ("g_u", "g_s"):
             e = np.linalg.norm(g[key] - g64[key]) / max(np.linalg.norm(g64[key]), 1e-300)
             say("%s %s vs FP64 gradient: rel L2 %.2e" % (nm, key, e))
-    say("loss FP64 %.12e FP32 %.12e" % (g64["loss"], g32["loss"]))
-    errs = fd_check(orc, pkg, scene, g64, u_init, s, target, args.dirs, args.eps, say=say)
-    say("FD vs FP64 adjoint: max rel %.2e over %d directions" % (max(errs), len(errs)))
-    errs = fd_check(orc, pkg, scene, g32, u_init, s, target, min(args.dirs, 4), args.eps, say=say)
-    say("FD vs FP32 adjoint: max rel %.2e" % max(errs))
+    say("loss FP64 %.12e FP32 %.12e MIXED %.12e" % (g64["loss"], g32["loss"], gm["loss"]))
+    if args.dirs > 0:
+        errs = fd_check(orc, pkg, scene, g64, u_init, s, target, args.dirs, args.eps, say=say)
+        say("FD vs FP64 adjoint: max rel %.2e over %d directions" % (max(errs), len(errs)))
+        errs = fd_check(orc, pkg, scene, g32, u_init, s, target, min(args.dirs, 4), args.eps, say=say)
+        say("FD vs FP32 adjoint: max rel %.2e" % max(errs))
 
 
 if __name__ == "__main__":
