@@ -108,6 +108,7 @@ class Config(ctypes.Structure):
         ("correction_spring", ctypes.c_float), ("correction_radius", ctypes.c_float),
         ("correct_bubble_air_ls", ctypes.c_int32),
         ("solid_bounce", ctypes.c_float), ("rigid_lock_y", ctypes.c_int32),
+        ("pcg_recompute_q", ctypes.c_int32),
     ]
 
 

@@ -323,10 +323,12 @@ __global__ void k_sph_force(Grid g, int64_t P, const f3* __restrict__ pos, const
 // (the bound keeps a margin against the rounding of the cell boundaries), and
 // a pair is evaluated only when its squared distance is below re^2 (1 + 2^-19),
 // a bound under which the rounded quotient d^2/re^2 is always >= 1.
-__global__ void k_correct_pos(Grid g, int64_t P, const f3* __restrict__ pin, f3* __restrict__ pout,
+constexpr int CP_LIST = 16;  // in-range neighbours listed per thread before a drain
+__global__ void __launch_bounds__(256) k_correct_pos(Grid g, int64_t P, const f3* __restrict__ pin, f3* __restrict__ pout,
                               const float* __restrict__ mass, const int8_t* __restrict__ flag,
                               const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend, float spring_k,
                               float radius, float ss, float dt) {
+  __shared__ uint32_t lds_hits[CP_LIST][256];  // blockDim.x == 256
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
   const int8_t fl = flag[p];
@@ -364,9 +366,48 @@ __global__ void k_correct_pos(Grid g, int64_t P, const f3* __restrict__ pin, f3*
     const float gx = gap(0, di), gy = gap(1, dj), gz = gap(2, dk);
     if (gx * gx + gy * gy + gz * gz <= cut) cand |= 1u << o;
   }
+  // Two passes.  The first reads distances only: the in-range neighbours of
+  // the reachable cells (cells by offset, particles by index, the reference's
+  // order) go to this thread's list in LDS, the first of each cell marked.
+  // The second evaluates the spring terms of the list, adding the running
+  // cell subtotal to the spring at each mark (sumcellspring returns one
+  // subtotal per cell; cells without an in-range neighbour add +-0).  A wave
+  // then pays the spring arithmetic for its lane with the most in-range
+  // neighbours, not for every candidate any lane has in range.  A full list
+  // is drained and refilled.  The particle itself (d = 0) adds nothing and
+  // is not listed.
+  uint32_t(*hits)[256] = lds_hits;
+  const int tid = threadIdx.x;
+  int nh = 0;
+  f3 sc = mk3(0.0f, 0.0f, 0.0f);
+  auto drain = [&]() {
+    if (nh == 0) return;
+    uint32_t e = hits[0][tid];
+    f3 y = pin[e & 0x7fffffffu];
+    float m = mass[e & 0x7fffffffu];
+    for (int h = 0; h < nh; ++h) {
+      const uint32_t ec = e;
+      const f3 yc = y;
+      const float mc = m;
+      if (h + 1 < nh) {  // next entry's loads in flight during this one
+        e = hits[h + 1][tid];
+        y = pin[e & 0x7fffffffu];
+        m = mass[e & 0x7fffffffu];
+      }
+      if (ec >> 31) {
+        spring = add3(spring, sc);
+        sc = mk3(0.0f, 0.0f, 0.0f);
+      }
+      const f3 dd = sub3(yc, x);
+      const float d2 = dot3(dd, dd);
+      const float dist = sqrtf(d2);
+      const float w = mc * fmaxf(1.0f - (dist * dist) / (re * re), 0.0f);
+      if (dist > 0.1f * re) sc = add3(sc, div3(smul3(w, sub3(x, yc)), dist));
+    }
+    nh = 0;
+  };
   // up to 8 cells at a time: their ranges read together, then each cell's
-  // particles four at a time (positions in flight together); the sums keep
-  // the reference's order (cells by offset, particles by index)
+  // particles four at a time (positions in flight together)
   constexpr int NCB = 8;
   while (cand) {
     uint32_t cs[NCB], ce[NCB];
@@ -387,29 +428,30 @@ __global__ void k_correct_pos(Grid g, int64_t P, const f3* __restrict__ pin, f3*
     for (int t = 0; t < NCB; ++t) {
       if (t >= nc) break;
       const uint32_t s0 = cs[t];
-      f3 sc = mk3(0.0f, 0.0f, 0.0f);
-      if (s0 != MPF_CELL_UNDEF) {
-        const uint32_t e0 = ce[t];
-        for (uint32_t q0 = s0; q0 < e0; q0 += 4) {
-          f3 y4[4];
+      if (s0 == MPF_CELL_UNDEF) continue;
+      const uint32_t e0 = ce[t];
+      uint32_t mark = 0x80000000u;
+      for (uint32_t q0 = s0; q0 < e0; q0 += 4) {
+        f3 y4[4];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) y4[u] = pin[min(q0 + (uint32_t)u, e0 - 1u)];
+        for (int u = 0; u < 4; ++u) y4[u] = pin[min(q0 + (uint32_t)u, e0 - 1u)];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
-            if (q0 + (uint32_t)u >= e0) break;
-            const f3 y = y4[u];
-            const f3 dd = sub3(y, x);
-            const float d2 = dot3(dd, dd);
-            if (!(d2 < cut)) continue;
-            const float dist = sqrtf(d2);
-            const float w = mass[q0 + u] * fmaxf(1.0f - (dist * dist) / (re * re), 0.0f);
-            if (dist > 0.1f * re) sc = add3(sc, div3(smul3(w, sub3(x, y)), dist));
-          }
+        for (int u = 0; u < 4; ++u) {
+          const uint32_t q = q0 + (uint32_t)u;
+          if (q >= e0) break;
+          const f3 dd = sub3(y4[u], x);
+          const float d2 = dot3(dd, dd);
+          if (!(d2 < cut) || (int64_t)q == p) continue;
+          if (nh == CP_LIST) drain();
+          hits[nh][tid] = q | mark;
+          mark = 0;
+          ++nh;
         }
       }
-      spring = add3(spring, sc);
     }
   }
+  drain();
+  spring = add3(spring, sc);
   spring = mul3(spring, spring_k * re);
   if (len3(smul3(dt, spring)) > 0.3f * hh)
     x = add3(x, div3(smul3(hh * 0.3f, spring), len3(spring)));

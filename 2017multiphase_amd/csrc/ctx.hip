@@ -380,6 +380,7 @@ mpf_status mpf_config_default(mpf_config* cfg, int32_t nx, int32_t ny, int32_t n
   cfg->correct_bubble_air_ls = 0;
   cfg->solid_bounce = 0.5f;                  // SolidbounceParam, spray.cpp:52
   cfg->rigid_lock_y = 0;
+  cfg->pcg_recompute_q = 0;
   return MPF_OK;
 }
 

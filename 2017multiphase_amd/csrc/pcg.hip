@@ -254,10 +254,13 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block-wide sum of N values over NT threads; the result is returned in every thread.
+// Block-wide sum of N values over NT threads; the result is returned in every
+// thread.  Thread q adds the wave sums of value q in wave order (one thread
+// per value keeps the other threads' registers free of the NT/64 x N terms).
 template <int N, int NT = PB>
 __device__ __forceinline__ void block_sum_all(double (&v)[N]) {
   __shared__ double lds[NT / 64][N];
+  __shared__ double tot[N];
   const int lane = __lane_id(), w = threadIdx.x >> 6;
 #pragma unroll
   for (int q = 0; q < N; ++q) v[q] = wave_sum(v[q]);
@@ -265,13 +268,15 @@ __device__ __forceinline__ void block_sum_all(double (&v)[N]) {
 #pragma unroll
     for (int q = 0; q < N; ++q) lds[w][q] = v[q];
   __syncthreads();
-#pragma unroll
-  for (int q = 0; q < N; ++q) {
+  if (threadIdx.x < N) {
     double s = 0.0;
 #pragma unroll
-    for (int ww = 0; ww < NT / 64; ++ww) s += lds[ww][q];
-    v[q] = s;
+    for (int ww = 0; ww < NT / 64; ++ww) s += lds[ww][threadIdx.x];
+    tot[threadIdx.x] = s;
   }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < N; ++q) v[q] = tot[q];
   __syncthreads();
 }
 
@@ -1029,59 +1034,36 @@ __global__ void __launch_bounds__(ST, MODE == 1 ? 2 : 4) k_resid64(OpArgs o, Til
   }
 }
 
-// ---- single-kernel iteration (MIXED form, solves without slow rows) ----
-// One launch per iteration k does what update(k-1) and spmv(k) do: with
-// alpha_{k-1} = (r.z)_{k-1} / (p.q)_{k-1} it forms r_k = r_{k-1} - alpha q_{k-1},
-// advances x by alpha p_{k-1}, and forms p_k = z_k + beta p_{k-1} and
-// q_k = A p_k.  beta needs (r.z)_k before any r_k exists, so it is taken from
-// the sums of the previous launch, which hold every term of
-//   (r - a q).D(r - a q) = r.z - 2 a q.z + a^2 q.Dq   (and the same for r.r):
-// each launch sums r.z, r.r of the r_k it formed (exact for its vectors) and
-// p.q, q.z, q.Dq, q.r, q.q, so the expansion spans one iteration only.  r, q
-// and p alternate between two buffers (ring cells read the previous values
-// of a neighbouring tile).  first = 1: k = 0 (sums of the init kernel, U slot
-// 1); first = 2: a restart from a true residual (sums of the residual pass).
-// Pres (when not first): a true-residual pass ran after the previous launch;
-// its r.z, r.r, q.z, q.r replace those of the previous launch.
-//
-// Solves with slow rows (SLOW): p_k of a slow row takes its own diagonal
-// (loaded with the vectors); its q_k and its share of the q sums come from
-// k_slow_q5 right after this launch, which writes them as T.ns5 more blocks
-// after this launch's T.nblk partials, so the next launch sums all of them
-// in one fixed order (likewise the true-residual sums of k_slow_resid64).
 constexpr int FDOT = 7;  // r.z, r.r, p.q, q.z, q.Dq, q.r, q.q
-template <int OPk, bool SLOW>
-__global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_cgf(OpArgs o, Tiling T, int first, int nbu, const double* __restrict__ Pprev,
-                                               const double* __restrict__ Pres, double* __restrict__ Pout,
-                                               const float* __restrict__ rin, const float* __restrict__ qin,
-                                               const float* __restrict__ pin, float* __restrict__ rout,
-                                               float* __restrict__ qout, float* __restrict__ pout,
-                                               float* __restrict__ x, double* __restrict__ x64, int fold,
-                                               SolveScalars* sc, int k, double tol, int cap) {
-  using V = float;
-  // a launch after the one that stopped the solve does nothing (the
-  // stopping launch's own late blocks still finish its x update)
-  if (sc->done && sc->done_k != k) return;
-  __shared__ V dtab[8];
-  if (threadIdx.x < 8) dtab[threadIdx.x] = (threadIdx.x < 7) ? dinv_tab<V>(o, threadIdx.x) : V(1);
+// The scalars of one single-kernel iteration, identical in every block:
+// alpha_{k-1}, beta_k and the stop decision from the previous launch's
+// partial sums (see k_cgf); block 0 records them for the host.
+struct CgScalars {
+  double ald, be;
+  bool stop, nonfin, zero_pq;
+};
+template <int NT>
+__device__ __forceinline__ CgScalars cg_scalars(const Tiling& T, int nres, int first, int nbu, const double* __restrict__ Pprev,
+                                                const double* __restrict__ Pres, SolveScalars* sc, int k, double tol,
+                                                int cap) {
   double ald = 0.0, rz, rr, be = 0.0, rz_prev = 0.0;
   bool zero_pq = false, bad_pq = false;
   if (first == 1) {
     double v[2];
-    read_partials<2, ST>(Pres, nbu, v);
+    read_partials<2, NT>(Pres, nbu, v);
     rz = v[0];
     rr = v[1];
   } else if (first == 2) {
     double v[4];
-    read_partials<4, ST>(Pres, T.nblk + T.ns5, v);
+    read_partials<4, NT>(Pres, nres, v);
     rz = v[0];
     rr = v[1];
   } else {
     double d[FDOT];
-    read_partials<FDOT, ST>(Pprev, T.nblk + T.ns5, d);
+    read_partials<FDOT, NT>(Pprev, T.nblk + T.ns5, d);
     if (Pres) {
       double e[4];
-      read_partials<4, ST>(Pres, T.nblk + T.ns5, e);
+      read_partials<4, NT>(Pres, nres, e);
       d[0] = e[0];
       d[1] = e[1];
       d[3] = e[2];
@@ -1108,7 +1090,6 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_cgf(OpArgs o, Tiling T, in
   const bool conv = zero_pq || rrc == 0.0 || spent || sqrt(rrc) <= tol * sqrt(bb);
   const bool stop = conv || nonfin || k >= cap;
   if (!first && !stop) be = rz / rz_prev;
-  const V al = (V)ald, bev = (V)be;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     sc->rz = rz;
     sc->rr = rr;
@@ -1122,6 +1103,46 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_cgf(OpArgs o, Tiling T, in
       sc->nonfinite = nonfin;
     }
   }
+  return CgScalars{ald, be, stop, nonfin, zero_pq};
+}
+
+// ---- single-kernel iteration (MIXED form, solves without slow rows) ----
+// One launch per iteration k does what update(k-1) and spmv(k) do: with
+// alpha_{k-1} = (r.z)_{k-1} / (p.q)_{k-1} it forms r_k = r_{k-1} - alpha q_{k-1},
+// advances x by alpha p_{k-1}, and forms p_k = z_k + beta p_{k-1} and
+// q_k = A p_k.  beta needs (r.z)_k before any r_k exists, so it is taken from
+// the sums of the previous launch, which hold every term of
+//   (r - a q).D(r - a q) = r.z - 2 a q.z + a^2 q.Dq   (and the same for r.r):
+// each launch sums r.z, r.r of the r_k it formed (exact for its vectors) and
+// p.q, q.z, q.Dq, q.r, q.q, so the expansion spans one iteration only.  r, q
+// and p alternate between two buffers (ring cells read the previous values
+// of a neighbouring tile).  first = 1: k = 0 (sums of the init kernel, U slot
+// 1); first = 2: a restart from a true residual (sums of the residual pass).
+// Pres (when not first): a true-residual pass ran after the previous launch;
+// its r.z, r.r, q.z, q.r replace those of the previous launch.
+//
+// Solves with slow rows (SLOW): p_k of a slow row takes its own diagonal
+// (loaded with the vectors); its q_k and its share of the q sums come from
+// k_slow_q5 right after this launch, which writes them as T.ns5 more blocks
+// after this launch's T.nblk partials, so the next launch sums all of them
+// in one fixed order (likewise the true-residual sums of k_slow_resid64).
+template <int OPk, bool SLOW>
+__global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_cgf(OpArgs o, Tiling T, int first, int nbu, const double* __restrict__ Pprev,
+                                               const double* __restrict__ Pres, double* __restrict__ Pout,
+                                               const float* __restrict__ rin, const float* __restrict__ qin,
+                                               const float* __restrict__ pin, float* __restrict__ rout,
+                                               float* __restrict__ qout, float* __restrict__ pout,
+                                               float* __restrict__ x, double* __restrict__ x64, int fold,
+                                               SolveScalars* sc, int k, double tol, int cap) {
+  using V = float;
+  // a launch after the one that stopped the solve does nothing (the
+  // stopping launch's own late blocks still finish its x update)
+  if (sc->done && sc->done_k != k) return;
+  __shared__ V dtab[8];
+  if (threadIdx.x < 8) dtab[threadIdx.x] = (threadIdx.x < 7) ? dinv_tab<V>(o, threadIdx.x) : V(1);
+  const CgScalars cs = cg_scalars<ST>(T, T.nblk + T.ns5, first, nbu, Pprev, Pres, sc, k, tol, cap);
+  const bool stop = cs.stop, nonfin = cs.nonfin, zero_pq = cs.zero_pq;
+  const V al = (V)cs.ald, bev = (V)cs.be;
   __shared__ V pl[3][LJ][LK];
   const Grid& g = o.g;
   const int lb = tile_of_block(T);
@@ -1328,6 +1349,287 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_cgf(OpArgs o, Tiling T, in
     }
   }
   write_partials<FDOT, ST>(a, Pout);
+}
+
+// ---- single-kernel iteration without a q vector (MIXED form, fast rows) ----
+// The iteration of k_cgf from r_{k-1}, p_{k-1} and x alone: q_{k-1} = A p_{k-1}
+// is formed again, on the tile and its first ring, from a p_{k-1} plane with
+// two rings in LDS, instead of being read; q_k is stored only when the caller
+// passes qout (the launch before a true-residual pass, which sums q.z and
+// q.r).  Per unknown it moves r (read and written), p_{k-1} (read), p_k
+// (written) and x (read and written): 24 bytes where k_cgf moves 32.  Every
+// cell value is the same expression on the same operands as in k_cgf (q_{k-1}
+// is the stencil k_cgf evaluated for q_k one launch earlier); the dot products
+// group their terms by other blocks, so the scalars, and with them the
+// iterates, agree with k_cgf's to rounding.
+//
+// One 1024-thread block per tile run (one block per CU): a thread holds one
+// tile cell and at most one ring cell of the first or the second ring, so the
+// longer pipeline fits the register file at four waves per SIMD.  Per plane i
+// of the run, behind one LDS barrier: the p_{k-1} plane i+2 goes to LDS (tile
+// and both rings); r_k and p_k of plane i+1 are formed on the tile and the
+// first ring (q_{k-1} from the LDS plane and the x neighbours held in
+// registers) and p_k goes to LDS; q_k of plane i is formed on the tile from
+// the p_k plane.  Loads run two planes ahead of their use (masks three).
+constexpr int SQ = 1024;                     // threads per block
+constexpr int LJ2 = TJ + 4, LK2 = TK + 4;    // LDS plane with two rings
+constexpr int NRING2 = 2 * TK + 2 * TJ + 4;  // second ring + the first ring's corners
+static_assert(TJ * TK == SQ, "one tile cell per thread");
+static_assert(NRING + NRING2 <= SQ, "at most one ring cell per thread");
+
+template <int OPk>
+__global__ void __launch_bounds__(SQ, 4) k_cgq(OpArgs o, Tiling T, int nres, int first, int nbu, const double* __restrict__ Pprev,
+                                            const double* __restrict__ Pres, double* __restrict__ Pout,
+                                            const float* __restrict__ rin, const float* __restrict__ pin,
+                                            float* __restrict__ rout, float* __restrict__ pout,
+                                            float* __restrict__ qout, float* __restrict__ x,
+                                            double* __restrict__ x64, int fold, SolveScalars* sc, int k, double tol,
+                                            int cap) {
+  using V = float;
+  if (sc->done && sc->done_k != k) return;
+  __shared__ V dtab[8];
+  if (threadIdx.x < 8) dtab[threadIdx.x] = (threadIdx.x < 7) ? dinv_tab<V>(o, threadIdx.x) : V(1);
+  const CgScalars cs = cg_scalars<SQ>(T, nres, first, nbu, Pprev, Pres, sc, k, tol, cap);
+  const V al = (V)cs.ald, bev = (V)cs.be;
+  __shared__ V pa[2][LJ2][LK2];  // p_{k-1}: planes with two rings
+  __shared__ V pk[2][LJ][LK];    // p_k: planes with one ring
+  const Grid& g = o.g;
+  const int lb = tile_of_block(T);
+  const int tk = lb % T.ntk;
+  const int rest = lb / T.ntk;
+  const int tj = rest % T.ntj;
+  const int i_lo = (rest / T.ntj) * T.seg;
+  const int i_hi = min(i_lo + T.seg, g.nx);
+  const int j0 = tj * TJ, k0 = tk * TK;
+  const int tid = threadIdx.x;
+  const int kl = tid & (TK - 1), jl = tid / TK;
+  const int64_t sx = (int64_t)g.ny * g.nz;
+  const bool cok = (j0 + jl) < g.ny && (k0 + kl) < g.nz;
+  const uint32_t col = cok ? (uint32_t)((j0 + jl) * g.nz + k0 + kl) : 0u;
+  auto plane = [&](int i) __attribute__((always_inline)) -> int64_t { return (int64_t)min(max(i, 0), g.nx - 1) * sx; };
+  if (cs.stop) {
+    if (cs.nonfin || cs.zero_pq || !cok) return;
+    for (int i = i_lo; i < i_hi; ++i) {
+      const int64_t idx = (int64_t)i * sx + col;
+      if (o.nbm[idx] & NB_UNKNOWN) {
+        const V xv = x[idx];
+        if (fold) {
+          x64[idx] = x64[idx] + (double)(xv + al * pin[idx]);
+          x[idx] = V(0);
+        } else if (!first) {
+          x[idx] = xv + al * pin[idx];
+        }
+      }
+    }
+    return;
+  }
+  // the thread's ring cell (hj, hk): first ring for tid < NRING (as in
+  // k_cgf), second ring and the first ring's corners for the next NRING2
+  int hj = 0, hk = 0;
+  const bool r1 = tid < NRING;
+  const bool r2 = !r1 && tid < NRING + NRING2;
+  if (tid < 2 * TK) {
+    hj = (tid < TK) ? -1 : TJ;
+    hk = tid & (TK - 1);
+  } else if (r1) {
+    const int t2 = tid - 2 * TK;
+    hj = t2 % TJ;
+    hk = (t2 < TJ) ? -1 : TK;
+  } else if (r2) {
+    const int t = tid - NRING;
+    if (t < 2 * TK) {
+      hj = (t < TK) ? -2 : TJ + 1;
+      hk = t & (TK - 1);
+    } else if (t < 2 * TK + 2 * TJ) {
+      const int t2 = t - 2 * TK;
+      hj = t2 % TJ;
+      hk = (t2 < TJ) ? -2 : TK + 1;
+    } else {
+      const int t3 = t - 2 * TK - 2 * TJ;
+      hj = (t3 & 1) ? TJ : -1;
+      hk = (t3 & 2) ? TK : -1;
+    }
+  }
+  const bool hr = r1 || r2;
+  const bool hok = hr && (j0 + hj) >= 0 && (j0 + hj) < g.ny && (k0 + hk) >= 0 && (k0 + hk) < g.nz;
+  const uint32_t hcol = hok ? (uint32_t)((j0 + hj) * g.nz + (k0 + hk)) : 0u;
+  // loads as in k_cgf: uniform plane base, 32-bit column, every lane issues
+  // every load (element 0 of the plane off the unknowns; r and x only where
+  // the cell forms r_k)
+  struct W {
+    V r, p, x;
+    uint8_t nb;
+  };
+  auto load_m = [&](int i, bool ok, uint32_t c) __attribute__((always_inline)) -> uint8_t {
+    const uint8_t* mp = o.nbm + plane(i);
+    return mp[(ok && i >= 0 && i < g.nx) ? c : 0u];
+  };
+  auto load_v = [&](int i, bool ok, uint8_t raw, uint32_t c, W& w, bool with_r, bool with_x) __attribute__((always_inline)) {
+    const uint8_t nb = (ok && i >= 0 && i < g.nx) ? raw : (uint8_t)0;
+    const int64_t pb = plane(i);
+    const uint32_t cv = (nb & NB_UNKNOWN) ? c : 0u;
+    w.nb = nb;
+    w.p = (pin + pb)[cv];
+    w.r = (rin + pb)[with_r ? cv : 0u];
+    w.x = with_x ? (x + pb)[cv] : V(0);
+  };
+  auto pval = [&](const W& w) __attribute__((always_inline)) -> V { return (w.nb & NB_UNKNOWN) ? w.p : V(0); };
+  auto d_of = [&](uint8_t nb) __attribute__((always_inline)) -> V { return o.precond ? dtab[__popc((unsigned)(nb & 63))] : V(1); };
+  const V qc = qcoef<V>(o);
+  // A p at a cell: neighbours +x, +y, +z, -x, -y, -z over its open faces, in
+  // k_cgf's order and operations
+  auto stencil = [&](uint8_t nb, V pc, V xp, V yp, V zp, V xm, V ym, V zm) __attribute__((always_inline)) -> V {
+    V sum = V(0);
+    sum = (nb & 1) ? sum + (xp - pc) : sum;
+    sum = (nb & 2) ? sum + (yp - pc) : sum;
+    sum = (nb & 4) ? sum + (zp - pc) : sum;
+    sum = (nb & 8) ? sum + (xm - pc) : sum;
+    sum = (nb & 16) ? sum + (ym - pc) : sum;
+    sum = (nb & 32) ? sum + (zm - pc) : sum;
+    return sum * qc;
+  };
+  // r_k of a cell from its loads and q_{k-1} (first: r_k = r_{k-1}), p_k
+  auto r_new = [&](const W& w, V qprev) __attribute__((always_inline)) -> V { return first ? w.r : w.r - al * qprev; };
+  auto p_new = [&](const W& w, V rk) __attribute__((always_inline)) -> V {
+    if (!(w.nb & NB_UNKNOWN)) return V(0);
+    return first ? rk * d_of(w.nb) : rk * d_of(w.nb) + bev * w.p;
+  };
+  double a[FDOT] = {0, 0, 0, 0, 0, 0, 0};
+  auto own = [&](int i, const W& w, V rk) __attribute__((always_inline)) {
+    if (!(w.nb & NB_UNKNOWN)) return;
+    const int64_t pb = (int64_t)i * sx;
+    (rout + pb)[col] = rk;
+    if (fold) {
+      const int64_t idx = pb + col;
+      x64[idx] = x64[idx] + (double)(w.x + al * w.p);
+      (x + pb)[col] = V(0);
+    } else if (!first) {
+      (x + pb)[col] = w.x + al * w.p;
+    }
+    const V zv = rk * d_of(w.nb);
+    a[0] += (double)rk * (double)zv;
+    a[1] += (double)rk * (double)rk;
+  };
+  // registers by plane q in slot (q - i_lo) & 3: loads L (tile cell), H
+  // (ring cell); p_{k-1} PP, PH; p_k, r_k and mask of the tile cell PK, RK, NK
+  W L[4], H[4];
+  V PP[4], PH[4], PK[4], RK[4];
+  uint8_t NK[4], M[2], HM[2];
+  // q_{k-1}, r_k and p_k of plane q on the tile and the first ring (slot s,
+  // x neighbours in slots sm (q - 1) and sp (q + 1), p_{k-1} plane pa[b]);
+  // p_k to pk[bk]; the tile cell stored when q is in the run
+  auto form = [&](int q, int s, int sm, int sp, int b, int bk) __attribute__((always_inline)) {
+    {
+      const W& w = L[s];
+      V qp = V(0);
+      if (!first)
+        qp = stencil(w.nb, PP[s], PP[sp], pa[b][jl + 3][kl + 2], pa[b][jl + 2][kl + 3], PP[sm], pa[b][jl + 1][kl + 2],
+                     pa[b][jl + 2][kl + 1]);
+      const V rk = r_new(w, qp);
+      const V pkv = p_new(w, rk);
+      PK[s] = pkv;
+      RK[s] = rk;
+      NK[s] = w.nb;
+      pk[bk][jl + 1][kl + 1] = pkv;
+      if (q >= i_lo && q < i_hi) own(q, w, rk);
+    }
+    if (r1) {
+      const W& w = H[s];
+      V qp = V(0);
+      if (!first)
+        qp = stencil(w.nb, PH[s], PH[sp], pa[b][hj + 3][hk + 2], pa[b][hj + 2][hk + 3], PH[sm], pa[b][hj + 1][hk + 2],
+                     pa[b][hj + 2][hk + 1]);
+      pk[bk][hj + 1][hk + 1] = p_new(w, r_new(w, qp));
+    }
+  };
+  // p_{k-1} of plane q (slot s) to pa[b]: tile cell and ring cell
+  auto stage = [&](int s, int b) __attribute__((always_inline)) {
+    PP[s] = pval(L[s]);
+    pa[b][jl + 2][kl + 2] = PP[s];
+    PH[s] = pval(H[s]);
+    if (hr) pa[b][hj + 2][hk + 2] = PH[s];
+  };
+  auto load_plane = [&](int q, int s) __attribute__((always_inline)) {
+    load_v(q, cok, load_m(q, cok, col), col, L[s], true, true);
+    load_v(q, hok, load_m(q, hok, hcol), hcol, H[s], r1, false);
+  };
+  // ---- prologue: planes i_lo - 2 .. i_lo + 3 ----
+  {
+    W t2, t1, h1;
+    load_v(i_lo - 2, cok, load_m(i_lo - 2, cok, col), col, t2, false, false);
+    load_v(i_lo - 1, cok, load_m(i_lo - 1, cok, col), col, t1, true, false);
+    load_v(i_lo - 1, hok && r1, load_m(i_lo - 1, hok && r1, hcol), hcol, h1, false, false);
+    load_plane(i_lo, 0);
+    load_plane(i_lo + 1, 1);
+    // plane i_lo - 1: p_{k-1} of tile and first ring to pa[1]; its p_k (tile
+    // cell only) is the -x neighbour of plane i_lo
+    PP[3] = pval(t1);
+    pa[1][jl + 2][kl + 2] = PP[3];
+    PH[3] = pval(h1);
+    if (r1) pa[1][hj + 2][hk + 2] = PH[3];
+    PP[0] = pval(L[0]);
+    lds_barrier();
+    {
+      V qp = V(0);
+      if (!first)
+        qp = stencil(t1.nb, PP[3], PP[0], pa[1][jl + 3][kl + 2], pa[1][jl + 2][kl + 3], pval(t2), pa[1][jl + 1][kl + 2],
+                     pa[1][jl + 2][kl + 1]);
+      PK[3] = p_new(t1, r_new(t1, qp));
+    }
+    load_plane(i_lo + 2, 2);
+    load_plane(i_lo + 3, 3);
+    M[0] = load_m(i_lo + 4, cok, col);
+    HM[0] = load_m(i_lo + 4, hok, hcol);
+    // plane i_lo: p_{k-1} to pa[0], then r_k, p_k (p_k to pk[0])
+    stage(0, 0);
+    lds_barrier();
+    PP[1] = pval(L[1]);
+    PH[1] = pval(H[1]);
+    form(i_lo, 0, 3, 1, 0, 0);
+    // plane i_lo + 1: p_{k-1} to pa[1]
+    stage(1, 1);
+    lds_barrier();
+  }
+  constexpr int UNR = 4;
+  for (int ib = i_lo; ib < i_hi; ib += UNR) {
+#pragma unroll
+    for (int d = 0; d < UNR; ++d) {
+      const int i = ib + d;
+      if (i < i_hi) {  // uniform over the block
+        const int s0 = d & 3, s1 = (d + 1) & 3, s2 = (d + 2) & 3, s3 = (d + 3) & 3;
+        const int mb = d & 1;
+        // masks of plane i + 5, vectors of plane i + 4 (into the slot of plane i)
+        M[mb ^ 1] = load_m(i + 5, cok, col);
+        HM[mb ^ 1] = load_m(i + 5, hok, hcol);
+        load_v(i + 4, cok, M[mb], col, L[s0], true, true);
+        load_v(i + 4, hok, HM[mb], hcol, H[s0], r1, false);
+        // p_{k-1} of plane i + 2 to pa; r_k, p_k of plane i + 1
+        stage(s2, d & 1);
+        form(i + 1, s1, s0, s2, (d + 1) & 1, (d + 1) & 1);
+        // q_k of plane i on the tile
+        const uint8_t nb = NK[s0];
+        if (nb & NB_UNKNOWN) {
+          const int64_t pb = (int64_t)i * sx;
+          const int bq = d & 1;
+          const V pcv = PK[s0];
+          const V qv = stencil(nb, pcv, PK[s1], pk[bq][jl + 2][kl + 1], pk[bq][jl + 1][kl + 2], PK[s3],
+                               pk[bq][jl][kl + 1], pk[bq][jl + 1][kl]);
+          const V rk = RK[s0];
+          const V dv = d_of(nb);
+          const V zv = rk * dv;
+          (pout + pb)[col] = pcv;
+          if (qout) (qout + pb)[col] = qv;
+          a[2] += (double)pcv * (double)qv;
+          a[3] += (double)qv * (double)zv;
+          a[4] += (double)qv * (double)(dv * qv);
+          a[5] += (double)qv * (double)rk;
+          a[6] += (double)qv * (double)qv;
+        }
+        lds_barrier();
+      }
+    }
+  }
+  write_partials<FDOT, SQ>(a, Pout);
 }
 
 // q_k = A p_k on the slow rows after k_cgf stored every p_k and r_k, with
@@ -1572,11 +1874,19 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
   auto resid64 = nslow > 0 ? k_resid64<OPk, 1> : k_resid64<OPk, 0>;
   // the MIXED form without slow rows runs one kernel per iteration (k_cgf)
   const bool fused = std::is_same<V, float>::value && mixed && !ref;
-  Tiling T = make_tiling(c.g, fused ? 512 : ((sizeof(V) == 4 && nslow == 0) ? 768 : 512));
+  // without slow rows the fused iteration can form q_{k-1} again instead of
+  // storing q (k_cgq, one 1024-thread block per CU; cfg.pcg_recompute_q)
+  const bool qfree = fused && nslow == 0 && c.cfg.pcg_recompute_q;
+  Tiling T = make_tiling(c.g, qfree ? 256 : fused ? 512 : ((sizeof(V) == 4 && nslow == 0) ? 768 : 512));
   T.ns5 = fused ? nsb : 0;
+  // the true-residual pass keeps two 512-thread blocks per CU (its own
+  // tiling when k_cgq runs one block per CU); its partial count is Tr.nblk
+  Tiling Tr = qfree ? make_tiling(c.g, 512) : T;
+  Tr.ns5 = T.ns5;
   // U-slot partials: update and init write L.nblk, the residual pass T.nblk;
   // both pad to npad, the count the spmv sums
-  const int npad = L.nblk > T.nblk ? L.nblk : T.nblk;
+  int npad = L.nblk > T.nblk ? L.nblk : T.nblk;
+  if (Tr.nblk > npad) npad = Tr.nblk;
   if (npad > MAX_BLK) {
     c.err = "grid too large for the solver's reduction buffers";
     return MPF_INVALID_ARG;
@@ -1604,11 +1914,11 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
       auto res_tiles = nslow > 0 ? k_resid64<OPk, 2> : k_resid64<OPk, 0>;
       // FP64 true residual: the tile pass, then the slow rows' list pass
       auto true_resid = [&](int kk, float* rr_, const float* qq_) {
-        launch(c, "pcg_resid64", dim3(T.nblk), dim3(ST), 0, res_tiles, o, T, (const float*)b, (const double*)x64,
+        launch(c, "pcg_resid64", dim3(Tr.nblk), dim3(ST), 0, res_tiles, o, Tr, (const float*)b, (const double*)x64,
                (const float*)x, rr_, P, c.scal, kk, npad, qq_, Rs);
         if (nslow > 0)
           launch(c, "pcg_slow_resid64", dim3(nsb), dim3(PB), 0, k_slow_resid64<OPk>, o, (const uint32_t*)slow_list,
-                 nslow, (const float*)b, (const double*)x64, (const float*)x, rr_, qq_, Rs + 4 * T.nblk, c.scal);
+                 nslow, (const float*)b, (const double*)x64, (const float*)x, rr_, qq_, Rs + 4 * Tr.nblk, c.scal);
       };
       int first = 1, fold = 0;
       bool use_res = false;
@@ -1616,10 +1926,17 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
         const int k_end = k + check;
         for (; k < k_end && k <= cap; ++k) {
           const double* pres = first == 1 ? (const double*)P.U1 : (first == 2 || use_res) ? Rs : nullptr;
-          launch(c, "pcg_cg_iter", dim3(T.nblk), dim3(ST), 0, cg_iter, o, T, first, npad,
-                 (const double*)S7[(k + 1) & 1], pres, S7[k & 1], (const float*)rb[k & 1],
-                 (const float*)qb[k & 1], (const float*)pf[k & 1], rb[(k + 1) & 1], qb[(k + 1) & 1],
-                 pf[(k + 1) & 1], (float*)x, x64, fold, c.scal, k, tol, cap);
+          if (qfree)  // q_k stored only for the true-residual pass after this launch
+            launch(c, "pcg_cg_iterq", dim3(T.nblk), dim3(SQ), 0, k_cgq<OPk>, o, T, Tr.nblk, first, npad,
+                   (const double*)S7[(k + 1) & 1], pres, S7[k & 1], (const float*)rb[k & 1],
+                   (const float*)pf[k & 1], rb[(k + 1) & 1], pf[(k + 1) & 1],
+                   (k + 1) % R == 0 ? qb[(k + 1) & 1] : (float*)nullptr, (float*)x, x64, fold, c.scal, k, tol,
+                   cap);
+          else
+            launch(c, "pcg_cg_iter", dim3(T.nblk), dim3(ST), 0, cg_iter, o, T, first, npad,
+                   (const double*)S7[(k + 1) & 1], pres, S7[k & 1], (const float*)rb[k & 1],
+                   (const float*)qb[k & 1], (const float*)pf[k & 1], rb[(k + 1) & 1], qb[(k + 1) & 1],
+                   pf[(k + 1) & 1], (float*)x, x64, fold, c.scal, k, tol, cap);
           if (nslow > 0)
             launch(c, "pcg_slow_q5", dim3(nsb), dim3(PB), 0, k_slow_q5<OPk>, o, (const uint32_t*)slow_list, nslow,
                    (const float*)pf[(k + 1) & 1], (const float*)rb[(k + 1) & 1], qb[(k + 1) & 1],
@@ -1642,7 +1959,7 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
         const int kd = s.iter;
         launch(c, "pcg_clear", dim3(1), dim3(64), 0, k_clear_done, c.scal);
         true_resid(kd, rb[kd & 1], (const float*)qb[kd & 1]);
-        launch(c, "pcg_collect", dim3(1), dim3(PB), 0, k_collect_rr4, (const double*)Rs, T.nblk + T.ns5, c.scal);
+        launch(c, "pcg_collect", dim3(1), dim3(PB), 0, k_collect_rr4, (const double*)Rs, Tr.nblk + T.ns5, c.scal);
         if (!read_scalars(c)) return MPF_HIP_ERROR;
         const SolveScalars& t = *c.scal_host;
         if (t.rr_true == 0.0 || sqrt(t.rr_true) <= tol * sqrt(t.bb)) break;

@@ -13,9 +13,9 @@ particle-steps/s, PCG iterations/s, whether every timed solve reached its
 tolerance on the FP64 true residual (the run exits 1 otherwise), a roofline
 record for the dominant kernel (average launch duration from HIP events on
 the context stream over the timed steps), the same mixed/FP64 comparison on a
-scene with air (the bubble column at 256^3: the solver's slow-row path) and a
-CPU baseline (the oracle with OpenMP, FP32 solve, one step of the same 256^3
-workload on this host's cores).
+scene with air (the bubble column at 256^3: the solver's slow-row path) and on
+the boiling scene at 256^3, and a CPU baseline (the oracle with OpenMP, FP32
+solve, two steps of the same 256^3 workload on this host's cores).
 """
 import argparse
 import importlib
@@ -47,6 +47,11 @@ KERNEL_BYTES = {
     # one-kernel iteration (MIXED form without slow rows): mask 1 per cell;
     # r, q, p_{k-1}, x read and r, q, p_k, x written per unknown
     "pcg_cg_iter": (1, 32, 0),
+    # the same iteration forming q_{k-1} = A p_{k-1} again (cfg.pcg_recompute_q):
+    # mask 1 per cell; r, p_{k-1}, x read and r, p_k, x written per unknown
+    # (q_k is written by one launch in refine_every, before the true
+    # residual: not counted)
+    "pcg_cg_iterq": (1, 24, 0),
     # start/end 8, mask 1, ls fluid+air 8 written; pos 12 + flag 1 per particle
     "genlevelset": (17, 0, 13),
     # one launch, 3 axes: start/end 8, mask 1, 3 faces 12 written; pos 12, vel 12, mass 4, flag 1

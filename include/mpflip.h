@@ -204,6 +204,12 @@ typedef struct mpf_config {
   /* rigid solid motion (solidmotion, spray.cu:2691-2774; fix_solids = 0) */
   float solid_bounce;        /* SolidbounceParam (0.5, spray.cpp:52)          */
   int32_t rigid_lock_y;      /* SCENE_INTERACTION: no translation along y (0) */
+  /* MPF_MIXED single-kernel iteration without slow rows: 1 forms q_{k-1} =
+     A p_{k-1} again from p_{k-1} instead of storing and reading q (24 bytes
+     per unknown and iteration instead of 32, iterates equal to rounding;
+     on MI355X it is bound by its arithmetic and runs within 2% of the
+     stored-q kernel); 0 stores q (0) */
+  int32_t pcg_recompute_q;
 } mpf_config;
 
 /* Opaque context. */

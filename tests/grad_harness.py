@@ -35,6 +35,7 @@ import argparse
 import importlib
 import os
 import sys
+import time
 
 import numpy as np
 
@@ -153,6 +154,7 @@ class Runner:
             rec["CELL_END"] = b.get_field("CELL_END")
         else:
             perm = frozen["perm"]
+            rec["perm"] = perm
             self.load({k: part[k][perm] for k in PART_KEYS})
             b.set_field("CELL_START", frozen["CELL_START"])
             b.set_field("CELL_END", frozen["CELL_END"])
@@ -503,6 +505,7 @@ def main():
     ap.add_argument("--save-dirs", type=int, default=64)
     ap.add_argument("--fd-from", help="no GPU: finite differences against a gradient saved with --save "
                                       "(records: the oracle's, rebuilt here and checked against the digests)")
+    ap.add_argument("--dir-start", type=int, default=0, help="with --fd-from: first direction to check")
     ap.add_argument("--threads", type=int, default=16)
     args = ap.parse_args()
     pkg = importlib.import_module("2017multiphase_amd")
@@ -526,12 +529,16 @@ def main():
             sys.exit("the saved gradient was taken with other decisions")
         forms = [k[3:] for k in z.files if k.startswith("ad_")]
         errs = {f: [] for f in forms}
-        for d, (du, ds) in enumerate(fd_directions(u_init, s, min(args.dirs, len(z["ad_fp64"])))):
+        nd = min(args.dir_start + args.dirs, len(z["ad_fp64"]))
+        for d, (du, ds) in enumerate(fd_directions(u_init, s, nd)):
+            if d < args.dir_start:
+                continue
+            t0 = time.time()
             fd = fd_value(orc, pkg, scene, records, u_init, s, target, du, ds, args.eps)
             for f in forms:
                 errs[f].append(rel_err(fd, float(z["ad_" + f][d])))
-            say("  direction %d: FD %.10e adjoint %s" % (d, fd, " ".join(
-                "%s %.10e (rel %.2e)" % (f, z["ad_" + f][d], errs[f][-1]) for f in forms)))
+            say("  direction %d: FD %.10e adjoint %s (%.0f s)" % (d, fd, " ".join(
+                "%s %.10e (rel %.2e)" % (f, z["ad_" + f][d], errs[f][-1]) for f in forms), time.time() - t0))
         for f in forms:
             say("FD vs %s adjoint: max rel %.2e over %d directions (eps %g)" % (f, max(errs[f]), len(errs[f]), args.eps))
         return

@@ -425,6 +425,47 @@ def test_project_tile_block_order(pkg, orc, with_air, precision):
         assert e <= TOL
 
 
+@pytest.mark.parametrize("dims", [(13, 17, 23), (16, 24, 24), (40, 136, 136)])
+def test_project_without_q_vector(pkg, orc, dims):
+    """The mixed single-kernel iteration that forms q_{k-1} = A p_{k-1} again
+    (cfg.pcg_recompute_q = 1) against the default one that stores q, on the
+    same system: both converge, iteration counts within 2, pressures within
+    1e-7 rel L2 of each other (the two kernels group the dot products'
+    terms by different blocks: rounding only), and the recomputing kernel's
+    pressure within 1e-5 of the FP64 oracle.  (40, 136, 136) has two k
+    tiles, one of them partial, and 17 j tiles."""
+    say("project without q vector", dims)
+    cfg, sim = _odd_grid_sim(pkg, False, 2, dims=dims)
+    assert cfg.pcg_recompute_q == 0
+    rq = pkg.Config.from_buffer_copy(cfg)
+    rq.pcg_recompute_q = 1
+    s2 = pkg.Simulator(rq)
+    p = sim.get_particles()
+    s2.set_particles(len(p["flag"]), **p)
+    s2.hash_sort()
+    s2.levelset_update()
+    s2.p2g()
+    s2.extrapolate_bc(True)
+    for f in ["MARK", "LSMERGE"] + FACE:
+        bitexact(s2.get_field(f), sim.get_field(f), f)
+    tight = pkg.Config.from_buffer_copy(cfg)
+    tight.tol = 1e-12
+    tight.max_iters = 20000
+    o64 = orc.Oracle(tight, use_double=True)
+    copy_fields(sim, o64, ["MARK", "LSMERGE"] + FACE)
+    gk = sim.project()
+    gq = s2.project()
+    o64.project()
+    say("  without q", gq)
+    say("  with q   ", gk)
+    assert gq["converged"] == 1 and gk["converged"] == 1
+    assert abs(gq["iterations"] - gk["iterations"]) <= 2
+    e = rel_l2(sim.get_field("PRESSURE"), s2.get_field("PRESSURE"))
+    e64 = rel_l2(s2.get_field("PRESSURE"), o64.get_field("PRESSURE"))
+    say("  pressure rel L2 between the kernels %.2e, vs FP64 oracle %.2e" % (e, e64))
+    assert e <= 1e-7 and e64 <= TOL
+
+
 def test_project_reference_cg(pkg, scenes, orc):
     """Reference CG (no preconditioner, r.r < 1e-2 G): same stopping rule."""
     say("project reference CG")
