@@ -58,7 +58,7 @@ __global__ void k_solo_air(Grid g, int64_t P, const f3* __restrict__ pos, f3* __
                            const float* __restrict__ auy, const float* __restrict__ auz,
                            const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend, float tol1,
                            float tol2) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= P) return;
   const int8_t fl = flag[p];
   if (fl == FLUID || fl == SOLID) return;
@@ -136,7 +136,7 @@ __global__ void k_st_particles(Grid g, int64_t P, const f3* __restrict__ pos, f3
                                const float* __restrict__ sf, const float* __restrict__ gx,
                                const float* __restrict__ gy, const float* __restrict__ gz,
                                const int8_t* __restrict__ mark, float dt) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= P) return;
   const int8_t fl = flag[p];
   if (fl == SOLID || fl == FLUID || fl == AIRSOLO) return;
@@ -169,7 +169,7 @@ __global__ void k_drag_particles(Grid g, int64_t P, const f3* __restrict__ pos, 
                                  const int8_t* __restrict__ flag, const float* __restrict__ ux,
                                  const float* __restrict__ uy, const float* __restrict__ uz, float solo, float grid,
                                  float dt, int update_particle, f3* __restrict__ s0, f3* __restrict__ s1) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= P || flag[p] != AIRSOLO) return;
   const f3 x = pos[p], v = vel[p];
   int i, j, k;
@@ -228,7 +228,7 @@ __global__ void k_drag_faces(Grid g, int64_t NF, float* __restrict__ u, const in
 __global__ void k_sph_density(Grid g, int64_t P, const f3* __restrict__ pos, const int8_t* __restrict__ flag,
                               const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
                               float* __restrict__ pdens, float* __restrict__ ppress, float m0poly6, float rho) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= P) return;
   const int8_t fl = flag[p];
   if (fl != AIR && fl != AIRSOLO) return;
@@ -272,7 +272,7 @@ __global__ void k_sph_force(Grid g, int64_t P, const f3* __restrict__ pos, const
                             f3* __restrict__ vout, const int8_t* __restrict__ flag, const uint32_t* __restrict__ cstart,
                             const uint32_t* __restrict__ cend, const float* __restrict__ pdens,
                             const float* __restrict__ ppress, float vterm, float spiky, float m0, float dt) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= P) return;
   const int8_t fl = flag[p];
   f3 iv = vin[p];
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(256) k_correct_pos(Grid g, int64_t P, const f3
                               const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend, float spring_k,
                               float radius, float ss, float dt) {
   __shared__ uint32_t lds_hits[CP_LIST][256];  // blockDim.x == 256
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= P) return;
   const int8_t fl = flag[p];
   f3 x = pin[p];
@@ -480,7 +480,7 @@ __global__ void k_correct_bubble(Grid g, int64_t P, f3* __restrict__ pos, const 
                                  const float* __restrict__ gy, const float* __restrict__ gz,
                                  const float* __restrict__ lsair, const float* __restrict__ ax,
                                  const float* __restrict__ ay, const float* __restrict__ az) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= P) return;
   const int8_t fl = flag[p];
   if (fl == AIRSOLO || fl == SOLID) return;

@@ -3,6 +3,7 @@
 // pressure gradient (a13), heat grid stages (a16), marching cubes (a18) and
 // the solid distance/velocity fields used by the collision advection (a15).
 #include "mpf_internal.h"
+#include "interp.h"
 
 #include <algorithm>
 #include <cmath>
@@ -87,7 +88,10 @@ __global__ void k_genlevelset(Grid g, const f3* __restrict__ pos, const int8_t* 
                               const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ cend,
                               const uint8_t* __restrict__ cmask, const uint8_t* __restrict__ dm,
                               float* __restrict__ lsfluid, float* __restrict__ lsair, int8_t* __restrict__ mark) {
-  CELL_LOOP_HEAD
+  const int64_t idx = xcd_block() * blockDim.x + threadIdx.x;  // XCD-contiguous cells
+  if (idx >= g.G) return;
+  int i, j, k;
+  ijk_of(g, idx, i, j, k);
   const float h = g.h;
   mark[idx] = VACUUM;
   float r = 0.5f * h;
@@ -567,7 +571,8 @@ __global__ void __launch_bounds__(ST_CELLS)
   __shared__ float sU[ST_RTOT];
   __shared__ uint32_t sS[ST_CELLS], sN[ST_CELLS], sList[ST_CELLS];
   __shared__ int sWc[ST_CELLS / 64];
-  int t = blockIdx.x;
+  const int tl = (int)xcd_block();  // XCD-contiguous tiles
+  int t = tl;
   const int tz = t % ntz;
   t /= ntz;
   const int ty = t % nty;
@@ -638,7 +643,7 @@ __global__ void __launch_bounds__(ST_CELLS)
   } else {
     __syncthreads();
   }
-  float* dst = parts + (int64_t)blockIdx.x * (2 * ST_RTOT);
+  float* dst = parts + (int64_t)tl * (2 * ST_RTOT);
   for (int q = threadIdx.x; q < ST_RTOT; q += ST_CELLS) {
     dst[q] = sW[q];
     dst[ST_RTOT + q] = sU[q];
@@ -994,7 +999,8 @@ __global__ void __launch_bounds__(ST_CELLS)
   __shared__ float sH[HR];
   __shared__ uint32_t sS[ST_CELLS], sN[ST_CELLS], sList[ST_CELLS];
   __shared__ int sWc[ST_CELLS / 64];
-  int t = blockIdx.x;
+  const int tl = (int)xcd_block();  // XCD-contiguous tiles
+  int t = tl;
   const int tz = t % ntz;
   t /= ntz;
   const int ty = t % nty;
@@ -1036,7 +1042,7 @@ __global__ void __launch_bounds__(ST_CELLS)
   } else {
     __syncthreads();
   }
-  float* dst = parts + (int64_t)blockIdx.x * (2 * HR);
+  float* dst = parts + (int64_t)tl * (2 * HR);
   for (int q = threadIdx.x; q < HR; q += ST_CELLS) {
     dst[q] = sW[q];
     dst[HR + q] = sH[q];

@@ -1,8 +1,20 @@
-// Grid interpolation shared by the particle kernels and their adjoints.
+// Grid interpolation shared by the particle kernels and their adjoints, and
+// the XCD-contiguous block mapping of the gather kernels.
 #pragma once
 #include "mpf_internal.h"
 
 namespace mpf {
+
+// Logical block index with the hardware's round-robin dispatch of blocks over
+// the 8 XCDs undone: XCD x runs the contiguous range of logical blocks
+// [x q + min(x, r), ...) (q, r = grid size / 8, remainder), so the neighbour
+// cells and particles that adjacent blocks share are read into one XCD's L2.
+// A bijection of [0, gridDim.x).
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t nb = gridDim.x, b = blockIdx.x;
+  const int64_t q = nb >> 3, r = nb & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
 
 // trilinear, spray_k.cu:111-122
 __device__ __forceinline__ float trilinear(const float* __restrict__ u, float x, float y, float z, int w, int hh,

@@ -46,7 +46,7 @@ __global__ void k_delta_u(int64_t n, const float* __restrict__ u, float* __restr
 // spray_k.cu:421-424 when alpha < 1.
 __global__ void k_g2p(Grid g, int64_t P, const f3* __restrict__ pos, f3* __restrict__ vel,
                       const int8_t* __restrict__ flag, Faces3 wdu, Faces3 adu, Faces3 wu, Faces3 au, float alpha) {
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= P) return;
   // the flag, position and velocity are read together, and the phase's face
   // arrays are chosen by value (a pointer into the kernel arguments would be
@@ -81,7 +81,7 @@ __global__ void k_zero_solid_vel(int64_t P, f3* __restrict__ vel, const int8_t* 
 __global__ void k_collide_move(Grid g, int64_t P, f3* __restrict__ pos, f3* __restrict__ vel,
                                const int8_t* __restrict__ flag, const float* __restrict__ phis, Faces3 su, float dt,
                                float bounce_vel, float bounce_pos) {
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= P) return;
   // flag, position and velocity are read together; solid particles are
   // computed like the others and not stored
@@ -132,7 +132,7 @@ __global__ void k_collide_freezing(Grid g, int64_t P, const f3* __restrict__ pin
                                    f3* __restrict__ vel, const int8_t* __restrict__ flag,
                                    const float* __restrict__ phis, const uint32_t* __restrict__ cstart,
                                    const uint32_t* __restrict__ cend, float dt, float ss) {
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= P) return;
   f3 x = pin[p];
   if (flag[p] == SOLID) {
@@ -206,7 +206,7 @@ __global__ void k_collide_freezing(Grid g, int64_t P, const f3* __restrict__ pin
 // bubble kernel's position write is commented out, spray_k.cu:3520-3521).
 __global__ void k_advect_rk2(Grid g, int64_t P, f3* __restrict__ pos, f3* __restrict__ vel,
                              const int8_t* __restrict__ flag, Faces3 wu, Faces3 au, float dt) {
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= P) return;
   int8_t fl = flag[p];
   if (fl != FLUID && fl != AIR) return;
@@ -237,7 +237,7 @@ __global__ void k_heat_g2p(Grid g, int64_t P, const f3* __restrict__ pos, const 
                            float* __restrict__ temp, float* __restrict__ lheat, const float* __restrict__ tchange,
                            const float* __restrict__ tgrid, float at, int melt_and_boil, float mp, float bp,
                            float th) {
-  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t p = xcd_block() * blockDim.x + threadIdx.x;
   if (p >= P) return;
   f3 x = pos[p];
   int8_t fl = flag[p];

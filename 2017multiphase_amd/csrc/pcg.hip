@@ -1126,8 +1126,11 @@ __device__ __forceinline__ CgScalars cg_scalars(const Tiling& T, int nres, int f
 // k_slow_q5 right after this launch, which writes them as T.ns5 more blocks
 // after this launch's T.nblk partials, so the next launch sums all of them
 // in one fixed order (likewise the true-residual sums of k_slow_resid64).
-template <int OPk, bool SLOW>
-__global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_cgf(OpArgs o, Tiling T, int first, int nbu, const double* __restrict__ Pprev,
+// STEADY: a launch with first = 0 and fold = 0 (all but one launch in
+// refine_every), compiled with both as constants so the plane loop carries
+// no branch on them.
+template <int OPk, bool SLOW, bool STEADY>
+__global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, int nbu, const double* __restrict__ Pprev,
                                                const double* __restrict__ Pres, double* __restrict__ Pout,
                                                const float* __restrict__ rin, const float* __restrict__ qin,
                                                const float* __restrict__ pin, float* __restrict__ rout,
@@ -1138,8 +1141,13 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_cgf(OpArgs o, Tiling T, in
   // a launch after the one that stopped the solve does nothing (the
   // stopping launch's own late blocks still finish its x update)
   if (sc->done && sc->done_k != k) return;
+  if (STEADY) {
+    first = 0;
+    fold = 0;
+  }
+  // 1/diag of the fast rows by open-face count (1 without a preconditioner)
   __shared__ V dtab[8];
-  if (threadIdx.x < 8) dtab[threadIdx.x] = (threadIdx.x < 7) ? dinv_tab<V>(o, threadIdx.x) : V(1);
+  if (threadIdx.x < 8) dtab[threadIdx.x] = (threadIdx.x < 7 && o.precond) ? dinv_tab<V>(o, threadIdx.x) : V(1);
   const CgScalars cs = cg_scalars<ST>(T, T.nblk + T.ns5, first, nbu, Pprev, Pres, sc, k, tol, cap);
   const bool stop = cs.stop, nonfin = cs.nonfin, zero_pq = cs.zero_pq;
   const V al = (V)cs.ald, bev = (V)cs.be;
@@ -1229,7 +1237,7 @@ __global__ void __launch_bounds__(ST, SLOW ? 2 : 4) k_cgf(OpArgs o, Tiling T, in
   };
   // r_k, and p_k = z_k + beta p_{k-1} (0 off the unknowns)
   auto r_of = [&](const Rw& w) -> V { return first ? w.r : w.r - al * w.q; };
-  auto d_of = [&](uint8_t nb) -> V { return o.precond ? dtab[__popc((unsigned)(nb & 63))] : V(1); };
+  auto d_of = [&](uint8_t nb) -> V { return dtab[__popc((unsigned)(nb & 63))]; };
   auto dw_of = [&](const Rw& w) -> V {
     if constexpr (SLOW) {
       if (w.nb & NB_SLOW) return o.precond ? w.ds : V(1);
@@ -1910,7 +1918,8 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
       // partial sums: per launch T.nblk tile blocks then T.ns5 slow-row blocks
       double* S7[2] = {c.partials + 8 * MAX_BLK, c.partials + 22 * MAX_BLK};
       double* Rs = c.partials + 36 * MAX_BLK;
-      auto cg_iter = nslow > 0 ? k_cgf<OPk, true> : k_cgf<OPk, false>;
+      auto cg_iter = nslow > 0 ? k_cgf<OPk, true, false> : k_cgf<OPk, false, false>;
+      auto cg_steady = nslow > 0 ? k_cgf<OPk, true, true> : k_cgf<OPk, false, true>;
       auto res_tiles = nslow > 0 ? k_resid64<OPk, 2> : k_resid64<OPk, 0>;
       // FP64 true residual: the tile pass, then the slow rows' list pass
       auto true_resid = [&](int kk, float* rr_, const float* qq_) {
@@ -1933,7 +1942,8 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
                    (k + 1) % R == 0 ? qb[(k + 1) & 1] : (float*)nullptr, (float*)x, x64, fold, c.scal, k, tol,
                    cap);
           else
-            launch(c, "pcg_cg_iter", dim3(T.nblk), dim3(ST), 0, cg_iter, o, T, first, npad,
+            launch(c, "pcg_cg_iter", dim3(T.nblk), dim3(ST), 0, (first == 0 && fold == 0) ? cg_steady : cg_iter, o, T,
+                   first, npad,
                    (const double*)S7[(k + 1) & 1], pres, S7[k & 1], (const float*)rb[k & 1],
                    (const float*)qb[k & 1], (const float*)pf[k & 1], rb[(k + 1) & 1], qb[(k + 1) & 1],
                    pf[(k + 1) & 1], (float*)x, x64, fold, c.scal, k, tol, cap);
