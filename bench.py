@@ -62,6 +62,8 @@ KERNEL_BYTES = {
     "heat_p2g": (13, 0, 17),
     # 45-byte record read and written + perm 4
     "reorder": (0, 0, 94),
+    # position and flag read, position written (neighbours from cache)
+    "correct_pos": (0, 0, 25),
     # level-set sweep: ls 4 read, mark 1, ls 4 written
     "sweep": (9, 0, 0),
     # extrapolation pass per face: u 4, out 4, ls 2x4, mark 2
