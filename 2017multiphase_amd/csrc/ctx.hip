@@ -118,7 +118,7 @@ void free_all(Ctx& c) {
   for (int a = 0; a < 3; ++a) dfree(c.phig[a]);
   dfree(c.phifluid); dfree(c.phiair); dfree(c.phisolid); dfree(c.ctmp); dfree(c.sf);
   dfree(c.div); dfree(c.press); dfree(c.tp); dfree(c.tpchange); dfree(c.tsave);
-  dfree(c.vx); dfree(c.vr); dfree(c.vp); dfree(c.vq); dfree(c.vp2); dfree(c.vr2); dfree(c.vq2); dfree(c.nbm); dfree(c.dslow);
+  dfree(c.vx); dfree(c.vr); dfree(c.vp); dfree(c.vq); dfree(c.vp2); dfree(c.vr2); dfree(c.vq2); dfree(c.vp3); dfree(c.nbm); dfree(c.dslow);
   dfree(c.partials); dfree(c.scal); dfree(c.counters);
   for (int a = 0; a < 3; ++a) {
     dfree(c.wu[a]); dfree(c.au[a]); dfree(c.wuo[a]); dfree(c.auo[a]); dfree(c.su[a]); dfree(c.ftmp[a]);
@@ -188,6 +188,7 @@ bool alloc_all(Ctx& c) {
   ok = ok && dalloc(c, dd, G); c.vp2 = dd;
   ok = ok && dalloc(c, dd, G); c.vr2 = dd;
   ok = ok && dalloc(c, dd, G); c.vq2 = dd;
+  ok = ok && dalloc(c, dd, G); c.vp3 = dd;
   ok = ok && dalloc(c, c.nbm, G) && dalloc(c, c.dslow, G);
   c.npart = 1024 * 48;  // solver partial sums (pcg.hip lays out regions of 1024-block slots)
   ok = ok && dalloc(c, c.partials, c.npart) && dalloc(c, c.scal, 1) && dalloc(c, c.counters, NCOUNTERS);

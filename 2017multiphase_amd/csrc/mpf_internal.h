@@ -114,6 +114,7 @@ struct SolveScalars {
   int iter, done, converged, nonfinite, refinements, rec_conv, fold_pending;
   int last_true;  // MIXED: update index whose residual the last executed true-residual pass replaced (-1 none)
   int done_k;     // single-kernel iteration: the k of the launch that set done
+  float alh[2];   // single-kernel iteration: the alpha launch k applied, in slot k&1
 };
 
 struct KernelStat {
@@ -175,6 +176,7 @@ struct Ctx {
   // iterative solve work vectors (sized for double)
   void *vx = nullptr, *vr = nullptr, *vp = nullptr, *vq = nullptr, *vp2 = nullptr;
   void *vr2 = nullptr, *vq2 = nullptr;  // second r and q of the single-kernel iteration
+  void* vp3 = nullptr;                   // third p: the single-kernel iteration keeps p_k-2
   uint8_t* nbm = nullptr;  // per-cell neighbour mask of the current solve
   double* dslow = nullptr;  // 1/diag of the current solve's slow rows (flags-dependent)
   double* vx64 = nullptr;

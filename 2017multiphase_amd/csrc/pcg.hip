@@ -336,11 +336,14 @@ struct Parts {
 };
 __device__ __forceinline__ double* uslot(const Parts& P, int k) { return (k & 1) ? P.U1 : P.U0; }
 
-// r = b (unknowns), x = 0 (and x64 = 0), both p buffers 0; partials r.z, r.r into U slot 1.
+// r = b (unknowns), x = 0 (and x64 = 0), the three p buffers 0 (p is stored
+// on the unknowns only, and the slow-row kernels read it off them as 0);
+// partials r.z, r.r into U slot 1.
 template <class V, class Dacc, int OPk>
 __global__ void __launch_bounds__(PB) k_init(OpArgs o, Launch L, const V* __restrict__ b, V* __restrict__ x,
                                              V* __restrict__ r, V* __restrict__ pa, V* __restrict__ pb,
-                                             double* __restrict__ x64, Parts P, SolveScalars* sc, int npad) {
+                                             V* __restrict__ pc, double* __restrict__ x64, Parts P,
+                                             SolveScalars* sc, int npad) {
   Dacc a_rz = 0, a_rr = 0;
   CELLS_OF_BLOCK(o.g.G) {
     const uint8_t nb = o.nbm[idx];
@@ -350,6 +353,7 @@ __global__ void __launch_bounds__(PB) k_init(OpArgs o, Launch L, const V* __rest
     r[idx] = bv;
     pa[idx] = V(0);
     pb[idx] = V(0);
+    pc[idx] = V(0);
     if (x64) x64[idx] = 0.0;
     a_rz += (Dacc)bv * (Dacc)z;
     a_rr += (Dacc)bv * (Dacc)bv;
@@ -1094,6 +1098,7 @@ __device__ __forceinline__ CgScalars cg_scalars(const Tiling& T, int nres, int f
     sc->rz = rz;
     sc->rr = rr;
     sc->iter = k;
+    sc->alh[k & 1] = (float)ald;
     if (first == 1 && k == 0) sc->bb = bb;
     if (stop) {
       sc->done_k = k;
@@ -1128,15 +1133,22 @@ __device__ __forceinline__ CgScalars cg_scalars(const Tiling& T, int nres, int f
 // in one fixed order (likewise the true-residual sums of k_slow_resid64).
 // STEADY: a launch with first = 0 and fold = 0 (all but one launch in
 // refine_every), compiled with both as constants so the plane loop carries
-// no branch on them.
-template <int OPk, bool SLOW, bool STEADY>
+// no branch on them.  XM (steady launches without slow rows): how x advances,
+// 1: x += alpha_k-1 p_k-1; 2: not at all (that term stays pending, x is
+// neither read nor written); 3: x = (x + alpha_k-2 p_k-2) + alpha_k-1 p_k-1,
+// the pending term first (p2in = p_k-2, alpha_k-2 as launch k-1 applied it),
+// so x takes the same roundings as when it advances every launch.  A launch
+// that stops the solve advances x by all it owes.
+template <int OPk, bool SLOW, bool STEADY, int XM>
 __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, int nbu, const double* __restrict__ Pprev,
                                                const double* __restrict__ Pres, double* __restrict__ Pout,
                                                const float* __restrict__ rin, const float* __restrict__ qin,
                                                const float* __restrict__ pin, float* __restrict__ rout,
                                                float* __restrict__ qout, float* __restrict__ pout,
                                                float* __restrict__ x, double* __restrict__ x64, int fold,
-                                               SolveScalars* sc, int k, double tol, int cap) {
+                                               SolveScalars* sc, int k, double tol, int cap,
+                                               const float* __restrict__ p2in) {
+  static_assert(XM == 1 || (STEADY && !SLOW && (XM == 2 || XM == 3)), "x modes 2 and 3 are steady launches");
   using V = float;
   // a launch after the one that stopped the solve does nothing (the
   // stopping launch's own late blocks still finish its x update)
@@ -1151,6 +1163,7 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
   const CgScalars cs = cg_scalars<ST>(T, T.nblk + T.ns5, first, nbu, Pprev, Pres, sc, k, tol, cap);
   const bool stop = cs.stop, nonfin = cs.nonfin, zero_pq = cs.zero_pq;
   const V al = (V)cs.ald, bev = (V)cs.be;
+  const V al2 = (XM == 3) ? sc->alh[(k - 1) & 1] : V(0);  // alpha_k-2, as launch k-1 applied it
   __shared__ V pl[3][LJ][LK];
   const Grid& g = o.g;
   const int lb = tile_of_block(T);
@@ -1180,7 +1193,8 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
       x64[idx] = x64[idx] + (double)(xv + al * pv);
       x[idx] = V(0);
     } else if (!first) {
-      x[idx] = xv + al * pv;
+      if (XM == 3) x[idx] = (xv + al2 * p2in[idx]) + al * pv;
+      else x[idx] = xv + al * pv;  // a stopping launch advances x in mode 2 as well
     }
   };
   if (stop) {
@@ -1215,7 +1229,7 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
     uint8_t nb;
   };
   struct RwF {
-    V r, q, p, x;
+    V r, q, p, x, p2;
     uint8_t nb;
   };
   using Rw = typename std::conditional<SLOW, RwS, RwF>::type;
@@ -1232,8 +1246,9 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
     w.r = (rin + pb)[cv];
     w.q = (qin + pb)[cv];
     w.p = (pin + pb)[cv];
-    w.x = with_x ? (x + pb)[cv] : V(0);
+    w.x = (with_x && XM != 2) ? (x + pb)[cv] : V(0);
     if constexpr (SLOW) w.ds = ((const V*)o.dslow + pb)[(un && (nb & NB_SLOW) && o.precond) ? c : 0u];
+    else w.p2 = (with_x && XM == 3) ? (p2in + pb)[cv] : V(0);
   };
   // r_k, and p_k = z_k + beta p_{k-1} (0 off the unknowns)
   auto r_of = [&](const Rw& w) -> V { return first ? w.r : w.r - al * w.q; };
@@ -1265,7 +1280,8 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
       x64[idx] = x64[idx] + (double)(w.x + al * w.p);
       (x + pb)[col[u]] = V(0);
     } else if (!first) {
-      (x + pb)[col[u]] = w.x + al * w.p;
+      if constexpr (XM == 1) (x + pb)[col[u]] = w.x + al * w.p;
+      if constexpr (XM == 3) (x + pb)[col[u]] = (w.x + al2 * w.p2) + al * w.p;
     }
     const V zv = rk * dw_of(w);
     a[0] += (double)rk * (double)zv;
@@ -1900,7 +1916,7 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
     return MPF_INVALID_ARG;
   }
   if (nsb > MAX_BLK) nsb = MAX_BLK;
-  launch(c, "pcg_init", dim3(L.nblk), dim3(PB), 0, k_init<V, Dacc, OPk>, o, L, b, x, r, pbuf[0], pbuf[1],
+  launch(c, "pcg_init", dim3(L.nblk), dim3(PB), 0, k_init<V, Dacc, OPk>, o, L, b, x, r, pbuf[0], pbuf[1], (V*)c.vp3,
          mixed ? x64 : (double*)nullptr, P, c.scal, npad);
   int k = 0;
   bool fold_next = false;
@@ -1914,12 +1930,18 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
     if (fused) {
       float* rb[2] = {(float*)c.vr, (float*)c.vr2};
       float* qb[2] = {(float*)c.vq, (float*)c.vq2};
-      float* pf[2] = {(float*)c.vp, (float*)c.vp2};
+      // p_k in pf[(k + 1) % 3]: a launch reads p_k-1 and p_k-2 and writes p_k
+      float* pf[3] = {(float*)c.vp, (float*)c.vp2, (float*)c.vp3};
       // partial sums: per launch T.nblk tile blocks then T.ns5 slow-row blocks
       double* S7[2] = {c.partials + 8 * MAX_BLK, c.partials + 22 * MAX_BLK};
       double* Rs = c.partials + 36 * MAX_BLK;
-      auto cg_iter = nslow > 0 ? k_cgf<OPk, true, false> : k_cgf<OPk, false, false>;
-      auto cg_steady = nslow > 0 ? k_cgf<OPk, true, true> : k_cgf<OPk, false, true>;
+      auto cg_iter = nslow > 0 ? k_cgf<OPk, true, false, 1> : k_cgf<OPk, false, false, 1>;
+      auto cg_steady = nslow > 0 ? k_cgf<OPk, true, true, 1> : k_cgf<OPk, false, true, 1>;
+      // without slow rows x advances every other launch (modes 2 and 3 of
+      // k_cgf), and fully before each true-residual pass
+      auto cg_skip = k_cgf<OPk, false, true, 2>;
+      auto cg_double = k_cgf<OPk, false, true, 3>;
+      bool pending = false;  // the last launch left alpha p in x's debt
       auto res_tiles = nslow > 0 ? k_resid64<OPk, 2> : k_resid64<OPk, 0>;
       // FP64 true residual: the tile pass, then the slow rows' list pass
       auto true_resid = [&](int kk, float* rr_, const float* qq_) {
@@ -1935,22 +1957,39 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
         const int k_end = k + check;
         for (; k < k_end && k <= cap; ++k) {
           const double* pres = first == 1 ? (const double*)P.U1 : (first == 2 || use_res) ? Rs : nullptr;
-          if (qfree)  // q_k stored only for the true-residual pass after this launch
+          float* p_prev = pf[k % 3];
+          float* p_next = pf[(k + 1) % 3];
+          const float* p_prev2 = pf[(k + 2) % 3];
+          if (qfree) {  // q_k stored only for the true-residual pass after this launch
             launch(c, "pcg_cg_iterq", dim3(T.nblk), dim3(SQ), 0, k_cgq<OPk>, o, T, Tr.nblk, first, npad,
+                   (const double*)S7[(k + 1) & 1], pres, S7[k & 1], (const float*)rb[k & 1], (const float*)p_prev,
+                   rb[(k + 1) & 1], p_next, (k + 1) % R == 0 ? qb[(k + 1) & 1] : (float*)nullptr, (float*)x, x64,
+                   fold, c.scal, k, tol, cap);
+          } else {
+            auto kern = cg_iter;
+            if (first == 0 && fold == 0) {
+              kern = cg_steady;
+              if (nslow == 0) {
+                if (pending) {
+                  kern = cg_double;
+                  pending = false;
+                } else if ((k + 1) % R != 0 && k < cap) {
+                  kern = cg_skip;
+                  pending = true;
+                }
+              }
+            } else {
+              pending = false;
+            }
+            launch(c, "pcg_cg_iter", dim3(T.nblk), dim3(ST), 0, kern, o, T, first, npad,
                    (const double*)S7[(k + 1) & 1], pres, S7[k & 1], (const float*)rb[k & 1],
-                   (const float*)pf[k & 1], rb[(k + 1) & 1], pf[(k + 1) & 1],
-                   (k + 1) % R == 0 ? qb[(k + 1) & 1] : (float*)nullptr, (float*)x, x64, fold, c.scal, k, tol,
-                   cap);
-          else
-            launch(c, "pcg_cg_iter", dim3(T.nblk), dim3(ST), 0, (first == 0 && fold == 0) ? cg_steady : cg_iter, o, T,
-                   first, npad,
-                   (const double*)S7[(k + 1) & 1], pres, S7[k & 1], (const float*)rb[k & 1],
-                   (const float*)qb[k & 1], (const float*)pf[k & 1], rb[(k + 1) & 1], qb[(k + 1) & 1],
-                   pf[(k + 1) & 1], (float*)x, x64, fold, c.scal, k, tol, cap);
+                   (const float*)qb[k & 1], (const float*)p_prev, rb[(k + 1) & 1], qb[(k + 1) & 1], p_next,
+                   (float*)x, x64, fold, c.scal, k, tol, cap, p_prev2);
+          }
           if (nslow > 0)
             launch(c, "pcg_slow_q5", dim3(nsb), dim3(PB), 0, k_slow_q5<OPk>, o, (const uint32_t*)slow_list, nslow,
-                   (const float*)pf[(k + 1) & 1], (const float*)rb[(k + 1) & 1], qb[(k + 1) & 1],
-                   S7[k & 1] + FDOT * T.nblk, c.scal, k);
+                   (const float*)p_next, (const float*)rb[(k + 1) & 1], qb[(k + 1) & 1], S7[k & 1] + FDOT * T.nblk,
+                   c.scal, k);
           first = 0;
           fold = 0;
           use_res = false;

@@ -45,8 +45,11 @@ KERNEL_BYTES = {
     # mask 1 per cell; b 4, x64 8, x 4 read, r 4 written per unknown
     "pcg_resid64": (1, 20, 0),
     # one-kernel iteration (MIXED form without slow rows): mask 1 per cell;
-    # r, q, p_{k-1}, x read and r, q, p_k, x written per unknown
-    "pcg_cg_iter": (1, 32, 0),
+    # per unknown r, q, p_k-1 read and r, q, p_k written (24), and x advanced
+    # every other launch (read, written, p_k-2 read: 12), fully before each
+    # true residual (x 8) and folded into x64 after it (x64 16 more): over a
+    # cycle of 20 launches 9 x 24 + 9 x 36 + 32 + 48 = 620, 31 per launch
+    "pcg_cg_iter": (1, 31, 0),
     # the same iteration forming q_{k-1} = A p_{k-1} again (cfg.pcg_recompute_q):
     # mask 1 per cell; r, p_{k-1}, x read and r, p_k, x written per unknown
     # (q_k is written by one launch in refine_every, before the true
