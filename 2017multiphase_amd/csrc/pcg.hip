@@ -417,6 +417,20 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Vector stores that nothing reads again within the launch (the next launch
+// or pass reads them): nontemporal, so they do not displace the lines that
+// neighbouring tiles of the same launch are about to read.
+template <class T>
+__device__ __forceinline__ void st_nt(T* p, T v) {
+  __builtin_nontemporal_store(v, p);
+}
+template <class T>
+__device__ __forceinline__ void st_nt4(T* p, const T (&e)[4]) {
+  using T4 = T __attribute__((ext_vector_type(4)));
+  const T4 t = {e[0], e[1], e[2], e[3]};
+  __builtin_nontemporal_store(t, (T4*)p);
+}
+
 // q of a slow row (flags, per-face 1/rho, heat diagonal) from the p_k of its
 // six neighbours (order +x, +y, +z, -x, -y, -z; 0 off the unknowns).
 template <class V, int OPk>
@@ -630,8 +644,8 @@ __global__ void __launch_bounds__(ST, (sizeof(V) == 4 && !SLOW) ? 6 : 4) k_spmv(
           const V qv = fast ? sum * qc : V(0);
           a_pq += (Dacc)pcv * (Dacc)qv;
           if (nb & NB_UNKNOWN) {
-            (pc + pb)[col[u]] = pcv;
-            if (fast) (q + pb)[col[u]] = qv;
+            st_nt(pc + pb + col[u], pcv);
+            if (fast) st_nt(q + pb + col[u], qv);
           }
         }
       }
@@ -801,16 +815,16 @@ __global__ void __launch_bounds__(PB) k_update(OpArgs o, Launch L, V* __restrict
                ro.e[e], x64o.e[e]);
       }
       if (un4 == 0x40404040u) {
-        *(Q4*)(r + b4) = ro;
-        if (mode != 0) *(Q4*)(x + b4) = xo;
-        if (fold) *(D4*)(x64 + b4) = x64o;
+        st_nt4(r + b4, ro.e);
+        if (mode != 0) st_nt4(x + b4, xo.e);
+        if (fold) st_nt4(x64 + b4, x64o.e);
       } else {
 #pragma unroll
         for (int e = 0; e < 4; ++e)
           if ((m4[u] >> (8 * e)) & NB_UNKNOWN) {
-            r[b4 + e] = ro.e[e];
-            if (mode != 0) x[b4 + e] = xo.e[e];
-            if (fold) x64[b4 + e] = x64o.e[e];
+            st_nt(r + b4 + e, ro.e[e]);
+            if (mode != 0) st_nt(x + b4 + e, xo.e[e]);
+            if (fold) st_nt(x64 + b4 + e, x64o.e[e]);
           }
       }
     }
@@ -1009,7 +1023,7 @@ __global__ void __launch_bounds__(ST, MODE == 1 ? 2 : 4) k_resid64(OpArgs o, Til
           }
           const double rv = (double)BV[s0][u] - ax;
           const float rf = (float)rv;
-          (r + pb)[col[u]] = rf;
+          st_nt(r + pb + col[u], rf);
           a_rr += rv * rv;
           const float dv = !o.precond ? 1.0f
                            : (MODE == 1 && (nb & NB_SLOW)) ? dslow_at<float>(o, idx)
@@ -1274,14 +1288,14 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
   auto own = [&](int i, int u, const Rw& w, V rk) {
     if (!(w.nb & NB_UNKNOWN)) return;
     const int64_t pb = (int64_t)i * sx;
-    (rout + pb)[col[u]] = rk;
+    st_nt(rout + pb + col[u], rk);
     if (fold) {
       const int64_t idx = pb + col[u];
       x64[idx] = x64[idx] + (double)(w.x + al * w.p);
       (x + pb)[col[u]] = V(0);
     } else if (!first) {
-      if constexpr (XM == 1) (x + pb)[col[u]] = w.x + al * w.p;
-      if constexpr (XM == 3) (x + pb)[col[u]] = (w.x + al2 * w.p2) + al * w.p;
+      if constexpr (XM == 1) st_nt(x + pb + col[u], w.x + al * w.p);
+      if constexpr (XM == 3) st_nt(x + pb + col[u], (w.x + al2 * w.p2) + al * w.p);
     }
     const V zv = rk * dw_of(w);
     a[0] += (double)rk * (double)zv;
@@ -1354,14 +1368,14 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
           sum = (nb & 32) ? sum + (nzm - pcv) : sum;
           if (SLOW && (nb & NB_SLOW)) {
             // slow row: p_k only, q_k from k_slow_q5
-            if (nb & NB_UNKNOWN) (pout + pb)[col[u]] = pcv;
+            if (nb & NB_UNKNOWN) st_nt(pout + pb + col[u], pcv);
           } else if (nb & NB_UNKNOWN) {
             const V qv = sum * qc;
             const V rk = RV[s0][u];
             const V dv = d_of(nb);
             const V zv = rk * dv;
-            (pout + pb)[col[u]] = pcv;
-            (qout + pb)[col[u]] = qv;
+            st_nt(pout + pb + col[u], pcv);
+            st_nt(qout + pb + col[u], qv);
             a[2] += (double)pcv * (double)qv;
             a[3] += (double)qv * (double)zv;
             a[4] += (double)qv * (double)(dv * qv);
