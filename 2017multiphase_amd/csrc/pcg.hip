@@ -417,6 +417,14 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// a + x y in FP64 for FP32 x, y: their product is exact in FP64, so the fused
+// form rounds exactly as a separate multiply and add (one instruction, not two).
+template <class X, class Y>
+__device__ __forceinline__ double acc64(double a, X x, Y y) {
+  static_assert(std::is_same<X, float>::value && std::is_same<Y, float>::value, "FP32 factors only");
+  return __builtin_fma((double)x, (double)y, a);
+}
+
 // Vector stores that nothing reads again within the launch (the next launch
 // or pass reads them): nontemporal, so they do not displace the lines that
 // neighbouring tiles of the same launch are about to read.
@@ -1028,11 +1036,11 @@ __global__ void __launch_bounds__(ST, MODE == 1 ? 2 : 4) k_resid64(OpArgs o, Til
           const float dv = !o.precond ? 1.0f
                            : (MODE == 1 && (nb & NB_SLOW)) ? dslow_at<float>(o, idx)
                                                            : dtab[__popc((unsigned)(nb & 63))];
-          a_rz += (double)rf * (double)(rf * dv);
+          a_rz = acc64(a_rz, rf, rf * dv);
           if (qf) {
             const float qv = QN[d & 1][u];
-            a_qz += (double)qv * (double)(rf * dv);
-            a_qr += (double)qv * (double)rf;
+            a_qz = acc64(a_qz, qv, rf * dv);
+            a_qr = acc64(a_qr, qv, rf);
           }
         }
       }
@@ -1151,8 +1159,10 @@ __device__ __forceinline__ CgScalars cg_scalars(const Tiling& T, int nres, int f
 // 1: x += alpha_k-1 p_k-1; 2: not at all (that term stays pending, x is
 // neither read nor written); 3: x = (x + alpha_k-2 p_k-2) + alpha_k-1 p_k-1,
 // the pending term first (p2in = p_k-2, alpha_k-2 as launch k-1 applied it),
-// so x takes the same roundings as when it advances every launch.  A launch
-// that stops the solve advances x by all it owes.
+// so x takes the same roundings as when it advances every launch; 4: the
+// launch after a true-residual pass (first = 0, fold = 1 as constants), whose
+// x64 += x + alpha p reads x64 in the plane pipeline with the vectors.  A
+// launch that stops the solve advances x by all it owes.
 template <int OPk, bool SLOW, bool STEADY, int XM>
 __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, int nbu, const double* __restrict__ Pprev,
                                                const double* __restrict__ Pres, double* __restrict__ Pout,
@@ -1162,7 +1172,8 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
                                                float* __restrict__ x, double* __restrict__ x64, int fold,
                                                SolveScalars* sc, int k, double tol, int cap,
                                                const float* __restrict__ p2in) {
-  static_assert(XM == 1 || (STEADY && !SLOW && (XM == 2 || XM == 3)), "x modes 2 and 3 are steady launches");
+  static_assert(XM == 1 || (STEADY && !SLOW && (XM == 2 || XM == 3)) || (XM == 4 && !STEADY && !SLOW),
+                "x modes 2 and 3 are steady launches, mode 4 the fold launch");
   using V = float;
   // a launch after the one that stopped the solve does nothing (the
   // stopping launch's own late blocks still finish its x update)
@@ -1170,6 +1181,10 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
   if (STEADY) {
     first = 0;
     fold = 0;
+  }
+  if (XM == 4) {
+    first = 0;
+    fold = 1;
   }
   // 1/diag of the fast rows by open-face count (1 without a preconditioner)
   __shared__ V dtab[8];
@@ -1278,28 +1293,33 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
     return first ? rk * dw_of(w) : rk * dw_of(w) + bev * w.p;
   };
   constexpr int UNR = 6;
+  static_assert(UNR % 6 == 0, "slots by d % 3 and d & 1 repeat across the unrolled loop");
   Rw R[3][RPT], HR[3];
   uint8_t M[3][RPT], HM[3];
   V PV[3][RPT], RV[3][RPT];  // p_k, r_k of planes i-1 .. i+1
   uint8_t NB[3][RPT];
   double a[FDOT] = {0, 0, 0, 0, 0, 0, 0};
+  // x64 of planes i+1 and i+2 in mode 4 (slot by plane parity within the
+  // unrolled loop), loaded a plane ahead of their use
+  double XD[2][RPT] = {};
   // a cell of this block's planes once its r_k is formed: r_k and x stored,
-  // r.z and r.r summed
-  auto own = [&](int i, int u, const Rw& w, V rk) {
+  // r.z and r.r summed (xd: its x64 in mode 4)
+  auto own = [&](int i, int u, const Rw& w, V rk, double xd) {
     if (!(w.nb & NB_UNKNOWN)) return;
     const int64_t pb = (int64_t)i * sx;
     st_nt(rout + pb + col[u], rk);
     if (fold) {
       const int64_t idx = pb + col[u];
-      x64[idx] = x64[idx] + (double)(w.x + al * w.p);
-      (x + pb)[col[u]] = V(0);
+      if constexpr (XM == 4) st_nt(x64 + idx, xd + (double)(w.x + al * w.p));
+      else x64[idx] = x64[idx] + (double)(w.x + al * w.p);
+      st_nt(x + idx, V(0));
     } else if (!first) {
       if constexpr (XM == 1) st_nt(x + pb + col[u], w.x + al * w.p);
       if constexpr (XM == 3) st_nt(x + pb + col[u], (w.x + al2 * w.p2) + al * w.p);
     }
     const V zv = rk * dw_of(w);
-    a[0] += (double)rk * (double)zv;
-    a[1] += (double)rk * (double)rk;
+    a[0] = acc64(a[0], rk, zv);
+    a[1] = acc64(a[1], rk, rk);
   };
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
@@ -1311,7 +1331,8 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
     PV[0][u] = p_of(w1, RV[0][u]);
     NB[0][u] = w1.nb;
     pl[i_lo % 3][jl0 + u + 1][kl + 1] = PV[0][u];
-    if (i_lo < i_hi) own(i_lo, u, w1, RV[0][u]);
+    if (i_lo < i_hi)
+      own(i_lo, u, w1, RV[0][u], XM == 4 ? (x64 + plane(i_lo))[(w1.nb & NB_UNKNOWN) ? col[u] : 0u] : 0.0);
   }
   {
     Rw w;
@@ -1320,9 +1341,15 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
   }
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
-    load_v(i_lo + 1, cok[u], load_m(i_lo + 1, cok[u], col[u]), col[u], R[1][u], true);
-    load_v(i_lo + 2, cok[u], load_m(i_lo + 2, cok[u], col[u]), col[u], R[2][u], true);
+    const uint8_t m1 = load_m(i_lo + 1, cok[u], col[u]);
+    load_v(i_lo + 1, cok[u], m1, col[u], R[1][u], true);
+    M[2][u] = load_m(i_lo + 2, cok[u], col[u]);
+    load_v(i_lo + 2, cok[u], M[2][u], col[u], R[2][u], true);
     M[0][u] = load_m(i_lo + 3, cok[u], col[u]);
+    if constexpr (XM == 4) {
+      const uint8_t nb1 = (cok[u] && i_lo + 1 < g.nx) ? m1 : (uint8_t)0;
+      XD[1][u] = (x64 + plane(i_lo + 1))[(nb1 & NB_UNKNOWN) ? col[u] : 0u];
+    }
   }
   load_v(i_lo + 1, hok, load_m(i_lo + 1, hok, hcol), hcol, HR[1], false);
   load_v(i_lo + 2, hok, load_m(i_lo + 2, hok, hcol), hcol, HR[2], false);
@@ -1341,13 +1368,20 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
 #pragma unroll
         for (int u = 0; u < RPT; ++u) load_v(i + 3, cok[u], M[s0][u], col[u], R[s0][u], true);
         load_v(i + 3, hok, HM[s0], hcol, HR[s0], false);
+        if constexpr (XM == 4) {  // x64 of plane i+2 (its mask M[s2] was read two planes ago)
+#pragma unroll
+          for (int u = 0; u < RPT; ++u) {
+            const uint8_t nb2 = (cok[u] && i + 2 < g.nx) ? M[s2][u] : (uint8_t)0;
+            XD[d & 1][u] = (x64 + plane(i + 2))[(nb2 & NB_UNKNOWN) ? col[u] : 0u];
+          }
+        }
 #pragma unroll
         for (int u = 0; u < RPT; ++u) {
           RV[s1][u] = r_of(R[s1][u]);
           PV[s1][u] = p_of(R[s1][u], RV[s1][u]);
           NB[s1][u] = R[s1][u].nb;
           pl[sn][jl0 + u + 1][kl + 1] = PV[s1][u];
-          if (i + 1 < i_hi) own(i + 1, u, R[s1][u], RV[s1][u]);
+          if (i + 1 < i_hi) own(i + 1, u, R[s1][u], RV[s1][u], XD[(d + 1) & 1][u]);
         }
         if (ring) pl[sn][hj + 1][hk + 1] = p_of(HR[s1], r_of(HR[s1]));
         lds_barrier();
@@ -1376,11 +1410,11 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
             const V zv = rk * dv;
             st_nt(pout + pb + col[u], pcv);
             st_nt(qout + pb + col[u], qv);
-            a[2] += (double)pcv * (double)qv;
-            a[3] += (double)qv * (double)zv;
-            a[4] += (double)qv * (double)(dv * qv);
-            a[5] += (double)qv * (double)rk;
-            a[6] += (double)qv * (double)qv;
+            a[2] = acc64(a[2], pcv, qv);
+            a[3] = acc64(a[3], qv, zv);
+            a[4] = acc64(a[4], qv, dv * qv);
+            a[5] = acc64(a[5], qv, rk);
+            a[6] = acc64(a[6], qv, qv);
           }
         }
       }
@@ -1536,17 +1570,17 @@ __global__ void __launch_bounds__(SQ, 4) k_cgq(OpArgs o, Tiling T, int nres, int
   auto own = [&](int i, const W& w, V rk) __attribute__((always_inline)) {
     if (!(w.nb & NB_UNKNOWN)) return;
     const int64_t pb = (int64_t)i * sx;
-    (rout + pb)[col] = rk;
+    st_nt(rout + pb + col, rk);
     if (fold) {
       const int64_t idx = pb + col;
       x64[idx] = x64[idx] + (double)(w.x + al * w.p);
       (x + pb)[col] = V(0);
     } else if (!first) {
-      (x + pb)[col] = w.x + al * w.p;
+      st_nt(x + pb + col, w.x + al * w.p);
     }
     const V zv = rk * d_of(w.nb);
-    a[0] += (double)rk * (double)zv;
-    a[1] += (double)rk * (double)rk;
+    a[0] = acc64(a[0], rk, zv);
+    a[1] = acc64(a[1], rk, rk);
   };
   // registers by plane q in slot (q - i_lo) & 3: loads L (tile cell), H
   // (ring cell); p_{k-1} PP, PH; p_k, r_k and mask of the tile cell PK, RK, NK
@@ -1655,13 +1689,13 @@ __global__ void __launch_bounds__(SQ, 4) k_cgq(OpArgs o, Tiling T, int nres, int
           const V rk = RK[s0];
           const V dv = d_of(nb);
           const V zv = rk * dv;
-          (pout + pb)[col] = pcv;
-          if (qout) (qout + pb)[col] = qv;
-          a[2] += (double)pcv * (double)qv;
-          a[3] += (double)qv * (double)zv;
-          a[4] += (double)qv * (double)(dv * qv);
-          a[5] += (double)qv * (double)rk;
-          a[6] += (double)qv * (double)qv;
+          st_nt(pout + pb + col, pcv);
+          if (qout) st_nt(qout + pb + col, qv);
+          a[2] = acc64(a[2], pcv, qv);
+          a[3] = acc64(a[3], qv, zv);
+          a[4] = acc64(a[4], qv, dv * qv);
+          a[5] = acc64(a[5], qv, rk);
+          a[6] = acc64(a[6], qv, qv);
         }
         lds_barrier();
       }
@@ -1700,11 +1734,11 @@ __global__ void __launch_bounds__(PB) k_slow_q5(OpArgs o, const uint32_t* __rest
       const V r = rk[idx];
       const V dv = o.precond ? dslow_at<V>(o, idx) : V(1);
       const V zv = r * dv;
-      a[2] += (double)pcv * (double)qv;
-      a[3] += (double)qv * (double)zv;
-      a[4] += (double)qv * (double)(dv * qv);
-      a[5] += (double)qv * (double)r;
-      a[6] += (double)qv * (double)qv;
+      a[2] = acc64(a[2], pcv, qv);
+      a[3] = acc64(a[3], qv, zv);
+      a[4] = acc64(a[4], qv, dv * qv);
+      a[5] = acc64(a[5], qv, r);
+      a[6] = acc64(a[6], qv, qv);
     }
   }
   write_partials<FDOT>(a, S7);
@@ -1743,11 +1777,11 @@ __global__ void __launch_bounds__(PB) k_slow_resid64(OpArgs o, const uint32_t* _
     const float rf = (float)rv;
     r[idx] = rf;
     const float dv = o.precond ? dslow_at<float>(o, idx) : 1.0f;
-    a[0] += (double)rf * (double)(rf * dv);
+    a[0] = acc64(a[0], rf, rf * dv);
     a[1] += rv * rv;
     const float qv = qf[idx];
-    a[2] += (double)qv * (double)(rf * dv);
-    a[3] += (double)qv * (double)rf;
+    a[2] = acc64(a[2], qv, rf * dv);
+    a[3] = acc64(a[3], qv, rf);
   }
   write_partials<4>(a, R4);
 }
@@ -1955,6 +1989,7 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
       // k_cgf), and fully before each true-residual pass
       auto cg_skip = k_cgf<OPk, false, true, 2>;
       auto cg_double = k_cgf<OPk, false, true, 3>;
+      auto cg_fold = k_cgf<OPk, false, false, 4>;  // x64 += x + alpha p after a true residual
       bool pending = false;  // the last launch left alpha p in x's debt
       auto res_tiles = nslow > 0 ? k_resid64<OPk, 2> : k_resid64<OPk, 0>;
       // FP64 true residual: the tile pass, then the slow rows' list pass
@@ -1994,6 +2029,7 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
               }
             } else {
               pending = false;
+              if (first == 0 && nslow == 0) kern = cg_fold;
             }
             launch(c, "pcg_cg_iter", dim3(T.nblk), dim3(ST), 0, kern, o, T, first, npad,
                    (const double*)S7[(k + 1) & 1], pres, S7[k & 1], (const float*)rb[k & 1],
