@@ -1941,7 +1941,11 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
   if (nslow > 0)
     launch(c, "pcg_slow_list", dim3(nblocks(G, 256)), dim3(256), 0, k_slow_list, G, (const uint8_t*)c.nbm,
            (const uint32_t*)c.count, slow_list);
+  // slow-row blocks: grid-stride over the list, at most MAX_BLK of them (the
+  // single-kernel iteration sums their partials after its tile blocks, in a
+  // region that holds 2 * MAX_BLK blocks)
   int nsb = (int)((nslow + PB - 1) / PB);
+  if (nsb > MAX_BLK) nsb = MAX_BLK;
   auto spmv = nslow > 0 ? k_spmv<V, Dacc, OPk, true> : k_spmv<V, Dacc, OPk, false>;
   auto resid64 = nslow > 0 ? k_resid64<OPk, 1> : k_resid64<OPk, 0>;
   // the MIXED form without slow rows runs one kernel per iteration (k_cgf)
@@ -1963,7 +1967,6 @@ int cg_solve(Ctx& c, const V* b, V* x, double* x64, bool ref, int precond, int c
     c.err = "grid too large for the solver's reduction buffers";
     return MPF_INVALID_ARG;
   }
-  if (nsb > MAX_BLK) nsb = MAX_BLK;
   launch(c, "pcg_init", dim3(L.nblk), dim3(PB), 0, k_init<V, Dacc, OPk>, o, L, b, x, r, pbuf[0], pbuf[1], (V*)c.vp3,
          mixed ? x64 : (double*)nullptr, P, c.scal, npad);
   int k = 0;
