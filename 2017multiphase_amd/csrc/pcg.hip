@@ -1321,39 +1321,53 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
     a[0] = acc64(a[0], rk, zv);
     a[1] = acc64(a[1], rk, rk);
   };
+  // prologue: the masks of planes i_lo - 1 .. i_lo + 3 first, then every
+  // vector load, then the arithmetic: two memory round trips before the run
+  // starts instead of one per plane and cell
+  uint8_t MA[2][RPT], HA, HB[2];
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
-    Rw w0, w1;
-    load_v(i_lo - 1, cok[u], load_m(i_lo - 1, cok[u], col[u]), col[u], w0, false);
-    load_v(i_lo, cok[u], load_m(i_lo, cok[u], col[u]), col[u], w1, true);
-    PV[2][u] = p_of(w0, r_of(w0));
-    RV[0][u] = r_of(w1);
-    PV[0][u] = p_of(w1, RV[0][u]);
-    NB[0][u] = w1.nb;
-    pl[i_lo % 3][jl0 + u + 1][kl + 1] = PV[0][u];
-    if (i_lo < i_hi)
-      own(i_lo, u, w1, RV[0][u], XM == 4 ? (x64 + plane(i_lo))[(w1.nb & NB_UNKNOWN) ? col[u] : 0u] : 0.0);
-  }
-  {
-    Rw w;
-    load_v(i_lo, hok, load_m(i_lo, hok, hcol), hcol, w, false);
-    if (ring) pl[i_lo % 3][hj + 1][hk + 1] = p_of(w, r_of(w));
-  }
-#pragma unroll
-  for (int u = 0; u < RPT; ++u) {
-    const uint8_t m1 = load_m(i_lo + 1, cok[u], col[u]);
-    load_v(i_lo + 1, cok[u], m1, col[u], R[1][u], true);
+    MA[0][u] = load_m(i_lo - 1, cok[u], col[u]);
+    MA[1][u] = load_m(i_lo, cok[u], col[u]);
+    M[1][u] = load_m(i_lo + 1, cok[u], col[u]);
     M[2][u] = load_m(i_lo + 2, cok[u], col[u]);
-    load_v(i_lo + 2, cok[u], M[2][u], col[u], R[2][u], true);
     M[0][u] = load_m(i_lo + 3, cok[u], col[u]);
+  }
+  HA = load_m(i_lo, hok, hcol);
+  HB[0] = load_m(i_lo + 1, hok, hcol);
+  HB[1] = load_m(i_lo + 2, hok, hcol);
+  HM[0] = load_m(i_lo + 3, hok, hcol);
+  Rw W0[RPT], W1[RPT], WH;
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    load_v(i_lo - 1, cok[u], MA[0][u], col[u], W0[u], false);
+    load_v(i_lo, cok[u], MA[1][u], col[u], W1[u], true);
+  }
+  load_v(i_lo, hok, HA, hcol, WH, false);
+  double X0[RPT] = {};
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    load_v(i_lo + 1, cok[u], M[1][u], col[u], R[1][u], true);
+    load_v(i_lo + 2, cok[u], M[2][u], col[u], R[2][u], true);
     if constexpr (XM == 4) {
-      const uint8_t nb1 = (cok[u] && i_lo + 1 < g.nx) ? m1 : (uint8_t)0;
+      const uint8_t nb0 = (cok[u] && i_lo < g.nx) ? MA[1][u] : (uint8_t)0;
+      X0[u] = (x64 + plane(i_lo))[(nb0 & NB_UNKNOWN) ? col[u] : 0u];
+      const uint8_t nb1 = (cok[u] && i_lo + 1 < g.nx) ? M[1][u] : (uint8_t)0;
       XD[1][u] = (x64 + plane(i_lo + 1))[(nb1 & NB_UNKNOWN) ? col[u] : 0u];
     }
   }
-  load_v(i_lo + 1, hok, load_m(i_lo + 1, hok, hcol), hcol, HR[1], false);
-  load_v(i_lo + 2, hok, load_m(i_lo + 2, hok, hcol), hcol, HR[2], false);
-  HM[0] = load_m(i_lo + 3, hok, hcol);
+  load_v(i_lo + 1, hok, HB[0], hcol, HR[1], false);
+  load_v(i_lo + 2, hok, HB[1], hcol, HR[2], false);
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    PV[2][u] = p_of(W0[u], r_of(W0[u]));
+    RV[0][u] = r_of(W1[u]);
+    PV[0][u] = p_of(W1[u], RV[0][u]);
+    NB[0][u] = W1[u].nb;
+    pl[i_lo % 3][jl0 + u + 1][kl + 1] = PV[0][u];
+    if (i_lo < i_hi) own(i_lo, u, W1[u], RV[0][u], X0[u]);
+  }
+  if (ring) pl[i_lo % 3][hj + 1][hk + 1] = p_of(WH, r_of(WH));
   const V qc = qcoef<V>(o);
   for (int ib = i_lo; ib < i_hi; ib += UNR) {
 #pragma unroll
