@@ -1189,11 +1189,6 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
   // 1/diag of the fast rows by open-face count (1 without a preconditioner)
   __shared__ V dtab[8];
   if (threadIdx.x < 8) dtab[threadIdx.x] = (threadIdx.x < 7 && o.precond) ? dinv_tab<V>(o, threadIdx.x) : V(1);
-  const CgScalars cs = cg_scalars<ST>(T, T.nblk + T.ns5, first, nbu, Pprev, Pres, sc, k, tol, cap);
-  const bool stop = cs.stop, nonfin = cs.nonfin, zero_pq = cs.zero_pq;
-  const V al = (V)cs.ald, bev = (V)cs.be;
-  const V al2 = (XM == 3) ? sc->alh[(k - 1) & 1] : V(0);  // alpha_k-2, as launch k-1 applied it
-  __shared__ V pl[3][LJ][LK];
   const Grid& g = o.g;
   const int lb = tile_of_block(T);
   const int tk = lb % T.ntk;
@@ -1215,6 +1210,46 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
     col[u] = cok[u] ? (uint32_t)(j * g.nz + kk) : 0u;
   }
   auto plane = [&](int i) -> int64_t { return (int64_t)min(max(i, 0), g.nx - 1) * sx; };
+  int hj = 0, hk = 0;
+  const bool ring = tid < NRING;
+  if (tid < 2 * TK) {
+    hj = (tid < TK) ? -1 : TJ;
+    hk = tid & (TK - 1);
+  } else if (ring) {
+    const int t2 = tid - 2 * TK;
+    hj = t2 % TJ;
+    hk = (t2 < TJ) ? -1 : TK;
+  }
+  const bool hok = ring && (j0 + hj) >= 0 && (j0 + hj) < g.ny && (k0 + hk) >= 0 && (k0 + hk) < g.nz;
+  const uint32_t hcol = hok ? (uint32_t)((j0 + hj) * g.nz + (k0 + hk)) : 0u;
+  auto load_m = [&](int i, bool ok, uint32_t c) -> uint8_t {
+    const uint8_t* mp = o.nbm + plane(i);
+    return mp[(ok && i >= 0 && i < g.nx) ? c : 0u];
+  };
+  // masks of planes i_lo - 1 .. i_lo + 3 (the plane loop's slots M, HM and
+  // the prologue's MA, HA, HB); they do not depend on alpha or beta
+  uint8_t M[3][RPT], HM[3];
+  uint8_t MA[2][RPT], HA, HB[2];
+  auto load_masks = [&]() {
+#pragma unroll
+    for (int u = 0; u < RPT; ++u) {
+      MA[0][u] = load_m(i_lo - 1, cok[u], col[u]);
+      MA[1][u] = load_m(i_lo, cok[u], col[u]);
+      M[1][u] = load_m(i_lo + 1, cok[u], col[u]);
+      M[2][u] = load_m(i_lo + 2, cok[u], col[u]);
+      M[0][u] = load_m(i_lo + 3, cok[u], col[u]);
+    }
+    HA = load_m(i_lo, hok, hcol);
+    HB[0] = load_m(i_lo + 1, hok, hcol);
+    HB[1] = load_m(i_lo + 2, hok, hcol);
+    HM[0] = load_m(i_lo + 3, hok, hcol);
+  };
+  if constexpr (XM != 4) load_masks();
+  const CgScalars cs = cg_scalars<ST>(T, T.nblk + T.ns5, first, nbu, Pprev, Pres, sc, k, tol, cap);
+  const bool stop = cs.stop, nonfin = cs.nonfin, zero_pq = cs.zero_pq;
+  const V al = (V)cs.ald, bev = (V)cs.be;
+  const V al2 = (XM == 3) ? sc->alh[(k - 1) & 1] : V(0);  // alpha_k-2, as launch k-1 applied it
+  __shared__ V pl[3][LJ][LK];
   // x += alpha p_{k-1} on the unknowns (x64 += x + alpha p, x = 0 when folding)
   auto advance_x = [&](int64_t idx, V pv) {
     const V xv = x[idx];
@@ -1237,18 +1272,6 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
       }
     return;
   }
-  int hj = 0, hk = 0;
-  const bool ring = tid < NRING;
-  if (tid < 2 * TK) {
-    hj = (tid < TK) ? -1 : TJ;
-    hk = tid & (TK - 1);
-  } else if (ring) {
-    const int t2 = tid - 2 * TK;
-    hj = t2 % TJ;
-    hk = (t2 < TJ) ? -1 : TK;
-  }
-  const bool hok = ring && (j0 + hj) >= 0 && (j0 + hj) < g.ny && (k0 + hk) >= 0 && (k0 + hk) < g.nz;
-  const uint32_t hcol = hok ? (uint32_t)((j0 + hj) * g.nz + (k0 + hk)) : 0u;
   // loads as in the spmv: uniform plane base, 32-bit column, every lane
   // issues every load (element 0 of the plane when there is nothing to read)
   // (the slow-row diagonal exists only in the SLOW form: a field that is
@@ -1262,10 +1285,6 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
     uint8_t nb;
   };
   using Rw = typename std::conditional<SLOW, RwS, RwF>::type;
-  auto load_m = [&](int i, bool ok, uint32_t c) -> uint8_t {
-    const uint8_t* mp = o.nbm + plane(i);
-    return mp[(ok && i >= 0 && i < g.nx) ? c : 0u];
-  };
   auto load_v = [&](int i, bool ok, uint8_t raw, uint32_t c, Rw& w, bool with_x) {
     const uint8_t nb = (ok && i >= 0 && i < g.nx) ? raw : (uint8_t)0;
     const int64_t pb = plane(i);
@@ -1295,7 +1314,6 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
   constexpr int UNR = 6;
   static_assert(UNR % 6 == 0, "slots by d % 3 and d & 1 repeat across the unrolled loop");
   Rw R[3][RPT], HR[3];
-  uint8_t M[3][RPT], HM[3];
   V PV[3][RPT], RV[3][RPT];  // p_k, r_k of planes i-1 .. i+1
   uint8_t NB[3][RPT];
   double a[FDOT] = {0, 0, 0, 0, 0, 0, 0};
@@ -1321,22 +1339,10 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
     a[0] = acc64(a[0], rk, zv);
     a[1] = acc64(a[1], rk, rk);
   };
-  // prologue: the masks of planes i_lo - 1 .. i_lo + 3 first, then every
-  // vector load, then the arithmetic: two memory round trips before the run
-  // starts instead of one per plane and cell
-  uint8_t MA[2][RPT], HA, HB[2];
-#pragma unroll
-  for (int u = 0; u < RPT; ++u) {
-    MA[0][u] = load_m(i_lo - 1, cok[u], col[u]);
-    MA[1][u] = load_m(i_lo, cok[u], col[u]);
-    M[1][u] = load_m(i_lo + 1, cok[u], col[u]);
-    M[2][u] = load_m(i_lo + 2, cok[u], col[u]);
-    M[0][u] = load_m(i_lo + 3, cok[u], col[u]);
-  }
-  HA = load_m(i_lo, hok, hcol);
-  HB[0] = load_m(i_lo + 1, hok, hcol);
-  HB[1] = load_m(i_lo + 2, hok, hcol);
-  HM[0] = load_m(i_lo + 3, hok, hcol);
+  // prologue: the masks of planes i_lo - 1 .. i_lo + 3 first (before the
+  // partial sums are read, except in the fold launch, whose registers are
+  // full), then every vector load, then the arithmetic
+  if constexpr (XM == 4) load_masks();
   Rw W0[RPT], W1[RPT], WH;
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
