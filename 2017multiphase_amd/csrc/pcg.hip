@@ -580,31 +580,40 @@ __global__ void __launch_bounds__(ST, (sizeof(V) == 4 && !SLOW) ? 6 : 4) k_spmv(
   uint8_t M[3][RPT], HM[3];  // masks of planes i+3, i+4
   V PV[3][RPT];              // p of planes i-1 .. i+1
   uint8_t NB[3][RPT];        // masks of planes i, i+1
-  // prologue: p of planes i_lo-1 and i_lo, vectors of i_lo+1 and i_lo+2, mask of i_lo+3
+  // prologue: p of planes i_lo-1 and i_lo, vectors of i_lo+1 and i_lo+2, mask
+  // of i_lo+3; every mask first, then every vector load, then the arithmetic
+  uint8_t MP[4][RPT], HP[3];  // masks of planes i_lo-1 .. i_lo+2
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
-    Raw<V> w0, w1;
-    load_v(i_lo - 1, cok[u], load_m(i_lo - 1, cok[u], col[u]), col[u], w0);
-    load_v(i_lo, cok[u], load_m(i_lo, cok[u], col[u]), col[u], w1);
-    PV[2][u] = form(w0);
-    PV[0][u] = form(w1);
-    NB[0][u] = w1.nb;
-    pl[i_lo % 3][jl0 + u + 1][kl + 1] = PV[0][u];
-  }
-  {
-    Raw<V> w;
-    load_v(i_lo, hok, load_m(i_lo, hok, hcol), hcol, w);
-    if (ring) pl[i_lo % 3][hj + 1][hk + 1] = form(w);
-  }
 #pragma unroll
-  for (int u = 0; u < RPT; ++u) {
-    load_v(i_lo + 1, cok[u], load_m(i_lo + 1, cok[u], col[u]), col[u], R[1][u]);
-    load_v(i_lo + 2, cok[u], load_m(i_lo + 2, cok[u], col[u]), col[u], R[2][u]);
+    for (int q = 0; q < 4; ++q) MP[q][u] = load_m(i_lo - 1 + q, cok[u], col[u]);
     M[0][u] = load_m(i_lo + 3, cok[u], col[u]);
   }
-  load_v(i_lo + 1, hok, load_m(i_lo + 1, hok, hcol), hcol, HR[1]);
-  load_v(i_lo + 2, hok, load_m(i_lo + 2, hok, hcol), hcol, HR[2]);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) HP[q] = load_m(i_lo + q, hok, hcol);
   HM[0] = load_m(i_lo + 3, hok, hcol);
+  Raw<V> W0[RPT], W1[RPT], WH;
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    load_v(i_lo - 1, cok[u], MP[0][u], col[u], W0[u]);
+    load_v(i_lo, cok[u], MP[1][u], col[u], W1[u]);
+  }
+  load_v(i_lo, hok, HP[0], hcol, WH);
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    load_v(i_lo + 1, cok[u], MP[2][u], col[u], R[1][u]);
+    load_v(i_lo + 2, cok[u], MP[3][u], col[u], R[2][u]);
+  }
+  load_v(i_lo + 1, hok, HP[1], hcol, HR[1]);
+  load_v(i_lo + 2, hok, HP[2], hcol, HR[2]);
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    PV[2][u] = form(W0[u]);
+    PV[0][u] = form(W1[u]);
+    NB[0][u] = W1[u].nb;
+    pl[i_lo % 3][jl0 + u + 1][kl + 1] = PV[0][u];
+  }
+  if (ring) pl[i_lo % 3][hj + 1][hk + 1] = form(WH);
   const V qc = qcoef<V>(o);
   Dacc a_pq = 0;
   for (int ib = i_lo; ib < i_hi; ib += UNR) {
@@ -941,31 +950,41 @@ __global__ void __launch_bounds__(ST, MODE == 1 ? 2 : 4) k_resid64(OpArgs o, Til
   float BV[3][RPT];
   float QN[2][RPT];  // q of plane i (slot d&1), loaded one step ahead (single-kernel form only)
   uint8_t NB[3][RPT];
+  // prologue: every mask of planes i_lo-1 .. i_lo+3 first, then every vector
+  // load, then the arithmetic
+  uint8_t MP[4][RPT], HP[3];  // masks of planes i_lo-1 .. i_lo+2
 #pragma unroll
   for (int u = 0; u < RPT; ++u) {
-    Rw w0, w1;
-    load_v(i_lo - 1, cok[u], load_m(i_lo - 1, cok[u], col[u]), col[u], w0);
-    load_v(i_lo, cok[u], load_m(i_lo, cok[u], col[u]), col[u], w1);
-    XV[2][u] = form(w0);
-    XV[0][u] = form(w1);
-    NB[0][u] = w1.nb;
-    BV[0][u] = w1.b;
-    pl[i_lo % 3][jl0 + u + 1][kl + 1] = XV[0][u];
-  }
-  {
-    Rw w;
-    load_v(i_lo, hok, load_m(i_lo, hok, hcol), hcol, w);
-    if (ring) pl[i_lo % 3][hj + 1][hk + 1] = form(w);
-  }
 #pragma unroll
-  for (int u = 0; u < RPT; ++u) {
-    load_v(i_lo + 1, cok[u], load_m(i_lo + 1, cok[u], col[u]), col[u], R[1][u]);
-    load_v(i_lo + 2, cok[u], load_m(i_lo + 2, cok[u], col[u]), col[u], R[2][u]);
+    for (int q = 0; q < 4; ++q) MP[q][u] = load_m(i_lo - 1 + q, cok[u], col[u]);
     M[0][u] = load_m(i_lo + 3, cok[u], col[u]);
   }
-  load_v(i_lo + 1, hok, load_m(i_lo + 1, hok, hcol), hcol, HR[1]);
-  load_v(i_lo + 2, hok, load_m(i_lo + 2, hok, hcol), hcol, HR[2]);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) HP[q] = load_m(i_lo + q, hok, hcol);
   HM[0] = load_m(i_lo + 3, hok, hcol);
+  Rw W0[RPT], W1[RPT], WH;
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    load_v(i_lo - 1, cok[u], MP[0][u], col[u], W0[u]);
+    load_v(i_lo, cok[u], MP[1][u], col[u], W1[u]);
+  }
+  load_v(i_lo, hok, HP[0], hcol, WH);
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    load_v(i_lo + 1, cok[u], MP[2][u], col[u], R[1][u]);
+    load_v(i_lo + 2, cok[u], MP[3][u], col[u], R[2][u]);
+  }
+  load_v(i_lo + 1, hok, HP[1], hcol, HR[1]);
+  load_v(i_lo + 2, hok, HP[2], hcol, HR[2]);
+#pragma unroll
+  for (int u = 0; u < RPT; ++u) {
+    XV[2][u] = form(W0[u]);
+    XV[0][u] = form(W1[u]);
+    NB[0][u] = W1[u].nb;
+    BV[0][u] = W1[u].b;
+    pl[i_lo % 3][jl0 + u + 1][kl + 1] = XV[0][u];
+  }
+  if (ring) pl[i_lo % 3][hj + 1][hk + 1] = form(WH);
   const double bc = (OPk == OP_UNIT) ? 1.0 : o.il;
   const double ih2 = 1.0 / ((double)g.h * (double)g.h);
   double a_rz = 0, a_rr = 0, a_qz = 0, a_qr = 0;
