@@ -425,6 +425,16 @@ __device__ __forceinline__ double acc64(double a, X x, Y y) {
   return __builtin_fma((double)x, (double)y, a);
 }
 
+// Byte loads through a buffer resource: the array in scalar registers, the
+// plane's offset as the scalar operand, the column in a vector register, so
+// no lane forms a 64-bit address (mask arrays hold G < 2^32 bytes).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t byte_rsrc(const uint8_t* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t*>(p), 0, -1, 0x00020000);
+}
+__device__ __forceinline__ uint8_t ld_byte(__amdgpu_buffer_rsrc_t r, uint32_t col, uint32_t plane_off) {
+  return __builtin_amdgcn_raw_buffer_load_b8(r, col, plane_off, 0);
+}
+
 // Vector stores that nothing reads again within the launch (the next launch
 // or pass reads them): nontemporal, so they do not displace the lines that
 // neighbouring tiles of the same launch are about to read.
@@ -1241,9 +1251,9 @@ __global__ void __launch_bounds__(ST, 4) k_cgf(OpArgs o, Tiling T, int first, in
   }
   const bool hok = ring && (j0 + hj) >= 0 && (j0 + hj) < g.ny && (k0 + hk) >= 0 && (k0 + hk) < g.nz;
   const uint32_t hcol = hok ? (uint32_t)((j0 + hj) * g.nz + (k0 + hk)) : 0u;
+  const auto nbm_r = byte_rsrc(o.nbm);
   auto load_m = [&](int i, bool ok, uint32_t c) -> uint8_t {
-    const uint8_t* mp = o.nbm + plane(i);
-    return mp[(ok && i >= 0 && i < g.nx) ? c : 0u];
+    return ld_byte(nbm_r, (ok && i >= 0 && i < g.nx) ? c : 0u, (uint32_t)plane(i));
   };
   // masks of planes i_lo - 1 .. i_lo + 3 (the plane loop's slots M, HM and
   // the prologue's MA, HA, HB); they do not depend on alpha or beta
