@@ -555,9 +555,9 @@ __global__ void __launch_bounds__(ST, (sizeof(V) == 4 && !SLOW) ? 6 : 4) k_spmv(
   // loaded; load_v applies the grid test, so that no instruction touches the
   // byte before it is needed.)
   auto plane = [&](int i) -> int64_t { return (int64_t)min(max(i, 0), g.nx - 1) * sx; };
+  const auto nbm_r = byte_rsrc(o.nbm);
   auto load_m = [&](int i, bool ok, uint32_t c) -> uint8_t {
-    const uint8_t* mp = o.nbm + plane(i);
-    return mp[(ok && i >= 0 && i < g.nx) ? c : 0u];
+    return ld_byte(nbm_r, (ok && i >= 0 && i < g.nx) ? c : 0u, (uint32_t)plane(i));
   };
   // vectors of one cell given its mask: r and p_{k-1} on the unknowns, the
   // diagonal on the slow unknowns
@@ -934,9 +934,9 @@ __global__ void __launch_bounds__(ST, MODE == 1 ? 2 : 4) k_resid64(OpArgs o, Til
   // loads as in the spmv: uniform plane base, 32-bit column, every lane
   // issues every load (element 0 of the plane when there is nothing to read)
   auto plane = [&](int i) -> int64_t { return (int64_t)min(max(i, 0), g.nx - 1) * sx; };
+  const auto nbm_r = byte_rsrc(o.nbm);
   auto load_m = [&](int i, bool ok, uint32_t c) -> uint8_t {
-    const uint8_t* mp = o.nbm + plane(i);
-    return mp[(ok && i >= 0 && i < g.nx) ? c : 0u];
+    return ld_byte(nbm_r, (ok && i >= 0 && i < g.nx) ? c : 0u, (uint32_t)plane(i));
   };
   struct Rw {
     double x64;
