@@ -248,10 +248,29 @@ __device__ __forceinline__ A apply_op(const OpArgs& o, const X* __restrict__ x, 
 // every block derives the same scalars (alpha, beta, convergence) bit for bit.
 // Kernel boundaries make the partials visible across XCDs.  Block 0 also
 // records the scalars for the host to poll.
+// Wave total by DPP moves (no LDS round trip): a Hillis-Steele prefix sum
+// within each row of 16 lanes (shifts 1, 2, 4, 8; lanes shifted in from
+// outside the row read 0), then row 1 and 3 add the last lane of the row
+// before them and rows 2 and 3 the last lane of row 1; lane 63 holds the
+// total, which every lane reads.  The order is fixed, so every block that
+// sums the same values gets the same bits.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_add(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWS, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROWS, 0xF, true);
+  return v + __builtin_bit_cast(double, ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
-  return v;
+  v = dpp_add<0x111, 0xF>(v);  // row_shr:1
+  v = dpp_add<0x112, 0xF>(v);  // row_shr:2
+  v = dpp_add<0x114, 0xF>(v);  // row_shr:4
+  v = dpp_add<0x118, 0xF>(v);  // row_shr:8
+  v = dpp_add<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+  v = dpp_add<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)b, 63), hi = __builtin_amdgcn_readlane((uint32_t)(b >> 32), 63);
+  return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 }
 
 // Block-wide sum of N values over NT threads; the result is returned in every
